@@ -1,0 +1,856 @@
+"""Python mirror of gartus's path-tracing API over the HIP library.
+
+Names, argument meaning and error behaviour follow the reference's
+``PathTracer`` / ``RayCamera`` / ``RayScene`` / ``SamplingTargetList`` /
+``EnvironmentLight`` / ``HdrImage`` / ``ToneMap``
+(src/graphics/raytracing/renderer.rs, src/graphics/camera.rs,
+src/graphics/raytracing/scene.rs, environment.rs, src/graphics/display.rs).
+Where the reference panics on an invalid argument, this mirror raises
+``InvalidArgument`` (a ValueError).
+
+Every render runs on the GPU through ``libgartus_hip.so``; a missing library
+or device raises, there is no CPU fallback.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import math
+from dataclasses import dataclass, replace
+from typing import Optional
+
+import numpy as np
+
+from . import abi
+from .scene_data import (DIELECTRIC, DIFFUSE_LIGHT, GGX, LAMBERTIAN, METAL, MOVING_SPHERE, QUAD, SPHERE,
+                         TRIANGLE, Material, SceneData)
+
+# ---------------------------------------------------------------- context ---
+
+_CONTEXT = None
+
+
+class Context:
+    """One HIP device and stream (gt_context)."""
+
+    def __init__(self, device: int = 0):
+        self.lib = abi.load()
+        h = C.c_void_p()
+        abi.check(self.lib.gt_context_create(device, C.byref(h)))
+        self.handle = h
+
+    @property
+    def tuning(self) -> abi.GtTuning:
+        t = abi.GtTuning()
+        abi.check(self.lib.gt_context_get_tuning(self.handle, C.byref(t)))
+        return t
+
+    def set_tuning(self, **kw) -> None:
+        t = self.tuning
+        for k, v in kw.items():
+            setattr(t, k, v)
+        abi.check(self.lib.gt_context_set_tuning(self.handle, C.byref(t)))
+
+    def stream(self) -> int:
+        return self.lib.gt_context_stream(self.handle) or 0
+
+    def synchronize(self) -> None:
+        abi.check(self.lib.gt_context_synchronize(self.handle))
+
+    def close(self) -> None:
+        if self.handle:
+            self.lib.gt_context_destroy(self.handle)
+            self.handle = None
+
+
+def default_context() -> Context:
+    global _CONTEXT
+    if _CONTEXT is None:
+        _CONTEXT = Context(0)
+    return _CONTEXT
+
+
+def _dp(a: np.ndarray):
+    return a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+def _v3(v) -> tuple:
+    t = tuple(float(x) for x in v)
+    if len(t) != 3:
+        raise abi.InvalidArgument(abi.GT_EINVAL, "expected three components")
+    return t
+
+
+# -------------------------------------------------------------- materials ---
+
+class RayMaterial:
+    """RayMaterial constructors (material.rs:2355-2582) for the five
+    RayScene BSDFs of the GPU path."""
+
+    @staticmethod
+    def lambertian(albedo) -> Material:
+        c = _v3(albedo)
+        if not all(math.isfinite(x) for x in c):
+            raise abi.InvalidArgument(abi.GT_EINVAL, "Lambertian albedo must be finite")
+        return Material(LAMBERTIAN, c)
+
+    @staticmethod
+    def diffuse_light(emit) -> Material:
+        c = _v3(emit)
+        if not all(math.isfinite(x) for x in c):
+            raise abi.InvalidArgument(abi.GT_EINVAL, "diffuse light color must be finite")
+        return Material(DIFFUSE_LIGHT, c)
+
+    @staticmethod
+    def ggx_microfacet(specular_color, roughness: float) -> Material:
+        c = _v3(specular_color)
+        if not (all(math.isfinite(x) for x in c) and math.isfinite(roughness)):
+            raise abi.InvalidArgument(abi.GT_EINVAL, "GGX microfacet material values must be finite")
+        return Material(GGX, c, min(max(float(roughness), 0.0), 1.0))
+
+    @staticmethod
+    def metal(albedo, fuzz: float) -> Material:
+        c = _v3(albedo)
+        if not (all(math.isfinite(x) for x in c) and math.isfinite(fuzz)):
+            raise abi.InvalidArgument(abi.GT_EINVAL, "metal material values must be finite")
+        return Material(METAL, c, min(max(float(fuzz), 0.0), 1.0))
+
+    @staticmethod
+    def dielectric(refraction_index: float) -> Material:
+        return Material(DIELECTRIC, (1.0, 1.0, 1.0), float(refraction_index))
+
+
+# ----------------------------------------------------------------- scene ---
+
+class RayScene:
+    """RayScene (scene.rs:1102-1578) compiled for the GPU on first use."""
+
+    def __init__(self, data: Optional[SceneData] = None, context: Optional[Context] = None):
+        self.data = data if data is not None else SceneData()
+        self._ctx = context
+        self._handle = None
+        self._leaf_size = None   # None: not built (lazy default build on first hit)
+        self._dirty = True
+
+    # construction ------------------------------------------------------
+    def add_material(self, material: Material) -> int:
+        self._dirty = True
+        return self.data.add_material(material)
+
+    def _add(self, kind, data, material) -> None:
+        try:
+            self.data.add(kind, data, material)
+        except ValueError as e:
+            raise abi.InvalidArgument(abi.GT_EINVAL, str(e)) from None
+        self._dirty = True
+        self._leaf_size = None   # invalidate_bvh (scene.rs:1575-1577)
+
+    def add_sphere(self, center, radius, material: int) -> None:
+        self._add(SPHERE, [*_v3(center), float(radius)], material)
+
+    def add_spheres(self, material: int, spheres) -> None:
+        self._add(SPHERE, spheres, material)
+
+    def add_moving_sphere(self, center_start, center_end, radius, material: int) -> None:
+        self._add(MOVING_SPHERE, [*_v3(center_start), *_v3(center_end), float(radius)], material)
+
+    def add_triangle(self, p0, p1, p2, material: int) -> None:
+        self._add(TRIANGLE, [*_v3(p0), *_v3(p1), *_v3(p2)], material)
+
+    def add_triangles(self, material, triangles) -> None:
+        self._add(TRIANGLE, triangles, material)
+
+    def add_quad(self, corner, u, v, material: int) -> None:
+        self._add(QUAD, [*_v3(corner), *_v3(u), *_v3(v)], material)
+
+    def add_quads(self, material, quads) -> None:
+        self._add(QUAD, quads, material)
+
+    def add_sphere_with_material(self, center, radius, material: Material) -> int:
+        mid = self.add_material(material)
+        self.add_sphere(center, radius, mid)
+        return mid
+
+    def add_quad_with_material(self, corner, u, v, material: Material) -> int:
+        mid = self.add_material(material)
+        self.add_quad(corner, u, v, mid)
+        return mid
+
+    def build_bvh(self) -> None:
+        self.build_bvh_with_options(4)
+
+    def build_bvh_with_options(self, leaf_size: int = 4) -> None:
+        self._leaf_size = max(int(leaf_size), 1)
+        self._compile(force_build=True)
+
+    def with_bvh(self) -> "RayScene":
+        self.build_bvh()
+        return self
+
+    # queries -------------------------------------------------------------
+    def __len__(self) -> int:
+        return len(self.data)
+
+    def material_count(self) -> int:
+        return len(self.data.materials)
+
+    def emissive_targets(self) -> "SamplingTargetList":
+        t = SamplingTargetList()
+        for kind, q, _ in self.data.emissive_targets():
+            t._add(kind, q, 1.0)
+        return t
+
+    @property
+    def context(self) -> Context:
+        if self._ctx is None:
+            self._ctx = default_context()
+        return self._ctx
+
+    def _compile(self, force_build: bool = False):
+        ctx = self.context
+        lib = ctx.lib
+        if self._handle is None or self._dirty:
+            if self._handle is not None:
+                lib.gt_scene_destroy(self._handle)
+            h = C.c_void_p()
+            abi.check(lib.gt_scene_create(ctx.handle, C.byref(h)))
+            self._handle = h
+            mid = C.c_uint32()
+            for m in self.data.materials:
+                r, g, b = m.color
+                if m.kind == LAMBERTIAN:
+                    rc = lib.gt_scene_add_lambertian(h, r, g, b, C.byref(mid))
+                elif m.kind == DIFFUSE_LIGHT:
+                    rc = lib.gt_scene_add_diffuse_light(h, r, g, b, C.byref(mid))
+                elif m.kind == GGX:
+                    rc = lib.gt_scene_add_ggx(h, r, g, b, m.param, C.byref(mid))
+                elif m.kind == METAL:
+                    rc = lib.gt_scene_add_metal(h, r, g, b, m.param, C.byref(mid))
+                else:
+                    rc = lib.gt_scene_add_dielectric(h, m.param, C.byref(mid))
+                abi.check(rc)
+            adders = {SPHERE: lib.gt_scene_add_spheres, MOVING_SPHERE: lib.gt_scene_add_moving_spheres,
+                      TRIANGLE: lib.gt_scene_add_triangles, QUAD: lib.gt_scene_add_quads}
+            for kind, data, mids in self.data.chunks:
+                abi.check(adders[kind](h, _dp(data), len(data), mids.ctypes.data_as(abi.U32)))
+            self._dirty = False
+            if self._leaf_size is not None:
+                abi.check(lib.gt_scene_build(h, self._leaf_size))
+        elif force_build:
+            abi.check(lib.gt_scene_build(self._handle, self._leaf_size))
+        return self._handle
+
+    def info(self) -> abi.GtSceneInfo:
+        h = self._compile()
+        info = abi.GtSceneInfo()
+        abi.check(self.context.lib.gt_scene_get_info(h, C.byref(info)))
+        return info
+
+    def bvh_node_count(self) -> Optional[int]:
+        if self._leaf_size is None:
+            return None
+        return int(self.info().bvh_node_count)
+
+    def intersect(self, origins, directions, times=None, t_min: float = 0.001, t_max=np.inf,
+                  with_stats: bool = False):
+        """Closest hits of many rays (RayScene::hit, scene.rs:1639-1657) with
+        the primitive id; returns a dict of numpy arrays (and stats)."""
+        o = np.asarray(origins, np.float32).reshape(-1, 3)
+        d = np.asarray(directions, np.float32).reshape(-1, 3)
+        n = len(o)
+        rays = np.zeros((n, 8), np.float32)
+        rays[:, 0:3] = o
+        rays[:, 3] = 0.0 if times is None else np.asarray(times, np.float32)
+        rays[:, 4:7] = d
+        rays[:, 7] = np.broadcast_to(np.asarray(t_max, np.float32), (n,))
+        hits = np.zeros((n, 8), np.float32)
+        h = self._compile()
+        st = abi.GtTraversalStats()
+        abi.check(self.context.lib.gt_intersect(
+            self.context.handle, h, rays.ctypes.data_as(C.POINTER(abi.GtRay)), n, float(t_min),
+            hits.ctypes.data_as(C.POINTER(abi.GtHit)), C.byref(st) if with_stats else None))
+        out = {
+            "t": hits[:, 0].copy(), "u": hits[:, 1].copy(), "v": hits[:, 2].copy(),
+            "prim": hits[:, 3].view(np.int32).copy(), "normal": hits[:, 4:7].copy(),
+            "front": hits[:, 7].view(np.uint32).astype(bool),
+        }
+        if with_stats:
+            out["stats"] = {k: int(getattr(st, k)) for k, _ in abi.GtTraversalStats._fields_}
+        return out
+
+    def close(self) -> None:
+        if self._handle is not None and self._ctx is not None:
+            self._ctx.lib.gt_scene_destroy(self._handle)
+        self._handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class WeightedSamplingTargetList:
+    """WeightedSamplingTargetList (scene.rs:458-609)."""
+
+    def __init__(self):
+        self.targets = []       # (kind, params, weight)
+        self.total_weight = 0.0
+        self._handle = None
+        self._ctx = None
+
+    def _add(self, kind, q, weight: float) -> None:
+        weight = float(weight)
+        if not (math.isfinite(weight) and weight > 0.0):
+            raise abi.InvalidArgument(abi.GT_EINVAL, "sampling target weight must be finite and positive")
+        if not math.isfinite(self.total_weight + weight):
+            raise abi.InvalidArgument(abi.GT_EINVAL, "sampling target total weight must remain finite")
+        self.total_weight += weight
+        self.targets.append((kind, np.asarray(q, np.float64), weight))
+        self._drop()
+
+    def add_sphere_weighted(self, center, radius, weight) -> None:
+        self._add(SPHERE, [*_v3(center), max(float(radius), 0.0)], weight)
+
+    def add_moving_sphere_weighted(self, c0, c1, radius, weight) -> None:
+        self._add(MOVING_SPHERE, [*_v3(c0), *_v3(c1), max(float(radius), 0.0)], weight)
+
+    def add_quad_weighted(self, corner, u, v, weight) -> None:
+        self._add(QUAD, [*_v3(corner), *_v3(u), *_v3(v)], weight)
+
+    def add_triangle_weighted(self, p0, p1, p2, weight) -> None:
+        self._add(TRIANGLE, [*_v3(p0), *_v3(p1), *_v3(p2)], weight)
+
+    def __len__(self) -> int:
+        return len(self.targets)
+
+    def is_empty(self) -> bool:
+        return not self.targets
+
+    def clear(self) -> None:
+        self.targets = []
+        self.total_weight = 0.0
+        self._drop()
+
+    def _drop(self) -> None:
+        if self._handle is not None:
+            self._ctx.lib.gt_lights_destroy(self._handle)
+        self._handle = None
+
+    def _compile(self, ctx: Context):
+        if self._handle is not None and self._ctx is ctx:
+            return self._handle
+        self._drop()
+        h = C.c_void_p()
+        abi.check(ctx.lib.gt_lights_create(ctx.handle, C.byref(h)))
+        for kind, q, w in self.targets:
+            arr = (C.c_double * 9)(*list(q) + [0.0] * (9 - len(q)))
+            a = C.cast(arr, abi.D)
+            b = C.cast(C.byref(arr, 24), abi.D)
+            c = C.cast(C.byref(arr, 48), abi.D)
+            if kind == SPHERE:
+                rc = ctx.lib.gt_lights_add_sphere(h, a, q[3], w)
+            elif kind == MOVING_SPHERE:
+                rc = ctx.lib.gt_lights_add_moving_sphere(h, a, b, q[6], w)
+            elif kind == QUAD:
+                rc = ctx.lib.gt_lights_add_quad(h, a, b, c, w)
+            else:
+                rc = ctx.lib.gt_lights_add_triangle(h, a, b, c, w)
+            abi.check(rc)
+        self._handle, self._ctx = h, ctx
+        return h
+
+    def __del__(self):
+        try:
+            self._drop()
+        except Exception:
+            pass
+
+
+class SamplingTargetList(WeightedSamplingTargetList):
+    """SamplingTargetList (scene.rs:363-450): every target has weight 1."""
+
+    def add_sphere(self, center, radius) -> None:
+        self.add_sphere_weighted(center, radius, 1.0)
+
+    def add_moving_sphere(self, c0, c1, radius) -> None:
+        self.add_moving_sphere_weighted(c0, c1, radius, 1.0)
+
+    def add_quad(self, corner, u, v) -> None:
+        self.add_quad_weighted(corner, u, v, 1.0)
+
+    def add_triangle(self, p0, p1, p2) -> None:
+        self.add_triangle_weighted(p0, p1, p2, 1.0)
+
+
+class EnvironmentLight:
+    """EnvironmentLight (environment.rs:22-195): a constant colour, or the
+    FP32 HDR lat-long form (rows from +y down to -y, linear RGB)."""
+
+    def __init__(self, rgb: Optional[np.ndarray] = None, color=None):
+        self.rgb = None if rgb is None else np.ascontiguousarray(np.asarray(rgb, np.float32))
+        self.color = None if color is None else _v3(color)
+        if self.rgb is not None and (self.rgb.ndim != 3 or self.rgb.shape[2] != 3 or self.rgb.size == 0):
+            raise abi.InvalidArgument(abi.GT_EINVAL, "environment map must be an (H, W, 3) array")
+        self._handle = None
+        self._ctx = None
+
+    @staticmethod
+    def constant(color) -> "EnvironmentLight":
+        return EnvironmentLight(color=color)
+
+    @staticmethod
+    def from_rgb32f(rgb) -> "EnvironmentLight":
+        return EnvironmentLight(rgb=rgb)
+
+    def _compile(self, ctx: Context):
+        if self._handle is not None and self._ctx is ctx:
+            return self._handle
+        h = C.c_void_p()
+        if self.rgb is None:
+            abi.check(ctx.lib.gt_env_create_constant(ctx.handle, *self.color, C.byref(h)))
+        else:
+            hh, ww, _ = self.rgb.shape
+            abi.check(ctx.lib.gt_env_create_rgb32f(ctx.handle, self.rgb.ctypes.data_as(abi.F), ww, hh, C.byref(h)))
+        self._handle, self._ctx = h, ctx
+        return h
+
+    def __del__(self):
+        try:
+            if self._handle is not None:
+                self._ctx.lib.gt_env_destroy(self._handle)
+        except Exception:
+            pass
+
+
+# ---------------------------------------------------------------- camera ---
+
+class PixelSampleMode:
+    RANDOM = abi.GT_PIXEL_RANDOM
+    STRATIFIED = abi.GT_PIXEL_STRATIFIED
+
+
+class DirectLightingMode:
+    CURRENT_PATH_CONTINUATION = abi.GT_LIGHTING_CONTINUATION
+    NEXT_EVENT_ESTIMATION = abi.GT_LIGHTING_NEE
+
+
+class RayCamera:
+    """RayCamera builder (camera.rs:1034-1481). Builders return a new camera;
+    invalid parameters raise InvalidArgument where the reference panics."""
+
+    def __init__(self, image_width: int = 100, aspect_ratio: float = 1.0):
+        if int(image_width) <= 0:
+            raise abi.InvalidArgument(abi.GT_EINVAL, "image width must be positive")
+        if not (math.isfinite(aspect_ratio) and aspect_ratio > 0.0):
+            raise abi.InvalidArgument(abi.GT_EINVAL, "aspect ratio must be positive and finite")
+        self.raw = abi.default_camera()
+        self.raw.image_width = int(image_width)
+        self.raw.aspect_ratio = float(aspect_ratio)
+        self._validate()
+
+    @classmethod
+    def new(cls, image_width: int, aspect_ratio: float) -> "RayCamera":
+        return cls(image_width, aspect_ratio)
+
+    def _copy(self) -> "RayCamera":
+        c = RayCamera.__new__(RayCamera)
+        c.raw = abi.GtCamera.from_buffer_copy(self.raw)
+        return c
+
+    def _validate(self) -> None:  # validate_view (camera.rs:1161-1201)
+        r = self.raw
+        if not (math.isfinite(r.vertical_fov) and 0.0 < r.vertical_fov < 180.0):
+            raise abi.InvalidArgument(abi.GT_EINVAL, "vertical field of view must be finite and in 0..180 degrees")
+        w = np.array(r.lookfrom[:]) - np.array(r.lookat[:])
+        if not float(w @ w) > np.finfo(float).eps:
+            raise abi.InvalidArgument(abi.GT_EINVAL, "lookfrom and lookat must be distinct")
+        up = np.array(r.view_up[:])
+        if not float(up @ up) > np.finfo(float).eps:
+            raise abi.InvalidArgument(abi.GT_EINVAL, "view-up vector must be nonzero")
+        fwd = -w / np.linalg.norm(w)
+        right = np.cross(fwd, up)
+        if np.linalg.norm(right) < np.finfo(float).eps:
+            raise abi.InvalidArgument(abi.GT_EINVAL, "view-up vector must not be parallel to the viewing direction")
+        if not (math.isfinite(r.defocus_angle) and 0.0 <= r.defocus_angle < 180.0):
+            raise abi.InvalidArgument(abi.GT_EINVAL, "defocus angle must be finite and in 0..180 degrees")
+        if not (math.isfinite(r.focus_distance) and r.focus_distance > 0.0):
+            raise abi.InvalidArgument(abi.GT_EINVAL, "focus distance must be positive and finite")
+        if not (math.isfinite(r.shutter_start) and math.isfinite(r.shutter_end) and r.shutter_start <= r.shutter_end):
+            raise abi.InvalidArgument(abi.GT_EINVAL, "shutter interval must be finite and ordered")
+        nbg = 9 if r.background_kind == abi.GT_BACKGROUND_VERTICAL_GRADIENT else 3
+        if not all(math.isfinite(x) for x in r.background[:nbg]):
+            raise abi.InvalidArgument(abi.GT_EINVAL, "background color components must be finite")
+
+    def _with(self, validate=True, **kw) -> "RayCamera":
+        c = self._copy()
+        for k, v in kw.items():
+            if k in ("lookfrom", "lookat", "view_up"):
+                getattr(c.raw, k)[:] = _v3(v)
+            else:
+                setattr(c.raw, k, v)
+        if validate:
+            c._validate()
+        return c
+
+    def with_image_width(self, w: int):
+        if int(w) <= 0:
+            raise abi.InvalidArgument(abi.GT_EINVAL, "image width must be positive")
+        return self._with(image_width=int(w))
+
+    def with_aspect_ratio(self, a: float):
+        if not (math.isfinite(a) and a > 0.0):
+            raise abi.InvalidArgument(abi.GT_EINVAL, "aspect ratio must be positive and finite")
+        return self._with(aspect_ratio=float(a))
+
+    def with_samples_per_pixel(self, n: int):
+        return self._with(samples_per_pixel=max(int(n), 1))
+
+    def with_pixel_sample_mode(self, mode: int):
+        return self._with(pixel_sample_mode=int(mode))
+
+    def with_stratified_sampling(self):
+        return self.with_pixel_sample_mode(abi.GT_PIXEL_STRATIFIED)
+
+    def with_stratified_grid_width(self, g: int):
+        g = max(int(g), 1)
+        return self._with(samples_per_pixel=g * g, pixel_sample_mode=abi.GT_PIXEL_STRATIFIED_GRID,
+                          stratified_grid_width=g)
+
+    def with_max_depth(self, d: int):
+        return self._with(max_depth=int(d))
+
+    def with_russian_roulette_min_depth(self, d: int):
+        return self._with(russian_roulette_min_depth=int(d))
+
+    def without_russian_roulette(self):
+        return self._with(russian_roulette_min_depth=-1)
+
+    def with_rng_seed(self, seed: int):
+        return self._with(rng_seed=int(seed) & (2 ** 64 - 1))
+
+    def with_vertical_fov(self, fov: float):
+        return self._with(vertical_fov=float(fov))
+
+    def with_look_at(self, lookfrom, lookat):
+        return self._with(lookfrom=lookfrom, lookat=lookat)
+
+    def with_view_up(self, up):
+        return self._with(view_up=up)
+
+    def with_defocus_angle(self, a: float):
+        return self._with(defocus_angle=float(a))
+
+    def with_focus_distance(self, d: float):
+        return self._with(focus_distance=float(d))
+
+    def with_shutter_interval(self, start: float, end: float):
+        if not (math.isfinite(start) and math.isfinite(end) and start <= end):
+            raise abi.InvalidArgument(abi.GT_EINVAL, "shutter interval must be finite and ordered")
+        return self._with(shutter_start=float(start), shutter_end=float(end))
+
+    def with_background(self, color):
+        c = self._with(validate=False, background_kind=abi.GT_BACKGROUND_CONSTANT)
+        c.raw.background[0:3] = _v3(color)
+        c._validate()
+        return c
+
+    def with_vertical_gradient_background(self, nadir, horizon, zenith):
+        c = self._with(validate=False, background_kind=abi.GT_BACKGROUND_VERTICAL_GRADIENT)
+        c.raw.background[0:9] = (*_v3(nadir), *_v3(horizon), *_v3(zenith))
+        c._validate()
+        return c
+
+    def with_direct_lighting_mode(self, mode: int):  # SamplingStrategy::from_direct_lighting_mode
+        w = 0.0 if mode == abi.GT_LIGHTING_NEE else 0.5
+        return self._with(direct_lighting_mode=int(mode), light_pdf_weight=w)
+
+    def with_light_pdf_weight(self, w: float):  # SamplingStrategy::with_light_pdf_weight
+        if not math.isfinite(w):
+            raise abi.InvalidArgument(abi.GT_EINVAL, "light PDF weight must be finite")
+        return self._with(direct_lighting_mode=abi.GT_LIGHTING_CONTINUATION, light_pdf_weight=min(max(w, 0.0), 1.0))
+
+    def image_width(self) -> int:
+        return int(self.raw.image_width)
+
+    def image_height(self) -> int:  # camera.rs:1071
+        return max(int(self.raw.image_width / self.raw.aspect_ratio), 1)
+
+    def samples_per_pixel(self) -> int:
+        return int(self.raw.samples_per_pixel)
+
+    def effective_samples_per_pixel(self) -> int:  # camera.rs:1547-1557
+        r = self.raw
+        s = max(int(r.samples_per_pixel), 1)
+        if r.pixel_sample_mode == abi.GT_PIXEL_RANDOM:
+            return s
+        if r.pixel_sample_mode == abi.GT_PIXEL_STRATIFIED:
+            g = max(int(math.sqrt(s)), 1)
+            return g * g
+        g = max(int(r.stratified_grid_width), 1)
+        return g * g
+
+    def max_depth(self) -> int:
+        return int(self.raw.max_depth)
+
+
+# ----------------------------------------------------------------- images ---
+
+class ToneMappingOperator:
+    NONE, REINHARD, ACES = 0, 1, 2
+
+
+@dataclass(frozen=True)
+class ToneMap:
+    """ToneMap (display.rs:191-278)."""
+    exposure: float = 1.0
+    operator: int = ToneMappingOperator.NONE
+    gamma: float = 2.0
+
+    def with_exposure(self, e: float) -> "ToneMap":
+        if not (math.isfinite(e) and e >= 0.0):
+            raise abi.InvalidArgument(abi.GT_EINVAL, "tone-map exposure must be finite and non-negative")
+        return replace(self, exposure=float(e))
+
+    def with_operator(self, op: int) -> "ToneMap":
+        return replace(self, operator=int(op))
+
+    def with_gamma(self, g: float) -> "ToneMap":
+        if not (math.isfinite(g) and g > 0.0):
+            raise abi.InvalidArgument(abi.GT_EINVAL, "tone-map gamma must be positive and finite")
+        return replace(self, gamma=float(g))
+
+    def map(self, rgb: np.ndarray) -> np.ndarray:
+        """ToneMap::map_color per channel (display.rs:264-277) -> uint8."""
+        v = np.asarray(rgb, np.float64) * self.exposure
+        with np.errstate(invalid="ignore", over="ignore"):
+            v = np.where(np.isfinite(v), np.maximum(v, 0.0), 0.0)
+            if self.operator == ToneMappingOperator.REINHARD:
+                v = v / (1.0 + v)
+            elif self.operator == ToneMappingOperator.ACES:
+                v = np.clip(v * (2.51 * v + 0.03) / (v * (2.43 * v + 0.59) + 0.14), 0.0, 1.0)
+            if abs(self.gamma - 2.0) <= np.finfo(float).eps:
+                g = np.where(v > 0.0, np.sqrt(np.maximum(v, 0.0)), 0.0)
+            else:
+                g = np.where(v <= 0.0, 0.0, np.power(np.maximum(v, 0.0), 1.0 / self.gamma))
+        return (256.0 * np.clip(g, 0.0, 0.999)).astype(np.uint8)
+
+
+class HdrImage:
+    """HdrImage (display.rs:282-409): linear RGB, row-major, top row first."""
+
+    def __init__(self, width: int, height: int, pixels: np.ndarray):
+        pixels = np.asarray(pixels)
+        if pixels.size != width * height * 3:
+            raise abi.InvalidArgument(abi.GT_EINVAL, "HDR image pixel count must match dimensions")
+        self.width, self.height = int(width), int(height)
+        self.pixels = pixels.reshape(height, width, 3)
+
+    def to_canvas(self) -> np.ndarray:
+        return self.to_canvas_tone_mapped(ToneMap())
+
+    def to_canvas_tone_mapped(self, tone_map: ToneMap) -> np.ndarray:
+        return tone_map.map(self.pixels)
+
+    def pfm_bytes(self) -> bytes:
+        """write_pfm (display.rs:365-387): little-endian f32, bottom row first,
+        non-finite values written as 0."""
+        head = f"PF\n{self.width} {self.height}\n-1.0\n".encode()
+        p = np.where(np.isfinite(self.pixels), self.pixels, 0.0).astype("<f4")
+        return head + p[::-1].tobytes()
+
+    def save_pfm(self, path: str) -> None:
+        with open(path, "wb") as f:
+            f.write(self.pfm_bytes())
+
+
+@dataclass
+class DenoisingAovs:
+    """DenoisingAovs (camera.rs:491-526)."""
+    beauty_linear: np.ndarray
+    albedo_linear: np.ndarray
+    normal_linear: np.ndarray
+
+    @property
+    def beauty(self) -> np.ndarray:  # Rgb::from_linear_color (gamma 2)
+        return ToneMap().map(self.beauty_linear)
+
+    @property
+    def albedo(self) -> np.ndarray:  # Rgb::from_raw_linear_color
+        return _raw_encode(self.albedo_linear)
+
+    @property
+    def normal(self) -> np.ndarray:
+        return _raw_encode(self.normal_linear)
+
+
+def _raw_encode(x: np.ndarray) -> np.ndarray:  # colors.rs:412-425
+    x = np.where(np.isnan(x), 0.0, x)
+    return (255.999 * np.clip(x, 0.0, 1.0)).astype(np.uint8)
+
+
+# ------------------------------------------------------------------ render ---
+
+@dataclass(frozen=True)
+class RenderOptions:
+    """RenderOptions (renderer.rs:20-45). Tile size is accepted for API
+    parity; the GPU path schedules whole-image wavefronts and its result does
+    not depend on it, as the reference's does not."""
+    tile_size: Optional[int] = None
+
+    def with_tile_size(self, t: int) -> "RenderOptions":
+        return RenderOptions(max(int(t), 1))
+
+
+class PathTracer:
+    """PathTracer (renderer.rs:47-887) over RayScene worlds."""
+
+    def __init__(self, camera: RayCamera, options: RenderOptions = RenderOptions()):
+        self.camera = camera
+        self.options = options
+
+    @classmethod
+    def new(cls, camera: RayCamera) -> "PathTracer":
+        return cls(camera)
+
+    def with_camera(self, camera: RayCamera) -> "PathTracer":
+        return PathTracer(camera, self.options)
+
+    def with_options(self, options: RenderOptions) -> "PathTracer":
+        return PathTracer(self.camera, options)
+
+    # core ---------------------------------------------------------------
+    def _render(self, world: RayScene, camera: RayCamera, lights=None, env=None, stats: bool = False):
+        if not isinstance(world, RayScene):
+            raise TypeError("the GPU path renders RayScene worlds")
+        ctx = world.context
+        sh = world._compile()
+        lh = lights._compile(ctx) if lights is not None else None
+        eh = env._compile(ctx) if env is not None else None
+        w, h = camera.image_width(), camera.image_height()
+        out = np.zeros((h, w, 3), np.float32)
+        st = abi.GtRenderStats()
+        abi.check(ctx.lib.gt_render_hdr(ctx.handle, sh, C.byref(camera.raw), lh, eh, out.ctypes.data_as(abi.F),
+                                        C.byref(st) if stats else None))
+        img = HdrImage(w, h, out)
+        if stats:
+            return img, {k: getattr(st, k) for k, _ in abi.GtRenderStats._fields_}
+        return img
+
+    def _render_aovs(self, world: RayScene, camera: RayCamera, lights=None, env=None) -> DenoisingAovs:
+        ctx = world.context
+        sh = world._compile()
+        lh = lights._compile(ctx) if lights is not None else None
+        eh = env._compile(ctx) if env is not None else None
+        w, h = camera.image_width(), camera.image_height()
+        b = np.zeros((h, w, 3), np.float32)
+        a = np.zeros_like(b)
+        n = np.zeros_like(b)
+        abi.check(ctx.lib.gt_render_aovs(ctx.handle, sh, C.byref(camera.raw), lh, eh, b.ctypes.data_as(abi.F),
+                                         a.ctypes.data_as(abi.F), n.ctypes.data_as(abi.F), None))
+        return DenoisingAovs(b, a, n)
+
+    @staticmethod
+    def _nee(camera: RayCamera) -> RayCamera:  # light_connection_camera / environment_importance_camera
+        return camera.with_direct_lighting_mode(abi.GT_LIGHTING_NEE)
+
+    # renderer.rs entry points ----------------------------------------------
+    def render(self, world) -> np.ndarray:
+        return self.render_hdr_image(world).to_canvas()
+
+    def render_tone_mapped(self, world, tone_map: ToneMap) -> np.ndarray:
+        return self.render_hdr_image(world).to_canvas_tone_mapped(tone_map)
+
+    def render_hdr_image(self, world) -> HdrImage:
+        return self._render(world, self.camera)
+
+    def render_denoising_aovs(self, world) -> DenoisingAovs:
+        return self._render_aovs(world, self.camera)
+
+    def render_with_lights(self, world, lights) -> np.ndarray:
+        return self.render_with_lights_hdr_image(world, lights).to_canvas()
+
+    def render_with_lights_hdr_image(self, world, lights) -> HdrImage:
+        return self._render(world, self.camera, lights)
+
+    def render_with_lights_denoising_aovs(self, world, lights) -> DenoisingAovs:
+        return self._render_aovs(world, self.camera, lights)
+
+    def render_with_light_connections(self, world, lights) -> np.ndarray:
+        return self._render(world, self._nee(self.camera), lights).to_canvas()
+
+    def render_with_environment(self, world, environment) -> np.ndarray:
+        return self.render_with_environment_hdr_image(world, environment).to_canvas()
+
+    def render_with_environment_hdr_image(self, world, environment) -> HdrImage:
+        return self._render(world, self._nee(self.camera), None, environment)
+
+    def render_with_lights_and_environment(self, world, lights, environment) -> np.ndarray:
+        return self.render_with_lights_and_environment_hdr_image(world, lights, environment).to_canvas()
+
+    def render_with_lights_and_environment_hdr_image(self, world, lights, environment) -> HdrImage:
+        return self._render(world, self._nee(self.camera), lights, environment)
+
+    def render_ray_scene(self, scene: RayScene) -> np.ndarray:
+        return self.render_ray_scene_hdr_image(scene).to_canvas()
+
+    def render_ray_scene_hdr_image(self, scene: RayScene) -> HdrImage:  # renderer.rs:193-201
+        lights = scene.emissive_targets()
+        if lights.is_empty():
+            return self.render_hdr_image(scene)
+        return self.render_with_lights_hdr_image(scene, lights)
+
+    def render_ray_scene_with_light_connections(self, scene: RayScene) -> np.ndarray:
+        lights = scene.emissive_targets()
+        if lights.is_empty():
+            return self.render(scene)
+        return self.render_with_light_connections(scene, lights)
+
+    def render_ray_scene_denoising_aovs(self, scene: RayScene) -> DenoisingAovs:
+        lights = scene.emissive_targets()
+        if lights.is_empty():
+            return self.render_denoising_aovs(scene)
+        return self.render_with_lights_denoising_aovs(scene, lights)
+
+    def render_scene(self, scene: "SurfaceScene") -> np.ndarray:
+        return self.render_scene_hdr_image(scene).to_canvas()
+
+    def render_scene_hdr_image(self, scene: "SurfaceScene") -> HdrImage:  # renderer.rs:283-286
+        return self.render_hdr_image(scene.to_ray_scene())
+
+    def render_scene_denoising_aovs(self, scene: "SurfaceScene") -> DenoisingAovs:
+        return self.render_denoising_aovs(scene.to_ray_scene())
+
+    def render_scene_with_lights(self, scene: "SurfaceScene", lights) -> np.ndarray:
+        return self.render_with_lights(scene.to_ray_scene(), lights)
+
+
+# ------------------------------------------------------------ SurfaceScene ---
+
+@dataclass
+class SurfaceMaterial:
+    """The part of SurfaceMaterial (src/graphics/material.rs) that
+    SurfaceRayMaterialMode::Lambertian reads: the base colour."""
+    base_color: tuple = (0.5, 0.5, 0.5)
+
+
+class SurfaceScene:
+    """SurfaceScene (src/graphics/scene.rs:80-145): triangle meshes with
+    materials; to_ray_scene compiles one Lambertian material per mesh and one
+    triangle per polygon (scene.rs:1517-1547)."""
+
+    def __init__(self):
+        self.meshes = []   # (triangles (n, 9), SurfaceMaterial)
+
+    def add_mesh(self, triangles, material: SurfaceMaterial = SurfaceMaterial()) -> None:
+        self.meshes.append((np.asarray(triangles, np.float64).reshape(-1, 9), material))
+
+    def __len__(self) -> int:
+        return len(self.meshes)
+
+    def to_ray_scene(self, leaf_size: int = 4) -> RayScene:
+        rs = RayScene()
+        for tris, mat in self.meshes:
+            mid = rs.add_material(RayMaterial.lambertian(mat.base_color))
+            rs.add_triangles(mid, tris)
+        rs.build_bvh_with_options(leaf_size)
+        return rs

@@ -1,0 +1,1006 @@
+// api.cpp — the C ABI of include/gartus_hip.h.
+//
+// Host side of the MI355X path tracer: objects mirror RayScene,
+// SamplingTargetList, EnvironmentLight and RayCamera; gt_render_* drives the
+// wavefront kernels of kernels.hip in batches of whole-image samples.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <limits>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "../../include/gartus_hip.h"
+#include "kernels.h"
+#include "scene_build.h"
+
+namespace {
+
+thread_local std::string g_error;
+
+int fail(int code, const std::string& msg) {
+  g_error = msg;
+  return code;
+}
+
+#define GT_HIP(expr)                                                                             \
+  do {                                                                                           \
+    hipError_t e_ = (expr);                                                                      \
+    if (e_ != hipSuccess) return fail(GT_EDEVICE, std::string(#expr) + ": " + hipGetErrorString(e_)); \
+  } while (0)
+
+constexpr double kEps = std::numeric_limits<double>::epsilon();
+
+template <class T>
+struct DevBuf {
+  T* p = nullptr;
+  size_t n = 0;
+  ~DevBuf() { release(); }
+  void release() {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    n = 0;
+  }
+  hipError_t ensure(size_t count) {
+    if (count <= n && p) return hipSuccess;
+    release();
+    hipError_t e = hipMalloc(reinterpret_cast<void**>(&p), std::max<size_t>(count, 1) * sizeof(T));
+    if (e == hipSuccess) n = count;
+    return e;
+  }
+};
+
+}  // namespace
+
+struct gt_context {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  uint32_t cus = 0;
+  gt_tuning tuning{};
+  // wavefront state
+  DevBuf<float4> q_o[2], q_d[2], q_t[2], s_o[2], s_d[2], s_w[2], radiance;
+  DevBuf<float2> hits;
+  DevBuf<uint32_t> ctr;
+  DevBuf<uint2> gstack;
+  DevBuf<double4> acc;
+  size_t cap_paths = 0;
+  uint32_t gstride = 0;
+  uint32_t* host_ctr = nullptr;  // pinned copy of the counters, one block per batch
+  size_t host_ctr_words = 0;
+  std::vector<hipEvent_t> events;
+  ~gt_context() {
+    for (hipEvent_t e : events) (void)hipEventDestroy(e);
+    if (host_ctr) (void)hipHostFree(host_ctr);
+    if (stream) (void)hipStreamDestroy(stream);
+  }
+};
+
+struct gt_scene {
+  gt_context* ctx = nullptr;
+  std::vector<gt::HostPrim> prims;
+  std::vector<gt::HostMat> mats;
+  bool built = false;
+  uint32_t leaf_size = 4;
+  gt::BinaryBvh bvh;
+  gt::PackedScene packed;
+  DevBuf<GNode2> nodes2;
+  DevBuf<GNode4> nodes4;
+  DevBuf<GPrim> dprims;
+  DevBuf<uint32_t> prim_ids;
+  DevBuf<GMat> dmats;
+  uint64_t device_bytes = 0;
+};
+
+struct gt_lights {
+  gt_context* ctx = nullptr;
+  std::vector<gt::HostPrim> targets;
+  std::vector<double> weights;
+  double total = 0.0;
+  bool dirty = true;
+  DevBuf<GLight> dev;
+};
+
+struct gt_env {
+  gt_context* ctx = nullptr;
+  uint32_t kind = 1;  // 1 constant, 2 texture
+  double color[3] = {0, 0, 0};
+  uint32_t w = 0, h = 0;
+  bool uniform = true;
+  DevBuf<float4> tex;
+  DevBuf<float> row_cdf, col_cdf, pdf;
+};
+
+namespace {
+
+bool finite3(double r, double g, double b) { return std::isfinite(r) && std::isfinite(g) && std::isfinite(b); }
+
+int upload_scene(gt_scene* s) {
+  gt::PackedScene& pk = s->packed;
+  GT_HIP(hipSetDevice(s->ctx->device));
+  GT_HIP(s->nodes2.ensure(pk.nodes2.size()));
+  GT_HIP(s->nodes4.ensure(pk.nodes4.size()));
+  GT_HIP(s->dprims.ensure(pk.prims.size()));
+  GT_HIP(s->prim_ids.ensure(pk.prim_ids.size()));
+  std::vector<GMat> gm(s->mats.size());
+  for (size_t i = 0; i < s->mats.size(); ++i) {
+    const gt::HostMat& m = s->mats[i];
+    GMat& g = gm[i];
+    std::memset(&g, 0, sizeof g);
+    for (int c = 0; c < 3; ++c) g.color[c] = float(m.color[c]);
+    g.kind = uint32_t(m.kind);
+    g.param = float(m.param);
+    double lum = 0.2126 * m.color[0] + 0.7152 * m.color[1] + 0.0722 * m.color[2];  // material.rs:1724-1726
+    g.f0 = float(std::min(std::max(lum, 0.0), 1.0));
+  }
+  GT_HIP(s->dmats.ensure(gm.size()));
+  hipStream_t st = s->ctx->stream;
+  if (!pk.nodes2.empty()) GT_HIP(hipMemcpyAsync(s->nodes2.p, pk.nodes2.data(), pk.nodes2.size() * sizeof(GNode2), hipMemcpyHostToDevice, st));
+  if (!pk.nodes4.empty()) GT_HIP(hipMemcpyAsync(s->nodes4.p, pk.nodes4.data(), pk.nodes4.size() * sizeof(GNode4), hipMemcpyHostToDevice, st));
+  if (!pk.prims.empty()) GT_HIP(hipMemcpyAsync(s->dprims.p, pk.prims.data(), pk.prims.size() * sizeof(GPrim), hipMemcpyHostToDevice, st));
+  if (!pk.prim_ids.empty()) GT_HIP(hipMemcpyAsync(s->prim_ids.p, pk.prim_ids.data(), pk.prim_ids.size() * sizeof(uint32_t), hipMemcpyHostToDevice, st));
+  if (!gm.empty()) GT_HIP(hipMemcpyAsync(s->dmats.p, gm.data(), gm.size() * sizeof(GMat), hipMemcpyHostToDevice, st));
+  GT_HIP(hipStreamSynchronize(st));
+  s->device_bytes = pk.nodes2.size() * sizeof(GNode2) + pk.nodes4.size() * sizeof(GNode4) +
+                    pk.prims.size() * sizeof(GPrim) + pk.prim_ids.size() * 4 + gm.size() * sizeof(GMat);
+  return GT_OK;
+}
+
+int build_scene(gt_scene* s, uint32_t leaf_size) {
+  s->leaf_size = leaf_size == 0 ? 1 : leaf_size;
+  s->packed = gt::PackedScene();
+  if (!s->prims.empty()) {
+    gt::build_binary_bvh(s->prims, s->leaf_size, &s->bvh);
+    gt::pack_scene(s->prims, s->bvh, &s->packed);
+  } else {
+    s->bvh = gt::BinaryBvh();
+    // empty scene: an inverted root box that every slab test rejects
+    for (int a = 0; a < 3; ++a) { s->packed.root_lo[a] = 1.0f; s->packed.root_hi[a] = -1.0f; }
+    s->packed.root_ref2 = s->packed.root_ref4 = GT_LEAF_BIT;
+  }
+  int rc = upload_scene(s);
+  if (rc != GT_OK) return rc;
+  s->built = true;
+  return GT_OK;
+}
+
+int ensure_built(gt_scene* s) {
+  if (s->built) return GT_OK;
+  return build_scene(s, 4);  // RayScene::cached_bvh builds with the default options (scene.rs:1567-1573)
+}
+
+GLight pack_light(const gt::HostPrim& p, double weight, double total) {
+  GLight g;
+  std::memset(&g, 0, sizeof g);
+  const double* q = p.q;
+  g.b[3] = float(weight / total);
+  g.w[2] = float(weight);
+  g.c[3] = float(p.kind);
+  switch (p.kind) {
+    case GK_SPHERE:
+      for (int i = 0; i < 3; ++i) g.a[i] = float(q[i]);
+      g.a[3] = float(std::fmax(q[3], 0.0));
+      break;
+    case GK_MOVING_SPHERE:
+      for (int i = 0; i < 3; ++i) { g.a[i] = float(q[i]); g.b[i] = float(q[3 + i]); }
+      g.a[3] = float(std::fmax(q[6], 0.0));
+      break;
+    case GK_TRIANGLE: {
+      double e1[3], e2[3];
+      for (int i = 0; i < 3; ++i) { e1[i] = q[3 + i] - q[i]; e2[i] = q[6 + i] - q[i]; }
+      double n[3] = {e1[1] * e2[2] - e1[2] * e2[1], e1[2] * e2[0] - e1[0] * e2[2], e1[0] * e2[1] - e1[1] * e2[0]};
+      double len = std::sqrt(n[0] * n[0] + n[1] * n[1] + n[2] * n[2]);
+      for (int i = 0; i < 3; ++i) {
+        g.a[i] = float(q[i]); g.b[i] = float(e1[i]); g.c[i] = float(e2[i]);
+        g.n[i] = len < kEps ? 0.0f : float(n[i] / len);
+      }
+      g.n[3] = float(0.5 * len);  // triangle_pdf_value (object.rs:1088-1099)
+      break;
+    }
+    default: {
+      gt::HostPrim hp = p;
+      GPrim gp = gt::pack_prim(hp, false);
+      for (int i = 0; i < 3; ++i) { g.a[i] = gp.p0[i]; g.b[i] = gp.p1[i]; g.c[i] = gp.p2[i]; g.n[i] = gp.p3[i]; }
+      g.w[0] = gp.p1[3];                 // 1 / |u x v|
+      g.w[1] = gp.p2[3];                 // |u x v|^2
+      g.w[3] = gp.p3[3];                 // plane d
+      // quad_pdf_value area = |u x v| (object.rs:1075-1086)
+      const double* u = q + 3;
+      const double* v = q + 6;
+      double nn[3] = {u[1] * v[2] - u[2] * v[1], u[2] * v[0] - u[0] * v[2], u[0] * v[1] - u[1] * v[0]};
+      g.n[3] = float(std::sqrt(nn[0] * nn[0] + nn[1] * nn[1] + nn[2] * nn[2]));
+      break;
+    }
+  }
+  return g;
+}
+
+int upload_lights(gt_lights* l) {
+  if (!l->dirty) return GT_OK;
+  std::vector<GLight> v(l->targets.size());
+  for (size_t i = 0; i < v.size(); ++i) v[i] = pack_light(l->targets[i], l->weights[i], l->total);
+  GT_HIP(l->dev.ensure(v.size()));
+  if (!v.empty()) {
+    GT_HIP(hipMemcpyAsync(l->dev.p, v.data(), v.size() * sizeof(GLight), hipMemcpyHostToDevice, l->ctx->stream));
+    GT_HIP(hipStreamSynchronize(l->ctx->stream));
+  }
+  l->dirty = false;
+  return GT_OK;
+}
+
+// RayCamera::initialized + validate_view (camera.rs:1068-1201)
+int make_camera(const gt_camera* c, GCamera* g) {
+  if (!c) return fail(GT_EINVAL, "camera is null");
+  if (c->image_width == 0) return fail(GT_EINVAL, "image width must be positive");
+  if (!(std::isfinite(c->aspect_ratio) && c->aspect_ratio > 0.0)) return fail(GT_EINVAL, "aspect ratio must be positive and finite");
+  if (!(std::isfinite(c->vertical_fov) && c->vertical_fov > 0.0 && c->vertical_fov < 180.0))
+    return fail(GT_EINVAL, "vertical field of view must be finite and in 0..180 degrees");
+  double from[3], at[3], up[3];
+  for (int i = 0; i < 3; ++i) { from[i] = c->lookfrom[i]; at[i] = c->lookat[i]; up[i] = c->view_up[i]; }
+  double wv[3] = {from[0] - at[0], from[1] - at[1], from[2] - at[2]};
+  if (!(wv[0] * wv[0] + wv[1] * wv[1] + wv[2] * wv[2] > kEps)) return fail(GT_EINVAL, "lookfrom and lookat must be distinct");
+  if (!(up[0] * up[0] + up[1] * up[1] + up[2] * up[2] > kEps)) return fail(GT_EINVAL, "view-up vector must be nonzero");
+  auto norm = [](const double* v, double* o) {
+    double l = std::sqrt(v[0] * v[0] + v[1] * v[1] + v[2] * v[2]);
+    if (l < kEps) { o[0] = o[1] = o[2] = 0; return; }
+    for (int i = 0; i < 3; ++i) o[i] = v[i] / l;
+  };
+  auto cross = [](const double* a, const double* b, double* o) {
+    o[0] = a[1] * b[2] - a[2] * b[1]; o[1] = a[2] * b[0] - a[0] * b[2]; o[2] = a[0] * b[1] - a[1] * b[0];
+  };
+  double fwd_raw[3] = {at[0] - from[0], at[1] - from[1], at[2] - from[2]}, fwd[3], right_raw[3], right[3], upv[3];
+  norm(fwd_raw, fwd);
+  cross(fwd, up, right_raw);
+  norm(right_raw, right);
+  double fl = fwd[0] * fwd[0] + fwd[1] * fwd[1] + fwd[2] * fwd[2];
+  double rl = right[0] * right[0] + right[1] * right[1] + right[2] * right[2];
+  if (fl <= kEps || rl <= kEps) return fail(GT_EINVAL, "view-up vector must not be parallel to the viewing direction");
+  cross(right, fwd, upv);
+  if (!(std::isfinite(c->defocus_angle) && c->defocus_angle >= 0.0 && c->defocus_angle < 180.0))
+    return fail(GT_EINVAL, "defocus angle must be finite and in 0..180 degrees");
+  if (!(std::isfinite(c->focus_distance) && c->focus_distance > 0.0)) return fail(GT_EINVAL, "focus distance must be positive and finite");
+  if (!(std::isfinite(c->shutter_start) && std::isfinite(c->shutter_end) && c->shutter_start <= c->shutter_end))
+    return fail(GT_EINVAL, "shutter interval must be finite and ordered");
+  int nbg = c->background_kind == GT_BACKGROUND_VERTICAL_GRADIENT ? 9 : 3;
+  for (int i = 0; i < nbg; ++i)
+    if (!std::isfinite(c->background[i])) return fail(GT_EINVAL, "background color components must be finite");
+  if (c->rng_mode != GT_RNG_COUNTER)
+    return fail(GT_EUNSUPPORTED, "the GPU path draws per-(pixel, sample, bounce) streams only (rng_mode GT_RNG_COUNTER)");
+  std::memset(g, 0, sizeof *g);
+  g->width = c->image_width;
+  g->height = gt_camera_image_height(c);
+  double theta = c->vertical_fov * 3.14159265358979323846 / 180.0;
+  double h = std::tan(theta * 0.5);
+  double vh = 2.0 * h * c->focus_distance;
+  double vw = vh * (double(g->width) / double(g->height));
+  double w[3] = {-fwd[0], -fwd[1], -fwd[2]};
+  double vu[3], vv[3], du[3], dv[3], ul[3], p00[3];
+  for (int i = 0; i < 3; ++i) {
+    vu[i] = vw * right[i];
+    vv[i] = vh * -upv[i];
+    du[i] = vu[i] / double(g->width);
+    dv[i] = vv[i] / double(g->height);
+  }
+  for (int i = 0; i < 3; ++i) ul[i] = from[i] - c->focus_distance * w[i] - vu[i] / 2.0 - vv[i] / 2.0;
+  for (int i = 0; i < 3; ++i) p00[i] = ul[i] + 0.5 * (du[i] + dv[i]);
+  double dr = c->focus_distance * std::tan(c->defocus_angle * 0.5 * 3.14159265358979323846 / 180.0);
+  for (int i = 0; i < 3; ++i) {
+    g->center[i] = float(from[i]);
+    g->pixel00[i] = float(p00[i]);
+    g->du[i] = float(du[i]);
+    g->dv[i] = float(dv[i]);
+    g->disk_u[i] = float(dr * right[i]);
+    g->disk_v[i] = float(dr * upv[i]);
+  }
+  g->shutter_start = float(c->shutter_start);
+  g->shutter_end = float(c->shutter_end);
+  g->spp = gt_camera_effective_spp(c);
+  g->mode = c->pixel_sample_mode == GT_PIXEL_RANDOM ? 0 : 1;
+  g->grid = 1;
+  if (c->pixel_sample_mode == GT_PIXEL_STRATIFIED)
+    g->grid = std::max<uint32_t>(uint32_t(std::sqrt(double(std::max<uint32_t>(c->samples_per_pixel, 1)))), 1u);
+  else if (c->pixel_sample_mode == GT_PIXEL_STRATIFIED_GRID)
+    g->grid = std::max<uint32_t>(c->stratified_grid_width, 1u);
+  g->defocus = c->defocus_angle <= 0.0 ? 0u : 1u;
+  g->seed = c->rng_seed;
+  g->max_depth = c->max_depth;
+  g->rr_min_depth = c->russian_roulette_min_depth;
+  g->background_kind = c->background_kind;
+  for (int i = 0; i < 9; ++i) g->background[i] = float(c->background[i]);
+  return GT_OK;
+}
+
+gtd::DScene device_scene(const gt_scene* s, uint32_t width) {
+  gtd::DScene d;
+  d.nodes2 = s->nodes2.p;
+  d.nodes4 = s->nodes4.p;
+  d.prims = s->dprims.p;
+  d.prim_ids = s->prim_ids.p;
+  d.mats = s->dmats.p;
+  for (int a = 0; a < 3; ++a) { d.root_lo[a] = s->packed.root_lo[a]; d.root_hi[a] = s->packed.root_hi[a]; }
+  bool wide = width == 4 && !s->packed.nodes4.empty();
+  d.width = wide ? 4 : 2;
+  d.root_ref = wide ? s->packed.root_ref4 : s->packed.root_ref2;
+  if (s->packed.nodes2.empty()) { d.width = 2; d.root_ref = s->packed.root_ref2; }
+  return d;
+}
+
+uint64_t stack_depth(const gt_scene* s, uint32_t width) {
+  uint64_t d = width == 4 ? s->packed.depth4 * 3 : s->packed.depth2;  // wide nodes push up to 3 per level
+  return d + 2;
+}
+
+constexpr uint32_t kStackCap = 16;
+
+// Persistent grid: blocks_per_cu per CU, or as many as the occupancy of the
+// traversal kernels allows (their LDS stack is the binding limit).
+uint32_t grid_for(gt_context* ctx) {
+  uint32_t bpc = ctx->tuning.blocks_per_cu;
+  if (bpc == 0) {
+    uint32_t b = ctx->tuning.block_size;
+    size_t lds = gtk::stack_lds_bytes(b, kStackCap);
+    bpc = std::max<uint32_t>(1, std::min(gtk::occupancy_blocks(0, b, lds), gtk::occupancy_blocks(2, b, lds)));
+  }
+  return std::max<uint32_t>(1, ctx->cus * bpc);
+}
+
+int ensure_queues(gt_context* ctx, size_t paths, uint64_t depth_needed, uint32_t max_depth) {
+  if (paths > ctx->cap_paths) {
+    for (int k = 0; k < 2; ++k) {
+      GT_HIP(ctx->q_o[k].ensure(paths)); GT_HIP(ctx->q_d[k].ensure(paths)); GT_HIP(ctx->q_t[k].ensure(paths));
+      GT_HIP(ctx->s_o[k].ensure(paths)); GT_HIP(ctx->s_d[k].ensure(paths)); GT_HIP(ctx->s_w[k].ensure(paths));
+    }
+    GT_HIP(ctx->radiance.ensure(paths));
+    GT_HIP(ctx->hits.ensure(paths));
+    ctx->cap_paths = paths;
+  }
+  (void)depth_needed;
+  size_t words = size_t(max_depth + 2) * gtk::kCtrStride;
+  GT_HIP(ctx->ctr.ensure(words));
+  return GT_OK;
+}
+
+int ensure_gstack(gt_context* ctx, uint64_t depth_needed, uint32_t cap) {
+  uint32_t grid = grid_for(ctx);
+  uint32_t stride = grid * ctx->tuning.block_size;
+  uint64_t levels = depth_needed > cap ? depth_needed - cap : 1;
+  GT_HIP(ctx->gstack.ensure(size_t(levels) * stride));
+  ctx->gstride = stride;
+  return GT_OK;
+}
+
+gtk::Queues queues_of(gt_context* ctx) {
+  gtk::Queues q;
+  for (int k = 0; k < 2; ++k) {
+    q.q_o[k] = ctx->q_o[k].p; q.q_d[k] = ctx->q_d[k].p; q.q_t[k] = ctx->q_t[k].p;
+    q.s_o[k] = ctx->s_o[k].p; q.s_d[k] = ctx->s_d[k].p; q.s_w[k] = ctx->s_w[k].p;
+  }
+  q.hits = ctx->hits.p;
+  q.radiance = ctx->radiance.p;
+  q.ctr = ctx->ctr.p;
+  q.gstack = ctx->gstack.p;
+  q.gstride = ctx->gstride;
+  q.stack_cap = kStackCap;
+  return q;
+}
+
+int make_frame(gt_context* ctx, const gt_scene* scene, const gt_camera* camera, const gt_lights* lights,
+               const gt_env* env, gtk::Frame* f) {
+  int rc = make_camera(camera, &f->cam);
+  if (rc != GT_OK) return rc;
+  f->scene = device_scene(scene, ctx->tuning.bvh_width);
+  f->lights.l = lights ? lights->dev.p : nullptr;
+  f->lights.n = lights ? uint32_t(lights->targets.size()) : 0u;
+  f->lights.total = lights ? float(lights->total) : 0.0f;
+  std::memset(&f->env, 0, sizeof f->env);
+  if (env) {
+    f->env.kind = env->kind;
+    for (int i = 0; i < 3; ++i) f->env.color[i] = float(env->color[i]);
+    f->env.w = env->w; f->env.h = env->h;
+    f->env.uniform = env->uniform ? 1u : 0u;
+    f->env.tex = env->tex.p; f->env.row_cdf = env->row_cdf.p; f->env.col_cdf = env->col_cdf.p; f->env.pdf = env->pdf.p;
+  }
+  f->nee = camera->direct_lighting_mode == GT_LIGHTING_NEE ? 1u : 0u;
+  f->has_lights = lights ? 1u : 0u;
+  f->light_weight = float(std::min(std::max(camera->light_pdf_weight, 0.0), 1.0));
+  f->npix = f->cam.width * f->cam.height;
+  return GT_OK;
+}
+
+int check_objects(gt_context* ctx, const gt_scene* scene, const gt_lights* lights, const gt_env* env) {
+  if (!ctx) return fail(GT_EINVAL, "context is null");
+  if (!scene) return fail(GT_EINVAL, "scene is null");
+  if (scene->ctx != ctx || (lights && lights->ctx != ctx) || (env && env->ctx != ctx))
+    return fail(GT_EINVAL, "scene, lights and environment must belong to the rendering context");
+  if (!std::isfinite(lights ? lights->total : 0.0)) return fail(GT_EINVAL, "light weights must be finite");
+  return GT_OK;
+}
+
+int render_impl(gt_context* ctx, const gt_scene* cscene, const gt_camera* camera, const gt_lights* clights,
+                const gt_env* env, void* d_out, int out_is_double, float* d_albedo, float* d_normal,
+                gt_render_stats* stats) {
+  int rc = check_objects(ctx, cscene, clights, env);
+  if (rc != GT_OK) return rc;
+  gt_scene* scene = const_cast<gt_scene*>(cscene);
+  gt_lights* lights = const_cast<gt_lights*>(clights);
+  GT_HIP(hipSetDevice(ctx->device));
+  if ((rc = ensure_built(scene)) != GT_OK) return rc;
+  if (lights && (rc = upload_lights(lights)) != GT_OK) return rc;
+  gtk::Frame f;
+  if ((rc = make_frame(ctx, scene, camera, lights, env, &f)) != GT_OK) return rc;
+  const uint32_t npix = f.npix;
+  const uint32_t spp = f.cam.spp;
+  const uint32_t depth = f.cam.max_depth;
+  if (depth > 4096) return fail(GT_EINVAL, "max_depth above 4096 is not supported");
+  uint64_t max_paths = ctx->tuning.max_paths ? ctx->tuning.max_paths : (uint64_t(1) << 23);
+  max_paths = std::min<uint64_t>(max_paths, uint64_t(1) << 31);
+  if (uint64_t(npix) > max_paths) max_paths = npix;
+  if (uint64_t(npix) >= (uint64_t(1) << 31)) return fail(GT_EINVAL, "image too large (2^31 pixels)");
+  uint32_t kmax = uint32_t(std::min<uint64_t>(max_paths / npix, spp));
+  kmax = std::max<uint32_t>(kmax, 1);
+  uint32_t nbatch = (spp + kmax - 1) / kmax;
+  uint32_t k = (spp + nbatch - 1) / nbatch;
+  if ((rc = ensure_queues(ctx, size_t(k) * npix, 0, depth)) != GT_OK) return rc;
+  uint64_t dneed = stack_depth(scene, f.scene.width);
+  if ((rc = ensure_gstack(ctx, dneed, kStackCap)) != GT_OK) return rc;
+  GT_HIP(ctx->acc.ensure(npix));
+  gtk::Queues q = queues_of(ctx);
+  gtk::Launch L{grid_for(ctx), ctx->tuning.block_size, ctx->stream};
+  hipStream_t st = ctx->stream;
+  size_t ctr_words = size_t(depth + 2) * gtk::kCtrStride;
+  const bool want_stats = stats != nullptr;
+  const bool time_kernels = want_stats;
+  size_t nev = time_kernels ? 2 + 5 * size_t(nbatch) * depth : 0;
+  while (ctx->events.size() < nev) {
+    hipEvent_t e;
+    GT_HIP(hipEventCreate(&e));
+    ctx->events.push_back(e);
+  }
+  if (want_stats && ctx->host_ctr_words < ctr_words * nbatch) {
+    if (ctx->host_ctr) GT_HIP(hipHostFree(ctx->host_ctr));
+    ctx->host_ctr = nullptr;
+    GT_HIP(hipHostMalloc(reinterpret_cast<void**>(&ctx->host_ctr), ctr_words * nbatch * sizeof(uint32_t)));
+    ctx->host_ctr_words = ctr_words * nbatch;
+  }
+  // events per bounce: [e0] extend [e1][e2] shade [e3] connect [e4]
+  size_t ev = 0;
+  if (time_kernels) GT_HIP(hipEventRecord(ctx->events[ev++], st));
+  GT_HIP(hipMemsetAsync(ctx->acc.p, 0, size_t(npix) * sizeof(double4), st));
+  const bool conn_l = f.nee && lights != nullptr;
+  const bool conn_e = f.nee && env != nullptr;
+  for (uint32_t b = 0; b < nbatch; ++b) {
+    uint32_t s0 = b * k;
+    uint32_t ns = std::min(k, spp - s0);
+    gtk::reset_counters(L, q.ctr, uint32_t(ctr_words));
+    gtk::raygen(L, f, q, s0, ns);
+    for (uint32_t bounce = 0; bounce < depth; ++bounce) {
+      if (time_kernels) GT_HIP(hipEventRecord(ctx->events[ev++], st));
+      gtk::extend(L, f, q, bounce);
+      if (time_kernels) {
+        GT_HIP(hipEventRecord(ctx->events[ev++], st));
+        GT_HIP(hipEventRecord(ctx->events[ev++], st));
+      }
+      gtk::shade(L, f, q, bounce, s0, bounce + 1 == depth ? 1u : 0u);
+      if (time_kernels) GT_HIP(hipEventRecord(ctx->events[ev++], st));
+      if (conn_l) gtk::connect(L, f, q, bounce, 0);
+      if (conn_e) gtk::connect(L, f, q, bounce, 1);
+      if (time_kernels) GT_HIP(hipEventRecord(ctx->events[ev++], st));
+    }
+    gtk::accumulate(L, f, q, ctx->acc.p, ns);
+    if (want_stats)
+      GT_HIP(hipMemcpyAsync(ctx->host_ctr + ctr_words * b, q.ctr, ctr_words * sizeof(uint32_t), hipMemcpyDeviceToHost, st));
+  }
+  if (d_out) gtk::finalize(L, ctx->acc.p, npix, d_out, out_is_double);
+  if (d_albedo) gtk::aov(L, f, q, d_albedo, d_normal);
+  if (time_kernels) GT_HIP(hipEventRecord(ctx->events[ev++], st));
+  GT_HIP(hipGetLastError());
+  if (want_stats) {
+    GT_HIP(hipStreamSynchronize(st));
+    std::memset(stats, 0, sizeof *stats);
+    stats->camera_paths = uint64_t(npix) * spp;
+    for (uint32_t b = 0; b < nbatch; ++b) {
+      const uint32_t* c = ctx->host_ctr + ctr_words * b;
+      for (uint32_t bounce = 0; bounce < depth; ++bounce) {
+        stats->extension_rays += c[bounce * gtk::kCtrStride + gtk::C_QCOUNT];
+        if (conn_l) stats->shadow_rays += c[bounce * gtk::kCtrStride + gtk::C_SL_COUNT];
+        if (conn_e) stats->shadow_rays += c[bounce * gtk::kCtrStride + gtk::C_SE_COUNT];
+      }
+    }
+    float ms = 0;
+    GT_HIP(hipEventElapsedTime(&ms, ctx->events[0], ctx->events[ev - 1]));
+    stats->device_ms = ms;
+    size_t base = 1;
+    for (uint32_t b = 0; b < nbatch; ++b) {
+      for (uint32_t bounce = 0; bounce < depth; ++bounce) {
+        float a = 0, s = 0, c = 0;
+        GT_HIP(hipEventElapsedTime(&a, ctx->events[base], ctx->events[base + 1]));
+        GT_HIP(hipEventElapsedTime(&s, ctx->events[base + 2], ctx->events[base + 3]));
+        GT_HIP(hipEventElapsedTime(&c, ctx->events[base + 3], ctx->events[base + 4]));
+        stats->extend_ms += a; stats->shade_ms += s; stats->connect_ms += c;
+        stats->extend_launches++;
+        base += 5;
+      }
+    }
+    stats->accepted_samples = 0;
+  }
+  return GT_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* gt_last_error(void) { return g_error.c_str(); }
+const char* gt_version(void) { return "gartus-mi355x 0.1 (gfx950 wavefront path tracer)"; }
+
+void gt_camera_default(gt_camera* c) {  // camera.rs:1034-1064
+  if (!c) return;
+  std::memset(c, 0, sizeof *c);
+  c->image_width = 100;
+  c->aspect_ratio = 1.0;
+  c->samples_per_pixel = 1;
+  c->pixel_sample_mode = GT_PIXEL_RANDOM;
+  c->stratified_grid_width = 1;
+  c->max_depth = 10;
+  c->russian_roulette_min_depth = 5;
+  c->rng_seed = 1;
+  c->vertical_fov = 90.0;
+  c->lookfrom[0] = 0; c->lookfrom[1] = 0; c->lookfrom[2] = 0;
+  c->lookat[0] = 0; c->lookat[1] = 0; c->lookat[2] = -1;
+  c->view_up[0] = 0; c->view_up[1] = 1; c->view_up[2] = 0;
+  c->defocus_angle = 0.0;
+  c->focus_distance = 1.0;
+  c->shutter_start = 0.0;
+  c->shutter_end = 1.0;
+  c->background_kind = GT_BACKGROUND_CONSTANT;
+  c->background[0] = 0.70; c->background[1] = 0.80; c->background[2] = 1.00;
+  c->direct_lighting_mode = GT_LIGHTING_CONTINUATION;
+  c->light_pdf_weight = 0.5;
+  c->rng_mode = GT_RNG_COUNTER;
+}
+
+uint32_t gt_camera_image_height(const gt_camera* c) {  // camera.rs:1071
+  if (!c || c->image_width == 0 || !(c->aspect_ratio > 0.0)) return 1;
+  double h = double(c->image_width) / c->aspect_ratio;
+  uint32_t hh = h >= 4294967295.0 ? 0xffffffffu : (h > 0.0 ? uint32_t(h) : 0u);
+  return std::max<uint32_t>(hh, 1u);
+}
+
+uint32_t gt_camera_effective_spp(const gt_camera* c) {  // camera.rs:1547-1557
+  uint32_t s = std::max<uint32_t>(c->samples_per_pixel, 1u);
+  if (c->pixel_sample_mode == GT_PIXEL_RANDOM) return s;
+  if (c->pixel_sample_mode == GT_PIXEL_STRATIFIED) {
+    uint32_t g = std::max<uint32_t>(uint32_t(std::sqrt(double(s))), 1u);
+    return g * g;
+  }
+  uint32_t g = std::max<uint32_t>(c->stratified_grid_width, 1u);
+  return g * g;
+}
+
+int gt_context_create(int device, gt_context** out) {
+  if (!out) return fail(GT_EINVAL, "out is null");
+  *out = nullptr;
+  int n = 0;
+  GT_HIP(hipGetDeviceCount(&n));
+  if (device < 0 || device >= n) return fail(GT_EINVAL, "no such HIP device");
+  std::unique_ptr<gt_context> ctx(new gt_context());
+  ctx->device = device;
+  GT_HIP(hipSetDevice(device));
+  GT_HIP(hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking));
+  hipDeviceProp_t props;
+  GT_HIP(hipGetDeviceProperties(&props, device));
+  ctx->cus = uint32_t(props.multiProcessorCount);
+  ctx->tuning.block_size = 128;
+  ctx->tuning.blocks_per_cu = 0;
+  ctx->tuning.max_paths = 0;
+  ctx->tuning.bvh_width = 2;
+  ctx->tuning.use_graph = 0;
+  *out = ctx.release();
+  return GT_OK;
+}
+
+int gt_context_destroy(gt_context* ctx) {
+  if (!ctx) return GT_OK;
+  (void)hipSetDevice(ctx->device);
+  (void)hipStreamSynchronize(ctx->stream);
+  delete ctx;
+  return GT_OK;
+}
+
+int gt_context_synchronize(gt_context* ctx) {
+  if (!ctx) return fail(GT_EINVAL, "context is null");
+  GT_HIP(hipStreamSynchronize(ctx->stream));
+  return GT_OK;
+}
+
+void* gt_context_stream(gt_context* ctx) { return ctx ? static_cast<void*>(ctx->stream) : nullptr; }
+
+int gt_context_get_tuning(gt_context* ctx, gt_tuning* out) {
+  if (!ctx || !out) return fail(GT_EINVAL, "null argument");
+  *out = ctx->tuning;
+  return GT_OK;
+}
+
+int gt_context_set_tuning(gt_context* ctx, const gt_tuning* t) {
+  if (!ctx || !t) return fail(GT_EINVAL, "null argument");
+  if (t->block_size < 64 || t->block_size > 1024 || t->block_size % 64 != 0)
+    return fail(GT_EINVAL, "block_size must be a multiple of 64 in 64..1024");
+  if (t->bvh_width != 2 && t->bvh_width != 4) return fail(GT_EINVAL, "bvh_width must be 2 or 4");
+  size_t lds = gtk::stack_lds_bytes(t->block_size, kStackCap);
+  if (lds > 64 * 1024) return fail(GT_EINVAL, "block_size too large for the LDS traversal stack");
+  ctx->tuning = *t;
+  return GT_OK;
+}
+
+// ---- scene ----
+int gt_scene_create(gt_context* ctx, gt_scene** out) {
+  if (!ctx || !out) return fail(GT_EINVAL, "null argument");
+  gt_scene* s = new gt_scene();
+  s->ctx = ctx;
+  *out = s;
+  return GT_OK;
+}
+int gt_scene_destroy(gt_scene* s) {
+  if (!s) return GT_OK;
+  (void)hipSetDevice(s->ctx->device);
+  (void)hipStreamSynchronize(s->ctx->stream);
+  delete s;
+  return GT_OK;
+}
+
+static int add_material(gt_scene* s, int kind, double r, double g, double b, double param, uint32_t* id) {
+  if (!s || !id) return fail(GT_EINVAL, "null argument");
+  if (s->mats.size() >= GT_MATERIAL_MASK) return fail(GT_EINVAL, "too many materials");
+  gt::HostMat m;
+  m.kind = kind;
+  m.color[0] = r; m.color[1] = g; m.color[2] = b;
+  m.param = param;
+  *id = uint32_t(s->mats.size());
+  s->mats.push_back(m);
+  s->built = false;
+  return GT_OK;
+}
+int gt_scene_add_lambertian(gt_scene* s, double r, double g, double b, uint32_t* id) {
+  if (!finite3(r, g, b)) return fail(GT_EINVAL, "Lambertian albedo must be finite");  // material.rs:363
+  return add_material(s, GM_LAMBERTIAN, r, g, b, 0.0, id);
+}
+int gt_scene_add_diffuse_light(gt_scene* s, double r, double g, double b, uint32_t* id) {
+  if (!finite3(r, g, b)) return fail(GT_EINVAL, "diffuse light color must be finite");  // material.rs:946
+  return add_material(s, GM_DIFFUSE_LIGHT, r, g, b, 0.0, id);
+}
+int gt_scene_add_ggx(gt_scene* s, double r, double g, double b, double roughness, uint32_t* id) {
+  if (!finite3(r, g, b) || !std::isfinite(roughness))
+    return fail(GT_EINVAL, "GGX microfacet material values must be finite");  // material.rs:1387
+  return add_material(s, GM_GGX, r, g, b, std::min(std::max(roughness, 0.0), 1.0), id);
+}
+int gt_scene_add_metal(gt_scene* s, double r, double g, double b, double fuzz, uint32_t* id) {
+  if (!finite3(r, g, b) || !std::isfinite(fuzz)) return fail(GT_EINVAL, "metal material values must be finite");  // material.rs:1968
+  return add_material(s, GM_METAL, r, g, b, std::min(std::max(fuzz, 0.0), 1.0), id);
+}
+int gt_scene_add_dielectric(gt_scene* s, double ior, uint32_t* id) {
+  if (!std::isfinite(ior)) return fail(GT_EINVAL, "refraction index must be finite");
+  return add_material(s, GM_DIELECTRIC, 1.0, 1.0, 1.0, ior, id);
+}
+
+static int add_prims(gt_scene* s, int kind, int stride, const double* data, size_t n, const uint32_t* mids) {
+  if (!s) return fail(GT_EINVAL, "scene is null");
+  if (n == 0) return GT_OK;
+  if (!data || !mids) return fail(GT_EINVAL, "null primitive data");
+  for (size_t i = 0; i < n; ++i)
+    if (mids[i] >= s->mats.size()) return fail(GT_EINVAL, "ray scene material id out of bounds");  // scene.rs:1158-1161
+  if (s->prims.size() + n >= (size_t(1) << 31)) return fail(GT_EINVAL, "too many primitives");
+  s->prims.reserve(s->prims.size() + n);
+  for (size_t i = 0; i < n; ++i) {
+    gt::HostPrim p;
+    std::memset(&p, 0, sizeof p);
+    p.kind = kind;
+    for (int j = 0; j < stride; ++j) p.q[j] = data[size_t(stride) * i + j];
+    if (kind == GK_SPHERE) p.q[3] = std::fmax(p.q[3], 0.0);          // geometry.rs:445-450
+    if (kind == GK_MOVING_SPHERE) p.q[6] = std::fmax(p.q[6], 0.0);   // geometry.rs:498-504
+    p.material = mids[i];
+    s->prims.push_back(p);
+  }
+  s->built = false;  // invalidate_bvh (scene.rs:1575-1577)
+  return GT_OK;
+}
+int gt_scene_add_spheres(gt_scene* s, const double* d, size_t n, const uint32_t* m) { return add_prims(s, GK_SPHERE, 4, d, n, m); }
+int gt_scene_add_moving_spheres(gt_scene* s, const double* d, size_t n, const uint32_t* m) { return add_prims(s, GK_MOVING_SPHERE, 7, d, n, m); }
+int gt_scene_add_triangles(gt_scene* s, const double* d, size_t n, const uint32_t* m) { return add_prims(s, GK_TRIANGLE, 9, d, n, m); }
+int gt_scene_add_quads(gt_scene* s, const double* d, size_t n, const uint32_t* m) { return add_prims(s, GK_QUAD, 9, d, n, m); }
+
+int gt_scene_build(gt_scene* s, uint32_t leaf_size) {
+  if (!s) return fail(GT_EINVAL, "scene is null");
+  return build_scene(s, leaf_size);
+}
+
+int gt_scene_get_info(const gt_scene* s, gt_scene_info* out) {
+  if (!s || !out) return fail(GT_EINVAL, "null argument");
+  std::memset(out, 0, sizeof *out);
+  out->primitive_count = s->prims.size();
+  out->material_count = s->mats.size();
+  if (s->built && !s->bvh.nodes.empty()) {
+    out->bvh_node_count = s->bvh.nodes.size();
+    out->bvh_depth = s->bvh.depth;
+    out->device_bytes = s->device_bytes;
+    for (int a = 0; a < 3; ++a) { out->bounds_min[a] = s->bvh.nodes[0].bounds.lo[a]; out->bounds_max[a] = s->bvh.nodes[0].bounds.hi[a]; }
+  }
+  return GT_OK;
+}
+
+// ---- intersect ----
+int gt_intersect(gt_context* ctx, const gt_scene* cs, const gt_ray* rays, size_t n, float t_min, gt_hit* out,
+                 gt_traversal_stats* stats) {
+  int rc = check_objects(ctx, cs, nullptr, nullptr);
+  if (rc != GT_OK) return rc;
+  if (n == 0) { if (stats) std::memset(stats, 0, sizeof *stats); return GT_OK; }
+  if (!rays || !out) return fail(GT_EINVAL, "null rays or hits");
+  gt_scene* s = const_cast<gt_scene*>(cs);
+  GT_HIP(hipSetDevice(ctx->device));
+  if ((rc = ensure_built(s)) != GT_OK) return rc;
+  gtk::Frame f;
+  std::memset(&f, 0, sizeof f);
+  f.scene = device_scene(s, ctx->tuning.bvh_width);
+  if ((rc = ensure_gstack(ctx, stack_depth(s, f.scene.width), kStackCap)) != GT_OK) return rc;
+  gtk::Queues q = queues_of(ctx);
+  gtk::Launch L{grid_for(ctx), ctx->tuning.block_size, ctx->stream};
+  const size_t chunk = size_t(1) << 22;
+  size_t m = std::min(n, chunk);
+  DevBuf<float4> dr;
+  DevBuf<float> dt, du, dv, dn;
+  DevBuf<int32_t> dp;
+  DevBuf<uint32_t> dfront;
+  DevBuf<unsigned long long> dstats;
+  GT_HIP(dr.ensure(2 * m)); GT_HIP(dt.ensure(m)); GT_HIP(du.ensure(m)); GT_HIP(dv.ensure(m));
+  GT_HIP(dn.ensure(3 * m)); GT_HIP(dp.ensure(m)); GT_HIP(dfront.ensure(m));
+  if (stats) {
+    GT_HIP(dstats.ensure(8));
+    GT_HIP(hipMemsetAsync(dstats.p, 0, 8 * sizeof(unsigned long long), ctx->stream));
+  }
+  std::vector<float4> hr(2 * m);
+  std::vector<float> ht(m), hu(m), hv(m), hn(3 * m);
+  std::vector<int32_t> hp(m);
+  std::vector<uint32_t> hf(m);
+  for (size_t off = 0; off < n; off += chunk) {
+    size_t c = std::min(chunk, n - off);
+    for (size_t i = 0; i < c; ++i) {
+      const gt_ray& r = rays[off + i];
+      hr[2 * i] = make_float4(r.origin[0], r.origin[1], r.origin[2], r.time);
+      hr[2 * i + 1] = make_float4(r.direction[0], r.direction[1], r.direction[2], r.t_max);
+    }
+    GT_HIP(hipMemcpyAsync(dr.p, hr.data(), 2 * c * sizeof(float4), hipMemcpyHostToDevice, ctx->stream));
+    gtk::IntersectOut o{dt.p, du.p, dv.p, dp.p, dn.p, dfront.p, stats ? dstats.p : nullptr};
+    gtk::intersect(L, f, q, dr.p, c, t_min, o);
+    GT_HIP(hipGetLastError());
+    GT_HIP(hipMemcpyAsync(ht.data(), dt.p, c * 4, hipMemcpyDeviceToHost, ctx->stream));
+    GT_HIP(hipMemcpyAsync(hu.data(), du.p, c * 4, hipMemcpyDeviceToHost, ctx->stream));
+    GT_HIP(hipMemcpyAsync(hv.data(), dv.p, c * 4, hipMemcpyDeviceToHost, ctx->stream));
+    GT_HIP(hipMemcpyAsync(hn.data(), dn.p, 3 * c * 4, hipMemcpyDeviceToHost, ctx->stream));
+    GT_HIP(hipMemcpyAsync(hp.data(), dp.p, c * 4, hipMemcpyDeviceToHost, ctx->stream));
+    GT_HIP(hipMemcpyAsync(hf.data(), dfront.p, c * 4, hipMemcpyDeviceToHost, ctx->stream));
+    GT_HIP(hipStreamSynchronize(ctx->stream));
+    for (size_t i = 0; i < c; ++i) {
+      gt_hit& h = out[off + i];
+      h.t = ht[i]; h.u = hu[i]; h.v = hv[i]; h.prim_id = hp[i];
+      h.normal[0] = hn[3 * i]; h.normal[1] = hn[3 * i + 1]; h.normal[2] = hn[3 * i + 2];
+      h.front_face = hf[i];
+    }
+  }
+  if (stats) {
+    unsigned long long v[8];
+    GT_HIP(hipMemcpy(v, dstats.p, sizeof v, hipMemcpyDeviceToHost));
+    stats->rays = v[0]; stats->node_bounds_tests = v[1]; stats->node_bounds_hits = v[2];
+    stats->leaf_visits = v[3]; stats->primitive_candidates = v[4]; stats->leaf_hit_results = v[5];
+    stats->max_stack_depth = v[6];
+  }
+  return GT_OK;
+}
+
+// ---- lights ----
+int gt_lights_create(gt_context* ctx, gt_lights** out) {
+  if (!ctx || !out) return fail(GT_EINVAL, "null argument");
+  gt_lights* l = new gt_lights();
+  l->ctx = ctx;
+  *out = l;
+  return GT_OK;
+}
+int gt_lights_destroy(gt_lights* l) {
+  if (!l) return GT_OK;
+  (void)hipStreamSynchronize(l->ctx->stream);
+  delete l;
+  return GT_OK;
+}
+static int add_light(gt_lights* l, int kind, const double* q, int nq, double weight) {
+  if (!l) return fail(GT_EINVAL, "lights is null");
+  if (!(std::isfinite(weight) && weight > 0.0)) return fail(GT_EINVAL, "sampling target weight must be finite and positive");  // scene.rs:529-537
+  double total = l->total + weight;
+  if (!std::isfinite(total)) return fail(GT_EINVAL, "sampling target total weight must remain finite");
+  gt::HostPrim p;
+  std::memset(&p, 0, sizeof p);
+  p.kind = kind;
+  for (int i = 0; i < nq; ++i) p.q[i] = q[i];
+  if (kind == GK_SPHERE) p.q[3] = std::fmax(p.q[3], 0.0);
+  if (kind == GK_MOVING_SPHERE) p.q[6] = std::fmax(p.q[6], 0.0);
+  l->total = total;
+  l->targets.push_back(p);
+  l->weights.push_back(weight);
+  l->dirty = true;
+  return GT_OK;
+}
+int gt_lights_add_sphere(gt_lights* l, const double c[3], double r, double w) {
+  double q[4] = {c[0], c[1], c[2], r};
+  return add_light(l, GK_SPHERE, q, 4, w);
+}
+int gt_lights_add_moving_sphere(gt_lights* l, const double c0[3], const double c1[3], double r, double w) {
+  double q[7] = {c0[0], c0[1], c0[2], c1[0], c1[1], c1[2], r};
+  return add_light(l, GK_MOVING_SPHERE, q, 7, w);
+}
+int gt_lights_add_quad(gt_lights* l, const double c[3], const double u[3], const double v[3], double w) {
+  double q[9] = {c[0], c[1], c[2], u[0], u[1], u[2], v[0], v[1], v[2]};
+  return add_light(l, GK_QUAD, q, 9, w);
+}
+int gt_lights_add_triangle(gt_lights* l, const double p0[3], const double p1[3], const double p2[3], double w) {
+  double q[9] = {p0[0], p0[1], p0[2], p1[0], p1[1], p1[2], p2[0], p2[1], p2[2]};
+  return add_light(l, GK_TRIANGLE, q, 9, w);
+}
+size_t gt_lights_len(const gt_lights* l) { return l ? l->targets.size() : 0; }
+
+int gt_scene_emissive_targets(const gt_scene* s, gt_lights** out) {  // scene.rs:1403-1423
+  if (!s || !out) return fail(GT_EINVAL, "null argument");
+  gt_lights* l = nullptr;
+  int rc = gt_lights_create(s->ctx, &l);
+  if (rc != GT_OK) return rc;
+  for (const gt::HostPrim& p : s->prims) {
+    if (s->mats[p.material].kind != GM_DIFFUSE_LIGHT) continue;
+    int nq = p.kind == GK_SPHERE ? 4 : (p.kind == GK_MOVING_SPHERE ? 7 : 9);
+    rc = add_light(l, p.kind, p.q, nq, 1.0);
+    if (rc != GT_OK) { delete l; return rc; }
+  }
+  *out = l;
+  return GT_OK;
+}
+
+// ---- environment ----
+int gt_env_create_constant(gt_context* ctx, double r, double g, double b, gt_env** out) {
+  if (!ctx || !out) return fail(GT_EINVAL, "null argument");
+  if (!finite3(r, g, b)) return fail(GT_EINVAL, "environment colour must be finite");
+  gt_env* e = new gt_env();
+  e->ctx = ctx;
+  e->kind = 1;
+  e->color[0] = r; e->color[1] = g; e->color[2] = b;
+  e->uniform = true;
+  *out = e;
+  return GT_OK;
+}
+
+int gt_env_create_rgb32f(gt_context* ctx, const float* rgb, uint32_t w, uint32_t h, gt_env** out) {
+  if (!ctx || !out || !rgb) return fail(GT_EINVAL, "null argument");
+  if (w == 0 || h == 0) return fail(GT_EINVAL, "environment map must be non-empty");
+  if (uint64_t(w) * h >= (uint64_t(1) << 31)) return fail(GT_EINVAL, "environment map too large");
+  std::unique_ptr<gt_env> e(new gt_env());
+  e->ctx = ctx;
+  e->kind = 2;
+  e->w = w; e->h = h;
+  const size_t n = size_t(w) * h;
+  std::vector<float4> tex(n);
+  std::vector<double> weight(n), row_total(h, 0.0);
+  double total = 0.0;
+  const double pi = 3.14159265358979323846;
+  // rebuild_distribution (environment.rs:158-184) over linear FP32 texels
+  for (uint32_t y = 0; y < h; ++y) {
+    double theta = pi * ((double(y) + 0.5) / double(h));
+    double st = std::fmax(std::sin(theta), 0.0);
+    for (uint32_t x = 0; x < w; ++x) {
+      size_t i = size_t(y) * w + x;
+      float r = rgb[3 * i], g = rgb[3 * i + 1], b = rgb[3 * i + 2];
+      if (!std::isfinite(r) || !std::isfinite(g) || !std::isfinite(b)) return fail(GT_EINVAL, "environment texels must be finite");
+      tex[i] = make_float4(r, g, b, 0.0f);
+      double wt = std::fmax(0.2126 * r + 0.7152 * g + 0.0722 * b, 0.0) * st;
+      weight[i] = wt;
+      total += wt;
+      row_total[y] += wt;
+    }
+  }
+  e->uniform = total <= kEps;
+  std::vector<float> row_cdf(h), col_cdf(n), pdf(n);
+  double cum = 0.0;
+  for (uint32_t y = 0; y < h; ++y) {
+    double rc = 0.0;
+    for (uint32_t x = 0; x < w; ++x) {
+      size_t i = size_t(y) * w + x;
+      rc += weight[i];
+      col_cdf[i] = row_total[y] > 0.0 ? float(rc / row_total[y]) : float(double(x + 1) / double(w));
+    }
+    cum += row_total[y];
+    row_cdf[y] = e->uniform ? float(double(y + 1) / double(h)) : float(cum / total);
+    double t0 = pi * (double(y) / double(h)), t1 = pi * (double(y + 1) / double(h));
+    double sa = (2.0 * pi / double(w)) * std::fmax(std::cos(t0) - std::cos(t1), 0.0);  // :186-190
+    for (uint32_t x = 0; x < w; ++x) {
+      size_t i = size_t(y) * w + x;
+      pdf[i] = (e->uniform || sa <= kEps) ? 0.0f : float((weight[i] / total) / sa);
+    }
+  }
+  row_cdf[h - 1] = 1.0f;
+  GT_HIP(hipSetDevice(ctx->device));
+  GT_HIP(e->tex.ensure(n)); GT_HIP(e->row_cdf.ensure(h)); GT_HIP(e->col_cdf.ensure(n)); GT_HIP(e->pdf.ensure(n));
+  GT_HIP(hipMemcpy(e->tex.p, tex.data(), n * sizeof(float4), hipMemcpyHostToDevice));
+  GT_HIP(hipMemcpy(e->row_cdf.p, row_cdf.data(), h * sizeof(float), hipMemcpyHostToDevice));
+  GT_HIP(hipMemcpy(e->col_cdf.p, col_cdf.data(), n * sizeof(float), hipMemcpyHostToDevice));
+  GT_HIP(hipMemcpy(e->pdf.p, pdf.data(), n * sizeof(float), hipMemcpyHostToDevice));
+  *out = e.release();
+  return GT_OK;
+}
+
+int gt_env_destroy(gt_env* e) {
+  if (!e) return GT_OK;
+  (void)hipStreamSynchronize(e->ctx->stream);
+  delete e;
+  return GT_OK;
+}
+
+// ---- rendering ----
+int gt_render_hdr_device(gt_context* ctx, const gt_scene* scene, const gt_camera* camera, const gt_lights* lights,
+                         const gt_env* env, void* d_out, int out_is_double, gt_render_stats* stats) {
+  if (!d_out) return fail(GT_EINVAL, "output is null");
+  return render_impl(ctx, scene, camera, lights, env, d_out, out_is_double, nullptr, nullptr, stats);
+}
+
+int gt_render_hdr(gt_context* ctx, const gt_scene* scene, const gt_camera* camera, const gt_lights* lights,
+                  const gt_env* env, float* out_rgb, gt_render_stats* stats) {
+  if (!out_rgb) return fail(GT_EINVAL, "output is null");
+  if (!ctx || !camera) return fail(GT_EINVAL, "null argument");
+  GT_HIP(hipSetDevice(ctx->device));
+  size_t npix = size_t(camera->image_width) * gt_camera_image_height(camera);
+  DevBuf<float> d;
+  GT_HIP(d.ensure(3 * npix));
+  int rc = render_impl(ctx, scene, camera, lights, env, d.p, 0, nullptr, nullptr, stats);
+  if (rc != GT_OK) return rc;
+  GT_HIP(hipMemcpyAsync(out_rgb, d.p, 3 * npix * sizeof(float), hipMemcpyDeviceToHost, ctx->stream));
+  GT_HIP(hipStreamSynchronize(ctx->stream));
+  return GT_OK;
+}
+
+int gt_render_aovs_device(gt_context* ctx, const gt_scene* scene, const gt_camera* camera, const gt_lights* lights,
+                          const gt_env* env, float* d_beauty, float* d_albedo, float* d_normal,
+                          gt_render_stats* stats) {
+  if (!d_beauty || !d_albedo || !d_normal) return fail(GT_EINVAL, "output is null");
+  return render_impl(ctx, scene, camera, lights, env, d_beauty, 0, d_albedo, d_normal, stats);
+}
+
+int gt_render_aovs(gt_context* ctx, const gt_scene* scene, const gt_camera* camera, const gt_lights* lights,
+                   const gt_env* env, float* beauty, float* albedo, float* normal, gt_render_stats* stats) {
+  if (!beauty || !albedo || !normal) return fail(GT_EINVAL, "output is null");
+  if (!ctx || !camera) return fail(GT_EINVAL, "null argument");
+  GT_HIP(hipSetDevice(ctx->device));
+  size_t npix = size_t(camera->image_width) * gt_camera_image_height(camera);
+  DevBuf<float> d;
+  GT_HIP(d.ensure(9 * npix));
+  int rc = render_impl(ctx, scene, camera, lights, env, d.p, 0, d.p + 3 * npix, d.p + 6 * npix, stats);
+  if (rc != GT_OK) return rc;
+  GT_HIP(hipMemcpyAsync(beauty, d.p, 3 * npix * 4, hipMemcpyDeviceToHost, ctx->stream));
+  GT_HIP(hipMemcpyAsync(albedo, d.p + 3 * npix, 3 * npix * 4, hipMemcpyDeviceToHost, ctx->stream));
+  GT_HIP(hipMemcpyAsync(normal, d.p + 6 * npix, 3 * npix * 4, hipMemcpyDeviceToHost, ctx->stream));
+  GT_HIP(hipStreamSynchronize(ctx->stream));
+  return GT_OK;
+}
+
+int gt_render_batch(gt_context* ctx, size_t n, const gt_scene* const* scenes, const gt_camera* cameras,
+                    const gt_lights* const* lights, float* const* out_rgb, gt_render_stats* stats) {
+  if (!ctx || (n && (!scenes || !cameras || !out_rgb))) return fail(GT_EINVAL, "null argument");
+  if (stats) std::memset(stats, 0, sizeof *stats);
+  for (size_t i = 0; i < n; ++i) {
+    gt_render_stats s;
+    int rc = gt_render_hdr(ctx, scenes[i], &cameras[i], lights ? lights[i] : nullptr, nullptr, out_rgb[i],
+                           stats ? &s : nullptr);
+    if (rc != GT_OK) return rc;
+    if (stats) {
+      stats->camera_paths += s.camera_paths; stats->extension_rays += s.extension_rays;
+      stats->shadow_rays += s.shadow_rays; stats->device_ms += s.device_ms; stats->extend_ms += s.extend_ms;
+      stats->extend_launches += s.extend_launches; stats->shade_ms += s.shade_ms; stats->connect_ms += s.connect_ms;
+    }
+  }
+  return GT_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,80 @@
+// kernels.h — host-side launch interface of the wavefront kernels.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "device.h"
+
+namespace gtk {
+
+// Counters per bounce b live at ctr[b * kCtrStride + ...].
+constexpr int kCtrStride = 16;
+enum {
+  C_QCOUNT = 0,      // extension rays queued for bounce b
+  C_EXT_WORK = 1,    // work cursor of extend(b)
+  C_SHADE_WORK = 2,  // work cursor of shade(b)
+  C_SL_COUNT = 3,    // light shadow rays queued at bounce b
+  C_SE_COUNT = 4,    // environment shadow rays queued at bounce b
+  C_SL_WORK = 5,
+  C_SE_WORK = 6,
+};
+
+struct Queues {
+  float4* q_o[2];   // extension queue (double-buffered by bounce parity): origin, time
+  float4* q_d[2];   // direction, path id (bit 31: last event was a scattering event)
+  float4* q_t[2];   // throughput, miss-radiance weight
+  float2* hits;     // t, reordered primitive index (0xffffffff = miss)
+  float4* s_o[2];   // shadow queues: [0] light targets, [1] environment
+  float4* s_d[2];
+  float4* s_w[2];   // contribution weight
+  float4* radiance; // per-path radiance L
+  uint32_t* ctr;
+  uint2* gstack;    // traversal-stack overflow, gstride entries per level
+  uint32_t gstride;
+  uint32_t stack_cap;  // LDS entries per lane
+};
+
+struct Frame {
+  gtd::DScene scene;
+  gtd::DLights lights;
+  gtd::DEnv env;
+  GCamera cam;
+  uint32_t nee;            // direct lighting by next-event estimation
+  uint32_t has_lights;
+  float light_weight;      // mixture weight in continuation mode
+  uint32_t npix;
+};
+
+struct Launch {
+  uint32_t grid, block;
+  hipStream_t stream;
+};
+
+void reset_counters(const Launch& l, uint32_t* ctr, uint32_t n);
+void raygen(const Launch& l, const Frame& f, const Queues& q, uint32_t sample0, uint32_t nsamples);
+void extend(const Launch& l, const Frame& f, const Queues& q, uint32_t bounce);
+void shade(const Launch& l, const Frame& f, const Queues& q, uint32_t bounce, uint32_t sample0, uint32_t last);
+void connect(const Launch& l, const Frame& f, const Queues& q, uint32_t bounce, uint32_t kind);
+void accumulate(const Launch& l, const Frame& f, const Queues& q, double4* acc, uint32_t nsamples);
+void finalize(const Launch& l, const double4* acc, uint32_t npix, void* out, int out_is_double);
+void aov(const Launch& l, const Frame& f, const Queues& q, float* albedo, float* normal);
+
+struct IntersectOut {
+  float* t;
+  float* u;
+  float* v;
+  int32_t* prim;
+  float* normal;
+  uint32_t* front;
+  unsigned long long* stats;  // 6 sums + max stack (7 words), or null
+};
+void intersect(const Launch& l, const Frame& f, const Queues& q, const float4* rays, size_t n, float tmin,
+               IntersectOut out);
+
+// Resident blocks per CU of a kernel: 0 extend, 1 shade, 2 connect.
+uint32_t occupancy_blocks(int kernel, uint32_t block, size_t lds_bytes);
+
+// Bytes of LDS per block for the traversal stack.
+inline size_t stack_lds_bytes(uint32_t block, uint32_t cap) { return size_t(block) * cap * sizeof(uint2); }
+
+}  // namespace gtk

@@ -1,0 +1,521 @@
+// kernels.hip — the wavefront path tracer for gfx950 (CDNA4).
+//
+// One batch of paths (a set of samples for every pixel) runs as
+//   raygen -> { extend(b) -> shade(b) -> connect(b, light) -> connect(b, env) }
+//   for b < max_depth -> accumulate
+// over queues in HBM. Every stage is a persistent kernel: a fixed grid whose
+// waves pull 64-item chunks from a device-side cursor, so the host never reads
+// a queue length and the loop needs no synchronisation. Stages that append
+// (shade) compact their output per wave: one ballot, one popcount prefix and
+// one atomic per wave and queue.
+//
+// The per-path integrator is RayCamera::ray_color (camera.rs:1730-1862), split
+// at its two traversals: the closest hit of the path ray (extend) and the
+// shadow rays of next-event estimation (connect). Radiance accumulates per
+// path in the order the reference adds it (emission, light NEE, environment
+// NEE, next bounce), so a path's result does not depend on queue order.
+#include "kernels.h"
+
+using namespace gtd;
+
+namespace gtk {
+
+namespace {
+
+constexpr uint32_t kMiss = 0xffffffffu;
+constexpr uint32_t kItemsPerLane = 2;  // items per lane per cursor fetch
+
+__device__ __forceinline__ uint32_t lane_id() { return __lane_id(); }
+
+// Claims kItemsPerLane * 64 consecutive items for the calling wave.
+__device__ __forceinline__ uint32_t wave_fetch(uint32_t* cursor) {
+  uint32_t base = 0;
+  if (lane_id() == 0) base = atomicAdd(cursor, kItemsPerLane * 64u);
+  return __shfl(base, 0);
+}
+
+// Wave-level append: returns this lane's slot, or kMiss when !pred.
+__device__ __forceinline__ uint32_t wave_append(uint32_t* counter, bool pred) {
+  unsigned long long mask = __ballot(pred);
+  if (mask == 0) return kMiss;
+  uint32_t leader = uint32_t(__ffsll((long long)mask) - 1);
+  uint32_t base = 0;
+  if (lane_id() == leader) base = atomicAdd(counter, uint32_t(__popcll(mask)));
+  base = __shfl(base, leader);
+  unsigned long long below = lane_id() == 0 ? 0ull : (mask & ((~0ull) >> (64 - lane_id())));
+  return pred ? base + uint32_t(__popcll(below)) : kMiss;
+}
+
+__device__ __forceinline__ Stack make_stack(const Queues& q, uint2* smem) {
+  Stack st;
+  st.lds = smem + threadIdx.x;
+  st.stride = blockDim.x;
+  st.glob = q.gstack + (blockIdx.x * blockDim.x + threadIdx.x);
+  st.gstride = q.gstride;
+  st.cap = int(q.stack_cap);
+  st.n = 0;
+  return st;
+}
+
+__device__ __forceinline__ void path_coords(const Frame& f, uint32_t path, uint32_t sample0, uint32_t* x, uint32_t* y,
+                                            uint32_t* sample) {
+  uint32_t pixel = path % f.npix;
+  *sample = sample0 + path / f.npix;
+  *x = pixel % f.cam.width;
+  *y = pixel / f.cam.width;
+}
+
+// Russian roulette (camera.rs:2498-2523)
+__device__ __forceinline__ bool rr_survives(uint32_t bounce, int32_t min_depth, f3* T, Rng& rng) {
+  if (min_depth < 0 || bounce < uint32_t(min_depth)) return true;
+  float m = fmaxf(fmaxf(T->x, T->y), T->z);
+  if (!isfinite(m) || m <= EPS) return false;
+  float p = clampf(m, 0.05f, 0.95f);
+  if (rng.next() >= p) return false;
+  *T = *T / p;
+  return true;
+}
+
+// Material::scattering_pdf for the RayScene materials (material.rs:522-529,
+// :1596-1612; the others return 0).
+__device__ __forceinline__ float scattering_pdf(const GMat& m, f3 d_in, f3 n, f3 dir) {
+  if (m.kind == GM_LAMBERTIAN) {
+    float c = dot(n, normalized(dir));
+    return c <= 0.f ? 0.f : c / PI;
+  }
+  if (m.kind == GM_GGX) {
+    f3 outgoing = -normalized(d_in);
+    f3 incoming = normalized(dir);
+    if (dot(incoming, n) <= 0.f || dot(outgoing, n) <= 0.f) return 0.f;
+    return ggx_weight(n, outgoing, incoming, m.param, m.f0);
+  }
+  return 0.f;
+}
+
+__global__ void k_reset(uint32_t* ctr, uint32_t n) {
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) ctr[i] = 0;
+}
+
+// Camera rays for samples [sample0, sample0 + nsamples) of every pixel
+// (render_world_linear_pixel_with_context camera.rs:2660-2723; rays :1658-1718).
+__global__ void k_raygen(Frame f, Queues q, uint32_t sample0, uint32_t nsamples) {
+  const uint32_t total = f.npix * nsamples;
+  if (blockIdx.x == 0 && threadIdx.x == 0) q.ctr[C_QCOUNT] = total;
+  for (uint32_t p = blockIdx.x * blockDim.x + threadIdx.x; p < total; p += gridDim.x * blockDim.x) {
+    uint32_t x, y, sample;
+    path_coords(f, p, sample0, &x, &y, &sample);
+    Rng rng{stream_seed(pixel_seed(f.cam.seed, x, y), sample, 0)};
+    f3 o, d;
+    float time;
+    camera_ray(f.cam, x, y, sample, rng, &o, &d, &time);
+    q.q_o[0][p] = make_float4(o.x, o.y, o.z, time);
+    q.q_d[0][p] = make_float4(d.x, d.y, d.z, __uint_as_float(p));
+    q.q_t[0][p] = make_float4(1.f, 1.f, 1.f, 1.f);
+    q.radiance[p] = make_float4(0.f, 0.f, 0.f, 0.f);
+  }
+}
+
+// Closest hit of every queued extension ray.
+__global__ void k_extend(Frame f, Queues q, uint32_t bounce) {
+  extern __shared__ uint2 smem[];
+  Stack st = make_stack(q, smem);
+  const uint32_t count = q.ctr[bounce * kCtrStride + C_QCOUNT];
+  const float4* __restrict__ qo = q.q_o[bounce & 1];
+  const float4* __restrict__ qd = q.q_d[bounce & 1];
+  uint32_t* cursor = q.ctr + bounce * kCtrStride + C_EXT_WORK;
+  for (;;) {
+    uint32_t base = wave_fetch(cursor);
+    if (base >= count) break;
+    for (uint32_t it = 0; it < kItemsPerLane; ++it) {
+      uint32_t i = base + it * 64u + lane_id();
+      if (i >= count) break;
+      float4 o4 = qo[i], d4 = qd[i];
+      float t;
+      uint32_t prim;
+      bool hit = trace<false, false>(f.scene, mk(o4.x, o4.y, o4.z), mk(d4.x, d4.y, d4.z), o4.w, SHADOW_ACNE, INF, st,
+                                     &t, &prim, nullptr, nullptr);
+      q.hits[i] = hit ? make_float2(t, __uint_as_float(prim)) : make_float2(0.f, __uint_as_float(kMiss));
+    }
+  }
+}
+
+// One bounce of ray_color for every hit: emission, scatter, NEE sampling and
+// the continuation ray.
+__global__ void k_shade(Frame f, Queues q, uint32_t bounce, uint32_t sample0, uint32_t last) {
+  const uint32_t count = q.ctr[bounce * kCtrStride + C_QCOUNT];
+  const float4* __restrict__ qo = q.q_o[bounce & 1];
+  const float4* __restrict__ qd = q.q_d[bounce & 1];
+  const float4* __restrict__ qt = q.q_t[bounce & 1];
+  float4* oo = q.q_o[(bounce + 1) & 1];
+  float4* od = q.q_d[(bounce + 1) & 1];
+  float4* ot = q.q_t[(bounce + 1) & 1];
+  uint32_t* cursor = q.ctr + bounce * kCtrStride + C_SHADE_WORK;
+  uint32_t* next_count = q.ctr + (bounce + 1) * kCtrStride + C_QCOUNT;
+  uint32_t* sl_count = q.ctr + bounce * kCtrStride + C_SL_COUNT;
+  uint32_t* se_count = q.ctr + bounce * kCtrStride + C_SE_COUNT;
+  const bool has_env = f.env.kind != 0;
+  for (;;) {
+    uint32_t base = wave_fetch(cursor);
+    if (base >= count) break;
+    for (uint32_t it = 0; it < kItemsPerLane; ++it) {
+      uint32_t i = base + it * 64u + lane_id();
+      bool push_ext = false, push_sl = false, push_se = false;
+      float4 n_o, n_d, n_t, l_o, l_d, l_w, e_o, e_d, e_w;
+      if (i < count) {
+        float4 o4 = qo[i], d4 = qd[i], t4 = qt[i];
+        float2 hit = q.hits[i];
+        uint32_t pbits = __float_as_uint(d4.w);
+        uint32_t path = pbits & 0x7fffffffu;
+        bool after_scatter = (pbits >> 31) != 0;
+        f3 o = mk(o4.x, o4.y, o4.z), d = mk(d4.x, d4.y, d4.z);
+        float time = o4.w;
+        f3 T = mk(t4.x, t4.y, t4.z);
+        float mw = t4.w;
+        uint32_t x, y, sample;
+        path_coords(f, path, sample0, &x, &y, &sample);
+        Rng rng{stream_seed(pixel_seed(f.cam.seed, x, y), sample, 1 + bounce)};
+        float4 L4 = q.radiance[path];
+        f3 L = mk(L4.x, L4.y, L4.z);
+        uint32_t prim = __float_as_uint(hit.y);
+        if (prim == kMiss) {
+          f3 miss = (has_env ? env_radiance(f.env, d) : background(f.cam, d)) * mw;
+          L = L + mul(T, miss);
+        } else {
+          f3 p = o + hit.x * d;
+          uint32_t mid, kind;
+          f3 outward = prim_outward_normal(f.scene.prims, prim, p, time, &mid, &kind);
+          bool front = dot(d, outward) < 0.f;  // object.rs:127-149
+          f3 n = front ? outward : -outward;
+          const GMat m = f.scene.mats[mid];
+          f3 color = ld3(m.color);
+          bool allow_emitted = !(f.nee && f.has_lights && after_scatter);
+          if (allow_emitted && m.kind == GM_DIFFUSE_LIGHT && front) L = L + mul(T, color);
+          if (m.kind == GM_METAL || m.kind == GM_DIELECTRIC) {
+            f3 dir;
+            bool ok = true;
+            if (m.kind == GM_METAL) {  // material.rs:2087-2107
+              f3 refl = reflected(normalized(d), n);
+              dir = refl + m.param * rng.unit_vector_spherical();
+              ok = dot(dir, n) > 0.f;
+              T = mul(T, color);
+            } else {  // material.rs:2321-2350
+              float ratio = front ? 1.0f / m.param : m.param;
+              f3 unit = normalized(d);
+              float ct = fminf(dot(-unit, n), 1.0f);
+              float stt = sqrtf(1.0f - ct * ct);
+              bool cannot = ratio * stt > 1.0f;
+              float r0 = (1.0f - ratio) / (1.0f + ratio);
+              r0 = r0 * r0;
+              bool refl = cannot || (r0 + (1.0f - r0) * pow5(1.0f - ct)) > rng.next();
+              dir = refl ? reflected(unit, n) : refracted(unit, n, ratio);
+            }
+            if (ok && !last && rr_survives(bounce, f.cam.rr_min_depth, &T, rng)) {
+              push_ext = true;
+              n_o = make_float4(p.x, p.y, p.z, time);
+              n_d = make_float4(dir.x, dir.y, dir.z, __uint_as_float(path));
+              n_t = make_float4(T.x, T.y, T.z, 1.0f);
+            }
+          } else if (m.kind == GM_LAMBERTIAN || m.kind == GM_GGX) {
+            MatPdf pdf;
+            bool ok;
+            if (m.kind == GM_LAMBERTIAN) {  // material.rs:508-520
+              pdf.ggx = false;
+              ok = onb_from_w(n, &pdf.basis);
+            } else {  // material.rs:1583-1594, sampling.rs:183-202
+              pdf.ggx = true;
+              ok = onb_from_w(n, &pdf.basis);
+              f3 outgoing = normalized(-normalized(d));
+              ok = ok && len2(outgoing) > EPS && dot(outgoing, pdf.basis.w) > 0.f;
+              pdf.outgoing = outgoing;
+              pdf.roughness = m.param;
+              pdf.alpha = ggx_alpha(m.param);
+            }
+            if (ok) {
+              if (f.nee) {
+                if (f.has_lights) {  // estimate_direct_lighting camera.rs:2225-2262
+                  f3 ldir = lights_random(f.lights, p, time, rng);
+                  float lpdf = lights_pdf(f.lights, p, time, ldir);
+                  if (isfinite(lpdf) && lpdf > EPS) {
+                    float spdf = scattering_pdf(m, d, n, ldir);
+                    if (isfinite(spdf) && spdf > 0.f) {
+                      f3 w = mul(T, color) * (spdf / lpdf);
+                      push_sl = true;
+                      l_o = make_float4(p.x, p.y, p.z, time);
+                      l_d = make_float4(ldir.x, ldir.y, ldir.z, __uint_as_float(path));
+                      l_w = make_float4(w.x, w.y, w.z, 0.f);
+                    }
+                  }
+                }
+                if (has_env) {  // estimate_environment_lighting camera.rs:2264-2297
+                  f3 edir;
+                  float epdf;
+                  env_sample(f.env, rng, &edir, &epdf);
+                  if (isfinite(epdf) && epdf > EPS) {
+                    float spdf = scattering_pdf(m, d, n, edir);
+                    if (isfinite(spdf) && spdf > 0.f) {
+                      float mis = power_heuristic(epdf, spdf);
+                      f3 w = mul(T, mul(color, env_radiance(f.env, edir)) * (mis * spdf / epdf));
+                      push_se = true;
+                      e_o = make_float4(p.x, p.y, p.z, time);
+                      e_d = make_float4(edir.x, edir.y, edir.z, __uint_as_float(path));
+                      e_w = make_float4(w.x, w.y, w.z, 0.f);
+                    }
+                  }
+                }
+              }
+              if (!last) {
+                // SamplingStrategy::continuation_sample camera.rs:165-195
+                f3 dir;
+                float cpdf;
+                bool suppress, weight_env;
+                if (!f.nee && f.has_lights) {
+                  float wgt = f.light_weight;
+                  if (rng.next() < wgt) dir = lights_random(f.lights, p, time, rng);
+                  else dir = pdf.generate(rng);
+                  cpdf = wgt * lights_pdf(f.lights, p, time, dir) + (1.0f - wgt) * pdf.value(dir);
+                  suppress = false;
+                  weight_env = false;
+                } else {
+                  dir = pdf.generate(rng);
+                  cpdf = pdf.value(dir);
+                  suppress = f.nee && f.has_lights;
+                  weight_env = f.nee && has_env;
+                }
+                if (isfinite(cpdf) && cpdf > EPS) {
+                  float spdf = scattering_pdf(m, d, n, dir);
+                  if (isfinite(spdf) && spdf > 0.f) {
+                    float next_w = weight_env ? power_heuristic(cpdf, env_pdf(f.env, dir)) : 1.0f;
+                    T = mul(T, color * (spdf / cpdf));
+                    if (rr_survives(bounce, f.cam.rr_min_depth, &T, rng)) {
+                      push_ext = true;
+                      n_o = make_float4(p.x, p.y, p.z, time);
+                      n_d = make_float4(dir.x, dir.y, dir.z, __uint_as_float(path | (suppress ? 0x80000000u : 0u)));
+                      n_t = make_float4(T.x, T.y, T.z, next_w);
+                    }
+                  }
+                }
+              }
+            }
+          }
+        }
+        q.radiance[path] = make_float4(L.x, L.y, L.z, 0.f);
+      }
+      uint32_t slot = wave_append(next_count, push_ext);
+      if (push_ext) { oo[slot] = n_o; od[slot] = n_d; ot[slot] = n_t; }
+      slot = wave_append(sl_count, push_sl);
+      if (push_sl) { q.s_o[0][slot] = l_o; q.s_d[0][slot] = l_d; q.s_w[0][slot] = l_w; }
+      slot = wave_append(se_count, push_se);
+      if (push_se) { q.s_o[1][slot] = e_o; q.s_d[1][slot] = e_d; q.s_w[1][slot] = e_w; }
+    }
+  }
+}
+
+// Shadow rays of one kind. Light targets: closest hit, then the emission of
+// whatever was hit (camera.rs:2248-2261). Environment: any hit occludes
+// (camera.rs:2284-2293; "is some hit" equals "is some closest hit").
+__global__ void k_connect(Frame f, Queues q, uint32_t bounce, uint32_t kind) {
+  extern __shared__ uint2 smem[];
+  Stack st = make_stack(q, smem);
+  const uint32_t count = q.ctr[bounce * kCtrStride + (kind == 0 ? C_SL_COUNT : C_SE_COUNT)];
+  uint32_t* cursor = q.ctr + bounce * kCtrStride + (kind == 0 ? C_SL_WORK : C_SE_WORK);
+  const float4* __restrict__ so = q.s_o[kind];
+  const float4* __restrict__ sd = q.s_d[kind];
+  const float4* __restrict__ sw = q.s_w[kind];
+  for (;;) {
+    uint32_t base = wave_fetch(cursor);
+    if (base >= count) break;
+    for (uint32_t it = 0; it < kItemsPerLane; ++it) {
+      uint32_t i = base + it * 64u + lane_id();
+      if (i >= count) break;
+      float4 o4 = so[i], d4 = sd[i], w4 = sw[i];
+      uint32_t path = __float_as_uint(d4.w);
+      f3 o = mk(o4.x, o4.y, o4.z), d = mk(d4.x, d4.y, d4.z);
+      float t;
+      uint32_t prim;
+      f3 add = mk(0.f, 0.f, 0.f);
+      bool contributes = false;
+      if (kind == 0) {
+        if (trace<false, false>(f.scene, o, d, o4.w, SHADOW_ACNE, INF, st, &t, &prim, nullptr, nullptr)) {
+          f3 p = o + t * d;
+          uint32_t mid, pk;
+          f3 outward = prim_outward_normal(f.scene.prims, prim, p, o4.w, &mid, &pk);
+          const GMat& m = f.scene.mats[mid];
+          if (m.kind == GM_DIFFUSE_LIGHT && dot(d, outward) < 0.f) {
+            f3 em = ld3(m.color);
+            if (finite3(em)) { add = mul(mk(w4.x, w4.y, w4.z), em); contributes = true; }
+          }
+        }
+      } else {
+        if (!trace<true, false>(f.scene, o, d, o4.w, SHADOW_ACNE, INF, st, &t, &prim, nullptr, nullptr)) {
+          add = mk(w4.x, w4.y, w4.z);
+          contributes = true;
+        }
+      }
+      if (contributes) {
+        float4 L = q.radiance[path];
+        q.radiance[path] = make_float4(L.x + add.x, L.y + add.y, L.z + add.z, 0.f);
+      }
+    }
+  }
+}
+
+// Adds each finished sample of the batch to its pixel in sample order,
+// dropping non-finite samples (add_finite_sample camera.rs:2984-2991).
+__global__ void k_accumulate(Frame f, Queues q, double4* acc, uint32_t nsamples) {
+  for (uint32_t px = blockIdx.x * blockDim.x + threadIdx.x; px < f.npix; px += gridDim.x * blockDim.x) {
+    double4 a = acc[px];
+    for (uint32_t s = 0; s < nsamples; ++s) {
+      float4 L = q.radiance[size_t(s) * f.npix + px];
+      if (isfinite(L.x) && isfinite(L.y) && isfinite(L.z)) {
+        a.x += double(L.x); a.y += double(L.y); a.z += double(L.z); a.w += 1.0;
+      }
+    }
+    acc[px] = a;
+  }
+}
+
+// average_accepted_samples (camera.rs:2993-2999)
+__global__ void k_finalize(const double4* acc, uint32_t npix, void* out, int out_is_double) {
+  for (uint32_t px = blockIdx.x * blockDim.x + threadIdx.x; px < npix; px += gridDim.x * blockDim.x) {
+    double4 a = acc[px];
+    double r = 0, g = 0, b = 0;
+    if (a.w > 0.0) { r = a.x / a.w; g = a.y / a.w; b = a.z / a.w; }
+    if (out_is_double) {
+      double* o = static_cast<double*>(out) + 3 * size_t(px);
+      o[0] = r; o[1] = g; o[2] = b;
+    } else {
+      float* o = static_cast<float*>(out) + 3 * size_t(px);
+      o[0] = float(r); o[1] = float(g); o[2] = float(b);
+    }
+  }
+}
+
+// First-hit AOVs (render_denoising_aov_linear_pixel camera.rs:3064-3116):
+// albedo = denoise_albedo, normal = 0.5 (n + 1); 0 on a miss.
+__global__ void k_aov(Frame f, Queues q, float* albedo, float* normal) {
+  extern __shared__ uint2 smem[];
+  Stack st = make_stack(q, smem);
+  for (uint32_t px = blockIdx.x * blockDim.x + threadIdx.x; px < f.npix; px += gridDim.x * blockDim.x) {
+    uint32_t x = px % f.cam.width, y = px / f.cam.width;
+    uint64_t key = pixel_seed(f.cam.seed, x, y);
+    f3 sa = mk(0.f, 0.f, 0.f), sn = mk(0.f, 0.f, 0.f);
+    for (uint32_t s = 0; s < f.cam.spp; ++s) {
+      Rng rng{stream_seed(key, s, 0)};
+      f3 o, d;
+      float time, t;
+      uint32_t prim;
+      camera_ray(f.cam, x, y, s, rng, &o, &d, &time);
+      if (trace<false, false>(f.scene, o, d, time, SHADOW_ACNE, INF, st, &t, &prim, nullptr, nullptr)) {
+        f3 p = o + t * d;
+        uint32_t mid, kind;
+        f3 outward = prim_outward_normal(f.scene.prims, prim, p, time, &mid, &kind);
+        f3 n = dot(d, outward) < 0.f ? outward : -outward;
+        const GMat& m = f.scene.mats[mid];
+        sa = sa + (m.kind == GM_DIELECTRIC ? mk(1.f, 1.f, 1.f) : ld3(m.color));
+        sn = sn + 0.5f * (n + mk(1.f, 1.f, 1.f));
+      }
+    }
+    float inv = float(f.cam.spp);
+    albedo[3 * size_t(px)] = sa.x / inv; albedo[3 * size_t(px) + 1] = sa.y / inv; albedo[3 * size_t(px) + 2] = sa.z / inv;
+    normal[3 * size_t(px)] = sn.x / inv; normal[3 * size_t(px) + 1] = sn.y / inv; normal[3 * size_t(px) + 2] = sn.z / inv;
+  }
+}
+
+// Closest-hit queries with the hit record of HitRecord::from_surface
+// (object.rs:435-450) and optional BvhTraversalStats (bvh.rs:9-52).
+template <bool STATS>
+__global__ void k_intersect(Frame f, Queues q, const float4* rays, size_t n, float tmin, IntersectOut out) {
+  extern __shared__ uint2 smem[];
+  Stack st = make_stack(q, smem);
+  for (size_t i = blockIdx.x * size_t(blockDim.x) + threadIdx.x; i < n; i += size_t(gridDim.x) * blockDim.x) {
+    float4 o4 = rays[2 * i], d4 = rays[2 * i + 1];
+    f3 o = mk(o4.x, o4.y, o4.z), d = mk(d4.x, d4.y, d4.z);
+    float t = 0.f;
+    uint32_t prim = 0;
+    PrimHit ph{0.f, 0.f, 0.f};
+    Stats s{0, 0, 0, 0, 0, 0};
+    bool hit = trace<false, STATS>(f.scene, o, d, o4.w, tmin, d4.w, st, &t, &prim, &ph, &s);
+    if (hit) {
+      f3 p = o + t * d;
+      uint32_t mid, kind;
+      f3 outward = prim_outward_normal(f.scene.prims, prim, p, o4.w, &mid, &kind);
+      bool front = dot(d, outward) < 0.f;
+      f3 nn = front ? outward : -outward;
+      float u = ph.u, v = ph.v;
+      if (kind == GK_SPHERE || kind == GK_MOVING_SPHERE) {  // sphere_uv object.rs:1038-1042
+        float theta = acosf(-outward.y);
+        float phi = atan2f(-outward.z, outward.x) + PI;
+        u = phi / (2.0f * PI);
+        v = theta / PI;
+      }
+      out.t[i] = t; out.u[i] = u; out.v[i] = v;
+      out.prim[i] = int32_t(f.scene.prim_ids[prim]);
+      out.normal[3 * i] = nn.x; out.normal[3 * i + 1] = nn.y; out.normal[3 * i + 2] = nn.z;
+      out.front[i] = front;
+    } else {
+      out.t[i] = 0.f; out.u[i] = 0.f; out.v[i] = 0.f; out.prim[i] = -1;
+      out.normal[3 * i] = 0.f; out.normal[3 * i + 1] = 0.f; out.normal[3 * i + 2] = 0.f;
+      out.front[i] = 0;
+    }
+    if (STATS) {
+      atomicAdd(&out.stats[0], 1ull);
+      atomicAdd(&out.stats[1], (unsigned long long)s.node_tests);
+      atomicAdd(&out.stats[2], (unsigned long long)s.node_hits);
+      atomicAdd(&out.stats[3], (unsigned long long)s.leaf_visits);
+      atomicAdd(&out.stats[4], (unsigned long long)s.candidates);
+      atomicAdd(&out.stats[5], (unsigned long long)s.leaf_hits);
+      atomicMax(&out.stats[6], (unsigned long long)s.max_stack);
+    }
+  }
+}
+
+}  // namespace
+
+uint32_t occupancy_blocks(int kernel, uint32_t block, size_t lds) {
+  int nb = 0;
+  hipError_t e = hipSuccess;
+  if (kernel == 0) e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_extend, int(block), lds);
+  else if (kernel == 1) e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_shade, int(block), 0);
+  else e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_connect, int(block), lds);
+  if (e != hipSuccess || nb <= 0) return 1;
+  return uint32_t(nb);
+}
+
+void reset_counters(const Launch& l, uint32_t* ctr, uint32_t n) {
+  hipLaunchKernelGGL(k_reset, dim3(1), dim3(256), 0, l.stream, ctr, n);
+}
+void raygen(const Launch& l, const Frame& f, const Queues& q, uint32_t sample0, uint32_t nsamples) {
+  hipLaunchKernelGGL(k_raygen, dim3(l.grid), dim3(l.block), 0, l.stream, f, q, sample0, nsamples);
+}
+void extend(const Launch& l, const Frame& f, const Queues& q, uint32_t bounce) {
+  hipLaunchKernelGGL(k_extend, dim3(l.grid), dim3(l.block), stack_lds_bytes(l.block, q.stack_cap), l.stream, f, q,
+                     bounce);
+}
+void shade(const Launch& l, const Frame& f, const Queues& q, uint32_t bounce, uint32_t sample0, uint32_t last) {
+  hipLaunchKernelGGL(k_shade, dim3(l.grid), dim3(l.block), 0, l.stream, f, q, bounce, sample0, last);
+}
+void connect(const Launch& l, const Frame& f, const Queues& q, uint32_t bounce, uint32_t kind) {
+  hipLaunchKernelGGL(k_connect, dim3(l.grid), dim3(l.block), stack_lds_bytes(l.block, q.stack_cap), l.stream, f, q,
+                     bounce, kind);
+}
+void accumulate(const Launch& l, const Frame& f, const Queues& q, double4* acc, uint32_t nsamples) {
+  hipLaunchKernelGGL(k_accumulate, dim3(l.grid), dim3(l.block), 0, l.stream, f, q, acc, nsamples);
+}
+void finalize(const Launch& l, const double4* acc, uint32_t npix, void* out, int out_is_double) {
+  hipLaunchKernelGGL(k_finalize, dim3(l.grid), dim3(l.block), 0, l.stream, acc, npix, out, out_is_double);
+}
+void aov(const Launch& l, const Frame& f, const Queues& q, float* albedo, float* normal) {
+  hipLaunchKernelGGL(k_aov, dim3(l.grid), dim3(l.block), stack_lds_bytes(l.block, q.stack_cap), l.stream, f, q,
+                     albedo, normal);
+}
+void intersect(const Launch& l, const Frame& f, const Queues& q, const float4* rays, size_t n, float tmin,
+               IntersectOut out) {
+  if (out.stats)
+    hipLaunchKernelGGL(k_intersect<true>, dim3(l.grid), dim3(l.block), stack_lds_bytes(l.block, q.stack_cap),
+                       l.stream, f, q, rays, n, tmin, out);
+  else
+    hipLaunchKernelGGL(k_intersect<false>, dim3(l.grid), dim3(l.block), stack_lds_bytes(l.block, q.stack_cap),
+                       l.stream, f, q, rays, n, tmin, out);
+}
+
+}  // namespace gtk

@@ -1,0 +1,247 @@
+/*
+ * gartus_hip.h — C ABI of the MI355X path tracer.
+ *
+ * This is the drop-in boundary for gartus's path-tracing hot path:
+ *   PathTracer::render / render_hdr_image / render_denoising_aovs /
+ *   render_with_lights* / render_with_environment* over a compiled RayScene
+ *   (src/graphics/raytracing/renderer.rs:101-224,374-609).
+ * Every entry point below names the reference interface it replaces.
+ *
+ * Conventions
+ *   - Every function returns int status: GT_OK (0) or a GT_E* code. The
+ *     message of the last failure on the calling thread is gt_last_error().
+ *     Conditions that panic (assert!) in the reference return GT_EINVAL.
+ *   - Inputs are copied; outputs are caller-owned buffers.
+ *   - Host-pointer outputs ("out_*") are written after the device work ends.
+ *     The *_device variants take device pointers and leave the result on the
+ *     context's stream; call gt_context_synchronize before reading it.
+ *   - A context owns one HIP stream and is not thread-safe; separate
+ *     contexts are independent.
+ *   - Scenes, light lists and environments belong to the context that
+ *     created them.
+ */
+#ifndef GARTUS_HIP_H
+#define GARTUS_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+  GT_OK = 0,
+  GT_EINVAL = 1,   /* invalid argument (reference: assert! panic) */
+  GT_ENOMEM = 2,   /* host or device allocation failed */
+  GT_EDEVICE = 3,  /* HIP runtime error */
+  GT_ESTATE = 4,   /* object in the wrong state (e.g. scene not built) */
+  GT_EUNSUPPORTED = 5
+};
+
+typedef struct gt_context gt_context;
+typedef struct gt_scene gt_scene;
+typedef struct gt_lights gt_lights;
+typedef struct gt_env gt_env;
+
+/* ---- camera ------------------------------------------------------------
+ * Plain-data mirror of RayCamera's builder state
+ * (src/graphics/camera.rs:612-643, defaults :1034-1064).
+ */
+enum { GT_PIXEL_RANDOM = 0, GT_PIXEL_STRATIFIED = 1, GT_PIXEL_STRATIFIED_GRID = 2 };
+enum { GT_LIGHTING_CONTINUATION = 0, GT_LIGHTING_NEE = 1 };
+enum { GT_BACKGROUND_CONSTANT = 0, GT_BACKGROUND_VERTICAL_GRADIENT = 1 };
+/* RNG streams: GT_RNG_COUNTER is the per-(pixel, sample, bounce) stream the
+ * GPU uses; GT_RNG_PIXEL_STREAM is the reference's one sequential stream per
+ * pixel (camera.rs:2669), available in the CPU oracle only. */
+enum { GT_RNG_COUNTER = 0, GT_RNG_PIXEL_STREAM = 1 };
+
+typedef struct gt_camera {
+  uint32_t image_width;             /* camera.rs:1035, > 0 */
+  uint32_t samples_per_pixel;       /* clamped to >= 1 */
+  double aspect_ratio;              /* finite, > 0 */
+  uint32_t pixel_sample_mode;       /* GT_PIXEL_* (camera.rs:40-49) */
+  uint32_t stratified_grid_width;   /* for GT_PIXEL_STRATIFIED_GRID */
+  uint32_t max_depth;               /* default 10 */
+  int32_t russian_roulette_min_depth; /* default 5; -1 = off */
+  uint64_t rng_seed;                /* default 1 */
+  double vertical_fov;              /* degrees, (0, 180) */
+  double lookfrom[3];
+  double lookat[3];
+  double view_up[3];
+  double defocus_angle;             /* degrees, [0, 180) */
+  double focus_distance;            /* > 0 */
+  double shutter_start, shutter_end;
+  uint32_t background_kind;         /* GT_BACKGROUND_* (camera.rs:212-322) */
+  uint32_t direct_lighting_mode;    /* GT_LIGHTING_* (camera.rs:92-196) */
+  double background[9];             /* constant: [0..2]; gradient: nadir, horizon, zenith */
+  double light_pdf_weight;          /* mixture weight for continuation mode */
+  uint32_t rng_mode;                /* GT_RNG_* */
+  uint32_t reserved;
+} gt_camera;
+
+/* RayCamera::new(100, 1.0) defaults (camera.rs:1034-1064). */
+void gt_camera_default(gt_camera* camera);
+/* floor(width / aspect) clamped to >= 1 (camera.rs:1071). */
+uint32_t gt_camera_image_height(const gt_camera* camera);
+/* samples actually taken per pixel (camera.rs:1547-1557). */
+uint32_t gt_camera_effective_spp(const gt_camera* camera);
+
+/* ---- context ----------------------------------------------------------- */
+int gt_context_create(int device, gt_context** out);
+int gt_context_destroy(gt_context* ctx);
+int gt_context_synchronize(gt_context* ctx);
+/* The context's hipStream_t, as an opaque pointer. */
+void* gt_context_stream(gt_context* ctx);
+/* Thread-local message of the last failure ("" when none). */
+const char* gt_last_error(void);
+const char* gt_version(void);
+
+/* Tunables picked by the autotuner (persistent grid, queue size, BVH width). */
+typedef struct gt_tuning {
+  uint32_t block_size;      /* threads per block of the wavefront kernels: 64..1024 */
+  uint32_t blocks_per_cu;   /* persistent blocks per CU; 0 = occupancy maximum */
+  uint64_t max_paths;       /* wavefront queue capacity in paths; 0 = automatic */
+  uint32_t bvh_width;       /* 2 or 4 */
+  uint32_t use_graph;       /* capture the per-batch bounce loop in a hipGraph */
+} gt_tuning;
+int gt_context_get_tuning(gt_context* ctx, gt_tuning* out);
+int gt_context_set_tuning(gt_context* ctx, const gt_tuning* tuning);
+
+/* ---- scene (RayScene, src/graphics/raytracing/scene.rs:1102-1578) ------- */
+int gt_scene_create(gt_context* ctx, gt_scene** out);
+int gt_scene_destroy(gt_scene* scene);
+/* RayScene::add_material with RayMaterial::{Lambertian (material.rs:353),
+ * DiffuseLight (:940), GgxMicrofacet (:1383), Metal (:1962),
+ * Dielectric (:2180)}; colours must be finite. */
+int gt_scene_add_lambertian(gt_scene* scene, double r, double g, double b, uint32_t* id);
+int gt_scene_add_diffuse_light(gt_scene* scene, double r, double g, double b, uint32_t* id);
+int gt_scene_add_ggx(gt_scene* scene, double r, double g, double b, double roughness, uint32_t* id);
+int gt_scene_add_metal(gt_scene* scene, double r, double g, double b, double fuzz, uint32_t* id);
+int gt_scene_add_dielectric(gt_scene* scene, double refraction_index, uint32_t* id);
+/* RayScene::add_{sphere,moving_sphere,triangle,quad}[s] (scene.rs:1237-1286).
+ * Layouts per primitive: sphere (cx,cy,cz,r); moving sphere
+ * (c0x,c0y,c0z,c1x,c1y,c1z,r); triangle (p0,p1,p2); quad (corner,u,v).
+ * material_ids[i] must name an existing material (scene.rs:1158-1161).
+ * Primitive ids are assigned in insertion order over all kinds. */
+int gt_scene_add_spheres(gt_scene* scene, const double* data, size_t n, const uint32_t* material_ids);
+int gt_scene_add_moving_spheres(gt_scene* scene, const double* data, size_t n, const uint32_t* material_ids);
+int gt_scene_add_triangles(gt_scene* scene, const double* data, size_t n, const uint32_t* material_ids);
+int gt_scene_add_quads(gt_scene* scene, const double* data, size_t n, const uint32_t* material_ids);
+/* RayScene::build_bvh_with_options (scene.rs:1562, bvh.rs:233-288): SAH
+ * build with the reference's exact splits, then upload to the GPU.
+ * leaf_size 0 is clamped to 1 (bvh.rs:73-75). Any later add invalidates it. */
+int gt_scene_build(gt_scene* scene, uint32_t leaf_size);
+
+typedef struct gt_scene_info {
+  uint64_t primitive_count;
+  uint64_t material_count;
+  uint64_t bvh_node_count;      /* reference FlatBvh node count (scene.rs:1438) */
+  uint64_t bvh_depth;           /* deepest root-to-leaf path, root = 1 */
+  uint64_t device_bytes;        /* nodes + primitives + materials on the GPU */
+  double bounds_min[3], bounds_max[3];
+} gt_scene_info;
+int gt_scene_get_info(const gt_scene* scene, gt_scene_info* out);
+
+/* ---- closest-hit queries (the intersection oracle) ---------------------
+ * RayScene::hit (scene.rs:1639-1657) with the hit primitive exposed
+ * (scene.rs:849-874). One record per ray; prim_id -1 means no hit.
+ */
+typedef struct gt_ray {
+  float origin[3];
+  float time;
+  float direction[3];
+  float t_max;   /* ray interval is (t_min, t_max) */
+} gt_ray;
+typedef struct gt_hit {
+  float t, u, v;
+  int32_t prim_id;
+  float normal[3];     /* oriented against the ray (object.rs:127-149) */
+  uint32_t front_face;
+} gt_hit;
+/* BvhTraversalStats (bvh.rs:9-17), summed over the rays. */
+typedef struct gt_traversal_stats {
+  uint64_t rays;
+  uint64_t node_bounds_tests;
+  uint64_t node_bounds_hits;
+  uint64_t leaf_visits;
+  uint64_t primitive_candidates;
+  uint64_t leaf_hit_results;
+  uint64_t max_stack_depth;
+} gt_traversal_stats;
+int gt_intersect(gt_context* ctx, const gt_scene* scene, const gt_ray* rays, size_t n,
+                 float t_min, gt_hit* out_hits, gt_traversal_stats* out_stats /* nullable */);
+
+/* ---- light targets (SamplingTargetList / WeightedSamplingTargetList,
+ *      scene.rs:363-609) ------------------------------------------------- */
+int gt_lights_create(gt_context* ctx, gt_lights** out);
+int gt_lights_destroy(gt_lights* lights);
+/* weight must be finite and > 0 (scene.rs:529-537); add_* without weight = 1. */
+int gt_lights_add_sphere(gt_lights* lights, const double center[3], double radius, double weight);
+int gt_lights_add_moving_sphere(gt_lights* lights, const double c0[3], const double c1[3],
+                                double radius, double weight);
+int gt_lights_add_quad(gt_lights* lights, const double corner[3], const double u[3],
+                       const double v[3], double weight);
+int gt_lights_add_triangle(gt_lights* lights, const double p0[3], const double p1[3],
+                           const double p2[3], double weight);
+size_t gt_lights_len(const gt_lights* lights);
+/* RayScene::emissive_targets (scene.rs:1421-1423): every primitive whose
+ * material is emissive, weight 1, in primitive order. */
+int gt_scene_emissive_targets(const gt_scene* scene, gt_lights** out);
+
+/* ---- environment light (EnvironmentLight, environment.rs:22-195) -------
+ * gt_env_create_constant mirrors EnvironmentLight::constant (:64-77).
+ * gt_env_create_rgb32f is the FP32 HDR lat-long form: linear RGB texels,
+ * row 0 at theta = 0 (+y), same lookup, CDF and PDF as the 8-bit form. */
+int gt_env_create_constant(gt_context* ctx, double r, double g, double b, gt_env** out);
+int gt_env_create_rgb32f(gt_context* ctx, const float* rgb, uint32_t width, uint32_t height, gt_env** out);
+int gt_env_destroy(gt_env* env);
+
+/* ---- rendering --------------------------------------------------------
+ * gt_render_hdr: RayCamera::render_world*_hdr_image (camera.rs:4246-4336)
+ * with the camera's strategy as given (the PathTracer layer forces NEE for
+ * environment and light-connection renders, camera.rs:4167-4173).
+ * lights and env may be NULL. out_rgb: W*H*3 floats, row-major, top row
+ * first (HdrImage::from_pixels, display.rs:295).
+ */
+typedef struct gt_render_stats {
+  uint64_t camera_paths;      /* paths started (= W*H*spp) */
+  uint64_t extension_rays;    /* closest-hit rays traced, primary included */
+  uint64_t shadow_rays;       /* NEE rays traced */
+  uint64_t accepted_samples;  /* finite samples (camera.rs:2984-2991) */
+  double device_ms;           /* device time of the render, HIP events */
+  double extend_ms;           /* summed duration of the extend kernel launches */
+  uint64_t extend_launches;
+  double shade_ms, connect_ms;
+} gt_render_stats;
+int gt_render_hdr(gt_context* ctx, const gt_scene* scene, const gt_camera* camera,
+                  const gt_lights* lights, const gt_env* env, float* out_rgb,
+                  gt_render_stats* out_stats /* nullable */);
+/* Same render; result stays on the device (float W*H*3, or double when
+ * out_is_double != 0, the FP64-accumulated form used for references). */
+int gt_render_hdr_device(gt_context* ctx, const gt_scene* scene, const gt_camera* camera,
+                         const gt_lights* lights, const gt_env* env, void* d_out,
+                         int out_is_double, gt_render_stats* out_stats);
+/* RayCamera::render_world*_denoising_aovs (camera.rs:4175-4244): beauty as
+ * gt_render_hdr, albedo = first-hit denoise_albedo, normal = 0.5*(n+1),
+ * both 0 on a miss, averaged over the same samples (camera.rs:3064-3116). */
+int gt_render_aovs(gt_context* ctx, const gt_scene* scene, const gt_camera* camera,
+                   const gt_lights* lights, const gt_env* env,
+                   float* out_beauty, float* out_albedo, float* out_normal,
+                   gt_render_stats* out_stats);
+int gt_render_aovs_device(gt_context* ctx, const gt_scene* scene, const gt_camera* camera,
+                          const gt_lights* lights, const gt_env* env,
+                          float* d_beauty, float* d_albedo, float* d_normal,
+                          gt_render_stats* out_stats);
+
+/* ---- batched rendering: many scenes and cameras, one persistent launch
+ * sequence. No reference counterpart (the reference renders one scene per
+ * call); each result equals gt_render_hdr of that scene alone. */
+int gt_render_batch(gt_context* ctx, size_t n, const gt_scene* const* scenes,
+                    const gt_camera* cameras, const gt_lights* const* lights,
+                    float* const* out_rgb, gt_render_stats* out_stats);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* GARTUS_HIP_H */

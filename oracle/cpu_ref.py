@@ -1,0 +1,278 @@
+"""ctypes wrapper of the CPU oracle (oracle/libcpu_ref.so).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and
+bench.py's CPU-baseline leg, as the checker. The product never imports it.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from pathlib import Path
+
+import numpy as np
+
+HERE = Path(__file__).resolve().parent
+LIB = HERE / "libcpu_ref.so"
+
+import sys  # noqa: E402
+
+sys.path.insert(0, str(HERE.parent))
+from gartus_amd import abi  # noqa: E402  (structure layouts only)
+from gartus_amd.scene_data import SceneData  # noqa: E402
+
+_lib = None
+
+P = C.c_void_p
+D = C.POINTER(C.c_double)
+U8 = C.POINTER(C.c_uint8)
+U32 = C.POINTER(C.c_uint32)
+U64 = C.POINTER(C.c_uint64)
+I64 = C.POINTER(C.c_int64)
+CAM = C.POINTER(abi.GtCamera)
+
+_PROTOS = {
+    "cr_last_error": (C.c_char_p, []),
+    "cr_scene_new": (P, []),
+    "cr_scene_free": (None, [P]),
+    "cr_scene_add_material": (C.c_int, [P, C.c_int, C.c_double, C.c_double, C.c_double, C.c_double, U32]),
+    "cr_scene_add_prims": (C.c_int, [P, C.c_int, D, C.c_size_t, U32]),
+    "cr_scene_build": (C.c_int, [P, C.c_uint32]),
+    "cr_scene_node_count": (C.c_uint64, [P]),
+    "cr_scene_depth": (C.c_uint64, [P]),
+    "cr_scene_bvh_dump": (C.c_int, [P, D, U8, U64, U64, U64]),
+    "cr_intersect": (C.c_int, [P, D, C.c_size_t, C.c_double, C.c_int, I64, D, D, U8, U64]),
+    "cr_lights_new": (P, []),
+    "cr_lights_free": (None, [P]),
+    "cr_lights_add": (C.c_int, [P, C.c_int, D, C.c_double]),
+    "cr_lights_len": (C.c_size_t, [P]),
+    "cr_scene_emissive_targets": (P, [P]),
+    "cr_lights_pdf_value": (C.c_double, [P, D, C.c_double, D]),
+    "cr_lights_random_direction": (None, [P, D, C.c_double, C.c_uint64, C.c_size_t, D]),
+    "cr_env_constant": (P, [C.c_double, C.c_double, C.c_double]),
+    "cr_env_rgb32f": (P, [C.POINTER(C.c_float), C.c_uint32, C.c_uint32]),
+    "cr_env_rgb8": (P, [U8, C.c_uint32, C.c_uint32]),
+    "cr_env_free": (None, [P]),
+    "cr_env_sample": (None, [P, C.c_uint64, C.c_size_t, D, D]),
+    "cr_env_eval": (None, [P, D, C.c_size_t, D, D]),
+    "cr_pixel_seed": (C.c_uint64, [C.c_uint64, C.c_uint32, C.c_uint32]),
+    "cr_stream_seed": (C.c_uint64, [C.c_uint64, C.c_uint32, C.c_uint32]),
+    "cr_rng_u64": (None, [C.c_uint64, C.c_size_t, U64]),
+    "cr_rng_double": (None, [C.c_uint64, C.c_size_t, D]),
+    "cr_rng_vectors": (None, [C.c_uint64, C.c_int, C.c_size_t, D]),
+    "cr_cosine_pdf_value": (C.c_double, [D, D]),
+    "cr_ggx_pdf": (C.c_int, [D, D, C.c_double, D, C.c_uint64, C.c_size_t, D, D]),
+    "cr_ggx_D": (C.c_double, [C.c_double, C.c_double]),
+    "cr_smith_g1": (C.c_double, [C.c_double, C.c_double]),
+    "cr_schlick": (C.c_double, [C.c_double, C.c_double]),
+    "cr_ggx_weight": (C.c_double, [D, D, D, C.c_double, C.c_double]),
+    "cr_power_heuristic": (C.c_double, [C.c_double, C.c_double]),
+    "cr_russian_roulette": (C.c_int, [C.c_uint32, C.c_int32, D, C.c_uint64]),
+    "cr_camera_frame": (C.c_int, [CAM, D, U32]),
+    "cr_camera_ray": (C.c_int, [CAM, C.c_uint32, C.c_uint32, C.c_uint32, D]),
+    "cr_material_scatter": (C.c_int, [P, C.c_uint32, D, D, C.c_uint64, D, D, D]),
+    "cr_material_scattering_pdf": (C.c_double, [P, C.c_uint32, D, D, D]),
+    "cr_render": (C.c_int, [P, CAM, P, P, C.c_int, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32, D, U64]),
+    "cr_render_aovs": (C.c_int, [P, CAM, P, P, C.c_int, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32,
+                                 D, D, D, U64]),
+    "cr_tone_map": (None, [D, C.c_size_t, C.c_double, C.c_int, C.c_double, U8]),
+}
+
+
+def lib() -> C.CDLL:
+    global _lib
+    if _lib is None:
+        if not LIB.exists():
+            raise RuntimeError(f"{LIB} missing: run `make -C oracle`")
+        L = C.CDLL(str(LIB))
+        for n, (r, a) in _PROTOS.items():
+            f = getattr(L, n)
+            f.restype = r
+            f.argtypes = a
+        _lib = L
+    return _lib
+
+
+def _d(a) -> np.ndarray:
+    return np.ascontiguousarray(np.asarray(a, dtype=np.float64))
+
+
+def _dp(a: np.ndarray):
+    return a.ctypes.data_as(D)
+
+
+def _check(rc: int) -> None:
+    if rc != 0:
+        raise ValueError(lib().cr_last_error().decode())
+
+
+class Scene:
+    """Oracle RayScene built from a SceneData (same insertion order)."""
+
+    def __init__(self, data: SceneData, leaf_size: int | None = 4):
+        L = lib()
+        self.h = L.cr_scene_new()
+        mid = C.c_uint32()
+        for m in data.materials:
+            _check(L.cr_scene_add_material(self.h, m.kind, *m.color, m.param, C.byref(mid)))
+        for kind, arr, mids in data.chunks:
+            a = _d(arr)
+            mm = np.ascontiguousarray(mids, dtype=np.uint32)
+            _check(L.cr_scene_add_prims(self.h, kind, _dp(a), len(a), mm.ctypes.data_as(U32)))
+        if leaf_size is not None:
+            L.cr_scene_build(self.h, int(leaf_size))
+        self.n = len(data)
+
+    def __del__(self):
+        try:
+            lib().cr_scene_free(self.h)
+        except Exception:
+            pass
+
+    def node_count(self) -> int:
+        return int(lib().cr_scene_node_count(self.h))
+
+    def depth(self) -> int:
+        return int(lib().cr_scene_depth(self.h))
+
+    def bvh(self):
+        n = self.node_count()
+        b = np.zeros((n, 6))
+        leaf = np.zeros(n, np.uint8)
+        a = np.zeros(n, np.uint64)
+        c = np.zeros(n, np.uint64)
+        idx = np.zeros(self.n, np.uint64)
+        _check(lib().cr_scene_bvh_dump(self.h, _dp(b), leaf.ctypes.data_as(U8), a.ctypes.data_as(U64),
+                                       c.ctypes.data_as(U64), idx.ctypes.data_as(U64)))
+        return b, leaf.astype(bool), a, c, idx
+
+    def intersect(self, origins, directions, times=None, t_min=0.001, t_max=np.inf, use_bvh=True):
+        o = _d(origins).reshape(-1, 3)
+        d = _d(directions).reshape(-1, 3)
+        n = len(o)
+        rays = np.zeros((n, 8))
+        rays[:, 0:3] = o
+        rays[:, 3] = 0.0 if times is None else _d(times)
+        rays[:, 4:7] = d
+        rays[:, 7] = np.broadcast_to(np.asarray(t_max, np.float64), (n,))
+        prim = np.zeros(n, np.int64)
+        tuv = np.zeros((n, 3))
+        nrm = np.zeros((n, 3))
+        front = np.zeros(n, np.uint8)
+        stats = np.zeros(7, np.uint64)
+        _check(lib().cr_intersect(self.h, _dp(rays), n, float(t_min), int(use_bvh), prim.ctypes.data_as(I64),
+                                  _dp(tuv), _dp(nrm), front.ctypes.data_as(U8), stats.ctypes.data_as(U64)))
+        keys = ["rays", "node_bounds_tests", "node_bounds_hits", "leaf_visits", "primitive_candidates",
+                "leaf_hit_results", "max_stack_depth"]
+        return {"prim": prim, "t": tuv[:, 0], "u": tuv[:, 1], "v": tuv[:, 2], "normal": nrm,
+                "front": front.astype(bool), "stats": dict(zip(keys, (int(x) for x in stats)))}
+
+    def emissive_targets(self) -> "Lights":
+        return Lights(handle=lib().cr_scene_emissive_targets(self.h))
+
+    def material_scatter(self, mid: int, ray7, hit7, seed: int):
+        out_dir = np.zeros(3)
+        att = np.zeros(3)
+        pdf = np.zeros(1)
+        kind = lib().cr_material_scatter(self.h, mid, _dp(_d(ray7)), _dp(_d(hit7)), seed, _dp(out_dir), _dp(att),
+                                         _dp(pdf))
+        return kind, out_dir, att, float(pdf[0])
+
+    def scattering_pdf(self, mid: int, ray7, hit7, direction) -> float:
+        return float(lib().cr_material_scattering_pdf(self.h, mid, _dp(_d(ray7)), _dp(_d(hit7)), _dp(_d(direction))))
+
+    def render(self, camera: abi.GtCamera, lights=None, env=None, threads: int = 0, crop=None):
+        """Linear HDR image (f64) of the whole image or crop=(x0, y0, w, h),
+        plus counters (paths, extension rays, shadow rays, accepted)."""
+        x0, y0, w, h = crop if crop else (0, 0, 0, 0)
+        W = camera.image_width
+        H = max(int(W / camera.aspect_ratio), 1)
+        if not crop:
+            w, h = W, H
+        out = np.zeros((h, w, 3))
+        ctr = np.zeros(4, np.uint64)
+        _check(lib().cr_render(self.h, C.byref(camera), lights.h if lights else None, env.h if env else None,
+                               int(threads), x0, y0, w, h, _dp(out), ctr.ctypes.data_as(U64)))
+        return out, {"paths": int(ctr[0]), "extension_rays": int(ctr[1]), "shadow_rays": int(ctr[2]),
+                     "accepted": int(ctr[3])}
+
+    def render_aovs(self, camera: abi.GtCamera, lights=None, env=None, threads: int = 0, crop=None):
+        x0, y0, w, h = crop if crop else (0, 0, 0, 0)
+        W = camera.image_width
+        H = max(int(W / camera.aspect_ratio), 1)
+        if not crop:
+            w, h = W, H
+        b = np.zeros((h, w, 3))
+        a = np.zeros_like(b)
+        n = np.zeros_like(b)
+        ctr = np.zeros(4, np.uint64)
+        _check(lib().cr_render_aovs(self.h, C.byref(camera), lights.h if lights else None, env.h if env else None,
+                                    int(threads), x0, y0, w, h, _dp(b), _dp(a), _dp(n), ctr.ctypes.data_as(U64)))
+        return b, a, n
+
+
+class Lights:
+    """WeightedSamplingTargetList: targets (kind, params, weight)."""
+
+    def __init__(self, targets=(), handle=None):
+        self.h = handle if handle is not None else lib().cr_lights_new()
+        for kind, q, w in targets:
+            q9 = np.zeros(9)
+            q = _d(q)
+            q9[: len(q)] = q
+            _check(lib().cr_lights_add(self.h, int(kind), _dp(q9), float(w)))
+
+    def __del__(self):
+        try:
+            lib().cr_lights_free(self.h)
+        except Exception:
+            pass
+
+    def __len__(self) -> int:
+        return int(lib().cr_lights_len(self.h))
+
+    def pdf_value(self, origin, time, direction) -> float:
+        return float(lib().cr_lights_pdf_value(self.h, _dp(_d(origin)), float(time), _dp(_d(direction))))
+
+    def random_directions(self, origin, time, seed, n) -> np.ndarray:
+        out = np.zeros((n, 3))
+        lib().cr_lights_random_direction(self.h, _dp(_d(origin)), float(time), seed, n, _dp(out))
+        return out
+
+
+class Env:
+    def __init__(self, color=None, rgb32f=None, rgb8=None):
+        L = lib()
+        if color is not None:
+            self.h = L.cr_env_constant(*[float(c) for c in color])
+        elif rgb32f is not None:
+            a = np.ascontiguousarray(rgb32f, np.float32)
+            self.h = L.cr_env_rgb32f(a.ctypes.data_as(C.POINTER(C.c_float)), a.shape[1], a.shape[0])
+        else:
+            a = np.ascontiguousarray(rgb8, np.uint8)
+            self.h = L.cr_env_rgb8(a.ctypes.data_as(U8), a.shape[1], a.shape[0])
+
+    def __del__(self):
+        try:
+            lib().cr_env_free(self.h)
+        except Exception:
+            pass
+
+    def sample(self, seed: int, n: int):
+        dirs = np.zeros((n, 3))
+        pdfs = np.zeros(n)
+        lib().cr_env_sample(self.h, seed, n, _dp(dirs), _dp(pdfs))
+        return dirs, pdfs
+
+    def eval(self, dirs):
+        d = _d(dirs).reshape(-1, 3)
+        rad = np.zeros((len(d), 3))
+        pdf = np.zeros(len(d))
+        lib().cr_env_eval(self.h, _dp(d), len(d), _dp(rad), _dp(pdf))
+        return rad, pdf
+
+
+def cpu_count() -> int:
+    try:
+        return len(os.sched_getaffinity(0))
+    except AttributeError:
+        return os.cpu_count() or 1
