@@ -52,6 +52,7 @@ class GtTuning(C.Structure):
         ("max_paths", C.c_uint64),
         ("bvh_width", C.c_uint32),
         ("use_graph", C.c_uint32),
+        ("count_traversal", C.c_uint32),
     ]
 
 
@@ -110,6 +111,10 @@ class GtRenderStats(C.Structure):
         ("extend_launches", C.c_uint64),
         ("shade_ms", C.c_double),
         ("connect_ms", C.c_double),
+        ("ext_node_fetches", C.c_uint64),
+        ("ext_prim_tests", C.c_uint64),
+        ("shadow_node_fetches", C.c_uint64),
+        ("shadow_prim_tests", C.c_uint64),
     ]
 
 
@@ -129,6 +134,9 @@ PROTOTYPES = {
     "gt_context_destroy": (I, [P]),
     "gt_context_synchronize": (I, [P]),
     "gt_context_stream": (P, [P]),
+    "gt_device_malloc": (I, [P, C.c_size_t, PP]),
+    "gt_device_free": (I, [P, P]),
+    "gt_memcpy_to_host": (I, [P, P, P, C.c_size_t]),
     "gt_last_error": (C.c_char_p, []),
     "gt_version": (C.c_char_p, []),
     "gt_context_get_tuning": (I, [P, C.POINTER(GtTuning)]),

@@ -451,6 +451,7 @@ int render_impl(gt_context* ctx, const gt_scene* cscene, const gt_camera* camera
   hipStream_t st = ctx->stream;
   size_t ctr_words = size_t(depth + 2) * gtk::kCtrStride;
   const bool want_stats = stats != nullptr;
+  const bool count = want_stats && ctx->tuning.count_traversal != 0;
   const bool time_kernels = want_stats;
   size_t nev = time_kernels ? 2 + 5 * size_t(nbatch) * depth : 0;
   while (ctx->events.size() < nev) {
@@ -477,15 +478,15 @@ int render_impl(gt_context* ctx, const gt_scene* cscene, const gt_camera* camera
     gtk::raygen(L, f, q, s0, ns);
     for (uint32_t bounce = 0; bounce < depth; ++bounce) {
       if (time_kernels) GT_HIP(hipEventRecord(ctx->events[ev++], st));
-      gtk::extend(L, f, q, bounce);
+      gtk::extend(L, f, q, bounce, count);
       if (time_kernels) {
         GT_HIP(hipEventRecord(ctx->events[ev++], st));
         GT_HIP(hipEventRecord(ctx->events[ev++], st));
       }
       gtk::shade(L, f, q, bounce, s0, bounce + 1 == depth ? 1u : 0u);
       if (time_kernels) GT_HIP(hipEventRecord(ctx->events[ev++], st));
-      if (conn_l) gtk::connect(L, f, q, bounce, 0);
-      if (conn_e) gtk::connect(L, f, q, bounce, 1);
+      if (conn_l) gtk::connect(L, f, q, bounce, 0, count);
+      if (conn_e) gtk::connect(L, f, q, bounce, 1, count);
       if (time_kernels) GT_HIP(hipEventRecord(ctx->events[ev++], st));
     }
     gtk::accumulate(L, f, q, ctx->acc.p, ns);
@@ -502,10 +503,17 @@ int render_impl(gt_context* ctx, const gt_scene* cscene, const gt_camera* camera
     stats->camera_paths = uint64_t(npix) * spp;
     for (uint32_t b = 0; b < nbatch; ++b) {
       const uint32_t* c = ctx->host_ctr + ctr_words * b;
+      stats->accepted_samples += c[gtk::C_ACCEPTED];
       for (uint32_t bounce = 0; bounce < depth; ++bounce) {
         stats->extension_rays += c[bounce * gtk::kCtrStride + gtk::C_QCOUNT];
         if (conn_l) stats->shadow_rays += c[bounce * gtk::kCtrStride + gtk::C_SL_COUNT];
         if (conn_e) stats->shadow_rays += c[bounce * gtk::kCtrStride + gtk::C_SE_COUNT];
+        if (count) {
+          uint64_t v[4];
+          std::memcpy(v, c + bounce * gtk::kCtrStride + gtk::C_COUNT64, sizeof v);
+          stats->ext_node_fetches += v[0]; stats->ext_prim_tests += v[1];
+          stats->shadow_node_fetches += v[2]; stats->shadow_prim_tests += v[3];
+        }
       }
     }
     float ms = 0;
@@ -523,7 +531,6 @@ int render_impl(gt_context* ctx, const gt_scene* cscene, const gt_camera* camera
         base += 5;
       }
     }
-    stats->accepted_samples = 0;
   }
   return GT_OK;
 }
@@ -616,6 +623,28 @@ int gt_context_synchronize(gt_context* ctx) {
 }
 
 void* gt_context_stream(gt_context* ctx) { return ctx ? static_cast<void*>(ctx->stream) : nullptr; }
+
+int gt_device_malloc(gt_context* ctx, size_t bytes, void** out) {
+  if (!ctx || !out) return fail(GT_EINVAL, "null argument");
+  GT_HIP(hipSetDevice(ctx->device));
+  GT_HIP(hipMalloc(out, std::max<size_t>(bytes, 1)));
+  return GT_OK;
+}
+
+int gt_device_free(gt_context* ctx, void* ptr) {
+  if (!ctx) return fail(GT_EINVAL, "null argument");
+  GT_HIP(hipSetDevice(ctx->device));
+  GT_HIP(hipStreamSynchronize(ctx->stream));
+  if (ptr) GT_HIP(hipFree(ptr));
+  return GT_OK;
+}
+
+int gt_memcpy_to_host(gt_context* ctx, void* dst, const void* src, size_t bytes) {
+  if (!ctx || (bytes && (!dst || !src))) return fail(GT_EINVAL, "null argument");
+  GT_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, ctx->stream));
+  GT_HIP(hipStreamSynchronize(ctx->stream));
+  return GT_OK;
+}
 
 int gt_context_get_tuning(gt_context* ctx, gt_tuning* out) {
   if (!ctx || !out) return fail(GT_EINVAL, "null argument");

@@ -17,6 +17,11 @@ enum {
   C_SE_COUNT = 4,    // environment shadow rays queued at bounce b
   C_SL_WORK = 5,
   C_SE_WORK = 6,
+  C_ACCEPTED = 7,    // finite samples accumulated in this batch (bounce 0 block)
+  // with counting: u64 totals at 64-bit word (b * kCtrStride + C_COUNT64) / 2 + k,
+  // k = 0 extension node fetches, 1 extension prim tests, 2 shadow node
+  // fetches, 3 shadow prim tests
+  C_COUNT64 = 8,
 };
 
 struct Queues {
@@ -52,9 +57,9 @@ struct Launch {
 
 void reset_counters(const Launch& l, uint32_t* ctr, uint32_t n);
 void raygen(const Launch& l, const Frame& f, const Queues& q, uint32_t sample0, uint32_t nsamples);
-void extend(const Launch& l, const Frame& f, const Queues& q, uint32_t bounce);
+void extend(const Launch& l, const Frame& f, const Queues& q, uint32_t bounce, bool count);
 void shade(const Launch& l, const Frame& f, const Queues& q, uint32_t bounce, uint32_t sample0, uint32_t last);
-void connect(const Launch& l, const Frame& f, const Queues& q, uint32_t bounce, uint32_t kind);
+void connect(const Launch& l, const Frame& f, const Queues& q, uint32_t bounce, uint32_t kind, bool count);
 void accumulate(const Launch& l, const Frame& f, const Queues& q, double4* acc, uint32_t nsamples);
 void finalize(const Launch& l, const double4* acc, uint32_t npix, void* out, int out_is_double);
 void aov(const Launch& l, const Frame& f, const Queues& q, float* albedo, float* normal);

@@ -115,7 +115,21 @@ __global__ void k_raygen(Frame f, Queues q, uint32_t sample0, uint32_t nsamples)
   }
 }
 
+// Wave-reduced 64-bit add of per-lane counts (one atomic per wave).
+__device__ __forceinline__ void wave_count(unsigned long long* dst, uint32_t v) {
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off);
+  if (lane_id() == 0 && v) atomicAdd(dst, (unsigned long long)v);
+}
+__device__ __forceinline__ unsigned long long* count_slot(const Queues& q, uint32_t bounce, int k) {
+  return reinterpret_cast<unsigned long long*>(q.ctr + bounce * kCtrStride + C_COUNT64) + k;
+}
+__device__ __forceinline__ uint32_t node_fetches(const Stats& s, uint32_t width) {
+  // every internal node fetched tests `width` child boxes; the root box test is not a fetch
+  return (s.node_tests - 1) / width;
+}
+
 // Closest hit of every queued extension ray.
+template <bool COUNT>
 __global__ void k_extend(Frame f, Queues q, uint32_t bounce) {
   extern __shared__ uint2 smem[];
   Stack st = make_stack(q, smem);
@@ -126,15 +140,22 @@ __global__ void k_extend(Frame f, Queues q, uint32_t bounce) {
   for (;;) {
     uint32_t base = wave_fetch(cursor);
     if (base >= count) break;
+    uint32_t nodes = 0, prims = 0;
     for (uint32_t it = 0; it < kItemsPerLane; ++it) {
       uint32_t i = base + it * 64u + lane_id();
       if (i >= count) break;
       float4 o4 = qo[i], d4 = qd[i];
       float t;
       uint32_t prim;
-      bool hit = trace<false, false>(f.scene, mk(o4.x, o4.y, o4.z), mk(d4.x, d4.y, d4.z), o4.w, SHADOW_ACNE, INF, st,
-                                     &t, &prim, nullptr, nullptr);
+      Stats s{0, 0, 0, 0, 0, 0};
+      bool hit = trace<false, COUNT>(f.scene, mk(o4.x, o4.y, o4.z), mk(d4.x, d4.y, d4.z), o4.w, SHADOW_ACNE, INF, st,
+                                     &t, &prim, nullptr, &s);
       q.hits[i] = hit ? make_float2(t, __uint_as_float(prim)) : make_float2(0.f, __uint_as_float(kMiss));
+      if (COUNT) { nodes += node_fetches(s, f.scene.width); prims += s.candidates; }
+    }
+    if (COUNT) {
+      wave_count(count_slot(q, bounce, 0), nodes);
+      wave_count(count_slot(q, bounce, 1), prims);
     }
   }
 }
@@ -313,6 +334,7 @@ __global__ void k_shade(Frame f, Queues q, uint32_t bounce, uint32_t sample0, ui
 // Shadow rays of one kind. Light targets: closest hit, then the emission of
 // whatever was hit (camera.rs:2248-2261). Environment: any hit occludes
 // (camera.rs:2284-2293; "is some hit" equals "is some closest hit").
+template <bool COUNT>
 __global__ void k_connect(Frame f, Queues q, uint32_t bounce, uint32_t kind) {
   extern __shared__ uint2 smem[];
   Stack st = make_stack(q, smem);
@@ -324,9 +346,11 @@ __global__ void k_connect(Frame f, Queues q, uint32_t bounce, uint32_t kind) {
   for (;;) {
     uint32_t base = wave_fetch(cursor);
     if (base >= count) break;
+    uint32_t nodes = 0, prims = 0;
     for (uint32_t it = 0; it < kItemsPerLane; ++it) {
       uint32_t i = base + it * 64u + lane_id();
       if (i >= count) break;
+      Stats s{0, 0, 0, 0, 0, 0};
       float4 o4 = so[i], d4 = sd[i], w4 = sw[i];
       uint32_t path = __float_as_uint(d4.w);
       f3 o = mk(o4.x, o4.y, o4.z), d = mk(d4.x, d4.y, d4.z);
@@ -335,7 +359,7 @@ __global__ void k_connect(Frame f, Queues q, uint32_t bounce, uint32_t kind) {
       f3 add = mk(0.f, 0.f, 0.f);
       bool contributes = false;
       if (kind == 0) {
-        if (trace<false, false>(f.scene, o, d, o4.w, SHADOW_ACNE, INF, st, &t, &prim, nullptr, nullptr)) {
+        if (trace<false, COUNT>(f.scene, o, d, o4.w, SHADOW_ACNE, INF, st, &t, &prim, nullptr, &s)) {
           f3 p = o + t * d;
           uint32_t mid, pk;
           f3 outward = prim_outward_normal(f.scene.prims, prim, p, o4.w, &mid, &pk);
@@ -346,7 +370,7 @@ __global__ void k_connect(Frame f, Queues q, uint32_t bounce, uint32_t kind) {
           }
         }
       } else {
-        if (!trace<true, false>(f.scene, o, d, o4.w, SHADOW_ACNE, INF, st, &t, &prim, nullptr, nullptr)) {
+        if (!trace<true, COUNT>(f.scene, o, d, o4.w, SHADOW_ACNE, INF, st, &t, &prim, nullptr, &s)) {
           add = mk(w4.x, w4.y, w4.z);
           contributes = true;
         }
@@ -355,6 +379,11 @@ __global__ void k_connect(Frame f, Queues q, uint32_t bounce, uint32_t kind) {
         float4 L = q.radiance[path];
         q.radiance[path] = make_float4(L.x + add.x, L.y + add.y, L.z + add.z, 0.f);
       }
+      if (COUNT) { nodes += node_fetches(s, f.scene.width); prims += s.candidates; }
+    }
+    if (COUNT) {
+      wave_count(count_slot(q, bounce, 2), nodes);
+      wave_count(count_slot(q, bounce, 3), prims);
     }
   }
 }
@@ -362,16 +391,25 @@ __global__ void k_connect(Frame f, Queues q, uint32_t bounce, uint32_t kind) {
 // Adds each finished sample of the batch to its pixel in sample order,
 // dropping non-finite samples (add_finite_sample camera.rs:2984-2991).
 __global__ void k_accumulate(Frame f, Queues q, double4* acc, uint32_t nsamples) {
-  for (uint32_t px = blockIdx.x * blockDim.x + threadIdx.x; px < f.npix; px += gridDim.x * blockDim.x) {
-    double4 a = acc[px];
-    for (uint32_t s = 0; s < nsamples; ++s) {
-      float4 L = q.radiance[size_t(s) * f.npix + px];
-      if (isfinite(L.x) && isfinite(L.y) && isfinite(L.z)) {
-        a.x += double(L.x); a.y += double(L.y); a.z += double(L.z); a.w += 1.0;
+  const uint32_t stride = gridDim.x * blockDim.x;
+  const uint32_t rounds = (f.npix + stride - 1) / stride;  // wave-uniform trip count
+  uint32_t accepted = 0;
+  for (uint32_t r = 0; r < rounds; ++r) {
+    uint32_t px = r * stride + blockIdx.x * blockDim.x + threadIdx.x;
+    if (px < f.npix) {
+      double4 a = acc[px];
+      for (uint32_t s = 0; s < nsamples; ++s) {
+        float4 L = q.radiance[size_t(s) * f.npix + px];
+        if (isfinite(L.x) && isfinite(L.y) && isfinite(L.z)) {
+          a.x += double(L.x); a.y += double(L.y); a.z += double(L.z); a.w += 1.0;
+          ++accepted;
+        }
       }
+      acc[px] = a;
     }
-    acc[px] = a;
   }
+  for (int off = 32; off > 0; off >>= 1) accepted += __shfl_down(accepted, off);
+  if (lane_id() == 0 && accepted) atomicAdd(q.ctr + C_ACCEPTED, accepted);
 }
 
 // average_accepted_samples (camera.rs:2993-2999)
@@ -474,9 +512,9 @@ __global__ void k_intersect(Frame f, Queues q, const float4* rays, size_t n, flo
 uint32_t occupancy_blocks(int kernel, uint32_t block, size_t lds) {
   int nb = 0;
   hipError_t e = hipSuccess;
-  if (kernel == 0) e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_extend, int(block), lds);
+  if (kernel == 0) e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_extend<false>, int(block), lds);
   else if (kernel == 1) e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_shade, int(block), 0);
-  else e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_connect, int(block), lds);
+  else e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_connect<false>, int(block), lds);
   if (e != hipSuccess || nb <= 0) return 1;
   return uint32_t(nb);
 }
@@ -487,16 +525,24 @@ void reset_counters(const Launch& l, uint32_t* ctr, uint32_t n) {
 void raygen(const Launch& l, const Frame& f, const Queues& q, uint32_t sample0, uint32_t nsamples) {
   hipLaunchKernelGGL(k_raygen, dim3(l.grid), dim3(l.block), 0, l.stream, f, q, sample0, nsamples);
 }
-void extend(const Launch& l, const Frame& f, const Queues& q, uint32_t bounce) {
-  hipLaunchKernelGGL(k_extend, dim3(l.grid), dim3(l.block), stack_lds_bytes(l.block, q.stack_cap), l.stream, f, q,
-                     bounce);
+void extend(const Launch& l, const Frame& f, const Queues& q, uint32_t bounce, bool count) {
+  if (count)
+    hipLaunchKernelGGL(k_extend<true>, dim3(l.grid), dim3(l.block), stack_lds_bytes(l.block, q.stack_cap), l.stream,
+                       f, q, bounce);
+  else
+    hipLaunchKernelGGL(k_extend<false>, dim3(l.grid), dim3(l.block), stack_lds_bytes(l.block, q.stack_cap), l.stream,
+                       f, q, bounce);
 }
 void shade(const Launch& l, const Frame& f, const Queues& q, uint32_t bounce, uint32_t sample0, uint32_t last) {
   hipLaunchKernelGGL(k_shade, dim3(l.grid), dim3(l.block), 0, l.stream, f, q, bounce, sample0, last);
 }
-void connect(const Launch& l, const Frame& f, const Queues& q, uint32_t bounce, uint32_t kind) {
-  hipLaunchKernelGGL(k_connect, dim3(l.grid), dim3(l.block), stack_lds_bytes(l.block, q.stack_cap), l.stream, f, q,
-                     bounce, kind);
+void connect(const Launch& l, const Frame& f, const Queues& q, uint32_t bounce, uint32_t kind, bool count) {
+  if (count)
+    hipLaunchKernelGGL(k_connect<true>, dim3(l.grid), dim3(l.block), stack_lds_bytes(l.block, q.stack_cap), l.stream,
+                       f, q, bounce, kind);
+  else
+    hipLaunchKernelGGL(k_connect<false>, dim3(l.grid), dim3(l.block), stack_lds_bytes(l.block, q.stack_cap), l.stream,
+                       f, q, bounce, kind);
 }
 void accumulate(const Launch& l, const Frame& f, const Queues& q, double4* acc, uint32_t nsamples) {
   hipLaunchKernelGGL(k_accumulate, dim3(l.grid), dim3(l.block), 0, l.stream, f, q, acc, nsamples);

@@ -93,6 +93,10 @@ int gt_context_destroy(gt_context* ctx);
 int gt_context_synchronize(gt_context* ctx);
 /* The context's hipStream_t, as an opaque pointer. */
 void* gt_context_stream(gt_context* ctx);
+/* Device memory on the context's device, for the *_device entry points. */
+int gt_device_malloc(gt_context* ctx, size_t bytes, void** out);
+int gt_device_free(gt_context* ctx, void* ptr);
+int gt_memcpy_to_host(gt_context* ctx, void* dst, const void* src, size_t bytes);
 /* Thread-local message of the last failure ("" when none). */
 const char* gt_last_error(void);
 const char* gt_version(void);
@@ -104,6 +108,7 @@ typedef struct gt_tuning {
   uint64_t max_paths;       /* wavefront queue capacity in paths; 0 = automatic */
   uint32_t bvh_width;       /* 2 or 4 */
   uint32_t use_graph;       /* capture the per-batch bounce loop in a hipGraph */
+  uint32_t count_traversal; /* count node fetches and primitive tests (gt_render_stats) */
 } gt_tuning;
 int gt_context_get_tuning(gt_context* ctx, gt_tuning* out);
 int gt_context_set_tuning(gt_context* ctx, const gt_tuning* tuning);
@@ -213,6 +218,9 @@ typedef struct gt_render_stats {
   double extend_ms;           /* summed duration of the extend kernel launches */
   uint64_t extend_launches;
   double shade_ms, connect_ms;
+  /* with gt_tuning.count_traversal: BVH nodes fetched and primitives tested */
+  uint64_t ext_node_fetches, ext_prim_tests;
+  uint64_t shadow_node_fetches, shadow_prim_tests;
 } gt_render_stats;
 int gt_render_hdr(gt_context* ctx, const gt_scene* scene, const gt_camera* camera,
                   const gt_lights* lights, const gt_env* env, float* out_rgb,
