@@ -1452,6 +1452,13 @@ double cr_ggx_weight(const double* n, const double* o, const double* i, double r
   return ggx_weight(V3(n[0], n[1], n[2]), V3(o[0], o[1], o[2]), V3(i[0], i[1], i[2]), r, f0);
 }
 double cr_power_heuristic(double a, double b) { return power_heuristic(a, b); }
+// RayTraversal::hit_bounds (bvh.rs:119-146): returns 1 and the entry t, or 0.
+int cr_hit_bounds(const double* box6, const double* ray7, double tmin, double tmax, double* entry) {
+  Ray r{V3(ray7[0], ray7[1], ray7[2]), V3(ray7[4], ray7[5], ray7[6]), ray7[3]};
+  Aabb b{V3(box6[0], box6[1], box6[2]), V3(box6[3], box6[4], box6[5])};
+  Traversal tr(r);
+  return tr.hit_bounds(b, tmin, tmax, entry) ? 1 : 0;
+}
 // Russian roulette on one attenuation (camera.rs:2498-2523). Returns survival, updates att.
 int cr_russian_roulette(uint32_t bounce, int32_t min_depth, double* att, uint64_t seed) {
   Rng rng(seed);

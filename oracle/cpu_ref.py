@@ -66,6 +66,7 @@ _PROTOS = {
     "cr_schlick": (C.c_double, [C.c_double, C.c_double]),
     "cr_ggx_weight": (C.c_double, [D, D, D, C.c_double, C.c_double]),
     "cr_power_heuristic": (C.c_double, [C.c_double, C.c_double]),
+    "cr_hit_bounds": (C.c_int, [D, D, C.c_double, C.c_double, D]),
     "cr_russian_roulette": (C.c_int, [C.c_uint32, C.c_int32, D, C.c_uint64]),
     "cr_camera_frame": (C.c_int, [CAM, D, U32]),
     "cr_camera_ray": (C.c_int, [CAM, C.c_uint32, C.c_uint32, C.c_uint32, D]),
