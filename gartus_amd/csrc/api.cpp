@@ -60,9 +60,10 @@ struct gt_context {
   hipStream_t stream = nullptr;
   uint32_t cus = 0;
   gt_tuning tuning{};
-  // wavefront state
-  DevBuf<float4> q_o[2], q_d[2], q_t[2], s_o[2], s_d[2], s_w[2], radiance;
+  // wavefront state: per-path records and path-id queues
+  DevBuf<float4> ray_o, ray_d, thr, s_o[2], s_d[2], s_w[2], radiance;
   DevBuf<float2> hits;
+  DevBuf<uint32_t> active[2], cat[gtk::kNumCategories], shadow[2];
   DevBuf<uint32_t> ctr;
   DevBuf<uint2> gstack;
   DevBuf<double4> acc;
@@ -111,6 +112,7 @@ struct gt_env {
   bool uniform = true;
   DevBuf<float4> tex;
   DevBuf<float> row_cdf, col_cdf, pdf;
+  DevBuf<uint32_t> row_guide, col_guide;
 };
 
 namespace {
@@ -150,6 +152,7 @@ int upload_scene(gt_scene* s) {
 
 int build_scene(gt_scene* s, uint32_t leaf_size) {
   s->leaf_size = leaf_size == 0 ? 1 : leaf_size;
+  for (gt::HostPrim& p : s->prims) p.mat_kind = uint32_t(s->mats[p.material].kind);
   s->packed = gt::PackedScene();
   if (!s->prims.empty()) {
     gt::build_binary_bvh(s->prims, s->leaf_size, &s->bvh);
@@ -348,12 +351,14 @@ uint32_t grid_for(gt_context* ctx) {
 
 int ensure_queues(gt_context* ctx, size_t paths, uint64_t depth_needed, uint32_t max_depth) {
   if (paths > ctx->cap_paths) {
+    GT_HIP(ctx->ray_o.ensure(paths)); GT_HIP(ctx->ray_d.ensure(paths)); GT_HIP(ctx->thr.ensure(paths));
     for (int k = 0; k < 2; ++k) {
-      GT_HIP(ctx->q_o[k].ensure(paths)); GT_HIP(ctx->q_d[k].ensure(paths)); GT_HIP(ctx->q_t[k].ensure(paths));
       GT_HIP(ctx->s_o[k].ensure(paths)); GT_HIP(ctx->s_d[k].ensure(paths)); GT_HIP(ctx->s_w[k].ensure(paths));
+      GT_HIP(ctx->active[k].ensure(paths)); GT_HIP(ctx->shadow[k].ensure(paths));
     }
     GT_HIP(ctx->radiance.ensure(paths));
     GT_HIP(ctx->hits.ensure(paths));
+    for (int c = 0; c < gtk::kNumCategories; ++c) GT_HIP(ctx->cat[c].ensure(paths));
     ctx->cap_paths = paths;
   }
   (void)depth_needed;
@@ -373,11 +378,13 @@ int ensure_gstack(gt_context* ctx, uint64_t depth_needed, uint32_t cap) {
 
 gtk::Queues queues_of(gt_context* ctx) {
   gtk::Queues q;
+  q.ray_o = ctx->ray_o.p; q.ray_d = ctx->ray_d.p; q.thr = ctx->thr.p;
   for (int k = 0; k < 2; ++k) {
-    q.q_o[k] = ctx->q_o[k].p; q.q_d[k] = ctx->q_d[k].p; q.q_t[k] = ctx->q_t[k].p;
     q.s_o[k] = ctx->s_o[k].p; q.s_d[k] = ctx->s_d[k].p; q.s_w[k] = ctx->s_w[k].p;
+    q.active[k] = ctx->active[k].p; q.shadow[k] = ctx->shadow[k].p;
   }
   q.hits = ctx->hits.p;
+  for (int c = 0; c < gtk::kNumCategories; ++c) q.cat[c] = ctx->cat[c].p;
   q.radiance = ctx->radiance.p;
   q.ctr = ctx->ctr.p;
   q.gstack = ctx->gstack.p;
@@ -401,6 +408,7 @@ int make_frame(gt_context* ctx, const gt_scene* scene, const gt_camera* camera, 
     f->env.w = env->w; f->env.h = env->h;
     f->env.uniform = env->uniform ? 1u : 0u;
     f->env.tex = env->tex.p; f->env.row_cdf = env->row_cdf.p; f->env.col_cdf = env->col_cdf.p; f->env.pdf = env->pdf.p;
+    f->env.row_guide = env->row_guide.p; f->env.col_guide = env->col_guide.p;
   }
   f->nee = camera->direct_lighting_mode == GT_LIGHTING_NEE ? 1u : 0u;
   f->has_lights = lights ? 1u : 0u;
@@ -434,7 +442,7 @@ int render_impl(gt_context* ctx, const gt_scene* cscene, const gt_camera* camera
   const uint32_t spp = f.cam.spp;
   const uint32_t depth = f.cam.max_depth;
   if (depth > 4096) return fail(GT_EINVAL, "max_depth above 4096 is not supported");
-  uint64_t max_paths = ctx->tuning.max_paths ? ctx->tuning.max_paths : (uint64_t(1) << 23);
+  uint64_t max_paths = ctx->tuning.max_paths ? ctx->tuning.max_paths : (uint64_t(1) << 25);
   max_paths = std::min<uint64_t>(max_paths, uint64_t(1) << 31);
   if (uint64_t(npix) > max_paths) max_paths = npix;
   if (uint64_t(npix) >= (uint64_t(1) << 31)) return fail(GT_EINVAL, "image too large (2^31 pixels)");
@@ -729,6 +737,7 @@ static int add_prims(gt_scene* s, int kind, int stride, const double* data, size
     if (kind == GK_SPHERE) p.q[3] = std::fmax(p.q[3], 0.0);          // geometry.rs:445-450
     if (kind == GK_MOVING_SPHERE) p.q[6] = std::fmax(p.q[6], 0.0);   // geometry.rs:498-504
     p.material = mids[i];
+    p.mat_kind = 0;
     s->prims.push_back(p);
   }
   s->built = false;  // invalidate_bvh (scene.rs:1575-1577)
@@ -951,7 +960,24 @@ int gt_env_create_rgb32f(gt_context* ctx, const float* rgb, uint32_t w, uint32_t
     }
   }
   row_cdf[h - 1] = 1.0f;
+  // guide tables over the float CDFs, compared exactly as the device compares
+  auto guide = [](const float* cdf, uint32_t n, uint32_t g, uint32_t* out) {
+    uint32_t i = 0;
+    for (uint32_t k = 0; k <= g; ++k) {
+      float level = float(k) / float(g);
+      while (i < n && cdf[i] < level) ++i;
+      out[k] = i;
+    }
+  };
+  std::vector<uint32_t> row_guide(gtd::kEnvRowGuide + 1), col_guide(size_t(h) * (gtd::kEnvColGuide + 1));
+  guide(row_cdf.data(), h, gtd::kEnvRowGuide, row_guide.data());
+  for (uint32_t y = 0; y < h; ++y)
+    guide(col_cdf.data() + size_t(y) * w, w, gtd::kEnvColGuide, col_guide.data() + size_t(y) * (gtd::kEnvColGuide + 1));
   GT_HIP(hipSetDevice(ctx->device));
+  GT_HIP(e->row_guide.ensure(row_guide.size()));
+  GT_HIP(e->col_guide.ensure(col_guide.size()));
+  GT_HIP(hipMemcpy(e->row_guide.p, row_guide.data(), row_guide.size() * 4, hipMemcpyHostToDevice));
+  GT_HIP(hipMemcpy(e->col_guide.p, col_guide.data(), col_guide.size() * 4, hipMemcpyHostToDevice));
   GT_HIP(e->tex.ensure(n)); GT_HIP(e->row_cdf.ensure(h)); GT_HIP(e->col_cdf.ensure(n)); GT_HIP(e->pdf.ensure(n));
   GT_HIP(hipMemcpy(e->tex.p, tex.data(), n * sizeof(float4), hipMemcpyHostToDevice));
   GT_HIP(hipMemcpy(e->row_cdf.p, row_cdf.data(), h * sizeof(float), hipMemcpyHostToDevice));

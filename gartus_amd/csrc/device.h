@@ -381,11 +381,11 @@ __device__ __forceinline__ bool slab(const RayPre& r, float lox, float hix, floa
 template <bool ANY, bool STATS>
 __device__ __forceinline__ bool trace2(const DScene& sc, f3 o, f3 d, float time, float tmin, float tmax,
                                        Stack& st, float* t_out, uint32_t* prim_out, PrimHit* hit_out,
-                                       Stats* stats) {
+                                       Stats* stats, uint32_t* meta_out = nullptr) {
   RayPre rp = ray_pre(o, d);
   float closest = tmax;
   bool any = false;
-  uint32_t best = 0;
+  uint32_t best = 0, best_meta = 0;
   PrimHit bh{0.f, 0.f, 0.f};
   if (STATS) { stats->node_tests += 1; }
   float root_entry;
@@ -407,6 +407,7 @@ __device__ __forceinline__ bool trace2(const DScene& sc, f3 o, f3 d, float time,
         if (prim_test(sc.prims, k, o, d, time, tmin, closest, &h, &meta)) {
           closest = h.t;
           best = k;
+          best_meta = meta;
           bh = h;
           any = true;
           leaf_hit = true;
@@ -455,6 +456,7 @@ __device__ __forceinline__ bool trace2(const DScene& sc, f3 o, f3 d, float time,
     *t_out = closest;
     *prim_out = best;
     if (hit_out) *hit_out = bh;
+    if (meta_out) *meta_out = best_meta;
   }
   return any;
 }
@@ -463,11 +465,11 @@ __device__ __forceinline__ bool trace2(const DScene& sc, f3 o, f3 d, float time,
 template <bool ANY, bool STATS>
 __device__ __forceinline__ bool trace4(const DScene& sc, f3 o, f3 d, float time, float tmin, float tmax,
                                        Stack& st, float* t_out, uint32_t* prim_out, PrimHit* hit_out,
-                                       Stats* stats) {
+                                       Stats* stats, uint32_t* meta_out = nullptr) {
   RayPre rp = ray_pre(o, d);
   float closest = tmax;
   bool any = false;
-  uint32_t best = 0;
+  uint32_t best = 0, best_meta = 0;
   PrimHit bh{0.f, 0.f, 0.f};
   float root_entry;
   if (STATS) stats->node_tests += 1;
@@ -487,7 +489,7 @@ __device__ __forceinline__ bool trace4(const DScene& sc, f3 o, f3 d, float time,
         uint32_t meta;
         if (STATS) stats->candidates += 1;
         if (prim_test(sc.prims, k, o, d, time, tmin, closest, &h, &meta)) {
-          closest = h.t; best = k; bh = h; any = true; leaf_hit = true;
+          closest = h.t; best = k; best_meta = meta; bh = h; any = true; leaf_hit = true;
           if (ANY) return true;
         }
         if (meta & GT_LAST_BIT) break;
@@ -541,15 +543,18 @@ __device__ __forceinline__ bool trace4(const DScene& sc, f3 o, f3 d, float time,
     *t_out = closest;
     *prim_out = best;
     if (hit_out) *hit_out = bh;
+    if (meta_out) *meta_out = best_meta;
   }
   return any;
 }
 
 template <bool ANY, bool STATS>
 __device__ __forceinline__ bool trace(const DScene& sc, f3 o, f3 d, float time, float tmin, float tmax, Stack& st,
-                                      float* t_out, uint32_t* prim_out, PrimHit* hit_out, Stats* stats) {
-  if (sc.width == 4) return trace4<ANY, STATS>(sc, o, d, time, tmin, tmax, st, t_out, prim_out, hit_out, stats);
-  return trace2<ANY, STATS>(sc, o, d, time, tmin, tmax, st, t_out, prim_out, hit_out, stats);
+                                      float* t_out, uint32_t* prim_out, PrimHit* hit_out, Stats* stats,
+                                      uint32_t* meta_out = nullptr) {
+  if (sc.width == 4)
+    return trace4<ANY, STATS>(sc, o, d, time, tmin, tmax, st, t_out, prim_out, hit_out, stats, meta_out);
+  return trace2<ANY, STATS>(sc, o, d, time, tmin, tmax, st, t_out, prim_out, hit_out, stats, meta_out);
 }
 
 // ---- light targets (scene.rs:363-609, object.rs:1048-1161) --------------------
@@ -647,30 +652,27 @@ struct DEnv {
   const float* __restrict__ row_cdf;   // normalised cumulative row weight
   const float* __restrict__ col_cdf;   // per-row normalised cumulative texel weight
   const float* __restrict__ pdf;       // solid-angle pdf per texel (:136-156)
+  // Guide tables (cutpoint method, Chen & Asau 1974): guide[k] is the first
+  // index whose CDF value is >= k / G. They narrow each inversion to the
+  // entries between guide[k] and guide[k + 1] without changing its result.
+  const uint32_t* __restrict__ row_guide;  // kEnvRowGuide + 1 entries
+  const uint32_t* __restrict__ col_guide;  // h * (kEnvColGuide + 1) entries
 };
+constexpr uint32_t kEnvRowGuide = 4096;
+constexpr uint32_t kEnvColGuide = 64;
 __device__ __forceinline__ float rem_euclid(float x, float m) {
   float r = fmodf(x, m);
   return r < 0.f ? r + fabsf(m) : r;
 }
-__device__ __forceinline__ f3 env_radiance(const DEnv& e, f3 dir) {  // :98-107, texture.rs:681-711
+__device__ __forceinline__ f3 env_radiance_uv(const DEnv& e, float u, float v);
+__device__ __forceinline__ f3 env_radiance(const DEnv& e, f3 dir) {  // :98-107
   if (e.kind == 1) return ld3(e.color);
   f3 d = normalized(dir);
   float phi = rem_euclid(atan2f(d.z, d.x), TAU);
   float theta = acosf(clampf(d.y, -1.f, 1.f));
   float u = phi / TAU, v = 1.0f - theta / PI;
   if (!isfinite(u) || !isfinite(v)) return mk(0.f, 0.f, 0.f);
-  float cx = rem_euclid(u, 1.0f) * float(e.w);
-  uint32_t x0 = min(uint32_t(floorf(cx)), e.w - 1);
-  uint32_t x1 = (x0 + 1 == e.w) ? 0 : x0 + 1;
-  float cy = clampf(1.0f - v, 0.f, 1.f) * float(e.h - 1);
-  uint32_t y0 = uint32_t(floorf(cy));
-  uint32_t y1 = min(y0 + 1, e.h - 1);
-  float tx = cx - float(x0), ty = cy - float(y0);
-  float4 a = e.tex[size_t(y0) * e.w + x0], b = e.tex[size_t(y0) * e.w + x1];
-  float4 c = e.tex[size_t(y1) * e.w + x0], f = e.tex[size_t(y1) * e.w + x1];
-  f3 top = mk(a.x + (b.x - a.x) * tx, a.y + (b.y - a.y) * tx, a.z + (b.z - a.z) * tx);
-  f3 bot = mk(c.x + (f.x - c.x) * tx, c.y + (f.y - c.y) * tx, c.z + (f.z - c.z) * tx);
-  return top + (bot - top) * ty;
+  return env_radiance_uv(e, u, v);
 }
 __device__ __forceinline__ float env_pdf(const DEnv& e, f3 dir) {  // :136-156
   if (e.kind != 2 || e.uniform) return 1.0f / (4.0f * PI);
@@ -692,20 +694,58 @@ __device__ __forceinline__ uint32_t lower_bound(const float* __restrict__ cdf, u
   }
   return lo;
 }
-__device__ __forceinline__ void env_sample(const DEnv& e, Rng& rng, f3* dir, float* pdf) {  // :109-132
+// lower_bound over cdf[0, n) for `target` in [0, 1), narrowed by a guide table
+// of g + 1 entries; returns the same index as the full search.
+__device__ __forceinline__ uint32_t guided_search(const float* __restrict__ cdf, uint32_t n,
+                                                  const uint32_t* __restrict__ guide, uint32_t g, float target) {
+  uint32_t k = min(uint32_t(target * float(g)), g - 1);
+  uint32_t lo = guide[k];
+  uint32_t hi = min(guide[k + 1], n - 1);
+  if (lo >= hi) return min(lo, n - 1);
+  return lo + lower_bound(cdf + lo, hi - lo, target);
+}
+// The sampled texel's radiance and pdf. The reference evaluates
+// pdf_value(direction) and radiance(direction) from the returned direction
+// (environment.rs:130-131); the direction lies in texel (x, y) by
+// construction, so the texel's pdf and the lat-long coordinates
+// u = (x + r) / w, theta = acos(cos theta) are used directly.
+struct EnvSample {
+  f3 dir;
+  float pdf;
+  float u, cos_theta;
+  bool texel;
+};
+__device__ __forceinline__ f3 env_radiance_uv(const DEnv& e, float u, float v) {  // texture.rs:681-711
+  float cx = rem_euclid(u, 1.0f) * float(e.w);
+  uint32_t x0 = min(uint32_t(floorf(cx)), e.w - 1);
+  uint32_t x1 = (x0 + 1 == e.w) ? 0 : x0 + 1;
+  float cy = clampf(1.0f - v, 0.f, 1.f) * float(e.h - 1);
+  uint32_t y0 = min(uint32_t(floorf(cy)), e.h - 1);
+  uint32_t y1 = min(y0 + 1, e.h - 1);
+  float tx = cx - float(x0), ty = cy - float(y0);
+  float4 a = e.tex[size_t(y0) * e.w + x0], b = e.tex[size_t(y0) * e.w + x1];
+  float4 c = e.tex[size_t(y1) * e.w + x0], f = e.tex[size_t(y1) * e.w + x1];
+  f3 top = mk(a.x + (b.x - a.x) * tx, a.y + (b.y - a.y) * tx, a.z + (b.z - a.z) * tx);
+  f3 bot = mk(c.x + (f.x - c.x) * tx, c.y + (f.y - c.y) * tx, c.z + (f.z - c.z) * tx);
+  return top + (bot - top) * ty;
+}
+__device__ __forceinline__ EnvSample env_sample(const DEnv& e, Rng& rng) {  // :109-132
+  EnvSample r;
   if (e.kind != 2 || e.uniform) {
-    *dir = rng.unit_vector_spherical();
-    *pdf = 1.0f / (4.0f * PI);
-    return;
+    r.dir = rng.unit_vector_spherical();
+    r.pdf = 1.0f / (4.0f * PI);
+    r.texel = false;
+    return r;
   }
   // The flat row-major CDF search of the reference, split into the row
   // marginal and the in-row conditional (the same texel for the same draw).
   float target = rng.next();
-  uint32_t y = min(lower_bound(e.row_cdf, e.h, target), e.h - 1);
+  uint32_t y = guided_search(e.row_cdf, e.h, e.row_guide, kEnvRowGuide, target);
   float r0 = y == 0 ? 0.f : e.row_cdf[y - 1];
   float span = e.row_cdf[y] - r0;
-  float local = span > 0.f ? (target - r0) / span : 0.f;
-  uint32_t x = min(lower_bound(e.col_cdf + size_t(y) * e.w, e.w, local), e.w - 1);
+  float local = span > 0.f ? fminf((target - r0) / span, 0.99999994f) : 0.f;
+  uint32_t x = guided_search(e.col_cdf + size_t(y) * e.w, e.w, e.col_guide + size_t(y) * (kEnvColGuide + 1),
+                             kEnvColGuide, local);
   float u = (float(x) + rng.next()) / float(e.w);
   float t0 = PI * (float(y) / float(e.h)), t1 = PI * (float(y + 1) / float(e.h));
   float c0 = cosf(t0), c1 = cosf(t1);
@@ -714,8 +754,18 @@ __device__ __forceinline__ void env_sample(const DEnv& e, Rng& rng, f3* dir, flo
   float phi = TAU * u;
   float s, c;
   sincosf(phi, &s, &c);
-  *dir = mk(st * c, ct, st * s);
-  *pdf = env_pdf(e, *dir);
+  r.dir = mk(st * c, ct, st * s);
+  r.pdf = e.pdf[size_t(y) * e.w + x];
+  r.u = u;
+  r.cos_theta = ct;
+  r.texel = true;
+  return r;
+}
+__device__ __forceinline__ f3 env_sample_radiance(const DEnv& e, const EnvSample& s) {
+  if (e.kind == 1) return ld3(e.color);
+  if (!s.texel) return env_radiance(e, s.dir);
+  float theta = acosf(clampf(s.cos_theta, -1.f, 1.f));
+  return env_radiance_uv(e, s.u, 1.0f - theta / PI);
 }
 
 __device__ __forceinline__ float power_heuristic(float a, float b) {  // camera.rs:2486-2496

@@ -8,7 +8,7 @@
 namespace gtk {
 
 // Counters per bounce b live at ctr[b * kCtrStride + ...].
-constexpr int kCtrStride = 16;
+constexpr int kCtrStride = 32;
 enum {
   C_QCOUNT = 0,      // extension rays queued for bounce b
   C_EXT_WORK = 1,    // work cursor of extend(b)
@@ -22,17 +22,30 @@ enum {
   // k = 0 extension node fetches, 1 extension prim tests, 2 shadow node
   // fetches, 3 shadow prim tests
   C_COUNT64 = 8,
+  C_CAT_COUNT = 16,  // hits binned by shading category (kNumCategories slots)
+  C_CAT_WORK = 24,   // work cursors of the category shade kernels
 };
 
+
+// Shading categories: extend bins every hit by the BSDF it needs, so each
+// shade kernel runs one BSDF without divergence across material types.
+enum { CAT_MISS = 0, CAT_EMIT = 1, CAT_LAMBERT = 2, CAT_GGX = 3, CAT_SPECULAR = 4, kNumCategories = 5 };
+
+// Path state lives at the path's index (path = sample-in-batch * npix +
+// pixel) and is updated in place; every queue is a list of 4-byte path ids.
+// Each wave reserves its share of a queue with one atomic per work chunk.
 struct Queues {
-  float4* q_o[2];   // extension queue (double-buffered by bounce parity): origin, time
-  float4* q_d[2];   // direction, path id (bit 31: last event was a scattering event)
-  float4* q_t[2];   // throughput, miss-radiance weight
+  float4* ray_o;    // origin, time
+  float4* ray_d;    // direction, flags (bit 31: last event was a scattering event)
+  float4* thr;      // throughput, miss-radiance weight
   float2* hits;     // t, reordered primitive index (0xffffffff = miss)
-  float4* s_o[2];   // shadow queues: [0] light targets, [1] environment
+  float4* s_o[2];   // shadow rays: [0] light targets, [1] environment
   float4* s_d[2];
   float4* s_w[2];   // contribution weight
   float4* radiance; // per-path radiance L
+  uint32_t* active[2];            // paths to extend at bounce b, parity b & 1 (bounce 0: all paths)
+  uint32_t* cat[kNumCategories];  // paths hit this bounce, per shading category
+  uint32_t* shadow[2];            // paths with a light / environment shadow ray this bounce
   uint32_t* ctr;
   uint2* gstack;    // traversal-stack overflow, gstride entries per level
   uint32_t gstride;
@@ -58,6 +71,7 @@ struct Launch {
 void reset_counters(const Launch& l, uint32_t* ctr, uint32_t n);
 void raygen(const Launch& l, const Frame& f, const Queues& q, uint32_t sample0, uint32_t nsamples);
 void extend(const Launch& l, const Frame& f, const Queues& q, uint32_t bounce, bool count);
+// One bounce of shading: one kernel per category (empty categories exit at once).
 void shade(const Launch& l, const Frame& f, const Queues& q, uint32_t bounce, uint32_t sample0, uint32_t last);
 void connect(const Launch& l, const Frame& f, const Queues& q, uint32_t bounce, uint32_t kind, bool count);
 void accumulate(const Launch& l, const Frame& f, const Queues& q, double4* acc, uint32_t nsamples);
@@ -76,7 +90,7 @@ struct IntersectOut {
 void intersect(const Launch& l, const Frame& f, const Queues& q, const float4* rays, size_t n, float tmin,
                IntersectOut out);
 
-// Resident blocks per CU of a kernel: 0 extend, 1 shade, 2 connect.
+// Resident blocks per CU of a kernel: 0 extend, 1 shade (GGX), 2 connect.
 uint32_t occupancy_blocks(int kernel, uint32_t block, size_t lds_bytes);
 
 // Bytes of LDS per block for the traversal stack.

@@ -1,13 +1,17 @@
 // kernels.hip — the wavefront path tracer for gfx950 (CDNA4).
 //
 // One batch of paths (a set of samples for every pixel) runs as
-//   raygen -> { extend(b) -> shade(b) -> connect(b, light) -> connect(b, env) }
-//   for b < max_depth -> accumulate
-// over queues in HBM. Every stage is a persistent kernel: a fixed grid whose
-// waves pull 64-item chunks from a device-side cursor, so the host never reads
-// a queue length and the loop needs no synchronisation. Stages that append
-// (shade) compact their output per wave: one ballot, one popcount prefix and
-// one atomic per wave and queue.
+//   raygen -> { extend(b) -> shade(b, per category) -> connect(b, light)
+//   -> connect(b, env) } for b < max_depth -> accumulate
+// over path state in HBM. Every stage is a persistent kernel: a fixed grid
+// whose waves pull chunks of work from a device-side cursor, so the host never
+// reads a queue length and the loop needs no synchronisation.
+//
+// Path state sits at the path's index and is updated in place; queues are
+// lists of path ids. A wave appends everything it produced from one chunk
+// with a single atomic per queue: same-address atomics are served at the
+// memory side across the chip's eight XCDs, and one per wave per item cost
+// more than the traversal itself.
 //
 // The per-path integrator is RayCamera::ray_color (camera.rs:1730-1862), split
 // at its two traversals: the closest hit of the path ray (extend) and the
@@ -23,27 +27,38 @@ namespace gtk {
 namespace {
 
 constexpr uint32_t kMiss = 0xffffffffu;
-constexpr uint32_t kItemsPerLane = 2;  // items per lane per cursor fetch
+constexpr uint32_t kChunk = 4;  // items per lane per cursor fetch (256 per wave)
 
 __device__ __forceinline__ uint32_t lane_id() { return __lane_id(); }
+__device__ __forceinline__ unsigned long long lanes_below() {
+  return lane_id() == 0 ? 0ull : ((~0ull) >> (64 - lane_id()));
+}
 
-// Claims kItemsPerLane * 64 consecutive items for the calling wave.
+// Claims kChunk * 64 consecutive work items for the calling wave.
 __device__ __forceinline__ uint32_t wave_fetch(uint32_t* cursor) {
   uint32_t base = 0;
-  if (lane_id() == 0) base = atomicAdd(cursor, kItemsPerLane * 64u);
+  if (lane_id() == 0) base = atomicAdd(cursor, kChunk * 64u);
   return __shfl(base, 0);
 }
 
-// Wave-level append: returns this lane's slot, or kMiss when !pred.
-__device__ __forceinline__ uint32_t wave_append(uint32_t* counter, bool pred) {
-  unsigned long long mask = __ballot(pred);
-  if (mask == 0) return kMiss;
-  uint32_t leader = uint32_t(__ffsll((long long)mask) - 1);
+// Appends the ids of the items of one chunk whose bit is set, one atomic per
+// wave: bit k of `bits` marks item k of this lane, `ids[k]` its path id.
+__device__ __forceinline__ void wave_append_chunk(uint32_t* counter, uint32_t* list, uint32_t bits,
+                                                  const uint32_t (&ids)[kChunk]) {
+  uint32_t total = 0;
+#pragma unroll
+  for (uint32_t k = 0; k < kChunk; ++k) total += uint32_t(__popcll(__ballot((bits >> k) & 1u)));
+  if (total == 0) return;
   uint32_t base = 0;
-  if (lane_id() == leader) base = atomicAdd(counter, uint32_t(__popcll(mask)));
-  base = __shfl(base, leader);
-  unsigned long long below = lane_id() == 0 ? 0ull : (mask & ((~0ull) >> (64 - lane_id())));
-  return pred ? base + uint32_t(__popcll(below)) : kMiss;
+  if (lane_id() == 0) base = atomicAdd(counter, total);
+  base = __shfl(base, 0);
+  const unsigned long long below = lanes_below();
+#pragma unroll
+  for (uint32_t k = 0; k < kChunk; ++k) {
+    unsigned long long m = __ballot((bits >> k) & 1u);
+    if ((bits >> k) & 1u) list[base + uint32_t(__popcll(m & below))] = ids[k];
+    base += uint32_t(__popcll(m));
+  }
 }
 
 __device__ __forceinline__ Stack make_stack(const Queues& q, uint2* smem) {
@@ -96,25 +111,6 @@ __global__ void k_reset(uint32_t* ctr, uint32_t n) {
   for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) ctr[i] = 0;
 }
 
-// Camera rays for samples [sample0, sample0 + nsamples) of every pixel
-// (render_world_linear_pixel_with_context camera.rs:2660-2723; rays :1658-1718).
-__global__ void k_raygen(Frame f, Queues q, uint32_t sample0, uint32_t nsamples) {
-  const uint32_t total = f.npix * nsamples;
-  if (blockIdx.x == 0 && threadIdx.x == 0) q.ctr[C_QCOUNT] = total;
-  for (uint32_t p = blockIdx.x * blockDim.x + threadIdx.x; p < total; p += gridDim.x * blockDim.x) {
-    uint32_t x, y, sample;
-    path_coords(f, p, sample0, &x, &y, &sample);
-    Rng rng{stream_seed(pixel_seed(f.cam.seed, x, y), sample, 0)};
-    f3 o, d;
-    float time;
-    camera_ray(f.cam, x, y, sample, rng, &o, &d, &time);
-    q.q_o[0][p] = make_float4(o.x, o.y, o.z, time);
-    q.q_d[0][p] = make_float4(d.x, d.y, d.z, __uint_as_float(p));
-    q.q_t[0][p] = make_float4(1.f, 1.f, 1.f, 1.f);
-    q.radiance[p] = make_float4(0.f, 0.f, 0.f, 0.f);
-  }
-}
-
 // Wave-reduced 64-bit add of per-lane counts (one atomic per wave).
 __device__ __forceinline__ void wave_count(unsigned long long* dst, uint32_t v) {
   for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off);
@@ -128,30 +124,73 @@ __device__ __forceinline__ uint32_t node_fetches(const Stats& s, uint32_t width)
   return (s.node_tests - 1) / width;
 }
 
-// Closest hit of every queued extension ray.
+// Camera rays for samples [sample0, sample0 + nsamples) of every pixel
+// (render_world_linear_pixel_with_context camera.rs:2660-2723; rays :1658-1718).
+__global__ void k_raygen(Frame f, Queues q, uint32_t sample0, uint32_t nsamples) {
+  const uint32_t total = f.npix * nsamples;
+  if (blockIdx.x == 0 && threadIdx.x == 0) q.ctr[C_QCOUNT] = total;
+  for (uint32_t p = blockIdx.x * blockDim.x + threadIdx.x; p < total; p += gridDim.x * blockDim.x) {
+    uint32_t x, y, sample;
+    path_coords(f, p, sample0, &x, &y, &sample);
+    Rng rng{stream_seed(pixel_seed(f.cam.seed, x, y), sample, 0)};
+    f3 o, d;
+    float time;
+    camera_ray(f.cam, x, y, sample, rng, &o, &d, &time);
+    q.ray_o[p] = make_float4(o.x, o.y, o.z, time);
+    q.ray_d[p] = make_float4(d.x, d.y, d.z, 0.f);
+    q.thr[p] = make_float4(1.f, 1.f, 1.f, 1.f);
+    q.radiance[p] = make_float4(0.f, 0.f, 0.f, 0.f);
+  }
+}
+
+// Closest hit of every active path; each hit is binned by the shading
+// category of its material (misses in CAT_MISS).
+__device__ __forceinline__ uint32_t category_of(uint32_t meta) {
+  switch ((meta >> GT_MATKIND_SHIFT) & 7u) {
+    case GM_DIFFUSE_LIGHT: return CAT_EMIT;
+    case GM_LAMBERTIAN: return CAT_LAMBERT;
+    case GM_GGX: return CAT_GGX;
+    default: return CAT_SPECULAR;
+  }
+}
+
 template <bool COUNT>
 __global__ void k_extend(Frame f, Queues q, uint32_t bounce) {
   extern __shared__ uint2 smem[];
   Stack st = make_stack(q, smem);
-  const uint32_t count = q.ctr[bounce * kCtrStride + C_QCOUNT];
-  const float4* __restrict__ qo = q.q_o[bounce & 1];
-  const float4* __restrict__ qd = q.q_d[bounce & 1];
-  uint32_t* cursor = q.ctr + bounce * kCtrStride + C_EXT_WORK;
+  uint32_t* const ctr = q.ctr + bounce * kCtrStride;
+  const uint32_t count = ctr[C_QCOUNT];
+  const uint32_t* __restrict__ active = q.active[bounce & 1];
   for (;;) {
-    uint32_t base = wave_fetch(cursor);
+    uint32_t base = wave_fetch(ctr + C_EXT_WORK);
     if (base >= count) break;
     uint32_t nodes = 0, prims = 0;
-    for (uint32_t it = 0; it < kItemsPerLane; ++it) {
-      uint32_t i = base + it * 64u + lane_id();
-      if (i >= count) break;
-      float4 o4 = qo[i], d4 = qd[i];
-      float t;
-      uint32_t prim;
-      Stats s{0, 0, 0, 0, 0, 0};
-      bool hit = trace<false, COUNT>(f.scene, mk(o4.x, o4.y, o4.z), mk(d4.x, d4.y, d4.z), o4.w, SHADOW_ACNE, INF, st,
-                                     &t, &prim, nullptr, &s);
-      q.hits[i] = hit ? make_float2(t, __uint_as_float(prim)) : make_float2(0.f, __uint_as_float(kMiss));
-      if (COUNT) { nodes += node_fetches(s, f.scene.width); prims += s.candidates; }
+    uint32_t ids[kChunk], cats[kChunk];
+#pragma unroll
+    for (uint32_t k = 0; k < kChunk; ++k) {
+      uint32_t j = base + k * 64u + lane_id();
+      cats[k] = kNumCategories;
+      ids[k] = 0;
+      if (j < count) {
+        uint32_t p = bounce == 0 ? j : active[j];
+        ids[k] = p;
+        float4 o4 = q.ray_o[p], d4 = q.ray_d[p];
+        float t;
+        uint32_t prim, meta = 0;
+        Stats s{0, 0, 0, 0, 0, 0};
+        bool hit = trace<false, COUNT>(f.scene, mk(o4.x, o4.y, o4.z), mk(d4.x, d4.y, d4.z), o4.w, SHADOW_ACNE, INF,
+                                       st, &t, &prim, nullptr, &s, &meta);
+        q.hits[p] = hit ? make_float2(t, __uint_as_float(prim)) : make_float2(0.f, __uint_as_float(kMiss));
+        cats[k] = hit ? category_of(meta) : uint32_t(CAT_MISS);
+        if (COUNT) { nodes += node_fetches(s, f.scene.width); prims += s.candidates; }
+      }
+    }
+#pragma unroll
+    for (uint32_t c = 0; c < kNumCategories; ++c) {
+      uint32_t bits = 0;
+#pragma unroll
+      for (uint32_t k = 0; k < kChunk; ++k) bits |= uint32_t(cats[k] == c) << k;
+      wave_append_chunk(ctr + C_CAT_COUNT + c, q.cat[c], bits, ids);
     }
     if (COUNT) {
       wave_count(count_slot(q, bounce, 0), nodes);
@@ -160,173 +199,204 @@ __global__ void k_extend(Frame f, Queues q, uint32_t bounce) {
   }
 }
 
-// One bounce of ray_color for every hit: emission, scatter, NEE sampling and
-// the continuation ray.
-__global__ void k_shade(Frame f, Queues q, uint32_t bounce, uint32_t sample0, uint32_t last) {
-  const uint32_t count = q.ctr[bounce * kCtrStride + C_QCOUNT];
-  const float4* __restrict__ qo = q.q_o[bounce & 1];
-  const float4* __restrict__ qd = q.q_d[bounce & 1];
-  const float4* __restrict__ qt = q.q_t[bounce & 1];
-  float4* oo = q.q_o[(bounce + 1) & 1];
-  float4* od = q.q_d[(bounce + 1) & 1];
-  float4* ot = q.q_t[(bounce + 1) & 1];
-  uint32_t* cursor = q.ctr + bounce * kCtrStride + C_SHADE_WORK;
-  uint32_t* next_count = q.ctr + (bounce + 1) * kCtrStride + C_QCOUNT;
-  uint32_t* sl_count = q.ctr + bounce * kCtrStride + C_SL_COUNT;
-  uint32_t* se_count = q.ctr + bounce * kCtrStride + C_SE_COUNT;
+// ---- shading: one bounce of ray_color (camera.rs:1742-1858) per category --
+struct Item {
+  uint32_t path;
+  bool after_scatter;
+  f3 o, d, T;
+  float time, mw, t;
+  uint32_t prim;
+};
+__device__ __forceinline__ Item load_item(const Queues& q, uint32_t p) {
+  Item it;
+  float4 o4 = q.ray_o[p], d4 = q.ray_d[p], t4 = q.thr[p];
+  float2 h = q.hits[p];
+  it.path = p;
+  it.after_scatter = (__float_as_uint(d4.w) >> 31) != 0;
+  it.o = mk(o4.x, o4.y, o4.z);
+  it.time = o4.w;
+  it.d = mk(d4.x, d4.y, d4.z);
+  it.T = mk(t4.x, t4.y, t4.z);
+  it.mw = t4.w;
+  it.t = h.x;
+  it.prim = __float_as_uint(h.y);
+  return it;
+}
+
+// Outputs of one shaded path, written in place at the path's index; the
+// returned bits say which queues the path joins (1 continuation, 2 light
+// shadow ray, 4 environment shadow ray).
+enum { OUT_EXT = 1, OUT_SL = 2, OUT_SE = 4 };
+
+__device__ __forceinline__ uint32_t continue_path(const Queues& q, uint32_t p, f3 pt, float time, f3 dir,
+                                                  bool scattered, f3 T, float mw) {
+  q.ray_o[p] = make_float4(pt.x, pt.y, pt.z, time);
+  q.ray_d[p] = make_float4(dir.x, dir.y, dir.z, __uint_as_float(scattered ? 0x80000000u : 0u));
+  q.thr[p] = make_float4(T.x, T.y, T.z, mw);
+  return OUT_EXT;
+}
+
+// Lambertian (material.rs:508-532) or GGX (:1583-1616) vertex: light NEE,
+// environment NEE, then the continuation sample (camera.rs:1779-1857).
+template <bool GGX>
+__device__ __forceinline__ uint32_t shade_scatter(const Frame& f, const Queues& q, const Item& it, const GMat& m, f3 p,
+                                                  f3 n, Rng& rng, bool last, uint32_t bounce) {
   const bool has_env = f.env.kind != 0;
-  for (;;) {
-    uint32_t base = wave_fetch(cursor);
-    if (base >= count) break;
-    for (uint32_t it = 0; it < kItemsPerLane; ++it) {
-      uint32_t i = base + it * 64u + lane_id();
-      bool push_ext = false, push_sl = false, push_se = false;
-      float4 n_o, n_d, n_t, l_o, l_d, l_w, e_o, e_d, e_w;
-      if (i < count) {
-        float4 o4 = qo[i], d4 = qd[i], t4 = qt[i];
-        float2 hit = q.hits[i];
-        uint32_t pbits = __float_as_uint(d4.w);
-        uint32_t path = pbits & 0x7fffffffu;
-        bool after_scatter = (pbits >> 31) != 0;
-        f3 o = mk(o4.x, o4.y, o4.z), d = mk(d4.x, d4.y, d4.z);
-        float time = o4.w;
-        f3 T = mk(t4.x, t4.y, t4.z);
-        float mw = t4.w;
-        uint32_t x, y, sample;
-        path_coords(f, path, sample0, &x, &y, &sample);
-        Rng rng{stream_seed(pixel_seed(f.cam.seed, x, y), sample, 1 + bounce)};
-        float4 L4 = q.radiance[path];
-        f3 L = mk(L4.x, L4.y, L4.z);
-        uint32_t prim = __float_as_uint(hit.y);
-        if (prim == kMiss) {
-          f3 miss = (has_env ? env_radiance(f.env, d) : background(f.cam, d)) * mw;
-          L = L + mul(T, miss);
-        } else {
-          f3 p = o + hit.x * d;
-          uint32_t mid, kind;
-          f3 outward = prim_outward_normal(f.scene.prims, prim, p, time, &mid, &kind);
-          bool front = dot(d, outward) < 0.f;  // object.rs:127-149
-          f3 n = front ? outward : -outward;
-          const GMat m = f.scene.mats[mid];
-          f3 color = ld3(m.color);
-          bool allow_emitted = !(f.nee && f.has_lights && after_scatter);
-          if (allow_emitted && m.kind == GM_DIFFUSE_LIGHT && front) L = L + mul(T, color);
-          if (m.kind == GM_METAL || m.kind == GM_DIELECTRIC) {
-            f3 dir;
-            bool ok = true;
-            if (m.kind == GM_METAL) {  // material.rs:2087-2107
-              f3 refl = reflected(normalized(d), n);
-              dir = refl + m.param * rng.unit_vector_spherical();
-              ok = dot(dir, n) > 0.f;
-              T = mul(T, color);
-            } else {  // material.rs:2321-2350
-              float ratio = front ? 1.0f / m.param : m.param;
-              f3 unit = normalized(d);
-              float ct = fminf(dot(-unit, n), 1.0f);
-              float stt = sqrtf(1.0f - ct * ct);
-              bool cannot = ratio * stt > 1.0f;
-              float r0 = (1.0f - ratio) / (1.0f + ratio);
-              r0 = r0 * r0;
-              bool refl = cannot || (r0 + (1.0f - r0) * pow5(1.0f - ct)) > rng.next();
-              dir = refl ? reflected(unit, n) : refracted(unit, n, ratio);
-            }
-            if (ok && !last && rr_survives(bounce, f.cam.rr_min_depth, &T, rng)) {
-              push_ext = true;
-              n_o = make_float4(p.x, p.y, p.z, time);
-              n_d = make_float4(dir.x, dir.y, dir.z, __uint_as_float(path));
-              n_t = make_float4(T.x, T.y, T.z, 1.0f);
-            }
-          } else if (m.kind == GM_LAMBERTIAN || m.kind == GM_GGX) {
-            MatPdf pdf;
-            bool ok;
-            if (m.kind == GM_LAMBERTIAN) {  // material.rs:508-520
-              pdf.ggx = false;
-              ok = onb_from_w(n, &pdf.basis);
-            } else {  // material.rs:1583-1594, sampling.rs:183-202
-              pdf.ggx = true;
-              ok = onb_from_w(n, &pdf.basis);
-              f3 outgoing = normalized(-normalized(d));
-              ok = ok && len2(outgoing) > EPS && dot(outgoing, pdf.basis.w) > 0.f;
-              pdf.outgoing = outgoing;
-              pdf.roughness = m.param;
-              pdf.alpha = ggx_alpha(m.param);
-            }
-            if (ok) {
-              if (f.nee) {
-                if (f.has_lights) {  // estimate_direct_lighting camera.rs:2225-2262
-                  f3 ldir = lights_random(f.lights, p, time, rng);
-                  float lpdf = lights_pdf(f.lights, p, time, ldir);
-                  if (isfinite(lpdf) && lpdf > EPS) {
-                    float spdf = scattering_pdf(m, d, n, ldir);
-                    if (isfinite(spdf) && spdf > 0.f) {
-                      f3 w = mul(T, color) * (spdf / lpdf);
-                      push_sl = true;
-                      l_o = make_float4(p.x, p.y, p.z, time);
-                      l_d = make_float4(ldir.x, ldir.y, ldir.z, __uint_as_float(path));
-                      l_w = make_float4(w.x, w.y, w.z, 0.f);
-                    }
-                  }
-                }
-                if (has_env) {  // estimate_environment_lighting camera.rs:2264-2297
-                  f3 edir;
-                  float epdf;
-                  env_sample(f.env, rng, &edir, &epdf);
-                  if (isfinite(epdf) && epdf > EPS) {
-                    float spdf = scattering_pdf(m, d, n, edir);
-                    if (isfinite(spdf) && spdf > 0.f) {
-                      float mis = power_heuristic(epdf, spdf);
-                      f3 w = mul(T, mul(color, env_radiance(f.env, edir)) * (mis * spdf / epdf));
-                      push_se = true;
-                      e_o = make_float4(p.x, p.y, p.z, time);
-                      e_d = make_float4(edir.x, edir.y, edir.z, __uint_as_float(path));
-                      e_w = make_float4(w.x, w.y, w.z, 0.f);
-                    }
-                  }
-                }
-              }
-              if (!last) {
-                // SamplingStrategy::continuation_sample camera.rs:165-195
-                f3 dir;
-                float cpdf;
-                bool suppress, weight_env;
-                if (!f.nee && f.has_lights) {
-                  float wgt = f.light_weight;
-                  if (rng.next() < wgt) dir = lights_random(f.lights, p, time, rng);
-                  else dir = pdf.generate(rng);
-                  cpdf = wgt * lights_pdf(f.lights, p, time, dir) + (1.0f - wgt) * pdf.value(dir);
-                  suppress = false;
-                  weight_env = false;
-                } else {
-                  dir = pdf.generate(rng);
-                  cpdf = pdf.value(dir);
-                  suppress = f.nee && f.has_lights;
-                  weight_env = f.nee && has_env;
-                }
-                if (isfinite(cpdf) && cpdf > EPS) {
-                  float spdf = scattering_pdf(m, d, n, dir);
-                  if (isfinite(spdf) && spdf > 0.f) {
-                    float next_w = weight_env ? power_heuristic(cpdf, env_pdf(f.env, dir)) : 1.0f;
-                    T = mul(T, color * (spdf / cpdf));
-                    if (rr_survives(bounce, f.cam.rr_min_depth, &T, rng)) {
-                      push_ext = true;
-                      n_o = make_float4(p.x, p.y, p.z, time);
-                      n_d = make_float4(dir.x, dir.y, dir.z, __uint_as_float(path | (suppress ? 0x80000000u : 0u)));
-                      n_t = make_float4(T.x, T.y, T.z, next_w);
-                    }
-                  }
-                }
-              }
-            }
-          }
+  uint32_t out = 0;
+  MatPdf pdf;
+  pdf.ggx = GGX;
+  bool ok = onb_from_w(n, &pdf.basis);
+  if (GGX) {  // GgxReflectionPdf::new (sampling.rs:183-202)
+    f3 outgoing = normalized(-normalized(it.d));
+    ok = ok && len2(outgoing) > EPS && dot(outgoing, pdf.basis.w) > 0.f;
+    pdf.outgoing = outgoing;
+    pdf.roughness = m.param;
+    pdf.alpha = ggx_alpha(m.param);
+  }
+  if (!ok) return 0;  // scatter returns None: the path ends
+  const f3 color = ld3(m.color);
+  f3 T = it.T;
+  if (f.nee) {
+    if (f.has_lights) {  // estimate_direct_lighting camera.rs:2225-2262
+      f3 ldir = lights_random(f.lights, p, it.time, rng);
+      float lpdf = lights_pdf(f.lights, p, it.time, ldir);
+      if (isfinite(lpdf) && lpdf > EPS) {
+        float spdf = scattering_pdf(m, it.d, n, ldir);
+        if (isfinite(spdf) && spdf > 0.f) {
+          f3 w = mul(T, color) * (spdf / lpdf);
+          q.s_o[0][it.path] = make_float4(p.x, p.y, p.z, it.time);
+          q.s_d[0][it.path] = make_float4(ldir.x, ldir.y, ldir.z, 0.f);
+          q.s_w[0][it.path] = make_float4(w.x, w.y, w.z, 0.f);
+          out |= OUT_SL;
         }
-        q.radiance[path] = make_float4(L.x, L.y, L.z, 0.f);
       }
-      uint32_t slot = wave_append(next_count, push_ext);
-      if (push_ext) { oo[slot] = n_o; od[slot] = n_d; ot[slot] = n_t; }
-      slot = wave_append(sl_count, push_sl);
-      if (push_sl) { q.s_o[0][slot] = l_o; q.s_d[0][slot] = l_d; q.s_w[0][slot] = l_w; }
-      slot = wave_append(se_count, push_se);
-      if (push_se) { q.s_o[1][slot] = e_o; q.s_d[1][slot] = e_d; q.s_w[1][slot] = e_w; }
+    }
+    if (has_env) {  // estimate_environment_lighting camera.rs:2264-2297
+      EnvSample es = env_sample(f.env, rng);
+      if (isfinite(es.pdf) && es.pdf > EPS) {
+        float spdf = scattering_pdf(m, it.d, n, es.dir);
+        if (isfinite(spdf) && spdf > 0.f) {
+          float mis = power_heuristic(es.pdf, spdf);
+          f3 w = mul(T, mul(color, env_sample_radiance(f.env, es)) * (mis * spdf / es.pdf));
+          q.s_o[1][it.path] = make_float4(p.x, p.y, p.z, it.time);
+          q.s_d[1][it.path] = make_float4(es.dir.x, es.dir.y, es.dir.z, 0.f);
+          q.s_w[1][it.path] = make_float4(w.x, w.y, w.z, 0.f);
+          out |= OUT_SE;
+        }
+      }
+    }
+  }
+  if (last) return out;  // the bounce loop ends here; the continuation is never traced
+  // SamplingStrategy::continuation_sample camera.rs:165-195
+  f3 dir;
+  float cpdf;
+  bool suppress, weight_env;
+  if (!f.nee && f.has_lights) {
+    float wgt = f.light_weight;
+    if (rng.next() < wgt) dir = lights_random(f.lights, p, it.time, rng);
+    else dir = pdf.generate(rng);
+    cpdf = wgt * lights_pdf(f.lights, p, it.time, dir) + (1.0f - wgt) * pdf.value(dir);
+    suppress = false;
+    weight_env = false;
+  } else {
+    dir = pdf.generate(rng);
+    cpdf = pdf.value(dir);
+    suppress = f.nee && f.has_lights;
+    weight_env = f.nee && has_env;
+  }
+  if (!(isfinite(cpdf) && cpdf > EPS)) return out;
+  float spdf = scattering_pdf(m, it.d, n, dir);
+  if (!(isfinite(spdf) && spdf > 0.f)) return out;
+  float next_w = weight_env ? power_heuristic(cpdf, env_pdf(f.env, dir)) : 1.0f;
+  T = mul(T, color * (spdf / cpdf));
+  if (rr_survives(bounce, f.cam.rr_min_depth, &T, rng))
+    out |= continue_path(q, it.path, p, it.time, dir, suppress, T, next_w);
+  return out;
+}
+
+// Metal (material.rs:2087-2107) or Dielectric (:2321-2350) vertex.
+__device__ __forceinline__ uint32_t shade_specular(const Frame& f, const Queues& q, const Item& it, const GMat& m, f3 p,
+                                                   f3 n, bool front, Rng& rng, bool last, uint32_t bounce) {
+  f3 T = it.T;
+  f3 dir;
+  if (m.kind == GM_METAL) {
+    f3 refl = reflected(normalized(it.d), n);
+    dir = refl + m.param * rng.unit_vector_spherical();
+    if (dot(dir, n) <= 0.f) return 0;  // absorbed
+    T = mul(T, ld3(m.color));
+  } else {
+    float ratio = front ? 1.0f / m.param : m.param;
+    f3 unit = normalized(it.d);
+    float ct = fminf(dot(-unit, n), 1.0f);
+    float stt = sqrtf(1.0f - ct * ct);
+    bool cannot = ratio * stt > 1.0f;
+    float r0 = (1.0f - ratio) / (1.0f + ratio);
+    r0 = r0 * r0;
+    bool refl = cannot || (r0 + (1.0f - r0) * pow5(1.0f - ct)) > rng.next();
+    dir = refl ? reflected(unit, n) : refracted(unit, n, ratio);
+  }
+  if (!last && rr_survives(bounce, f.cam.rr_min_depth, &T, rng))
+    return continue_path(q, it.path, p, it.time, dir, false, T, 1.0f);
+  return 0;
+}
+
+template <int CAT>
+__global__ void k_shade(Frame f, Queues q, uint32_t bounce, uint32_t sample0, uint32_t last) {
+  uint32_t* const ctr = q.ctr + bounce * kCtrStride;
+  const uint32_t count = ctr[C_CAT_COUNT + CAT];
+  const uint32_t* __restrict__ list = q.cat[CAT];
+  const bool has_env = f.env.kind != 0;
+  constexpr bool kScatters = CAT == CAT_LAMBERT || CAT == CAT_GGX || CAT == CAT_SPECULAR;
+  for (;;) {
+    uint32_t base = wave_fetch(ctr + C_CAT_WORK + CAT);
+    if (base >= count) break;
+    uint32_t ids[kChunk], outs[kChunk];
+#pragma unroll
+    for (uint32_t k = 0; k < kChunk; ++k) {
+      uint32_t j = base + k * 64u + lane_id();
+      ids[k] = 0;
+      outs[k] = 0;
+      if (j >= count) continue;
+      const uint32_t p = list[j];
+      ids[k] = p;
+      const Item it = load_item(q, p);
+      if (CAT == CAT_MISS) {  // camera.rs:1747-1750: background or environment, times the MIS weight
+        f3 miss = (has_env ? env_radiance(f.env, it.d) : background(f.cam, it.d)) * it.mw;
+        float4 L = q.radiance[p];
+        q.radiance[p] = make_float4(L.x + it.T.x * miss.x, L.y + it.T.y * miss.y, L.z + it.T.z * miss.z, 0.f);
+        continue;
+      }
+      f3 pt = it.o + it.t * it.d;
+      uint32_t mid, kind;
+      f3 outward = prim_outward_normal(f.scene.prims, it.prim, pt, it.time, &mid, &kind);
+      bool front = dot(it.d, outward) < 0.f;  // object.rs:127-149
+      f3 n = front ? outward : -outward;
+      const GMat m = f.scene.mats[mid];
+      if (CAT == CAT_EMIT) {  // emitted_at camera.rs:1753-1755, DiffuseLight material.rs:1009-1022
+        if (!it.after_scatter && front) {
+          float4 L = q.radiance[p];
+          q.radiance[p] = make_float4(L.x + it.T.x * m.color[0], L.y + it.T.y * m.color[1],
+                                      L.z + it.T.z * m.color[2], 0.f);
+        }
+        continue;
+      }
+      uint32_t x, y, sample;
+      path_coords(f, p, sample0, &x, &y, &sample);
+      Rng rng{stream_seed(pixel_seed(f.cam.seed, x, y), sample, 1 + bounce)};
+      if (CAT == CAT_SPECULAR) outs[k] = shade_specular(f, q, it, m, pt, n, front, rng, last != 0, bounce);
+      else outs[k] = shade_scatter<CAT == CAT_GGX>(f, q, it, m, pt, n, rng, last != 0, bounce);
+    }
+    if (kScatters) {
+      uint32_t bits_ext = 0, bits_sl = 0, bits_se = 0;
+#pragma unroll
+      for (uint32_t k = 0; k < kChunk; ++k) {
+        bits_ext |= uint32_t((outs[k] & OUT_EXT) != 0) << k;
+        bits_sl |= uint32_t((outs[k] & OUT_SL) != 0) << k;
+        bits_se |= uint32_t((outs[k] & OUT_SE) != 0) << k;
+      }
+      wave_append_chunk(q.ctr + (bounce + 1) * kCtrStride + C_QCOUNT, q.active[(bounce + 1) & 1], bits_ext, ids);
+      wave_append_chunk(ctr + C_SL_COUNT, q.shadow[0], bits_sl, ids);
+      wave_append_chunk(ctr + C_SE_COUNT, q.shadow[1], bits_se, ids);
     }
   }
 }
@@ -338,8 +408,10 @@ template <bool COUNT>
 __global__ void k_connect(Frame f, Queues q, uint32_t bounce, uint32_t kind) {
   extern __shared__ uint2 smem[];
   Stack st = make_stack(q, smem);
-  const uint32_t count = q.ctr[bounce * kCtrStride + (kind == 0 ? C_SL_COUNT : C_SE_COUNT)];
-  uint32_t* cursor = q.ctr + bounce * kCtrStride + (kind == 0 ? C_SL_WORK : C_SE_WORK);
+  uint32_t* const ctr = q.ctr + bounce * kCtrStride;
+  const uint32_t count = ctr[kind == 0 ? C_SL_COUNT : C_SE_COUNT];
+  uint32_t* cursor = ctr + (kind == 0 ? C_SL_WORK : C_SE_WORK);
+  const uint32_t* __restrict__ list = q.shadow[kind];
   const float4* __restrict__ so = q.s_o[kind];
   const float4* __restrict__ sd = q.s_d[kind];
   const float4* __restrict__ sw = q.s_w[kind];
@@ -347,12 +419,12 @@ __global__ void k_connect(Frame f, Queues q, uint32_t bounce, uint32_t kind) {
     uint32_t base = wave_fetch(cursor);
     if (base >= count) break;
     uint32_t nodes = 0, prims = 0;
-    for (uint32_t it = 0; it < kItemsPerLane; ++it) {
-      uint32_t i = base + it * 64u + lane_id();
-      if (i >= count) break;
+    for (uint32_t k = 0; k < kChunk; ++k) {
+      uint32_t j = base + k * 64u + lane_id();
+      if (j >= count) break;
+      const uint32_t p = list[j];
       Stats s{0, 0, 0, 0, 0, 0};
-      float4 o4 = so[i], d4 = sd[i], w4 = sw[i];
-      uint32_t path = __float_as_uint(d4.w);
+      float4 o4 = so[p], d4 = sd[p], w4 = sw[p];
       f3 o = mk(o4.x, o4.y, o4.z), d = mk(d4.x, d4.y, d4.z);
       float t;
       uint32_t prim;
@@ -360,9 +432,9 @@ __global__ void k_connect(Frame f, Queues q, uint32_t bounce, uint32_t kind) {
       bool contributes = false;
       if (kind == 0) {
         if (trace<false, COUNT>(f.scene, o, d, o4.w, SHADOW_ACNE, INF, st, &t, &prim, nullptr, &s)) {
-          f3 p = o + t * d;
+          f3 pt = o + t * d;
           uint32_t mid, pk;
-          f3 outward = prim_outward_normal(f.scene.prims, prim, p, o4.w, &mid, &pk);
+          f3 outward = prim_outward_normal(f.scene.prims, prim, pt, o4.w, &mid, &pk);
           const GMat& m = f.scene.mats[mid];
           if (m.kind == GM_DIFFUSE_LIGHT && dot(d, outward) < 0.f) {
             f3 em = ld3(m.color);
@@ -376,8 +448,8 @@ __global__ void k_connect(Frame f, Queues q, uint32_t bounce, uint32_t kind) {
         }
       }
       if (contributes) {
-        float4 L = q.radiance[path];
-        q.radiance[path] = make_float4(L.x + add.x, L.y + add.y, L.z + add.z, 0.f);
+        float4 L = q.radiance[p];
+        q.radiance[p] = make_float4(L.x + add.x, L.y + add.y, L.z + add.z, 0.f);
       }
       if (COUNT) { nodes += node_fetches(s, f.scene.width); prims += s.candidates; }
     }
@@ -513,7 +585,7 @@ uint32_t occupancy_blocks(int kernel, uint32_t block, size_t lds) {
   int nb = 0;
   hipError_t e = hipSuccess;
   if (kernel == 0) e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_extend<false>, int(block), lds);
-  else if (kernel == 1) e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_shade, int(block), 0);
+  else if (kernel == 1) e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_shade<CAT_GGX>, int(block), 0);
   else e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_connect<false>, int(block), lds);
   if (e != hipSuccess || nb <= 0) return 1;
   return uint32_t(nb);
@@ -534,7 +606,11 @@ void extend(const Launch& l, const Frame& f, const Queues& q, uint32_t bounce, b
                        f, q, bounce);
 }
 void shade(const Launch& l, const Frame& f, const Queues& q, uint32_t bounce, uint32_t sample0, uint32_t last) {
-  hipLaunchKernelGGL(k_shade, dim3(l.grid), dim3(l.block), 0, l.stream, f, q, bounce, sample0, last);
+  hipLaunchKernelGGL(k_shade<CAT_MISS>, dim3(l.grid), dim3(l.block), 0, l.stream, f, q, bounce, sample0, last);
+  hipLaunchKernelGGL(k_shade<CAT_EMIT>, dim3(l.grid), dim3(l.block), 0, l.stream, f, q, bounce, sample0, last);
+  hipLaunchKernelGGL(k_shade<CAT_LAMBERT>, dim3(l.grid), dim3(l.block), 0, l.stream, f, q, bounce, sample0, last);
+  hipLaunchKernelGGL(k_shade<CAT_GGX>, dim3(l.grid), dim3(l.block), 0, l.stream, f, q, bounce, sample0, last);
+  hipLaunchKernelGGL(k_shade<CAT_SPECULAR>, dim3(l.grid), dim3(l.block), 0, l.stream, f, q, bounce, sample0, last);
 }
 void connect(const Launch& l, const Frame& f, const Queues& q, uint32_t bounce, uint32_t kind, bool count) {
   if (count)

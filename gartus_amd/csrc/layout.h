@@ -14,7 +14,9 @@
 //   moving sphere  p0 = (c0, meta)      p1 = (c1, r)
 //   triangle       p0 = (v0, meta)      p1 = (v1-v0, 0) p2 = (v2-v0, 0) p3 = (unit normal, 0)
 //   quad           p0 = (corner, meta)  p1 = (u, 1/|u x v|) p2 = (v, 0) p3 = (unit normal, plane d)
-// meta = material | kind << 28 | last-in-leaf << 31.
+// meta = material id (25 bits) | material kind << 25 (3 bits) | primitive
+// kind << 28 (3 bits) | last-in-leaf << 31. The material kind lets the
+// traversal kernel sort hits by BSDF without another load.
 #pragma once
 #include <stdint.h>
 
@@ -22,7 +24,8 @@
 #define GT_LAST_BIT 0x80000000u
 #define GT_KIND_SHIFT 28
 #define GT_KIND_MASK 0x7u
-#define GT_MATERIAL_MASK 0x0fffffffu
+#define GT_MATKIND_SHIFT 25
+#define GT_MATERIAL_MASK 0x01ffffffu
 
 enum { GK_SPHERE = 0, GK_MOVING_SPHERE = 1, GK_TRIANGLE = 2, GK_QUAD = 3 };
 enum { GM_LAMBERTIAN = 0, GM_DIFFUSE_LIGHT = 1, GM_GGX = 2, GM_METAL = 3, GM_DIELECTRIC = 4 };

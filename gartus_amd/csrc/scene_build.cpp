@@ -216,7 +216,8 @@ GPrim pack_prim(const HostPrim& p, bool last) {
   GPrim g;
   std::memset(&g, 0, sizeof g);
   const double* q = p.q;
-  uint32_t meta = (p.material & GT_MATERIAL_MASK) | (uint32_t(p.kind) << GT_KIND_SHIFT) | (last ? GT_LAST_BIT : 0u);
+  uint32_t meta = (p.material & GT_MATERIAL_MASK) | ((p.mat_kind & 7u) << GT_MATKIND_SHIFT) |
+                  (uint32_t(p.kind) << GT_KIND_SHIFT) | (last ? GT_LAST_BIT : 0u);
   g.p0[3] = from_bits(meta);
   switch (p.kind) {
     case GK_SPHERE:

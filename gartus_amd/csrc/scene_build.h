@@ -15,6 +15,7 @@ struct HostPrim {
   int kind;          // GK_*
   double q[9];       // sphere: c, r | moving: c0, c1, r | triangle: p0, p1, p2 | quad: corner, u, v
   uint32_t material;
+  uint32_t mat_kind;   // GM_* of the material, packed into the record's meta word
 };
 
 struct HostMat {

@@ -15,6 +15,7 @@ extern "C" uint64_t shim_build(size_t n, const int* kinds, const double* q9, uin
     prims[i].kind = kinds[i];
     std::memcpy(prims[i].q, q9 + 9 * i, 9 * sizeof(double));
     prims[i].material = 0;
+    prims[i].mat_kind = 0;
   }
   gt::build_binary_bvh(prims, leaf_size, &g_bvh);
   gt::pack_scene(prims, g_bvh, &g_packed);
