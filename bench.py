@@ -31,6 +31,9 @@ ROOT = Path(__file__).resolve().parent
 sys.path.insert(0, str(ROOT))
 
 HBM_PEAK_GBS = 8000.0  # MI355X HBM3E
+# HBM bytes per extend launch measured with rocprofv3 --pmc FETCH_SIZE and
+# WRITE_SIZE, scaled by the same counters on a plain copy of known size
+TRAFFIC_FILE = ROOT / "profiles" / "extend_hbm_traffic.json"
 
 
 def log(msg: str) -> None:
@@ -98,6 +101,45 @@ def cpu_baseline(data, env_rgb, lights_gpu, cam_spec, seconds: float) -> dict:
     }
 
 
+def calibrate_copy(mib: int, reps: int) -> int:
+    """A plain device copy of known size (out = -in over `mib` MiB of
+    float32): profiled with the same counters as the extend kernel, it gives
+    the factor between counter readings and bytes moved."""
+    import torch
+    n = mib * (1 << 20) // 4
+    a = torch.ones(n, dtype=torch.float32, device="cuda")
+    b = torch.empty_like(a)
+    for _ in range(reps):
+        torch.neg(a, out=b)
+    torch.cuda.synchronize()
+    log(f"copy calibration: {reps} x neg over {mib} MiB read + {mib} MiB written")
+    return 0
+
+
+def summarize_counters(paths: list[str]) -> int:
+    """Per-kernel mean of each counter over the dispatches in rocprofv3 CSV
+    counter files (counter_collection.csv)."""
+    import csv
+    acc: dict[tuple[str, str], list[float]] = {}
+    for p in paths:
+        with open(p, newline="") as f:
+            for row in csv.DictReader(f):
+                key = (row["Kernel_Name"], row["Counter_Name"])
+                acc.setdefault(key, []).append(float(row["Counter_Value"]))
+    out = {f"{k} | {c}": {"dispatches": len(v), "mean": sum(v) / len(v)} for (k, c), v in sorted(acc.items())}
+    print(json.dumps(out, indent=1))
+    return 0
+
+
+def traffic_for(workload: str):
+    try:
+        rec = json.loads(TRAFFIC_FILE.read_text())
+    except (OSError, ValueError):
+        return None
+    w = rec.get(workload)
+    return None if w is None else w.get("bytes_per_launch")
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -109,7 +151,15 @@ def main() -> int:
     ap.add_argument("--block", type=int, default=0)
     ap.add_argument("--bvh-width", type=int, default=0)
     ap.add_argument("--max-paths", type=int, default=0)
+    ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
+                    help="run only a plain device copy of MIB MiB (counter calibration)")
+    ap.add_argument("--summarize-counters", nargs="+", metavar="CSV",
+                    help="print per-kernel counter means from rocprofv3 CSV files and exit")
     args = ap.parse_args()
+    if args.summarize_counters:
+        return summarize_counters(args.summarize_counters)
+    if args.calibrate_copy:
+        return calibrate_copy(args.calibrate_copy, max(args.steps, 1))
     if args.gpus != 1:
         print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported (use --gpus 1)", file=sys.stderr)
         return 2
@@ -210,7 +260,7 @@ def main() -> int:
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4),
-            "traffic": None,
+            "traffic": traffic_for(args.workload),
             "kernel": "k_extend (closest-hit traversal)",
             "bytes_per_launch": round(ext_bytes / launches),
             "avg_launch_ms": round(avg_launch_ms, 4),

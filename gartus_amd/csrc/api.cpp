@@ -65,9 +65,11 @@ struct gt_context {
   DevBuf<float2> hits;
   DevBuf<uint32_t> active[2], cat[gtk::kNumCategories], shadow[2];
   DevBuf<uint32_t> ctr;
+  DevBuf<unsigned long long> tally;
   DevBuf<uint2> gstack;
   DevBuf<double4> acc;
   size_t cap_paths = 0;
+  uint32_t region = 0;  // queue entries per slice
   uint32_t gstride = 0;
   uint32_t* host_ctr = nullptr;  // pinned copy of the counters, one block per batch
   size_t host_ctr_words = 0;
@@ -349,21 +351,23 @@ uint32_t grid_for(gt_context* ctx) {
   return std::max<uint32_t>(1, ctx->cus * bpc);
 }
 
-int ensure_queues(gt_context* ctx, size_t paths, uint64_t depth_needed, uint32_t max_depth) {
+int ensure_queues(gt_context* ctx, size_t paths, uint32_t max_depth) {
   if (paths > ctx->cap_paths) {
+    const size_t region = (paths + gtk::kSlices - 1) / gtk::kSlices;
+    const size_t lists = region * gtk::kSlices;
     GT_HIP(ctx->ray_o.ensure(paths)); GT_HIP(ctx->ray_d.ensure(paths)); GT_HIP(ctx->thr.ensure(paths));
     for (int k = 0; k < 2; ++k) {
       GT_HIP(ctx->s_o[k].ensure(paths)); GT_HIP(ctx->s_d[k].ensure(paths)); GT_HIP(ctx->s_w[k].ensure(paths));
-      GT_HIP(ctx->active[k].ensure(paths)); GT_HIP(ctx->shadow[k].ensure(paths));
+      GT_HIP(ctx->active[k].ensure(lists)); GT_HIP(ctx->shadow[k].ensure(lists));
     }
     GT_HIP(ctx->radiance.ensure(paths));
     GT_HIP(ctx->hits.ensure(paths));
-    for (int c = 0; c < gtk::kNumCategories; ++c) GT_HIP(ctx->cat[c].ensure(paths));
+    for (int c = 0; c < gtk::kNumCategories; ++c) GT_HIP(ctx->cat[c].ensure(lists));
     ctx->cap_paths = paths;
+    ctx->region = uint32_t(region);
   }
-  (void)depth_needed;
-  size_t words = size_t(max_depth + 2) * gtk::kCtrStride;
-  GT_HIP(ctx->ctr.ensure(words));
+  GT_HIP(ctx->ctr.ensure(size_t(max_depth + 1) * gtk::kBounceWords));
+  GT_HIP(ctx->tally.ensure(gtk::tally_words(max_depth)));
   return GT_OK;
 }
 
@@ -386,7 +390,9 @@ gtk::Queues queues_of(gt_context* ctx) {
   q.hits = ctx->hits.p;
   for (int c = 0; c < gtk::kNumCategories; ++c) q.cat[c] = ctx->cat[c].p;
   q.radiance = ctx->radiance.p;
+  q.region = ctx->region;
   q.ctr = ctx->ctr.p;
+  q.tally = ctx->tally.p;
   q.gstack = ctx->gstack.p;
   q.gstride = ctx->gstride;
   q.stack_cap = kStackCap;
@@ -450,14 +456,16 @@ int render_impl(gt_context* ctx, const gt_scene* cscene, const gt_camera* camera
   kmax = std::max<uint32_t>(kmax, 1);
   uint32_t nbatch = (spp + kmax - 1) / kmax;
   uint32_t k = (spp + nbatch - 1) / nbatch;
-  if ((rc = ensure_queues(ctx, size_t(k) * npix, 0, depth)) != GT_OK) return rc;
+  if ((rc = ensure_queues(ctx, size_t(k) * npix, depth)) != GT_OK) return rc;
   uint64_t dneed = stack_depth(scene, f.scene.width);
   if ((rc = ensure_gstack(ctx, dneed, kStackCap)) != GT_OK) return rc;
   GT_HIP(ctx->acc.ensure(npix));
   gtk::Queues q = queues_of(ctx);
   gtk::Launch L{grid_for(ctx), ctx->tuning.block_size, ctx->stream};
   hipStream_t st = ctx->stream;
-  size_t ctr_words = size_t(depth + 2) * gtk::kCtrStride;
+  const size_t ctr_words = size_t(depth + 1) * gtk::kBounceWords;
+  const size_t tally_words = gtk::tally_words(depth);
+  const size_t host_words = ctr_words + 2 * tally_words;  // counters, then the tallies as u32 pairs
   const bool want_stats = stats != nullptr;
   const bool count = want_stats && ctx->tuning.count_traversal != 0;
   const bool time_kernels = want_stats;
@@ -467,11 +475,11 @@ int render_impl(gt_context* ctx, const gt_scene* cscene, const gt_camera* camera
     GT_HIP(hipEventCreate(&e));
     ctx->events.push_back(e);
   }
-  if (want_stats && ctx->host_ctr_words < ctr_words * nbatch) {
+  if (want_stats && ctx->host_ctr_words < host_words * nbatch) {
     if (ctx->host_ctr) GT_HIP(hipHostFree(ctx->host_ctr));
     ctx->host_ctr = nullptr;
-    GT_HIP(hipHostMalloc(reinterpret_cast<void**>(&ctx->host_ctr), ctr_words * nbatch * sizeof(uint32_t)));
-    ctx->host_ctr_words = ctr_words * nbatch;
+    GT_HIP(hipHostMalloc(reinterpret_cast<void**>(&ctx->host_ctr), host_words * nbatch * sizeof(uint32_t)));
+    ctx->host_ctr_words = host_words * nbatch;
   }
   // events per bounce: [e0] extend [e1][e2] shade [e3] connect [e4]
   size_t ev = 0;
@@ -482,7 +490,7 @@ int render_impl(gt_context* ctx, const gt_scene* cscene, const gt_camera* camera
   for (uint32_t b = 0; b < nbatch; ++b) {
     uint32_t s0 = b * k;
     uint32_t ns = std::min(k, spp - s0);
-    gtk::reset_counters(L, q.ctr, uint32_t(ctr_words));
+    gtk::reset_counters(L, q, depth);
     gtk::raygen(L, f, q, s0, ns);
     for (uint32_t bounce = 0; bounce < depth; ++bounce) {
       if (time_kernels) GT_HIP(hipEventRecord(ctx->events[ev++], st));
@@ -498,8 +506,11 @@ int render_impl(gt_context* ctx, const gt_scene* cscene, const gt_camera* camera
       if (time_kernels) GT_HIP(hipEventRecord(ctx->events[ev++], st));
     }
     gtk::accumulate(L, f, q, ctx->acc.p, ns);
-    if (want_stats)
-      GT_HIP(hipMemcpyAsync(ctx->host_ctr + ctr_words * b, q.ctr, ctr_words * sizeof(uint32_t), hipMemcpyDeviceToHost, st));
+    if (want_stats) {
+      uint32_t* h = ctx->host_ctr + host_words * b;
+      GT_HIP(hipMemcpyAsync(h, q.ctr, ctr_words * sizeof(uint32_t), hipMemcpyDeviceToHost, st));
+      GT_HIP(hipMemcpyAsync(h + ctr_words, q.tally, tally_words * sizeof(uint64_t), hipMemcpyDeviceToHost, st));
+    }
   }
   if (d_out) gtk::finalize(L, ctx->acc.p, npix, d_out, out_is_double);
   if (d_albedo) gtk::aov(L, f, q, d_albedo, d_normal);
@@ -510,15 +521,18 @@ int render_impl(gt_context* ctx, const gt_scene* cscene, const gt_camera* camera
     std::memset(stats, 0, sizeof *stats);
     stats->camera_paths = uint64_t(npix) * spp;
     for (uint32_t b = 0; b < nbatch; ++b) {
-      const uint32_t* c = ctx->host_ctr + ctr_words * b;
-      stats->accepted_samples += c[gtk::C_ACCEPTED];
+      const uint32_t* c = ctx->host_ctr + host_words * b;
+      std::vector<uint64_t> tally(tally_words);
+      std::memcpy(tally.data(), c + ctr_words, tally_words * sizeof(uint64_t));
+      stats->accepted_samples += tally[0];
       for (uint32_t bounce = 0; bounce < depth; ++bounce) {
-        stats->extension_rays += c[bounce * gtk::kCtrStride + gtk::C_QCOUNT];
-        if (conn_l) stats->shadow_rays += c[bounce * gtk::kCtrStride + gtk::C_SL_COUNT];
-        if (conn_e) stats->shadow_rays += c[bounce * gtk::kCtrStride + gtk::C_SE_COUNT];
+        for (uint32_t sl = 0; sl < gtk::kSlices; ++sl) {
+          stats->extension_rays += c[gtk::ctr_at(bounce, gtk::S_QCOUNT, sl)];
+          if (conn_l) stats->shadow_rays += c[gtk::ctr_at(bounce, gtk::S_SL_COUNT, sl)];
+          if (conn_e) stats->shadow_rays += c[gtk::ctr_at(bounce, gtk::S_SE_COUNT, sl)];
+        }
         if (count) {
-          uint64_t v[4];
-          std::memcpy(v, c + bounce * gtk::kCtrStride + gtk::C_COUNT64, sizeof v);
+          const uint64_t* v = tally.data() + 1 + 4 * bounce;
           stats->ext_node_fetches += v[0]; stats->ext_prim_tests += v[1];
           stats->shadow_node_fetches += v[2]; stats->shadow_prim_tests += v[3];
         }

@@ -7,29 +7,39 @@
 
 namespace gtk {
 
-// Counters per bounce b live at ctr[b * kCtrStride + ...].
-constexpr int kCtrStride = 32;
-enum {
-  C_QCOUNT = 0,      // extension rays queued for bounce b
-  C_EXT_WORK = 1,    // work cursor of extend(b)
-  C_SHADE_WORK = 2,  // work cursor of shade(b)
-  C_SL_COUNT = 3,    // light shadow rays queued at bounce b
-  C_SE_COUNT = 4,    // environment shadow rays queued at bounce b
-  C_SL_WORK = 5,
-  C_SE_WORK = 6,
-  C_ACCEPTED = 7,    // finite samples accumulated in this batch (bounce 0 block)
-  // with counting: u64 totals at 64-bit word (b * kCtrStride + C_COUNT64) / 2 + k,
-  // k = 0 extension node fetches, 1 extension prim tests, 2 shadow node
-  // fetches, 3 shadow prim tests
-  C_COUNT64 = 8,
-  C_CAT_COUNT = 16,  // hits binned by shading category (kNumCategories slots)
-  C_CAT_WORK = 24,   // work cursors of the category shade kernels
-};
-
-
 // Shading categories: extend bins every hit by the BSDF it needs, so each
 // shade kernel runs one BSDF without divergence across material types.
 enum { CAT_MISS = 0, CAT_EMIT = 1, CAT_LAMBERT = 2, CAT_GGX = 3, CAT_SPECULAR = 4, kNumCategories = 5 };
+
+// Work is split into kSlices slices, one per XCD: the dispatcher hands
+// workgroups to the eight XCDs round-robin, so block b works on slice
+// b % kSlices. Each slice owns region s of every queue (entries
+// [s * region, (s + 1) * region)) and its own counters, each on its own
+// 128-byte line: same-address device atomics are served one at a time at
+// the memory side, so one address per slice lets eight proceed at once.
+// A path never leaves the slice it started in.
+constexpr uint32_t kSlices = 8;
+constexpr uint32_t kLineWords = 32;
+enum : uint32_t {
+  S_QCOUNT = 0,   // extension rays queued for bounce b
+  S_BASE,         // bounce 0: first path id of the slice
+  S_EXT_WORK,     // work cursor of extend(b)
+  S_SL_COUNT,     // light shadow rays queued at bounce b
+  S_SE_COUNT,     // environment shadow rays queued at bounce b
+  S_SL_WORK,
+  S_SE_WORK,
+  S_CAT_COUNT,    // hits binned by shading category (kNumCategories slots)
+  S_CAT_WORK = S_CAT_COUNT + kNumCategories,
+  kSlots = S_CAT_WORK + kNumCategories
+};
+constexpr uint32_t kBounceWords = kSlots * kSlices * kLineWords;
+__host__ __device__ constexpr uint32_t ctr_at(uint32_t bounce, uint32_t slot, uint32_t slice) {
+  return ((bounce * kSlots + slot) * kSlices + slice) * kLineWords;
+}
+// 64-bit tallies: [0] finite samples accumulated in the batch; with counting,
+// [1 + 4 b + k]: k = 0 extension node fetches, 1 extension prim tests,
+// 2 shadow node fetches, 3 shadow prim tests at bounce b.
+inline size_t tally_words(uint32_t depth) { return 1 + 4 * size_t(depth); }
 
 // Path state lives at the path's index (path = sample-in-batch * npix +
 // pixel) and is updated in place; every queue is a list of 4-byte path ids.
@@ -46,7 +56,9 @@ struct Queues {
   uint32_t* active[2];            // paths to extend at bounce b, parity b & 1 (bounce 0: all paths)
   uint32_t* cat[kNumCategories];  // paths hit this bounce, per shading category
   uint32_t* shadow[2];            // paths with a light / environment shadow ray this bounce
-  uint32_t* ctr;
+  uint32_t region;                // queue entries per slice
+  uint32_t* ctr;                  // per-bounce counters, ctr_at()
+  unsigned long long* tally;
   uint2* gstack;    // traversal-stack overflow, gstride entries per level
   uint32_t gstride;
   uint32_t stack_cap;  // LDS entries per lane
@@ -68,7 +80,7 @@ struct Launch {
   hipStream_t stream;
 };
 
-void reset_counters(const Launch& l, uint32_t* ctr, uint32_t n);
+void reset_counters(const Launch& l, const Queues& q, uint32_t depth);
 void raygen(const Launch& l, const Frame& f, const Queues& q, uint32_t sample0, uint32_t nsamples);
 void extend(const Launch& l, const Frame& f, const Queues& q, uint32_t bounce, bool count);
 // One bounce of shading: one kernel per category (empty categories exit at once).

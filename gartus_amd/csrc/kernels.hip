@@ -107,9 +107,17 @@ __device__ __forceinline__ float scattering_pdf(const GMat& m, f3 d_in, f3 n, f3
   return 0.f;
 }
 
-__global__ void k_reset(uint32_t* ctr, uint32_t n) {
-  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) ctr[i] = 0;
+// Clears the counters of bounces 0..depth (one word per 128-byte line) and
+// the tallies.
+__global__ void k_reset(uint32_t* ctr, uint32_t ncounters, unsigned long long* tally, uint32_t ntally) {
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < ncounters; i += gridDim.x * blockDim.x)
+    ctr[size_t(i) * kLineWords] = 0;
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < ntally; i += gridDim.x * blockDim.x) tally[i] = 0;
 }
+
+// Slices a block works on: b % G, b % G + G, ... with G = min(grid, kSlices),
+// so every slice has at least one block whatever the grid.
+__device__ __forceinline__ uint32_t slice_step() { return gridDim.x < kSlices ? gridDim.x : kSlices; }
 
 // Wave-reduced 64-bit add of per-lane counts (one atomic per wave).
 __device__ __forceinline__ void wave_count(unsigned long long* dst, uint32_t v) {
@@ -117,7 +125,7 @@ __device__ __forceinline__ void wave_count(unsigned long long* dst, uint32_t v) 
   if (lane_id() == 0 && v) atomicAdd(dst, (unsigned long long)v);
 }
 __device__ __forceinline__ unsigned long long* count_slot(const Queues& q, uint32_t bounce, int k) {
-  return reinterpret_cast<unsigned long long*>(q.ctr + bounce * kCtrStride + C_COUNT64) + k;
+  return q.tally + 1 + 4 * bounce + k;
 }
 __device__ __forceinline__ uint32_t node_fetches(const Stats& s, uint32_t width) {
   // every internal node fetched tests `width` child boxes; the root box test is not a fetch
@@ -128,7 +136,12 @@ __device__ __forceinline__ uint32_t node_fetches(const Stats& s, uint32_t width)
 // (render_world_linear_pixel_with_context camera.rs:2660-2723; rays :1658-1718).
 __global__ void k_raygen(Frame f, Queues q, uint32_t sample0, uint32_t nsamples) {
   const uint32_t total = f.npix * nsamples;
-  if (blockIdx.x == 0 && threadIdx.x == 0) q.ctr[C_QCOUNT] = total;
+  if (blockIdx.x == 0 && threadIdx.x < kSlices) {  // slice s starts extend(0) at path total * s / kSlices
+    const uint32_t s = threadIdx.x;
+    const uint32_t lo = uint32_t(uint64_t(total) * s / kSlices), hi = uint32_t(uint64_t(total) * (s + 1) / kSlices);
+    q.ctr[ctr_at(0, S_BASE, s)] = lo;
+    q.ctr[ctr_at(0, S_QCOUNT, s)] = hi - lo;
+  }
   for (uint32_t p = blockIdx.x * blockDim.x + threadIdx.x; p < total; p += gridDim.x * blockDim.x) {
     uint32_t x, y, sample;
     path_coords(f, p, sample0, &x, &y, &sample);
@@ -158,11 +171,13 @@ template <bool COUNT>
 __global__ void k_extend(Frame f, Queues q, uint32_t bounce) {
   extern __shared__ uint2 smem[];
   Stack st = make_stack(q, smem);
-  uint32_t* const ctr = q.ctr + bounce * kCtrStride;
-  const uint32_t count = ctr[C_QCOUNT];
-  const uint32_t* __restrict__ active = q.active[bounce & 1];
+  for (uint32_t s = blockIdx.x % slice_step(); s < kSlices; s += slice_step()) {
+  const uint32_t count = q.ctr[ctr_at(bounce, S_QCOUNT, s)];
+  if (count == 0) continue;
+  const uint32_t first = bounce == 0 ? q.ctr[ctr_at(0, S_BASE, s)] : 0;
+  const uint32_t* __restrict__ active = q.active[bounce & 1] + size_t(s) * q.region;
   for (;;) {
-    uint32_t base = wave_fetch(ctr + C_EXT_WORK);
+    uint32_t base = wave_fetch(q.ctr + ctr_at(bounce, S_EXT_WORK, s));
     if (base >= count) break;
     uint32_t nodes = 0, prims = 0;
     uint32_t ids[kChunk], cats[kChunk];
@@ -172,7 +187,7 @@ __global__ void k_extend(Frame f, Queues q, uint32_t bounce) {
       cats[k] = kNumCategories;
       ids[k] = 0;
       if (j < count) {
-        uint32_t p = bounce == 0 ? j : active[j];
+        uint32_t p = bounce == 0 ? first + j : active[j];
         ids[k] = p;
         float4 o4 = q.ray_o[p], d4 = q.ray_d[p];
         float t;
@@ -190,12 +205,13 @@ __global__ void k_extend(Frame f, Queues q, uint32_t bounce) {
       uint32_t bits = 0;
 #pragma unroll
       for (uint32_t k = 0; k < kChunk; ++k) bits |= uint32_t(cats[k] == c) << k;
-      wave_append_chunk(ctr + C_CAT_COUNT + c, q.cat[c], bits, ids);
+      wave_append_chunk(q.ctr + ctr_at(bounce, S_CAT_COUNT + c, s), q.cat[c] + size_t(s) * q.region, bits, ids);
     }
     if (COUNT) {
       wave_count(count_slot(q, bounce, 0), nodes);
       wave_count(count_slot(q, bounce, 1), prims);
     }
+  }
   }
 }
 
@@ -342,13 +358,15 @@ __device__ __forceinline__ uint32_t shade_specular(const Frame& f, const Queues&
 
 template <int CAT>
 __global__ void k_shade(Frame f, Queues q, uint32_t bounce, uint32_t sample0, uint32_t last) {
-  uint32_t* const ctr = q.ctr + bounce * kCtrStride;
-  const uint32_t count = ctr[C_CAT_COUNT + CAT];
-  const uint32_t* __restrict__ list = q.cat[CAT];
   const bool has_env = f.env.kind != 0;
   constexpr bool kScatters = CAT == CAT_LAMBERT || CAT == CAT_GGX || CAT == CAT_SPECULAR;
+  for (uint32_t s = blockIdx.x % slice_step(); s < kSlices; s += slice_step()) {
+  const uint32_t count = q.ctr[ctr_at(bounce, S_CAT_COUNT + CAT, s)];
+  if (count == 0) continue;
+  const size_t region = size_t(s) * q.region;
+  const uint32_t* __restrict__ list = q.cat[CAT] + region;
   for (;;) {
-    uint32_t base = wave_fetch(ctr + C_CAT_WORK + CAT);
+    uint32_t base = wave_fetch(q.ctr + ctr_at(bounce, S_CAT_WORK + CAT, s));
     if (base >= count) break;
     uint32_t ids[kChunk], outs[kChunk];
 #pragma unroll
@@ -394,10 +412,11 @@ __global__ void k_shade(Frame f, Queues q, uint32_t bounce, uint32_t sample0, ui
         bits_sl |= uint32_t((outs[k] & OUT_SL) != 0) << k;
         bits_se |= uint32_t((outs[k] & OUT_SE) != 0) << k;
       }
-      wave_append_chunk(q.ctr + (bounce + 1) * kCtrStride + C_QCOUNT, q.active[(bounce + 1) & 1], bits_ext, ids);
-      wave_append_chunk(ctr + C_SL_COUNT, q.shadow[0], bits_sl, ids);
-      wave_append_chunk(ctr + C_SE_COUNT, q.shadow[1], bits_se, ids);
+      wave_append_chunk(q.ctr + ctr_at(bounce + 1, S_QCOUNT, s), q.active[(bounce + 1) & 1] + region, bits_ext, ids);
+      wave_append_chunk(q.ctr + ctr_at(bounce, S_SL_COUNT, s), q.shadow[0] + region, bits_sl, ids);
+      wave_append_chunk(q.ctr + ctr_at(bounce, S_SE_COUNT, s), q.shadow[1] + region, bits_se, ids);
     }
+  }
   }
 }
 
@@ -408,13 +427,14 @@ template <bool COUNT>
 __global__ void k_connect(Frame f, Queues q, uint32_t bounce, uint32_t kind) {
   extern __shared__ uint2 smem[];
   Stack st = make_stack(q, smem);
-  uint32_t* const ctr = q.ctr + bounce * kCtrStride;
-  const uint32_t count = ctr[kind == 0 ? C_SL_COUNT : C_SE_COUNT];
-  uint32_t* cursor = ctr + (kind == 0 ? C_SL_WORK : C_SE_WORK);
-  const uint32_t* __restrict__ list = q.shadow[kind];
   const float4* __restrict__ so = q.s_o[kind];
   const float4* __restrict__ sd = q.s_d[kind];
   const float4* __restrict__ sw = q.s_w[kind];
+  for (uint32_t s = blockIdx.x % slice_step(); s < kSlices; s += slice_step()) {
+  const uint32_t count = q.ctr[ctr_at(bounce, kind == 0 ? S_SL_COUNT : S_SE_COUNT, s)];
+  if (count == 0) continue;
+  uint32_t* cursor = q.ctr + ctr_at(bounce, kind == 0 ? S_SL_WORK : S_SE_WORK, s);
+  const uint32_t* __restrict__ list = q.shadow[kind] + size_t(s) * q.region;
   for (;;) {
     uint32_t base = wave_fetch(cursor);
     if (base >= count) break;
@@ -458,6 +478,7 @@ __global__ void k_connect(Frame f, Queues q, uint32_t bounce, uint32_t kind) {
       wave_count(count_slot(q, bounce, 3), prims);
     }
   }
+  }
 }
 
 // Adds each finished sample of the batch to its pixel in sample order,
@@ -481,7 +502,7 @@ __global__ void k_accumulate(Frame f, Queues q, double4* acc, uint32_t nsamples)
     }
   }
   for (int off = 32; off > 0; off >>= 1) accepted += __shfl_down(accepted, off);
-  if (lane_id() == 0 && accepted) atomicAdd(q.ctr + C_ACCEPTED, accepted);
+  if (lane_id() == 0 && accepted) atomicAdd(q.tally, (unsigned long long)accepted);
 }
 
 // average_accepted_samples (camera.rs:2993-2999)
@@ -591,8 +612,10 @@ uint32_t occupancy_blocks(int kernel, uint32_t block, size_t lds) {
   return uint32_t(nb);
 }
 
-void reset_counters(const Launch& l, uint32_t* ctr, uint32_t n) {
-  hipLaunchKernelGGL(k_reset, dim3(1), dim3(256), 0, l.stream, ctr, n);
+void reset_counters(const Launch& l, const Queues& q, uint32_t depth) {
+  const uint32_t n = (depth + 1) * kSlots * kSlices;
+  hipLaunchKernelGGL(k_reset, dim3((n + 255) / 256), dim3(256), 0, l.stream, q.ctr, n, q.tally,
+                     uint32_t(tally_words(depth)));
 }
 void raygen(const Launch& l, const Frame& f, const Queues& q, uint32_t sample0, uint32_t nsamples) {
   hipLaunchKernelGGL(k_raygen, dim3(l.grid), dim3(l.block), 0, l.stream, f, q, sample0, nsamples);
