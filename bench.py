@@ -228,6 +228,7 @@ def main() -> int:
     avg_launch_ms = sum(s.extend_ms for s in stats) / sum(s.extend_launches for s in stats)
     achieved = (ext_bytes / launches) / (avg_launch_ms * 1e-3) / 1e9
     paths = stats[0].camera_paths
+    traffic = traffic_for(args.workload)
     out = np.zeros(W * H * 3, np.float32)
     abi.check(lib.gt_memcpy_to_host(ctx.handle, out.ctypes.data_as(C.c_void_p), dout, out.nbytes))
     log(f"image mean {out.reshape(-1, 3).mean(0)}; samples/s {paths * args.steps / wall:.4g}; "
@@ -260,10 +261,14 @@ def main() -> int:
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4),
-            "traffic": traffic_for(args.workload),
+            "traffic": traffic,
             "kernel": "k_extend (closest-hit traversal)",
             "bytes_per_launch": round(ext_bytes / launches),
             "avg_launch_ms": round(avg_launch_ms, 4),
+            # HBM bytes actually moved per launch (rocprofv3 FETCH_SIZE + WRITE_SIZE, copy-calibrated)
+            # over the live launch time: the BVH and primitives are served from L2, so this is
+            # far below the algorithmic figure
+            "hbm_measured_gbs": None if traffic is None else round(traffic / (avg_launch_ms * 1e-3) / 1e9, 1),
         },
     }
     if not args.no_cpu_baseline:
