@@ -171,8 +171,10 @@ PROTOTYPES = {
     "gt_render_hdr_device": (I, [P, P, C.POINTER(GtCamera), P, P, P, I, C.POINTER(GtRenderStats)]),
     "gt_render_aovs": (I, [P, P, C.POINTER(GtCamera), P, P, F, F, F, C.POINTER(GtRenderStats)]),
     "gt_render_aovs_device": (I, [P, P, C.POINTER(GtCamera), P, P, F, F, F, C.POINTER(GtRenderStats)]),
-    "gt_render_batch": (I, [P, C.c_size_t, PP, C.POINTER(GtCamera), PP, C.POINTER(F),
+    "gt_render_batch": (I, [P, C.c_size_t, PP, C.POINTER(GtCamera), PP, PP, C.POINTER(F),
                             C.POINTER(GtRenderStats)]),
+    "gt_render_batch_device": (I, [P, C.c_size_t, PP, C.POINTER(GtCamera), PP, PP, PP, I,
+                                   C.POINTER(GtRenderStats)]),
 }
 
 _lib = None

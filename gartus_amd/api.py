@@ -854,3 +854,41 @@ class SurfaceScene:
             rs.add_triangles(mid, tris)
         rs.build_bvh_with_options(leaf_size)
         return rs
+
+
+def render_batch(items, stats: bool = False):
+    """Renders many (scene, camera, lights, environment) frames in one wavefront
+    (gt_render_batch): every launch mixes the frames' paths. `items` is a list
+    of (RayScene, RayCamera, SamplingTargetList | None, EnvironmentLight | None);
+    all scenes must share one Context. Returns a list of HdrImage (and the
+    summed render statistics when `stats`). Each image equals the frame's own
+    single render."""
+    items = list(items)
+    if not items:
+        return ([], {}) if stats else []
+    ctx = items[0][0].context
+    n = len(items)
+    scenes = (C.c_void_p * n)()
+    lights = (C.c_void_p * n)()
+    envs = (C.c_void_p * n)()
+    cams = (abi.GtCamera * n)()
+    outs = (abi.F * n)()
+    images = []
+    for i, (world, camera, L, env) in enumerate(items):
+        if not isinstance(world, RayScene) or world.context is not ctx:
+            raise abi.InvalidArgument(abi.GT_EINVAL, "batched scenes must be RayScenes of one context")
+        scenes[i] = world._compile()
+        lights[i] = L._compile(ctx) if L is not None else None
+        envs[i] = env._compile(ctx) if env is not None else None
+        cams[i] = camera.raw
+        w, h = camera.image_width(), camera.image_height()
+        img = np.zeros((h, w, 3), np.float32)
+        images.append(img)
+        outs[i] = img.ctypes.data_as(abi.F)
+    st = abi.GtRenderStats()
+    abi.check(ctx.lib.gt_render_batch(ctx.handle, n, scenes, cams, lights, envs, outs,
+                                      C.byref(st) if stats else None))
+    result = [HdrImage(im.shape[1], im.shape[0], im) for im in images]
+    if stats:
+        return result, {k: getattr(st, k) for k, _ in abi.GtRenderStats._fields_}
+    return result

@@ -68,6 +68,8 @@ struct gt_context {
   DevBuf<unsigned long long> tally;
   DevBuf<uint2> gstack;
   DevBuf<double4> acc;
+  DevBuf<gtk::Frame> frame_table;  // batched renders
+  DevBuf<gtk::Seg> seg_table;
   size_t cap_paths = 0;
   uint32_t region = 0;  // queue entries per slice
   uint32_t gstride = 0;
@@ -432,44 +434,133 @@ int check_objects(gt_context* ctx, const gt_scene* scene, const gt_lights* light
   return GT_OK;
 }
 
-int render_impl(gt_context* ctx, const gt_scene* cscene, const gt_camera* camera, const gt_lights* clights,
-                const gt_env* env, void* d_out, int out_is_double, float* d_albedo, float* d_normal,
-                gt_render_stats* stats) {
-  int rc = check_objects(ctx, cscene, clights, env);
-  if (rc != GT_OK) return rc;
-  gt_scene* scene = const_cast<gt_scene*>(cscene);
-  gt_lights* lights = const_cast<gt_lights*>(clights);
-  GT_HIP(hipSetDevice(ctx->device));
-  if ((rc = ensure_built(scene)) != GT_OK) return rc;
-  if (lights && (rc = upload_lights(lights)) != GT_OK) return rc;
+// One frame of a render: what the kernels read plus what the host plans with.
+struct FrameJob {
   gtk::Frame f;
-  if ((rc = make_frame(ctx, scene, camera, lights, env, &f)) != GT_OK) return rc;
-  const uint32_t npix = f.npix;
-  const uint32_t spp = f.cam.spp;
-  const uint32_t depth = f.cam.max_depth;
-  if (depth > 4096) return fail(GT_EINVAL, "max_depth above 4096 is not supported");
-  uint64_t max_paths = ctx->tuning.max_paths ? ctx->tuning.max_paths : (uint64_t(1) << 25);
-  max_paths = std::min<uint64_t>(max_paths, uint64_t(1) << 31);
-  if (uint64_t(npix) > max_paths) max_paths = npix;
-  if (uint64_t(npix) >= (uint64_t(1) << 31)) return fail(GT_EINVAL, "image too large (2^31 pixels)");
-  uint32_t kmax = uint32_t(std::min<uint64_t>(max_paths / npix, spp));
-  kmax = std::max<uint32_t>(kmax, 1);
-  uint32_t nbatch = (spp + kmax - 1) / kmax;
-  uint32_t k = (spp + nbatch - 1) / nbatch;
-  if ((rc = ensure_queues(ctx, size_t(k) * npix, depth)) != GT_OK) return rc;
-  uint64_t dneed = stack_depth(scene, f.scene.width);
-  if ((rc = ensure_gstack(ctx, dneed, kStackCap)) != GT_OK) return rc;
-  GT_HIP(ctx->acc.ensure(npix));
+  uint32_t spp = 0, depth = 0;
+  bool conn_l = false, conn_e = false;  // light / environment shadow rays
+  uint64_t stack_need = 0;
+  void* d_out = nullptr;                // finalize target (float or double W*H*3), or null
+};
+
+// Splits the frames' samples into batches of at most `cap` paths: each batch
+// is a list of segments (whole-image sample ranges), at most one per frame,
+// and a frame's segments come in sample order.
+std::vector<std::vector<gtk::Seg>> plan_batches(const std::vector<FrameJob>& jobs, uint64_t cap, bool single) {
+  std::vector<std::vector<gtk::Seg>> batches;
+  if (single) {  // equal batches of k whole-image samples
+    const FrameJob& j = jobs[0];
+    const uint32_t npix = j.f.npix;
+    uint32_t kmax = uint32_t(std::max<uint64_t>(1, std::min<uint64_t>(cap / npix, j.spp)));
+    uint32_t nb = (j.spp + kmax - 1) / kmax, k = (j.spp + nb - 1) / nb;
+    for (uint32_t b = 0; b < nb; ++b) {
+      gtk::Seg sg{};
+      sg.sample0 = b * k;
+      sg.nsamples = std::min(k, j.spp - sg.sample0);
+      batches.push_back({sg});
+    }
+    return batches;
+  }
+  std::vector<gtk::Seg> cur;
+  uint64_t used = 0, pix = 0;
+  for (uint32_t i = 0; i < jobs.size(); ++i) {
+    const uint32_t npix = jobs[i].f.npix;
+    uint32_t s0 = 0;
+    while (s0 < jobs[i].spp) {
+      uint64_t fit = (cap - used) / npix;
+      if (fit == 0) {
+        batches.push_back(cur);
+        cur.clear();
+        used = pix = 0;
+        continue;
+      }
+      gtk::Seg sg{};
+      sg.path0 = uint32_t(used);
+      sg.frame = i;
+      sg.sample0 = s0;
+      sg.nsamples = uint32_t(std::min<uint64_t>(fit, jobs[i].spp - s0));
+      sg.pix0 = uint32_t(pix);
+      cur.push_back(sg);
+      used += uint64_t(sg.nsamples) * npix;
+      pix += npix;
+      s0 += sg.nsamples;
+    }
+  }
+  if (!cur.empty()) batches.push_back(cur);
+  return batches;
+}
+
+// The wavefront engine: renders every job's samples in batches and writes
+// each job's finalized image. `multi` selects the frame-table kernels.
+int run_jobs(gt_context* ctx, std::vector<FrameJob>& jobs, bool multi, int out_is_double, gt_render_stats* stats,
+             uint64_t max_paths_override = 0) {
+  uint64_t cap = max_paths_override ? max_paths_override
+                                    : (ctx->tuning.max_paths ? ctx->tuning.max_paths : (uint64_t(1) << 25));
+  cap = std::min<uint64_t>(cap, uint64_t(1) << 31);
+  uint64_t acc_total = 0, stack_need = 0;
+  uint32_t depth_all = 0;
+  for (FrameJob& j : jobs) {
+    if (uint64_t(j.f.npix) >= (uint64_t(1) << 31)) return fail(GT_EINVAL, "image too large (2^31 pixels)");
+    if (j.depth > 4096) return fail(GT_EINVAL, "max_depth above 4096 is not supported");
+    cap = std::max<uint64_t>(cap, j.f.npix);
+    acc_total += j.f.npix;
+    depth_all = std::max(depth_all, j.depth);
+    stack_need = std::max(stack_need, j.stack_need);
+  }
+  if (acc_total >= (uint64_t(1) << 31)) return fail(GT_EINVAL, "batch too large (2^31 pixels in total)");
+  std::vector<std::vector<gtk::Seg>> batches = plan_batches(jobs, cap, !multi);
+  uint64_t max_batch = 0;
+  for (auto& b : batches) {
+    uint64_t t = 0;
+    for (auto& sg : b) t += uint64_t(sg.nsamples) * jobs[sg.frame].f.npix;
+    max_batch = std::max(max_batch, t);
+  }
+  int rc;
+  if ((rc = ensure_queues(ctx, size_t(max_batch), depth_all)) != GT_OK) return rc;
+  if ((rc = ensure_gstack(ctx, stack_need, kStackCap)) != GT_OK) return rc;
+  GT_HIP(ctx->acc.ensure(acc_total));
+  uint64_t off = 0;
+  for (FrameJob& j : jobs) {
+    j.f.acc = ctx->acc.p + off;
+    off += j.f.npix;
+  }
+  size_t seg_total = 0;
+  for (auto& b : batches) seg_total += b.size();
+  if (multi) {
+    std::vector<gtk::Frame> frames(jobs.size());
+    for (size_t i = 0; i < jobs.size(); ++i) frames[i] = jobs[i].f;
+    std::vector<gtk::Seg> segs;
+    for (auto& b : batches) segs.insert(segs.end(), b.begin(), b.end());
+    GT_HIP(ctx->frame_table.ensure(frames.size()));
+    GT_HIP(ctx->seg_table.ensure(segs.size()));
+    GT_HIP(hipMemcpyAsync(ctx->frame_table.p, frames.data(), frames.size() * sizeof(gtk::Frame),
+                          hipMemcpyHostToDevice, ctx->stream));
+    GT_HIP(hipMemcpyAsync(ctx->seg_table.p, segs.data(), segs.size() * sizeof(gtk::Seg), hipMemcpyHostToDevice,
+                          ctx->stream));
+    GT_HIP(hipStreamSynchronize(ctx->stream));  // the host tables go out of scope
+  }
   gtk::Queues q = queues_of(ctx);
   gtk::Launch L{grid_for(ctx), ctx->tuning.block_size, ctx->stream};
   hipStream_t st = ctx->stream;
-  const size_t ctr_words = size_t(depth + 1) * gtk::kBounceWords;
-  const size_t tally_words = gtk::tally_words(depth);
+  const size_t nbatch = batches.size();
+  const size_t ctr_words = size_t(depth_all + 1) * gtk::kBounceWords;
+  const size_t tally_words = gtk::tally_words(depth_all);
   const size_t host_words = ctr_words + 2 * tally_words;  // counters, then the tallies as u32 pairs
   const bool want_stats = stats != nullptr;
   const bool count = want_stats && ctx->tuning.count_traversal != 0;
-  const bool time_kernels = want_stats;
-  size_t nev = time_kernels ? 2 + 5 * size_t(nbatch) * depth : 0;
+  size_t bounces_total = 0;
+  std::vector<uint32_t> batch_depth(nbatch, 0);
+  std::vector<char> batch_l(nbatch, 0), batch_e(nbatch, 0);
+  for (size_t b = 0; b < nbatch; ++b) {
+    for (auto& sg : batches[b]) {
+      const FrameJob& j = jobs[multi ? sg.frame : 0];
+      batch_depth[b] = std::max(batch_depth[b], j.depth);
+      batch_l[b] |= j.conn_l;
+      batch_e[b] |= j.conn_e;
+    }
+    bounces_total += batch_depth[b];
+  }
+  size_t nev = want_stats ? 2 + 5 * bounces_total : 0;
   while (ctx->events.size() < nev) {
     hipEvent_t e;
     GT_HIP(hipEventCreate(&e));
@@ -483,53 +574,70 @@ int render_impl(gt_context* ctx, const gt_scene* cscene, const gt_camera* camera
   }
   // events per bounce: [e0] extend [e1][e2] shade [e3] connect [e4]
   size_t ev = 0;
-  if (time_kernels) GT_HIP(hipEventRecord(ctx->events[ev++], st));
-  GT_HIP(hipMemsetAsync(ctx->acc.p, 0, size_t(npix) * sizeof(double4), st));
-  const bool conn_l = f.nee && lights != nullptr;
-  const bool conn_e = f.nee && env != nullptr;
-  for (uint32_t b = 0; b < nbatch; ++b) {
-    uint32_t s0 = b * k;
-    uint32_t ns = std::min(k, spp - s0);
-    gtk::reset_counters(L, q, depth);
-    gtk::raygen(L, f, q, s0, ns);
+  if (want_stats) GT_HIP(hipEventRecord(ctx->events[ev++], st));
+  gtk::zero_acc(L, ctx->acc.p, acc_total);
+  size_t seg_off = 0;
+  for (size_t b = 0; b < nbatch; ++b) {
+    gtk::Work w{};
+    w.multi = multi;
+    w.bt.nseg = uint32_t(batches[b].size());
+    uint64_t total = 0, pix = 0;
+    for (auto& sg : batches[b]) {
+      total += uint64_t(sg.nsamples) * jobs[multi ? sg.frame : 0].f.npix;
+      pix += jobs[multi ? sg.frame : 0].f.npix;
+    }
+    w.bt.total = uint32_t(total);
+    w.bt.npix_total = uint32_t(pix);
+    if (multi) {
+      w.bt.frames = ctx->frame_table.p;
+      w.bt.segs = ctx->seg_table.p + seg_off;
+      std::memset(&w.f, 0, sizeof w.f);
+    } else {
+      w.f = jobs[0].f;
+      w.bt.seg0 = batches[b][0];
+    }
+    seg_off += batches[b].size();
+    const uint32_t depth = batch_depth[b];
+    gtk::reset_counters(L, q, depth_all);
+    gtk::raygen(L, w, q);
     for (uint32_t bounce = 0; bounce < depth; ++bounce) {
-      if (time_kernels) GT_HIP(hipEventRecord(ctx->events[ev++], st));
-      gtk::extend(L, f, q, bounce, count);
-      if (time_kernels) {
+      if (want_stats) GT_HIP(hipEventRecord(ctx->events[ev++], st));
+      gtk::extend(L, w, q, bounce, count);
+      if (want_stats) {
         GT_HIP(hipEventRecord(ctx->events[ev++], st));
         GT_HIP(hipEventRecord(ctx->events[ev++], st));
       }
-      gtk::shade(L, f, q, bounce, s0, bounce + 1 == depth ? 1u : 0u);
-      if (time_kernels) GT_HIP(hipEventRecord(ctx->events[ev++], st));
-      if (conn_l) gtk::connect(L, f, q, bounce, 0, count);
-      if (conn_e) gtk::connect(L, f, q, bounce, 1, count);
-      if (time_kernels) GT_HIP(hipEventRecord(ctx->events[ev++], st));
+      gtk::shade(L, w, q, bounce);
+      if (want_stats) GT_HIP(hipEventRecord(ctx->events[ev++], st));
+      if (batch_l[b]) gtk::connect(L, w, q, bounce, 0, count);
+      if (batch_e[b]) gtk::connect(L, w, q, bounce, 1, count);
+      if (want_stats) GT_HIP(hipEventRecord(ctx->events[ev++], st));
     }
-    gtk::accumulate(L, f, q, ctx->acc.p, ns);
+    gtk::accumulate(L, w, q);
     if (want_stats) {
       uint32_t* h = ctx->host_ctr + host_words * b;
       GT_HIP(hipMemcpyAsync(h, q.ctr, ctr_words * sizeof(uint32_t), hipMemcpyDeviceToHost, st));
       GT_HIP(hipMemcpyAsync(h + ctr_words, q.tally, tally_words * sizeof(uint64_t), hipMemcpyDeviceToHost, st));
     }
   }
-  if (d_out) gtk::finalize(L, ctx->acc.p, npix, d_out, out_is_double);
-  if (d_albedo) gtk::aov(L, f, q, d_albedo, d_normal);
-  if (time_kernels) GT_HIP(hipEventRecord(ctx->events[ev++], st));
+  for (FrameJob& j : jobs)
+    if (j.d_out) gtk::finalize(L, j.f.acc, j.f.npix, j.d_out, out_is_double);
+  if (want_stats) GT_HIP(hipEventRecord(ctx->events[ev++], st));
   GT_HIP(hipGetLastError());
   if (want_stats) {
     GT_HIP(hipStreamSynchronize(st));
     std::memset(stats, 0, sizeof *stats);
-    stats->camera_paths = uint64_t(npix) * spp;
-    for (uint32_t b = 0; b < nbatch; ++b) {
+    for (const FrameJob& j : jobs) stats->camera_paths += uint64_t(j.f.npix) * j.spp;
+    for (size_t b = 0; b < nbatch; ++b) {
       const uint32_t* c = ctx->host_ctr + host_words * b;
       std::vector<uint64_t> tally(tally_words);
       std::memcpy(tally.data(), c + ctr_words, tally_words * sizeof(uint64_t));
       stats->accepted_samples += tally[0];
-      for (uint32_t bounce = 0; bounce < depth; ++bounce) {
+      for (uint32_t bounce = 0; bounce < batch_depth[b]; ++bounce) {
         for (uint32_t sl = 0; sl < gtk::kSlices; ++sl) {
           stats->extension_rays += c[gtk::ctr_at(bounce, gtk::S_QCOUNT, sl)];
-          if (conn_l) stats->shadow_rays += c[gtk::ctr_at(bounce, gtk::S_SL_COUNT, sl)];
-          if (conn_e) stats->shadow_rays += c[gtk::ctr_at(bounce, gtk::S_SE_COUNT, sl)];
+          if (batch_l[b]) stats->shadow_rays += c[gtk::ctr_at(bounce, gtk::S_SL_COUNT, sl)];
+          if (batch_e[b]) stats->shadow_rays += c[gtk::ctr_at(bounce, gtk::S_SE_COUNT, sl)];
         }
         if (count) {
           const uint64_t* v = tally.data() + 1 + 4 * bounce;
@@ -542,17 +650,51 @@ int render_impl(gt_context* ctx, const gt_scene* cscene, const gt_camera* camera
     GT_HIP(hipEventElapsedTime(&ms, ctx->events[0], ctx->events[ev - 1]));
     stats->device_ms = ms;
     size_t base = 1;
-    for (uint32_t b = 0; b < nbatch; ++b) {
-      for (uint32_t bounce = 0; bounce < depth; ++bounce) {
-        float a = 0, s = 0, c = 0;
-        GT_HIP(hipEventElapsedTime(&a, ctx->events[base], ctx->events[base + 1]));
-        GT_HIP(hipEventElapsedTime(&s, ctx->events[base + 2], ctx->events[base + 3]));
-        GT_HIP(hipEventElapsedTime(&c, ctx->events[base + 3], ctx->events[base + 4]));
-        stats->extend_ms += a; stats->shade_ms += s; stats->connect_ms += c;
-        stats->extend_launches++;
-        base += 5;
-      }
+    for (size_t i = 0; i < bounces_total; ++i) {
+      float a = 0, sh = 0, cn = 0;
+      GT_HIP(hipEventElapsedTime(&a, ctx->events[base], ctx->events[base + 1]));
+      GT_HIP(hipEventElapsedTime(&sh, ctx->events[base + 2], ctx->events[base + 3]));
+      GT_HIP(hipEventElapsedTime(&cn, ctx->events[base + 3], ctx->events[base + 4]));
+      stats->extend_ms += a; stats->shade_ms += sh; stats->connect_ms += cn;
+      stats->extend_launches++;
+      base += 5;
     }
+  }
+  return GT_OK;
+}
+
+int prepare_job(gt_context* ctx, const gt_scene* cscene, const gt_camera* camera, const gt_lights* clights,
+                const gt_env* env, FrameJob* j) {
+  int rc = check_objects(ctx, cscene, clights, env);
+  if (rc != GT_OK) return rc;
+  if (!camera) return fail(GT_EINVAL, "camera is null");
+  gt_scene* scene = const_cast<gt_scene*>(cscene);
+  gt_lights* lights = const_cast<gt_lights*>(clights);
+  if ((rc = ensure_built(scene)) != GT_OK) return rc;
+  if (lights && (rc = upload_lights(lights)) != GT_OK) return rc;
+  if ((rc = make_frame(ctx, scene, camera, lights, env, &j->f)) != GT_OK) return rc;
+  j->spp = j->f.cam.spp;
+  j->depth = j->f.cam.max_depth;
+  j->conn_l = j->f.nee && lights != nullptr;
+  j->conn_e = j->f.nee && env != nullptr;
+  j->stack_need = stack_depth(scene, j->f.scene.width);
+  return GT_OK;
+}
+
+int render_impl(gt_context* ctx, const gt_scene* scene, const gt_camera* camera, const gt_lights* lights,
+                const gt_env* env, void* d_out, int out_is_double, float* d_albedo, float* d_normal,
+                gt_render_stats* stats) {
+  if (!ctx) return fail(GT_EINVAL, "context is null");
+  GT_HIP(hipSetDevice(ctx->device));
+  std::vector<FrameJob> jobs(1);
+  int rc = prepare_job(ctx, scene, camera, lights, env, &jobs[0]);
+  if (rc != GT_OK) return rc;
+  jobs[0].d_out = d_out;
+  if ((rc = run_jobs(ctx, jobs, false, out_is_double, stats)) != GT_OK) return rc;
+  if (d_albedo) {
+    gtk::Launch L{grid_for(ctx), ctx->tuning.block_size, ctx->stream};
+    gtk::aov(L, jobs[0].f, queues_of(ctx), d_albedo, d_normal);
+    GT_HIP(hipGetLastError());
   }
   return GT_OK;
 }
@@ -1054,20 +1196,50 @@ int gt_render_aovs(gt_context* ctx, const gt_scene* scene, const gt_camera* came
   return GT_OK;
 }
 
-int gt_render_batch(gt_context* ctx, size_t n, const gt_scene* const* scenes, const gt_camera* cameras,
-                    const gt_lights* const* lights, float* const* out_rgb, gt_render_stats* stats) {
-  if (!ctx || (n && (!scenes || !cameras || !out_rgb))) return fail(GT_EINVAL, "null argument");
-  if (stats) std::memset(stats, 0, sizeof *stats);
+int gt_render_batch_device(gt_context* ctx, size_t n, const gt_scene* const* scenes, const gt_camera* cameras,
+                           const gt_lights* const* lights, const gt_env* const* envs, void* const* d_out,
+                           int out_is_double, gt_render_stats* stats) {
+  if (!ctx) return fail(GT_EINVAL, "context is null");
+  if (n == 0) {
+    if (stats) std::memset(stats, 0, sizeof *stats);
+    return GT_OK;
+  }
+  if (!scenes || !cameras || !d_out) return fail(GT_EINVAL, "null argument");
+  if (n > (size_t(1) << 20)) return fail(GT_EINVAL, "too many frames in one batch");
+  GT_HIP(hipSetDevice(ctx->device));
+  std::vector<FrameJob> jobs(n);
   for (size_t i = 0; i < n; ++i) {
-    gt_render_stats s;
-    int rc = gt_render_hdr(ctx, scenes[i], &cameras[i], lights ? lights[i] : nullptr, nullptr, out_rgb[i],
-                           stats ? &s : nullptr);
+    int rc = prepare_job(ctx, scenes[i], &cameras[i], lights ? lights[i] : nullptr, envs ? envs[i] : nullptr, &jobs[i]);
     if (rc != GT_OK) return rc;
-    if (stats) {
-      stats->camera_paths += s.camera_paths; stats->extension_rays += s.extension_rays;
-      stats->shadow_rays += s.shadow_rays; stats->device_ms += s.device_ms; stats->extend_ms += s.extend_ms;
-      stats->extend_launches += s.extend_launches; stats->shade_ms += s.shade_ms; stats->connect_ms += s.connect_ms;
-    }
+    jobs[i].d_out = d_out[i];
+  }
+  return run_jobs(ctx, jobs, true, out_is_double, stats);
+}
+
+int gt_render_batch(gt_context* ctx, size_t n, const gt_scene* const* scenes, const gt_camera* cameras,
+                    const gt_lights* const* lights, const gt_env* const* envs, float* const* out_rgb,
+                    gt_render_stats* stats) {
+  if (!ctx) return fail(GT_EINVAL, "context is null");
+  if (n && (!cameras || !out_rgb)) return fail(GT_EINVAL, "null argument");
+  GT_HIP(hipSetDevice(ctx->device));
+  size_t total = 0;
+  std::vector<size_t> offs(n);
+  for (size_t i = 0; i < n; ++i) {
+    offs[i] = total;
+    const uint64_t px = uint64_t(cameras[i].image_width) * gt_camera_image_height(&cameras[i]);
+    if (px == 0 || px >= (uint64_t(1) << 31)) return fail(GT_EINVAL, "image size out of range");
+    total += size_t(px) * 3;
+  }
+  DevBuf<float> tmp;
+  GT_HIP(tmp.ensure(std::max<size_t>(total, 1)));
+  std::vector<void*> d_out(n);
+  for (size_t i = 0; i < n; ++i) d_out[i] = tmp.p + offs[i];
+  int rc = gt_render_batch_device(ctx, n, scenes, cameras, lights, envs, d_out.data(), 0, stats);
+  if (rc != GT_OK) return rc;
+  GT_HIP(hipStreamSynchronize(ctx->stream));
+  for (size_t i = 0; i < n; ++i) {
+    const size_t len = (i + 1 < n ? offs[i + 1] : total) - offs[i];
+    GT_HIP(hipMemcpy(out_rgb[i], tmp.p + offs[i], len * sizeof(float), hipMemcpyDeviceToHost));
   }
   return GT_OK;
 }

@@ -64,6 +64,8 @@ struct Queues {
   uint32_t stack_cap;  // LDS entries per lane
 };
 
+// Everything one render of one scene reads: scene, lights, environment,
+// camera, and the FP64 per-pixel accumulator it writes.
 struct Frame {
   gtd::DScene scene;
   gtd::DLights lights;
@@ -73,6 +75,25 @@ struct Frame {
   uint32_t has_lights;
   float light_weight;      // mixture weight in continuation mode
   uint32_t npix;
+  double4* acc;            // per-pixel sums (rgb, accepted count)
+};
+
+// A batch of paths is a list of segments: segment i renders samples
+// [sample0, sample0 + nsamples) of every pixel of frame `frame` as paths
+// [path0, path0 + nsamples * npix), sample-major. Its pixels are accumulate
+// work items [pix0, pix0 + npix). Segments are sorted by path0 and pix0, and
+// a frame has at most one segment per batch.
+struct Seg {
+  uint32_t path0, frame, sample0, nsamples, pix0;
+  uint32_t pad[3];
+};
+struct Batch {
+  const Frame* frames;  // batched renders: frame table in device memory
+  const Seg* segs;      // batched renders: segment table in device memory
+  uint32_t nseg;
+  uint32_t total;       // paths in the batch
+  uint32_t npix_total;  // accumulate work items
+  Seg seg0;             // single-frame renders: the one segment (frame 0)
 };
 
 struct Launch {
@@ -80,13 +101,23 @@ struct Launch {
   hipStream_t stream;
 };
 
+// What the wavefront kernels of one batch read: for a single-frame render the
+// frame itself (multi = false, passed by value); for a batched render the
+// frame and segment tables in bt (multi = true).
+struct Work {
+  Frame f;
+  Batch bt;
+  bool multi;
+};
+
 void reset_counters(const Launch& l, const Queues& q, uint32_t depth);
-void raygen(const Launch& l, const Frame& f, const Queues& q, uint32_t sample0, uint32_t nsamples);
-void extend(const Launch& l, const Frame& f, const Queues& q, uint32_t bounce, bool count);
+void zero_acc(const Launch& l, double4* acc, size_t n);
+void raygen(const Launch& l, const Work& w, const Queues& q);
+void extend(const Launch& l, const Work& w, const Queues& q, uint32_t bounce, bool count);
 // One bounce of shading: one kernel per category (empty categories exit at once).
-void shade(const Launch& l, const Frame& f, const Queues& q, uint32_t bounce, uint32_t sample0, uint32_t last);
-void connect(const Launch& l, const Frame& f, const Queues& q, uint32_t bounce, uint32_t kind, bool count);
-void accumulate(const Launch& l, const Frame& f, const Queues& q, double4* acc, uint32_t nsamples);
+void shade(const Launch& l, const Work& w, const Queues& q, uint32_t bounce);
+void connect(const Launch& l, const Work& w, const Queues& q, uint32_t bounce, uint32_t kind, bool count);
+void accumulate(const Launch& l, const Work& w, const Queues& q);
 void finalize(const Launch& l, const double4* acc, uint32_t npix, void* out, int out_is_double);
 void aov(const Launch& l, const Frame& f, const Queues& q, float* albedo, float* normal);
 

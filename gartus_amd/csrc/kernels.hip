@@ -72,12 +72,54 @@ __device__ __forceinline__ Stack make_stack(const Queues& q, uint2* smem) {
   return st;
 }
 
-__device__ __forceinline__ void path_coords(const Frame& f, uint32_t path, uint32_t sample0, uint32_t* x, uint32_t* y,
+// Pixel and sample of a path: a segment's paths are sample-major,
+// path = path0 + (sample - sample0) * npix + pixel.
+__device__ __forceinline__ void path_coords(const Frame& f, const Seg& sg, uint32_t path, uint32_t* x, uint32_t* y,
                                             uint32_t* sample) {
-  uint32_t pixel = path % f.npix;
-  *sample = sample0 + path / f.npix;
+  const uint32_t local = path - sg.path0;
+  const uint32_t pixel = local % f.npix;
+  *sample = sg.sample0 + local / f.npix;
   *x = pixel % f.cam.width;
   *y = pixel / f.cam.width;
+}
+
+// Index of the segment holding `key` in a table sorted by `field`
+// (the last entry whose field is <= key).
+template <uint32_t Seg::*FIELD>
+__device__ __forceinline__ uint32_t find_seg(const Batch& bt, uint32_t key) {
+  uint32_t lo = 0, n = bt.nseg;
+  while (n > 1) {
+    const uint32_t half = n >> 1;
+    if (bt.segs[lo + half].*FIELD <= key) lo += half;
+    n -= half;
+  }
+  return lo;
+}
+
+// Runs body(frame, segment) for a lane's path with the frame wave-uniform.
+// A single-frame render passes its frame and segment as kernel arguments. A
+// batched render looks the segment up and loops once per distinct segment
+// among the wave's lanes (a "waterfall"), so the frame is always read at a
+// uniform address; segments are contiguous in path id, so a wave normally
+// sees one.
+template <bool MULTI, uint32_t Seg::*FIELD = &Seg::path0, typename Body>
+__device__ __forceinline__ void with_frame(const Frame& f0, const Batch& bt, bool valid, uint32_t key, Body&& body) {
+  if constexpr (!MULTI) {
+    if (valid) body(f0, bt.seg0);
+  } else {
+    const uint32_t mine = valid ? find_seg<FIELD>(bt, key) : 0u;
+    bool pending = valid;
+    while (__ballot(pending)) {
+      if (pending) {
+        const uint32_t u = __builtin_amdgcn_readfirstlane(mine);
+        if (mine == u) {
+          const Seg sg = bt.segs[u];
+          body(bt.frames[sg.frame], sg);
+          pending = false;
+        }
+      }
+    }
+  }
 }
 
 // Russian roulette (camera.rs:2498-2523)
@@ -134,25 +176,31 @@ __device__ __forceinline__ uint32_t node_fetches(const Stats& s, uint32_t width)
 
 // Camera rays for samples [sample0, sample0 + nsamples) of every pixel
 // (render_world_linear_pixel_with_context camera.rs:2660-2723; rays :1658-1718).
-__global__ void k_raygen(Frame f, Queues q, uint32_t sample0, uint32_t nsamples) {
-  const uint32_t total = f.npix * nsamples;
+template <bool MULTI>
+__global__ void k_raygen(Frame f0, Batch bt, Queues q) {
+  const uint32_t total = bt.total;
   if (blockIdx.x == 0 && threadIdx.x < kSlices) {  // slice s starts extend(0) at path total * s / kSlices
     const uint32_t s = threadIdx.x;
     const uint32_t lo = uint32_t(uint64_t(total) * s / kSlices), hi = uint32_t(uint64_t(total) * (s + 1) / kSlices);
     q.ctr[ctr_at(0, S_BASE, s)] = lo;
     q.ctr[ctr_at(0, S_QCOUNT, s)] = hi - lo;
   }
-  for (uint32_t p = blockIdx.x * blockDim.x + threadIdx.x; p < total; p += gridDim.x * blockDim.x) {
-    uint32_t x, y, sample;
-    path_coords(f, p, sample0, &x, &y, &sample);
-    Rng rng{stream_seed(pixel_seed(f.cam.seed, x, y), sample, 0)};
-    f3 o, d;
-    float time;
-    camera_ray(f.cam, x, y, sample, rng, &o, &d, &time);
-    q.ray_o[p] = make_float4(o.x, o.y, o.z, time);
-    q.ray_d[p] = make_float4(d.x, d.y, d.z, 0.f);
-    q.thr[p] = make_float4(1.f, 1.f, 1.f, 1.f);
-    q.radiance[p] = make_float4(0.f, 0.f, 0.f, 0.f);
+  const uint32_t stride = gridDim.x * blockDim.x;
+  const uint32_t rounds = (total + stride - 1) / stride;  // wave-uniform trip count
+  for (uint32_t r = 0; r < rounds; ++r) {
+    const uint32_t p = r * stride + blockIdx.x * blockDim.x + threadIdx.x;
+    with_frame<MULTI>(f0, bt, p < total, p, [&](const Frame& f, const Seg& sg) {
+      uint32_t x, y, sample;
+      path_coords(f, sg, p, &x, &y, &sample);
+      Rng rng{stream_seed(pixel_seed(f.cam.seed, x, y), sample, 0)};
+      f3 o, d;
+      float time;
+      camera_ray(f.cam, x, y, sample, rng, &o, &d, &time);
+      q.ray_o[p] = make_float4(o.x, o.y, o.z, time);
+      q.ray_d[p] = make_float4(d.x, d.y, d.z, 0.f);
+      q.thr[p] = make_float4(1.f, 1.f, 1.f, 1.f);
+      q.radiance[p] = make_float4(0.f, 0.f, 0.f, 0.f);
+    });
   }
 }
 
@@ -167,8 +215,8 @@ __device__ __forceinline__ uint32_t category_of(uint32_t meta) {
   }
 }
 
-template <bool COUNT>
-__global__ void k_extend(Frame f, Queues q, uint32_t bounce) {
+template <bool COUNT, bool MULTI>
+__global__ void k_extend(Frame f0, Batch bt, Queues q, uint32_t bounce) {
   extern __shared__ uint2 smem[];
   Stack st = make_stack(q, smem);
   for (uint32_t s = blockIdx.x % slice_step(); s < kSlices; s += slice_step()) {
@@ -186,9 +234,10 @@ __global__ void k_extend(Frame f, Queues q, uint32_t bounce) {
       uint32_t j = base + k * 64u + lane_id();
       cats[k] = kNumCategories;
       ids[k] = 0;
-      if (j < count) {
-        uint32_t p = bounce == 0 ? first + j : active[j];
-        ids[k] = p;
+      const bool valid = j < count;
+      const uint32_t p = !valid ? 0u : bounce == 0 ? first + j : active[j];
+      ids[k] = p;
+      with_frame<MULTI>(f0, bt, valid, p, [&](const Frame& f, const Seg&) {
         float4 o4 = q.ray_o[p], d4 = q.ray_d[p];
         float t;
         uint32_t prim, meta = 0;
@@ -198,7 +247,7 @@ __global__ void k_extend(Frame f, Queues q, uint32_t bounce) {
         q.hits[p] = hit ? make_float2(t, __uint_as_float(prim)) : make_float2(0.f, __uint_as_float(kMiss));
         cats[k] = hit ? category_of(meta) : uint32_t(CAT_MISS);
         if (COUNT) { nodes += node_fetches(s, f.scene.width); prims += s.candidates; }
-      }
+      });
     }
 #pragma unroll
     for (uint32_t c = 0; c < kNumCategories; ++c) {
@@ -356,9 +405,8 @@ __device__ __forceinline__ uint32_t shade_specular(const Frame& f, const Queues&
   return 0;
 }
 
-template <int CAT>
-__global__ void k_shade(Frame f, Queues q, uint32_t bounce, uint32_t sample0, uint32_t last) {
-  const bool has_env = f.env.kind != 0;
+template <int CAT, bool MULTI>
+__global__ void k_shade(Frame f0, Batch bt, Queues q, uint32_t bounce) {
   constexpr bool kScatters = CAT == CAT_LAMBERT || CAT == CAT_GGX || CAT == CAT_SPECULAR;
   for (uint32_t s = blockIdx.x % slice_step(); s < kSlices; s += slice_step()) {
   const uint32_t count = q.ctr[ctr_at(bounce, S_CAT_COUNT + CAT, s)];
@@ -371,38 +419,42 @@ __global__ void k_shade(Frame f, Queues q, uint32_t bounce, uint32_t sample0, ui
     uint32_t ids[kChunk], outs[kChunk];
 #pragma unroll
     for (uint32_t k = 0; k < kChunk; ++k) {
-      uint32_t j = base + k * 64u + lane_id();
-      ids[k] = 0;
-      outs[k] = 0;
-      if (j >= count) continue;
-      const uint32_t p = list[j];
+      const uint32_t j = base + k * 64u + lane_id();
+      const bool valid = j < count;
+      const uint32_t p = valid ? list[j] : 0u;
       ids[k] = p;
-      const Item it = load_item(q, p);
-      if (CAT == CAT_MISS) {  // camera.rs:1747-1750: background or environment, times the MIS weight
-        f3 miss = (has_env ? env_radiance(f.env, it.d) : background(f.cam, it.d)) * it.mw;
-        float4 L = q.radiance[p];
-        q.radiance[p] = make_float4(L.x + it.T.x * miss.x, L.y + it.T.y * miss.y, L.z + it.T.z * miss.z, 0.f);
-        continue;
-      }
-      f3 pt = it.o + it.t * it.d;
-      uint32_t mid, kind;
-      f3 outward = prim_outward_normal(f.scene.prims, it.prim, pt, it.time, &mid, &kind);
-      bool front = dot(it.d, outward) < 0.f;  // object.rs:127-149
-      f3 n = front ? outward : -outward;
-      const GMat m = f.scene.mats[mid];
-      if (CAT == CAT_EMIT) {  // emitted_at camera.rs:1753-1755, DiffuseLight material.rs:1009-1022
-        if (!it.after_scatter && front) {
-          float4 L = q.radiance[p];
-          q.radiance[p] = make_float4(L.x + it.T.x * m.color[0], L.y + it.T.y * m.color[1],
-                                      L.z + it.T.z * m.color[2], 0.f);
+      outs[k] = 0;
+      with_frame<MULTI>(f0, bt, valid, p, [&](const Frame& f, const Seg& sg) {
+        if (CAT == CAT_MISS) {  // camera.rs:1747-1750: background or environment, times the MIS weight
+          const float4 d4 = q.ray_d[p], t4 = q.thr[p];
+          const f3 d = mk(d4.x, d4.y, d4.z);
+          const f3 miss = (f.env.kind != 0 ? env_radiance(f.env, d) : background(f.cam, d)) * t4.w;
+          const float4 L = q.radiance[p];
+          q.radiance[p] = make_float4(L.x + t4.x * miss.x, L.y + t4.y * miss.y, L.z + t4.z * miss.z, 0.f);
+          return;
         }
-        continue;
-      }
-      uint32_t x, y, sample;
-      path_coords(f, p, sample0, &x, &y, &sample);
-      Rng rng{stream_seed(pixel_seed(f.cam.seed, x, y), sample, 1 + bounce)};
-      if (CAT == CAT_SPECULAR) outs[k] = shade_specular(f, q, it, m, pt, n, front, rng, last != 0, bounce);
-      else outs[k] = shade_scatter<CAT == CAT_GGX>(f, q, it, m, pt, n, rng, last != 0, bounce);
+        const Item it = load_item(q, p);
+        f3 pt = it.o + it.t * it.d;
+        uint32_t mid, kind;
+        f3 outward = prim_outward_normal(f.scene.prims, it.prim, pt, it.time, &mid, &kind);
+        bool front = dot(it.d, outward) < 0.f;  // object.rs:127-149
+        f3 n = front ? outward : -outward;
+        const GMat m = f.scene.mats[mid];
+        if (CAT == CAT_EMIT) {  // emitted_at camera.rs:1753-1755, DiffuseLight material.rs:1009-1022
+          if (!it.after_scatter && front) {
+            float4 L = q.radiance[p];
+            q.radiance[p] = make_float4(L.x + it.T.x * m.color[0], L.y + it.T.y * m.color[1],
+                                        L.z + it.T.z * m.color[2], 0.f);
+          }
+          return;
+        }
+        uint32_t x, y, sample;
+        path_coords(f, sg, p, &x, &y, &sample);
+        Rng rng{stream_seed(pixel_seed(f.cam.seed, x, y), sample, 1 + bounce)};
+        const bool last = bounce + 1 >= f.cam.max_depth;  // the camera's final bounce
+        if (CAT == CAT_SPECULAR) outs[k] = shade_specular(f, q, it, m, pt, n, front, rng, last, bounce);
+        else outs[k] = shade_scatter<CAT == CAT_GGX>(f, q, it, m, pt, n, rng, last, bounce);
+      });
     }
     if (kScatters) {
       uint32_t bits_ext = 0, bits_sl = 0, bits_se = 0;
@@ -423,8 +475,8 @@ __global__ void k_shade(Frame f, Queues q, uint32_t bounce, uint32_t sample0, ui
 // Shadow rays of one kind. Light targets: closest hit, then the emission of
 // whatever was hit (camera.rs:2248-2261). Environment: any hit occludes
 // (camera.rs:2284-2293; "is some hit" equals "is some closest hit").
-template <bool COUNT>
-__global__ void k_connect(Frame f, Queues q, uint32_t bounce, uint32_t kind) {
+template <bool COUNT, bool MULTI>
+__global__ void k_connect(Frame f0, Batch bt, Queues q, uint32_t bounce, uint32_t kind) {
   extern __shared__ uint2 smem[];
   Stack st = make_stack(q, smem);
   const float4* __restrict__ so = q.s_o[kind];
@@ -440,9 +492,10 @@ __global__ void k_connect(Frame f, Queues q, uint32_t bounce, uint32_t kind) {
     if (base >= count) break;
     uint32_t nodes = 0, prims = 0;
     for (uint32_t k = 0; k < kChunk; ++k) {
-      uint32_t j = base + k * 64u + lane_id();
-      if (j >= count) break;
-      const uint32_t p = list[j];
+      const uint32_t j = base + k * 64u + lane_id();
+      const bool valid = j < count;
+      const uint32_t p = valid ? list[j] : 0u;
+      with_frame<MULTI>(f0, bt, valid, p, [&](const Frame& f, const Seg&) {
       Stats s{0, 0, 0, 0, 0, 0};
       float4 o4 = so[p], d4 = sd[p], w4 = sw[p];
       f3 o = mk(o4.x, o4.y, o4.z), d = mk(d4.x, d4.y, d4.z);
@@ -472,6 +525,7 @@ __global__ void k_connect(Frame f, Queues q, uint32_t bounce, uint32_t kind) {
         q.radiance[p] = make_float4(L.x + add.x, L.y + add.y, L.z + add.z, 0.f);
       }
       if (COUNT) { nodes += node_fetches(s, f.scene.width); prims += s.candidates; }
+      });
     }
     if (COUNT) {
       wave_count(count_slot(q, bounce, 2), nodes);
@@ -483,26 +537,37 @@ __global__ void k_connect(Frame f, Queues q, uint32_t bounce, uint32_t kind) {
 
 // Adds each finished sample of the batch to its pixel in sample order,
 // dropping non-finite samples (add_finite_sample camera.rs:2984-2991).
-__global__ void k_accumulate(Frame f, Queues q, double4* acc, uint32_t nsamples) {
+// Work item w is pixel w - pix0 of the segment holding w; a frame has at most
+// one segment per batch, so each accumulator is updated by one thread.
+template <bool MULTI>
+__global__ void k_accumulate(Frame f0, Batch bt, Queues q) {
   const uint32_t stride = gridDim.x * blockDim.x;
-  const uint32_t rounds = (f.npix + stride - 1) / stride;  // wave-uniform trip count
+  const uint32_t rounds = (bt.npix_total + stride - 1) / stride;  // wave-uniform trip count
   uint32_t accepted = 0;
   for (uint32_t r = 0; r < rounds; ++r) {
-    uint32_t px = r * stride + blockIdx.x * blockDim.x + threadIdx.x;
-    if (px < f.npix) {
-      double4 a = acc[px];
-      for (uint32_t s = 0; s < nsamples; ++s) {
-        float4 L = q.radiance[size_t(s) * f.npix + px];
+    const uint32_t w = r * stride + blockIdx.x * blockDim.x + threadIdx.x;
+    with_frame<MULTI, &Seg::pix0>(f0, bt, w < bt.npix_total, w, [&](const Frame& f, const Seg& sg) {
+      const uint32_t px = w - sg.pix0;
+      double4 a = f.acc[px];
+      for (uint32_t s = 0; s < sg.nsamples; ++s) {
+        float4 L = q.radiance[sg.path0 + size_t(s) * f.npix + px];
         if (isfinite(L.x) && isfinite(L.y) && isfinite(L.z)) {
           a.x += double(L.x); a.y += double(L.y); a.z += double(L.z); a.w += 1.0;
           ++accepted;
         }
       }
-      acc[px] = a;
-    }
+      f.acc[px] = a;
+    });
   }
   for (int off = 32; off > 0; off >>= 1) accepted += __shfl_down(accepted, off);
   if (lane_id() == 0 && accepted) atomicAdd(q.tally, (unsigned long long)accepted);
+}
+
+// Clears accumulators with a kernel (no memset node, so the sequence can be
+// captured in a graph and replayed).
+__global__ void k_zero(double4* acc, size_t n) {
+  for (size_t i = blockIdx.x * size_t(blockDim.x) + threadIdx.x; i < n; i += size_t(gridDim.x) * blockDim.x)
+    acc[i] = make_double4(0.0, 0.0, 0.0, 0.0);
 }
 
 // average_accepted_samples (camera.rs:2993-2999)
@@ -605,9 +670,9 @@ __global__ void k_intersect(Frame f, Queues q, const float4* rays, size_t n, flo
 uint32_t occupancy_blocks(int kernel, uint32_t block, size_t lds) {
   int nb = 0;
   hipError_t e = hipSuccess;
-  if (kernel == 0) e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_extend<false>, int(block), lds);
-  else if (kernel == 1) e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_shade<CAT_GGX>, int(block), 0);
-  else e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_connect<false>, int(block), lds);
+  if (kernel == 0) e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_extend<false, false>, int(block), lds);
+  else if (kernel == 1) e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_shade<CAT_GGX, false>, int(block), 0);
+  else e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_connect<false, false>, int(block), lds);
   if (e != hipSuccess || nb <= 0) return 1;
   return uint32_t(nb);
 }
@@ -617,34 +682,50 @@ void reset_counters(const Launch& l, const Queues& q, uint32_t depth) {
   hipLaunchKernelGGL(k_reset, dim3((n + 255) / 256), dim3(256), 0, l.stream, q.ctr, n, q.tally,
                      uint32_t(tally_words(depth)));
 }
-void raygen(const Launch& l, const Frame& f, const Queues& q, uint32_t sample0, uint32_t nsamples) {
-  hipLaunchKernelGGL(k_raygen, dim3(l.grid), dim3(l.block), 0, l.stream, f, q, sample0, nsamples);
+void zero_acc(const Launch& l, double4* acc, size_t n) {
+  hipLaunchKernelGGL(k_zero, dim3(l.grid), dim3(l.block), 0, l.stream, acc, n);
 }
-void extend(const Launch& l, const Frame& f, const Queues& q, uint32_t bounce, bool count) {
-  if (count)
-    hipLaunchKernelGGL(k_extend<true>, dim3(l.grid), dim3(l.block), stack_lds_bytes(l.block, q.stack_cap), l.stream,
-                       f, q, bounce);
-  else
-    hipLaunchKernelGGL(k_extend<false>, dim3(l.grid), dim3(l.block), stack_lds_bytes(l.block, q.stack_cap), l.stream,
-                       f, q, bounce);
+void raygen(const Launch& l, const Work& w, const Queues& q) {
+  if (w.multi) hipLaunchKernelGGL(k_raygen<true>, dim3(l.grid), dim3(l.block), 0, l.stream, w.f, w.bt, q);
+  else hipLaunchKernelGGL(k_raygen<false>, dim3(l.grid), dim3(l.block), 0, l.stream, w.f, w.bt, q);
 }
-void shade(const Launch& l, const Frame& f, const Queues& q, uint32_t bounce, uint32_t sample0, uint32_t last) {
-  hipLaunchKernelGGL(k_shade<CAT_MISS>, dim3(l.grid), dim3(l.block), 0, l.stream, f, q, bounce, sample0, last);
-  hipLaunchKernelGGL(k_shade<CAT_EMIT>, dim3(l.grid), dim3(l.block), 0, l.stream, f, q, bounce, sample0, last);
-  hipLaunchKernelGGL(k_shade<CAT_LAMBERT>, dim3(l.grid), dim3(l.block), 0, l.stream, f, q, bounce, sample0, last);
-  hipLaunchKernelGGL(k_shade<CAT_GGX>, dim3(l.grid), dim3(l.block), 0, l.stream, f, q, bounce, sample0, last);
-  hipLaunchKernelGGL(k_shade<CAT_SPECULAR>, dim3(l.grid), dim3(l.block), 0, l.stream, f, q, bounce, sample0, last);
+void extend(const Launch& l, const Work& w, const Queues& q, uint32_t bounce, bool count) {
+  const size_t lds = stack_lds_bytes(l.block, q.stack_cap);
+#define GT_EXTEND(C, M) \
+  hipLaunchKernelGGL((k_extend<C, M>), dim3(l.grid), dim3(l.block), lds, l.stream, w.f, w.bt, q, bounce)
+  if (w.multi) {
+    if (count) GT_EXTEND(true, true); else GT_EXTEND(false, true);
+  } else {
+    if (count) GT_EXTEND(true, false); else GT_EXTEND(false, false);
+  }
+#undef GT_EXTEND
 }
-void connect(const Launch& l, const Frame& f, const Queues& q, uint32_t bounce, uint32_t kind, bool count) {
-  if (count)
-    hipLaunchKernelGGL(k_connect<true>, dim3(l.grid), dim3(l.block), stack_lds_bytes(l.block, q.stack_cap), l.stream,
-                       f, q, bounce, kind);
-  else
-    hipLaunchKernelGGL(k_connect<false>, dim3(l.grid), dim3(l.block), stack_lds_bytes(l.block, q.stack_cap), l.stream,
-                       f, q, bounce, kind);
+template <bool M>
+static void shade_all(const Launch& l, const Work& w, const Queues& q, uint32_t bounce) {
+  hipLaunchKernelGGL((k_shade<CAT_MISS, M>), dim3(l.grid), dim3(l.block), 0, l.stream, w.f, w.bt, q, bounce);
+  hipLaunchKernelGGL((k_shade<CAT_EMIT, M>), dim3(l.grid), dim3(l.block), 0, l.stream, w.f, w.bt, q, bounce);
+  hipLaunchKernelGGL((k_shade<CAT_LAMBERT, M>), dim3(l.grid), dim3(l.block), 0, l.stream, w.f, w.bt, q, bounce);
+  hipLaunchKernelGGL((k_shade<CAT_GGX, M>), dim3(l.grid), dim3(l.block), 0, l.stream, w.f, w.bt, q, bounce);
+  hipLaunchKernelGGL((k_shade<CAT_SPECULAR, M>), dim3(l.grid), dim3(l.block), 0, l.stream, w.f, w.bt, q, bounce);
 }
-void accumulate(const Launch& l, const Frame& f, const Queues& q, double4* acc, uint32_t nsamples) {
-  hipLaunchKernelGGL(k_accumulate, dim3(l.grid), dim3(l.block), 0, l.stream, f, q, acc, nsamples);
+void shade(const Launch& l, const Work& w, const Queues& q, uint32_t bounce) {
+  if (w.multi) shade_all<true>(l, w, q, bounce);
+  else shade_all<false>(l, w, q, bounce);
+}
+void connect(const Launch& l, const Work& w, const Queues& q, uint32_t bounce, uint32_t kind, bool count) {
+  const size_t lds = stack_lds_bytes(l.block, q.stack_cap);
+#define GT_CONNECT(C, M) \
+  hipLaunchKernelGGL((k_connect<C, M>), dim3(l.grid), dim3(l.block), lds, l.stream, w.f, w.bt, q, bounce, kind)
+  if (w.multi) {
+    if (count) GT_CONNECT(true, true); else GT_CONNECT(false, true);
+  } else {
+    if (count) GT_CONNECT(true, false); else GT_CONNECT(false, false);
+  }
+#undef GT_CONNECT
+}
+void accumulate(const Launch& l, const Work& w, const Queues& q) {
+  if (w.multi) hipLaunchKernelGGL(k_accumulate<true>, dim3(l.grid), dim3(l.block), 0, l.stream, w.f, w.bt, q);
+  else hipLaunchKernelGGL(k_accumulate<false>, dim3(l.grid), dim3(l.block), 0, l.stream, w.f, w.bt, q);
 }
 void finalize(const Launch& l, const double4* acc, uint32_t npix, void* out, int out_is_double) {
   hipLaunchKernelGGL(k_finalize, dim3(l.grid), dim3(l.block), 0, l.stream, acc, npix, out, out_is_double);

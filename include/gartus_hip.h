@@ -242,12 +242,24 @@ int gt_render_aovs_device(gt_context* ctx, const gt_scene* scene, const gt_camer
                           float* d_beauty, float* d_albedo, float* d_normal,
                           gt_render_stats* out_stats);
 
-/* ---- batched rendering: many scenes and cameras, one persistent launch
- * sequence. No reference counterpart (the reference renders one scene per
- * call); each result equals gt_render_hdr of that scene alone. */
+/* ---- batched rendering: many scenes and cameras in one wavefront ------
+ * No reference counterpart: the reference renders one scene per call, and a
+ * caller loops over PathTracer::render_* (renderer.rs:101-224,374-609). Here
+ * the samples of all n frames share each launch: every batch of paths mixes
+ * frames, and each kernel looks up its path's frame. Each result equals
+ * gt_render_hdr of that frame alone (bit for bit: same streams, same
+ * accumulation order). lights and envs are arrays of n entries (entries and
+ * the arrays themselves may be NULL). out_rgb[i] is a host buffer of
+ * W_i*H_i*3 floats; the _device form takes device pointers and leaves the
+ * results on the context's stream. */
 int gt_render_batch(gt_context* ctx, size_t n, const gt_scene* const* scenes,
                     const gt_camera* cameras, const gt_lights* const* lights,
-                    float* const* out_rgb, gt_render_stats* out_stats);
+                    const gt_env* const* envs, float* const* out_rgb,
+                    gt_render_stats* out_stats);
+int gt_render_batch_device(gt_context* ctx, size_t n, const gt_scene* const* scenes,
+                           const gt_camera* cameras, const gt_lights* const* lights,
+                           const gt_env* const* envs, void* const* d_out, int out_is_double,
+                           gt_render_stats* out_stats);
 
 #ifdef __cplusplus
 }

@@ -1,0 +1,102 @@
+"""Batched rendering (gt_render_batch): many frames share each wavefront launch.
+
+Every path draws from its own (pixel, sample, bounce) streams and each pixel
+sums its samples in sample order in FP64, so a frame rendered inside a batch
+must equal the same frame rendered alone bit for bit, whatever the batch
+boundaries and whatever other frames share the launches. The frames below
+differ in scene, primitive kinds, materials, resolution, spp, depth, lights,
+environment and lighting mode.
+"""
+import sys
+
+import numpy as np
+import pytest
+
+from gartus_amd import abi, api, scenes
+
+pytestmark = pytest.mark.gpu
+
+
+def say(msg: str) -> None:
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+def _camera(spec):
+    cam = api.RayCamera.__new__(api.RayCamera)
+    cam.raw = scenes.camera_record(spec)
+    return cam
+
+
+def _frames():
+    frames = []
+    data, lights, cam = scenes.cornell()
+    cam.update(width=48, grid=2)
+    L = api.SamplingTargetList()
+    for kind, q, w in lights:
+        L._add(kind, q, w)
+    frames.append((api.RayScene(data).with_bvh(), _camera(cam), L, None))
+
+    data, cam = scenes.bench_scene()
+    cam["spp"] = 3
+    frames.append((api.RayScene(data).with_bvh(), _camera(cam), None, None))
+
+    data, env, cam = scenes.ggx_environment(n_random=60, env_size=(128, 64))
+    cam.update(width=64, spp=2, depth=6)
+    c = _camera(cam).with_direct_lighting_mode(abi.GT_LIGHTING_NEE)
+    frames.append((api.RayScene(data).with_bvh(), c, None, api.EnvironmentLight.from_rgb32f(env)))
+
+    data = scenes.randomized_ray_scene(5)
+    spec = dict(width=40, aspect=4 / 3, spp=5, depth=4, vfov=60.0, lookfrom=(0.0, 0.0, 2.0),
+                lookat=(0.0, 0.0, -4.0), background=(0.6, 0.7, 0.9), rr=2)
+    frames.append((api.RayScene(data).with_bvh(), _camera(spec), None, None))
+    return frames
+
+
+def _singles(frames):
+    out = []
+    for world, cam, L, env in frames:
+        out.append(api.PathTracer(cam)._render(world, cam, L, env).pixels.copy())
+    return out
+
+
+def test_batch_equals_single_renders(gpu_context):
+    say("batch: 4 frames (cornell, bench, ggx env, randomized) vs single renders")
+    frames = _frames()
+    singles = _singles(frames)
+    images, st = api.render_batch(frames, stats=True)
+    for i, (a, b) in enumerate(zip(images, singles)):
+        assert a.pixels.shape == b.shape
+        assert np.isfinite(a.pixels).all()
+        np.testing.assert_array_equal(a.pixels, b, err_msg=f"frame {i}")
+    expect_paths = sum(c.image_width() * c.image_height() * c.effective_samples_per_pixel() for _, c, _, _ in frames)
+    assert st["camera_paths"] == expect_paths
+    assert st["accepted_samples"] == expect_paths
+    assert st["extension_rays"] >= expect_paths
+
+
+def test_batch_split_over_small_queues(gpu_context):
+    """A queue of 5,000 paths forces every frame's samples over several
+    batches, and some batches mix frames; the images must not change."""
+    say("batch: max_paths 5000")
+    frames = _frames()
+    singles = _singles(frames)
+    old_max = gpu_context.tuning.max_paths
+    try:
+        gpu_context.set_tuning(max_paths=5000)
+        images = api.render_batch(frames)
+    finally:
+        gpu_context.set_tuning(max_paths=old_max)
+    for i, (a, b) in enumerate(zip(images, singles)):
+        np.testing.assert_array_equal(a.pixels, b, err_msg=f"frame {i}")
+
+
+def test_batch_rejects_mixed_contexts(gpu_context):
+    other = api.Context(0)
+    data = scenes.randomized_ray_scene(1)
+    a = api.RayScene(data).with_bvh()
+    b = api.RayScene(data, other).with_bvh()
+    cam = _camera(dict(width=8, aspect=1.0, spp=1, depth=2, vfov=60.0, lookfrom=(0.0, 0.0, 2.0),
+                       lookat=(0.0, 0.0, -4.0)))
+    with pytest.raises(abi.InvalidArgument):
+        api.render_batch([(a, cam, None, None), (b, cam, None, None)])
