@@ -1,19 +1,25 @@
 #!/usr/bin/env python3
 """Benchmark of the MI355X path tracer (one GPU).
 
-A step is one full render of the workload frame. The default workload is the
+A step is one full render of the workload. The default workload is the
 1920x1080 GGX/environment scene: 1,101 spheres (100 GGX microfacet spheres,
 roughness 0.02-1.0, metallic 0/1 by row, plus 1,000 random Dielectric, Metal
 and Lambertian spheres on a ground sphere), lit by a 2048x1024 FP32 HDR sky
 with a 0.5 degree sun of radiance 5e4, environment importance sampling with
 MIS, 16 spp, max depth 10, Russian roulette from bounce 5.
 
+Other workloads: cornell_256 (the Cornell box, 256x256, 64 spp stratified,
+depth 8, NEE), terrain_720p (524,288-triangle Perlin terrain + 2,000
+spheres, 1280x720, 32 spp, depth 6) and batch64 (64 seeded scenes of 1k-2M
+triangles at 256^2 to 1080p, 8 spp, depth 8, NEE to 1-8 quad lights each,
+all in one batched render with the tuning picked by the autotuner).
+
 value = (extension rays + shadow rays traced) / wall time of the K timed
-steps, in Mrays/s, with the scene, environment and output already resident
-in HBM. The JSON line also carries the roofline of the dominant kernel
-(extend, the closest-hit traversal) and a CPU baseline: the f64 CPU oracle
-(the reference's algorithm, reference RNG stream) on a bounded row sample of
-the same frame, timed on this host's cores.
+steps, in Mrays/s, with scenes, environment and outputs already resident in
+HBM. The JSON line also carries the roofline of the dominant kernel (extend,
+the closest-hit traversal) and a CPU baseline: the f64 CPU oracle (the
+reference's algorithm, reference RNG stream) on a bounded row sample of the
+first frame, timed on this host's cores.
 """
 from __future__ import annotations
 
@@ -23,6 +29,7 @@ import json
 import os
 import sys
 import time
+from concurrent.futures import ThreadPoolExecutor
 from pathlib import Path
 
 import numpy as np
@@ -34,49 +41,73 @@ HBM_PEAK_GBS = 8000.0  # MI355X HBM3E
 # HBM bytes per extend launch measured with rocprofv3 --pmc FETCH_SIZE and
 # WRITE_SIZE, scaled by the same counters on a plain copy of known size
 TRAFFIC_FILE = ROOT / "profiles" / "extend_hbm_traffic.json"
+WORKLOADS = ["ggx_env_1080p", "cornell_256", "terrain_720p", "batch64"]
 
 
 def log(msg: str) -> None:
     print(msg, file=sys.stderr, flush=True)
 
 
-def prim_bytes(data) -> float:
-    """Algorithmic bytes of one primitive test: the geometry plus a 4-byte
-    material id (triangle 36, quad 48, sphere 16, moving sphere 28)."""
+def prim_bytes(frames) -> float:
+    """Algorithmic bytes of one primitive test, averaged over the primitives:
+    the geometry plus a 4-byte material id (triangle 36, quad 48, sphere 16,
+    moving sphere 28)."""
     from gartus_amd.scene_data import MOVING_SPHERE, QUAD, SPHERE, TRIANGLE
     size = {SPHERE: 16 + 4, MOVING_SPHERE: 28 + 4, TRIANGLE: 36 + 4, QUAD: 48 + 4}
-    n = sum(len(d) for _, d, _ in data.chunks)
-    return sum(size[k] * len(d) for k, d, _ in data.chunks) / max(n, 1)
+    n = b = 0
+    for fr in frames:
+        for k, d, _ in fr["data"].chunks:
+            n += len(d)
+            b += size[k] * len(d)
+    return b / max(n, 1)
+
+
+def _lights(targets):
+    from gartus_amd import api
+    L = api.SamplingTargetList()
+    for k, q, w in targets:
+        L._add(k, q, w)
+    return L
 
 
 def build_workload(name: str):
+    """Returns (frames, description); a frame is a dict of data, lights,
+    env, env_rgb, cam (camera spec) and targets."""
     from gartus_amd import api, scenes
     if name == "ggx_env_1080p":
         data, env, cam = scenes.ggx_environment()
-        return data, None, api.EnvironmentLight.from_rgb32f(env), env, cam, \
-            "1920x1080 GGX/HDR-environment, 1101 spheres, 16 spp, depth 10"
+        cam["nee"] = True
+        return [dict(data=data, lights=None, env=api.EnvironmentLight.from_rgb32f(env), env_rgb=env, cam=cam,
+                     targets=None)], "1920x1080 GGX/HDR-environment, 1101 spheres, 16 spp, depth 10"
     if name == "cornell_256":
-        data, lights, cam = scenes.cornell()
-        L = api.SamplingTargetList()
-        for k, q, w in lights:
-            L._add(k, q, w)
-        return data, L, None, None, cam, "Cornell box 256x256, 64 spp stratified, depth 8, NEE"
+        data, targets, cam = scenes.cornell()
+        return [dict(data=data, lights=_lights(targets), env=None, env_rgb=None, cam=cam, targets=targets)], \
+            "Cornell box 256x256, 64 spp stratified, depth 8, NEE"
     if name == "terrain_720p":
         data, cam = scenes.terrain()
-        return data, None, None, None, cam, "1280x720 Perlin terrain 524,288 triangles + 2,000 spheres, 32 spp, depth 6"
+        return [dict(data=data, lights=None, env=None, env_rgb=None, cam=cam, targets=None)], \
+            "1280x720 Perlin terrain 524,288 triangles + 2,000 spheres, 32 spp, depth 6"
+    if name == "batch64":
+        frames = []
+        for data, targets, cam in scenes.batch_scenes(64):
+            frames.append(dict(data=data, lights=_lights(targets), env=None, env_rgb=None, cam=cam,
+                               targets=targets))
+        prims = sum(len(f["data"]) for f in frames)
+        return frames, f"64 seeded scenes, {prims:,} primitives (1k-2M triangles each), 256^2-1080p, 8 spp, " \
+                       f"depth 8, NEE, one batched render"
     raise SystemExit(f"unknown workload {name}")
 
 
-def cpu_baseline(data, env_rgb, lights_gpu, cam_spec, seconds: float) -> dict:
-    """The CPU oracle on a bounded row sample of the same frame: reference
-    RNG stream, all host cores, 16x16 tiles (render_values_tiled)."""
+def cpu_baseline(frame, seconds: float) -> dict:
+    """The CPU oracle on a bounded row sample of the frame: reference RNG
+    stream, all host cores, 16x16 tiles (render_values_tiled)."""
     sys.path.insert(0, str(ROOT / "oracle"))
     import cpu_ref
     from gartus_amd import abi, scenes
-    scene = cpu_ref.Scene(data)
-    env = cpu_ref.Env(rgb32f=env_rgb) if env_rgb is not None else None
-    lights = cpu_ref.Lights(lights_gpu.targets) if lights_gpu is not None else None
-    cam = scenes.camera_record(cam_spec)
+    scene = cpu_ref.Scene(frame["data"])
+    env = cpu_ref.Env(rgb32f=frame["env_rgb"]) if frame["env_rgb"] is not None else None
+    lights = cpu_ref.Lights(frame["targets"]) if frame["targets"] is not None else None
+    cam = scenes.camera_record(frame["cam"])
     cam.rng_mode = abi.GT_RNG_PIXEL_STREAM
     threads = cpu_ref.cpu_count()
     W = cam.image_width
@@ -84,7 +115,7 @@ def cpu_baseline(data, env_rgb, lights_gpu, cam_spec, seconds: float) -> dict:
     # pilot: one row near the middle, then enough rows for ~`seconds`
     y0 = H // 2
     t = time.perf_counter()
-    _, c1 = scene.render(cam, lights, env, threads=threads, crop=(0, y0, W, 1))
+    scene.render(cam, lights, env, threads=threads, crop=(0, y0, W, 1))
     pilot = max(time.perf_counter() - t, 1e-3)
     rows = int(max(1, min(H - y0, seconds / pilot)))
     t = time.perf_counter()
@@ -96,7 +127,7 @@ def cpu_baseline(data, env_rgb, lights_gpu, cam_spec, seconds: float) -> dict:
         "unit": "Mrays/s",
         "cores": threads,
         "kind": "port",
-        "sample": f"{W}x{rows} rows centred on the frame, full spp, f64 oracle, reference RNG stream; "
+        "sample": f"{W}x{rows} rows centred on the first frame, full spp, f64 oracle, reference RNG stream; "
                   f"{c['paths'] / dt:.4g} samples/s over {dt:.1f} s ({rays} rays)",
     }
 
@@ -145,12 +176,14 @@ def main() -> int:
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--workload", default="ggx_env_1080p", choices=["ggx_env_1080p", "cornell_256", "terrain_720p"])
+    ap.add_argument("--workload", default="ggx_env_1080p", choices=WORKLOADS)
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--block", type=int, default=0)
     ap.add_argument("--bvh-width", type=int, default=0)
     ap.add_argument("--max-paths", type=int, default=0)
+    ap.add_argument("--graph", action="store_true", help="replay the render as a captured hipGraph")
+    ap.add_argument("--autotune", action="store_true", help="pick the tuning with gt_autotune (batch64: always)")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="run only a plain device copy of MIB MiB (counter calibration)")
     ap.add_argument("--summarize-counters", nargs="+", metavar="CSV",
@@ -166,7 +199,7 @@ def main() -> int:
 
     from gartus_amd import abi, api, scenes
     t0 = time.perf_counter()
-    data, lights, env, env_rgb, cam_spec, desc = build_workload(args.workload)
+    frames, desc = build_workload(args.workload)
     log(f"workload {args.workload}: {desc}; generated in {time.perf_counter() - t0:.1f} s")
     ctx = api.Context(0)
     kw = {}
@@ -179,61 +212,107 @@ def main() -> int:
     if kw:
         ctx.set_tuning(**kw)
     t0 = time.perf_counter()
-    rs = api.RayScene(data, ctx).with_bvh()
-    info = rs.info()
-    log(f"scene compiled in {time.perf_counter() - t0:.2f} s: {info.primitive_count} prims, "
-        f"{info.bvh_node_count} nodes, depth {info.bvh_depth}, {info.device_bytes / 1e6:.1f} MB on device")
-    camera = api.RayCamera.__new__(api.RayCamera)
-    camera.raw = scenes.camera_record(cam_spec)
-    if env is not None or lights is not None:
-        camera = camera.with_direct_lighting_mode(abi.GT_LIGHTING_NEE)
-    W, H = camera.image_width(), camera.image_height()
-    spp = camera.effective_samples_per_pixel()
+    n = len(frames)
     lib = ctx.lib
-    sh = rs._compile()
-    lh = lights._compile(ctx) if lights is not None else None
-    eh = env._compile(ctx) if env is not None else None
-    dout = C.c_void_p()
-    abi.check(lib.gt_device_malloc(ctx.handle, W * H * 3 * 4, C.byref(dout)))
+    scenes_h = (C.c_void_p * n)()
+    lights_h = (C.c_void_p * n)()
+    envs_h = (C.c_void_p * n)()
+    cams = (abi.GtCamera * n)()
+    outs = (C.c_void_p * n)()
+    npaths = 0
+    prims = nodes = 0
+    # scene compiles (the host SAH build) run in parallel: ctypes drops the GIL
+    with ThreadPoolExecutor(max_workers=min(16, os.cpu_count() or 1)) as ex:
+        built = list(ex.map(lambda fr: api.RayScene(fr["data"], ctx).with_bvh(), frames))
+    for i, fr in enumerate(frames):
+        rs = built[i]
+        fr["scene"] = rs
+        info = rs.info()
+        prims += info.primitive_count
+        nodes += info.bvh_node_count
+        scenes_h[i] = rs._compile()
+        lights_h[i] = fr["lights"]._compile(ctx) if fr["lights"] is not None else None
+        envs_h[i] = fr["env"]._compile(ctx) if fr["env"] is not None else None
+        cams[i] = scenes.camera_record(fr["cam"])
+        W, H = cams[i].image_width, lib.gt_camera_image_height(C.byref(cams[i]))
+        spp = lib.gt_camera_effective_spp(C.byref(cams[i]))
+        npaths += W * H * spp
+        fr["wh"] = (W, H)
+        d = C.c_void_p()
+        abi.check(lib.gt_device_malloc(ctx.handle, W * H * 3 * 4, C.byref(d)))
+        outs[i] = d
+    log(f"{n} scene(s) compiled in {time.perf_counter() - t0:.2f} s: {prims:,} prims, {nodes:,} BVH nodes")
+    tuned = None
+    if args.autotune or args.workload == "batch64":
+        best = abi.GtTuning()
+        ms = C.c_double()
+        t0 = time.perf_counter()
+        abi.check(lib.gt_autotune(ctx.handle, n, scenes_h, cams, lights_h, envs_h, 1, C.byref(best), C.byref(ms)))
+        tuned = {"block_size": best.block_size, "bvh_width": best.bvh_width, "max_paths": best.max_paths}
+        log(f"autotune ({time.perf_counter() - t0:.1f} s): {tuned}, {ms.value:.2f} ms at 1 spp")
+    if args.graph:
+        ctx.set_tuning(use_graph=1)
 
-    def render(count: bool = False):
+    def render(stats: bool = True, count: bool = False):
         if count:
-            ctx.set_tuning(count_traversal=1)
+            ctx.set_tuning(count_traversal=1, use_graph=0)
         st = abi.GtRenderStats()
-        abi.check(lib.gt_render_hdr_device(ctx.handle, sh, C.byref(camera.raw), lh, eh, dout, 0, C.byref(st)))
+        sp = C.byref(st) if stats else None
+        if n == 1:
+            abi.check(lib.gt_render_hdr_device(ctx.handle, scenes_h[0], C.byref(cams[0]), lights_h[0], envs_h[0],
+                                               outs[0], 0, sp))
+        else:
+            abi.check(lib.gt_render_batch_device(ctx.handle, n, scenes_h, cams, lights_h, envs_h, outs, 0, sp))
         if count:
-            ctx.set_tuning(count_traversal=0)
+            ctx.set_tuning(count_traversal=0, use_graph=1 if args.graph else 0)
         return st
 
+    # one instrumented render gives the ray counts and per-kernel times (the
+    # counts are deterministic); with --graph the timed steps replay the
+    # captured graph uninstrumented
+    ref = render(stats=True)
     for i in range(args.warmup):
-        st = render()
-        log(f"warmup {i}: {st.device_ms:.1f} ms device, {st.extension_rays + st.shadow_rays} rays")
+        st = render(stats=not args.graph)
+        if not args.graph:
+            log(f"warmup {i}: {st.device_ms:.1f} ms device")
     ctx.synchronize()
     t0 = time.perf_counter()
-    stats = [render() for _ in range(args.steps)]
+    stats = [render(stats=not args.graph) for _ in range(args.steps)]
     ctx.synchronize()
     wall = time.perf_counter() - t0
-    for i, st in enumerate(stats):
-        log(f"step {i}: {st.device_ms:.1f} ms device, extend {st.extend_ms:.1f} ms, shade {st.shade_ms:.1f} ms, "
-            f"connect {st.connect_ms:.1f} ms")
-    rays = sum(s.extension_rays + s.shadow_rays for s in stats)
-    ext_rays = stats[0].extension_rays
+    if not args.graph:
+        for i, st in enumerate(stats):
+            log(f"step {i}: {st.device_ms:.1f} ms device, extend {st.extend_ms:.1f} ms, shade {st.shade_ms:.1f} ms, "
+                f"connect {st.connect_ms:.1f} ms")
+    rays_step = ref.extension_rays + ref.shadow_rays
+    rays = rays_step * args.steps
+    ext_rays = ref.extension_rays
     value = rays / wall / 1e6
-    # traversal counts are deterministic: one counting render gives them
     cst = render(count=True)
     assert cst.extension_rays == ext_rays, "renders are not deterministic"
-    launches = stats[0].extend_launches
-    pb = prim_bytes(data)
+    launches = ref.extend_launches
+    pb = prim_bytes(frames)
     ext_bytes = ext_rays * (32 + 16) + 64 * cst.ext_node_fetches + pb * cst.ext_prim_tests
-    avg_launch_ms = sum(s.extend_ms for s in stats) / sum(s.extend_launches for s in stats)
+    timed = stats if not args.graph else [ref]
+    avg_launch_ms = sum(s.extend_ms for s in timed) / sum(s.extend_launches for s in timed)
     achieved = (ext_bytes / launches) / (avg_launch_ms * 1e-3) / 1e9
-    paths = stats[0].camera_paths
     traffic = traffic_for(args.workload)
-    out = np.zeros(W * H * 3, np.float32)
-    abi.check(lib.gt_memcpy_to_host(ctx.handle, out.ctypes.data_as(C.c_void_p), dout, out.nbytes))
-    log(f"image mean {out.reshape(-1, 3).mean(0)}; samples/s {paths * args.steps / wall:.4g}; "
+    W0, H0 = frames[0]["wh"]
+    out = np.zeros(W0 * H0 * 3, np.float32)
+    abi.check(lib.gt_memcpy_to_host(ctx.handle, out.ctypes.data_as(C.c_void_p), outs[0], out.nbytes))
+    log(f"frame 0 mean {out.reshape(-1, 3).mean(0)}; samples/s {npaths * args.steps / wall:.4g}; "
         f"ext nodes/ray {cst.ext_node_fetches / ext_rays:.2f} prims/ray {cst.ext_prim_tests / ext_rays:.2f}")
-    lib.gt_device_free(ctx.handle, dout)
+    for i in range(n):
+        lib.gt_device_free(ctx.handle, outs[i])
+    cam0 = cams[0]
+    config = {"workload": desc, "frames": n, "primitives": int(prims), "paths_per_step": int(npaths),
+              "rays_per_step": int(rays_step), "samples_per_s": round(npaths * args.steps / wall, 1),
+              "hipgraph": bool(args.graph)}
+    if n == 1:
+        config.update(width=W0, height=H0, spp=int(lib.gt_camera_effective_spp(C.byref(cam0))),
+                      max_depth=int(cam0.max_depth))
+    if tuned:
+        config["autotuned"] = tuned
     result = {
         "metric": "Mrays/s",
         "value": round(value, 2),
@@ -246,15 +325,8 @@ def main() -> int:
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f32",
-        "data": "synthetic (seeded procedural scene and sky)",
-        "config": {
-            "workload": desc,
-            "width": W, "height": H, "spp": spp, "max_depth": camera.max_depth(),
-            "primitives": int(info.primitive_count),
-            "paths_per_step": int(paths),
-            "rays_per_step": int(rays // args.steps),
-            "samples_per_s": round(paths * args.steps / wall, 1),
-        },
+        "data": "synthetic (seeded procedural scenes and sky)",
+        "config": config,
         "roofline": {
             "bound": "hbm",
             "achieved": round(achieved, 1),
@@ -266,14 +338,14 @@ def main() -> int:
             "bytes_per_launch": round(ext_bytes / launches),
             "avg_launch_ms": round(avg_launch_ms, 4),
             # HBM bytes actually moved per launch (rocprofv3 FETCH_SIZE + WRITE_SIZE, copy-calibrated)
-            # over the live launch time: the BVH and primitives are served from L2, so this is
-            # far below the algorithmic figure
+            # over the live launch time: BVH and primitives that fit in L2 are served from there, so
+            # this can be far below the algorithmic figure
             "hbm_measured_gbs": None if traffic is None else round(traffic / (avg_launch_ms * 1e-3) / 1e9, 1),
         },
     }
     if not args.no_cpu_baseline:
         log("CPU baseline ...")
-        result["cpu_baseline"] = cpu_baseline(data, env_rgb, lights, cam_spec, args.cpu_seconds)
+        result["cpu_baseline"] = cpu_baseline(frames[0], args.cpu_seconds)
     else:
         result["cpu_baseline"] = None
     print(json.dumps(result), flush=True)

@@ -175,6 +175,8 @@ PROTOTYPES = {
                             C.POINTER(GtRenderStats)]),
     "gt_render_batch_device": (I, [P, C.c_size_t, PP, C.POINTER(GtCamera), PP, PP, PP, I,
                                    C.POINTER(GtRenderStats)]),
+    "gt_autotune": (I, [P, C.c_size_t, PP, C.POINTER(GtCamera), PP, PP, C.c_uint32, C.POINTER(GtTuning),
+                        C.POINTER(C.c_double)]),
 }
 
 _lib = None

@@ -68,6 +68,7 @@ struct gt_context {
   DevBuf<unsigned long long> tally;
   DevBuf<uint2> gstack;
   DevBuf<double4> acc;
+  DevBuf<float> staging;           // device results of the host-output entry points
   DevBuf<gtk::Frame> frame_table;  // batched renders
   DevBuf<gtk::Seg> seg_table;
   size_t cap_paths = 0;
@@ -76,7 +77,11 @@ struct gt_context {
   uint32_t* host_ctr = nullptr;  // pinned copy of the counters, one block per batch
   size_t host_ctr_words = 0;
   std::vector<hipEvent_t> events;
+  // the captured launch sequence of the last render, and what it was captured for
+  hipGraphExec_t graph_exec = nullptr;
+  std::vector<unsigned char> graph_key;
   ~gt_context() {
+    if (graph_exec) (void)hipGraphExecDestroy(graph_exec);
     for (hipEvent_t e : events) (void)hipEventDestroy(e);
     if (host_ctr) (void)hipHostFree(host_ctr);
     if (stream) (void)hipStreamDestroy(stream);
@@ -436,7 +441,7 @@ int check_objects(gt_context* ctx, const gt_scene* scene, const gt_lights* light
 
 // One frame of a render: what the kernels read plus what the host plans with.
 struct FrameJob {
-  gtk::Frame f;
+  gtk::Frame f{};
   uint32_t spp = 0, depth = 0;
   bool conn_l = false, conn_e = false;  // light / environment shadow rays
   uint64_t stack_need = 0;
@@ -574,55 +579,100 @@ int run_jobs(gt_context* ctx, std::vector<FrameJob>& jobs, bool multi, int out_i
   }
   // events per bounce: [e0] extend [e1][e2] shade [e3] connect [e4]
   size_t ev = 0;
-  if (want_stats) GT_HIP(hipEventRecord(ctx->events[ev++], st));
-  gtk::zero_acc(L, ctx->acc.p, acc_total);
-  size_t seg_off = 0;
-  for (size_t b = 0; b < nbatch; ++b) {
-    gtk::Work w{};
-    w.multi = multi;
-    w.bt.nseg = uint32_t(batches[b].size());
-    uint64_t total = 0, pix = 0;
-    for (auto& sg : batches[b]) {
-      total += uint64_t(sg.nsamples) * jobs[multi ? sg.frame : 0].f.npix;
-      pix += jobs[multi ? sg.frame : 0].f.npix;
-    }
-    w.bt.total = uint32_t(total);
-    w.bt.npix_total = uint32_t(pix);
-    if (multi) {
-      w.bt.frames = ctx->frame_table.p;
-      w.bt.segs = ctx->seg_table.p + seg_off;
-      std::memset(&w.f, 0, sizeof w.f);
-    } else {
-      w.f = jobs[0].f;
-      w.bt.seg0 = batches[b][0];
-    }
-    seg_off += batches[b].size();
-    const uint32_t depth = batch_depth[b];
-    gtk::reset_counters(L, q, depth_all);
-    gtk::raygen(L, w, q);
-    for (uint32_t bounce = 0; bounce < depth; ++bounce) {
-      if (want_stats) GT_HIP(hipEventRecord(ctx->events[ev++], st));
-      gtk::extend(L, w, q, bounce, count);
-      if (want_stats) {
-        GT_HIP(hipEventRecord(ctx->events[ev++], st));
-        GT_HIP(hipEventRecord(ctx->events[ev++], st));
+  auto enqueue = [&]() -> int {
+    if (want_stats) GT_HIP(hipEventRecord(ctx->events[ev++], st));
+    gtk::zero_acc(L, ctx->acc.p, acc_total);
+    size_t seg_off = 0;
+    for (size_t b = 0; b < nbatch; ++b) {
+      gtk::Work w{};
+      w.multi = multi;
+      w.bt.nseg = uint32_t(batches[b].size());
+      uint64_t total = 0, pix = 0;
+      for (auto& sg : batches[b]) {
+        total += uint64_t(sg.nsamples) * jobs[multi ? sg.frame : 0].f.npix;
+        pix += jobs[multi ? sg.frame : 0].f.npix;
       }
-      gtk::shade(L, w, q, bounce);
-      if (want_stats) GT_HIP(hipEventRecord(ctx->events[ev++], st));
-      if (batch_l[b]) gtk::connect(L, w, q, bounce, 0, count);
-      if (batch_e[b]) gtk::connect(L, w, q, bounce, 1, count);
-      if (want_stats) GT_HIP(hipEventRecord(ctx->events[ev++], st));
+      w.bt.total = uint32_t(total);
+      w.bt.npix_total = uint32_t(pix);
+      if (multi) {
+        w.bt.frames = ctx->frame_table.p;
+        w.bt.segs = ctx->seg_table.p + seg_off;
+      } else {
+        w.f = jobs[0].f;
+        w.bt.seg0 = batches[b][0];
+      }
+      seg_off += batches[b].size();
+      const uint32_t depth = batch_depth[b];
+      gtk::reset_counters(L, q, depth_all);
+      gtk::raygen(L, w, q);
+      for (uint32_t bounce = 0; bounce < depth; ++bounce) {
+        if (want_stats) GT_HIP(hipEventRecord(ctx->events[ev++], st));
+        gtk::extend(L, w, q, bounce, count);
+        if (want_stats) {
+          GT_HIP(hipEventRecord(ctx->events[ev++], st));
+          GT_HIP(hipEventRecord(ctx->events[ev++], st));
+        }
+        gtk::shade(L, w, q, bounce);
+        if (want_stats) GT_HIP(hipEventRecord(ctx->events[ev++], st));
+        if (batch_l[b]) gtk::connect(L, w, q, bounce, 0, count);
+        if (batch_e[b]) gtk::connect(L, w, q, bounce, 1, count);
+        if (want_stats) GT_HIP(hipEventRecord(ctx->events[ev++], st));
+      }
+      gtk::accumulate(L, w, q);
+      if (want_stats) {
+        uint32_t* h = ctx->host_ctr + host_words * b;
+        GT_HIP(hipMemcpyAsync(h, q.ctr, ctr_words * sizeof(uint32_t), hipMemcpyDeviceToHost, st));
+        GT_HIP(hipMemcpyAsync(h + ctr_words, q.tally, tally_words * sizeof(uint64_t), hipMemcpyDeviceToHost, st));
+      }
     }
-    gtk::accumulate(L, w, q);
-    if (want_stats) {
-      uint32_t* h = ctx->host_ctr + host_words * b;
-      GT_HIP(hipMemcpyAsync(h, q.ctr, ctr_words * sizeof(uint32_t), hipMemcpyDeviceToHost, st));
-      GT_HIP(hipMemcpyAsync(h + ctr_words, q.tally, tally_words * sizeof(uint64_t), hipMemcpyDeviceToHost, st));
+    for (FrameJob& j : jobs)
+      if (j.d_out) gtk::finalize(L, j.f.acc, j.f.npix, j.d_out, out_is_double);
+    if (want_stats) GT_HIP(hipEventRecord(ctx->events[ev++], st));
+    return GT_OK;
+  };
+  if (ctx->tuning.use_graph && !want_stats) {
+    // The whole launch sequence is captured once and replayed while nothing
+    // it was captured with changes: frames (with their accumulators),
+    // segments, queue and table addresses, grid, outputs.
+    std::vector<unsigned char> key;
+    auto put = [&key](const void* p, size_t n) {
+      const unsigned char* c = static_cast<const unsigned char*>(p);
+      key.insert(key.end(), c, c + n);
+    };
+    for (const FrameJob& j : jobs) {
+      put(&j.f, sizeof j.f);
+      put(&j.d_out, sizeof j.d_out);
     }
+    for (auto& b : batches) put(b.data(), b.size() * sizeof(gtk::Seg));
+    put(&q, sizeof q);
+    put(&L, sizeof L);
+    const void* tables[2] = {ctx->frame_table.p, ctx->seg_table.p};
+    put(tables, sizeof tables);
+    const uint32_t flags[3] = {multi ? 1u : 0u, uint32_t(out_is_double), depth_all};
+    put(flags, sizeof flags);
+    if (!ctx->graph_exec || key != ctx->graph_key) {
+      if (ctx->graph_exec) GT_HIP(hipGraphExecDestroy(ctx->graph_exec));
+      ctx->graph_exec = nullptr;
+      ctx->graph_key.clear();
+      hipGraph_t graph = nullptr;
+      GT_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+      int erc = enqueue();
+      hipError_t ce = hipStreamEndCapture(st, &graph);
+      if (erc != GT_OK) {
+        if (graph) (void)hipGraphDestroy(graph);
+        return erc;
+      }
+      GT_HIP(ce);
+      hipError_t ie = hipGraphInstantiate(&ctx->graph_exec, graph, nullptr, nullptr, 0);
+      (void)hipGraphDestroy(graph);
+      GT_HIP(ie);
+      ctx->graph_key = std::move(key);
+    }
+    GT_HIP(hipGraphLaunch(ctx->graph_exec, st));
+  } else {
+    int erc = enqueue();
+    if (erc != GT_OK) return erc;
   }
-  for (FrameJob& j : jobs)
-    if (j.d_out) gtk::finalize(L, j.f.acc, j.f.npix, j.d_out, out_is_double);
-  if (want_stats) GT_HIP(hipEventRecord(ctx->events[ev++], st));
   GT_HIP(hipGetLastError());
   if (want_stats) {
     GT_HIP(hipStreamSynchronize(st));
@@ -1163,7 +1213,7 @@ int gt_render_hdr(gt_context* ctx, const gt_scene* scene, const gt_camera* camer
   if (!ctx || !camera) return fail(GT_EINVAL, "null argument");
   GT_HIP(hipSetDevice(ctx->device));
   size_t npix = size_t(camera->image_width) * gt_camera_image_height(camera);
-  DevBuf<float> d;
+  DevBuf<float>& d = ctx->staging;
   GT_HIP(d.ensure(3 * npix));
   int rc = render_impl(ctx, scene, camera, lights, env, d.p, 0, nullptr, nullptr, stats);
   if (rc != GT_OK) return rc;
@@ -1185,7 +1235,7 @@ int gt_render_aovs(gt_context* ctx, const gt_scene* scene, const gt_camera* came
   if (!ctx || !camera) return fail(GT_EINVAL, "null argument");
   GT_HIP(hipSetDevice(ctx->device));
   size_t npix = size_t(camera->image_width) * gt_camera_image_height(camera);
-  DevBuf<float> d;
+  DevBuf<float>& d = ctx->staging;
   GT_HIP(d.ensure(9 * npix));
   int rc = render_impl(ctx, scene, camera, lights, env, d.p, 0, d.p + 3 * npix, d.p + 6 * npix, stats);
   if (rc != GT_OK) return rc;
@@ -1230,7 +1280,7 @@ int gt_render_batch(gt_context* ctx, size_t n, const gt_scene* const* scenes, co
     if (px == 0 || px >= (uint64_t(1) << 31)) return fail(GT_EINVAL, "image size out of range");
     total += size_t(px) * 3;
   }
-  DevBuf<float> tmp;
+  DevBuf<float>& tmp = ctx->staging;
   GT_HIP(tmp.ensure(std::max<size_t>(total, 1)));
   std::vector<void*> d_out(n);
   for (size_t i = 0; i < n; ++i) d_out[i] = tmp.p + offs[i];
@@ -1241,6 +1291,86 @@ int gt_render_batch(gt_context* ctx, size_t n, const gt_scene* const* scenes, co
     const size_t len = (i + 1 < n ? offs[i + 1] : total) - offs[i];
     GT_HIP(hipMemcpy(out_rgb[i], tmp.p + offs[i], len * sizeof(float), hipMemcpyDeviceToHost));
   }
+  return GT_OK;
+}
+
+int gt_autotune(gt_context* ctx, size_t n, const gt_scene* const* scenes, const gt_camera* cameras,
+                const gt_lights* const* lights, const gt_env* const* envs, uint32_t sample_spp,
+                gt_tuning* out_best, double* out_best_ms) {
+  if (!ctx) return fail(GT_EINVAL, "context is null");
+  if (n == 0 || !scenes || !cameras) return fail(GT_EINVAL, "nothing to tune on");
+  GT_HIP(hipSetDevice(ctx->device));
+  const gt_tuning start = ctx->tuning;
+  // the workload: every frame at sample_spp random samples, no output
+  std::vector<gt_camera> cams(cameras, cameras + n);
+  uint64_t paths = 0;
+  for (gt_camera& c : cams) {
+    c.samples_per_pixel = std::max<uint32_t>(sample_spp, 1u);
+    c.pixel_sample_mode = GT_PIXEL_RANDOM;
+    paths += uint64_t(c.image_width) * gt_camera_image_height(&c) * c.samples_per_pixel;
+  }
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  GT_HIP(hipEventCreate(&e0));
+  GT_HIP(hipEventCreate(&e1));
+  struct Ev {
+    hipEvent_t a, b;
+    ~Ev() { (void)hipEventDestroy(a); (void)hipEventDestroy(b); }
+  } guard{e0, e1};
+  const uint32_t blocks[] = {64, 128, 256};
+  const uint32_t widths[] = {2, 4};
+  const uint64_t caps[] = {uint64_t(1) << 23, uint64_t(1) << 24, uint64_t(1) << 25};
+  gt_tuning best = start;
+  double best_ms = 1e300;
+  for (uint32_t width : widths) {
+    for (uint32_t block : blocks) {
+      uint64_t prev_cap = 0;
+      for (uint64_t cap : caps) {
+        // a queue larger than the whole workload behaves like the previous one
+        if (prev_cap >= paths) break;
+        prev_cap = cap;
+        gt_tuning t = start;
+        t.block_size = block;
+        t.bvh_width = width;
+        t.max_paths = cap;
+        t.count_traversal = 0;
+        t.use_graph = 0;
+        int rc = gt_context_set_tuning(ctx, &t);
+        if (rc != GT_OK) continue;
+        std::vector<FrameJob> jobs(n);
+        for (size_t i = 0; i < n; ++i) {
+          rc = prepare_job(ctx, scenes[i], &cams[i], lights ? lights[i] : nullptr, envs ? envs[i] : nullptr,
+                           &jobs[i]);
+          if (rc != GT_OK) {
+            ctx->tuning = start;
+            return rc;
+          }
+        }
+        const bool multi = n > 1;
+        if ((rc = run_jobs(ctx, jobs, multi, 0, nullptr)) != GT_OK) {  // warm: allocations, code load
+          ctx->tuning = start;
+          return rc;
+        }
+        GT_HIP(hipEventRecord(e0, ctx->stream));
+        if ((rc = run_jobs(ctx, jobs, multi, 0, nullptr)) != GT_OK) {
+          ctx->tuning = start;
+          return rc;
+        }
+        GT_HIP(hipEventRecord(e1, ctx->stream));
+        GT_HIP(hipEventSynchronize(e1));
+        float ms = 0;
+        GT_HIP(hipEventElapsedTime(&ms, e0, e1));
+        if (ms < best_ms) {
+          best_ms = ms;
+          best = t;
+        }
+      }
+    }
+  }
+  best.use_graph = start.use_graph;
+  best.count_traversal = start.count_traversal;
+  ctx->tuning = best;
+  if (out_best) *out_best = best;
+  if (out_best_ms) *out_best_ms = best_ms;
   return GT_OK;
 }
 

@@ -389,3 +389,138 @@ def final_scene(seed: int = 61) -> SceneData:
     add((-4.0, 1.0, 0.0), 1.0, Material(LAMBERTIAN, (0.4, 0.2, 0.1)))
     add((4.0, 1.0, 0.0), 1.0, Material(METAL, (0.7, 0.6, 0.5), 0.0))
     return s
+
+
+# ------------------------------------------------------- batched scenes ---
+
+def torus_triangles(center, r_small: float, r_big: float, steps: int) -> np.ndarray:
+    """PolygonMatrix::add_torus (polygon_matrix.rs:795-864): steps x steps
+    quads of (phi, theta) rings, two triangles (p0, p3, p2), (p0, p2, p1)
+    each, 2 steps^2 triangles."""
+    t = np.arange(steps) / steps * 2.0 * np.pi
+    st, ct = np.sin(t), np.cos(t)
+    sp, cp = np.sin(t), np.cos(t)
+    rad = r_small * ct[None, :] + r_big                     # [1, theta]
+    P = np.stack([cp[:, None] * rad + center[0],           # [phi, theta, xyz]
+                  np.broadcast_to(r_small * st[None, :] + center[1], (steps, steps)),
+                  -sp[:, None] * rad + center[2]], axis=-1)
+    nxt = np.roll(np.arange(steps), -1)
+    p0, p1 = P, P[:, nxt]
+    p3, p2 = P[nxt], P[nxt][:, nxt]
+    t1 = np.concatenate([p0, p3, p2], axis=-1)
+    t2 = np.concatenate([p0, p2, p1], axis=-1)
+    return np.stack([t1, t2], axis=2).reshape(-1, 9)
+
+
+def crystal_triangles(center, sides: int, radius: float, height: float, yaw: float) -> np.ndarray:
+    """PolygonMatrix::add_crystal_with_yaw (polygon_matrix.rs:680-704): a
+    double pyramid, 2 sides triangles."""
+    cx, cy, cz = center
+    a = yaw + np.arange(sides) / sides * 2.0 * np.pi
+    ring = np.stack([cx + np.cos(a) * radius, np.full(sides, cy), cz + np.sin(a) * radius], axis=-1)
+    nxt = np.roll(ring, -1, axis=0)
+    top = np.broadcast_to([cx, cy + height * 0.5, cz], ring.shape)
+    bot = np.broadcast_to([cx, cy - height * 0.5, cz], ring.shape)
+    up = np.concatenate([top, ring, nxt], axis=-1)
+    down = np.concatenate([bot, nxt, ring], axis=-1)
+    return np.stack([up, down], axis=1).reshape(-1, 9)
+
+
+def crystal_field_triangles(centers: np.ndarray, sides: int, radius: np.ndarray, height: np.ndarray,
+                            yaw: np.ndarray) -> np.ndarray:
+    """crystal_triangles for many crystals at once, crystal after crystal."""
+    c = np.asarray(centers, np.float64)
+    a = yaw[:, None] + np.arange(sides)[None, :] / sides * 2.0 * np.pi       # [crystal, side]
+    ring = np.stack([c[:, 0:1] + np.cos(a) * radius[:, None], np.broadcast_to(c[:, 1:2], a.shape),
+                     c[:, 2:3] + np.sin(a) * radius[:, None]], axis=-1)
+    nxt = np.roll(ring, -1, axis=1)
+    top = np.broadcast_to((c + np.stack([0 * height, height * 0.5, 0 * height], -1))[:, None, :], ring.shape)
+    bot = np.broadcast_to((c - np.stack([0 * height, height * 0.5, 0 * height], -1))[:, None, :], ring.shape)
+    up = np.concatenate([top, ring, nxt], axis=-1)
+    down = np.concatenate([bot, nxt, ring], axis=-1)
+    return np.stack([up, down], axis=2).reshape(-1, 9)
+
+
+BATCH_RESOLUTIONS = ((256, 256), (512, 512), (1280, 720), (1920, 1080))
+
+
+def batch_scene(seed: int):
+    """One scene of the 64-scene batched workload, from SampleRng(seed).
+
+    Triangle count N = exp(U(ln 1e3, ln 2e6)) over one of three families: a
+    torus (add_torus), a field of crystals (add_crystal_with_yaw), or a Perlin
+    height map (from_height_map). Materials are mixed Lambertian / GGX /
+    Dielectric, with 1-8 DiffuseLight quads overhead (also the NEE light
+    list). The camera takes a resolution from {256^2, 512^2, 720p, 1080p}, at
+    8 spp, depth 8, NEE. Returns (SceneData, lights, camera spec)."""
+    rng = SampleRng(seed)
+    n_target = math.exp(rng.random_range(math.log(1e3), math.log(2e6)))
+    family = int(rng.random_double() * 3)
+    s = SceneData()
+    mats = []
+    for k in range(6):
+        pick = rng.random_double()
+        if pick < 0.45:
+            mats.append(s.add_material(Material(LAMBERTIAN, rng.random_vector_range(0.15, 0.85))))
+        elif pick < 0.8:
+            m = Material(GGX, rng.random_vector_range(0.3, 0.95), rng.random_range(0.05, 0.8))
+            mats.append(s.add_material(m))
+        else:
+            mats.append(s.add_material(Material(DIELECTRIC, (1.0, 1.0, 1.0), 1.5)))
+    floor = s.add_material(Material(LAMBERTIAN, (0.6, 0.6, 0.6)))
+    extent = 10.0
+    if family == 0:
+        steps = max(int(math.sqrt(n_target / 2.0)), 8)
+        tris = torus_triangles((0.0, 3.0, 0.0), 1.6, 4.5, steps)
+        tri_mats = np.array(mats, np.uint32)[(np.arange(len(tris)) // max(len(tris) // 6, 1)) % 6]
+    elif family == 1:
+        sides = 6 + int(rng.random_double() * 10)
+        count = max(int(n_target / (2 * sides)), 1)
+        side = int(math.ceil(math.sqrt(count)))
+        params = np.empty((count, 4))
+        pick = np.empty(count, np.int64)
+        for i in range(count):  # r, h, yaw, material in draw order
+            r = extent / side * rng.random_range(0.25, 0.45)
+            params[i] = (r, r * rng.random_range(1.5, 4.0), rng.random_range(0.0, math.pi), 0.0)
+            pick[i] = int(rng.random_double() * 6) % 6
+        i = np.arange(count)
+        x = ((i % side) + 0.5) / side * 2.0 * extent - extent
+        z = ((i // side) + 0.5) / side * 2.0 * extent - extent
+        r, h, yaw = params[:, 0], params[:, 1], params[:, 2]
+        tris = crystal_field_triangles(np.stack([x, h * 0.5, z], -1), sides, r, h, yaw)
+        tri_mats = np.repeat(np.array(mats, np.uint32)[pick], 2 * sides)
+    else:
+        grid = max(int(math.sqrt(n_target / 2.0)), 2)
+        perlin = Perlin(seed & 0xFFFF)
+        n = grid + 1
+        gx, gz = np.meshgrid(np.linspace(0.0, 3.0, n), np.linspace(0.0, 3.0, n))
+        h = perlin.turbulence(np.stack([gx, np.full_like(gx, 0.25), gz], axis=-1), 5)
+        tris = height_map_triangles(h, 2.0 * extent, 2.0 * extent, 4.0, 0.01)
+        cy = tris[:, [1, 4, 7]].mean(axis=1)
+        band = np.clip(((cy - cy.min()) / max(np.ptp(cy), 1e-9) * 6).astype(np.int64), 0, 5)
+        tri_mats = np.array(mats, np.uint32)[band]
+    s.add(QUAD, [[-3 * extent, 0.0, -3 * extent, 6 * extent, 0.0, 0.0, 0.0, 0.0, 6 * extent]], floor)
+    s.add(TRIANGLE, tris, tri_mats)
+    lights = []
+    n_lights = 1 + int(rng.random_double() * 8)
+    for _ in range(n_lights):
+        size = rng.random_range(1.5, 4.0)
+        x, z = rng.random_range(-extent, extent), rng.random_range(-extent, extent)
+        y = rng.random_range(14.0, 20.0)
+        power = rng.random_range(4.0, 12.0)
+        lm = s.add_material(Material(DIFFUSE_LIGHT, (power, power * 0.95, power * 0.85)))
+        q = [x - size / 2, y, z + size / 2, size, 0.0, 0.0, 0.0, 0.0, -size]  # faces -y
+        s.add(QUAD, [q], lm)
+        lights.append((QUAD, np.array(q, np.float64), 1.0))
+    w, hh = BATCH_RESOLUTIONS[int(rng.random_double() * len(BATCH_RESOLUTIONS)) % len(BATCH_RESOLUTIONS)]
+    ang = rng.random_range(0.0, 2.0 * math.pi)
+    dist = rng.random_range(20.0, 30.0)
+    cam = dict(width=w, aspect=w / hh, spp=8, depth=8, vfov=45.0,
+               lookfrom=(dist * math.cos(ang), rng.random_range(8.0, 16.0), dist * math.sin(ang)),
+               lookat=(0.0, 2.0, 0.0), background=(0.02, 0.02, 0.03), seed=1 + seed, nee=True)
+    return s, lights, cam
+
+
+def batch_scenes(n: int = 64, seed: int = 1000):
+    """The batched workload: n scenes from consecutive seeds."""
+    return [batch_scene(seed + i) for i in range(n)]

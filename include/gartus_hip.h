@@ -261,6 +261,18 @@ int gt_render_batch_device(gt_context* ctx, size_t n, const gt_scene* const* sce
                            const gt_env* const* envs, void* const* d_out, int out_is_double,
                            gt_render_stats* out_stats);
 
+/* ---- autotuner ---------------------------------------------------------
+ * No reference counterpart (the reference has no device tunables). Times the
+ * given frames, each at sample_spp random samples with no output, under
+ * every candidate gt_tuning (threads per block 64/128/256 x BVH width 2/4 x
+ * queue capacity 2^23/2^24/2^25 paths, capacities above the workload's path
+ * count skipped) and makes the fastest the context's tuning. Results do not
+ * depend on the tuning, only speed does. out_best and out_best_ms may be NULL. */
+int gt_autotune(gt_context* ctx, size_t n, const gt_scene* const* scenes,
+                const gt_camera* cameras, const gt_lights* const* lights,
+                const gt_env* const* envs, uint32_t sample_spp,
+                gt_tuning* out_best, double* out_best_ms);
+
 #ifdef __cplusplus
 }
 #endif

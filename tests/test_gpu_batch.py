@@ -100,3 +100,53 @@ def test_batch_rejects_mixed_contexts(gpu_context):
                        lookat=(0.0, 0.0, -4.0)))
     with pytest.raises(abi.InvalidArgument):
         api.render_batch([(a, cam, None, None), (b, cam, None, None)])
+
+
+def test_graph_replays_equal_eager_renders(gpu_context):
+    """With gt_tuning.use_graph the launch sequence is captured once and
+    replayed; every replay, with eager renders of other frames in between,
+    must reproduce the eager image exactly (single and batched renders)."""
+    say("graph: capture + 3 replays, single and batched")
+    frames = _frames()
+    eager = _singles(frames)
+    eager_batch = [im.pixels.copy() for im in api.render_batch(frames[:3])]
+    world, cam, L, env = frames[2]
+    try:
+        gpu_context.set_tuning(use_graph=1)
+        for rep in range(3):
+            img = api.PathTracer(cam)._render(world, cam, L, env).pixels
+            np.testing.assert_array_equal(img, eager[2], err_msg=f"single replay {rep}")
+            gpu_context.set_tuning(use_graph=0)
+            other = api.PathTracer(frames[0][1])._render(*frames[0]).pixels  # eager work in between
+            np.testing.assert_array_equal(other, eager[0])
+            gpu_context.set_tuning(use_graph=1)
+            for i, im in enumerate(api.render_batch(frames[:3])):
+                np.testing.assert_array_equal(im.pixels, eager_batch[i], err_msg=f"batch replay {rep} frame {i}")
+    finally:
+        gpu_context.set_tuning(use_graph=0)
+
+
+def test_autotune_picks_a_valid_tuning_and_keeps_images(gpu_context):
+    say("autotune over 2 frames at 1 spp")
+    frames = _frames()[:2]
+    eager = _singles(frames)
+    old = gpu_context.tuning
+    n = len(frames)
+    scenes_h = (abi.C.c_void_p * n)(*[f[0]._compile() for f in frames])
+    lights_h = (abi.C.c_void_p * n)(*[f[2]._compile(gpu_context) if f[2] is not None else None for f in frames])
+    envs_h = (abi.C.c_void_p * n)()
+    cams = (abi.GtCamera * n)(*[f[1].raw for f in frames])
+    best = abi.GtTuning()
+    ms = abi.C.c_double()
+    try:
+        abi.check(gpu_context.lib.gt_autotune(gpu_context.handle, n, scenes_h, cams, lights_h, envs_h, 1,
+                                              abi.C.byref(best), abi.C.byref(ms)))
+        now = gpu_context.tuning
+        assert best.block_size in (64, 128, 256) and best.bvh_width in (2, 4)
+        assert (now.block_size, now.bvh_width, now.max_paths) == (best.block_size, best.bvh_width, best.max_paths)
+        assert 0 < ms.value < 1e4
+        for i, (world, cam, L, env) in enumerate(frames):
+            img = api.PathTracer(cam)._render(world, cam, L, env).pixels
+            np.testing.assert_array_equal(img, eager[i], err_msg=f"frame {i} after autotune")
+    finally:
+        gpu_context.set_tuning(block_size=old.block_size, bvh_width=old.bvh_width, max_paths=old.max_paths)
