@@ -126,6 +126,14 @@ U32 = C.POINTER(C.c_uint32)
 I = C.c_int
 
 # name -> (restype, argtypes); every exported symbol of include/gartus_hip.h
+class GtDenoiserConfig(C.Structure):
+    _fields_ = [
+        ("levels", C.c_uint32),
+        ("channels", C.c_uint32 * 6),
+        ("seed", C.c_uint64),
+    ]
+
+
 PROTOTYPES = {
     "gt_camera_default": (None, [C.POINTER(GtCamera)]),
     "gt_camera_image_height": (C.c_uint32, [C.POINTER(GtCamera)]),
@@ -137,6 +145,7 @@ PROTOTYPES = {
     "gt_device_malloc": (I, [P, C.c_size_t, PP]),
     "gt_device_free": (I, [P, P]),
     "gt_memcpy_to_host": (I, [P, P, P, C.c_size_t]),
+    "gt_memcpy_to_device": (I, [P, P, P, C.c_size_t]),
     "gt_last_error": (C.c_char_p, []),
     "gt_version": (C.c_char_p, []),
     "gt_context_get_tuning": (I, [P, C.POINTER(GtTuning)]),
@@ -177,6 +186,16 @@ PROTOTYPES = {
                                    C.POINTER(GtRenderStats)]),
     "gt_autotune": (I, [P, C.c_size_t, PP, C.POINTER(GtCamera), PP, PP, C.c_uint32, C.POINTER(GtTuning),
                         C.POINTER(C.c_double)]),
+    # learned denoiser (gartus_denoise.h)
+    "gt_denoiser_default_config": (None, [C.POINTER(GtDenoiserConfig)]),
+    "gt_denoiser_create": (I, [P, C.POINTER(GtDenoiserConfig), PP]),
+    "gt_denoiser_destroy": (I, [P]),
+    "gt_denoiser_param_count": (I, [P, C.POINTER(C.c_uint64)]),
+    "gt_denoiser_get_params": (I, [P, F]),
+    "gt_denoiser_set_params": (I, [P, F]),
+    "gt_denoiser_infer": (I, [P, C.c_uint32, C.c_uint32, C.c_uint32, P, P, P, P]),
+    "gt_denoiser_loss_and_gradients": (I, [P, C.c_uint32, C.c_uint32, C.c_uint32, P, P, P, P, F, F]),
+    "gt_denoiser_train_step": (I, [P, C.c_uint32, C.c_uint32, C.c_uint32, P, P, P, P, C.c_float, F]),
 }
 
 _lib = None

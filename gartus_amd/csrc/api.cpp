@@ -16,6 +16,7 @@
 #include "../../include/gartus_hip.h"
 #include "kernels.h"
 #include "scene_build.h"
+#include "status.h"
 
 namespace {
 
@@ -439,6 +440,16 @@ int check_objects(gt_context* ctx, const gt_scene* scene, const gt_lights* light
   return GT_OK;
 }
 
+}  // namespace
+
+namespace gt {
+int set_error(int code, const std::string& msg) { return fail(code, msg); }
+int context_device(const gt_context* ctx) { return ctx->device; }
+hipStream_t context_stream(const gt_context* ctx) { return ctx->stream; }
+}  // namespace gt
+
+namespace {
+
 // One frame of a render: what the kernels read plus what the host plans with.
 struct FrameJob {
   gtk::Frame f{};
@@ -856,6 +867,13 @@ int gt_device_free(gt_context* ctx, void* ptr) {
 int gt_memcpy_to_host(gt_context* ctx, void* dst, const void* src, size_t bytes) {
   if (!ctx || (bytes && (!dst || !src))) return fail(GT_EINVAL, "null argument");
   GT_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, ctx->stream));
+  GT_HIP(hipStreamSynchronize(ctx->stream));
+  return GT_OK;
+}
+
+int gt_memcpy_to_device(gt_context* ctx, void* dst, const void* src, size_t bytes) {
+  if (!ctx || (bytes && (!dst || !src))) return fail(GT_EINVAL, "null argument");
+  GT_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, ctx->stream));
   GT_HIP(hipStreamSynchronize(ctx->stream));
   return GT_OK;
 }

@@ -1,0 +1,61 @@
+// denoise.h — launch interface of the denoiser's kernels (denoise.hip).
+//
+// Tensors are NHWC with bf16 elements (uint16_t storage) and a channel count
+// that is a multiple of 16; P = N*H*W pixels. Convolutions are 3x3, stride 1,
+// zero padding 1, with weights stored [Cout][9][Cin] (tap = 3*(dy+1)+(dx+1))
+// and, for the data gradient, transposed as [Cin][9][Cout].
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace gdn {
+
+struct Dims {
+  int n, h, w;
+  int pixels() const { return n * h * w; }
+};
+
+// y = act(conv(x) + bias); y is bf16, or fp32 when out_f32.
+void conv_fwd(hipStream_t s, const uint16_t* x, int cin, const uint16_t* w, const float* bias, void* y, int cout,
+              Dims d, bool relu, bool out_f32);
+// dx = conv^T(dy * [y > 0]) (mask only when y != null).
+void conv_dgrad(hipStream_t s, const uint16_t* dy, const uint16_t* y, int cout, const uint16_t* wt, uint16_t* dx,
+                int cin, Dims d);
+// dW[co][t][ci] = sum_p dPre[p][co] x[p + off_t][ci], dB[co] = sum_p dPre[p][co], dPre = dy * [y > 0];
+// summed in a fixed order (per-split partials in `scratch`, then reduced).
+void conv_wgrad(hipStream_t s, const uint16_t* x, int cin, const uint16_t* dy, const uint16_t* y, int cout, Dims d,
+                float* dw, float* db, float* scratch, size_t scratch_floats);
+size_t wgrad_scratch_floats(int cin, int cout, Dims d);
+
+// 2x2 max pool (h, w even).
+void pool_fwd(hipStream_t s, const uint16_t* x, uint16_t* y, int c, Dims d);
+// dx = route dy to each window's first maximum, plus add[:, add_off : add_off + c] when add != null
+// (add has add_c channels at the full resolution).
+void pool_bwd(hipStream_t s, const uint16_t* x, const uint16_t* dy, const uint16_t* add, int add_c, int add_off,
+              uint16_t* dx, int c, Dims d);
+// out[:, 0:ca] = a at half resolution (nearest 2x up), out[:, ca:ca+cb] = b; d is the full resolution.
+void upcat_fwd(hipStream_t s, const uint16_t* a, int ca, const uint16_t* b, int cb, uint16_t* out, Dims d);
+// da = sum over the 4 children of dout[:, 0:ca] (dout has ca + cb channels).
+void upcat_bwd(hipStream_t s, const uint16_t* dout, int ca, int cb, uint16_t* da, Dims d);
+
+// x (16 channels, padded dims dp) from fp32 images [n][h][w][3]: log1p(max(beauty, 0)), albedo,
+// normal, zeros; pixels beyond (h, w) replicate the edge.
+void pack_input(hipStream_t s, const float* beauty, const float* albedo, const float* normal, int h, int w,
+                uint16_t* x, Dims dp);
+// Loss L1 on Reinhard-tone-mapped radiance: per element |T(z) - t/(1+t)|, T(z) = 1 - exp(-z) for
+// z >= 0 and z below; mean over n*h*w*3. dz (bf16, 16 channels) gets d loss / d z, zero in padding.
+// Block partial sums go to partial[0..blocks).
+int loss_blocks(Dims dp);
+void loss_l1(hipStream_t s, const float* z, const float* target, int h, int w, uint16_t* dz, float* partial,
+             Dims dp);
+// radiance = expm1(max(z, 0)) for the valid region, fp32 [n][h][w][3].
+void unpack_output(hipStream_t s, const float* z, float* out, int h, int w, Dims dp);
+
+// Adam on fp32 master weights; also writes the bf16 copies (w16 [cout][9][cin] and wt16
+// [cin][9][cout]) when w16 != null.
+void adam(hipStream_t s, float* p, const float* g, float* m, float* v, size_t n, float lr, float b1, float b2,
+          float eps, float c1, float c2, uint16_t* w16, uint16_t* wt16, int cin, int cout);
+// bf16 copies of fp32 weights (after loading parameters).
+void to_bf16(hipStream_t s, const float* p, uint16_t* w16, uint16_t* wt16, int cin, int cout);
+
+}  // namespace gdn

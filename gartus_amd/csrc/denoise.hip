@@ -1,0 +1,530 @@
+// denoise.hip — kernels of the learned denoiser for gfx950.
+//
+// The network is a convolutional encoder-decoder over NHWC bf16 tensors.
+// Every 3x3 convolution is an implicit GEMM on the matrix cores
+// (v_mfma_f32_16x16x16_bf16: A 16x16 bf16, B 16x16 bf16, D 16x16 fp32):
+//   forward      M = pixels, N = Cout, K = 9 * Cin
+//   data grad    M = pixels, N = Cin,  K = 9 * Cout (taps mirrored)
+//   weight grad  M = Cout,   N = Cin,  K = pixels (per tap)
+// Operand lanes (checked on the device against a host matrix product):
+// lane l holds A[l % 16][4 (l / 16) + j] and B[4 (l / 16) + j][l % 16] for
+// j < 4, and D[4 (l / 16) + r][l % 16] for r < 4. Channel counts are
+// multiples of 16, so every A/B fragment is one aligned 8-byte load.
+// Accumulation is fp32; weight gradients are summed per split and reduced
+// in a fixed order, so training steps are deterministic.
+#include "denoise.h"
+
+#include <algorithm>
+
+namespace gdn {
+
+namespace {
+
+typedef short v4s __attribute__((ext_vector_type(4)));
+typedef float v4f __attribute__((ext_vector_type(4)));
+
+__device__ __forceinline__ float bf2f(uint16_t b) { return __uint_as_float(uint32_t(b) << 16); }
+__device__ __forceinline__ uint16_t f2bf(float f) {  // round to nearest even; NaN stays NaN
+  uint32_t u = __float_as_uint(f);
+  if ((u & 0x7fffffffu) > 0x7f800000u) return 0x7fc0;
+  u += 0x7fffu + ((u >> 16) & 1u);
+  return uint16_t(u >> 16);
+}
+
+__device__ __forceinline__ v4f mfma(v4s a, v4s b, v4f c) {
+  return __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, c, 0, 0, 0);
+}
+
+__device__ __forceinline__ v4s load4(const uint16_t* p) { return *reinterpret_cast<const v4s*>(p); }
+
+// Zeroes the bf16 lanes of a whose mask element is not positive.
+__device__ __forceinline__ v4s mask4(v4s a, v4s m) {
+  v4s r;
+#pragma unroll
+  for (int j = 0; j < 4; ++j) r[j] = bf2f(uint16_t(m[j])) > 0.f ? a[j] : short(0);
+  return r;
+}
+
+// ---- convolution, forward and data gradient --------------------------------
+// One wave computes 16 consecutive pixels x NT*16 output channels. DGRAD
+// reads the input at p - off_t (the transposed convolution) and masks it by
+// the ReLU of the forward output.
+template <int NT, bool DGRAD, bool OUT_F32>
+__global__ __launch_bounds__(256) void k_conv(const uint16_t* __restrict__ x, const uint16_t* __restrict__ mask,
+                                              int cin, const uint16_t* __restrict__ wgt,
+                                              const float* __restrict__ bias, void* __restrict__ y, int cout, int H,
+                                              int W, int P, int relu) {
+  const int lane = threadIdx.x & 63;
+  const int p0 = (blockIdx.x * 4 + (threadIdx.x >> 6)) * 16;
+  if (p0 >= P) return;  // whole wave
+  const int pa = p0 + (lane & 15);
+  const bool va = pa < P;
+  int n = 0, h = 0, w = 0;
+  if (va) {
+    w = pa % W;
+    const int t = pa / W;
+    h = t % H;
+    n = t / H;
+  }
+  const int kq = (lane >> 4) * 4;
+  v4f acc[NT];
+#pragma unroll
+  for (int j = 0; j < NT; ++j) acc[j] = v4f{0.f, 0.f, 0.f, 0.f};
+  const size_t wrow = size_t(9) * cin;  // weights per output column
+  const uint16_t* wl = wgt + size_t(lane & 15) * wrow + kq;
+  for (int tap = 0; tap < 9; ++tap) {
+    const int dy = tap / 3 - 1, dx = tap % 3 - 1;
+    const int hh = DGRAD ? h - dy : h + dy, ww = DGRAD ? w - dx : w + dx;
+    const bool ok = va && hh >= 0 && hh < H && ww >= 0 && ww < W;
+    const size_t src = ok ? ((size_t(n) * H + hh) * W + ww) * cin + kq : 0;
+    const uint16_t* wt = wl + size_t(tap) * cin;
+    for (int c0 = 0; c0 < cin; c0 += 16) {
+      v4s a = v4s{0, 0, 0, 0};
+      if (ok) {
+        a = load4(x + src + c0);
+        if (DGRAD && mask) a = mask4(a, load4(mask + src + c0));
+      }
+#pragma unroll
+      for (int j = 0; j < NT; ++j) acc[j] = mfma(a, load4(wt + size_t(j) * 16 * wrow + c0), acc[j]);
+    }
+  }
+#pragma unroll
+  for (int j = 0; j < NT; ++j) {
+    const int co = j * 16 + (lane & 15);
+    const float b = (!DGRAD && bias) ? bias[co] : 0.f;
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      const int p = p0 + (lane >> 4) * 4 + r;
+      if (p >= P) continue;
+      float v = acc[j][r] + b;
+      if (relu) v = fmaxf(v, 0.f);
+      if (OUT_F32) static_cast<float*>(y)[size_t(p) * cout + co] = v;
+      else static_cast<uint16_t*>(y)[size_t(p) * cout + co] = f2bf(v);
+    }
+  }
+}
+
+// ---- convolution, weight gradient ------------------------------------------
+// Block = one wave. blockIdx.x = (tap, co group, ci group), blockIdx.y =
+// pixel split. The wave sums its split's pixels 16 at a time into
+// NCO x NCI fp32 tiles and writes them to partial[split].
+template <int NCO, int NCI>
+__global__ __launch_bounds__(64) void k_wgrad(const uint16_t* __restrict__ x, int cin,
+                                              const uint16_t* __restrict__ dy, const uint16_t* __restrict__ mask,
+                                              int cout, int H, int W, int P, int split_px, int ncog,
+                                              float* __restrict__ partial) {
+  const int lane = threadIdx.x;
+  const int tap = blockIdx.x % 9;
+  const int g = blockIdx.x / 9;
+  const int co0 = (g % ncog) * NCO * 16, ci0 = (g / ncog) * NCI * 16;
+  const int dyo = tap / 3 - 1, dxo = tap % 3 - 1;
+  const int pbeg = blockIdx.y * split_px, pend = min(P, pbeg + split_px);
+  v4f acc[NCO][NCI];
+#pragma unroll
+  for (int a = 0; a < NCO; ++a)
+#pragma unroll
+    for (int b = 0; b < NCI; ++b) acc[a][b] = v4f{0.f, 0.f, 0.f, 0.f};
+  for (int pk0 = pbeg; pk0 < pend; pk0 += 16) {
+    // the 4 pixels of this lane's K slots
+    size_t own[4], nb[4];
+    bool vo[4], vn[4];
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      const int p = pk0 + (lane >> 4) * 4 + j;
+      vo[j] = p < pend;
+      own[j] = size_t(vo[j] ? p : 0);
+      int w = 0, h = 0, n = 0;
+      if (vo[j]) {
+        w = p % W;
+        const int t = p / W;
+        h = t % H;
+        n = t / H;
+      }
+      const int hh = h + dyo, ww = w + dxo;
+      vn[j] = vo[j] && hh >= 0 && hh < H && ww >= 0 && ww < W;
+      nb[j] = vn[j] ? (size_t(n) * H + hh) * W + ww : 0;
+    }
+    v4s A[NCO], B[NCI];
+#pragma unroll
+    for (int a = 0; a < NCO; ++a) {
+      const int co = co0 + a * 16 + (lane & 15);
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        short v = 0;
+        if (vo[j]) {
+          v = short(dy[own[j] * cout + co]);
+          if (mask && !(bf2f(mask[own[j] * cout + co]) > 0.f)) v = 0;
+        }
+        A[a][j] = v;
+      }
+    }
+#pragma unroll
+    for (int b = 0; b < NCI; ++b) {
+      const int ci = ci0 + b * 16 + (lane & 15);
+#pragma unroll
+      for (int j = 0; j < 4; ++j) B[b][j] = vn[j] ? short(x[nb[j] * cin + ci]) : short(0);
+    }
+#pragma unroll
+    for (int a = 0; a < NCO; ++a)
+#pragma unroll
+      for (int b = 0; b < NCI; ++b) acc[a][b] = mfma(A[a], B[b], acc[a][b]);
+  }
+  float* out = partial + size_t(blockIdx.y) * cout * 9 * cin;
+#pragma unroll
+  for (int a = 0; a < NCO; ++a)
+#pragma unroll
+    for (int b = 0; b < NCI; ++b)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int co = co0 + a * 16 + (lane >> 4) * 4 + r;
+        const int ci = ci0 + b * 16 + (lane & 15);
+        out[(size_t(co) * 9 + tap) * cin + ci] = acc[a][b][r];
+      }
+}
+
+// Bias gradient partials: block = one split, lanes stride over channels.
+__global__ void k_bgrad(const uint16_t* __restrict__ dy, const uint16_t* __restrict__ mask, int cout, int P,
+                        int split_px, float* __restrict__ partial) {
+  const int pbeg = blockIdx.x * split_px, pend = min(P, pbeg + split_px);
+  for (int co = threadIdx.x; co < cout; co += blockDim.x) {
+    float s = 0.f;
+    for (int p = pbeg; p < pend; ++p) {
+      const size_t i = size_t(p) * cout + co;
+      if (!mask || bf2f(mask[i]) > 0.f) s += bf2f(dy[i]);
+    }
+    partial[size_t(blockIdx.x) * cout + co] = s;
+  }
+}
+
+// out[i] = sum over s < splits of partial[s * n + i], in split order.
+__global__ void k_reduce(const float* __restrict__ partial, int splits, size_t n, float* __restrict__ out) {
+  for (size_t i = blockIdx.x * size_t(blockDim.x) + threadIdx.x; i < n; i += size_t(gridDim.x) * blockDim.x) {
+    float s = 0.f;
+    for (int k = 0; k < splits; ++k) s += partial[size_t(k) * n + i];
+    out[i] = s;
+  }
+}
+
+// ---- pooling, upsampling, concatenation -----------------------------------
+__global__ void k_pool_fwd(const uint16_t* __restrict__ x, uint16_t* __restrict__ y, int c, int H, int W, size_t n) {
+  const int H2 = H / 2, W2 = W / 2;
+  for (size_t i = blockIdx.x * size_t(blockDim.x) + threadIdx.x; i < n; i += size_t(gridDim.x) * blockDim.x) {
+    const int ch = int(i % c);
+    size_t q = i / c;
+    const int w2 = int(q % W2);
+    q /= W2;
+    const int h2 = int(q % H2);
+    const size_t b = q / H2;
+    const size_t base = ((b * H + 2 * h2) * W + 2 * w2) * c + ch;
+    float m = bf2f(x[base]);
+    m = fmaxf(m, bf2f(x[base + c]));
+    m = fmaxf(m, bf2f(x[base + size_t(W) * c]));
+    m = fmaxf(m, bf2f(x[base + size_t(W) * c + c]));
+    y[i] = f2bf(m);
+  }
+}
+
+__global__ void k_pool_bwd(const uint16_t* __restrict__ x, const uint16_t* __restrict__ dy,
+                           const uint16_t* __restrict__ add, int add_c, int add_off, uint16_t* __restrict__ dx, int c,
+                           int H, int W, size_t n) {
+  const int H2 = H / 2, W2 = W / 2;
+  for (size_t i = blockIdx.x * size_t(blockDim.x) + threadIdx.x; i < n; i += size_t(gridDim.x) * blockDim.x) {
+    const int ch = int(i % c);
+    const size_t p = i / c;
+    const int w = int(p % W);
+    const size_t t = p / W;
+    const int h = int(t % H);
+    const size_t b = t / H;
+    // the window's first maximum in scan order takes the gradient
+    const size_t base = ((b * H + (h & ~1)) * W + (w & ~1)) * c + ch;
+    const size_t cand[4] = {base, base + c, base + size_t(W) * c, base + size_t(W) * c + c};
+    int arg = 0;
+    float m = bf2f(x[cand[0]]);
+    for (int k = 1; k < 4; ++k) {
+      const float v = bf2f(x[cand[k]]);
+      if (v > m) { m = v; arg = k; }
+    }
+    const int mine = (h & 1) * 2 + (w & 1);
+    float g = 0.f;
+    if (mine == arg) g = bf2f(dy[((b * H2 + h / 2) * W2 + w / 2) * c + ch]);
+    if (add) g += bf2f(add[p * add_c + add_off + ch]);
+    dx[i] = f2bf(g);
+  }
+}
+
+__global__ void k_upcat_fwd(const uint16_t* __restrict__ a, int ca, const uint16_t* __restrict__ bsrc, int cb,
+                            uint16_t* __restrict__ out, int H, int W, size_t n) {
+  const int c = ca + cb;
+  for (size_t i = blockIdx.x * size_t(blockDim.x) + threadIdx.x; i < n; i += size_t(gridDim.x) * blockDim.x) {
+    const int ch = int(i % c);
+    const size_t p = i / c;
+    if (ch < ca) {
+      const int w = int(p % W);
+      const size_t t = p / W;
+      const int h = int(t % H);
+      const size_t b = t / H;
+      out[i] = a[((b * (H / 2) + h / 2) * (W / 2) + w / 2) * ca + ch];
+    } else {
+      out[i] = bsrc[p * cb + (ch - ca)];
+    }
+  }
+}
+
+__global__ void k_upcat_bwd(const uint16_t* __restrict__ dout, int ca, int cb, uint16_t* __restrict__ da, int H,
+                            int W, size_t n) {
+  const int c = ca + cb, H2 = H / 2, W2 = W / 2;
+  for (size_t i = blockIdx.x * size_t(blockDim.x) + threadIdx.x; i < n; i += size_t(gridDim.x) * blockDim.x) {
+    const int ch = int(i % ca);
+    size_t q = i / ca;
+    const int w2 = int(q % W2);
+    q /= W2;
+    const int h2 = int(q % H2);
+    const size_t b = q / H2;
+    const size_t base = ((b * H + 2 * h2) * W + 2 * w2) * c + ch;
+    float s = bf2f(dout[base]) + bf2f(dout[base + c]);
+    s += bf2f(dout[base + size_t(W) * c]) + bf2f(dout[base + size_t(W) * c + c]);
+    da[i] = f2bf(s);
+  }
+}
+
+// ---- input, loss, output ---------------------------------------------------
+__global__ void k_pack(const float* __restrict__ beauty, const float* __restrict__ albedo,
+                       const float* __restrict__ normal, int h, int w, uint16_t* __restrict__ x, int Hp, int Wp,
+                       size_t npx) {
+  for (size_t p = blockIdx.x * size_t(blockDim.x) + threadIdx.x; p < npx; p += size_t(gridDim.x) * blockDim.x) {
+    const int xx = int(p % Wp);
+    const size_t t = p / Wp;
+    const int yy = int(t % Hp);
+    const size_t b = t / Hp;
+    const size_t s = ((b * h + min(yy, h - 1)) * w + min(xx, w - 1)) * 3;
+    uint16_t* o = x + p * 16;
+    for (int c = 0; c < 3; ++c) {
+      o[c] = f2bf(log1pf(fmaxf(beauty[s + c], 0.f)));
+      o[3 + c] = f2bf(albedo[s + c]);
+      o[6 + c] = f2bf(normal[s + c]);
+    }
+    for (int c = 9; c < 16; ++c) o[c] = 0;
+  }
+}
+
+__device__ __forceinline__ float tone(float z) { return z >= 0.f ? 1.f - expf(-z) : z; }
+__device__ __forceinline__ float dtone(float z) { return z >= 0.f ? expf(-z) : 1.f; }
+
+__global__ __launch_bounds__(256) void k_loss(const float* __restrict__ z, const float* __restrict__ target, int h,
+                                              int w, uint16_t* __restrict__ dz, float* __restrict__ partial, int Hp,
+                                              int Wp, size_t npx, float inv_count) {
+  __shared__ float red[256];
+  float acc = 0.f;
+  for (size_t p = blockIdx.x * size_t(blockDim.x) + threadIdx.x; p < npx; p += size_t(gridDim.x) * blockDim.x) {
+    const int xx = int(p % Wp);
+    const size_t t = p / Wp;
+    const int yy = int(t % Hp);
+    const size_t b = t / Hp;
+    uint16_t* g = dz + p * 16;
+    for (int c = 3; c < 16; ++c) g[c] = 0;
+    if (yy >= h || xx >= w) {
+      g[0] = g[1] = g[2] = 0;
+      continue;
+    }
+    const float* tg = target + ((b * h + yy) * w + xx) * 3;
+    for (int c = 0; c < 3; ++c) {
+      const float zc = z[p * 16 + c];
+      const float tc = fmaxf(tg[c], 0.f);
+      const float d = tone(zc) - tc / (1.f + tc);
+      acc += fabsf(d);
+      const float s = d > 0.f ? 1.f : (d < 0.f ? -1.f : 0.f);
+      g[c] = f2bf(s * dtone(zc) * inv_count);
+    }
+  }
+  red[threadIdx.x] = acc;
+  __syncthreads();
+  for (int o = 128; o > 0; o >>= 1) {
+    if (int(threadIdx.x) < o) red[threadIdx.x] += red[threadIdx.x + o];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) partial[blockIdx.x] = red[0];
+}
+
+__global__ void k_unpack(const float* __restrict__ z, float* __restrict__ out, int h, int w, int Hp, int Wp,
+                         size_t n) {
+  for (size_t i = blockIdx.x * size_t(blockDim.x) + threadIdx.x; i < n; i += size_t(gridDim.x) * blockDim.x) {
+    const int c = int(i % 3);
+    const size_t p = i / 3;
+    const int xx = int(p % w);
+    const size_t t = p / w;
+    const int yy = int(t % h);
+    const size_t b = t / h;
+    out[i] = expm1f(fmaxf(z[((b * Hp + yy) * Wp + xx) * 16 + c], 0.f));
+  }
+}
+
+// ---- optimiser ---------------------------------------------------------------
+__global__ void k_adam(float* __restrict__ p, const float* __restrict__ g, float* __restrict__ m,
+                       float* __restrict__ v, size_t n, float lr, float b1, float b2, float eps, float c1, float c2,
+                       uint16_t* __restrict__ w16, uint16_t* __restrict__ wt16, int cin, int cout) {
+  for (size_t i = blockIdx.x * size_t(blockDim.x) + threadIdx.x; i < n; i += size_t(gridDim.x) * blockDim.x) {
+    const float gi = g[i];
+    const float mi = b1 * m[i] + (1.f - b1) * gi;
+    const float vi = b2 * v[i] + (1.f - b2) * gi * gi;
+    m[i] = mi;
+    v[i] = vi;
+    const float pi = p[i] - lr * (mi / c1) / (sqrtf(vi / c2) + eps);
+    p[i] = pi;
+    if (w16) {
+      const uint16_t b = f2bf(pi);
+      w16[i] = b;
+      const size_t ci = i % cin, t = (i / cin) % 9, co = i / (size_t(cin) * 9);
+      wt16[(ci * 9 + t) * cout + co] = b;
+    }
+  }
+}
+
+__global__ void k_to_bf16(const float* __restrict__ p, uint16_t* __restrict__ w16, uint16_t* __restrict__ wt16,
+                          int cin, int cout, size_t n) {
+  for (size_t i = blockIdx.x * size_t(blockDim.x) + threadIdx.x; i < n; i += size_t(gridDim.x) * blockDim.x) {
+    const uint16_t b = f2bf(p[i]);
+    w16[i] = b;
+    const size_t ci = i % cin, t = (i / cin) % 9, co = i / (size_t(cin) * 9);
+    wt16[(ci * 9 + t) * cout + co] = b;
+  }
+}
+
+unsigned grid_for(size_t n, unsigned block = 256) {
+  size_t g = (n + block - 1) / block;
+  return unsigned(g < 65535u * 8 ? (g ? g : 1) : 65535u * 8);
+}
+
+template <bool DGRAD, bool F32, int NT>
+void launch_nt(hipStream_t s, const uint16_t* x, const uint16_t* mask, int cin, const uint16_t* w, const float* bias,
+               void* y, int cout, Dims d, int relu) {
+  const int P = d.pixels();
+  hipLaunchKernelGGL((k_conv<NT, DGRAD, F32>), dim3((P + 63) / 64), dim3(256), 0, s, x, mask, cin, w, bias, y, cout,
+                     d.h, d.w, P, relu);
+}
+
+// NT = output channels / 16: up to 8 for the forward (cout <= 128), up to 16
+// for the data gradient (cin <= 256, a concatenation).
+template <bool DGRAD, bool F32>
+void launch_conv(hipStream_t s, int nt, const uint16_t* x, const uint16_t* mask, int cin, const uint16_t* w,
+                 const float* bias, void* y, int cout, Dims d, int relu) {
+#define GDN_NT(N) \
+  case N: launch_nt<DGRAD, F32, N>(s, x, mask, cin, w, bias, y, cout, d, relu); break
+  switch (nt) {
+    GDN_NT(1); GDN_NT(2); GDN_NT(3); GDN_NT(4); GDN_NT(5); GDN_NT(6); GDN_NT(7); GDN_NT(8);
+    default:
+      if constexpr (DGRAD) {
+        switch (nt) {
+          GDN_NT(9); GDN_NT(10); GDN_NT(11); GDN_NT(12); GDN_NT(13); GDN_NT(14); GDN_NT(15); GDN_NT(16);
+          default: break;  // rejected by the network builder
+        }
+      }
+      break;
+  }
+#undef GDN_NT
+}
+
+constexpr int kSplitPx = 4096;  // pixels per weight-gradient split
+
+}  // namespace
+
+void conv_fwd(hipStream_t s, const uint16_t* x, int cin, const uint16_t* w, const float* bias, void* y, int cout,
+              Dims d, bool relu, bool out_f32) {
+  if (out_f32) launch_conv<false, true>(s, cout / 16, x, nullptr, cin, w, bias, y, cout, d, relu ? 1 : 0);
+  else launch_conv<false, false>(s, cout / 16, x, nullptr, cin, w, bias, y, cout, d, relu ? 1 : 0);
+}
+
+void conv_dgrad(hipStream_t s, const uint16_t* dy, const uint16_t* y, int cout, const uint16_t* wt, uint16_t* dx,
+                int cin, Dims d) {
+  // the transposed problem: input dy (cout channels), output dx (cin channels)
+  launch_conv<true, false>(s, cin / 16, dy, y, cout, wt, nullptr, dx, cin, d, 0);
+}
+
+size_t wgrad_scratch_floats(int cin, int cout, Dims d) {
+  const size_t splits = (size_t(d.pixels()) + kSplitPx - 1) / kSplitPx;
+  return splits * (size_t(cout) * 9 * cin + cout);
+}
+
+void conv_wgrad(hipStream_t s, const uint16_t* x, int cin, const uint16_t* dy, const uint16_t* y, int cout, Dims d,
+                float* dw, float* db, float* scratch, size_t scratch_floats) {
+  const int P = d.pixels();
+  const int splits = (P + kSplitPx - 1) / kSplitPx;
+  const size_t nw = size_t(cout) * 9 * cin;
+  if (scratch_floats < size_t(splits) * (nw + cout)) return;  // sized by wgrad_scratch_floats
+  float* pw = scratch;
+  float* pb = scratch + size_t(splits) * nw;
+  const int tco = cout / 16, tci = cin / 16;
+  // tile groups of up to 4 x 4 (co x ci) 16-wide tiles per wave
+  const int nco = tco % 4 == 0 ? 4 : (tco % 3 == 0 ? 3 : (tco % 2 == 0 ? 2 : 1));
+  const int nci = tci % 4 == 0 ? 4 : (tci % 3 == 0 ? 3 : (tci % 2 == 0 ? 2 : 1));
+  const int ncog = tco / nco, ncig = tci / nci;
+  const dim3 grid(9 * ncog * ncig, splits), block(64);
+#define GDN_WG(A, B)                                                                                           \
+  if (nco == A && nci == B)                                                                                     \
+  hipLaunchKernelGGL((k_wgrad<A, B>), grid, block, 0, s, x, cin, dy, y, cout, d.h, d.w, P, kSplitPx, ncog, pw)
+  GDN_WG(1, 1); GDN_WG(1, 2); GDN_WG(1, 3); GDN_WG(1, 4);
+  GDN_WG(2, 1); GDN_WG(2, 2); GDN_WG(2, 3); GDN_WG(2, 4);
+  GDN_WG(3, 1); GDN_WG(3, 2); GDN_WG(3, 3); GDN_WG(3, 4);
+  GDN_WG(4, 1); GDN_WG(4, 2); GDN_WG(4, 3); GDN_WG(4, 4);
+#undef GDN_WG
+  hipLaunchKernelGGL(k_bgrad, dim3(splits), dim3(64), 0, s, dy, y, cout, P, kSplitPx, pb);
+  hipLaunchKernelGGL(k_reduce, dim3(grid_for(nw)), dim3(256), 0, s, pw, splits, nw, dw);
+  hipLaunchKernelGGL(k_reduce, dim3(grid_for(cout)), dim3(256), 0, s, pb, splits, size_t(cout), db);
+}
+
+void pool_fwd(hipStream_t s, const uint16_t* x, uint16_t* y, int c, Dims d) {
+  const size_t n = size_t(d.n) * (d.h / 2) * (d.w / 2) * c;
+  hipLaunchKernelGGL(k_pool_fwd, dim3(grid_for(n)), dim3(256), 0, s, x, y, c, d.h, d.w, n);
+}
+
+void pool_bwd(hipStream_t s, const uint16_t* x, const uint16_t* dy, const uint16_t* add, int add_c, int add_off,
+              uint16_t* dx, int c, Dims d) {
+  const size_t n = size_t(d.pixels()) * c;
+  hipLaunchKernelGGL(k_pool_bwd, dim3(grid_for(n)), dim3(256), 0, s, x, dy, add, add_c, add_off, dx, c, d.h, d.w, n);
+}
+
+void upcat_fwd(hipStream_t s, const uint16_t* a, int ca, const uint16_t* b, int cb, uint16_t* out, Dims d) {
+  const size_t n = size_t(d.pixels()) * (ca + cb);
+  hipLaunchKernelGGL(k_upcat_fwd, dim3(grid_for(n)), dim3(256), 0, s, a, ca, b, cb, out, d.h, d.w, n);
+}
+
+void upcat_bwd(hipStream_t s, const uint16_t* dout, int ca, int cb, uint16_t* da, Dims d) {
+  const size_t n = size_t(d.n) * (d.h / 2) * (d.w / 2) * ca;
+  hipLaunchKernelGGL(k_upcat_bwd, dim3(grid_for(n)), dim3(256), 0, s, dout, ca, cb, da, d.h, d.w, n);
+}
+
+void pack_input(hipStream_t s, const float* beauty, const float* albedo, const float* normal, int h, int w,
+                uint16_t* x, Dims dp) {
+  const size_t n = size_t(dp.pixels());
+  hipLaunchKernelGGL(k_pack, dim3(grid_for(n)), dim3(256), 0, s, beauty, albedo, normal, h, w, x, dp.h, dp.w, n);
+}
+
+int loss_blocks(Dims dp) {
+  const size_t n = size_t(dp.pixels());
+  return int(std::min<size_t>((n + 255) / 256, 1024));
+}
+
+void loss_l1(hipStream_t s, const float* z, const float* target, int h, int w, uint16_t* dz, float* partial,
+             Dims dp) {
+  const size_t n = size_t(dp.pixels());
+  const float inv = 1.0f / float(size_t(dp.n) * h * w * 3);
+  hipLaunchKernelGGL(k_loss, dim3(loss_blocks(dp)), dim3(256), 0, s, z, target, h, w, dz, partial, dp.h, dp.w, n,
+                     inv);
+}
+
+void unpack_output(hipStream_t s, const float* z, float* out, int h, int w, Dims dp) {
+  const size_t n = size_t(dp.n) * h * w * 3;
+  hipLaunchKernelGGL(k_unpack, dim3(grid_for(n)), dim3(256), 0, s, z, out, h, w, dp.h, dp.w, n);
+}
+
+void adam(hipStream_t s, float* p, const float* g, float* m, float* v, size_t n, float lr, float b1, float b2,
+          float eps, float c1, float c2, uint16_t* w16, uint16_t* wt16, int cin, int cout) {
+  hipLaunchKernelGGL(k_adam, dim3(grid_for(n)), dim3(256), 0, s, p, g, m, v, n, lr, b1, b2, eps, c1, c2, w16, wt16,
+                     cin, cout);
+}
+
+void to_bf16(hipStream_t s, const float* p, uint16_t* w16, uint16_t* wt16, int cin, int cout) {
+  const size_t n = size_t(cout) * 9 * cin;
+  hipLaunchKernelGGL(k_to_bf16, dim3(grid_for(n)), dim3(256), 0, s, p, w16, wt16, cin, cout, n);
+}
+
+}  // namespace gdn

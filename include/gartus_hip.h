@@ -97,6 +97,7 @@ void* gt_context_stream(gt_context* ctx);
 int gt_device_malloc(gt_context* ctx, size_t bytes, void** out);
 int gt_device_free(gt_context* ctx, void* ptr);
 int gt_memcpy_to_host(gt_context* ctx, void* dst, const void* src, size_t bytes);
+int gt_memcpy_to_device(gt_context* ctx, void* dst, const void* src, size_t bytes);
 /* Thread-local message of the last failure ("" when none). */
 const char* gt_last_error(void);
 const char* gt_version(void);

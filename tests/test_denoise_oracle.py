@@ -1,0 +1,124 @@
+"""Pins of the denoiser oracle (oracle/denoise_ref.py), CPU only.
+
+  - conv against a brute-force loop over pixels, taps and channels;
+  - max-pool routing, ties included (first maximum in scan order);
+  - the loss against its closed form;
+  - the whole-network gradient against central finite differences in
+    float64, so a dropped term, a flipped tap or a transposed weight in the
+    backward restatement fails here before the GPU is compared with it;
+  - bf16 rounding against known values.
+"""
+import numpy as np
+import pytest
+
+import denoise_ref as dr
+
+
+def test_conv_matches_brute_force():
+    rng = np.random.default_rng(0)
+    x = rng.normal(size=(2, 5, 6, 16))
+    w = rng.normal(size=(32, 9, 16))
+    b = rng.normal(size=32)
+    y = dr.conv(x, w, b)
+    ref = np.zeros_like(y)
+    for n in range(2):
+        for h in range(5):
+            for ww in range(6):
+                for co in range(32):
+                    s = b[co]
+                    for t in range(9):
+                        hh, xx = h + t // 3 - 1, ww + t % 3 - 1
+                        if 0 <= hh < 5 and 0 <= xx < 6:
+                            s += x[n, hh, xx] @ w[co, t]
+                    ref[n, h, ww, co] = s
+    np.testing.assert_allclose(y, ref, rtol=1e-12, atol=1e-12)
+
+
+def test_conv_backward_matches_brute_force():
+    rng = np.random.default_rng(1)
+    x = rng.normal(size=(1, 4, 5, 16))
+    w = rng.normal(size=(16, 9, 16))
+    g = rng.normal(size=(1, 4, 5, 16))
+    dx, dw, db = dr.conv_backward(x, w, g)
+    # brute force through the linear map y = A x: dx = A^T g
+    ref_dx = np.zeros_like(x)
+    ref_dw = np.zeros_like(w)
+    for h in range(4):
+        for ww in range(5):
+            for t in range(9):
+                hh, xx = h + t // 3 - 1, ww + t % 3 - 1
+                if 0 <= hh < 4 and 0 <= xx < 5:
+                    ref_dx[0, hh, xx] += w[:, t].T @ g[0, h, ww]
+                    ref_dw[:, t] += np.outer(g[0, h, ww], x[0, hh, xx])
+    np.testing.assert_allclose(dx, ref_dx, rtol=1e-12, atol=1e-12)
+    np.testing.assert_allclose(dw, ref_dw, rtol=1e-12, atol=1e-12)
+    np.testing.assert_allclose(db, g.sum(axis=(0, 1, 2)), rtol=1e-12)
+
+
+def test_pool_routes_to_first_maximum():
+    x = np.zeros((1, 2, 2, 2))
+    x[0, :, :, 0] = [[1.0, 3.0], [3.0, 2.0]]   # tie: (0,1) before (1,0)
+    x[0, :, :, 1] = [[5.0, 5.0], [5.0, 5.0]]   # all equal: (0,0)
+    assert dr.pool(x).tolist() == [[[[3.0, 5.0]]]]
+    g = dr.pool_backward(x, np.array([[[[7.0, 9.0]]]]))
+    assert g[0, :, :, 0].tolist() == [[0.0, 7.0], [0.0, 0.0]]
+    assert g[0, :, :, 1].tolist() == [[9.0, 0.0], [0.0, 0.0]]
+
+
+def test_loss_closed_form():
+    t = np.full((1, 2, 2, 3), 3.0)                 # T(t) = 3/4
+    z = np.zeros((1, 2, 2, 16))
+    z[..., :3] = np.log(4.0)                       # 1 - exp(-z) = 3/4
+    loss, dz = dr.loss_and_dz(z, t)
+    assert loss == pytest.approx(0.0, abs=1e-15)
+    z[..., :3] = 0.0                                # T = 0: |0 - 3/4|
+    loss, dz = dr.loss_and_dz(z, t)
+    assert loss == pytest.approx(0.75)
+    assert np.allclose(dz[..., :3], -1.0 / 12.0) and not dz[..., 3:].any()
+    z[..., :3] = -2.0                               # below zero the tone map is the identity
+    loss, _ = dr.loss_and_dz(z, t)
+    assert loss == pytest.approx(2.75)
+
+
+def test_bf16_rounding():
+    assert dr.bf16(np.array([1.0, 1.00390625, 1.005859375, -2.0]))[:4].tolist() == [1.0, 1.0, 1.0078125, -2.0]
+    assert np.isnan(dr.bf16(np.array([np.nan]))[0])
+
+
+def _problem(levels, channels, n=2, h=8, w=8, seed=3):
+    rng = np.random.default_rng(seed)
+    net = dr.Net(levels, channels)
+    net.params = rng.normal(scale=0.15, size=net.nparams)
+    beauty = rng.uniform(0.0, 3.0, (n, h, w, 3))
+    albedo = rng.uniform(0.0, 1.0, (n, h, w, 3))
+    normal = rng.uniform(0.0, 1.0, (n, h, w, 3))
+    target = rng.uniform(0.0, 3.0, (n, h, w, 3))
+    return net, beauty, albedo, normal, target
+
+
+@pytest.mark.parametrize("levels,channels,h,w", [(2, (16, 16), 8, 8), (3, (16, 32, 16), 7, 9)])
+def test_gradient_matches_finite_differences(levels, channels, h, w):
+    net, b, a, nr, t = _problem(levels, channels, h=h, w=w)
+    loss, g, _ = dr.loss_and_gradients(net, b, a, nr, t)
+    rng = np.random.default_rng(11)
+    idx = set(rng.choice(net.nparams, 40, replace=False).tolist())
+    for i in range(len(net.convs)):  # at least one weight and the first bias of every convolution
+        w0, b0 = net.offsets[i]
+        cin = net.convs[i][0]
+        idx.update([w0 + 4 * cin + 1, b0])
+    eps = 1e-6
+    checked = 0
+    for i in sorted(idx):
+        p = net.params.copy()
+        p[i] += eps
+        lp = dr.loss_and_gradients(net, b, a, nr, t, params=p)[0]
+        p[i] -= 2 * eps
+        lm = dr.loss_and_gradients(net, b, a, nr, t, params=p)[0]
+        fd = (lp - lm) / (2 * eps)
+        assert fd == pytest.approx(g[i], rel=1e-4, abs=1e-8), f"param {i}"
+        checked += 1
+    assert checked >= 40
+    # padded input channels (9..15) carry no gradient
+    w0, _ = net.offsets[0]
+    wg = g[w0:w0 + net.convs[0][1] * 9 * 16].reshape(-1, 9, 16)
+    assert not wg[:, :, 9:].any()

@@ -1,0 +1,122 @@
+"""The denoiser's MFMA kernels against the float64 oracle (oracle/denoise_ref.py).
+
+The oracle rounds to bf16 wherever the GPU stores bf16 (bf16=True), so the
+remaining differences come from fp32 accumulation order, which moves an
+occasional bf16 rounding by one step (2^-8 relative) and propagates through
+later layers. Tolerances:
+  - network output z: |z_gpu - z_ref| <= 0.02 + 0.02 |z_ref| for 99.5% of
+    values; mean absolute difference below 2^-8 (one bf16 step at 1) times
+    sqrt(number of convolutions), the growth of independent one-step
+    rounding flips over the network's depth;
+  - loss: 1e-3 relative;
+  - gradients: per convolution, ||g_gpu - g_ref|| / ||g_ref|| below
+    1% times sqrt(number of convolutions) (the backward chain of the deepest
+    layer crosses every convolution twice), and cosine similarity above
+    0.998. A transposed weight, a wrong tap or a lost term gives errors of
+    order one.
+"""
+import sys
+
+import numpy as np
+import pytest
+
+import denoise_ref as dr
+from gartus_amd.denoiser import Denoiser
+
+pytestmark = pytest.mark.gpu
+
+
+def say(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+def _problem(levels, channels, n, h, w, seed):
+    rng = np.random.default_rng(seed)
+    net = dr.Net(levels, channels)
+    # a He-like scale keeps activations O(1) through the layers
+    p = np.zeros(net.nparams)
+    for i, (cin, cout, _) in enumerate(net.convs):
+        w0, b0 = net.offsets[i]
+        p[w0:b0] = rng.normal(scale=np.sqrt(2.0 / (9 * cin)), size=b0 - w0)
+        p[b0:b0 + cout] = rng.normal(scale=0.05, size=cout)
+    # input channels 9..15 and output channels 3..15 are padding
+    w0, b0 = net.offsets[0]
+    p[w0:b0].reshape(net.convs[0][1], 9, 16)[:, :, 9:] = 0.0
+    w0, b0 = net.offsets[-1]
+    p[w0:b0].reshape(16, 9, net.convs[-1][0])[3:] = 0.0
+    p[b0 + 3:b0 + 16] = 0.0
+    net.params = dr.bf16(p)  # the GPU's bf16 copy is then exact
+    beauty = rng.gamma(1.0, 1.0, (n, h, w, 3)).astype(np.float32)
+    albedo = rng.uniform(0, 1, (n, h, w, 3)).astype(np.float32)
+    normal = rng.uniform(0, 1, (n, h, w, 3)).astype(np.float32)
+    target = rng.gamma(1.0, 1.0, (n, h, w, 3)).astype(np.float32)
+    return net, beauty, albedo, normal, target
+
+
+CASES = [(2, (16, 16), 2, 16, 16), (3, (32, 48, 16), 1, 20, 28), (5, (32, 48, 64, 96, 128), 2, 32, 48)]
+
+
+@pytest.mark.parametrize("levels,channels,n,h,w", CASES)
+def test_forward_and_gradients_match_oracle(gpu_context, levels, channels, n, h, w):
+    say(f"denoiser levels {levels} channels {channels} batch {n}x{h}x{w}")
+    net, b, a, nr, t = _problem(levels, channels, n, h, w, seed=levels)
+    dn = Denoiser(gpu_context, levels, channels)
+    assert dn.param_count == net.nparams
+    dn.set_params(net.params.astype(np.float32))
+    np.testing.assert_array_equal(dn.get_params(), net.params.astype(np.float32))
+
+    loss_ref, g_ref, z_ref = dr.loss_and_gradients(net, b.astype(np.float64), a.astype(np.float64),
+                                                   nr.astype(np.float64), t.astype(np.float64), bf=True)
+    rad_gpu = dn.infer(b, a, nr)
+    rad_ref = dr.radiance(z_ref, h, w)
+    # compare in the network's output space, z = log1p(radiance) where z >= 0
+    zg, zr = np.log1p(rad_gpu), np.log1p(rad_ref)
+    err = np.abs(zg - zr)
+    assert np.mean(err <= 0.02 + 0.02 * np.abs(zr)) > 0.995, f"max {err.max()}"
+    assert err.mean() < 2.0 ** -8 * np.sqrt(len(net.convs)), err.mean()
+
+    loss_gpu, g_gpu = dn.loss_and_gradients(b, a, nr, t)
+    assert loss_gpu == pytest.approx(loss_ref, rel=1e-3)
+    for i, (cin, cout, _) in enumerate(net.convs):
+        w0, b0 = net.offsets[i]
+        for name, sl in (("w", slice(w0, b0)), ("b", slice(b0, b0 + cout))):
+            gr, gg = g_ref[sl], g_gpu[sl].astype(np.float64)
+            nref = np.linalg.norm(gr)
+            if nref == 0.0:
+                assert np.linalg.norm(gg) < 1e-12
+                continue
+            rel = np.linalg.norm(gg - gr) / nref
+            cos = gg @ gr / (np.linalg.norm(gg) * nref)
+            assert rel < 0.01 * np.sqrt(len(net.convs)) and cos > 0.998, \
+                f"conv {i} {name}: rel {rel:.4f} cos {cos:.6f}"
+    dn.close()
+
+
+def test_gradients_are_deterministic(gpu_context):
+    net, b, a, nr, t = _problem(3, (32, 48, 16), 2, 24, 24, seed=9)
+    dn = Denoiser(gpu_context, 3, (32, 48, 16))
+    dn.set_params(net.params.astype(np.float32))
+    l1, g1 = dn.loss_and_gradients(b, a, nr, t)
+    l2, g2 = dn.loss_and_gradients(b, a, nr, t)
+    assert l1 == l2
+    np.testing.assert_array_equal(g1, g2)
+    dn.close()
+
+
+def test_training_reduces_loss(gpu_context):
+    """Adam steps on a fixed synthetic task (target = the clean version of a
+    noisy beauty) must drive the loss down."""
+    rng = np.random.default_rng(5)
+    n, h, w = 4, 32, 32
+    clean = np.repeat(np.repeat(rng.uniform(0.2, 2.0, (n, h // 8, w // 8, 3)), 8, axis=1), 8, axis=2)
+    noisy = clean * rng.gamma(4.0, 0.25, clean.shape)
+    albedo = np.clip(clean / 2.0, 0, 1)
+    normal = np.full(clean.shape, 0.5)
+    args = [x.astype(np.float32) for x in (noisy, albedo, normal, clean)]
+    dn = Denoiser(gpu_context, 3, (32, 48, 64), seed=2)
+    first = dn.train_step(*args, lr=2e-3)
+    for _ in range(60):
+        last = dn.train_step(*args, lr=2e-3)
+    assert np.isfinite(last) and last < 0.5 * first, (first, last)
+    dn.close()
