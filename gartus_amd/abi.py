@@ -15,6 +15,7 @@ GT_OK, GT_EINVAL, GT_ENOMEM, GT_EDEVICE, GT_ESTATE, GT_EUNSUPPORTED = range(6)
 GT_PIXEL_RANDOM, GT_PIXEL_STRATIFIED, GT_PIXEL_STRATIFIED_GRID = range(3)
 GT_LIGHTING_CONTINUATION, GT_LIGHTING_NEE = range(2)
 GT_BACKGROUND_CONSTANT, GT_BACKGROUND_VERTICAL_GRADIENT = range(2)
+GT_FLAG_GGX_REJECT_BELOW_HORIZON = 1
 GT_RNG_COUNTER, GT_RNG_PIXEL_STREAM = range(2)
 
 
@@ -41,7 +42,7 @@ class GtCamera(C.Structure):
         ("background", C.c_double * 9),
         ("light_pdf_weight", C.c_double),
         ("rng_mode", C.c_uint32),
-        ("reserved", C.c_uint32),
+        ("flags", C.c_uint32),
     ]
 
 
@@ -202,7 +203,12 @@ _lib = None
 
 
 def load() -> C.CDLL:
-    """Loads libgartus_hip.so (built by ``gartus_amd.build``). No fallback."""
+    """Loads libgartus_hip.so (built by ``gartus_amd.build``). No fallback.
+
+    The HIP runtime is per process. A process that also uses PyTorch on the
+    GPU must import torch first: torch ships its own libamdhip64, and the
+    library then binds to that copy (same soname) instead of loading
+    /opt/rocm's, which could not open the device a second time."""
     global _lib
     if _lib is None:
         if not LIB_PATH.exists():

@@ -279,6 +279,7 @@ int make_camera(const gt_camera* c, GCamera* g) {
   int nbg = c->background_kind == GT_BACKGROUND_VERTICAL_GRADIENT ? 9 : 3;
   for (int i = 0; i < nbg; ++i)
     if (!std::isfinite(c->background[i])) return fail(GT_EINVAL, "background color components must be finite");
+  if (c->flags & ~uint32_t(GT_FLAG_GGX_REJECT_BELOW_HORIZON)) return fail(GT_EINVAL, "unknown camera flags");
   if (c->rng_mode != GT_RNG_COUNTER)
     return fail(GT_EUNSUPPORTED, "the GPU path draws per-(pixel, sample, bounce) streams only (rng_mode GT_RNG_COUNTER)");
   std::memset(g, 0, sizeof *g);
@@ -321,6 +322,7 @@ int make_camera(const gt_camera* c, GCamera* g) {
   g->max_depth = c->max_depth;
   g->rr_min_depth = c->russian_roulette_min_depth;
   g->background_kind = c->background_kind;
+  g->flags = c->flags;
   for (int i = 0; i < 9; ++i) g->background[i] = float(c->background[i]);
   return GT_OK;
 }

@@ -155,7 +155,9 @@ struct MatPdf {
     float ndh = dot(half, basis.w);
     return ggx_D(ndh, roughness) * fmaxf(ndh, 0.f) / (4.0f * fabsf(odh));
   }
-  __device__ f3 generate(Rng& rng) const {
+  // *below (when given) reports a GGX reflection below the surface, where
+  // the reference falls back to the outgoing direction.
+  __device__ f3 generate(Rng& rng, bool* below = nullptr) const {
     if (!ggx) return basis.local(rng.cosine_direction());
     float r1 = rng.next(), r2 = rng.next();
     float phi = 2.0f * PI * r1;
@@ -168,7 +170,9 @@ struct MatPdf {
     sincosf(phi, &s, &c);
     f3 half = basis.local(mk(c * st, s * st, ct));
     f3 refl = reflected(-outgoing, normalized(half));
-    return dot(refl, basis.w) > 0.f ? refl : outgoing;
+    const bool up = dot(refl, basis.w) > 0.f;
+    if (below) *below = !up;
+    return up ? refl : outgoing;
   }
 };
 // ggx_reflection_scattering_weight (material.rs:1686-1722)

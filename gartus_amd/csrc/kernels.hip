@@ -20,6 +20,8 @@
 // NEE, next bounce), so a path's result does not depend on queue order.
 #include "kernels.h"
 
+#include "../../include/gartus_hip.h"
+
 using namespace gtd;
 
 namespace gtk {
@@ -356,19 +358,21 @@ __device__ __forceinline__ uint32_t shade_scatter(const Frame& f, const Queues& 
   f3 dir;
   float cpdf;
   bool suppress, weight_env;
+  bool below = false;
   if (!f.nee && f.has_lights) {
     float wgt = f.light_weight;
     if (rng.next() < wgt) dir = lights_random(f.lights, p, it.time, rng);
-    else dir = pdf.generate(rng);
+    else dir = pdf.generate(rng, &below);
     cpdf = wgt * lights_pdf(f.lights, p, it.time, dir) + (1.0f - wgt) * pdf.value(dir);
     suppress = false;
     weight_env = false;
   } else {
-    dir = pdf.generate(rng);
+    dir = pdf.generate(rng, &below);
     cpdf = pdf.value(dir);
     suppress = f.nee && f.has_lights;
     weight_env = f.nee && has_env;
   }
+  if (below && (f.cam.flags & GT_FLAG_GGX_REJECT_BELOW_HORIZON)) return out;
   if (!(isfinite(cpdf) && cpdf > EPS)) return out;
   float spdf = scattering_pdf(m, it.d, n, dir);
   if (!(isfinite(spdf) && spdf > 0.f)) return out;

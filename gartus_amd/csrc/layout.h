@@ -83,4 +83,5 @@ struct GCamera {
   int32_t rr_min_depth;
   uint32_t background_kind;
   float background[9];
+  uint32_t flags;  // GT_FLAG_* of gt_camera
 };

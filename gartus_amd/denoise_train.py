@@ -1,0 +1,180 @@
+"""Training data, training loop and error measurement for the learned denoiser.
+
+Data. Seeded scenes in the style of the 1080p GGX/environment workload
+(scenes.c3_style_scene) are rendered by this library's own path tracer as
+256x256 crops:
+  - inputs: a 4-spp render with its first-hit albedo and normal AOVs
+    (gt_render_aovs_device);
+  - target: a 1024-spp render of the same crop (gt_render_hdr_device) with
+    another RNG seed, so input and target noise are independent.
+All images stay in device memory as torch tensors; torch supplies the memory
+only, and the network runs on the HIP library.
+
+Error. relMSE(x, r) = mean over pixels and channels of (x - r)^2 / (r^2 + eps),
+with eps = 1e-2, against a high-spp reference accumulated in FP64.
+
+Usage, on a GPU (writes the trained parameters as .npy):
+    python -m gartus_amd.denoise_train --scenes 400 --steps 3000 --out weights.npy
+"""
+from __future__ import annotations
+
+import argparse
+import ctypes as C
+import sys
+import time
+
+import numpy as np
+# torch (device memory only) must load its HIP runtime before libgartus_hip.so does: the
+# library then binds to the same runtime instead of opening the device a second time
+import torch  # noqa: F401
+
+from . import abi, api, scenes
+from .denoiser import Denoiser
+
+RELMSE_EPS = 1e-2
+TRIM = 1e-3  # fraction of pixels with the largest error left out of the robust relMSE
+
+
+def _pixel_relmse(x, ref, eps):
+    """Per-pixel relMSE (mean over the 3 channels) as a float64 torch or numpy array."""
+    if type(x).__module__.startswith("torch"):
+        d = x.double() - ref.double()
+        return (d * d / (ref.double() ** 2 + eps)).mean(dim=-1).flatten()
+    x = np.asarray(x, np.float64)
+    ref = np.asarray(ref, np.float64)
+    return ((x - ref) ** 2 / (ref ** 2 + eps)).mean(axis=-1).ravel()
+
+
+def relmse(x, ref, eps: float = RELMSE_EPS) -> float:
+    """Mean of (x - ref)^2 / (ref^2 + eps) over pixels and channels (torch or numpy)."""
+    e = _pixel_relmse(x, ref, eps)
+    return float(e.mean())
+
+
+def relmse_trimmed(x, ref, eps: float = RELMSE_EPS, trim: float = TRIM) -> float:
+    """relMSE without the `trim` fraction of pixels with the largest error: the
+    usual firefly-robust form, since a single heavy-tailed sample can dominate
+    the plain mean."""
+    e = _pixel_relmse(x, ref, eps)
+    n = e.shape[0]
+    keep = n - int(n * trim)
+    if type(e).__module__.startswith("torch"):
+        import torch
+        return float(torch.sort(e).values[:keep].mean())
+    return float(np.sort(e)[:keep].mean())
+
+
+class SceneHandle:
+    """A compiled scene with its environment and camera, ready to render."""
+
+    def __init__(self, ctx: api.Context, data, env_rgb, cam_spec):
+        self.ctx = ctx
+        self.scene = api.RayScene(data, ctx).with_bvh()
+        self.env = api.EnvironmentLight.from_rgb32f(env_rgb)
+        self.sh = self.scene._compile()
+        self.eh = self.env._compile(ctx)
+        self.cam = scenes.camera_record(cam_spec)
+        self.cam.direct_lighting_mode = abi.GT_LIGHTING_NEE
+        self.w = self.cam.image_width
+        self.h = int(self.ctx.lib.gt_camera_image_height(C.byref(self.cam)))
+
+    def camera(self, spp: int, seed: int):
+        c = abi.GtCamera()
+        C.memmove(C.byref(c), C.byref(self.cam), C.sizeof(c))
+        c.samples_per_pixel = spp
+        c.pixel_sample_mode = abi.GT_PIXEL_RANDOM
+        c.rng_seed = seed
+        return c
+
+    def render(self, out, spp: int, seed: int, double: bool = False) -> None:
+        """Renders into a device tensor out [h][w][3] (float32, or float64 with double)."""
+        c = self.camera(spp, seed)
+        abi.check(self.ctx.lib.gt_render_hdr_device(self.ctx.handle, self.sh, C.byref(c), None, self.eh,
+                                                    C.c_void_p(out.data_ptr()), 1 if double else 0, None))
+        self.ctx.synchronize()
+
+    def render_aovs(self, beauty, albedo, normal, spp: int, seed: int) -> None:
+        c = self.camera(spp, seed)
+        abi.check(self.ctx.lib.gt_render_aovs_device(
+            self.ctx.handle, self.sh, C.byref(c), None, self.eh, C.cast(beauty.data_ptr(), abi.F),
+            C.cast(albedo.data_ptr(), abi.F), C.cast(normal.data_ptr(), abi.F), None))
+        self.ctx.synchronize()
+
+
+def make_dataset(ctx: api.Context, n_scenes: int, first_seed: int = 10_000, size: int = 256, spp_in: int = 4,
+                 spp_ref: int = 1024, log=print):
+    """Device tensors [n][size][size][3]: beauty, albedo, normal (spp_in) and target (spp_ref)."""
+    import torch
+    shape = (n_scenes, size, size, 3)
+    data = {k: torch.empty(shape, dtype=torch.float32, device="cuda") for k in ("beauty", "albedo", "normal", "target")}
+    t0 = time.perf_counter()
+    for i in range(n_scenes):
+        seed = first_seed + i
+        sd, env, cam = scenes.c3_style_scene(seed, size, size)
+        sh = SceneHandle(ctx, sd, env, cam)
+        torch.cuda.synchronize()
+        sh.render_aovs(data["beauty"][i], data["albedo"][i], data["normal"][i], spp_in, 2 * seed)
+        sh.render(data["target"][i], spp_ref, 2 * seed + 1)
+        if (i + 1) % 50 == 0 or i + 1 == n_scenes:
+            log(f"dataset: {i + 1}/{n_scenes} scenes rendered ({time.perf_counter() - t0:.1f} s)")
+    return data
+
+
+def augment(batch, k: int):
+    """The k-th of the 8 flips/transposes of a [n][h][w][3] square batch."""
+    if k & 1:
+        batch = batch.flip(2)
+    if k & 2:
+        batch = batch.flip(1)
+    if k & 4:
+        batch = batch.transpose(1, 2)
+    return batch.contiguous()
+
+
+def train(dn: Denoiser, data, steps: int, batch: int = 16, lr: float = 1e-4, seed: int = 0, log=print,
+          log_every: int = 100):
+    import torch
+    n = data["beauty"].shape[0]
+    g = torch.Generator(device="cpu")
+    g.manual_seed(seed)
+    losses = []
+    t0 = time.perf_counter()
+    for step in range(steps):
+        idx = torch.randint(0, n, (batch,), generator=g).cuda()
+        k = int(torch.randint(0, 8, (1,), generator=g))
+        b, a, nr, t = (augment(data[key].index_select(0, idx), k) for key in ("beauty", "albedo", "normal", "target"))
+        losses.append(dn.train_step(b, a, nr, t, lr=lr))
+        if (step + 1) % log_every == 0 or step + 1 == steps:
+            recent = float(np.mean(losses[-log_every:]))
+            log(f"train: step {step + 1}/{steps} loss {recent:.5f} ({time.perf_counter() - t0:.1f} s)")
+    return losses
+
+
+def main(argv=None) -> int:
+    ap = argparse.ArgumentParser(description=__doc__.split("\n\n")[0])
+    ap.add_argument("--scenes", type=int, default=400)
+    ap.add_argument("--steps", type=int, default=3000)
+    ap.add_argument("--batch", type=int, default=16)
+    ap.add_argument("--lr", type=float, default=1e-4)
+    ap.add_argument("--spp-ref", type=int, default=1024)
+    ap.add_argument("--init", default="", help="start from these parameters (.npy)")
+    ap.add_argument("--out", required=True)
+    args = ap.parse_args(argv)
+
+    def log(m):
+        print(m, flush=True)
+
+    ctx = api.default_context()
+    dn = Denoiser(ctx)
+    if args.init:
+        dn.load(args.init)
+    log(f"denoiser: {dn.param_count} parameters")
+    data = make_dataset(ctx, args.scenes, spp_ref=args.spp_ref, log=log)
+    train(dn, data, args.steps, args.batch, args.lr, log=log)
+    dn.save(args.out)
+    log(f"saved {args.out}")
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

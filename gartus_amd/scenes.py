@@ -282,7 +282,8 @@ def sky_environment(width: int = 2048, height: int = 1024, sun_radiance: float =
     return rgb.astype(np.float32), sun
 
 
-def ggx_environment(n_random: int = 1000, seed: int = 3, env_size=(2048, 1024)):
+def ggx_environment(n_random: int = 1000, seed: int = 3, env_size=(2048, 1024), sun_elevation_deg: float = 35.0,
+                    sun_azimuth_deg: float = 60.0):
     """10x10 GGX sphere grid (roughness 0.02..1.0 across, metallic 0/1 by
     row) + n_random Dielectric/Metal/Lambertian spheres on a ground sphere.
     Metallic 1 = GgxMicrofacet(base colour, r); metallic 0 =
@@ -316,10 +317,31 @@ def ggx_environment(n_random: int = 1000, seed: int = 3, env_size=(2048, 1024)):
         spheres.append([x, r, z, r])
         placed += 1
     s.add(SPHERE, spheres, np.array(mids, np.uint32))
-    env, _ = sky_environment(*env_size)
+    env, _ = sky_environment(*env_size, sun_elevation_deg=sun_elevation_deg, sun_azimuth_deg=sun_azimuth_deg)
     cam = dict(width=1920, aspect=16.0 / 9.0, spp=16, depth=10, vfov=32.0, lookfrom=(0.0, 5.5, 15.0),
                lookat=(0.0, 0.3, 0.0), seed=1, nee=True)
     return s, env, cam
+
+
+def c3_style_scene(seed: int, width: int = 256, height: int = 256):
+    """A training scene in the style of the GGX/environment workload, from
+    SampleRng(seed): its own random sphere field (150-600 spheres), sun
+    direction (elevation 12-70 degrees, any azimuth) and camera (orbit,
+    height, distance, field of view, look-at jitter), rendered at
+    width x height. Returns (SceneData, env rgb, camera spec)."""
+    rng = SampleRng(seed)
+    n_random = 150 + int(rng.random_double() * 450)
+    elev = rng.random_range(12.0, 70.0)
+    azim = rng.random_range(0.0, 360.0)
+    data, env, cam = ggx_environment(n_random=n_random, seed=seed * 7 + 1, sun_elevation_deg=elev,
+                                     sun_azimuth_deg=azim)
+    ang = rng.random_range(0.0, 2.0 * math.pi)
+    dist = rng.random_range(6.0, 18.0)
+    eye = (dist * math.sin(ang), rng.random_range(1.0, 7.0), dist * math.cos(ang))
+    at = (rng.random_range(-3.0, 3.0), rng.random_range(0.0, 0.8), rng.random_range(-3.0, 3.0))
+    cam.update(width=width, aspect=width / height, vfov=rng.random_range(20.0, 50.0), lookfrom=eye, lookat=at,
+               seed=1 + seed)
+    return data, env, cam
 
 
 def bench_scene():

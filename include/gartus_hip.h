@@ -55,6 +55,16 @@ enum { GT_BACKGROUND_CONSTANT = 0, GT_BACKGROUND_VERTICAL_GRADIENT = 1 };
  * GPU uses; GT_RNG_PIXEL_STREAM is the reference's one sequential stream per
  * pixel (camera.rs:2669), available in the CPU oracle only. */
 enum { GT_RNG_COUNTER = 0, GT_RNG_PIXEL_STREAM = 1 };
+/* gt_camera.flags. 0 renders exactly the reference's estimator.
+ * GT_FLAG_GGX_REJECT_BELOW_HORIZON ends a path whose GGX continuation
+ * sample reflects below the surface. The reference's
+ * GgxReflectionPdf::generate (sampling.rs:313-318) returns the outgoing
+ * direction there, and ray_color then weights that retro-reflection by
+ * scattering_pdf / pdf as if it were a density sample. Between facing
+ * GGX surfaces this compounds (x88 per bounce measured), giving a biased
+ * estimator with unbounded variance. Rejecting is the unbiased choice,
+ * since the BSDF is zero below the horizon; the RNG draws are unchanged. */
+enum { GT_FLAG_GGX_REJECT_BELOW_HORIZON = 1 };
 
 typedef struct gt_camera {
   uint32_t image_width;             /* camera.rs:1035, > 0 */
@@ -77,7 +87,7 @@ typedef struct gt_camera {
   double background[9];             /* constant: [0..2]; gradient: nadir, horizon, zenith */
   double light_pdf_weight;          /* mixture weight for continuation mode */
   uint32_t rng_mode;                /* GT_RNG_* */
-  uint32_t reserved;
+  uint32_t flags;                   /* GT_FLAG_* (0: the reference's behaviour) */
 } gt_camera;
 
 /* RayCamera::new(100, 1.0) defaults (camera.rs:1034-1064). */

@@ -414,7 +414,9 @@ struct MatPdf {  // MaterialPdf::{Cosine, GgxReflection} (pdf.rs:46-104)
     double ndh = dot(half, basis.w);  // half_vector_pdf :263-266
     return ggx_D(ndh, roughness) * fmaxr(ndh, 0.0) / (4.0 * std::fabs(odh));
   }
-  V3 generate(Rng& rng) const {
+  // *below (when given) reports a GGX reflection below the surface, where
+  // the reference returns the outgoing direction (sampling.rs:313-318).
+  V3 generate(Rng& rng, bool* below = nullptr) const {
     if (kind == 0) return basis.local(rng.cosine_direction());  // sampling.rs:79-81
     double r1 = rng.random_double(), r2 = rng.random_double();  // sampling.rs:283-307
     double phi = 2.0 * PI * r1;
@@ -425,7 +427,9 @@ struct MatPdf {  // MaterialPdf::{Cosine, GgxReflection} (pdf.rs:46-104)
     double st = std::sqrt(fmaxr(1.0 - ct * ct, 0.0));
     V3 half = basis.local(V3(std::cos(phi) * st, std::sin(phi) * st, ct));
     V3 refl = reflected(-outgoing, normalized(half));
-    return dot(refl, basis.w) > 0.0 ? refl : outgoing;
+    const bool up = dot(refl, basis.w) > 0.0;
+    if (below) *below = !up;
+    return up ? refl : outgoing;
   }
 };
 static bool ggx_pdf_new(V3 normal, V3 outgoing, double roughness, MatPdf* out) {  // sampling.rs:183-202
@@ -1180,18 +1184,21 @@ static V3 ray_color(const Ctx& c, Ray ray, uint32_t sample, PathRng& pr, Counter
     // SamplingStrategy::continuation_sample camera.rs:165-195
     V3 dir; double cpdf;
     bool suppress, weight_env;
+    bool below = false;
     if (!c.nee && has_lights) {
       double wgt = c.light_weight;  // MixturePdf::weighted clamps (sampling.rs:355-365)
       if (rng.random_double() < wgt) dir = lights_random_direction(*c.lights, rec.s.point, ray.time, rng);
-      else dir = sc.pdf.generate(rng);
+      else dir = sc.pdf.generate(rng, &below);
       cpdf = wgt * lights_pdf_value(*c.lights, rec.s.point, ray.time, dir) + (1.0 - wgt) * sc.pdf.value(dir);
       suppress = false; weight_env = false;
     } else {
-      dir = sc.pdf.generate(rng);
+      dir = sc.pdf.generate(rng, &below);
       cpdf = sc.pdf.value(dir);
       suppress = c.nee && has_lights;
       weight_env = c.nee && has_env;
     }
+    // opt-in: end the path instead of the reference's retro-reflection fallback
+    if (below && (c.cam->p.flags & GT_FLAG_GGX_REJECT_BELOW_HORIZON)) return color;
     if (!std::isfinite(cpdf) || cpdf <= EPS) return color;
     Ray scattered{rec.s.point, dir, ray.time};
     double spdf = mat_scattering_pdf(*rec.mat, ray, rec, dir);

@@ -150,6 +150,45 @@ def test_render_ggx_environment_matches_oracle(gpu_context):
     assert _compare_images(np.minimum(gpu, 50.0), np.minimum(ref, 50.0), 8, "ggx-env") > 0.9
 
 
+def test_render_ggx_environment_with_rejection_flag_matches_oracle(gpu_context):
+    """GT_FLAG_GGX_REJECT_BELOW_HORIZON on both sides: same streams, same
+    per-pixel agreement; without the compounding fallback no pixel needs
+    clipping."""
+    say("render ggx env 96x54 spp 8 depth 10, reject below-horizon GGX samples")
+    data, env, cam = scenes.ggx_environment(n_random=150, env_size=(256, 128))
+    data = data.as_float32_exact()
+    cam.update(width=96, spp=8)
+    c = scenes.camera_record(cam)
+    c.flags = abi.GT_FLAG_GGX_REJECT_BELOW_HORIZON
+    rs = api.RayScene(data).with_bvh()
+    tracer = api.PathTracer(api.RayCamera.__new__(api.RayCamera))
+    tracer.camera.raw = c
+    gpu = tracer.render_with_environment_hdr_image(rs, api.EnvironmentLight.from_rgb32f(env)).pixels
+    ref, _ = cpu_ref.Scene(data).render(c, None, cpu_ref.Env(rgb32f=env), threads=0)
+    assert _compare_images(gpu, ref, 8, "ggx-env-reject") > 0.9
+
+
+def test_c3_fallback_firefly_and_rejection_on_gpu(gpu_context):
+    """The 1080p GGX/environment frame at 1 spp, seed 5: pixel (760, 717) is
+    the compounding-fallback sample the oracle reproduces (2.9e8, see
+    test_oracle_render); the flag makes it an ordinary sample."""
+    say("render ggx env 1920x1080 spp 1 seed 5, with and without rejection")
+    data, env, cam = scenes.ggx_environment()
+    rs = api.RayScene(data).with_bvh()
+    envl = api.EnvironmentLight.from_rgb32f(env)
+    out = {}
+    for flags in (0, abi.GT_FLAG_GGX_REJECT_BELOW_HORIZON):
+        c = scenes.camera_record(cam)
+        c.rng_seed, c.samples_per_pixel, c.flags = 5, 1, flags
+        tracer = api.PathTracer(api.RayCamera.__new__(api.RayCamera))
+        tracer.camera.raw = c
+        out[flags] = tracer.render_with_environment_hdr_image(rs, envl).pixels
+    assert out[0][717, 760, 0] > 1e8
+    fixed = out[abi.GT_FLAG_GGX_REJECT_BELOW_HORIZON]
+    assert fixed[717, 760].max() < 10.0
+    assert np.percentile(fixed.max(axis=2), 99.99) < np.percentile(out[0].max(axis=2), 99.99)
+
+
 def test_render_aovs_match_oracle(gpu_context):
     say("render aovs final_scene 64x36 spp 4")
     data = scenes.final_scene().as_float32_exact()

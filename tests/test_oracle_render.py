@@ -124,3 +124,30 @@ def test_aov_passes_are_first_hit_albedo_and_normal():
     np.testing.assert_allclose(a[8, 8], [0.2, 0.4, 0.6])
     np.testing.assert_allclose(n[8, 8], [0.5, 0.5, 1.0])
     np.testing.assert_allclose(a[0, 0], [0, 0, 0])
+
+
+def _c3_pixel(flags):
+    """Pixel (760, 717) of the 1920x1080 GGX/environment frame, seed 5, 1 spp:
+    the path enters a gap between two GGX spheres where the reference's
+    GgxReflectionPdf::generate (sampling.rs:313-318) falls back to the
+    outgoing direction five times in a row."""
+    data, env, cam = scenes.ggx_environment()
+    c = scenes.camera_record(cam)
+    c.rng_seed = 5
+    c.samples_per_pixel = 1
+    c.direct_lighting_mode = abi.GT_LIGHTING_NEE
+    c.flags = flags
+    img, _ = cpu_ref.Scene(data).render(c, None, cpu_ref.Env(rgb32f=env), threads=1, crop=(760, 717, 1, 1))
+    return np.asarray(img).reshape(-1)[:3]
+
+
+def test_reference_ggx_fallback_compounds_and_the_flag_rejects_it():
+    """The reference's estimator: each below-horizon fallback retro-reflects
+    with weight scattering_pdf / pdf (x88, x67 here), so this one sample is
+    worth 2.9e8 while the frame's median pixel is about 0.3. With
+    GT_FLAG_GGX_REJECT_BELOW_HORIZON the path ends at the first fallback
+    and the sample is an ordinary one."""
+    ref = _c3_pixel(0)
+    assert ref[0] > 1e8
+    fixed = _c3_pixel(abi.GT_FLAG_GGX_REJECT_BELOW_HORIZON)
+    assert np.all(np.isfinite(fixed)) and fixed.max() < 10.0
