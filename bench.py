@@ -162,6 +162,34 @@ def summarize_counters(paths: list[str]) -> int:
     return 0
 
 
+DENOISER_WEIGHTS = ROOT / "gartus_amd" / "weights" / "denoiser_c3.npy"
+
+
+def denoise_summary(ctx) -> dict:
+    """Time to trimmed relMSE targets on this frame, plain path tracing
+    against path tracing + the trained denoiser (gartus_amd.denoise_eval),
+    with GT_FLAG_GGX_REJECT_BELOW_HORIZON: the reference's own estimator
+    diverges on this scene (its GGX below-horizon fallback compounds)."""
+    from gartus_amd import abi
+    from gartus_amd.denoise_eval import evaluate
+    from gartus_amd.denoiser import Denoiser
+    dn = Denoiser(ctx)
+    dn.load(str(DENOISER_WEIGHTS))
+    r = evaluate(dn, ref_spp=16384, flags=abi.GT_FLAG_GGX_REJECT_BELOW_HORIZON, log=log)
+    dn.close()
+    best_plain = min(r["plain"], key=lambda p: p["relmse_trimmed"])
+    d4 = next(d for d in r["denoised"] if d["spp"] == 4)
+    return {
+        "metric": "time to trimmed relMSE (eps 1e-2, worst 0.1% of pixels left out)",
+        "estimator": "GT_FLAG_GGX_REJECT_BELOW_HORIZON",
+        "reference_spp": r["reference_spp"],
+        "reference_error_relmse_trimmed": round(r["reference_error_relmse_trimmed"], 5),
+        "time_to_targets_s": r["time_to_targets"],
+        "plain_best": {k: best_plain[k] for k in ("spp", "time_s", "relmse_trimmed")},
+        "denoised_4spp": {k: d4[k] for k in ("time_s", "infer_s", "relmse_trimmed", "relmse_trimmed_input")},
+    }
+
+
 def traffic_for(workload: str):
     try:
         rec = json.loads(TRAFFIC_FILE.read_text())
@@ -184,6 +212,8 @@ def main() -> int:
     ap.add_argument("--max-paths", type=int, default=0)
     ap.add_argument("--graph", action="store_true", help="replay the render as a captured hipGraph")
     ap.add_argument("--autotune", action="store_true", help="pick the tuning with gt_autotune (batch64: always)")
+    ap.add_argument("--no-denoise-eval", action="store_true",
+                    help="skip the time-to-relMSE comparison of plain and denoised rendering (ggx_env_1080p)")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="run only a plain device copy of MIB MiB (counter calibration)")
     ap.add_argument("--summarize-counters", nargs="+", metavar="CSV",
@@ -343,6 +373,9 @@ def main() -> int:
             "hbm_measured_gbs": None if traffic is None else round(traffic / (avg_launch_ms * 1e-3) / 1e9, 1),
         },
     }
+    if args.workload == "ggx_env_1080p" and not args.no_denoise_eval and DENOISER_WEIGHTS.exists():
+        log("denoiser: time to relMSE, plain against denoised (well-posed estimator, 2 x 16384-spp reference) ...")
+        result["denoise"] = denoise_summary(ctx)
     if not args.no_cpu_baseline:
         log("CPU baseline ...")
         result["cpu_baseline"] = cpu_baseline(frames[0], args.cpu_seconds)

@@ -8,7 +8,9 @@ Data. Seeded scenes in the style of the 1080p GGX/environment workload
   - target: a 1024-spp render of the same crop (gt_render_hdr_device) with
     another RNG seed, so input and target noise are independent.
 All images stay in device memory as torch tensors; torch supplies the memory
-only, and the network runs on the HIP library.
+and the batch gathers only, and the network runs on the HIP library. A process
+that uses torch must import it before the library loads (abi.load), which
+main() does.
 
 Error. relMSE(x, r) = mean over pixels and channels of (x - r)^2 / (r^2 + eps),
 with eps = 1e-2, against a high-spp reference accumulated in FP64.
@@ -24,9 +26,6 @@ import sys
 import time
 
 import numpy as np
-# torch (device memory only) must load its HIP runtime before libgartus_hip.so does: the
-# library then binds to the same runtime instead of opening the device a second time
-import torch  # noqa: F401
 
 from . import abi, api, scenes
 from .denoiser import Denoiser
@@ -64,10 +63,48 @@ def relmse_trimmed(x, ref, eps: float = RELMSE_EPS, trim: float = TRIM) -> float
     return float(np.sort(e)[:keep].mean())
 
 
+def _ptr(a):
+    """Device address of a torch tensor or a DeviceArray."""
+    return a.data_ptr() if hasattr(a, "data_ptr") else a.ptr.value
+
+
+class DeviceArray:
+    """A device buffer from the library's own allocator (no torch), with a
+    host copy on request."""
+
+    def __init__(self, ctx: api.Context, shape, dtype=np.float32):
+        self.ctx, self.shape, self.dtype = ctx, tuple(shape), np.dtype(dtype)
+        self.nbytes = int(np.prod(self.shape)) * self.dtype.itemsize
+        self.ptr = C.c_void_p()
+        abi.check(ctx.lib.gt_device_malloc(ctx.handle, max(self.nbytes, 4), C.byref(self.ptr)))
+
+    def sub(self, i: int) -> "DeviceArray":
+        """View of entry i along the first axis (shares the memory)."""
+        v = DeviceArray.__new__(DeviceArray)
+        v.ctx, v.shape, v.dtype = self.ctx, self.shape[1:], self.dtype
+        v.nbytes = self.nbytes // self.shape[0]
+        v.ptr = C.c_void_p(self.ptr.value + i * v.nbytes)
+        v._owner = self
+        return v
+
+    def numpy(self) -> np.ndarray:
+        out = np.empty(self.shape, self.dtype)
+        abi.check(self.ctx.lib.gt_memcpy_to_host(self.ctx.handle, out.ctypes.data_as(C.c_void_p), self.ptr,
+                                                 self.nbytes))
+        return out
+
+    def __del__(self):
+        if getattr(self, "_owner", None) is None and getattr(self, "ptr", None) is not None and self.ptr.value:
+            try:
+                self.ctx.lib.gt_device_free(self.ctx.handle, self.ptr)
+            except Exception:
+                pass
+
+
 class SceneHandle:
     """A compiled scene with its environment and camera, ready to render."""
 
-    def __init__(self, ctx: api.Context, data, env_rgb, cam_spec):
+    def __init__(self, ctx: api.Context, data, env_rgb, cam_spec, flags: int = 0):
         self.ctx = ctx
         self.scene = api.RayScene(data, ctx).with_bvh()
         self.env = api.EnvironmentLight.from_rgb32f(env_rgb)
@@ -75,6 +112,7 @@ class SceneHandle:
         self.eh = self.env._compile(ctx)
         self.cam = scenes.camera_record(cam_spec)
         self.cam.direct_lighting_mode = abi.GT_LIGHTING_NEE
+        self.cam.flags = flags
         self.w = self.cam.image_width
         self.h = int(self.ctx.lib.gt_camera_image_height(C.byref(self.cam)))
 
@@ -90,19 +128,19 @@ class SceneHandle:
         """Renders into a device tensor out [h][w][3] (float32, or float64 with double)."""
         c = self.camera(spp, seed)
         abi.check(self.ctx.lib.gt_render_hdr_device(self.ctx.handle, self.sh, C.byref(c), None, self.eh,
-                                                    C.c_void_p(out.data_ptr()), 1 if double else 0, None))
+                                                    C.c_void_p(_ptr(out)), 1 if double else 0, None))
         self.ctx.synchronize()
 
     def render_aovs(self, beauty, albedo, normal, spp: int, seed: int) -> None:
         c = self.camera(spp, seed)
         abi.check(self.ctx.lib.gt_render_aovs_device(
-            self.ctx.handle, self.sh, C.byref(c), None, self.eh, C.cast(beauty.data_ptr(), abi.F),
-            C.cast(albedo.data_ptr(), abi.F), C.cast(normal.data_ptr(), abi.F), None))
+            self.ctx.handle, self.sh, C.byref(c), None, self.eh, C.cast(_ptr(beauty), abi.F),
+            C.cast(_ptr(albedo), abi.F), C.cast(_ptr(normal), abi.F), None))
         self.ctx.synchronize()
 
 
 def make_dataset(ctx: api.Context, n_scenes: int, first_seed: int = 10_000, size: int = 256, spp_in: int = 4,
-                 spp_ref: int = 1024, log=print):
+                 spp_ref: int = 1024, flags: int = 0, log=print):
     """Device tensors [n][size][size][3]: beauty, albedo, normal (spp_in) and target (spp_ref)."""
     import torch
     shape = (n_scenes, size, size, 3)
@@ -111,7 +149,7 @@ def make_dataset(ctx: api.Context, n_scenes: int, first_seed: int = 10_000, size
     for i in range(n_scenes):
         seed = first_seed + i
         sd, env, cam = scenes.c3_style_scene(seed, size, size)
-        sh = SceneHandle(ctx, sd, env, cam)
+        sh = SceneHandle(ctx, sd, env, cam, flags)
         torch.cuda.synchronize()
         sh.render_aovs(data["beauty"][i], data["albedo"][i], data["normal"][i], spp_in, 2 * seed)
         sh.render(data["target"][i], spp_ref, 2 * seed + 1)
@@ -151,6 +189,7 @@ def train(dn: Denoiser, data, steps: int, batch: int = 16, lr: float = 1e-4, see
 
 
 def main(argv=None) -> int:
+    import torch  # noqa: F401  (before the library loads: one HIP runtime per process)
     ap = argparse.ArgumentParser(description=__doc__.split("\n\n")[0])
     ap.add_argument("--scenes", type=int, default=400)
     ap.add_argument("--steps", type=int, default=3000)
@@ -158,6 +197,8 @@ def main(argv=None) -> int:
     ap.add_argument("--lr", type=float, default=1e-4)
     ap.add_argument("--spp-ref", type=int, default=1024)
     ap.add_argument("--init", default="", help="start from these parameters (.npy)")
+    ap.add_argument("--reject-ggx-fallback", action="store_true",
+                    help="render with GT_FLAG_GGX_REJECT_BELOW_HORIZON (the well-posed estimator)")
     ap.add_argument("--out", required=True)
     args = ap.parse_args(argv)
 
@@ -169,7 +210,8 @@ def main(argv=None) -> int:
     if args.init:
         dn.load(args.init)
     log(f"denoiser: {dn.param_count} parameters")
-    data = make_dataset(ctx, args.scenes, spp_ref=args.spp_ref, log=log)
+    flags = abi.GT_FLAG_GGX_REJECT_BELOW_HORIZON if args.reject_ggx_fallback else 0
+    data = make_dataset(ctx, args.scenes, spp_ref=args.spp_ref, flags=flags, log=log)
     train(dn, data, args.steps, args.batch, args.lr, log=log)
     dn.save(args.out)
     log(f"saved {args.out}")

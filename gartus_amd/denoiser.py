@@ -54,6 +54,11 @@ def _is_torch(a) -> bool:
     return type(a).__module__.startswith("torch")
 
 
+def _is_device(a) -> bool:
+    """A library-allocated device buffer (denoise_train.DeviceArray)."""
+    return hasattr(a, "ptr") and hasattr(a, "shape") and not _is_torch(a)
+
+
 class Denoiser:
     """U-Net denoiser: `levels` resolutions with `channels[l]` features."""
 
@@ -103,7 +108,9 @@ class Denoiser:
         for a in arrays:
             if tuple(a.shape)[-3:] != shape[1:] or (len(a.shape) == 4 and a.shape[0] != shape[0]):
                 raise abi.InvalidArgument(abi.GT_EINVAL, "image shapes differ")
-        if _is_torch(beauty):
+        if _is_device(beauty):
+            ptrs = [a.ptr for a in arrays]
+        elif _is_torch(beauty):
             import torch
             for a in arrays:
                 if not (a.is_cuda and a.dtype == torch.float32 and a.is_contiguous()):
@@ -117,9 +124,15 @@ class Denoiser:
     # calls ---------------------------------------------------------------
     def infer(self, beauty, albedo, normal, out=None):
         """Denoised radiance, same shape as beauty (numpy in, numpy out;
-        torch in, torch out unless `out` is given)."""
+        torch in, torch out unless `out` is given; device arrays in, `out`
+        required)."""
         shape, ptrs = self._inputs(beauty, albedo, normal)
         n, h, w, _ = shape
+        if _is_device(beauty):
+            if out is None:
+                raise abi.InvalidArgument(abi.GT_EINVAL, "device-array inputs need a device-array out")
+            abi.check(self.ctx.lib.gt_denoiser_infer(self.handle, n, h, w, *ptrs, out.ptr))
+            return out
         if _is_torch(beauty):
             import torch
             res = out if out is not None else torch.empty((n, h, w, 3), dtype=torch.float32, device=beauty.device)
