@@ -277,9 +277,10 @@ def main() -> int:
         best = abi.GtTuning()
         ms = C.c_double()
         t0 = time.perf_counter()
-        abi.check(lib.gt_autotune(ctx.handle, n, scenes_h, cams, lights_h, envs_h, 1, C.byref(best), C.byref(ms)))
-        tuned = {"block_size": best.block_size, "bvh_width": best.bvh_width, "max_paths": best.max_paths}
-        log(f"autotune ({time.perf_counter() - t0:.1f} s): {tuned}, {ms.value:.2f} ms at 1 spp")
+        abi.check(lib.gt_autotune(ctx.handle, n, scenes_h, cams, lights_h, envs_h, 0, C.byref(best), C.byref(ms)))
+        tuned = {"block_size": best.block_size, "bvh_width": best.bvh_width, "max_paths": best.max_paths,
+                 "stack_lds": best.stack_lds}
+        log(f"autotune ({time.perf_counter() - t0:.1f} s): {tuned}, {ms.value:.2f} ms per render")
     if args.graph:
         ctx.set_tuning(use_graph=1)
 

@@ -54,6 +54,7 @@ class GtTuning(C.Structure):
         ("bvh_width", C.c_uint32),
         ("use_graph", C.c_uint32),
         ("count_traversal", C.c_uint32),
+        ("stack_lds", C.c_uint32),
     ]
 
 

@@ -347,7 +347,12 @@ uint64_t stack_depth(const gt_scene* s, uint32_t width) {
   return d + 2;
 }
 
-constexpr uint32_t kStackCap = 16;
+// LDS traversal-stack entries per lane (gt_tuning.stack_lds); 8 measured
+// fastest on the 1080p GGX/environment frame (occupancy against spills).
+constexpr uint32_t kDefaultStackLds = 8;
+uint32_t stack_cap(const gt_context* ctx) {
+  return ctx->tuning.stack_lds ? ctx->tuning.stack_lds : kDefaultStackLds;
+}
 
 // Persistent grid: blocks_per_cu per CU, or as many as the occupancy of the
 // traversal kernels allows (their LDS stack is the binding limit).
@@ -355,7 +360,7 @@ uint32_t grid_for(gt_context* ctx) {
   uint32_t bpc = ctx->tuning.blocks_per_cu;
   if (bpc == 0) {
     uint32_t b = ctx->tuning.block_size;
-    size_t lds = gtk::stack_lds_bytes(b, kStackCap);
+    size_t lds = gtk::stack_lds_bytes(b, stack_cap(ctx));
     bpc = std::max<uint32_t>(1, std::min(gtk::occupancy_blocks(0, b, lds), gtk::occupancy_blocks(2, b, lds)));
   }
   return std::max<uint32_t>(1, ctx->cus * bpc);
@@ -405,7 +410,7 @@ gtk::Queues queues_of(gt_context* ctx) {
   q.tally = ctx->tally.p;
   q.gstack = ctx->gstack.p;
   q.gstride = ctx->gstride;
-  q.stack_cap = kStackCap;
+  q.stack_cap = stack_cap(ctx);
   return q;
 }
 
@@ -535,7 +540,7 @@ int run_jobs(gt_context* ctx, std::vector<FrameJob>& jobs, bool multi, int out_i
   }
   int rc;
   if ((rc = ensure_queues(ctx, size_t(max_batch), depth_all)) != GT_OK) return rc;
-  if ((rc = ensure_gstack(ctx, stack_need, kStackCap)) != GT_OK) return rc;
+  if ((rc = ensure_gstack(ctx, stack_need, stack_cap(ctx))) != GT_OK) return rc;
   GT_HIP(ctx->acc.ensure(acc_total));
   uint64_t off = 0;
   for (FrameJob& j : jobs) {
@@ -891,7 +896,9 @@ int gt_context_set_tuning(gt_context* ctx, const gt_tuning* t) {
   if (t->block_size < 64 || t->block_size > 1024 || t->block_size % 64 != 0)
     return fail(GT_EINVAL, "block_size must be a multiple of 64 in 64..1024");
   if (t->bvh_width != 2 && t->bvh_width != 4) return fail(GT_EINVAL, "bvh_width must be 2 or 4");
-  size_t lds = gtk::stack_lds_bytes(t->block_size, kStackCap);
+  if (t->stack_lds != 0 && (t->stack_lds < 4 || t->stack_lds > 32))
+    return fail(GT_EINVAL, "stack_lds must be 0 or 4..32");
+  size_t lds = gtk::stack_lds_bytes(t->block_size, t->stack_lds ? t->stack_lds : kDefaultStackLds);
   if (lds > 64 * 1024) return fail(GT_EINVAL, "block_size too large for the LDS traversal stack");
   ctx->tuning = *t;
   return GT_OK;
@@ -1006,7 +1013,7 @@ int gt_intersect(gt_context* ctx, const gt_scene* cs, const gt_ray* rays, size_t
   gtk::Frame f;
   std::memset(&f, 0, sizeof f);
   f.scene = device_scene(s, ctx->tuning.bvh_width);
-  if ((rc = ensure_gstack(ctx, stack_depth(s, f.scene.width), kStackCap)) != GT_OK) return rc;
+  if ((rc = ensure_gstack(ctx, stack_depth(s, f.scene.width), stack_cap(ctx))) != GT_OK) return rc;
   gtk::Queues q = queues_of(ctx);
   gtk::Launch L{grid_for(ctx), ctx->tuning.block_size, ctx->stream};
   const size_t chunk = size_t(1) << 22;
@@ -1321,13 +1328,15 @@ int gt_autotune(gt_context* ctx, size_t n, const gt_scene* const* scenes, const 
   if (n == 0 || !scenes || !cameras) return fail(GT_EINVAL, "nothing to tune on");
   GT_HIP(hipSetDevice(ctx->device));
   const gt_tuning start = ctx->tuning;
-  // the workload: every frame at sample_spp random samples, no output
+  // the workload: every frame at its own spp (or sample_spp random samples), no output
   std::vector<gt_camera> cams(cameras, cameras + n);
   uint64_t paths = 0;
   for (gt_camera& c : cams) {
-    c.samples_per_pixel = std::max<uint32_t>(sample_spp, 1u);
-    c.pixel_sample_mode = GT_PIXEL_RANDOM;
-    paths += uint64_t(c.image_width) * gt_camera_image_height(&c) * c.samples_per_pixel;
+    if (sample_spp) {
+      c.samples_per_pixel = sample_spp;
+      c.pixel_sample_mode = GT_PIXEL_RANDOM;
+    }
+    paths += uint64_t(c.image_width) * gt_camera_image_height(&c) * gt_camera_effective_spp(&c);
   }
   hipEvent_t e0 = nullptr, e1 = nullptr;
   GT_HIP(hipEventCreate(&e0));
@@ -1336,55 +1345,66 @@ int gt_autotune(gt_context* ctx, size_t n, const gt_scene* const* scenes, const 
     hipEvent_t a, b;
     ~Ev() { (void)hipEventDestroy(a); (void)hipEventDestroy(b); }
   } guard{e0, e1};
+  // time one candidate: a warm run (allocations, code load), then a timed one
+  auto measure = [&](const gt_tuning& t, double* ms_out) -> int {
+    int rc = gt_context_set_tuning(ctx, &t);
+    if (rc != GT_OK) return rc;
+    std::vector<FrameJob> jobs(n);
+    for (size_t i = 0; i < n; ++i) {
+      rc = prepare_job(ctx, scenes[i], &cams[i], lights ? lights[i] : nullptr, envs ? envs[i] : nullptr, &jobs[i]);
+      if (rc != GT_OK) return rc;
+    }
+    const bool multi = n > 1;
+    if ((rc = run_jobs(ctx, jobs, multi, 0, nullptr)) != GT_OK) return rc;
+    GT_HIP(hipEventRecord(e0, ctx->stream));
+    if ((rc = run_jobs(ctx, jobs, multi, 0, nullptr)) != GT_OK) return rc;
+    GT_HIP(hipEventRecord(e1, ctx->stream));
+    GT_HIP(hipEventSynchronize(e1));
+    float ms = 0;
+    GT_HIP(hipEventElapsedTime(&ms, e0, e1));
+    *ms_out = ms;
+    return GT_OK;
+  };
+  gt_tuning best = start;
+  best.count_traversal = 0;
+  best.use_graph = 0;
+  best.max_paths = uint64_t(1) << 25;
+  double best_ms = 1e300;
+  // stage 1: threads per block x BVH width x LDS stack depth at the default queue
   const uint32_t blocks[] = {64, 128, 256};
   const uint32_t widths[] = {2, 4};
-  const uint64_t caps[] = {uint64_t(1) << 23, uint64_t(1) << 24, uint64_t(1) << 25};
-  gt_tuning best = start;
-  double best_ms = 1e300;
-  for (uint32_t width : widths) {
-    for (uint32_t block : blocks) {
-      uint64_t prev_cap = 0;
-      for (uint64_t cap : caps) {
-        // a queue larger than the whole workload behaves like the previous one
-        if (prev_cap >= paths) break;
-        prev_cap = cap;
-        gt_tuning t = start;
+  const uint32_t stacks[] = {8, 16};
+  const gt_tuning base = best;
+  for (uint32_t width : widths)
+    for (uint32_t block : blocks)
+      for (uint32_t stack : stacks) {
+        gt_tuning t = base;
         t.block_size = block;
         t.bvh_width = width;
-        t.max_paths = cap;
-        t.count_traversal = 0;
-        t.use_graph = 0;
-        int rc = gt_context_set_tuning(ctx, &t);
-        if (rc != GT_OK) continue;
-        std::vector<FrameJob> jobs(n);
-        for (size_t i = 0; i < n; ++i) {
-          rc = prepare_job(ctx, scenes[i], &cams[i], lights ? lights[i] : nullptr, envs ? envs[i] : nullptr,
-                           &jobs[i]);
-          if (rc != GT_OK) {
-            ctx->tuning = start;
-            return rc;
-          }
-        }
-        const bool multi = n > 1;
-        if ((rc = run_jobs(ctx, jobs, multi, 0, nullptr)) != GT_OK) {  // warm: allocations, code load
+        t.stack_lds = stack;
+        double ms = 0;
+        int rc = measure(t, &ms);
+        if (rc == GT_EINVAL) continue;  // shape not allowed (LDS)
+        if (rc != GT_OK) {
           ctx->tuning = start;
           return rc;
         }
-        GT_HIP(hipEventRecord(e0, ctx->stream));
-        if ((rc = run_jobs(ctx, jobs, multi, 0, nullptr)) != GT_OK) {
-          ctx->tuning = start;
-          return rc;
-        }
-        GT_HIP(hipEventRecord(e1, ctx->stream));
-        GT_HIP(hipEventSynchronize(e1));
-        float ms = 0;
-        GT_HIP(hipEventElapsedTime(&ms, e0, e1));
-        if (ms < best_ms) {
-          best_ms = ms;
-          best = t;
-        }
+        if (ms < best_ms) { best_ms = ms; best = t; }
       }
+  // stage 2: queue capacity for the winner (capacities beyond twice the workload add nothing)
+  const gt_tuning winner = best;
+  for (uint64_t cap : {uint64_t(1) << 23, uint64_t(1) << 24, uint64_t(1) << 26}) {
+    if (cap > 2 * std::max<uint64_t>(paths, 1)) continue;
+    gt_tuning t = winner;
+    t.max_paths = cap;
+    double ms = 0;
+    int rc = measure(t, &ms);
+    if (rc == GT_EINVAL || rc == GT_ENOMEM) continue;
+    if (rc != GT_OK) {
+      ctx->tuning = start;
+      return rc;
     }
+    if (ms < best_ms) { best_ms = ms; best = t; }
   }
   best.use_graph = start.use_graph;
   best.count_traversal = start.count_traversal;

@@ -120,6 +120,8 @@ typedef struct gt_tuning {
   uint32_t bvh_width;       /* 2 or 4 */
   uint32_t use_graph;       /* capture the per-batch bounce loop in a hipGraph */
   uint32_t count_traversal; /* count node fetches and primitive tests (gt_render_stats) */
+  uint32_t stack_lds;       /* traversal-stack entries per lane kept in LDS (4..32; deeper levels
+                               spill to a per-lane global slice); 0 = 8 */
 } gt_tuning;
 int gt_context_get_tuning(gt_context* ctx, gt_tuning* out);
 int gt_context_set_tuning(gt_context* ctx, const gt_tuning* tuning);
@@ -274,11 +276,13 @@ int gt_render_batch_device(gt_context* ctx, size_t n, const gt_scene* const* sce
 
 /* ---- autotuner ---------------------------------------------------------
  * No reference counterpart (the reference has no device tunables). Times the
- * given frames, each at sample_spp random samples with no output, under
- * every candidate gt_tuning (threads per block 64/128/256 x BVH width 2/4 x
- * queue capacity 2^23/2^24/2^25 paths, capacities above the workload's path
- * count skipped) and makes the fastest the context's tuning. Results do not
- * depend on the tuning, only speed does. out_best and out_best_ms may be NULL. */
+ * given frames, without output, at their own spp (or at sample_spp random
+ * samples when nonzero), under candidate tunings. Stage 1 tries threads per
+ * block 64/128/256 x BVH width 2/4 x LDS stack 8/16 at a 2^25-path queue;
+ * stage 2 tries queue capacities 2^23/2^24/2^26 for the winner (capacities
+ * above twice the workload are skipped). The fastest becomes the context's
+ * tuning. Results do not depend on the tuning, only speed does.
+ * out_best and out_best_ms may be NULL. */
 int gt_autotune(gt_context* ctx, size_t n, const gt_scene* const* scenes,
                 const gt_camera* cameras, const gt_lights* const* lights,
                 const gt_env* const* envs, uint32_t sample_spp,

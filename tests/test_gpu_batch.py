@@ -142,11 +142,13 @@ def test_autotune_picks_a_valid_tuning_and_keeps_images(gpu_context):
         abi.check(gpu_context.lib.gt_autotune(gpu_context.handle, n, scenes_h, cams, lights_h, envs_h, 1,
                                               abi.C.byref(best), abi.C.byref(ms)))
         now = gpu_context.tuning
-        assert best.block_size in (64, 128, 256) and best.bvh_width in (2, 4)
-        assert (now.block_size, now.bvh_width, now.max_paths) == (best.block_size, best.bvh_width, best.max_paths)
+        assert best.block_size in (64, 128, 256) and best.bvh_width in (2, 4) and best.stack_lds in (8, 16)
+        assert (now.block_size, now.bvh_width, now.max_paths, now.stack_lds) == \
+            (best.block_size, best.bvh_width, best.max_paths, best.stack_lds)
         assert 0 < ms.value < 1e4
         for i, (world, cam, L, env) in enumerate(frames):
             img = api.PathTracer(cam)._render(world, cam, L, env).pixels
             np.testing.assert_array_equal(img, eager[i], err_msg=f"frame {i} after autotune")
     finally:
-        gpu_context.set_tuning(block_size=old.block_size, bvh_width=old.bvh_width, max_paths=old.max_paths)
+        gpu_context.set_tuning(block_size=old.block_size, bvh_width=old.bvh_width, max_paths=old.max_paths,
+                               stack_lds=old.stack_lds)
