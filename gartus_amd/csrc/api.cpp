@@ -121,8 +121,8 @@ struct gt_env {
   uint32_t w = 0, h = 0;
   bool uniform = true;
   DevBuf<float4> tex;
-  DevBuf<float> row_cdf, col_cdf, pdf;
-  DevBuf<uint32_t> row_guide, col_guide;
+  DevBuf<float> pdf;
+  DevBuf<uint2> alias;  // (keep threshold, alias texel) per texel
 };
 
 namespace {
@@ -182,6 +182,44 @@ int build_scene(gt_scene* s, uint32_t leaf_size) {
 int ensure_built(gt_scene* s) {
   if (s->built) return GT_OK;
   return build_scene(s, 4);  // RayScene::cached_bvh builds with the default options (scene.rs:1567-1573)
+}
+
+// Vose's alias method (Vose, "A linear algorithm for generating random numbers
+// with a given distribution", IEEE TSE 1991) over weights w[0, n): slot i keeps
+// itself when the low 32 bits of its draw are below thr[i] and otherwise
+// yields alias[i]. Small and large slots are paired from the back of two
+// index-ordered work lists, so the table is a fixed function of the weights.
+uint32_t alias_threshold(double p) {
+  if (!(p > 0.0)) return 0u;
+  if (p >= 1.0) return 0xffffffffu;
+  double t = std::floor(p * 4294967296.0);
+  return t >= 4294967295.0 ? 0xffffffffu : uint32_t(t);
+}
+void build_alias(const std::vector<double>& w, double total, std::vector<uint32_t>* thr,
+                 std::vector<uint32_t>* alias) {
+  const size_t n = w.size();
+  std::vector<double> p(n);
+  std::vector<uint32_t> small, large;
+  thr->assign(n, 0xffffffffu);
+  alias->resize(n);
+  for (size_t i = 0; i < n; ++i) {
+    p[i] = (w[i] * double(n)) / total;
+    (*alias)[i] = uint32_t(i);
+    if (p[i] < 1.0) small.push_back(uint32_t(i));
+    else large.push_back(uint32_t(i));
+  }
+  while (!small.empty() && !large.empty()) {
+    uint32_t sm = small.back();
+    small.pop_back();
+    uint32_t lg = large.back();
+    (*thr)[sm] = alias_threshold(p[sm]);
+    (*alias)[sm] = lg;
+    p[lg] = (p[lg] + p[sm]) - 1.0;
+    if (p[lg] < 1.0) {
+      large.pop_back();
+      small.push_back(lg);
+    }
+  }
 }
 
 GLight pack_light(const gt::HostPrim& p, double weight, double total) {
@@ -428,8 +466,7 @@ int make_frame(gt_context* ctx, const gt_scene* scene, const gt_camera* camera, 
     for (int i = 0; i < 3; ++i) f->env.color[i] = float(env->color[i]);
     f->env.w = env->w; f->env.h = env->h;
     f->env.uniform = env->uniform ? 1u : 0u;
-    f->env.tex = env->tex.p; f->env.row_cdf = env->row_cdf.p; f->env.col_cdf = env->col_cdf.p; f->env.pdf = env->pdf.p;
-    f->env.row_guide = env->row_guide.p; f->env.col_guide = env->col_guide.p;
+    f->env.tex = env->tex.p; f->env.pdf = env->pdf.p; f->env.alias = env->alias.p;
   }
   f->nee = camera->direct_lighting_mode == GT_LIGHTING_NEE ? 1u : 0u;
   f->has_lights = lights ? 1u : 0u;
@@ -1155,7 +1192,7 @@ int gt_env_create_rgb32f(gt_context* ctx, const float* rgb, uint32_t w, uint32_t
   e->w = w; e->h = h;
   const size_t n = size_t(w) * h;
   std::vector<float4> tex(n);
-  std::vector<double> weight(n), row_total(h, 0.0);
+  std::vector<double> weight(n);
   double total = 0.0;
   const double pi = 3.14159265358979323846;
   // rebuild_distribution (environment.rs:158-184) over linear FP32 texels
@@ -1170,21 +1207,11 @@ int gt_env_create_rgb32f(gt_context* ctx, const float* rgb, uint32_t w, uint32_t
       double wt = std::fmax(0.2126 * r + 0.7152 * g + 0.0722 * b, 0.0) * st;
       weight[i] = wt;
       total += wt;
-      row_total[y] += wt;
     }
   }
   e->uniform = total <= kEps;
-  std::vector<float> row_cdf(h), col_cdf(n), pdf(n);
-  double cum = 0.0;
+  std::vector<float> pdf(n);
   for (uint32_t y = 0; y < h; ++y) {
-    double rc = 0.0;
-    for (uint32_t x = 0; x < w; ++x) {
-      size_t i = size_t(y) * w + x;
-      rc += weight[i];
-      col_cdf[i] = row_total[y] > 0.0 ? float(rc / row_total[y]) : float(double(x + 1) / double(w));
-    }
-    cum += row_total[y];
-    row_cdf[y] = e->uniform ? float(double(y + 1) / double(h)) : float(cum / total);
     double t0 = pi * (double(y) / double(h)), t1 = pi * (double(y + 1) / double(h));
     double sa = (2.0 * pi / double(w)) * std::fmax(std::cos(t0) - std::cos(t1), 0.0);  // :186-190
     for (uint32_t x = 0; x < w; ++x) {
@@ -1192,30 +1219,17 @@ int gt_env_create_rgb32f(gt_context* ctx, const float* rgb, uint32_t w, uint32_t
       pdf[i] = (e->uniform || sa <= kEps) ? 0.0f : float((weight[i] / total) / sa);
     }
   }
-  row_cdf[h - 1] = 1.0f;
-  // guide tables over the float CDFs, compared exactly as the device compares
-  auto guide = [](const float* cdf, uint32_t n, uint32_t g, uint32_t* out) {
-    uint32_t i = 0;
-    for (uint32_t k = 0; k <= g; ++k) {
-      float level = float(k) / float(g);
-      while (i < n && cdf[i] < level) ++i;
-      out[k] = i;
-    }
-  };
-  std::vector<uint32_t> row_guide(gtd::kEnvRowGuide + 1), col_guide(size_t(h) * (gtd::kEnvColGuide + 1));
-  guide(row_cdf.data(), h, gtd::kEnvRowGuide, row_guide.data());
-  for (uint32_t y = 0; y < h; ++y)
-    guide(col_cdf.data() + size_t(y) * w, w, gtd::kEnvColGuide, col_guide.data() + size_t(y) * (gtd::kEnvColGuide + 1));
+  std::vector<uint2> alias(n);
+  if (!e->uniform) {
+    std::vector<uint32_t> thr, idx;
+    build_alias(weight, total, &thr, &idx);
+    for (size_t i = 0; i < n; ++i) alias[i] = make_uint2(thr[i], idx[i]);
+  }
   GT_HIP(hipSetDevice(ctx->device));
-  GT_HIP(e->row_guide.ensure(row_guide.size()));
-  GT_HIP(e->col_guide.ensure(col_guide.size()));
-  GT_HIP(hipMemcpy(e->row_guide.p, row_guide.data(), row_guide.size() * 4, hipMemcpyHostToDevice));
-  GT_HIP(hipMemcpy(e->col_guide.p, col_guide.data(), col_guide.size() * 4, hipMemcpyHostToDevice));
-  GT_HIP(e->tex.ensure(n)); GT_HIP(e->row_cdf.ensure(h)); GT_HIP(e->col_cdf.ensure(n)); GT_HIP(e->pdf.ensure(n));
+  GT_HIP(e->tex.ensure(n)); GT_HIP(e->pdf.ensure(n)); GT_HIP(e->alias.ensure(n));
   GT_HIP(hipMemcpy(e->tex.p, tex.data(), n * sizeof(float4), hipMemcpyHostToDevice));
-  GT_HIP(hipMemcpy(e->row_cdf.p, row_cdf.data(), h * sizeof(float), hipMemcpyHostToDevice));
-  GT_HIP(hipMemcpy(e->col_cdf.p, col_cdf.data(), n * sizeof(float), hipMemcpyHostToDevice));
   GT_HIP(hipMemcpy(e->pdf.p, pdf.data(), n * sizeof(float), hipMemcpyHostToDevice));
+  GT_HIP(hipMemcpy(e->alias.p, alias.data(), n * sizeof(uint2), hipMemcpyHostToDevice));
   *out = e.release();
   return GT_OK;
 }

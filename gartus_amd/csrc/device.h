@@ -70,6 +70,10 @@ struct Rng {
     s += 0x9e3779b97f4a7c15ULL;
     return float(uint32_t(mix64(s) >> 40)) * 5.9604644775390625e-8f;
   }
+  __device__ __forceinline__ uint64_t next_u64() {  // next_u64 (random.rs:23-29)
+    s += 0x9e3779b97f4a7c15ULL;
+    return mix64(s);
+  }
   __device__ __forceinline__ float range(float lo, float hi) { return lo + (hi - lo) * next(); }
   __device__ __forceinline__ f3 unit_vector_spherical() {  // random.rs:107-115
     float r1 = next(), r2 = next();
@@ -653,17 +657,13 @@ struct DEnv {
   uint32_t uniform;     // total weight <= EPS: uniform sphere sampling
   float color[3];
   const float4* __restrict__ tex;      // linear RGB texels, row-major, row 0 at +y
-  const float* __restrict__ row_cdf;   // normalised cumulative row weight
-  const float* __restrict__ col_cdf;   // per-row normalised cumulative texel weight
   const float* __restrict__ pdf;       // solid-angle pdf per texel (:136-156)
-  // Guide tables (cutpoint method, Chen & Asau 1974): guide[k] is the first
-  // index whose CDF value is >= k / G. They narrow each inversion to the
-  // entries between guide[k] and guide[k + 1] without changing its result.
-  const uint32_t* __restrict__ row_guide;  // kEnvRowGuide + 1 entries
-  const uint32_t* __restrict__ col_guide;  // h * (kEnvColGuide + 1) entries
+  // Alias table over the texels (Vose's method): slot i keeps texel i when
+  // the low 32 bits of its draw are below .x and otherwise yields texel .y.
+  // It samples the reference's texel distribution (rebuild_distribution,
+  // :158-184) with one draw and two dependent loads instead of a search.
+  const uint2* __restrict__ alias;
 };
-constexpr uint32_t kEnvRowGuide = 4096;
-constexpr uint32_t kEnvColGuide = 64;
 __device__ __forceinline__ float rem_euclid(float x, float m) {
   float r = fmodf(x, m);
   return r < 0.f ? r + fabsf(m) : r;
@@ -687,26 +687,6 @@ __device__ __forceinline__ float env_pdf(const DEnv& e, f3 dir) {  // :136-156
   uint32_t x = fx > 0.f ? min(uint32_t(fx), e.w - 1) : 0u;
   uint32_t y = fy > 0.f ? min(uint32_t(fy), e.h - 1) : 0u;
   return e.pdf[size_t(y) * e.w + x];
-}
-// first index i in [0, n) with cdf[i] >= target (partition_point(< target))
-__device__ __forceinline__ uint32_t lower_bound(const float* __restrict__ cdf, uint32_t n, float target) {
-  uint32_t lo = 0, len = n;
-  while (len > 0) {
-    uint32_t half = len >> 1;
-    if (cdf[lo + half] < target) { lo += half + 1; len -= half + 1; }
-    else len = half;
-  }
-  return lo;
-}
-// lower_bound over cdf[0, n) for `target` in [0, 1), narrowed by a guide table
-// of g + 1 entries; returns the same index as the full search.
-__device__ __forceinline__ uint32_t guided_search(const float* __restrict__ cdf, uint32_t n,
-                                                  const uint32_t* __restrict__ guide, uint32_t g, float target) {
-  uint32_t k = min(uint32_t(target * float(g)), g - 1);
-  uint32_t lo = guide[k];
-  uint32_t hi = min(guide[k + 1], n - 1);
-  if (lo >= hi) return min(lo, n - 1);
-  return lo + lower_bound(cdf + lo, hi - lo, target);
 }
 // The sampled texel's radiance and pdf. The reference evaluates
 // pdf_value(direction) and radiance(direction) from the returned direction
@@ -741,15 +721,15 @@ __device__ __forceinline__ EnvSample env_sample(const DEnv& e, Rng& rng) {  // :
     r.texel = false;
     return r;
   }
-  // The flat row-major CDF search of the reference, split into the row
-  // marginal and the in-row conditional (the same texel for the same draw).
-  float target = rng.next();
-  uint32_t y = guided_search(e.row_cdf, e.h, e.row_guide, kEnvRowGuide, target);
-  float r0 = y == 0 ? 0.f : e.row_cdf[y - 1];
-  float span = e.row_cdf[y] - r0;
-  float local = span > 0.f ? fminf((target - r0) / span, 0.99999994f) : 0.f;
-  uint32_t x = guided_search(e.col_cdf + size_t(y) * e.w, e.w, e.col_guide + size_t(y) * (kEnvColGuide + 1),
-                             kEnvColGuide, local);
+  // The texel: slot = floor(draw * n) from the high bits of draw * n, the
+  // keep-or-alias test from its low bits (one 64-bit draw, integer math).
+  const uint64_t draw = rng.next_u64();
+  const uint64_t n = uint64_t(e.w) * e.h;
+  const uint32_t slot = uint32_t(__umul64hi(draw, n));
+  const uint32_t frac = uint32_t((draw * n) >> 32);
+  const uint2 rec = e.alias[slot];
+  const uint32_t texel = frac < rec.x ? slot : rec.y;
+  const uint32_t x = texel % e.w, y = texel / e.w;
   float u = (float(x) + rng.next()) / float(e.w);
   float t0 = PI * (float(y) / float(e.h)), t1 = PI * (float(y + 1) / float(e.h));
   float c0 = cosf(t0), c1 = cosf(t1);

@@ -888,6 +888,47 @@ static V3 lights_random_direction(const Lights& L, V3 origin, double time, Rng& 
 }
 
 // ---- environment (src/graphics/raytracing/environment.rs) ---------------------
+// Vose's alias method (Vose, "A linear algorithm for generating random numbers
+// with a given distribution", IEEE TSE 1991) over weights w[0, n): slot i keeps
+// itself when the low 32 bits of its draw are below thr[i] and otherwise
+// yields alias[i]. The GPU samples environment texels this way in the
+// per-(pixel, sample, bounce) stream mode (one 64-bit draw, integer
+// arithmetic); this restatement builds the same table in the same order so
+// the oracle makes the same choices. The reference's CDF search stays for
+// the reference stream.
+static uint32_t alias_threshold(double p) {
+  if (!(p > 0.0)) return 0u;
+  if (p >= 1.0) return 0xffffffffu;
+  double t = std::floor(p * 4294967296.0);
+  return t >= 4294967295.0 ? 0xffffffffu : uint32_t(t);
+}
+static void build_alias(const std::vector<double>& w, double total, std::vector<uint32_t>* thr,
+                        std::vector<uint32_t>* alias) {
+  const size_t n = w.size();
+  std::vector<double> p(n);
+  std::vector<uint32_t> small, large;
+  thr->assign(n, 0xffffffffu);
+  alias->resize(n);
+  for (size_t i = 0; i < n; ++i) {
+    p[i] = (w[i] * double(n)) / total;
+    (*alias)[i] = uint32_t(i);
+    if (p[i] < 1.0) small.push_back(uint32_t(i));
+    else large.push_back(uint32_t(i));
+  }
+  while (!small.empty() && !large.empty()) {
+    uint32_t sm = small.back();
+    small.pop_back();
+    uint32_t lg = large.back();
+    (*thr)[sm] = alias_threshold(p[sm]);
+    (*alias)[sm] = lg;
+    p[lg] = (p[lg] + p[sm]) - 1.0;
+    if (p[lg] < 1.0) {
+      large.pop_back();
+      small.push_back(lg);
+    }
+  }
+}
+
 struct Env {
   bool constant = true;
   V3 color;
@@ -896,6 +937,7 @@ struct Env {
   std::vector<float> rgb32;      // linear RGB texels (FP32 HDR form)
   std::vector<uint8_t> rgb8px;   // 8-bit texels (reference form)
   std::vector<double> weights, cdf;
+  std::vector<uint32_t> alias_thr, alias_idx;  // counter-stream texel choice (build_alias)
   double total = 0.0;
 
   V3 texel_linear(uint32_t x, uint32_t y) const {
@@ -921,6 +963,7 @@ struct Env {
         cdf.push_back(total);
       }
     }
+    if (total > EPS) build_alias(weights, total, &alias_thr, &alias_idx);
   }
   // linear_axis (texture.rs:695-711); bilinear (texture.rs:681-692)
   static void axis_repeat(double value, uint32_t size, double* coord, uint32_t* lo, uint32_t* hi) {
@@ -986,15 +1029,26 @@ struct Env {
     double sa = pixel_solid_angle(uint32_t(y));
     return sa <= EPS ? 0.0 : prob / sa;
   }
-  void sample_direction(Rng& rng, V3* dir, double* pdf) const {  // :109-132
+  // alias: the texel from one 64-bit draw through the alias table (counter
+  // streams, the GPU's choice); otherwise the reference's CDF search.
+  void sample_direction(Rng& rng, V3* dir, double* pdf, bool alias = false) const {  // :109-132
     if (total <= EPS || weights.empty()) {
       *dir = rng.unit_vector_spherical();
       *pdf = 1.0 / (4.0 * PI);
       return;
     }
-    double target = rng.random_double() * total;
-    size_t index = size_t(std::lower_bound(cdf.begin(), cdf.end(), target) - cdf.begin());  // partition_point(< target)
-    index = std::min(index, weights.size() - 1);
+    size_t index;
+    if (alias) {
+      const uint64_t x = rng.next_u64(), n = weights.size();
+      const unsigned __int128 m = (unsigned __int128)x * n;
+      const size_t slot = size_t(uint64_t(m >> 64));
+      const uint32_t frac = uint32_t(uint64_t(m) >> 32);
+      index = frac < alias_thr[slot] ? slot : alias_idx[slot];
+    } else {
+      double target = rng.random_double() * total;
+      index = size_t(std::lower_bound(cdf.begin(), cdf.end(), target) - cdf.begin());  // partition_point(< target)
+      index = std::min(index, weights.size() - 1);
+    }
     size_t x = index % w, y = index / w;
     double u = (double(x) + rng.random_double()) / double(w);
     double t0 = PI * (double(y) / double(h)), t1 = PI * (double(y + 1) / double(h));
@@ -1137,7 +1191,7 @@ static V3 estimate_direct(const Ctx& c, const Ray& in, const Hit& h, V3 satt, Rn
 // estimate_environment_lighting camera.rs:2264-2297
 static V3 estimate_env(const Ctx& c, const Ray& in, const Hit& h, V3 satt, Rng& rng, Counters& k) {
   V3 dir; double epdf;
-  c.env->sample_direction(rng, &dir, &epdf);
+  c.env->sample_direction(rng, &dir, &epdf, c.cam->p.rng_mode == GT_RNG_COUNTER);
   if (!std::isfinite(epdf) || epdf <= EPS) return V3();
   Ray shadow{h.s.point, dir, in.time};
   double spdf = mat_scattering_pdf(*h.mat, in, h, dir);
@@ -1412,6 +1466,21 @@ void cr_env_sample(void* ep, uint64_t seed, size_t n, double* dirs, double* pdfs
     static_cast<Env*>(ep)->sample_direction(rng, &d, &p);
     dirs[3 * i] = d.x; dirs[3 * i + 1] = d.y; dirs[3 * i + 2] = d.z; pdfs[i] = p;
   }
+}
+void cr_env_sample_alias(void* ep, uint64_t seed, size_t n, double* dirs, double* pdfs) {
+  Rng rng(seed);
+  for (size_t i = 0; i < n; ++i) {
+    V3 d; double p;
+    static_cast<Env*>(ep)->sample_direction(rng, &d, &p, true);
+    dirs[3 * i] = d.x; dirs[3 * i + 1] = d.y; dirs[3 * i + 2] = d.z; pdfs[i] = p;
+  }
+}
+// The alias table (n entries each; 0 when the map has no weight).
+size_t cr_env_alias_table(void* ep, uint32_t* thr, uint32_t* alias) {
+  const Env* e = static_cast<Env*>(ep);
+  if (thr) std::copy(e->alias_thr.begin(), e->alias_thr.end(), thr);
+  if (alias) std::copy(e->alias_idx.begin(), e->alias_idx.end(), alias);
+  return e->alias_thr.size();
 }
 void cr_env_eval(void* ep, const double* dirs, size_t n, double* radiance, double* pdf) {
   Env* e = static_cast<Env*>(ep);

@@ -53,6 +53,8 @@ _PROTOS = {
     "cr_env_rgb8": (P, [U8, C.c_uint32, C.c_uint32]),
     "cr_env_free": (None, [P]),
     "cr_env_sample": (None, [P, C.c_uint64, C.c_size_t, D, D]),
+    "cr_env_sample_alias": (None, [P, C.c_uint64, C.c_size_t, D, D]),
+    "cr_env_alias_table": (C.c_size_t, [P, C.POINTER(C.c_uint32), C.POINTER(C.c_uint32)]),
     "cr_env_eval": (None, [P, D, C.c_size_t, D, D]),
     "cr_pixel_seed": (C.c_uint64, [C.c_uint64, C.c_uint32, C.c_uint32]),
     "cr_stream_seed": (C.c_uint64, [C.c_uint64, C.c_uint32, C.c_uint32]),
@@ -258,11 +260,22 @@ class Env:
         except Exception:
             pass
 
-    def sample(self, seed: int, n: int):
+    def sample(self, seed: int, n: int, alias: bool = False):
+        """The reference's CDF search, or (alias=True) the alias-table choice
+        the counter-stream mode and the GPU use."""
         dirs = np.zeros((n, 3))
         pdfs = np.zeros(n)
-        lib().cr_env_sample(self.h, seed, n, _dp(dirs), _dp(pdfs))
+        fn = lib().cr_env_sample_alias if alias else lib().cr_env_sample
+        fn(self.h, seed, n, _dp(dirs), _dp(pdfs))
         return dirs, pdfs
+
+    def alias_table(self):
+        n = lib().cr_env_alias_table(self.h, None, None)
+        thr = np.zeros(n, np.uint32)
+        ali = np.zeros(n, np.uint32)
+        lib().cr_env_alias_table(self.h, thr.ctypes.data_as(C.POINTER(C.c_uint32)),
+                                 ali.ctypes.data_as(C.POINTER(C.c_uint32)))
+        return thr, ali
 
     def eval(self, dirs):
         d = _d(dirs).reshape(-1, 3)

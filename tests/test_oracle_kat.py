@@ -566,3 +566,58 @@ def test_product_tone_map_matches_oracle():
         L.cr_tone_map(dp(x), len(x) // 3, e, op, g, ref.ctypes.data_as(C.POINTER(C.c_uint8)))
         got = ToneMap(e, op, g).map(x)
         np.testing.assert_array_equal(got, ref)
+
+
+def _alias_probabilities(thr, alias):
+    """Texel probabilities implied by an alias table: slot j (probability 1/n)
+    keeps j with probability thr[j] / 2^32 and yields alias[j] otherwise."""
+    n = len(thr)
+    keep = thr.astype(np.float64) / 2.0 ** 32
+    own = np.arange(n)
+    p = np.where(alias == own, 1.0, keep)
+    np.add.at(p, alias[alias != own], 1.0 - keep[alias != own])
+    return p / n
+
+
+def test_environment_alias_table_reproduces_the_reference_distribution():
+    """The counter-stream (GPU) texel choice samples exactly the reference's
+    distribution w_i / W (rebuild_distribution, environment.rs:158-184): the
+    alias table's implied probabilities equal it to within the 2^-32
+    threshold quantisation."""
+    rng = np.random.default_rng(7)
+    h, w = 16, 32
+    rgb = rng.uniform(0, 1, (h, w, 3)).astype(np.float32)
+    rgb[3, 7] = 500.0
+    rgb[12, 30] = 0.0
+    e = cpu_ref.Env(rgb32f=rgb)
+    thr, alias = e.alias_table()
+    assert len(thr) == h * w
+    theta = PI * (np.arange(h) + 0.5) / h
+    lum = (0.2126 * rgb[..., 0].astype(np.float64) + 0.7152 * rgb[..., 1] + 0.0722 * rgb[..., 2])
+    wt = np.maximum(lum, 0.0) * np.sin(theta)[:, None]
+    expect = (wt / wt.sum()).ravel()
+    got = _alias_probabilities(thr, alias)
+    assert np.max(np.abs(got - expect)) < 1e-9
+    assert got[12 * w + 30] == 0.0          # a black texel is never chosen
+    assert abs(got.sum() - 1.0) < 1e-12
+
+
+def test_environment_alias_sampling_histogram_matches_pdf():
+    rng = np.random.default_rng(3)
+    rgb = rng.uniform(0, 1, (8, 16, 3)).astype(np.float32)
+    rgb[2, 5] = 40.0
+    e = cpu_ref.Env(rgb32f=rgb)
+    dirs, pdfs = e.sample(11, 200000, alias=True)
+    theta = np.arccos(np.clip(dirs[:, 1], -1, 1))
+    phi = np.mod(np.arctan2(dirs[:, 2], dirs[:, 0]), 2 * PI)
+    x = np.minimum((phi / (2 * PI) * 16).astype(int), 15)
+    y = np.minimum((theta / PI * 8).astype(int), 7)
+    hist = np.bincount(y * 16 + x, minlength=128) / len(dirs)
+    yy = np.arange(8)
+    sa = (2 * PI / 16) * (np.cos(PI * yy / 8) - np.cos(PI * (yy + 1) / 8))
+    thr, alias = e.alias_table()
+    expect = _alias_probabilities(thr, alias)
+    assert np.max(np.abs(hist - expect)) < 0.005
+    # the returned pdf is the texel probability over the texel's solid angle
+    k = y * 16 + x
+    np.testing.assert_allclose(pdfs * sa[k // 16], expect[k], rtol=1e-6)
