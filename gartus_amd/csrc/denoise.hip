@@ -107,23 +107,30 @@ __global__ __launch_bounds__(256) void k_conv(const uint16_t* __restrict__ x, co
 // ---- convolution, weight gradient ------------------------------------------
 // Block = one wave. blockIdx.x = (tap, co group, ci group), blockIdx.y =
 // pixel split. The wave sums its split's pixels 16 at a time into
-// NCO x NCI fp32 tiles and writes them to partial[split].
+// NCO x NCI fp32 tiles and writes them to partial[split]. The waves of the
+// centre tap and the first ci group also multiply their dPre fragments by an
+// all-ones B operand: column 0 of that product is the bias gradient
+// sum_p dPre[p][co], written to bpartial[split].
 template <int NCO, int NCI>
 __global__ __launch_bounds__(64) void k_wgrad(const uint16_t* __restrict__ x, int cin,
                                               const uint16_t* __restrict__ dy, const uint16_t* __restrict__ mask,
                                               int cout, int H, int W, int P, int split_px, int ncog,
-                                              float* __restrict__ partial) {
+                                              float* __restrict__ partial, float* __restrict__ bpartial) {
   const int lane = threadIdx.x;
   const int tap = blockIdx.x % 9;
   const int g = blockIdx.x / 9;
   const int co0 = (g % ncog) * NCO * 16, ci0 = (g / ncog) * NCI * 16;
   const int dyo = tap / 3 - 1, dxo = tap % 3 - 1;
   const int pbeg = blockIdx.y * split_px, pend = min(P, pbeg + split_px);
-  v4f acc[NCO][NCI];
+  v4f acc[NCO][NCI], bacc[NCO];
 #pragma unroll
-  for (int a = 0; a < NCO; ++a)
+  for (int a = 0; a < NCO; ++a) {
+    bacc[a] = v4f{0.f, 0.f, 0.f, 0.f};
 #pragma unroll
     for (int b = 0; b < NCI; ++b) acc[a][b] = v4f{0.f, 0.f, 0.f, 0.f};
+  }
+  const bool bias = tap == 4 && ci0 == 0;  // uniform over the wave
+  const v4s ones = v4s{short(0x3f80), short(0x3f80), short(0x3f80), short(0x3f80)};  // bf16 1.0
   for (int pk0 = pbeg; pk0 < pend; pk0 += 16) {
     // the 4 pixels of this lane's K slots
     size_t own[4], nb[4];
@@ -168,6 +175,10 @@ __global__ __launch_bounds__(64) void k_wgrad(const uint16_t* __restrict__ x, in
     for (int a = 0; a < NCO; ++a)
 #pragma unroll
       for (int b = 0; b < NCI; ++b) acc[a][b] = mfma(A[a], B[b], acc[a][b]);
+    if (bias) {
+#pragma unroll
+      for (int a = 0; a < NCO; ++a) bacc[a] = mfma(A[a], ones, bacc[a]);
+    }
   }
   float* out = partial + size_t(blockIdx.y) * cout * 9 * cin;
 #pragma unroll
@@ -180,19 +191,12 @@ __global__ __launch_bounds__(64) void k_wgrad(const uint16_t* __restrict__ x, in
         const int ci = ci0 + b * 16 + (lane & 15);
         out[(size_t(co) * 9 + tap) * cin + ci] = acc[a][b][r];
       }
-}
-
-// Bias gradient partials: block = one split, lanes stride over channels.
-__global__ void k_bgrad(const uint16_t* __restrict__ dy, const uint16_t* __restrict__ mask, int cout, int P,
-                        int split_px, float* __restrict__ partial) {
-  const int pbeg = blockIdx.x * split_px, pend = min(P, pbeg + split_px);
-  for (int co = threadIdx.x; co < cout; co += blockDim.x) {
-    float s = 0.f;
-    for (int p = pbeg; p < pend; ++p) {
-      const size_t i = size_t(p) * cout + co;
-      if (!mask || bf2f(mask[i]) > 0.f) s += bf2f(dy[i]);
-    }
-    partial[size_t(blockIdx.x) * cout + co] = s;
+  if (bias && (lane & 15) == 0) {  // column 0 of the all-ones product
+#pragma unroll
+    for (int a = 0; a < NCO; ++a)
+#pragma unroll
+      for (int r = 0; r < 4; ++r)
+        bpartial[size_t(blockIdx.y) * cout + co0 + a * 16 + (lane >> 4) * 4 + r] = bacc[a][r];
   }
 }
 
@@ -423,7 +427,7 @@ void launch_conv(hipStream_t s, int nt, const uint16_t* x, const uint16_t* mask,
 #undef GDN_NT
 }
 
-constexpr int kSplitPx = 4096;  // pixels per weight-gradient split
+constexpr int kSplitPx = 1024;  // pixels per weight-gradient split
 
 }  // namespace
 
@@ -460,13 +464,12 @@ void conv_wgrad(hipStream_t s, const uint16_t* x, int cin, const uint16_t* dy, c
   const dim3 grid(9 * ncog * ncig, splits), block(64);
 #define GDN_WG(A, B)                                                                                           \
   if (nco == A && nci == B)                                                                                     \
-  hipLaunchKernelGGL((k_wgrad<A, B>), grid, block, 0, s, x, cin, dy, y, cout, d.h, d.w, P, kSplitPx, ncog, pw)
+  hipLaunchKernelGGL((k_wgrad<A, B>), grid, block, 0, s, x, cin, dy, y, cout, d.h, d.w, P, kSplitPx, ncog, pw, pb)
   GDN_WG(1, 1); GDN_WG(1, 2); GDN_WG(1, 3); GDN_WG(1, 4);
   GDN_WG(2, 1); GDN_WG(2, 2); GDN_WG(2, 3); GDN_WG(2, 4);
   GDN_WG(3, 1); GDN_WG(3, 2); GDN_WG(3, 3); GDN_WG(3, 4);
   GDN_WG(4, 1); GDN_WG(4, 2); GDN_WG(4, 3); GDN_WG(4, 4);
 #undef GDN_WG
-  hipLaunchKernelGGL(k_bgrad, dim3(splits), dim3(64), 0, s, dy, y, cout, P, kSplitPx, pb);
   hipLaunchKernelGGL(k_reduce, dim3(grid_for(nw)), dim3(256), 0, s, pw, splits, nw, dw);
   hipLaunchKernelGGL(k_reduce, dim3(grid_for(cout)), dim3(256), 0, s, pb, splits, size_t(cout), db);
 }
