@@ -128,6 +128,14 @@ U32 = C.POINTER(C.c_uint32)
 I = C.c_int
 
 # name -> (restype, argtypes); every exported symbol of include/gartus_hip.h
+GT_TONE_NONE, GT_TONE_REINHARD, GT_TONE_ACES = 0, 1, 2
+GT_ENCODE_CANVAS, GT_ENCODE_RAW, GT_ENCODE_RGBE, GT_ENCODE_PFM = 0, 1, 2, 3
+
+
+class GtToneMap(C.Structure):
+    _fields_ = [("exposure", C.c_double), ("gamma", C.c_double), ("op", C.c_uint32), ("reserved", C.c_uint32)]
+
+
 class GtDenoiserConfig(C.Structure):
     _fields_ = [
         ("levels", C.c_uint32),
@@ -198,6 +206,11 @@ PROTOTYPES = {
     "gt_denoiser_infer": (I, [P, C.c_uint32, C.c_uint32, C.c_uint32, P, P, P, P]),
     "gt_denoiser_loss_and_gradients": (I, [P, C.c_uint32, C.c_uint32, C.c_uint32, P, P, P, P, F, F]),
     "gt_denoiser_train_step": (I, [P, C.c_uint32, C.c_uint32, C.c_uint32, P, P, P, P, C.c_float, F]),
+    # output boundary
+    "gt_encoded_size": (C.c_uint64, [C.c_int, C.c_uint32, C.c_uint32]),
+    "gt_encoded_header": (I, [C.c_int, C.c_uint32, C.c_uint32, C.c_char_p, C.c_size_t, C.POINTER(C.c_size_t)]),
+    "gt_encode_image": (I, [P, P, C.c_int, C.c_int, C.c_uint32, C.c_uint32, C.c_int, C.POINTER(GtToneMap),
+                            P, C.c_int]),
 }
 
 _lib = None

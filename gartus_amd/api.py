@@ -599,6 +599,34 @@ class ToneMappingOperator:
     NONE, REINHARD, ACES = 0, 1, 2
 
 
+def _encode(pixels: np.ndarray, width: int, height: int, fmt: int, tone_map: Optional["ToneMap"] = None,
+            context: Optional[Context] = None) -> np.ndarray:
+    """gt_encode_image on a host image: float32 or float64 [h][w][3] in, the
+    format's body out (uint8 [h][w][3|4] or float32 [h][w][3])."""
+    ctx = context or default_context()
+    src = np.ascontiguousarray(pixels)
+    if src.dtype not in (np.float32, np.float64):
+        src = src.astype(np.float64)
+    if src.size != width * height * 3:
+        raise abi.InvalidArgument(abi.GT_EINVAL, "HDR image pixel count must match dimensions")
+    chans = {abi.GT_ENCODE_RGBE: 4}.get(fmt, 3)
+    out = np.zeros((height, width, chans), np.float32 if fmt == abi.GT_ENCODE_PFM else np.uint8)
+    tm = tone_map._raw() if tone_map is not None else None
+    abi.check(ctx.lib.gt_encode_image(ctx.handle, src.ctypes.data_as(C.c_void_p), int(src.dtype == np.float64), 0,
+                                      width, height, fmt, C.byref(tm) if tm is not None else None,
+                                      out.ctypes.data_as(C.c_void_p), 0))
+    return out
+
+
+def encoded_header(fmt: int, width: int, height: int) -> bytes:
+    """The file header the reference writes before an encoded body."""
+    lib = abi.load()
+    n = C.c_size_t()
+    buf = C.create_string_buffer(128)
+    abi.check(lib.gt_encoded_header(fmt, width, height, buf, 128, C.byref(n)))
+    return buf.raw[:n.value]
+
+
 @dataclass(frozen=True)
 class ToneMap:
     """ToneMap (display.rs:191-278)."""
@@ -619,73 +647,104 @@ class ToneMap:
             raise abi.InvalidArgument(abi.GT_EINVAL, "tone-map gamma must be positive and finite")
         return replace(self, gamma=float(g))
 
-    def map(self, rgb: np.ndarray) -> np.ndarray:
-        """ToneMap::map_color per channel (display.rs:264-277) -> uint8."""
-        v = np.asarray(rgb, np.float64) * self.exposure
-        with np.errstate(invalid="ignore", over="ignore"):
-            v = np.where(np.isfinite(v), np.maximum(v, 0.0), 0.0)
-            if self.operator == ToneMappingOperator.REINHARD:
-                v = v / (1.0 + v)
-            elif self.operator == ToneMappingOperator.ACES:
-                v = np.clip(v * (2.51 * v + 0.03) / (v * (2.43 * v + 0.59) + 0.14), 0.0, 1.0)
-            if abs(self.gamma - 2.0) <= np.finfo(float).eps:
-                g = np.where(v > 0.0, np.sqrt(np.maximum(v, 0.0)), 0.0)
-            else:
-                g = np.where(v <= 0.0, 0.0, np.power(np.maximum(v, 0.0), 1.0 / self.gamma))
-        return (256.0 * np.clip(g, 0.0, 0.999)).astype(np.uint8)
+    def _raw(self) -> abi.GtToneMap:
+        return abi.GtToneMap(self.exposure, self.gamma, self.operator, 0)
+
+    def map(self, rgb: np.ndarray, context: Optional[Context] = None) -> np.ndarray:
+        """ToneMap::map_color per channel (display.rs:264-277) -> uint8, on
+        the device; rgb is any array of 3-channel pixels."""
+        a = np.asarray(rgb)
+        if a.size % 3:
+            raise abi.InvalidArgument(abi.GT_EINVAL, "pixels must have 3 channels")
+        n = a.size // 3
+        return _encode(a.reshape(-1, 3), n, 1, abi.GT_ENCODE_CANVAS, self, context).reshape(a.shape)
 
 
 class HdrImage:
-    """HdrImage (display.rs:282-409): linear RGB, row-major, top row first."""
+    """HdrImage (display.rs:282-441): linear RGB, row-major, top row first.
+    Conversions and file bodies are computed by the library on the device of
+    `context` (the rendering context, else the default one)."""
 
-    def __init__(self, width: int, height: int, pixels: np.ndarray):
+    def __init__(self, width: int, height: int, pixels: np.ndarray, context: Optional[Context] = None):
         pixels = np.asarray(pixels)
         if pixels.size != width * height * 3:
             raise abi.InvalidArgument(abi.GT_EINVAL, "HDR image pixel count must match dimensions")
         self.width, self.height = int(width), int(height)
         self.pixels = pixels.reshape(height, width, 3)
+        self.context = context
+
+    def _enc(self, fmt: int, tone_map: Optional[ToneMap] = None) -> np.ndarray:
+        return _encode(self.pixels, self.width, self.height, fmt, tone_map, self.context)
 
     def to_canvas(self) -> np.ndarray:
         return self.to_canvas_tone_mapped(ToneMap())
 
     def to_canvas_tone_mapped(self, tone_map: ToneMap) -> np.ndarray:
-        return tone_map.map(self.pixels)
+        return self._enc(abi.GT_ENCODE_CANVAS, tone_map)
 
     def pfm_bytes(self) -> bytes:
         """write_pfm (display.rs:365-387): little-endian f32, bottom row first,
         non-finite values written as 0."""
-        head = f"PF\n{self.width} {self.height}\n-1.0\n".encode()
-        p = np.where(np.isfinite(self.pixels), self.pixels, 0.0).astype("<f4")
-        return head + p[::-1].tobytes()
+        return encoded_header(abi.GT_ENCODE_PFM, self.width, self.height) + \
+            self._enc(abi.GT_ENCODE_PFM).astype("<f4").tobytes()
+
+    def radiance_hdr_bytes(self) -> bytes:
+        """write_radiance_hdr (display.rs:397-409): flat RGBE, top row first."""
+        return encoded_header(abi.GT_ENCODE_RGBE, self.width, self.height) + self._enc(abi.GT_ENCODE_RGBE).tobytes()
 
     def save_pfm(self, path: str) -> None:
         with open(path, "wb") as f:
             f.write(self.pfm_bytes())
 
+    def save_radiance_hdr(self, path: str) -> None:
+        with open(path, "wb") as f:
+            f.write(self.radiance_hdr_bytes())
+
+    def save_extension(self, path: str) -> None:
+        """save_extension (display.rs:412-431): .pfm and .hdr/.rgbe keep HDR
+        data; .ppm writes the default tone-mapped canvas (binary P6). The
+        reference's other raster encoders (PNG, JPEG, ...) belong to its
+        Canvas and are not part of this library."""
+        ext = path.rsplit(".", 1)[-1].lower() if "." in path else ""
+        if ext == "pfm":
+            self.save_pfm(path)
+        elif ext in ("hdr", "rgbe"):
+            self.save_radiance_hdr(path)
+        elif ext == "ppm":
+            self.save_tone_mapped(path, ToneMap())
+        else:
+            raise abi.InvalidArgument(abi.GT_EINVAL, f"unsupported image extension: {path}")
+
+    def save_tone_mapped(self, path: str, tone_map: ToneMap) -> None:
+        canvas = self.to_canvas_tone_mapped(tone_map)
+        with open(path, "wb") as f:
+            f.write(f"P6\n{self.width} {self.height}\n255\n".encode() + canvas.tobytes())
+
 
 @dataclass
 class DenoisingAovs:
-    """DenoisingAovs (camera.rs:491-526)."""
+    """DenoisingAovs (camera.rs:491-526) with the canvases of
+    camera.rs:4212-4243, encoded on the device."""
     beauty_linear: np.ndarray
     albedo_linear: np.ndarray
     normal_linear: np.ndarray
+    context: Optional[Context] = None
+
+    def _canvas(self, a: np.ndarray, fmt: int) -> np.ndarray:
+        h, w = a.shape[0], a.shape[1]
+        return _encode(a, w, h, fmt, None, self.context)
 
     @property
     def beauty(self) -> np.ndarray:  # Rgb::from_linear_color (gamma 2)
-        return ToneMap().map(self.beauty_linear)
+        return self._canvas(self.beauty_linear, abi.GT_ENCODE_CANVAS)
 
     @property
     def albedo(self) -> np.ndarray:  # Rgb::from_raw_linear_color
-        return _raw_encode(self.albedo_linear)
+        return self._canvas(self.albedo_linear, abi.GT_ENCODE_RAW)
 
     @property
     def normal(self) -> np.ndarray:
-        return _raw_encode(self.normal_linear)
-
-
-def _raw_encode(x: np.ndarray) -> np.ndarray:  # colors.rs:412-425
-    x = np.where(np.isnan(x), 0.0, x)
-    return (255.999 * np.clip(x, 0.0, 1.0)).astype(np.uint8)
+        return self._canvas(self.normal_linear, abi.GT_ENCODE_RAW)
 
 
 # ------------------------------------------------------------------ render ---
@@ -731,7 +790,7 @@ class PathTracer:
         st = abi.GtRenderStats()
         abi.check(ctx.lib.gt_render_hdr(ctx.handle, sh, C.byref(camera.raw), lh, eh, out.ctypes.data_as(abi.F),
                                         C.byref(st) if stats else None))
-        img = HdrImage(w, h, out)
+        img = HdrImage(w, h, out, ctx)
         if stats:
             return img, {k: getattr(st, k) for k, _ in abi.GtRenderStats._fields_}
         return img
@@ -747,7 +806,7 @@ class PathTracer:
         n = np.zeros_like(b)
         abi.check(ctx.lib.gt_render_aovs(ctx.handle, sh, C.byref(camera.raw), lh, eh, b.ctypes.data_as(abi.F),
                                          a.ctypes.data_as(abi.F), n.ctypes.data_as(abi.F), None))
-        return DenoisingAovs(b, a, n)
+        return DenoisingAovs(b, a, n, ctx)
 
     @staticmethod
     def _nee(camera: RayCamera) -> RayCamera:  # light_connection_camera / environment_importance_camera
@@ -888,7 +947,7 @@ def render_batch(items, stats: bool = False):
     st = abi.GtRenderStats()
     abi.check(ctx.lib.gt_render_batch(ctx.handle, n, scenes, cams, lights, envs, outs,
                                       C.byref(st) if stats else None))
-    result = [HdrImage(im.shape[1], im.shape[0], im) for im in images]
+    result = [HdrImage(im.shape[1], im.shape[0], im, ctx) for im in images]
     if stats:
         return result, {k: getattr(st, k) for k, _ in abi.GtRenderStats._fields_}
     return result

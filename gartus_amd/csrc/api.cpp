@@ -70,6 +70,7 @@ struct gt_context {
   DevBuf<uint2> gstack;
   DevBuf<double4> acc;
   DevBuf<float> staging;           // device results of the host-output entry points
+  DevBuf<unsigned char> scratch[2];  // staging of the other translation units (gt::context_scratch)
   DevBuf<gtk::Frame> frame_table;  // batched renders
   DevBuf<gtk::Seg> seg_table;
   size_t cap_paths = 0;
@@ -490,6 +491,13 @@ namespace gt {
 int set_error(int code, const std::string& msg) { return fail(code, msg); }
 int context_device(const gt_context* ctx) { return ctx->device; }
 hipStream_t context_stream(const gt_context* ctx) { return ctx->stream; }
+int context_scratch(gt_context* ctx, int slot, size_t bytes, void** out) {
+  if (slot < 0 || slot > 1) return fail(GT_EINVAL, "scratch slot must be 0 or 1");
+  GT_HIP(hipSetDevice(ctx->device));
+  GT_HIP(ctx->scratch[slot].ensure(bytes));
+  *out = ctx->scratch[slot].p;
+  return GT_OK;
+}
 }  // namespace gt
 
 namespace {

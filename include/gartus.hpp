@@ -7,7 +7,7 @@
 //   EnvironmentLight src/graphics/raytracing/environment.rs:22-195 (+ FP32 HDR maps)
 //   RayCamera        src/graphics/camera.rs:612-1557 (builder subset used by renders)
 //   PathTracer       src/graphics/raytracing/renderer.rs:55-886
-//   HdrImage, ToneMap src/graphics/display.rs:164-409
+//   HdrImage, ToneMap src/graphics/display.rs:164-468 (encodings on the device)
 //   DenoisingAovs    src/graphics/camera.rs:491-526
 // Names and argument meaning follow the reference. Where the reference
 // panics on invalid input (assert!), these types throw gartus::Panic.
@@ -15,6 +15,7 @@
 
 #include <algorithm>
 #include <array>
+#include <cctype>
 #include <cmath>
 #include <cstdint>
 #include <cstdio>
@@ -47,7 +48,10 @@ class Context {
     check(gt_context_create(device, &c));
     ctx_.reset(c, [](gt_context* p) { gt_context_destroy(p); });
   }
+  // No device: a placeholder that only compares and copies.
+  explicit Context(std::nullptr_t) {}
   gt_context* get() const { return ctx_.get(); }
+  bool valid() const { return ctx_ != nullptr; }
   gt_tuning tuning() const {
     gt_tuning t{};
     check(gt_context_get_tuning(ctx_.get(), &t));
@@ -282,8 +286,8 @@ class RayCamera {
 // ---- images ---------------------------------------------------------------------
 enum class ToneMappingOperator { None, Reinhard, Aces };
 
-// ToneMap::map_color (display.rs:264-277): exposure, operator, gamma, then
-// Rgb quantisation 256 * clamp(x, 0, 0.999).
+// ToneMap (display.rs:191-278): exposure, operator, gamma; map_color runs on
+// the device (gt_encode_image, GT_ENCODE_CANVAS).
 struct ToneMap {
   double exposure = 1.0;
   ToneMappingOperator op = ToneMappingOperator::None;
@@ -297,41 +301,80 @@ struct ToneMap {
     if (!(std::isfinite(g) && g > 0.0)) throw Panic(GT_EINVAL, "tone-map gamma must be positive and finite");
     auto t = *this; t.gamma = g; return t;
   }
-  uint8_t map_channel(double v) const {
-    v *= exposure;
-    v = std::isfinite(v) ? std::max(v, 0.0) : 0.0;
-    if (op == ToneMappingOperator::Reinhard) v = v / (1.0 + v);
-    else if (op == ToneMappingOperator::Aces)
-      v = std::min(std::max(v * (2.51 * v + 0.03) / (v * (2.43 * v + 0.59) + 0.14), 0.0), 1.0);
-    double g = v <= 0.0 ? 0.0 : (std::fabs(gamma - 2.0) <= 2.220446049250313e-16 ? std::sqrt(v) : std::pow(v, 1.0 / gamma));
-    return uint8_t(256.0 * std::min(std::max(g, 0.0), 0.999));
-  }
+  gt_tone_map raw() const { return gt_tone_map{exposure, gamma, uint32_t(op), 0}; }
 };
 
-// HdrImage (display.rs:282-409): linear RGB, row-major, top row first.
+// The header text gt_encoded_header gives for a format.
+inline std::string encoded_header(int format, uint32_t width, uint32_t height) {
+  char buf[128];
+  size_t n = 0;
+  check(gt_encoded_header(format, width, height, buf, sizeof buf, &n));
+  return std::string(buf, std::min(n, sizeof buf));
+}
+
+// HdrImage (display.rs:282-441): linear RGB, row-major, top row first. The
+// conversions and file bodies are computed on the device of `context` (the
+// rendering context for rendered images).
 struct HdrImage {
   uint32_t width = 0, height = 0;
   std::vector<float> pixels;  // width * height * 3
-  std::vector<uint8_t> to_canvas(const ToneMap& tm = ToneMap()) const {
-    std::vector<uint8_t> out(pixels.size());
-    for (size_t i = 0; i < pixels.size(); ++i) out[i] = tm.map_channel(pixels[i]);
-    return out;
+  Context context{nullptr};
+
+  HdrImage() = default;
+  HdrImage(const Context& ctx, uint32_t w, uint32_t h, std::vector<float> px)
+      : width(w), height(h), pixels(std::move(px)), context(ctx) {
+    if (pixels.size() != size_t(w) * h * 3) throw Panic(GT_EINVAL, "HDR image pixel count must match dimensions");
   }
+
+  std::vector<uint8_t> to_canvas(const ToneMap& tm = ToneMap()) const { return encode<uint8_t>(GT_ENCODE_CANVAS, tm); }
+  std::vector<uint8_t> to_canvas_tone_mapped(const ToneMap& tm) const { return to_canvas(tm); }
+  // Rgb::from_raw_linear_color per pixel (colors.rs:412-422): AOV canvases.
+  std::vector<uint8_t> to_raw_canvas() const { return encode<uint8_t>(GT_ENCODE_RAW, ToneMap()); }
+  std::vector<uint8_t> rgbe() const { return encode<uint8_t>(GT_ENCODE_RGBE, ToneMap()); }
   // write_pfm (display.rs:365-387): little-endian f32, bottom row first,
   // non-finite values written as 0.
-  void save_pfm(const std::string& path) const {
+  void save_pfm(const std::string& path) const { write(path, GT_ENCODE_PFM, encode<float>(GT_ENCODE_PFM, ToneMap())); }
+  // write_radiance_hdr (display.rs:397-409): flat RGBE, top row first.
+  void save_radiance_hdr(const std::string& path) const { write(path, GT_ENCODE_RGBE, rgbe()); }
+  // save_tone_mapped (display.rs:437-440) as a binary PPM (the reference's
+  // Canvas::write_binary_ppm, display.rs:1744-1754).
+  void save_tone_mapped(const std::string& path, const ToneMap& tm) const {
+    const std::string head = "P6\n" + std::to_string(width) + " " + std::to_string(height) + "\n255\n";
+    write_bytes(path, head, to_canvas(tm));
+  }
+  // save_extension (display.rs:412-431): .pfm, .hdr/.rgbe keep HDR data;
+  // .ppm is the default canvas. Other raster formats are the reference's
+  // Canvas encoders and are not provided here.
+  void save_extension(const std::string& path) const {
+    const size_t dot = path.rfind('.');
+    std::string ext = dot == std::string::npos ? "" : path.substr(dot + 1);
+    for (char& c : ext) c = char(std::tolower(static_cast<unsigned char>(c)));
+    if (ext == "pfm") save_pfm(path);
+    else if (ext == "hdr" || ext == "rgbe") save_radiance_hdr(path);
+    else if (ext == "ppm") save_tone_mapped(path, ToneMap());
+    else throw Panic(GT_EINVAL, "unsupported image extension: " + path);
+  }
+
+ private:
+  template <class T>
+  std::vector<T> encode(int format, const ToneMap& tm) const {
+    if (!context.valid()) throw Panic(GT_EINVAL, "HdrImage has no device context");
+    std::vector<T> out(gt_encoded_size(format, width, height) / sizeof(T));
+    const gt_tone_map t = tm.raw();
+    check(gt_encode_image(context.get(), pixels.data(), 0, 0, width, height, format, &t, out.data(), 0));
+    return out;
+  }
+  template <class T>
+  void write(const std::string& path, int format, const std::vector<T>& body) const {
+    write_bytes(path, encoded_header(format, width, height), body);
+  }
+  template <class T>
+  static void write_bytes(const std::string& path, const std::string& head, const std::vector<T>& body) {
     std::FILE* f = std::fopen(path.c_str(), "wb");
     if (!f) throw std::runtime_error("cannot open " + path);
-    std::fprintf(f, "PF\n%u %u\n-1.0\n", width, height);
-    std::vector<float> row(size_t(width) * 3);
-    for (uint32_t y = height; y-- > 0;) {
-      for (size_t i = 0; i < row.size(); ++i) {
-        float v = pixels[size_t(y) * width * 3 + i];
-        row[i] = std::isfinite(v) ? v : 0.0f;
-      }
-      std::fwrite(row.data(), sizeof(float), row.size(), f);
-    }
-    std::fclose(f);
+    std::fwrite(head.data(), 1, head.size(), f);
+    std::fwrite(body.data(), sizeof(T), body.size(), f);
+    if (std::fclose(f) != 0) throw std::runtime_error("cannot write " + path);
   }
 };
 
@@ -377,22 +420,19 @@ class PathTracer {
 
  private:
   static RayCamera nee(const RayCamera& c) { return c.with_direct_lighting_mode(DirectLightingMode::NextEventEstimation); }
-  static HdrImage blank(const RayCamera& c) {
-    HdrImage img;
-    img.width = c.image_width();
-    img.height = c.image_height();
-    img.pixels.assign(size_t(img.width) * img.height * 3, 0.0f);
-    return img;
+  static HdrImage blank(const RayScene& world, const RayCamera& c) {
+    return HdrImage(world.context(), c.image_width(), c.image_height(),
+                    std::vector<float>(size_t(c.image_width()) * c.image_height() * 3, 0.0f));
   }
   static HdrImage render(const RayScene& world, const RayCamera& cam, const SamplingTargetList* lights,
                          const EnvironmentLight* env) {
-    HdrImage img = blank(cam);
+    HdrImage img = blank(world, cam);
     check(gt_render_hdr(world.context().get(), world.s(), &cam.raw(), lights ? lights->l() : nullptr,
                         env ? env->e() : nullptr, img.pixels.data(), nullptr));
     return img;
   }
   static DenoisingAovs aovs(const RayScene& world, const RayCamera& cam, const SamplingTargetList* lights) {
-    DenoisingAovs a{blank(cam), blank(cam), blank(cam)};
+    DenoisingAovs a{blank(world, cam), blank(world, cam), blank(world, cam)};
     check(gt_render_aovs(world.context().get(), world.s(), &cam.raw(), lights ? lights->l() : nullptr, nullptr,
                          a.beauty.pixels.data(), a.albedo.pixels.data(), a.normal.pixels.data(), nullptr));
     return a;

@@ -288,6 +288,53 @@ int gt_autotune(gt_context* ctx, size_t n, const gt_scene* const* scenes,
                 const gt_env* const* envs, uint32_t sample_spp,
                 gt_tuning* out_best, double* out_best_ms);
 
+/* ---- output boundary: HdrImage / ToneMap encodings (display.rs:164-468) --
+ * Per-pixel conversions of a linear RGB image (width*height*3 values,
+ * row-major, top row first) into the formats the reference writes, computed
+ * on the device. File headers are text and come from gt_encoded_header. */
+enum {
+  GT_TONE_NONE = 0,     /* ToneMappingOperator::None: clip */
+  GT_TONE_REINHARD = 1, /* x / (1 + x) */
+  GT_TONE_ACES = 2      /* ACES filmic fit, clamped to [0, 1] */
+};
+/* ToneMap (display.rs:191-208; builders :224-258 panic where these fail
+ * with GT_EINVAL): exposure finite and >= 0, gamma finite and > 0. */
+typedef struct gt_tone_map {
+  double exposure; /* default 1 */
+  double gamma;    /* default 2 (sqrt, Rgb::linear_to_gamma_component) */
+  uint32_t op;     /* GT_TONE_*, default GT_TONE_NONE */
+  uint32_t reserved;
+} gt_tone_map;
+enum {
+  /* HdrImage::to_canvas_tone_mapped (display.rs:330-344): ToneMap::map_color
+   * (:264-277) per channel; w*h*3 bytes, top row first. */
+  GT_ENCODE_CANVAS = 0,
+  /* Rgb::from_raw_linear_color (colors.rs:412-422), the albedo and normal
+   * canvases of DenoisingAovs (camera.rs:4212-4243); w*h*3 bytes. */
+  GT_ENCODE_RAW = 1,
+  /* encode_rgbe (display.rs:444-468) per pixel: write_radiance_hdr's body
+   * (:397-409); w*h*4 bytes, top row first. */
+  GT_ENCODE_RGBE = 2,
+  /* write_pfm's body (display.rs:365-387): w*h*3 little-endian floats,
+   * bottom row first, non-finite values as 0. */
+  GT_ENCODE_PFM = 3
+};
+/* Size in bytes of the encoded body; 0 for an unknown format. */
+uint64_t gt_encoded_size(int format, uint32_t width, uint32_t height);
+/* The file header the reference writes before the body: "PF\n<w> <h>\n-1.0\n"
+ * (display.rs:366-369) for GT_ENCODE_PFM, "#?RADIANCE\nFORMAT=32-bit_rle_rgbe
+ * \n\n-Y <h> +X <w>\n" (:398-401) for GT_ENCODE_RGBE, "" for the canvases.
+ * Writes at most cap bytes (no terminator) and the full length to *out_len. */
+int gt_encoded_header(int format, uint32_t width, uint32_t height, char* buf, size_t cap,
+                      size_t* out_len);
+/* Encodes src (float, or double when src_is_double; host memory, or device
+ * memory when src_on_device) into dst (gt_encoded_size bytes; host, or
+ * device when dst_on_device). tone_map is read by GT_ENCODE_CANVAS only
+ * (NULL = defaults). Returns when dst holds the result. */
+int gt_encode_image(gt_context* ctx, const void* src, int src_is_double, int src_on_device,
+                    uint32_t width, uint32_t height, int format, const gt_tone_map* tone_map,
+                    void* dst, int dst_on_device);
+
 #ifdef __cplusplus
 }
 #endif

@@ -52,6 +52,22 @@ constexpr int BVH_BUCKETS = 12;                     // bvh.rs:4
 static inline double clampd(double x, double lo, double hi) {
   return x < lo ? lo : (x > hi ? hi : x);
 }
+// Rust's `f64 as u8`: truncation toward zero, saturating at 0 and 255, NaN -> 0.
+static inline uint8_t as_u8(double x) { return std::isnan(x) || x <= 0.0 ? 0 : (x >= 255.0 ? 255 : uint8_t(x)); }
+// f64::powi as Rust lowers it (the llvm.powi intrinsic, compiler-rt's
+// __powidf2): binary powering of |b|, then a reciprocal for b < 0, so
+// 2^-1024 evaluates as 1 / inf = 0.
+static inline double powi_f64(double a, int b) {
+  const bool recip = b < 0;
+  double r = 1.0;
+  for (;;) {
+    if (b & 1) r *= a;
+    b /= 2;
+    if (b == 0) break;
+    a *= a;
+  }
+  return recip ? 1.0 / r : r;
+}
 // Rust f64::min / max: a NaN operand yields the other operand.
 static inline double fminr(double a, double b) { return std::fmin(a, b); }
 static inline double fmaxr(double a, double b) { return std::fmax(a, b); }
@@ -1668,8 +1684,43 @@ void cr_tone_map(const double* rgb, size_t n, double exposure, int op, double ga
     double g;
     if (std::fabs(gamma - 2.0) <= EPS) g = std::isnan(v) ? 0.0 : (v > 0.0 ? std::sqrt(v) : 0.0);
     else g = v <= 0.0 ? 0.0 : std::pow(v, 1.0 / gamma);
-    out[i] = uint8_t(256.0 * clampd(g, 0.0, 0.999));
+    out[i] = as_u8(256.0 * clampd(g, 0.0, 0.999));
   }
+}
+
+// Rgb::from_raw_linear_color (colors.rs:412-422): the AOV canvases
+// (camera.rs:4212-4243). NaN -> 0, clamp to [0, 1], times 255.999.
+void cr_encode_raw(const double* rgb, size_t n, uint8_t* out) {
+  for (size_t i = 0; i < 3 * n; ++i) out[i] = as_u8(255.999 * clampd(std::isnan(rgb[i]) ? 0.0 : rgb[i], 0.0, 1.0));
+}
+
+// encode_rgbe (display.rs:444-468), 4 bytes per pixel in storage order
+// (write_radiance_hdr, display.rs:397-409).
+void cr_encode_rgbe(const double* rgb, size_t n, uint8_t* out) {
+  for (size_t p = 0; p < n; ++p) {
+    double c[3];
+    for (int k = 0; k < 3; ++k) c[k] = std::isfinite(rgb[3 * p + k]) ? fmaxr(rgb[3 * p + k], 0.0) : 0.0;
+    const double m = fmaxr(fmaxr(c[0], c[1]), c[2]);
+    uint8_t* o = out + 4 * p;
+    if (m <= 1.0e-32) {
+      o[0] = o[1] = o[2] = o[3] = 0;
+      continue;
+    }
+    const int e = int(std::floor(std::log2(m))) + 1;
+    const double scale = 256.0 * powi_f64(2.0, -e);
+    for (int k = 0; k < 3; ++k) o[k] = as_u8(clampd(c[k] * scale, 0.0, 255.0));
+    o[3] = uint8_t(std::min(std::max(e + 128, 0), 255));
+  }
+}
+
+// write_pfm's body (display.rs:371-386): rows bottom first, f32 little
+// endian, a non-finite value written as 0 (`component as f32` otherwise).
+void cr_encode_pfm(const double* rgb, uint32_t w, uint32_t h, float* out) {
+  for (uint32_t y = 0; y < h; ++y)
+    for (size_t i = 0; i < size_t(w) * 3; ++i) {
+      const double v = rgb[size_t(h - 1 - y) * w * 3 + i];
+      out[size_t(y) * w * 3 + i] = std::isfinite(v) ? float(v) : 0.0f;
+    }
 }
 
 }  // extern "C"

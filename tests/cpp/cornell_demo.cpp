@@ -1,7 +1,7 @@
 // cornell_demo.cpp — the C++ host API (gartus.hpp) used the way the
 // reference's own examples use RayScene / RayCamera / PathTracer.
 //
-//   cornell_demo host            camera and tone-map checks that need no GPU
+//   cornell_demo host            camera, tone-map and header checks that need no GPU
 //   cornell_demo render OUT.pfm  Cornell box (scenes.rs:226-294) with light
 //                                connections, 64x64, 2x2 stratified, depth 8
 #include <cmath>
@@ -61,26 +61,27 @@ int host_checks() {
     threw = p.code == GT_EINVAL;
   }
   expect(threw, "zero image width panics (camera.rs:1035)");
-  gartus::ToneMap tm;
-  expect(tm.map_channel(0.25) == 128, "gamma 2: 0.25 -> 256 * 0.5");
-  expect(tm.map_channel(4.0) == 255 && tm.map_channel(-1.0) == 0 && tm.map_channel(NAN) == 0, "clamping");
-  expect(tm.with_operator(gartus::ToneMappingOperator::Reinhard).map_channel(1.0) == uint8_t(256.0 * std::sqrt(0.5)),
-         "Reinhard 1 -> 0.5 before gamma");
-  gartus::HdrImage img;
-  img.width = 2;
-  img.height = 1;
-  img.pixels = {1, 2, 3, 4, 5, NAN};
-  std::string tmp = "/tmp/gartus_demo_host.pfm";
-  img.save_pfm(tmp);
-  std::FILE* f = std::fopen(tmp.c_str(), "rb");
-  char head[32] = {0};
-  size_t n = std::fread(head, 1, 12, f);  // "PF\n2 1\n-1.0\n"
-  float px[6];
-  size_t m = std::fread(px, sizeof(float), 6, f);
-  std::fclose(f);
-  std::remove(tmp.c_str());
-  expect(n == 12 && std::strncmp(head, "PF\n2 1\n-1.0\n", 12) == 0, "PFM header");
-  expect(m == 6 && px[0] == 1.0f && px[5] == 0.0f, "PFM body, non-finite written as 0");
+  threw = false;
+  try {
+    gartus::ToneMap().with_gamma(0.0);
+  } catch (const gartus::Panic& p) {
+    threw = p.code == GT_EINVAL;
+  }
+  expect(threw, "gamma 0 panics (display.rs:251-257)");
+  expect(gartus::encoded_header(GT_ENCODE_PFM, 2, 1) == "PF\n2 1\n-1.0\n", "PFM header (display.rs:366-369)");
+  expect(gartus::encoded_header(GT_ENCODE_RGBE, 3, 2) == "#?RADIANCE\nFORMAT=32-bit_rle_rgbe\n\n-Y 2 +X 3\n",
+         "RGBE header (display.rs:398-401)");
+  expect(gt_encoded_size(GT_ENCODE_RGBE, 3, 2) == 24, "RGBE body size");
+  threw = false;
+  try {
+    gartus::HdrImage img;
+    img.width = img.height = 1;
+    img.pixels = {1, 1, 1};
+    img.to_canvas();
+  } catch (const gartus::Panic& p) {
+    threw = p.code == GT_EINVAL;
+  }
+  expect(threw, "an image without a device context cannot encode");
   std::printf("host checks: %d failure(s)\n", failures);
   return failures == 0 ? 0 : 1;
 }
@@ -113,6 +114,26 @@ int render(const char* out) {
                  .with_background({0.0, 0.0, 0.0});
   gartus::HdrImage img = gartus::PathTracer(cam).render_with_light_connections_hdr_image(scene, lights);
   img.save_pfm(out);
+  // the device encoders on a small image of known values
+  gartus::HdrImage k(ctx, 2, 1, {4.0f, 1.0f, 0.25f, 1.0f, 0.5f, NAN});
+  std::vector<uint8_t> c = k.to_canvas();
+  expect(c[0] == 255 && c[1] == 255 && c[2] == 128 && c[5] == 0, "gamma-2 canvas (display.rs:2158-2163)");
+  c = k.to_canvas(gartus::ToneMap().with_operator(gartus::ToneMappingOperator::Reinhard).with_gamma(1.0));
+  expect(c[0] == 204 && c[1] == 128 && c[2] == 51, "Reinhard canvas (display.rs:2164-2169)");
+  std::vector<uint8_t> e = k.rgbe();
+  expect(e[4] == 128 && e[5] == 64 && e[6] == 0 && e[7] == 129, "RGBE of (1, 0.5, NaN)");
+  const std::string tmp = std::string(out) + ".check.pfm";
+  k.save_pfm(tmp);
+  std::FILE* f = std::fopen(tmp.c_str(), "rb");
+  char head[32] = {0};
+  size_t n = std::fread(head, 1, 12, f);
+  float px[6];
+  size_t m = std::fread(px, sizeof(float), 6, f);
+  std::fclose(f);
+  std::remove(tmp.c_str());
+  expect(n == 12 && std::strncmp(head, "PF\n2 1\n-1.0\n", 12) == 0, "PFM header");
+  expect(m == 6 && px[0] == 4.0f && px[5] == 0.0f, "PFM body, non-finite written as 0");
+  if (failures) return 1;
   std::printf("rendered %ux%u, %zu primitives, %zu lights -> %s\n", img.width, img.height, scene.len(), lights.len(),
               out);
   return 0;

@@ -4,9 +4,10 @@ A small program, tests/cpp/cornell_demo.cpp, builds the Cornell box
 (scenes.rs:226-294) with RayScene / SamplingTargetList / RayCamera /
 PathTracer exactly as a reference user would. The CPU test compiles it and
 runs its host-only checks: camera defaults and heights, panics on invalid
-input, tone mapping and the PFM writer. The GPU test renders through it and
-requires the image to be bit-identical to the same render made through the
-Python API, so both host layers hand the library identical arguments.
+input, the file headers. The GPU test renders through it (which also checks
+the device encoders on known values) and requires the image to be
+bit-identical to the same render made through the Python API, so both host
+layers hand the library identical arguments.
 """
 import shutil
 import subprocess

@@ -556,16 +556,63 @@ def test_tone_map_known_answers():
     assert tuple(out) == (204, 128, 51)
 
 
-def test_product_tone_map_matches_oracle():
-    """gartus_amd.api.ToneMap (the output boundary) against the oracle."""
-    from gartus_amd.api import ToneMap, ToneMappingOperator
-    rng = np.random.default_rng(1)
-    x = np.ascontiguousarray(np.concatenate([rng.uniform(0, 3, 3000), [np.nan, np.inf, -1.0]]))
-    for op, g, e in [(0, 2.0, 1.0), (1, 1.0, 1.0), (2, 2.2, 0.7), (1, 2.0, 2.0)]:
-        ref = np.zeros(len(x), np.uint8)
-        L.cr_tone_map(dp(x), len(x) // 3, e, op, g, ref.ctypes.data_as(C.POINTER(C.c_uint8)))
-        got = ToneMap(e, op, g).map(x)
-        np.testing.assert_array_equal(got, ref)
+def test_rgbe_known_answer_and_exponent_cases():
+    """display.rs:2188-2201: (1, 0.5, 0.25) encodes as [128, 64, 32, 129].
+    Then what the arithmetic fixes: 0 and values at or below 1e-32 give four
+    zero bytes; a power of two 2^k gives mantissa 128 and exponent k + 129;
+    non-finite and negative channels count as 0; 2^-e is evaluated like
+    f64::powi (the reciprocal of 2^e), so from e = 1024 on it is 0."""
+    def enc(*px):
+        a = np.ascontiguousarray(np.array(px, np.float64).reshape(-1))
+        out = np.zeros(len(a) // 3 * 4, np.uint8)
+        L.cr_encode_rgbe(dp(a), len(a) // 3, out.ctypes.data_as(C.POINTER(C.c_uint8)))
+        return [tuple(int(v) for v in out[i:i + 4]) for i in range(0, len(out), 4)]
+    assert enc(1.0, 0.5, 0.25) == [(128, 64, 32, 129)]
+    assert enc(0.0, 0.0, 0.0, 1e-32, 0.0, 0.0, -5.0, np.nan, -np.inf) == [(0, 0, 0, 0)] * 3
+    for k in (-100, -3, 0, 1, 7, 60, 126, 127):
+        assert enc(2.0 ** k, 0.0, 0.0) == [(128, 0, 0, min(k + 129, 255))]
+    assert enc(np.inf, 3.0, 0.0) == [(0, 192, 0, 130)]  # inf -> 0; 3 = 0.75 * 2^2
+    assert enc(2.0 ** 1023, 0.0, 0.0)[0][:3] == (0, 0, 0)  # 2^-1024 evaluates to 0
+    assert enc(2.0 ** 1022, 0.0, 0.0) == [(128, 0, 0, 255)]  # exponent byte clamps at 255
+
+
+def test_raw_and_pfm_encodings():
+    """Rgb::from_raw_linear_color (colors.rs:412-422): 255.999 * clamp(x, 0, 1)
+    truncated, NaN -> 0. write_pfm (display.rs:365-387): rows bottom first,
+    non-finite -> 0; the reference's PFM test (display.rs:2172-2186) fixes the
+    12 bytes of a 1x1 image."""
+    x = np.ascontiguousarray([0.0, 0.5, 1.0, 2.0, -1.0, np.nan, np.inf, 0.999, 1.0 / 255.999], np.float64)
+    out = np.zeros(9, np.uint8)
+    L.cr_encode_raw(dp(x), 3, out.ctypes.data_as(C.POINTER(C.c_uint8)))
+    assert list(out) == [0, 127, 255, 255, 0, 0, 255, 255, 1]
+    img = np.ascontiguousarray(np.arange(2 * 3 * 3, dtype=np.float64).reshape(3, 2, 3))
+    img[0, 0, 0] = np.nan
+    body = np.zeros(18, np.float32)
+    L.cr_encode_pfm(dp(img), 2, 3, body.ctypes.data_as(C.POINTER(C.c_float)))
+    expect = np.where(np.isfinite(img), img, 0.0)[::-1].reshape(-1).astype(np.float32)
+    np.testing.assert_array_equal(body, expect)
+    one = np.ascontiguousarray([2.0, 0.5, 0.25])
+    b1 = np.zeros(3, np.float32)
+    L.cr_encode_pfm(dp(one), 1, 1, b1.ctypes.data_as(C.POINTER(C.c_float)))
+    assert b1.astype("<f4").tobytes() == np.array([2.0, 0.5, 0.25], "<f4").tobytes()
+
+
+def test_encoded_headers_and_sizes():
+    """The library's header text (pure host code, no device needed) is the
+    reference's: display.rs:366-369 and :398-401, as its tests read them
+    (display.rs:2183,2199)."""
+    from gartus_amd import api
+    assert api.encoded_header(abi.GT_ENCODE_PFM, 1, 1) == b"PF\n1 1\n-1.0\n"
+    assert api.encoded_header(abi.GT_ENCODE_RGBE, 1, 1) == b"#?RADIANCE\nFORMAT=32-bit_rle_rgbe\n\n-Y 1 +X 1\n"
+    assert api.encoded_header(abi.GT_ENCODE_RGBE, 1920, 1080).endswith(b"-Y 1080 +X 1920\n")
+    assert api.encoded_header(abi.GT_ENCODE_CANVAS, 4, 4) == b""
+    lib = abi.load()
+    assert lib.gt_encoded_size(abi.GT_ENCODE_PFM, 7, 5) == 7 * 5 * 12
+    assert lib.gt_encoded_size(abi.GT_ENCODE_RGBE, 7, 5) == 7 * 5 * 4
+    assert lib.gt_encoded_size(abi.GT_ENCODE_RAW, 7, 5) == 7 * 5 * 3
+    assert lib.gt_encoded_size(9, 7, 5) == 0
+    with pytest.raises(abi.InvalidArgument):
+        api.encoded_header(9, 1, 1)
 
 
 def _alias_probabilities(thr, alias):
