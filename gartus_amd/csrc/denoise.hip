@@ -1,15 +1,14 @@
 // denoise.hip — kernels of the learned denoiser for gfx950.
 //
 // The network is a convolutional encoder-decoder over NHWC bf16 tensors.
-// Every 3x3 convolution is an implicit GEMM on the matrix cores
-// (v_mfma_f32_16x16x16_bf16: A 16x16 bf16, B 16x16 bf16, D 16x16 fp32):
-//   forward      M = pixels, N = Cout, K = 9 * Cin
-//   data grad    M = pixels, N = Cin,  K = 9 * Cout (taps mirrored)
-//   weight grad  M = Cout,   N = Cin,  K = pixels (per tap)
-// Operand lanes (checked on the device against a host matrix product):
-// lane l holds A[l % 16][4 (l / 16) + j] and B[4 (l / 16) + j][l % 16] for
-// j < 4, and D[4 (l / 16) + r][l % 16] for r < 4. Channel counts are
-// multiples of 16, so every A/B fragment is one aligned 8-byte load.
+// Every 3x3 convolution is an implicit GEMM on the matrix cores:
+//   forward      M = pixels, N = Cout, K = 9 * Cin   (v_mfma_f32_16x16x32_bf16)
+//   data grad    M = pixels, N = Cin,  K = 9 * Cout  (taps mirrored; same kernel)
+//   weight grad  M = Cout,   N = Cin,  K = pixels    (v_mfma_f32_16x16x16_bf16, per tap)
+// Operand lanes, with k-group size q = 4 (16x16x16) or 8 (16x16x32): lane l
+// holds A[l % 16][q (l / 16) + j] and B[q (l / 16) + j][l % 16] for j < q,
+// and D[4 (l / 16) + r][l % 16] for r < 4. Channel counts are multiples of
+// 16, so every fragment is one aligned 8- or 16-byte load.
 // Accumulation is fp32; weight gradients are summed per split and reduced
 // in a fixed order, so training steps are deterministic.
 #include "denoise.h"
@@ -35,72 +34,135 @@ __device__ __forceinline__ v4f mfma(v4s a, v4s b, v4f c) {
   return __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, c, 0, 0, 0);
 }
 
-__device__ __forceinline__ v4s load4(const uint16_t* p) { return *reinterpret_cast<const v4s*>(p); }
+// ---- convolution, forward and data gradient --------------------------------
+// A block owns a tile of kTR image rows x kTC columns of one image and NT*16
+// output channels; wave w computes row w of the tile: 4 pixel fragments of
+// 16 columns x NT channel fragments. The input channels run in chunks of
+// kCK: per chunk the block stages the tile's input halo ((kTR+2) x (kTC+2)
+// pixels, read once for all 9 taps) and the chunk's weights in LDS, then
+// every wave runs K = 9 * chunk through v_mfma_f32_16x16x32_bf16. One K=32
+// step covers two 16-channel sub-slices (tap, channel group) in tap-major
+// order: lanes 0-31 take the first, lanes 32-63 the second (zero past the
+// end). DGRAD reads the input at p - off_t (the transposed convolution) and
+// masks it by the ReLU of the forward output while staging.
+constexpr int kTR = 4;               // tile rows, one per wave
+constexpr int kTC = 64;              // tile columns, 4 fragments of 16
+constexpr int kHR = kTR + 2, kHC = kTC + 2;
+constexpr int kCK = 32;              // input channels per staged chunk
+constexpr int kLS = kCK + 8;         // LDS row stride in bf16: 80 B keeps 16-B reads conflict free
 
-// Zeroes the bf16 lanes of a whose mask element is not positive.
-__device__ __forceinline__ v4s mask4(v4s a, v4s m) {
-  v4s r;
-#pragma unroll
-  for (int j = 0; j < 4; ++j) r[j] = bf2f(uint16_t(m[j])) > 0.f ? a[j] : short(0);
-  return r;
+template <int NT>
+constexpr size_t conv_lds_bytes() {
+  return size_t(kHR * kHC + 9 * NT * 16) * kLS * sizeof(uint16_t);
 }
 
-// ---- convolution, forward and data gradient --------------------------------
-// One wave computes 16 consecutive pixels x NT*16 output channels. DGRAD
-// reads the input at p - off_t (the transposed convolution) and masks it by
-// the ReLU of the forward output.
+typedef __bf16 v8bf __attribute__((ext_vector_type(8)));
+
+__device__ __forceinline__ v4f mfma32(uint4 a, uint4 b, v4f c) {
+  return __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(v8bf, a), __builtin_bit_cast(v8bf, b), c, 0, 0,
+                                                 0);
+}
+
+// Zeroes the bf16 elements of a whose mask element is not positive.
+__device__ __forceinline__ uint4 mask8(uint4 a, uint4 m) {
+  uint32_t av[4] = {a.x, a.y, a.z, a.w}, mv[4] = {m.x, m.y, m.z, m.w};
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    const uint32_t lo = bf2f(uint16_t(mv[k])) > 0.f ? 0x0000ffffu : 0u;
+    const uint32_t hi = bf2f(uint16_t(mv[k] >> 16)) > 0.f ? 0xffff0000u : 0u;
+    av[k] &= lo | hi;
+  }
+  return make_uint4(av[0], av[1], av[2], av[3]);
+}
+
+// The MFMA steps of one staged chunk of G 16-channel groups.
+template <int NT, bool DGRAD, int G>
+__device__ __forceinline__ void conv_chunk(const uint16_t* halo, const uint16_t* wts, int lane, int wave,
+                                           v4f (&acc)[4][NT]) {
+  const int half = (lane >> 4) & 1;
+  const int col = lane & 15;
+#pragma unroll
+  for (int i = 0; i < (9 * G + 1) / 2; ++i) {
+    const int sub = 2 * i + (lane >> 5);
+    const bool valid = sub < 9 * G;
+    const int tap = valid ? sub / G : 0;
+    const int ch = (sub % G) * 16 + 8 * half;
+    int dy = tap / 3 - 1, dx = tap % 3 - 1;
+    if (DGRAD) dy = -dy, dx = -dx;
+    uint4 b[NT];
+#pragma unroll
+    for (int j = 0; j < NT; ++j)
+      b[j] = valid ? *reinterpret_cast<const uint4*>(wts + (tap * NT * 16 + j * 16 + col) * kLS + ch)
+                   : make_uint4(0, 0, 0, 0);
+    const uint16_t* arow = halo + ((wave + 1 + dy) * kHC + col + 1 + dx) * kLS + ch;
+#pragma unroll
+    for (int f = 0; f < 4; ++f) {
+      const uint4 a = valid ? *reinterpret_cast<const uint4*>(arow + f * 16 * kLS) : make_uint4(0, 0, 0, 0);
+#pragma unroll
+      for (int j = 0; j < NT; ++j) acc[f][j] = mfma32(a, b[j], acc[f][j]);
+    }
+  }
+}
+
 template <int NT, bool DGRAD, bool OUT_F32>
 __global__ __launch_bounds__(256) void k_conv(const uint16_t* __restrict__ x, const uint16_t* __restrict__ mask,
                                               int cin, const uint16_t* __restrict__ wgt,
-                                              const float* __restrict__ bias, void* __restrict__ y, int cout, int H,
-                                              int W, int P, int relu) {
-  const int lane = threadIdx.x & 63;
-  const int p0 = (blockIdx.x * 4 + (threadIdx.x >> 6)) * 16;
-  if (p0 >= P) return;  // whole wave
-  const int pa = p0 + (lane & 15);
-  const bool va = pa < P;
-  int n = 0, h = 0, w = 0;
-  if (va) {
-    w = pa % W;
-    const int t = pa / W;
-    h = t % H;
-    n = t / H;
-  }
-  const int kq = (lane >> 4) * 4;
-  v4f acc[NT];
+                                              const float* __restrict__ bias, void* __restrict__ y, int cout,
+                                              int co_first, int H, int W, int nrt, int nct, int relu) {
+  extern __shared__ uint4 lds4[];
+  uint16_t* halo = reinterpret_cast<uint16_t*>(lds4);
+  uint16_t* wts = halo + kHR * kHC * kLS;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int ct = blockIdx.x % nct, rt = (blockIdx.x / nct) % nrt, n = blockIdx.x / (nct * nrt);
+  const int h0 = rt * kTR, w0 = ct * kTC;
+  const int co0 = co_first + blockIdx.y * NT * 16;
+  v4f acc[4][NT];
 #pragma unroll
-  for (int j = 0; j < NT; ++j) acc[j] = v4f{0.f, 0.f, 0.f, 0.f};
-  const size_t wrow = size_t(9) * cin;  // weights per output column
-  const uint16_t* wl = wgt + size_t(lane & 15) * wrow + kq;
-  for (int tap = 0; tap < 9; ++tap) {
-    const int dy = tap / 3 - 1, dx = tap % 3 - 1;
-    const int hh = DGRAD ? h - dy : h + dy, ww = DGRAD ? w - dx : w + dx;
-    const bool ok = va && hh >= 0 && hh < H && ww >= 0 && ww < W;
-    const size_t src = ok ? ((size_t(n) * H + hh) * W + ww) * cin + kq : 0;
-    const uint16_t* wt = wl + size_t(tap) * cin;
-    for (int c0 = 0; c0 < cin; c0 += 16) {
-      v4s a = v4s{0, 0, 0, 0};
-      if (ok) {
-        a = load4(x + src + c0);
-        if (DGRAD && mask) a = mask4(a, load4(mask + src + c0));
+  for (int f = 0; f < 4; ++f)
+#pragma unroll
+    for (int j = 0; j < NT; ++j) acc[f][j] = v4f{0.f, 0.f, 0.f, 0.f};
+  for (int c0 = 0; c0 < cin; c0 += kCK) {
+    const int ck = min(kCK, cin - c0), q8 = ck / 8;  // 16-byte pieces per pixel or weight row
+    __syncthreads();                                 // the previous chunk's reads are done
+    for (int i = tid; i < kHR * kHC * q8; i += 256) {
+      const int px = i / q8, q = i - px * q8;
+      const int hh = h0 - 1 + px / kHC, ww = w0 - 1 + px % kHC;
+      uint4 v = make_uint4(0, 0, 0, 0);
+      if (hh >= 0 && hh < H && ww >= 0 && ww < W) {
+        const size_t off = ((size_t(n) * H + hh) * W + ww) * cin + c0 + 8 * q;
+        v = *reinterpret_cast<const uint4*>(x + off);
+        if (DGRAD && mask) v = mask8(v, *reinterpret_cast<const uint4*>(mask + off));
       }
-#pragma unroll
-      for (int j = 0; j < NT; ++j) acc[j] = mfma(a, load4(wt + size_t(j) * 16 * wrow + c0), acc[j]);
+      *reinterpret_cast<uint4*>(halo + px * kLS + 8 * q) = v;
     }
+    for (int i = tid; i < 9 * NT * 16 * q8; i += 256) {
+      const int r = i / q8, q = i - r * q8;
+      const int co = r % (NT * 16), tap = r / (NT * 16);
+      *reinterpret_cast<uint4*>(wts + r * kLS + 8 * q) =
+          *reinterpret_cast<const uint4*>(wgt + (size_t(co0 + co) * 9 + tap) * cin + c0 + 8 * q);
+    }
+    __syncthreads();
+    if (ck == kCK) conv_chunk<NT, DGRAD, 2>(halo, wts, lane, wave, acc);
+    else conv_chunk<NT, DGRAD, 1>(halo, wts, lane, wave, acc);
   }
+  const int row = h0 + wave;
+  if (row >= H) return;
 #pragma unroll
   for (int j = 0; j < NT; ++j) {
-    const int co = j * 16 + (lane & 15);
+    const int co = co0 + j * 16 + (lane & 15);
     const float b = (!DGRAD && bias) ? bias[co] : 0.f;
 #pragma unroll
-    for (int r = 0; r < 4; ++r) {
-      const int p = p0 + (lane >> 4) * 4 + r;
-      if (p >= P) continue;
-      float v = acc[j][r] + b;
-      if (relu) v = fmaxf(v, 0.f);
-      if (OUT_F32) static_cast<float*>(y)[size_t(p) * cout + co] = v;
-      else static_cast<uint16_t*>(y)[size_t(p) * cout + co] = f2bf(v);
-    }
+    for (int f = 0; f < 4; ++f)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int colw = w0 + f * 16 + (lane >> 4) * 4 + r;
+        if (colw >= W) continue;
+        float v = acc[f][j][r] + b;
+        if (relu) v = fmaxf(v, 0.f);
+        const size_t o = ((size_t(n) * H + row) * W + colw) * cout + co;
+        if (OUT_F32) static_cast<float*>(y)[o] = v;
+        else static_cast<uint16_t*>(y)[o] = f2bf(v);
+      }
   }
 }
 
@@ -400,31 +462,30 @@ unsigned grid_for(size_t n, unsigned block = 256) {
 
 template <bool DGRAD, bool F32, int NT>
 void launch_nt(hipStream_t s, const uint16_t* x, const uint16_t* mask, int cin, const uint16_t* w, const float* bias,
-               void* y, int cout, Dims d, int relu) {
-  const int P = d.pixels();
-  hipLaunchKernelGGL((k_conv<NT, DGRAD, F32>), dim3((P + 63) / 64), dim3(256), 0, s, x, mask, cin, w, bias, y, cout,
-                     d.h, d.w, P, relu);
+               void* y, int cout, int co_first, int groups, Dims d, int relu) {
+  const int nrt = (d.h + kTR - 1) / kTR, nct = (d.w + kTC - 1) / kTC;
+  constexpr size_t lds = conv_lds_bytes<NT>();
+  static const hipError_t attr =  // once per instantiation: allow more than 64 KiB of LDS
+      hipFuncSetAttribute(reinterpret_cast<const void*>(&k_conv<NT, DGRAD, F32>),
+                          hipFuncAttributeMaxDynamicSharedMemorySize, int(lds));
+  (void)attr;
+  hipLaunchKernelGGL((k_conv<NT, DGRAD, F32>), dim3(d.n * nrt * nct, groups), dim3(256), lds, s, x, mask, cin, w,
+                     bias, y, cout, co_first, d.h, d.w, nrt, nct, relu);
 }
 
-// NT = output channels / 16: up to 8 for the forward (cout <= 128), up to 16
-// for the data gradient (cin <= 256, a concatenation).
+// Output channels in groups of 64 (NT = 4), then one launch for the rest.
 template <bool DGRAD, bool F32>
-void launch_conv(hipStream_t s, int nt, const uint16_t* x, const uint16_t* mask, int cin, const uint16_t* w,
+void launch_conv(hipStream_t s, const uint16_t* x, const uint16_t* mask, int cin, const uint16_t* w,
                  const float* bias, void* y, int cout, Dims d, int relu) {
-#define GDN_NT(N) \
-  case N: launch_nt<DGRAD, F32, N>(s, x, mask, cin, w, bias, y, cout, d, relu); break
-  switch (nt) {
-    GDN_NT(1); GDN_NT(2); GDN_NT(3); GDN_NT(4); GDN_NT(5); GDN_NT(6); GDN_NT(7); GDN_NT(8);
-    default:
-      if constexpr (DGRAD) {
-        switch (nt) {
-          GDN_NT(9); GDN_NT(10); GDN_NT(11); GDN_NT(12); GDN_NT(13); GDN_NT(14); GDN_NT(15); GDN_NT(16);
-          default: break;  // rejected by the network builder
-        }
-      }
-      break;
+  const int frags = cout / 16, full = frags / 4, rest = frags % 4;
+  if (full) launch_nt<DGRAD, F32, 4>(s, x, mask, cin, w, bias, y, cout, 0, full, d, relu);
+  const int c0 = full * 64;
+  switch (rest) {
+    case 1: launch_nt<DGRAD, F32, 1>(s, x, mask, cin, w, bias, y, cout, c0, 1, d, relu); break;
+    case 2: launch_nt<DGRAD, F32, 2>(s, x, mask, cin, w, bias, y, cout, c0, 1, d, relu); break;
+    case 3: launch_nt<DGRAD, F32, 3>(s, x, mask, cin, w, bias, y, cout, c0, 1, d, relu); break;
+    default: break;
   }
-#undef GDN_NT
 }
 
 constexpr int kSplitPx = 1024;  // pixels per weight-gradient split
@@ -433,14 +494,14 @@ constexpr int kSplitPx = 1024;  // pixels per weight-gradient split
 
 void conv_fwd(hipStream_t s, const uint16_t* x, int cin, const uint16_t* w, const float* bias, void* y, int cout,
               Dims d, bool relu, bool out_f32) {
-  if (out_f32) launch_conv<false, true>(s, cout / 16, x, nullptr, cin, w, bias, y, cout, d, relu ? 1 : 0);
-  else launch_conv<false, false>(s, cout / 16, x, nullptr, cin, w, bias, y, cout, d, relu ? 1 : 0);
+  if (out_f32) launch_conv<false, true>(s, x, nullptr, cin, w, bias, y, cout, d, relu ? 1 : 0);
+  else launch_conv<false, false>(s, x, nullptr, cin, w, bias, y, cout, d, relu ? 1 : 0);
 }
 
 void conv_dgrad(hipStream_t s, const uint16_t* dy, const uint16_t* y, int cout, const uint16_t* wt, uint16_t* dx,
                 int cin, Dims d) {
   // the transposed problem: input dy (cout channels), output dx (cin channels)
-  launch_conv<true, false>(s, cin / 16, dy, y, cout, wt, nullptr, dx, cin, d, 0);
+  launch_conv<true, false>(s, dy, y, cout, wt, nullptr, dx, cin, d, 0);
 }
 
 size_t wgrad_scratch_floats(int cin, int cout, Dims d) {
