@@ -18,6 +18,10 @@ struct Dims {
 // y = act(conv(x) + bias); y is bf16, or fp32 when out_f32.
 void conv_fwd(hipStream_t s, const uint16_t* x, int cin, const uint16_t* w, const float* bias, void* y, int cout,
               Dims d, bool relu, bool out_f32);
+// y = act(conv(cat(up2(a), b)) + bias) without materialising the concatenation: a has ca channels
+// at half resolution, b has cb channels at d; the same result as upcat_fwd followed by conv_fwd.
+void conv_fwd_upcat(hipStream_t s, const uint16_t* a, int ca, const uint16_t* b, int cb, const uint16_t* w,
+                    const float* bias, uint16_t* y, int cout, Dims d, bool relu);
 // dx = conv^T(dy * [y > 0]) (mask only when y != null).
 void conv_dgrad(hipStream_t s, const uint16_t* dy, const uint16_t* y, int cout, const uint16_t* wt, uint16_t* dx,
                 int cin, Dims d);

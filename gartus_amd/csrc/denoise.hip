@@ -104,11 +104,75 @@ __device__ __forceinline__ void conv_chunk(const uint16_t* halo, const uint16_t*
   }
 }
 
+// Stages one chunk (G 16-channel groups from channel c0) of the tile's input
+// halo and of the weights into LDS. Each batch of loads is issued before
+// its LDS stores, halo first, then weights, so a block waits for memory twice
+// per chunk rather than once per piece. When xa is
+// set the input is the decoder's concatenation [up2(xa) (ca channels), x]:
+// channels below ca come from xa at half resolution (nearest), the rest from x.
+template <int NT, bool DGRAD, int G>
+__device__ __forceinline__ void conv_stage(uint16_t* halo, uint16_t* wts, const uint16_t* __restrict__ x,
+                                           const uint16_t* __restrict__ mask, int cin,
+                                           const uint16_t* __restrict__ xa, int ca,
+                                           const uint16_t* __restrict__ wgt, int co0, int c0, int n, int h0,
+                                           int w0, int H, int W, int tid) {
+  constexpr int QP = 2 * G;  // 16-byte pieces per pixel or weight row
+  constexpr int HP = kHR * kHC * QP, HI = (HP + 255) / 256;
+  constexpr int WP = 9 * NT * 16 * QP, WI = (WP + 255) / 256;
+  const int cx = xa ? cin - ca : cin, cs = xa ? ca : 0;  // channels of x, first channel from x
+  {
+    uint4 hv[HI];
+#pragma unroll
+    for (int k = 0; k < HI; ++k) {
+      const int i = tid + 256 * k;
+      hv[k] = make_uint4(0, 0, 0, 0);
+      if (i < HP) {
+        const int px = i / QP, c = c0 + 8 * (i % QP);
+        const int hh = h0 - 1 + px / kHC, ww = w0 - 1 + px % kHC;
+        if (hh >= 0 && hh < H && ww >= 0 && ww < W) {
+          if (c < cs) {
+            hv[k] = *reinterpret_cast<const uint4*>(
+                xa + ((size_t(n) * (H / 2) + hh / 2) * (W / 2) + ww / 2) * ca + c);
+          } else {
+            const size_t off = ((size_t(n) * H + hh) * W + ww) * cx + (c - cs);
+            hv[k] = *reinterpret_cast<const uint4*>(x + off);
+            if (DGRAD && mask) hv[k] = mask8(hv[k], *reinterpret_cast<const uint4*>(mask + off));
+          }
+        }
+      }
+    }
+#pragma unroll
+    for (int k = 0; k < HI; ++k) {
+      const int i = tid + 256 * k;
+      if (i < HP) *reinterpret_cast<uint4*>(halo + (i / QP) * kLS + 8 * (i % QP)) = hv[k];
+    }
+  }
+  __builtin_amdgcn_sched_barrier(0);  // keep the two batches apart: fewer live registers
+  {
+    uint4 wv[WI];
+#pragma unroll
+    for (int k = 0; k < WI; ++k) {
+      const int i = tid + 256 * k;
+      wv[k] = make_uint4(0, 0, 0, 0);
+      if (i < WP) {
+        const int r = i / QP, co = r % (NT * 16), tap = r / (NT * 16);
+        wv[k] = *reinterpret_cast<const uint4*>(wgt + (size_t(co0 + co) * 9 + tap) * cin + c0 + 8 * (i % QP));
+      }
+    }
+#pragma unroll
+    for (int k = 0; k < WI; ++k) {
+      const int i = tid + 256 * k;
+      if (i < WP) *reinterpret_cast<uint4*>(wts + (i / QP) * kLS + 8 * (i % QP)) = wv[k];
+    }
+  }
+}
+
 template <int NT, bool DGRAD, bool OUT_F32>
 __global__ __launch_bounds__(256) void k_conv(const uint16_t* __restrict__ x, const uint16_t* __restrict__ mask,
-                                              int cin, const uint16_t* __restrict__ wgt,
-                                              const float* __restrict__ bias, void* __restrict__ y, int cout,
-                                              int co_first, int H, int W, int nrt, int nct, int relu) {
+                                              int cin, const uint16_t* __restrict__ xa, int ca,
+                                              const uint16_t* __restrict__ wgt, const float* __restrict__ bias,
+                                              void* __restrict__ y, int cout, int co_first, int H, int W, int nrt,
+                                              int nct, int relu) {
   extern __shared__ uint4 lds4[];
   uint16_t* halo = reinterpret_cast<uint16_t*>(lds4);
   uint16_t* wts = halo + kHR * kHC * kLS;
@@ -122,28 +186,16 @@ __global__ __launch_bounds__(256) void k_conv(const uint16_t* __restrict__ x, co
 #pragma unroll
     for (int j = 0; j < NT; ++j) acc[f][j] = v4f{0.f, 0.f, 0.f, 0.f};
   for (int c0 = 0; c0 < cin; c0 += kCK) {
-    const int ck = min(kCK, cin - c0), q8 = ck / 8;  // 16-byte pieces per pixel or weight row
-    __syncthreads();                                 // the previous chunk's reads are done
-    for (int i = tid; i < kHR * kHC * q8; i += 256) {
-      const int px = i / q8, q = i - px * q8;
-      const int hh = h0 - 1 + px / kHC, ww = w0 - 1 + px % kHC;
-      uint4 v = make_uint4(0, 0, 0, 0);
-      if (hh >= 0 && hh < H && ww >= 0 && ww < W) {
-        const size_t off = ((size_t(n) * H + hh) * W + ww) * cin + c0 + 8 * q;
-        v = *reinterpret_cast<const uint4*>(x + off);
-        if (DGRAD && mask) v = mask8(v, *reinterpret_cast<const uint4*>(mask + off));
-      }
-      *reinterpret_cast<uint4*>(halo + px * kLS + 8 * q) = v;
+    __syncthreads();  // the previous chunk's reads are done
+    if (cin - c0 >= kCK) {
+      conv_stage<NT, DGRAD, 2>(halo, wts, x, mask, cin, xa, ca, wgt, co0, c0, n, h0, w0, H, W, tid);
+      __syncthreads();
+      conv_chunk<NT, DGRAD, 2>(halo, wts, lane, wave, acc);
+    } else {
+      conv_stage<NT, DGRAD, 1>(halo, wts, x, mask, cin, xa, ca, wgt, co0, c0, n, h0, w0, H, W, tid);
+      __syncthreads();
+      conv_chunk<NT, DGRAD, 1>(halo, wts, lane, wave, acc);
     }
-    for (int i = tid; i < 9 * NT * 16 * q8; i += 256) {
-      const int r = i / q8, q = i - r * q8;
-      const int co = r % (NT * 16), tap = r / (NT * 16);
-      *reinterpret_cast<uint4*>(wts + r * kLS + 8 * q) =
-          *reinterpret_cast<const uint4*>(wgt + (size_t(co0 + co) * 9 + tap) * cin + c0 + 8 * q);
-    }
-    __syncthreads();
-    if (ck == kCK) conv_chunk<NT, DGRAD, 2>(halo, wts, lane, wave, acc);
-    else conv_chunk<NT, DGRAD, 1>(halo, wts, lane, wave, acc);
   }
   const int row = h0 + wave;
   if (row >= H) return;
@@ -461,29 +513,29 @@ unsigned grid_for(size_t n, unsigned block = 256) {
 }
 
 template <bool DGRAD, bool F32, int NT>
-void launch_nt(hipStream_t s, const uint16_t* x, const uint16_t* mask, int cin, const uint16_t* w, const float* bias,
-               void* y, int cout, int co_first, int groups, Dims d, int relu) {
+void launch_nt(hipStream_t s, const uint16_t* x, const uint16_t* mask, int cin, const uint16_t* xa, int ca,
+               const uint16_t* w, const float* bias, void* y, int cout, int co_first, int groups, Dims d, int relu) {
   const int nrt = (d.h + kTR - 1) / kTR, nct = (d.w + kTC - 1) / kTC;
   constexpr size_t lds = conv_lds_bytes<NT>();
   static const hipError_t attr =  // once per instantiation: allow more than 64 KiB of LDS
       hipFuncSetAttribute(reinterpret_cast<const void*>(&k_conv<NT, DGRAD, F32>),
                           hipFuncAttributeMaxDynamicSharedMemorySize, int(lds));
   (void)attr;
-  hipLaunchKernelGGL((k_conv<NT, DGRAD, F32>), dim3(d.n * nrt * nct, groups), dim3(256), lds, s, x, mask, cin, w,
-                     bias, y, cout, co_first, d.h, d.w, nrt, nct, relu);
+  hipLaunchKernelGGL((k_conv<NT, DGRAD, F32>), dim3(d.n * nrt * nct, groups), dim3(256), lds, s, x, mask, cin, xa,
+                     ca, w, bias, y, cout, co_first, d.h, d.w, nrt, nct, relu);
 }
 
 // Output channels in groups of 64 (NT = 4), then one launch for the rest.
 template <bool DGRAD, bool F32>
-void launch_conv(hipStream_t s, const uint16_t* x, const uint16_t* mask, int cin, const uint16_t* w,
-                 const float* bias, void* y, int cout, Dims d, int relu) {
+void launch_conv(hipStream_t s, const uint16_t* x, const uint16_t* mask, int cin, const uint16_t* xa, int ca,
+                 const uint16_t* w, const float* bias, void* y, int cout, Dims d, int relu) {
   const int frags = cout / 16, full = frags / 4, rest = frags % 4;
-  if (full) launch_nt<DGRAD, F32, 4>(s, x, mask, cin, w, bias, y, cout, 0, full, d, relu);
+  if (full) launch_nt<DGRAD, F32, 4>(s, x, mask, cin, xa, ca, w, bias, y, cout, 0, full, d, relu);
   const int c0 = full * 64;
   switch (rest) {
-    case 1: launch_nt<DGRAD, F32, 1>(s, x, mask, cin, w, bias, y, cout, c0, 1, d, relu); break;
-    case 2: launch_nt<DGRAD, F32, 2>(s, x, mask, cin, w, bias, y, cout, c0, 1, d, relu); break;
-    case 3: launch_nt<DGRAD, F32, 3>(s, x, mask, cin, w, bias, y, cout, c0, 1, d, relu); break;
+    case 1: launch_nt<DGRAD, F32, 1>(s, x, mask, cin, xa, ca, w, bias, y, cout, c0, 1, d, relu); break;
+    case 2: launch_nt<DGRAD, F32, 2>(s, x, mask, cin, xa, ca, w, bias, y, cout, c0, 1, d, relu); break;
+    case 3: launch_nt<DGRAD, F32, 3>(s, x, mask, cin, xa, ca, w, bias, y, cout, c0, 1, d, relu); break;
     default: break;
   }
 }
@@ -494,14 +546,19 @@ constexpr int kSplitPx = 1024;  // pixels per weight-gradient split
 
 void conv_fwd(hipStream_t s, const uint16_t* x, int cin, const uint16_t* w, const float* bias, void* y, int cout,
               Dims d, bool relu, bool out_f32) {
-  if (out_f32) launch_conv<false, true>(s, x, nullptr, cin, w, bias, y, cout, d, relu ? 1 : 0);
-  else launch_conv<false, false>(s, x, nullptr, cin, w, bias, y, cout, d, relu ? 1 : 0);
+  if (out_f32) launch_conv<false, true>(s, x, nullptr, cin, nullptr, 0, w, bias, y, cout, d, relu ? 1 : 0);
+  else launch_conv<false, false>(s, x, nullptr, cin, nullptr, 0, w, bias, y, cout, d, relu ? 1 : 0);
+}
+
+void conv_fwd_upcat(hipStream_t s, const uint16_t* a, int ca, const uint16_t* b, int cb, const uint16_t* w,
+                    const float* bias, uint16_t* y, int cout, Dims d, bool relu) {
+  launch_conv<false, false>(s, b, nullptr, ca + cb, a, ca, w, bias, y, cout, d, relu ? 1 : 0);
 }
 
 void conv_dgrad(hipStream_t s, const uint16_t* dy, const uint16_t* y, int cout, const uint16_t* wt, uint16_t* dx,
                 int cin, Dims d) {
   // the transposed problem: input dy (cout channels), output dx (cin channels)
-  launch_conv<true, false>(s, dy, y, cout, wt, nullptr, dx, cin, d, 0);
+  launch_conv<true, false>(s, dy, y, cout, nullptr, 0, wt, nullptr, dx, cin, d, 0);
 }
 
 size_t wgrad_scratch_floats(int cin, int cout, Dims d) {

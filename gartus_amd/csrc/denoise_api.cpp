@@ -185,7 +185,10 @@ int prepare(gt_denoiser* dn, uint32_t n, uint32_t h, uint32_t w, const float* b,
   return GT_OK;
 }
 
-void forward(gt_denoiser* dn) {
+// keep_cat: also materialise each decoder level's concatenation (the weight
+// gradient of its first convolution reads it); inference reads the two
+// halves in place.
+void forward(gt_denoiser* dn, bool keep_cat) {
   hipStream_t s = dn->stream;
   const int L = dn->levels;
   for (int l = 0; l < L; ++l) {
@@ -202,10 +205,15 @@ void forward(gt_denoiser* dn) {
     Level& v = *dn->lv[l];
     const gdn::Dims d = level_dims(dn, l);
     const uint16_t* up = l == L - 2 ? dn->lv[L - 1]->enc.p : dn->lv[l + 1]->dec.p;
-    gdn::upcat_fwd(s, up, dn->ch[l + 1], v.enc.p, dn->ch[l], v.cat.p, d);
     Conv& c0 = dn->dec_conv(l, 0);
     Conv& c1 = dn->dec_conv(l, 1);
-    gdn::conv_fwd(s, v.cat.p, c0.cin, c0.w16.p, dn->params.p + c0.b_off, v.dec_a.p, c0.cout, d, true, false);
+    if (keep_cat) {
+      gdn::upcat_fwd(s, up, dn->ch[l + 1], v.enc.p, dn->ch[l], v.cat.p, d);
+      gdn::conv_fwd(s, v.cat.p, c0.cin, c0.w16.p, dn->params.p + c0.b_off, v.dec_a.p, c0.cout, d, true, false);
+    } else {
+      gdn::conv_fwd_upcat(s, up, dn->ch[l + 1], v.enc.p, dn->ch[l], c0.w16.p, dn->params.p + c0.b_off, v.dec_a.p,
+                          c0.cout, d, true);
+    }
     gdn::conv_fwd(s, v.dec_a.p, c1.cin, c1.w16.p, dn->params.p + c1.b_off, v.dec.p, c1.cout, d, true, false);
   }
   Conv& co = dn->out_conv();
@@ -378,7 +386,7 @@ int gt_denoiser_infer(gt_denoiser* dn, uint32_t n, uint32_t h, uint32_t w, const
   if (!out) return bad(GT_EINVAL, "null output");
   int rc = prepare(dn, n, h, w, b, a, nr);
   if (rc != GT_OK) return rc;
-  forward(dn);
+  forward(dn, false);
   gdn::unpack_output(dn->stream, dn->z.p, out, int(h), int(w), dn->shape);
   GT_CHECK_HIP(hipGetLastError());
   GT_CHECK_HIP(hipStreamSynchronize(dn->stream));
@@ -391,7 +399,7 @@ int gt_denoiser_loss_and_gradients(gt_denoiser* dn, uint32_t n, uint32_t h, uint
   if (!target) return bad(GT_EINVAL, "null target");
   int rc = prepare(dn, n, h, w, b, a, nr);
   if (rc != GT_OK) return rc;
-  forward(dn);
+  forward(dn, true);
   if ((rc = loss_into(dn, target, h, w, out_loss)) != GT_OK) return rc;
   backward(dn);
   GT_CHECK_HIP(hipGetLastError());
@@ -408,7 +416,7 @@ int gt_denoiser_train_step(gt_denoiser* dn, uint32_t n, uint32_t h, uint32_t w, 
   if (!(std::isfinite(lr) && lr > 0.0f)) return bad(GT_EINVAL, "learning rate must be positive and finite");
   int rc = prepare(dn, n, h, w, b, a, nr);
   if (rc != GT_OK) return rc;
-  forward(dn);
+  forward(dn, true);
   if ((rc = loss_into(dn, target, h, w, out_loss)) != GT_OK) return rc;
   backward(dn);
   dn->step += 1;
