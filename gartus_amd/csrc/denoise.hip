@@ -314,12 +314,35 @@ __global__ __launch_bounds__(64) void k_wgrad(const uint16_t* __restrict__ x, in
   }
 }
 
-// out[i] = sum over s < splits of partial[s * n + i], in split order.
-__global__ void k_reduce(const float* __restrict__ partial, int splits, size_t n, float* __restrict__ out) {
-  for (size_t i = blockIdx.x * size_t(blockDim.x) + threadIdx.x; i < n; i += size_t(gridDim.x) * blockDim.x) {
-    float s = 0.f;
-    for (int k = 0; k < splits; ++k) s += partial[size_t(k) * n + i];
-    out[i] = s;
+// out[i] = sum over s < splits of partial[s * n + i]. A block of 256 threads
+// owns 32 consecutive outputs; its 8 thread groups sum the splits congruent
+// to their group index mod 8, in increasing order, and group sums are added
+// in group order: a fixed order, so results are deterministic.
+__global__ __launch_bounds__(256) void k_reduce(const float* __restrict__ partial, int splits, size_t n,
+                                                float* __restrict__ out) {
+  __shared__ float part[8][33];
+  const int o = threadIdx.x & 31, g = threadIdx.x >> 5;
+  const size_t i = blockIdx.x * size_t(32) + o;
+  float acc = 0.f;
+  if (i < n) {
+    int k = g;
+    for (; k + 24 < splits; k += 32) {  // four independent loads in flight
+      const float a = partial[size_t(k) * n + i], b = partial[size_t(k + 8) * n + i];
+      const float c = partial[size_t(k + 16) * n + i], d = partial[size_t(k + 24) * n + i];
+      acc += a;
+      acc += b;
+      acc += c;
+      acc += d;
+    }
+    for (; k < splits; k += 8) acc += partial[size_t(k) * n + i];
+  }
+  part[g][o] = acc;
+  __syncthreads();
+  if (g == 0 && i < n) {
+    float t = part[0][o];
+#pragma unroll
+    for (int j = 1; j < 8; ++j) t += part[j][o];
+    out[i] = t;
   }
 }
 
@@ -588,8 +611,8 @@ void conv_wgrad(hipStream_t s, const uint16_t* x, int cin, const uint16_t* dy, c
   GDN_WG(3, 1); GDN_WG(3, 2); GDN_WG(3, 3); GDN_WG(3, 4);
   GDN_WG(4, 1); GDN_WG(4, 2); GDN_WG(4, 3); GDN_WG(4, 4);
 #undef GDN_WG
-  hipLaunchKernelGGL(k_reduce, dim3(grid_for(nw)), dim3(256), 0, s, pw, splits, nw, dw);
-  hipLaunchKernelGGL(k_reduce, dim3(grid_for(cout)), dim3(256), 0, s, pb, splits, size_t(cout), db);
+  hipLaunchKernelGGL(k_reduce, dim3(unsigned((nw + 31) / 32)), dim3(256), 0, s, pw, splits, nw, dw);
+  hipLaunchKernelGGL(k_reduce, dim3(unsigned((cout + 31) / 32)), dim3(256), 0, s, pb, splits, size_t(cout), db);
 }
 
 void pool_fwd(hipStream_t s, const uint16_t* x, uint16_t* y, int c, Dims d) {
