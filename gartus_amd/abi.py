@@ -43,6 +43,9 @@ class GtCamera(C.Structure):
         ("light_pdf_weight", C.c_double),
         ("rng_mode", C.c_uint32),
         ("flags", C.c_uint32),
+        ("adaptive_min_samples", C.c_uint32),
+        ("adaptive_max_samples", C.c_uint32),
+        ("adaptive_error_threshold", C.c_double),
     ]
 
 

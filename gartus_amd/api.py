@@ -569,6 +569,26 @@ class RayCamera:
             raise abi.InvalidArgument(abi.GT_EINVAL, "light PDF weight must be finite")
         return self._with(direct_lighting_mode=abi.GT_LIGHTING_CONTINUATION, light_pdf_weight=min(max(w, 0.0), 1.0))
 
+    def with_adaptive_sampling(self, min_samples: int, max_samples: int, error_threshold: float):
+        """with_adaptive_sampling (camera.rs:1244-1264): AdaptiveSampling::new
+        (:377-393) raises min to 1 and max to min; spp becomes max."""
+        if not (math.isfinite(error_threshold) and error_threshold > 0.0):
+            raise abi.InvalidArgument(abi.GT_EINVAL, "adaptive sampling error threshold must be positive and finite")
+        mn = max(int(min_samples), 1)
+        mx = max(int(max_samples), mn)
+        return self._with(adaptive_min_samples=mn, adaptive_max_samples=mx,
+                          adaptive_error_threshold=float(error_threshold), samples_per_pixel=mx)
+
+    def without_adaptive_sampling(self):  # camera.rs:1266-1271
+        return self._with(adaptive_min_samples=0, adaptive_max_samples=0, adaptive_error_threshold=0.0)
+
+    def adaptive_sampling(self):
+        """(min_samples, max_samples, error_threshold), or None (camera.rs:1513-1517)."""
+        r = self.raw
+        if r.adaptive_error_threshold == 0.0:
+            return None
+        return int(r.adaptive_min_samples), int(r.adaptive_max_samples), float(r.adaptive_error_threshold)
+
     def image_width(self) -> int:
         return int(self.raw.image_width)
 

@@ -71,6 +71,9 @@ struct gt_context {
   DevBuf<double4> acc;
   DevBuf<float> staging;           // device results of the host-output entry points
   DevBuf<unsigned char> scratch[2];  // staging of the other translation units (gt::context_scratch)
+  // adaptive sampling: per-pixel Welford state, stop flags, active pixel lists
+  DevBuf<double4> ad_mean, ad_m2;
+  DevBuf<uint32_t> ad_done, ad_map[2], ad_count;
   DevBuf<gtk::Frame> frame_table;  // batched renders
   DevBuf<gtk::Seg> seg_table;
   size_t cap_paths = 0;
@@ -319,6 +322,9 @@ int make_camera(const gt_camera* c, GCamera* g) {
   for (int i = 0; i < nbg; ++i)
     if (!std::isfinite(c->background[i])) return fail(GT_EINVAL, "background color components must be finite");
   if (c->flags & ~uint32_t(GT_FLAG_GGX_REJECT_BELOW_HORIZON)) return fail(GT_EINVAL, "unknown camera flags");
+  if (c->adaptive_error_threshold != 0.0 &&
+      !(std::isfinite(c->adaptive_error_threshold) && c->adaptive_error_threshold > 0.0))
+    return fail(GT_EINVAL, "adaptive sampling error threshold must be positive and finite");
   if (c->rng_mode != GT_RNG_COUNTER)
     return fail(GT_EUNSUPPORTED, "the GPU path draws per-(pixel, sample, bounce) streams only (rng_mode GT_RNG_COUNTER)");
   std::memset(g, 0, sizeof *g);
@@ -473,6 +479,8 @@ int make_frame(gt_context* ctx, const gt_scene* scene, const gt_camera* camera, 
   f->has_lights = lights ? 1u : 0u;
   f->light_weight = float(std::min(std::max(camera->light_pdf_weight, 0.0), 1.0));
   f->npix = f->cam.width * f->cam.height;
+  f->nact = f->npix;
+  f->pixmap = nullptr;
   return GT_OK;
 }
 
@@ -506,6 +514,8 @@ namespace {
 struct FrameJob {
   gtk::Frame f{};
   uint32_t spp = 0, depth = 0;
+  uint32_t sample_base = 0;             // first sample index rendered (adaptive rounds)
+  const gtk::Adapt* adapt = nullptr;    // adaptive round: Welford update instead of accumulation
   bool conn_l = false, conn_e = false;  // light / environment shadow rays
   uint64_t stack_need = 0;
   void* d_out = nullptr;                // finalize target (float or double W*H*3), or null
@@ -518,13 +528,13 @@ std::vector<std::vector<gtk::Seg>> plan_batches(const std::vector<FrameJob>& job
   std::vector<std::vector<gtk::Seg>> batches;
   if (single) {  // equal batches of k whole-image samples
     const FrameJob& j = jobs[0];
-    const uint32_t npix = j.f.npix;
+    const uint32_t npix = j.f.nact;
     uint32_t kmax = uint32_t(std::max<uint64_t>(1, std::min<uint64_t>(cap / npix, j.spp)));
     uint32_t nb = (j.spp + kmax - 1) / kmax, k = (j.spp + nb - 1) / nb;
     for (uint32_t b = 0; b < nb; ++b) {
       gtk::Seg sg{};
-      sg.sample0 = b * k;
-      sg.nsamples = std::min(k, j.spp - sg.sample0);
+      sg.sample0 = j.sample_base + b * k;
+      sg.nsamples = std::min(k, j.spp - b * k);
       batches.push_back({sg});
     }
     return batches;
@@ -532,7 +542,7 @@ std::vector<std::vector<gtk::Seg>> plan_batches(const std::vector<FrameJob>& job
   std::vector<gtk::Seg> cur;
   uint64_t used = 0, pix = 0;
   for (uint32_t i = 0; i < jobs.size(); ++i) {
-    const uint32_t npix = jobs[i].f.npix;
+    const uint32_t npix = jobs[i].f.nact;
     uint32_t s0 = 0;
     while (s0 < jobs[i].spp) {
       uint64_t fit = (cap - used) / npix;
@@ -545,7 +555,7 @@ std::vector<std::vector<gtk::Seg>> plan_batches(const std::vector<FrameJob>& job
       gtk::Seg sg{};
       sg.path0 = uint32_t(used);
       sg.frame = i;
-      sg.sample0 = s0;
+      sg.sample0 = jobs[i].sample_base + s0;
       sg.nsamples = uint32_t(std::min<uint64_t>(fit, jobs[i].spp - s0));
       sg.pix0 = uint32_t(pix);
       cur.push_back(sg);
@@ -567,10 +577,11 @@ int run_jobs(gt_context* ctx, std::vector<FrameJob>& jobs, bool multi, int out_i
   cap = std::min<uint64_t>(cap, uint64_t(1) << 31);
   uint64_t acc_total = 0, stack_need = 0;
   uint32_t depth_all = 0;
+  const bool adaptive = !multi && jobs.size() == 1 && jobs[0].adapt != nullptr;
   for (FrameJob& j : jobs) {
     if (uint64_t(j.f.npix) >= (uint64_t(1) << 31)) return fail(GT_EINVAL, "image too large (2^31 pixels)");
     if (j.depth > 4096) return fail(GT_EINVAL, "max_depth above 4096 is not supported");
-    cap = std::max<uint64_t>(cap, j.f.npix);
+    cap = std::max<uint64_t>(cap, j.f.nact);
     acc_total += j.f.npix;
     depth_all = std::max(depth_all, j.depth);
     stack_need = std::max(stack_need, j.stack_need);
@@ -580,7 +591,7 @@ int run_jobs(gt_context* ctx, std::vector<FrameJob>& jobs, bool multi, int out_i
   uint64_t max_batch = 0;
   for (auto& b : batches) {
     uint64_t t = 0;
-    for (auto& sg : b) t += uint64_t(sg.nsamples) * jobs[sg.frame].f.npix;
+    for (auto& sg : b) t += uint64_t(sg.nsamples) * jobs[sg.frame].f.nact;
     max_batch = std::max(max_batch, t);
   }
   int rc;
@@ -644,7 +655,7 @@ int run_jobs(gt_context* ctx, std::vector<FrameJob>& jobs, bool multi, int out_i
   size_t ev = 0;
   auto enqueue = [&]() -> int {
     if (want_stats) GT_HIP(hipEventRecord(ctx->events[ev++], st));
-    gtk::zero_acc(L, ctx->acc.p, acc_total);
+    if (!adaptive) gtk::zero_acc(L, ctx->acc.p, acc_total);
     size_t seg_off = 0;
     for (size_t b = 0; b < nbatch; ++b) {
       gtk::Work w{};
@@ -652,8 +663,8 @@ int run_jobs(gt_context* ctx, std::vector<FrameJob>& jobs, bool multi, int out_i
       w.bt.nseg = uint32_t(batches[b].size());
       uint64_t total = 0, pix = 0;
       for (auto& sg : batches[b]) {
-        total += uint64_t(sg.nsamples) * jobs[multi ? sg.frame : 0].f.npix;
-        pix += jobs[multi ? sg.frame : 0].f.npix;
+        total += uint64_t(sg.nsamples) * jobs[multi ? sg.frame : 0].f.nact;
+        pix += jobs[multi ? sg.frame : 0].f.nact;
       }
       w.bt.total = uint32_t(total);
       w.bt.npix_total = uint32_t(pix);
@@ -681,7 +692,8 @@ int run_jobs(gt_context* ctx, std::vector<FrameJob>& jobs, bool multi, int out_i
         if (batch_e[b]) gtk::connect(L, w, q, bounce, 1, count);
         if (want_stats) GT_HIP(hipEventRecord(ctx->events[ev++], st));
       }
-      gtk::accumulate(L, w, q);
+      if (adaptive) gtk::adapt(L, w, q, *jobs[0].adapt);
+      else gtk::accumulate(L, w, q);
       if (want_stats) {
         uint32_t* h = ctx->host_ctr + host_words * b;
         GT_HIP(hipMemcpyAsync(h, q.ctr, ctr_words * sizeof(uint32_t), hipMemcpyDeviceToHost, st));
@@ -689,11 +701,11 @@ int run_jobs(gt_context* ctx, std::vector<FrameJob>& jobs, bool multi, int out_i
       }
     }
     for (FrameJob& j : jobs)
-      if (j.d_out) gtk::finalize(L, j.f.acc, j.f.npix, j.d_out, out_is_double);
+      if (j.d_out && !adaptive) gtk::finalize(L, j.f.acc, j.f.npix, j.d_out, out_is_double);
     if (want_stats) GT_HIP(hipEventRecord(ctx->events[ev++], st));
     return GT_OK;
   };
-  if (ctx->tuning.use_graph && !want_stats) {
+  if (ctx->tuning.use_graph && !want_stats && !adaptive) {
     // The whole launch sequence is captured once and replayed while nothing
     // it was captured with changes: frames (with their accumulators),
     // segments, queue and table addresses, grid, outputs.
@@ -740,7 +752,7 @@ int run_jobs(gt_context* ctx, std::vector<FrameJob>& jobs, bool multi, int out_i
   if (want_stats) {
     GT_HIP(hipStreamSynchronize(st));
     std::memset(stats, 0, sizeof *stats);
-    for (const FrameJob& j : jobs) stats->camera_paths += uint64_t(j.f.npix) * j.spp;
+    for (const FrameJob& j : jobs) stats->camera_paths += uint64_t(j.f.nact) * j.spp;
     for (size_t b = 0; b < nbatch; ++b) {
       const uint32_t* c = ctx->host_ctr + host_words * b;
       std::vector<uint64_t> tally(tally_words);
@@ -794,6 +806,86 @@ int prepare_job(gt_context* ctx, const gt_scene* cscene, const gt_camera* camera
   return GT_OK;
 }
 
+bool adaptive_camera(const gt_camera* c) {
+  return c->adaptive_error_threshold != 0.0 && c->pixel_sample_mode == GT_PIXEL_RANDOM;
+}
+
+void add_stats(gt_render_stats* t, const gt_render_stats& r) {
+  t->camera_paths += r.camera_paths;
+  t->extension_rays += r.extension_rays;
+  t->shadow_rays += r.shadow_rays;
+  t->accepted_samples += r.accepted_samples;
+  t->device_ms += r.device_ms;
+  t->extend_ms += r.extend_ms;
+  t->extend_launches += r.extend_launches;
+  t->shade_ms += r.shade_ms;
+  t->connect_ms += r.connect_ms;
+  t->ext_node_fetches += r.ext_node_fetches;
+  t->ext_prim_tests += r.ext_prim_tests;
+  t->shadow_node_fetches += r.shadow_node_fetches;
+  t->shadow_prim_tests += r.shadow_prim_tests;
+}
+
+// render_world_pixel_adaptive (camera.rs:2937-2970) for every pixel, in
+// rounds: every pixel takes its first min samples, then the pixels that have
+// not stopped take further rounds (half the samples taken so far, at least
+// min) until none is left or max samples are taken. The Welford update runs
+// over each round's samples in sample order and stops a pixel exactly where
+// the sequential loop would; samples of a round after a pixel's stop are
+// traced but unused. Paths depend only on (pixel, sample), so the image does
+// not depend on the rounds.
+int render_adaptive(gt_context* ctx, FrameJob job, const gt_camera* cam, int out_is_double,
+                    gt_render_stats* stats) {
+  const uint32_t npix = job.f.npix;
+  const uint32_t mn = std::max<uint32_t>(cam->adaptive_min_samples, 1);
+  const uint32_t mx = std::max(cam->adaptive_max_samples, mn);
+  GT_HIP(ctx->ad_mean.ensure(npix));
+  GT_HIP(ctx->ad_m2.ensure(npix));
+  GT_HIP(ctx->ad_done.ensure(npix));
+  GT_HIP(ctx->ad_map[0].ensure(npix));
+  GT_HIP(ctx->ad_map[1].ensure(npix));
+  GT_HIP(ctx->ad_count.ensure(1));
+  hipStream_t st = ctx->stream;
+  GT_HIP(hipMemsetAsync(ctx->ad_mean.p, 0, size_t(npix) * sizeof(double4), st));
+  GT_HIP(hipMemsetAsync(ctx->ad_m2.p, 0, size_t(npix) * sizeof(double4), st));
+  GT_HIP(hipMemsetAsync(ctx->ad_done.p, 0, size_t(npix) * sizeof(uint32_t), st));
+  gtk::Adapt ad{ctx->ad_mean.p, ctx->ad_m2.p, ctx->ad_done.p, mn, cam->adaptive_error_threshold};
+  gtk::Launch L{grid_for(ctx), ctx->tuning.block_size, st};
+  if (stats) std::memset(stats, 0, sizeof *stats);
+  const uint32_t* map = nullptr;
+  uint32_t nact = npix, s = 0, k = std::min(mn, mx);
+  int cur = 0;
+  for (;;) {
+    std::vector<FrameJob> jobs(1, job);
+    jobs[0].f.pixmap = map;
+    jobs[0].f.nact = nact;
+    jobs[0].sample_base = s;
+    jobs[0].spp = k;
+    jobs[0].adapt = &ad;
+    jobs[0].d_out = nullptr;
+    gt_render_stats rs{};
+    int rc = run_jobs(ctx, jobs, false, out_is_double, stats ? &rs : nullptr);
+    if (rc != GT_OK) return rc;
+    if (stats) add_stats(stats, rs);
+    s += k;
+    if (s >= mx) break;
+    GT_HIP(hipMemsetAsync(ctx->ad_count.p, 0, sizeof(uint32_t), st));
+    gtk::adapt_compact(L, map, nact, ctx->ad_done.p, ctx->ad_map[cur].p, ctx->ad_count.p);
+    GT_HIP(hipGetLastError());
+    uint32_t left = 0;
+    GT_HIP(hipMemcpyAsync(&left, ctx->ad_count.p, sizeof left, hipMemcpyDeviceToHost, st));
+    GT_HIP(hipStreamSynchronize(st));
+    if (left == 0) break;
+    nact = left;
+    map = ctx->ad_map[cur].p;
+    cur ^= 1;
+    k = std::min(mx - s, std::max(mn, s / 2));
+  }
+  if (job.d_out) gtk::adapt_finalize(L, ctx->ad_mean.p, npix, job.d_out, out_is_double);
+  GT_HIP(hipGetLastError());
+  return GT_OK;
+}
+
 int render_impl(gt_context* ctx, const gt_scene* scene, const gt_camera* camera, const gt_lights* lights,
                 const gt_env* env, void* d_out, int out_is_double, float* d_albedo, float* d_normal,
                 gt_render_stats* stats) {
@@ -803,7 +895,9 @@ int render_impl(gt_context* ctx, const gt_scene* scene, const gt_camera* camera,
   int rc = prepare_job(ctx, scene, camera, lights, env, &jobs[0]);
   if (rc != GT_OK) return rc;
   jobs[0].d_out = d_out;
-  if ((rc = run_jobs(ctx, jobs, false, out_is_double, stats)) != GT_OK) return rc;
+  if (adaptive_camera(camera)) rc = render_adaptive(ctx, jobs[0], camera, out_is_double, stats);
+  else rc = run_jobs(ctx, jobs, false, out_is_double, stats);
+  if (rc != GT_OK) return rc;
   if (d_albedo) {
     gtk::Launch L{grid_for(ctx), ctx->tuning.block_size, ctx->stream};
     gtk::aov(L, jobs[0].f, queues_of(ctx), d_albedo, d_normal);
@@ -1308,6 +1402,8 @@ int gt_render_batch_device(gt_context* ctx, size_t n, const gt_scene* const* sce
   GT_HIP(hipSetDevice(ctx->device));
   std::vector<FrameJob> jobs(n);
   for (size_t i = 0; i < n; ++i) {
+    if (adaptive_camera(&cameras[i]))
+      return fail(GT_EUNSUPPORTED, "adaptive sampling renders one frame at a time (gt_render_hdr*)");
     int rc = prepare_job(ctx, scenes[i], &cameras[i], lights ? lights[i] : nullptr, envs ? envs[i] : nullptr, &jobs[i]);
     if (rc != GT_OK) return rc;
     jobs[i].d_out = d_out[i];

@@ -76,6 +76,21 @@ struct Frame {
   float light_weight;      // mixture weight in continuation mode
   uint32_t npix;
   double4* acc;            // per-pixel sums (rgb, accepted count)
+  // Pixels this render covers: paths of a sample are pixel indices
+  // 0..nact-1, mapped through pixmap when it is set (adaptive rounds render
+  // only the pixels still sampling); nact = npix and no map otherwise.
+  const uint32_t* pixmap;
+  uint32_t nact;
+};
+
+// Per-pixel state of adaptive sampling (render_world_pixel_adaptive,
+// camera.rs:2937-2970), kept across rounds.
+struct Adapt {
+  double4* mean;        // running mean (rgb) and accepted sample count (w)
+  double4* m2;          // sums of squared deviations (rgb)
+  uint32_t* done;       // 1 once the pixel has stopped
+  uint32_t min_samples;
+  double threshold;
 };
 
 // A batch of paths is a list of segments: segment i renders samples
@@ -118,6 +133,15 @@ void extend(const Launch& l, const Work& w, const Queues& q, uint32_t bounce, bo
 void shade(const Launch& l, const Work& w, const Queues& q, uint32_t bounce);
 void connect(const Launch& l, const Work& w, const Queues& q, uint32_t bounce, uint32_t kind, bool count);
 void accumulate(const Launch& l, const Work& w, const Queues& q);
+// Adaptive sampling (single-frame renders): the Welford update of every
+// covered pixel over this batch's samples, in sample order, up to its stop.
+void adapt(const Launch& l, const Work& w, const Queues& q, const Adapt& a);
+// The covered pixels (map, or 0..n-1 without one) that have not stopped,
+// appended to out in any order; *count must be 0 on entry.
+void adapt_compact(const Launch& l, const uint32_t* map, uint32_t n, const uint32_t* done, uint32_t* out,
+                   uint32_t* count);
+// out[pixel] = the running mean (float or double rgb).
+void adapt_finalize(const Launch& l, const double4* mean, uint32_t npix, void* out, int out_is_double);
 void finalize(const Launch& l, const double4* acc, uint32_t npix, void* out, int out_is_double);
 void aov(const Launch& l, const Frame& f, const Queues& q, float* albedo, float* normal);
 

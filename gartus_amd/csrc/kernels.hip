@@ -75,12 +75,13 @@ __device__ __forceinline__ Stack make_stack(const Queues& q, uint2* smem) {
 }
 
 // Pixel and sample of a path: a segment's paths are sample-major,
-// path = path0 + (sample - sample0) * npix + pixel.
+// path = path0 + (sample - sample0) * nact + idx, pixel = pixmap[idx] (or idx).
 __device__ __forceinline__ void path_coords(const Frame& f, const Seg& sg, uint32_t path, uint32_t* x, uint32_t* y,
                                             uint32_t* sample) {
   const uint32_t local = path - sg.path0;
-  const uint32_t pixel = local % f.npix;
-  *sample = sg.sample0 + local / f.npix;
+  const uint32_t idx = local % f.nact;
+  const uint32_t pixel = f.pixmap ? f.pixmap[idx] : idx;
+  *sample = sg.sample0 + local / f.nact;
   *x = pixel % f.cam.width;
   *y = pixel / f.cam.width;
 }
@@ -131,7 +132,8 @@ __device__ __forceinline__ bool rr_survives(uint32_t bounce, int32_t min_depth, 
   if (!isfinite(m) || m <= EPS) return false;
   float p = clampf(m, 0.05f, 0.95f);
   if (rng.next() >= p) return false;
-  *T = *T / p;
+  const float inv = 1.0f / p;  // LinearRgb / f64 multiplies by the reciprocal (colors.rs:185-190)
+  *T = mk(inv * T->x, inv * T->y, inv * T->z);
   return true;
 }
 
@@ -558,19 +560,107 @@ __global__ void k_accumulate(Frame f0, Batch bt, Queues q) {
     const uint32_t w = r * stride + blockIdx.x * blockDim.x + threadIdx.x;
     with_frame<MULTI, &Seg::pix0>(f0, bt, w < bt.npix_total, w, [&](const Frame& f, const Seg& sg) {
       const uint32_t px = w - sg.pix0;
-      double4 a = f.acc[px];
+      const uint32_t pixel = f.pixmap ? f.pixmap[px] : px;
+      double4 a = f.acc[pixel];
       for (uint32_t s = 0; s < sg.nsamples; ++s) {
-        float4 L = q.radiance[sg.path0 + size_t(s) * f.npix + px];
+        float4 L = q.radiance[sg.path0 + size_t(s) * f.nact + px];
         if (isfinite(L.x) && isfinite(L.y) && isfinite(L.z)) {
           a.x += double(L.x); a.y += double(L.y); a.z += double(L.z); a.w += 1.0;
           ++accepted;
         }
       }
-      f.acc[px] = a;
+      f.acc[pixel] = a;
     });
   }
   for (int off = 32; off > 0; off >>= 1) accepted += __shfl_down(accepted, off);
   if (lane_id() == 0 && accepted) atomicAdd(q.tally, (unsigned long long)accepted);
+}
+
+// adaptive_error (camera.rs:2972-2982): the largest channel's standard error
+// of the mean, infinite below two samples.
+__device__ __forceinline__ double adaptive_error(double4 m2, uint32_t n) {
+#pragma clang fp contract(off)
+  if (n <= 1) return __builtin_inf();
+  // (m2 / (n - 1)) / n, each LinearRgb / f64 a multiply by the reciprocal (colors.rs:185-190)
+  const double c = 1.0 / double(n - 1), k = 1.0 / double(n);
+  const double vx = k * (c * m2.x), vy = k * (c * m2.y), vz = k * (c * m2.z);
+  return sqrt(fmax(fmax(fmax(vx, vy), vz), 0.0));
+}
+
+// render_world_pixel_adaptive (camera.rs:2937-2970) over the samples of one
+// batch: each covered pixel that has not stopped takes them in sample order,
+// skipping non-finite ones, and stops at the first accepted count >= min
+// whose error is below the threshold; later samples of the batch are unused.
+__global__ void k_adapt(Frame f, Batch bt, Queues q, Adapt ad) {
+#pragma clang fp contract(off)
+  const Seg& sg = bt.seg0;
+  uint32_t accepted = 0;
+  const uint32_t stride = gridDim.x * blockDim.x;
+  const uint32_t rounds = (f.nact + stride - 1) / stride;  // wave-uniform trip count
+  for (uint32_t r = 0; r < rounds; ++r) {
+    const uint32_t w = r * stride + blockIdx.x * blockDim.x + threadIdx.x;
+    if (w >= f.nact) continue;
+    const uint32_t pixel = f.pixmap ? f.pixmap[w] : w;
+    if (ad.done[pixel]) continue;
+    double4 mean = ad.mean[pixel], m2 = ad.m2[pixel];
+    uint32_t n = uint32_t(mean.w);
+    bool stop = false;
+    for (uint32_t s = 0; s < sg.nsamples && !stop; ++s) {
+      const float4 L = q.radiance[sg.path0 + size_t(s) * f.nact + w];
+      if (!(isfinite(L.x) && isfinite(L.y) && isfinite(L.z))) continue;
+      ++n;
+      ++accepted;
+      const double inv = 1.0 / double(n);  // delta / count as LinearRgb / f64 (colors.rs:185-190)
+      const double dx = double(L.x) - mean.x, dy = double(L.y) - mean.y, dz = double(L.z) - mean.z;
+      mean.x += inv * dx;
+      mean.y += inv * dy;
+      mean.z += inv * dz;
+      m2.x += dx * (double(L.x) - mean.x);
+      m2.y += dy * (double(L.y) - mean.y);
+      m2.z += dz * (double(L.z) - mean.z);
+      stop = n >= ad.min_samples && adaptive_error(m2, n) < ad.threshold;
+    }
+    mean.w = double(n);
+    ad.mean[pixel] = mean;
+    ad.m2[pixel] = m2;
+    if (stop) ad.done[pixel] = 1u;
+  }
+  for (int off = 32; off > 0; off >>= 1) accepted += __shfl_down(accepted, off);
+  if (lane_id() == 0 && accepted) atomicAdd(q.tally, (unsigned long long)accepted);
+}
+
+__global__ void k_adapt_compact(const uint32_t* map, uint32_t n, const uint32_t* done, uint32_t* out,
+                                uint32_t* count) {
+  const uint32_t stride = gridDim.x * blockDim.x;
+  const uint32_t rounds = (n + stride - 1) / stride;  // wave-uniform trip count
+  for (uint32_t r = 0; r < rounds; ++r) {
+    const uint32_t w = r * stride + blockIdx.x * blockDim.x + threadIdx.x;
+    uint32_t pixel = 0;
+    bool keep = false;
+    if (w < n) {
+      pixel = map ? map[w] : w;
+      keep = done[pixel] == 0u;
+    }
+    const unsigned long long bal = __ballot(keep);
+    if (bal == 0ull) continue;
+    uint32_t base = 0;
+    if (lane_id() == 0) base = atomicAdd(count, uint32_t(__popcll(bal)));
+    base = __shfl(base, 0);
+    if (keep) out[base + uint32_t(__popcll(bal & ((1ull << lane_id()) - 1ull)))] = pixel;
+  }
+}
+
+__global__ void k_adapt_finalize(const double4* mean, uint32_t npix, void* out, int out_is_double) {
+  for (uint32_t px = blockIdx.x * blockDim.x + threadIdx.x; px < npix; px += gridDim.x * blockDim.x) {
+    const double4 m = mean[px];
+    if (out_is_double) {
+      double* o = static_cast<double*>(out) + 3 * size_t(px);
+      o[0] = m.x; o[1] = m.y; o[2] = m.z;
+    } else {
+      float* o = static_cast<float*>(out) + 3 * size_t(px);
+      o[0] = float(m.x); o[1] = float(m.y); o[2] = float(m.z);
+    }
+  }
 }
 
 // Clears accumulators with a kernel (no memset node, so the sequence can be
@@ -585,7 +675,10 @@ __global__ void k_finalize(const double4* acc, uint32_t npix, void* out, int out
   for (uint32_t px = blockIdx.x * blockDim.x + threadIdx.x; px < npix; px += gridDim.x * blockDim.x) {
     double4 a = acc[px];
     double r = 0, g = 0, b = 0;
-    if (a.w > 0.0) { r = a.x / a.w; g = a.y / a.w; b = a.z / a.w; }
+    if (a.w > 0.0) {  // LinearRgb / f64 multiplies by the reciprocal (colors.rs:185-190)
+      const double inv = 1.0 / a.w;
+      r = inv * a.x; g = inv * a.y; b = inv * a.z;
+    }
     if (out_is_double) {
       double* o = static_cast<double*>(out) + 3 * size_t(px);
       o[0] = r; o[1] = g; o[2] = b;
@@ -737,6 +830,19 @@ void accumulate(const Launch& l, const Work& w, const Queues& q) {
   if (w.multi) hipLaunchKernelGGL(k_accumulate<true>, dim3(l.grid), dim3(l.block), 0, l.stream, w.f, w.bt, q);
   else hipLaunchKernelGGL(k_accumulate<false>, dim3(l.grid), dim3(l.block), 0, l.stream, w.f, w.bt, q);
 }
+void adapt(const Launch& l, const Work& w, const Queues& q, const Adapt& a) {
+  hipLaunchKernelGGL(k_adapt, dim3(l.grid), dim3(l.block), 0, l.stream, w.f, w.bt, q, a);
+}
+
+void adapt_compact(const Launch& l, const uint32_t* map, uint32_t n, const uint32_t* done, uint32_t* out,
+                   uint32_t* count) {
+  hipLaunchKernelGGL(k_adapt_compact, dim3(l.grid), dim3(l.block), 0, l.stream, map, n, done, out, count);
+}
+
+void adapt_finalize(const Launch& l, const double4* mean, uint32_t npix, void* out, int out_is_double) {
+  hipLaunchKernelGGL(k_adapt_finalize, dim3(l.grid), dim3(l.block), 0, l.stream, mean, npix, out, out_is_double);
+}
+
 void finalize(const Launch& l, const double4* acc, uint32_t npix, void* out, int out_is_double) {
   hipLaunchKernelGGL(k_finalize, dim3(l.grid), dim3(l.block), 0, l.stream, acc, npix, out, out_is_double);
 }

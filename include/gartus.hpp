@@ -232,6 +232,23 @@ class RayCamera {
     return c;
   }
   RayCamera with_samples_per_pixel(uint32_t n) const { auto c = *this; c.c_.samples_per_pixel = n ? n : 1; return c; }
+  // with_adaptive_sampling (camera.rs:1244-1264; AdaptiveSampling::new :377-393)
+  RayCamera with_adaptive_sampling(uint32_t min_samples, uint32_t max_samples, double error_threshold) const {
+    if (!(std::isfinite(error_threshold) && error_threshold > 0.0))
+      throw Panic(GT_EINVAL, "adaptive sampling error threshold must be positive and finite");
+    auto c = *this;
+    c.c_.adaptive_min_samples = std::max<uint32_t>(min_samples, 1);
+    c.c_.adaptive_max_samples = std::max(max_samples, c.c_.adaptive_min_samples);
+    c.c_.adaptive_error_threshold = error_threshold;
+    c.c_.samples_per_pixel = c.c_.adaptive_max_samples;
+    return c;
+  }
+  RayCamera without_adaptive_sampling() const {
+    auto c = *this;
+    c.c_.adaptive_min_samples = c.c_.adaptive_max_samples = 0;
+    c.c_.adaptive_error_threshold = 0.0;
+    return c;
+  }
   RayCamera with_pixel_sample_mode(PixelSampleMode m) const { auto c = *this; c.c_.pixel_sample_mode = uint32_t(m); return c; }
   RayCamera with_stratified_sampling() const { return with_pixel_sample_mode(PixelSampleMode::Stratified); }
   RayCamera with_stratified_grid_width(uint32_t g) const {

@@ -88,6 +88,17 @@ typedef struct gt_camera {
   double light_pdf_weight;          /* mixture weight for continuation mode */
   uint32_t rng_mode;                /* GT_RNG_* */
   uint32_t flags;                   /* GT_FLAG_* (0: the reference's behaviour) */
+  /* RayCamera::with_adaptive_sampling (camera.rs:1244-1264; AdaptiveSampling
+   * :362-394). Off while adaptive_error_threshold is 0; otherwise it must be
+   * finite and > 0. In GT_PIXEL_RANDOM mode each pixel then runs
+   * render_world_pixel_adaptive (:2937-2982): up to adaptive_max_samples
+   * samples, stopping at the first accepted sample count >= adaptive_min_samples
+   * whose largest per-channel standard error of the mean (Welford, FP64) is
+   * below the threshold; the pixel is that running mean. min is raised to 1
+   * and max to min, as AdaptiveSampling::new does. Stratified modes ignore it. */
+  uint32_t adaptive_min_samples;
+  uint32_t adaptive_max_samples;
+  double adaptive_error_threshold;
 } gt_camera;
 
 /* RayCamera::new(100, 1.0) defaults (camera.rs:1034-1064). */

@@ -90,7 +90,9 @@ static inline V3 operator-(V3 a, V3 b) { return {a.x - b.x, a.y - b.y, a.z - b.z
 static inline V3 operator-(V3 a) { return {-a.x, -a.y, -a.z}; }
 static inline V3 operator*(double s, V3 a) { return {s * a.x, s * a.y, s * a.z}; }
 static inline V3 operator*(V3 a, double s) { return {a.x * s, a.y * s, a.z * s}; }
-static inline V3 operator/(V3 a, double s) { return {a.x / s, a.y / s, a.z / s}; }
+static inline V3 operator/(V3 a, double s) { return {a.x / s, a.y / s, a.z / s}; }  // Vector / f64
+// LinearRgb / f64 (colors.rs:185-190) multiplies by the reciprocal.
+static inline V3 color_div(V3 a, double s) { const double r = 1.0 / s; return {r * a.x, r * a.y, r * a.z}; }
 static inline V3 mul(V3 a, V3 b) { return {a.x * b.x, a.y * b.y, a.z * b.z}; }  // component_mul
 static inline double dot(V3 a, V3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }  // vector.rs:133-135
 static inline V3 cross(V3 a, V3 b) {  // vector.rs:121-129
@@ -1101,6 +1103,7 @@ struct Camera {
     if (!(std::isfinite(c.shutter_start) && std::isfinite(c.shutter_end) && c.shutter_start <= c.shutter_end)) { *err = "shutter interval must be finite and ordered"; return false; }
     int nbg = c.background_kind == GT_BACKGROUND_VERTICAL_GRADIENT ? 9 : 3;
     for (int i = 0; i < nbg; ++i) if (!std::isfinite(c.background[i])) { *err = "background color components must be finite"; return false; }
+    if (c.adaptive_error_threshold != 0.0 && !(std::isfinite(c.adaptive_error_threshold) && c.adaptive_error_threshold > 0.0)) { *err = "adaptive sampling error threshold must be positive and finite"; return false; }
     width = c.image_width;
     double hh = double(width) / c.aspect_ratio;
     height = std::max<uint32_t>(hh >= 4294967295.0 ? 0xffffffffu : uint32_t(hh), 1u);
@@ -1185,7 +1188,7 @@ static bool rr_survives(uint32_t bounce, int32_t min_depth, V3* att, Rng& rng) {
   if (!std::isfinite(m) || m <= EPS) return false;
   double p = clampd(m, RR_MIN, RR_MAX);
   if (rng.random_double() >= p) return false;
-  *att = *att / p;
+  *att = color_div(*att, p);
   return true;
 }
 
@@ -1283,10 +1286,53 @@ static V3 ray_color(const Ctx& c, Ray ray, uint32_t sample, PathRng& pr, Counter
   return color;
 }
 
+// adaptive_error (camera.rs:2972-2982): the largest channel's standard error
+// of the mean; infinite below two accepted samples.
+static double adaptive_error(V3 m2, uint32_t n) {
+  if (n <= 1) return INF;
+  V3 variance = color_div(m2, double(n - 1));
+  return std::sqrt(fmaxr(max_component(color_div(variance, double(n))), 0.0));
+}
+
+// The Welford state of render_world_pixel_adaptive (camera.rs:2944-2966).
+struct Welford {
+  V3 mean, m2;
+  uint32_t n = 0;
+  // Adds a finite sample; true when the pixel stops.
+  bool add(V3 v, uint32_t min_samples, double threshold) {
+    ++n;
+    const double count = double(n);
+    V3 delta = v - mean;
+    mean = mean + color_div(delta, count);
+    V3 delta2 = v - mean;
+    m2 = m2 + mul(delta, delta2);
+    return n >= min_samples && adaptive_error(m2, n) < threshold;
+  }
+};
+
+static bool adaptive_on(const gt_camera& p) {
+  return p.adaptive_error_threshold != 0.0 && p.pixel_sample_mode == GT_PIXEL_RANDOM;
+}
+
 // render_world_linear_pixel_with_context camera.rs:2660-2723
 static V3 render_pixel(const Ctx& c, uint32_t x, uint32_t y, Counters& k) {
   const Camera& cam = *c.cam;
   PathRng pr(int(cam.p.rng_mode), pixel_seed(cam.p.rng_seed, x, y));
+  if (adaptive_on(cam.p)) {  // render_world_pixel_adaptive :2937-2970
+    const uint32_t mn = std::max<uint32_t>(cam.p.adaptive_min_samples, 1);
+    const uint32_t mx = std::max(cam.p.adaptive_max_samples, mn);
+    Welford w;
+    for (uint32_t s = 0; s < mx; ++s) {
+      pr.begin(s, 0);
+      Ray r = cam.sample_ray(x, y, s, pr.rng);
+      k.paths++;
+      V3 v = ray_color(c, r, s, pr, k);
+      if (!finite3(v)) continue;
+      if (w.add(v, mn, cam.p.adaptive_error_threshold)) break;
+    }
+    k.accepted += w.n;
+    return w.mean;
+  }
   V3 sum;
   uint32_t accepted = 0;
   for (uint32_t s = 0; s < cam.spp; ++s) {
@@ -1297,7 +1343,7 @@ static V3 render_pixel(const Ctx& c, uint32_t x, uint32_t y, Counters& k) {
     if (finite3(v)) { sum = sum + v; ++accepted; }  // add_finite_sample :2984-2991
   }
   k.accepted += accepted;
-  return accepted == 0 ? V3() : sum / double(accepted);  // :2993-2999
+  return accepted == 0 ? V3() : color_div(sum, double(accepted));  // :2993-2999
 }
 // render_denoising_aov_linear_pixel camera.rs:3064-3116 (albedo and normal in one pass)
 static void render_aov_pixel(const Ctx& c, uint32_t x, uint32_t y, V3* albedo, V3* normal) {
@@ -1314,8 +1360,8 @@ static void render_aov_pixel(const Ctx& c, uint32_t x, uint32_t y, V3* albedo, V
     if (c.scene->hit(ra, SHADOW_ACNE_EPSILON, INF, &h)) sa = sa + mat_denoise_albedo(*h.mat);
     if (c.scene->hit(rn, SHADOW_ACNE_EPSILON, INF, &h)) sn = sn + 0.5 * (h.s.normal + V3(1.0, 1.0, 1.0));
   }
-  *albedo = sa / double(cam.spp);
-  *normal = sn / double(cam.spp);
+  *albedo = color_div(sa, double(cam.spp));  // camera.rs:3095
+  *normal = color_div(sn, double(cam.spp));
 }
 
 }  // namespace cr
@@ -1687,6 +1733,26 @@ void cr_tone_map(const double* rgb, size_t n, double exposure, int op, double ga
     out[i] = as_u8(256.0 * clampd(g, 0.0, 0.999));
   }
 }
+
+// render_world_pixel_adaptive's loop (camera.rs:2944-2966) over a given
+// sequence of n samples (rgb triples; a non-finite triple is skipped), with
+// max_samples = n: the mean, the number of samples taken and accepted.
+void cr_adaptive_run(const double* samples, uint32_t n, uint32_t min_samples, double threshold, double* mean,
+                     uint32_t* taken, uint32_t* accepted) {
+  Welford w;
+  uint32_t t = 0;
+  for (uint32_t s = 0; s < n; ++s) {
+    ++t;
+    V3 v(samples[3 * s], samples[3 * s + 1], samples[3 * s + 2]);
+    if (!finite3(v)) continue;
+    if (w.add(v, std::max<uint32_t>(min_samples, 1), threshold)) break;
+  }
+  mean[0] = w.mean.x; mean[1] = w.mean.y; mean[2] = w.mean.z;
+  *taken = t;
+  *accepted = w.n;
+}
+
+double cr_adaptive_error(const double* m2, uint32_t n) { return adaptive_error(V3(m2[0], m2[1], m2[2]), n); }
 
 // Rgb::from_raw_linear_color (colors.rs:412-422): the AOV canvases
 // (camera.rs:4212-4243). NaN -> 0, clamp to [0, 1], times 255.999.

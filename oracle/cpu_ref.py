@@ -79,6 +79,9 @@ _PROTOS = {
                                  D, D, D, U64]),
     "cr_tone_map": (None, [D, C.c_size_t, C.c_double, C.c_int, C.c_double, U8]),
     "cr_encode_raw": (None, [D, C.c_size_t, U8]),
+    "cr_adaptive_run": (None, [D, C.c_uint32, C.c_uint32, C.c_double, D, C.POINTER(C.c_uint32),
+                               C.POINTER(C.c_uint32)]),
+    "cr_adaptive_error": (C.c_double, [D, C.c_uint32]),
     "cr_encode_rgbe": (None, [D, C.c_size_t, U8]),
     "cr_encode_pfm": (None, [D, C.c_uint32, C.c_uint32, C.POINTER(C.c_float)]),
 }

@@ -40,5 +40,40 @@ def test_struct_layouts_match_header_sizes():
     # sizes of the plain-data records shared across the boundary
     assert C.sizeof(abi.GtRay) == 32
     assert C.sizeof(abi.GtHit) == 32
-    assert C.sizeof(abi.GtCamera) == 4 + 4 + 8 + 4 * 4 + 8 + 8 + 72 + 8 * 4 + 4 + 4 + 72 + 8 + 4 + 4
     assert C.sizeof(abi.GtTraversalStats) == 56
+
+
+STRUCTS = {"gt_camera": "GtCamera", "gt_tuning": "GtTuning", "gt_render_stats": "GtRenderStats",
+           "gt_ray": "GtRay", "gt_hit": "GtHit", "gt_traversal_stats": "GtTraversalStats",
+           "gt_scene_info": "GtSceneInfo", "gt_tone_map": "GtToneMap", "gt_denoiser_config": "GtDenoiserConfig"}
+
+
+def test_struct_field_offsets_match_the_c_compiler(tmp_path):
+    """Every field offset and struct size of the ctypes mirror equals what the
+    C compiler lays out from the headers."""
+    import shutil
+    import subprocess
+    gcc = shutil.which("gcc")
+    if gcc is None:
+        import pytest
+        pytest.skip("gcc not available")
+    lines = ['#include <stddef.h>', '#include <stdio.h>', '#include "gartus_hip.h"', '#include "gartus_denoise.h"',
+             'int main(void) {']
+    for cname, pyname in STRUCTS.items():
+        st = getattr(abi, pyname)
+        lines.append(f'  printf("{cname} size %zu\\n", sizeof({cname}));')
+        for field, _ in st._fields_:
+            lines.append(f'  printf("{cname}.{field} %zu\\n", offsetof({cname}, {field}));')
+    lines += ['  return 0;', '}']
+    src = tmp_path / "layout.c"
+    src.write_text("\n".join(lines) + "\n")
+    exe = tmp_path / "layout"
+    subprocess.run([gcc, "-std=c99", f"-I{ROOT / 'include'}", str(src), "-o", str(exe)], check=True,
+                   capture_output=True, text=True)
+    out = subprocess.run([str(exe)], check=True, capture_output=True, text=True).stdout.split("\n")
+    want = dict(line.rsplit(" ", 1) for line in out if line)
+    for cname, pyname in STRUCTS.items():
+        st = getattr(abi, pyname)
+        assert int(want[f"{cname} size"]) == C.sizeof(st), cname
+        for field, _ in st._fields_:
+            assert int(want[f"{cname}.{field}"]) == getattr(st, field).offset, f"{cname}.{field}"

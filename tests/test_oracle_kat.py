@@ -668,3 +668,69 @@ def test_environment_alias_sampling_histogram_matches_pdf():
     # the returned pdf is the texel probability over the texel's solid angle
     k = y * 16 + x
     np.testing.assert_allclose(pdfs * sa[k // 16], expect[k], rtol=1e-6)
+
+
+# ------------------------------------------------------------ adaptive sampling ---
+
+def _adaptive(samples, min_samples, threshold):
+    s = np.ascontiguousarray(np.asarray(samples, np.float64).reshape(-1, 3))
+    mean = np.zeros(3)
+    taken, accepted = C.c_uint32(), C.c_uint32()
+    L.cr_adaptive_run(dp(s), len(s), min_samples, threshold, dp(mean), C.byref(taken), C.byref(accepted))
+    return mean, taken.value, accepted.value
+
+
+def test_adaptive_error_known_answers():
+    """camera.rs:4967-4972: m2 (1, 4, 9) over 4 samples gives sqrt(0.75);
+    one sample gives infinity."""
+    m2 = np.array([1.0, 4.0, 9.0])
+    assert L.cr_adaptive_error(dp(m2), 4) == pytest.approx(math.sqrt(0.75), rel=1e-15)
+    assert math.isinf(L.cr_adaptive_error(dp(m2), 1))
+
+
+def test_adaptive_stops_at_min_for_zero_variance_and_runs_to_max_for_noise():
+    """camera.rs:4975-4991 (a constant pixel takes exactly min samples) and
+    :5006-5019 (samples alternating 0, 1 never get below 0.001 and take max,
+    mean 0.5)."""
+    mean, taken, acc = _adaptive(np.full((16, 3), 0.25), 4, 1e-3)
+    assert (taken, acc) == (4, 4) and np.all(mean == 0.25)
+    alt = np.repeat((np.arange(16) % 2).astype(float)[:, None], 3, axis=1)
+    mean, taken, acc = _adaptive(alt, 4, 1e-3)
+    assert (taken, acc) == (16, 16)
+    assert np.all(np.abs(mean - 0.5) < 1e-15)
+
+
+def test_adaptive_skips_non_finite_samples_but_counts_them_against_max():
+    seq = np.full((10, 3), 1.0)
+    seq[[0, 2, 3]] = np.nan
+    mean, taken, acc = _adaptive(seq, 4, 1e-3)
+    assert acc == 4 and taken == 7 and np.all(mean == 1.0)
+    mean, taken, acc = _adaptive(np.full((5, 3), np.inf), 1, 1e-3)
+    assert (taken, acc) == (5, 0) and np.all(mean == 0.0)
+
+
+def test_adaptive_matches_a_brute_force_restatement():
+    """The stopping sample and the mean against numpy recomputed from
+    scratch at every step (two-pass mean and unbiased variance): the
+    Welford recurrence, the min-samples gate and the max-channel rule."""
+    rng = np.random.default_rng(3)
+    for trial in range(200):
+        n = int(rng.integers(1, 200))
+        seq = rng.gamma(rng.uniform(0.5, 4.0), 1.0, (n, 3)) * rng.uniform(0.1, 3.0, 3)
+        mn = int(rng.integers(1, 12))
+        thr = float(rng.uniform(0.02, 0.5))
+        mean, taken, acc = _adaptive(seq, mn, thr)
+        stop = n
+        for k in range(1, n + 1):
+            if k >= mn and k > 1:
+                se = np.sqrt(np.max(seq[:k].var(axis=0, ddof=1) / k))
+                if se < thr * (1 - 1e-9):
+                    stop = k
+                    break
+                if se < thr * (1 + 1e-9):  # too close to call in floating point
+                    stop = None
+                    break
+        if stop is None:
+            continue
+        assert taken == stop, (trial, taken, stop)
+        np.testing.assert_allclose(mean, seq[:stop].mean(axis=0), rtol=1e-12)
