@@ -5,14 +5,15 @@ reached through the C ABI of ``include/gartus_hip.h``. ``gartus_amd.api``
 mirrors the reference's PathTracer / RayCamera / RayScene API on top of it.
 """
 from .api import (Context, DenoisingAovs, DirectLightingMode, EnvironmentLight, HdrImage, PathTracer,
-                  PixelSampleMode, RayCamera, RayMaterial, RayScene, RenderOptions, SamplingTargetList,
-                  SurfaceMaterial, SurfaceScene, ToneMap, ToneMappingOperator, WeightedSamplingTargetList,
-                  default_context)
+                  PixelSampleMode, ProgressiveRenderUpdate, RayCamera, RayMaterial, RayScene, RenderOptions,
+                  RenderProgress, RenderTile, SamplingTargetList, SurfaceMaterial, SurfaceScene, ToneMap,
+                  ToneMappingOperator, WeightedSamplingTargetList, default_context)
 from .scene_data import SceneData
 
 __all__ = [
     "Context", "DenoisingAovs", "DirectLightingMode", "EnvironmentLight", "HdrImage", "PathTracer",
-    "PixelSampleMode", "RayCamera", "RayMaterial", "RayScene", "RenderOptions", "SamplingTargetList",
+    "PixelSampleMode", "ProgressiveRenderUpdate", "RayCamera", "RayMaterial", "RayScene", "RenderOptions",
+    "RenderProgress", "RenderTile", "SamplingTargetList",
     "SceneData", "SurfaceMaterial", "SurfaceScene", "ToneMap", "ToneMappingOperator",
     "WeightedSamplingTargetList", "default_context",
 ]

@@ -131,6 +131,20 @@ U32 = C.POINTER(C.c_uint32)
 I = C.c_int
 
 # name -> (restype, argtypes); every exported symbol of include/gartus_hip.h
+GT_ECANCELLED = 6
+
+
+class GtRenderTile(C.Structure):
+    _fields_ = [("x", C.c_uint32), ("y", C.c_uint32), ("width", C.c_uint32), ("height", C.c_uint32)]
+
+
+class GtRenderProgress(C.Structure):
+    _fields_ = [("tile", GtRenderTile), ("completed_tiles", C.c_uint64), ("total_tiles", C.c_uint64),
+                ("image_width", C.c_uint32), ("image_height", C.c_uint32), ("pixels", C.POINTER(C.c_uint8))]
+
+
+PROGRESS_FN = C.CFUNCTYPE(C.c_int, C.c_void_p, C.POINTER(GtRenderProgress))
+
 GT_TONE_NONE, GT_TONE_REINHARD, GT_TONE_ACES = 0, 1, 2
 GT_ENCODE_CANVAS, GT_ENCODE_RAW, GT_ENCODE_RGBE, GT_ENCODE_PFM = 0, 1, 2, 3
 
@@ -209,6 +223,8 @@ PROTOTYPES = {
     "gt_denoiser_infer": (I, [P, C.c_uint32, C.c_uint32, C.c_uint32, P, P, P, P]),
     "gt_denoiser_loss_and_gradients": (I, [P, C.c_uint32, C.c_uint32, C.c_uint32, P, P, P, P, F, F]),
     "gt_denoiser_train_step": (I, [P, C.c_uint32, C.c_uint32, C.c_uint32, P, P, P, P, C.c_float, F]),
+    "gt_render_progressive": (I, [P, P, C.POINTER(GtCamera), P, P, C.c_uint32, PROGRESS_FN, C.c_void_p,
+                                  C.POINTER(C.c_uint8)]),
     # output boundary
     "gt_encoded_size": (C.c_uint64, [C.c_int, C.c_uint32, C.c_uint32]),
     "gt_encoded_header": (I, [C.c_int, C.c_uint32, C.c_uint32, C.c_char_p, C.c_size_t, C.POINTER(C.c_size_t)]),

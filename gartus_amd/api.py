@@ -770,6 +770,57 @@ class DenoisingAovs:
 # ------------------------------------------------------------------ render ---
 
 @dataclass(frozen=True)
+class RenderTile:
+    """RenderTile (camera.rs:398-424)."""
+    x: int
+    y: int
+    width: int
+    height: int
+
+    def x_end(self) -> int:
+        return self.x + self.width
+
+    def y_end(self) -> int:
+        return self.y + self.height
+
+
+@dataclass(frozen=True)
+class RenderProgress:
+    """RenderProgress (camera.rs:430-447)."""
+    tile: RenderTile
+    completed_tiles: int
+    total_tiles: int
+
+    def is_complete(self) -> bool:
+        return self.completed_tiles >= self.total_tiles
+
+
+class ProgressiveRenderUpdate:
+    """ProgressiveRenderUpdate (camera.rs:451-490): the tile counters and the
+    canvas so far (a read-only view, valid during the callback)."""
+
+    def __init__(self, raw: abi.GtRenderProgress, canvas: np.ndarray):
+        t = raw.tile
+        self._progress = RenderProgress(RenderTile(t.x, t.y, t.width, t.height), int(raw.completed_tiles),
+                                        int(raw.total_tiles))
+        self._w, self._h = int(raw.image_width), int(raw.image_height)
+        self._pixels = canvas.view()
+        self._pixels.flags.writeable = False
+
+    def progress(self) -> RenderProgress:
+        return self._progress
+
+    def image_width(self) -> int:
+        return self._w
+
+    def image_height(self) -> int:
+        return self._h
+
+    def pixels(self) -> np.ndarray:
+        return self._pixels
+
+
+@dataclass(frozen=True)
 class RenderOptions:
     """RenderOptions (renderer.rs:20-45). Tile size is accepted for API
     parity; the GPU path schedules whole-image wavefronts and its result does
@@ -831,6 +882,49 @@ class PathTracer:
     @staticmethod
     def _nee(camera: RayCamera) -> RayCamera:  # light_connection_camera / environment_importance_camera
         return camera.with_direct_lighting_mode(abi.GT_LIGHTING_NEE)
+
+    def _progressive(self, world: RayScene, camera: RayCamera, lights, env, progress, tile_size) -> np.ndarray:
+        if not isinstance(world, RayScene):
+            raise TypeError("the GPU path renders RayScene worlds")
+        ctx = world.context
+        sh = world._compile()
+        lh = lights._compile(ctx) if lights is not None else None
+        eh = env._compile(ctx) if env is not None else None
+        w, h = camera.image_width(), camera.image_height()
+        canvas = np.zeros((h, w, 3), np.uint8)
+        failure = []
+
+        def cb(_user, up):
+            try:
+                if progress is not None:
+                    progress(ProgressiveRenderUpdate(up.contents, canvas))
+                return 0
+            except BaseException as e:  # noqa: BLE001 (handed back to the caller below)
+                failure.append(e)
+                return 1
+
+        fn = abi.PROGRESS_FN(cb)
+        ts = tile_size if tile_size is not None else (self.options.tile_size or 0)
+        rc = ctx.lib.gt_render_progressive(ctx.handle, sh, C.byref(camera.raw), lh, eh, int(ts), fn, None,
+                                           canvas.ctypes.data_as(C.POINTER(C.c_uint8)))
+        if failure:
+            raise failure[0]
+        abi.check(rc)
+        return canvas
+
+    def render_progressive(self, world, progress, tile_size: Optional[int] = None) -> np.ndarray:
+        """render_progressive (renderer.rs:130-144): the display canvas, with
+        progress(update) called on this thread after each tile; an exception
+        raised by progress stops the render and propagates (the reference's
+        Err)."""
+        return self._progressive(world, self.camera, None, None, progress, tile_size)
+
+    def render_with_lights_progressive(self, world, lights, progress, tile_size: Optional[int] = None) -> np.ndarray:
+        return self._progressive(world, self.camera, lights, None, progress, tile_size)
+
+    def render_with_light_connections_progressive(self, world, lights, progress,
+                                                  tile_size: Optional[int] = None) -> np.ndarray:
+        return self._progressive(world, self._nee(self.camera), lights, None, progress, tile_size)
 
     # renderer.rs entry points ----------------------------------------------
     def render(self, world) -> np.ndarray:

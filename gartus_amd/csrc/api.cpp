@@ -74,6 +74,7 @@ struct gt_context {
   // adaptive sampling: per-pixel Welford state, stop flags, active pixel lists
   DevBuf<double4> ad_mean, ad_m2;
   DevBuf<uint32_t> ad_done, ad_map[2], ad_count;
+  DevBuf<uint32_t> band_map;  // progressive renders: the pixels of the current band
   DevBuf<gtk::Frame> frame_table;  // batched renders
   DevBuf<gtk::Seg> seg_table;
   size_t cap_paths = 0;
@@ -835,7 +836,7 @@ void add_stats(gt_render_stats* t, const gt_render_stats& r) {
 // traced but unused. Paths depend only on (pixel, sample), so the image does
 // not depend on the rounds.
 int render_adaptive(gt_context* ctx, FrameJob job, const gt_camera* cam, int out_is_double,
-                    gt_render_stats* stats) {
+                    gt_render_stats* stats, const uint32_t* first_map = nullptr, uint32_t first_n = 0) {
   const uint32_t npix = job.f.npix;
   const uint32_t mn = std::max<uint32_t>(cam->adaptive_min_samples, 1);
   const uint32_t mx = std::max(cam->adaptive_max_samples, mn);
@@ -852,8 +853,8 @@ int render_adaptive(gt_context* ctx, FrameJob job, const gt_camera* cam, int out
   gtk::Adapt ad{ctx->ad_mean.p, ctx->ad_m2.p, ctx->ad_done.p, mn, cam->adaptive_error_threshold};
   gtk::Launch L{grid_for(ctx), ctx->tuning.block_size, st};
   if (stats) std::memset(stats, 0, sizeof *stats);
-  const uint32_t* map = nullptr;
-  uint32_t nact = npix, s = 0, k = std::min(mn, mx);
+  const uint32_t* map = first_map;
+  uint32_t nact = first_map ? first_n : npix, s = 0, k = std::min(mn, mx);
   int cur = 0;
   for (;;) {
     std::vector<FrameJob> jobs(1, job);
@@ -881,7 +882,8 @@ int render_adaptive(gt_context* ctx, FrameJob job, const gt_camera* cam, int out
     cur ^= 1;
     k = std::min(mx - s, std::max(mn, s / 2));
   }
-  if (job.d_out) gtk::adapt_finalize(L, ctx->ad_mean.p, npix, job.d_out, out_is_double);
+  if (job.d_out)
+    gtk::adapt_finalize(L, ctx->ad_mean.p, first_map, first_map ? first_n : npix, job.d_out, out_is_double);
   GT_HIP(hipGetLastError());
   return GT_OK;
 }
@@ -1435,6 +1437,77 @@ int gt_render_batch(gt_context* ctx, size_t n, const gt_scene* const* scenes, co
   for (size_t i = 0; i < n; ++i) {
     const size_t len = (i + 1 < n ? offs[i + 1] : total) - offs[i];
     GT_HIP(hipMemcpy(out_rgb[i], tmp.p + offs[i], len * sizeof(float), hipMemcpyDeviceToHost));
+  }
+  return GT_OK;
+}
+
+int gt_render_progressive(gt_context* ctx, const gt_scene* scene, const gt_camera* camera, const gt_lights* lights,
+                          const gt_env* env, uint32_t tile_size, gt_progress_fn progress, void* user,
+                          uint8_t* out_canvas) {
+  if (!ctx) return fail(GT_EINVAL, "context is null");
+  if (!out_canvas) return fail(GT_EINVAL, "null canvas");
+  GT_HIP(hipSetDevice(ctx->device));
+  FrameJob job;
+  int rc = prepare_job(ctx, scene, camera, lights, env, &job);
+  if (rc != GT_OK) return rc;
+  const uint32_t W = job.f.cam.width, H = job.f.cam.height, npix = job.f.npix;
+  const uint32_t t = tile_size ? tile_size : 16;
+  const uint32_t tiles_x = (W + t - 1) / t, tiles_y = (H + t - 1) / t;
+  const uint64_t total_tiles = uint64_t(tiles_x) * tiles_y;
+  // tile rows per band: about half a queue of paths, at least one row
+  const uint64_t cap = ctx->tuning.max_paths ? ctx->tuning.max_paths : (uint64_t(1) << 25);
+  const bool adaptive = adaptive_camera(camera);
+  const uint64_t spp = adaptive ? std::max<uint32_t>(camera->adaptive_min_samples, 1) : std::max<uint32_t>(job.spp, 1);
+  const uint64_t row_paths = uint64_t(W) * t * spp;
+  const uint32_t band_rows = uint32_t(std::max<uint64_t>(1, std::min<uint64_t>(tiles_y, cap / 2 / row_paths)));
+  std::memset(out_canvas, 0, size_t(npix) * 3);
+  GT_HIP(ctx->staging.ensure(size_t(npix) * 3));
+  GT_HIP(ctx->band_map.ensure(npix));
+  gtk::Launch L{grid_for(ctx), ctx->tuning.block_size, ctx->stream};
+  std::vector<uint32_t> ids;
+  uint64_t done = 0;
+  for (uint32_t ty0 = 0; ty0 < tiles_y; ty0 += band_rows) {
+    const uint32_t ty1 = std::min(tiles_y, ty0 + band_rows);
+    const uint32_t y0 = ty0 * t, y1 = std::min(H, ty1 * t);
+    const uint32_t n = (y1 - y0) * W;
+    ids.resize(n);
+    for (uint32_t i = 0; i < n; ++i) ids[i] = y0 * W + i;
+    GT_HIP(hipMemcpyAsync(ctx->band_map.p, ids.data(), size_t(n) * sizeof(uint32_t), hipMemcpyHostToDevice,
+                          ctx->stream));
+    FrameJob j = job;
+    j.d_out = ctx->staging.p;
+    if (adaptive) {
+      if ((rc = render_adaptive(ctx, j, camera, 0, nullptr, ctx->band_map.p, n)) != GT_OK) return rc;
+    } else {
+      j.f.pixmap = ctx->band_map.p;
+      j.f.nact = n;
+      j.d_out = nullptr;
+      std::vector<FrameJob> jobs(1, j);
+      if ((rc = run_jobs(ctx, jobs, false, 0, nullptr)) != GT_OK) return rc;
+      gtk::finalize_map(L, jobs[0].f.acc, ctx->band_map.p, n, ctx->staging.p, 0);
+      GT_HIP(hipGetLastError());
+    }
+    // the band's display rows (Rgb::from_linear_color) into the host canvas
+    if ((rc = gt_encode_image(ctx, ctx->staging.p + size_t(y0) * W * 3, 0, 1, W, y1 - y0, GT_ENCODE_CANVAS,
+                              nullptr, out_canvas + size_t(y0) * W * 3, 0)) != GT_OK)
+      return rc;
+    for (uint32_t ty = ty0; ty < ty1; ++ty)
+      for (uint32_t tx = 0; tx < tiles_x; ++tx) {
+        gt_render_progress u{};
+        u.tile.x = tx * t;
+        u.tile.y = ty * t;
+        u.tile.width = std::min(W, (tx + 1) * t) - tx * t;
+        u.tile.height = std::min(H, (ty + 1) * t) - ty * t;
+        u.completed_tiles = ++done;
+        u.total_tiles = total_tiles;
+        u.image_width = W;
+        u.image_height = H;
+        u.pixels = out_canvas;
+        if (progress) {
+          const int r = progress(user, &u);
+          if (r != 0) return fail(GT_ECANCELLED, "progressive render stopped by its callback (" + std::to_string(r) + ")");
+        }
+      }
   }
   return GT_OK;
 }

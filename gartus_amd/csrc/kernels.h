@@ -140,9 +140,14 @@ void adapt(const Launch& l, const Work& w, const Queues& q, const Adapt& a);
 // appended to out in any order; *count must be 0 on entry.
 void adapt_compact(const Launch& l, const uint32_t* map, uint32_t n, const uint32_t* done, uint32_t* out,
                    uint32_t* count);
-// out[pixel] = the running mean (float or double rgb).
-void adapt_finalize(const Launch& l, const double4* mean, uint32_t npix, void* out, int out_is_double);
+// out[pixel] = the running mean (float or double rgb) for the n pixels of
+// map (pixels 0..n-1 without one).
+void adapt_finalize(const Launch& l, const double4* mean, const uint32_t* map, uint32_t n, void* out,
+                    int out_is_double);
 void finalize(const Launch& l, const double4* acc, uint32_t npix, void* out, int out_is_double);
+// finalize for the n pixels of map only.
+void finalize_map(const Launch& l, const double4* acc, const uint32_t* map, uint32_t n, void* out,
+                  int out_is_double);
 void aov(const Launch& l, const Frame& f, const Queues& q, float* albedo, float* normal);
 
 struct IntersectOut {

@@ -650,8 +650,10 @@ __global__ void k_adapt_compact(const uint32_t* map, uint32_t n, const uint32_t*
   }
 }
 
-__global__ void k_adapt_finalize(const double4* mean, uint32_t npix, void* out, int out_is_double) {
-  for (uint32_t px = blockIdx.x * blockDim.x + threadIdx.x; px < npix; px += gridDim.x * blockDim.x) {
+__global__ void k_adapt_finalize(const double4* mean, const uint32_t* map, uint32_t n, void* out,
+                                 int out_is_double) {
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    const uint32_t px = map ? map[i] : i;
     const double4 m = mean[px];
     if (out_is_double) {
       double* o = static_cast<double*>(out) + 3 * size_t(px);
@@ -670,9 +672,11 @@ __global__ void k_zero(double4* acc, size_t n) {
     acc[i] = make_double4(0.0, 0.0, 0.0, 0.0);
 }
 
-// average_accepted_samples (camera.rs:2993-2999)
-__global__ void k_finalize(const double4* acc, uint32_t npix, void* out, int out_is_double) {
-  for (uint32_t px = blockIdx.x * blockDim.x + threadIdx.x; px < npix; px += gridDim.x * blockDim.x) {
+// average_accepted_samples (camera.rs:2993-2999) for the n pixels of map
+// (pixels 0..n-1 without one).
+__global__ void k_finalize(const double4* acc, const uint32_t* map, uint32_t n, void* out, int out_is_double) {
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    const uint32_t px = map ? map[i] : i;
     double4 a = acc[px];
     double r = 0, g = 0, b = 0;
     if (a.w > 0.0) {  // LinearRgb / f64 multiplies by the reciprocal (colors.rs:185-190)
@@ -839,12 +843,18 @@ void adapt_compact(const Launch& l, const uint32_t* map, uint32_t n, const uint3
   hipLaunchKernelGGL(k_adapt_compact, dim3(l.grid), dim3(l.block), 0, l.stream, map, n, done, out, count);
 }
 
-void adapt_finalize(const Launch& l, const double4* mean, uint32_t npix, void* out, int out_is_double) {
-  hipLaunchKernelGGL(k_adapt_finalize, dim3(l.grid), dim3(l.block), 0, l.stream, mean, npix, out, out_is_double);
+void adapt_finalize(const Launch& l, const double4* mean, const uint32_t* map, uint32_t n, void* out,
+                    int out_is_double) {
+  hipLaunchKernelGGL(k_adapt_finalize, dim3(l.grid), dim3(l.block), 0, l.stream, mean, map, n, out, out_is_double);
 }
 
 void finalize(const Launch& l, const double4* acc, uint32_t npix, void* out, int out_is_double) {
-  hipLaunchKernelGGL(k_finalize, dim3(l.grid), dim3(l.block), 0, l.stream, acc, npix, out, out_is_double);
+  hipLaunchKernelGGL(k_finalize, dim3(l.grid), dim3(l.block), 0, l.stream, acc, nullptr, npix, out, out_is_double);
+}
+
+void finalize_map(const Launch& l, const double4* acc, const uint32_t* map, uint32_t n, void* out,
+                  int out_is_double) {
+  hipLaunchKernelGGL(k_finalize, dim3(l.grid), dim3(l.block), 0, l.stream, acc, map, n, out, out_is_double);
 }
 void aov(const Launch& l, const Frame& f, const Queues& q, float* albedo, float* normal) {
   hipLaunchKernelGGL(k_aov, dim3(l.grid), dim3(l.block), stack_lds_bytes(l.block, q.stack_cap), l.stream, f, q,

@@ -19,6 +19,7 @@
 #include <cmath>
 #include <cstdint>
 #include <cstdio>
+#include <functional>
 #include <memory>
 #include <stdexcept>
 #include <string>
@@ -412,6 +413,18 @@ class PathTracer {
   HdrImage render_hdr_image(const RayScene& world) const { return render(world, camera_, nullptr, nullptr); }
   HdrImage render_ray_scene_hdr_image(const RayScene& scene) const { return render_hdr_image(scene); }
   std::vector<uint8_t> render(const RayScene& world) const { return render_hdr_image(world).to_canvas(); }
+  // render_progressive (renderer.rs:130-144; camera.rs:3172-3294): the display
+  // canvas, with progress(update) after each tile on this thread. progress
+  // returns false to stop; the render then throws Panic(GT_ECANCELLED).
+  using Progress = std::function<bool(const gt_render_progress&)>;
+  std::vector<uint8_t> render_progressive(const RayScene& world, const Progress& progress,
+                                          uint32_t tile_size = 0) const {
+    return progressive(world, camera_, nullptr, progress, tile_size);
+  }
+  std::vector<uint8_t> render_with_lights_progressive(const RayScene& world, const SamplingTargetList& lights,
+                                                      const Progress& progress, uint32_t tile_size = 0) const {
+    return progressive(world, camera_, &lights, progress, tile_size);
+  }
   // renderer.rs:391-407: lights force next-event estimation only through
   // the light-connection entry points; this one keeps the camera's strategy.
   HdrImage render_with_lights_hdr_image(const RayScene& world, const SamplingTargetList& lights) const {
@@ -447,6 +460,21 @@ class PathTracer {
     check(gt_render_hdr(world.context().get(), world.s(), &cam.raw(), lights ? lights->l() : nullptr,
                         env ? env->e() : nullptr, img.pixels.data(), nullptr));
     return img;
+  }
+  static std::vector<uint8_t> progressive(const RayScene& world, const RayCamera& cam,
+                                          const SamplingTargetList* lights, const Progress& progress,
+                                          uint32_t tile_size) {
+    std::vector<uint8_t> canvas(size_t(cam.image_width()) * cam.image_height() * 3);
+    struct Hook {
+      const Progress* fn;
+      static int call(void* user, const gt_render_progress* u) {
+        const Progress& f = *static_cast<Hook*>(user)->fn;
+        return (!f || f(*u)) ? 0 : 1;
+      }
+    } hook{&progress};
+    check(gt_render_progressive(world.context().get(), world.s(), &cam.raw(), lights ? lights->l() : nullptr,
+                                nullptr, tile_size, &Hook::call, &hook, canvas.data()));
+    return canvas;
   }
   static DenoisingAovs aovs(const RayScene& world, const RayCamera& cam, const SamplingTargetList* lights) {
     DenoisingAovs a{blank(world, cam), blank(world, cam), blank(world, cam)};

@@ -266,6 +266,35 @@ int gt_render_aovs_device(gt_context* ctx, const gt_scene* scene, const gt_camer
                           float* d_beauty, float* d_albedo, float* d_normal,
                           gt_render_stats* out_stats);
 
+/* ---- progressive rendering ----------------------------------------------
+ * RayCamera::render_world*_tiled_progressive (camera.rs:3172-3294,
+ * 3468-3500, 4084-4165, 4414-4437). The image is split into tiles of
+ * tile_size pixels in rows of tiles, top to bottom, left to right
+ * (render_tiles :3272-3294). After each tile is final, progress(user,
+ * &update) runs on the calling thread with the display canvas so far
+ * (Rgb::from_linear_color per pixel; pixels of tiles not yet reported are
+ * 0). The GPU renders bands of whole tile rows at a time, so updates come in
+ * bursts, in tile order. A nonzero return from progress stops the render:
+ * the call returns GT_ECANCELLED and out_canvas holds the tiles reported so
+ * far. The finished canvas equals the display image of gt_render_hdr. */
+enum { GT_ECANCELLED = 6 };
+typedef struct gt_render_tile {  /* RenderTile (camera.rs:398-409) */
+  uint32_t x, y, width, height;
+} gt_render_tile;
+typedef struct gt_render_progress {  /* ProgressiveRenderUpdate (camera.rs:430-490) */
+  gt_render_tile tile;       /* the tile that just completed */
+  uint64_t completed_tiles;  /* including tile */
+  uint64_t total_tiles;
+  uint32_t image_width, image_height;
+  const uint8_t* pixels;     /* image_width * image_height * 3 bytes, row-major */
+} gt_render_progress;
+typedef int (*gt_progress_fn)(void* user, const gt_render_progress* update);
+/* tile_size 0 = 16 (DEFAULT_RENDER_TILE_SIZE, camera.rs:34); out_canvas is a
+ * host buffer of W*H*3 bytes. */
+int gt_render_progressive(gt_context* ctx, const gt_scene* scene, const gt_camera* camera,
+                          const gt_lights* lights, const gt_env* env, uint32_t tile_size,
+                          gt_progress_fn progress, void* user, uint8_t* out_canvas);
+
 /* ---- batched rendering: many scenes and cameras in one wavefront ------
  * No reference counterpart: the reference renders one scene per call, and a
  * caller loops over PathTracer::render_* (renderer.rs:101-224,374-609). Here

@@ -133,6 +133,27 @@ int render(const char* out) {
   std::remove(tmp.c_str());
   expect(n == 12 && std::strncmp(head, "PF\n2 1\n-1.0\n", 12) == 0, "PFM header");
   expect(m == 6 && px[0] == 4.0f && px[5] == 0.0f, "PFM body, non-finite written as 0");
+  // progressive: 64x64 in tiles of 16 reports 16 tiles in order; false stops it
+  {
+    uint64_t seen = 0;
+    bool in_order = true;
+    std::vector<uint8_t> prog = gartus::PathTracer(cam).render_with_lights_progressive(
+        scene, lights, [&](const gt_render_progress& u) {
+          in_order = in_order && u.completed_tiles == seen + 1;
+          seen = u.completed_tiles;
+          return true;
+        }, 16);
+    (void)prog;
+    expect(seen == 16 && in_order, "16 tiles of 16 reported in order");
+    bool threw_cancel = false;
+    try {
+      gartus::PathTracer(cam).render_with_lights_progressive(scene, lights,
+                                                             [](const gt_render_progress&) { return false; });
+    } catch (const gartus::Panic& p) {
+      threw_cancel = p.code == GT_ECANCELLED;
+    }
+    expect(threw_cancel, "a callback returning false stops the render");
+  }
   if (failures) return 1;
   std::printf("rendered %ux%u, %zu primitives, %zu lights -> %s\n", img.width, img.height, scene.len(), lights.len(),
               out);
