@@ -70,6 +70,9 @@ class GtSceneInfo(C.Structure):
         ("device_bytes", C.c_uint64),
         ("bounds_min", C.c_double * 3),
         ("bounds_max", C.c_double * 3),
+        ("build_ms", C.c_double),
+        ("built_on_device", C.c_uint32),
+        ("reserved", C.c_uint32),
     ]
 
 
@@ -189,6 +192,7 @@ PROTOTYPES = {
     "gt_scene_add_triangles": (I, [P, D, C.c_size_t, U32]),
     "gt_scene_add_quads": (I, [P, D, C.c_size_t, U32]),
     "gt_scene_build": (I, [P, C.c_uint32]),
+    "gt_scene_build_device": (I, [P, C.c_uint32]),
     "gt_scene_get_info": (I, [P, C.POINTER(GtSceneInfo)]),
     "gt_intersect": (I, [P, P, C.POINTER(GtRay), C.c_size_t, C.c_float, C.POINTER(GtHit),
                          C.POINTER(GtTraversalStats)]),

@@ -129,6 +129,7 @@ class RayScene:
         self._ctx = context
         self._handle = None
         self._leaf_size = None   # None: not built (lazy default build on first hit)
+        self._on_device = False  # build with gt_scene_build_device
         self._dirty = True
 
     # construction ------------------------------------------------------
@@ -180,10 +181,23 @@ class RayScene:
 
     def build_bvh_with_options(self, leaf_size: int = 4) -> None:
         self._leaf_size = max(int(leaf_size), 1)
+        self._on_device = False
         self._compile(force_build=True)
 
     def with_bvh(self) -> "RayScene":
         self.build_bvh()
+        return self
+
+    def build_bvh_on_device(self, leaf_size: int = 4) -> None:
+        """gt_scene_build_device: the scene compile on the GPU (a linear BVH
+        over Morton codes; no reference counterpart). Closest hits are those
+        of build_bvh; the tree, and so the traversal counts, differ."""
+        self._leaf_size = max(int(leaf_size), 1)
+        self._on_device = True
+        self._compile(force_build=True)
+
+    def with_device_bvh(self, leaf_size: int = 4) -> "RayScene":
+        self.build_bvh_on_device(leaf_size)
         return self
 
     # queries -------------------------------------------------------------
@@ -234,9 +248,11 @@ class RayScene:
                 abi.check(adders[kind](h, _dp(data), len(data), mids.ctypes.data_as(abi.U32)))
             self._dirty = False
             if self._leaf_size is not None:
-                abi.check(lib.gt_scene_build(h, self._leaf_size))
+                build = lib.gt_scene_build_device if self._on_device else lib.gt_scene_build
+                abi.check(build(h, self._leaf_size))
         elif force_build:
-            abi.check(lib.gt_scene_build(self._handle, self._leaf_size))
+            build = lib.gt_scene_build_device if self._on_device else lib.gt_scene_build
+            abi.check(build(self._handle, self._leaf_size))
         return self._handle
 
     def info(self) -> abi.GtSceneInfo:

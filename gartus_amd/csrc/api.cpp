@@ -14,6 +14,7 @@
 #include <vector>
 
 #include "../../include/gartus_hip.h"
+#include "bvh_build.h"
 #include "kernels.h"
 #include "scene_build.h"
 #include "status.h"
@@ -108,6 +109,9 @@ struct gt_scene {
   DevBuf<uint32_t> prim_ids;
   DevBuf<GMat> dmats;
   uint64_t device_bytes = 0;
+  uint64_t n_nodes2 = 0, n_nodes4 = 0;  // node records the traversal may reference
+  bool on_device = false;               // built by gt_scene_build_device
+  gbvh::Result device_build{};
 };
 
 struct gt_lights {
@@ -134,13 +138,7 @@ namespace {
 
 bool finite3(double r, double g, double b) { return std::isfinite(r) && std::isfinite(g) && std::isfinite(b); }
 
-int upload_scene(gt_scene* s) {
-  gt::PackedScene& pk = s->packed;
-  GT_HIP(hipSetDevice(s->ctx->device));
-  GT_HIP(s->nodes2.ensure(pk.nodes2.size()));
-  GT_HIP(s->nodes4.ensure(pk.nodes4.size()));
-  GT_HIP(s->dprims.ensure(pk.prims.size()));
-  GT_HIP(s->prim_ids.ensure(pk.prim_ids.size()));
+int upload_materials(gt_scene* s, size_t* bytes) {
   std::vector<GMat> gm(s->mats.size());
   for (size_t i = 0; i < s->mats.size(); ++i) {
     const gt::HostMat& m = s->mats[i];
@@ -153,15 +151,33 @@ int upload_scene(gt_scene* s) {
     g.f0 = float(std::min(std::max(lum, 0.0), 1.0));
   }
   GT_HIP(s->dmats.ensure(gm.size()));
+  if (!gm.empty())
+    GT_HIP(hipMemcpyAsync(s->dmats.p, gm.data(), gm.size() * sizeof(GMat), hipMemcpyHostToDevice, s->ctx->stream));
+  GT_HIP(hipStreamSynchronize(s->ctx->stream));
+  *bytes = gm.size() * sizeof(GMat);
+  return GT_OK;
+}
+
+int upload_scene(gt_scene* s) {
+  gt::PackedScene& pk = s->packed;
+  GT_HIP(hipSetDevice(s->ctx->device));
+  GT_HIP(s->nodes2.ensure(pk.nodes2.size()));
+  GT_HIP(s->nodes4.ensure(pk.nodes4.size()));
+  GT_HIP(s->dprims.ensure(pk.prims.size()));
+  GT_HIP(s->prim_ids.ensure(pk.prim_ids.size()));
   hipStream_t st = s->ctx->stream;
   if (!pk.nodes2.empty()) GT_HIP(hipMemcpyAsync(s->nodes2.p, pk.nodes2.data(), pk.nodes2.size() * sizeof(GNode2), hipMemcpyHostToDevice, st));
   if (!pk.nodes4.empty()) GT_HIP(hipMemcpyAsync(s->nodes4.p, pk.nodes4.data(), pk.nodes4.size() * sizeof(GNode4), hipMemcpyHostToDevice, st));
   if (!pk.prims.empty()) GT_HIP(hipMemcpyAsync(s->dprims.p, pk.prims.data(), pk.prims.size() * sizeof(GPrim), hipMemcpyHostToDevice, st));
   if (!pk.prim_ids.empty()) GT_HIP(hipMemcpyAsync(s->prim_ids.p, pk.prim_ids.data(), pk.prim_ids.size() * sizeof(uint32_t), hipMemcpyHostToDevice, st));
-  if (!gm.empty()) GT_HIP(hipMemcpyAsync(s->dmats.p, gm.data(), gm.size() * sizeof(GMat), hipMemcpyHostToDevice, st));
   GT_HIP(hipStreamSynchronize(st));
+  size_t mat_bytes = 0;
+  if (int rc = upload_materials(s, &mat_bytes)) return rc;
   s->device_bytes = pk.nodes2.size() * sizeof(GNode2) + pk.nodes4.size() * sizeof(GNode4) +
-                    pk.prims.size() * sizeof(GPrim) + pk.prim_ids.size() * 4 + gm.size() * sizeof(GMat);
+                    pk.prims.size() * sizeof(GPrim) + pk.prim_ids.size() * 4 + mat_bytes;
+  s->n_nodes2 = pk.nodes2.size();
+  s->n_nodes4 = pk.nodes4.size();
+  s->on_device = false;
   return GT_OK;
 }
 
@@ -180,6 +196,40 @@ int build_scene(gt_scene* s, uint32_t leaf_size) {
   }
   int rc = upload_scene(s);
   if (rc != GT_OK) return rc;
+  s->built = true;
+  return GT_OK;
+}
+
+int build_scene_device(gt_scene* s, uint32_t leaf_size) {
+  s->leaf_size = leaf_size == 0 ? 1 : leaf_size;
+  const size_t n = s->prims.size();
+  if (n == 0) return build_scene(s, leaf_size);  // the empty scene needs no device work
+  if (n >= (size_t(1) << 30)) return fail(GT_EINVAL, "too many primitives for a device build (2^30)");
+  for (gt::HostPrim& p : s->prims) p.mat_kind = uint32_t(s->mats[p.material].kind);
+  GT_HIP(hipSetDevice(s->ctx->device));
+  const size_t m = std::max<size_t>(n - 1, 1);
+  GT_HIP(s->nodes2.ensure(m));
+  GT_HIP(s->nodes4.ensure(m));
+  GT_HIP(s->dprims.ensure(n));
+  GT_HIP(s->prim_ids.ensure(n));
+  DevBuf<unsigned char> scratch;
+  GT_HIP(scratch.ensure(gbvh::scratch_bytes(uint32_t(n))));
+  gbvh::Result r{};
+  GT_HIP(gbvh::build(s->ctx->stream, s->prims.data(), uint32_t(n), s->leaf_size, scratch.p, s->dprims.p,
+                     s->prim_ids.p, s->nodes2.p, s->nodes4.p, &r));
+  size_t mat_bytes = 0;
+  if (int rc = upload_materials(s, &mat_bytes)) return rc;
+  s->bvh = gt::BinaryBvh();
+  s->packed = gt::PackedScene();
+  s->packed.root_ref2 = s->packed.root_ref4 = r.root_ref;
+  for (int a = 0; a < 3; ++a) { s->packed.root_lo[a] = r.root_lo[a]; s->packed.root_hi[a] = r.root_hi[a]; }
+  s->packed.depth2 = r.depth;
+  s->packed.depth4 = r.depth;  // a wide level spans one or two binary levels
+  const bool root_leaf = r.root_ref & GT_LEAF_BIT;
+  s->n_nodes2 = s->n_nodes4 = root_leaf ? 0 : m;
+  s->device_bytes = m * (sizeof(GNode2) + sizeof(GNode4)) + n * (sizeof(GPrim) + 4) + mat_bytes;
+  s->device_build = r;
+  s->on_device = true;
   s->built = true;
   return GT_OK;
 }
@@ -381,10 +431,10 @@ gtd::DScene device_scene(const gt_scene* s, uint32_t width) {
   d.prim_ids = s->prim_ids.p;
   d.mats = s->dmats.p;
   for (int a = 0; a < 3; ++a) { d.root_lo[a] = s->packed.root_lo[a]; d.root_hi[a] = s->packed.root_hi[a]; }
-  bool wide = width == 4 && !s->packed.nodes4.empty();
+  bool wide = width == 4 && s->n_nodes4 != 0;
   d.width = wide ? 4 : 2;
   d.root_ref = wide ? s->packed.root_ref4 : s->packed.root_ref2;
-  if (s->packed.nodes2.empty()) { d.width = 2; d.root_ref = s->packed.root_ref2; }
+  if (s->n_nodes2 == 0) { d.width = 2; d.root_ref = s->packed.root_ref2; }
   return d;
 }
 
@@ -1127,12 +1177,25 @@ int gt_scene_build(gt_scene* s, uint32_t leaf_size) {
   return build_scene(s, leaf_size);
 }
 
+int gt_scene_build_device(gt_scene* s, uint32_t leaf_size) {
+  if (!s) return fail(GT_EINVAL, "scene is null");
+  return build_scene_device(s, leaf_size);
+}
+
 int gt_scene_get_info(const gt_scene* s, gt_scene_info* out) {
   if (!s || !out) return fail(GT_EINVAL, "null argument");
   std::memset(out, 0, sizeof *out);
   out->primitive_count = s->prims.size();
   out->material_count = s->mats.size();
-  if (s->built && !s->bvh.nodes.empty()) {
+  if (s->built && s->on_device) {
+    const gbvh::Result& r = s->device_build;
+    out->bvh_node_count = r.internal_nodes + r.leaves;
+    out->bvh_depth = r.depth;
+    out->device_bytes = s->device_bytes;
+    for (int a = 0; a < 3; ++a) { out->bounds_min[a] = r.root_lo[a]; out->bounds_max[a] = r.root_hi[a]; }
+    out->build_ms = r.ms;
+    out->built_on_device = 1;
+  } else if (s->built && !s->bvh.nodes.empty()) {
     out->bvh_node_count = s->bvh.nodes.size();
     out->bvh_depth = s->bvh.depth;
     out->device_bytes = s->device_bytes;

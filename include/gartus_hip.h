@@ -161,6 +161,16 @@ int gt_scene_add_quads(gt_scene* scene, const double* data, size_t n, const uint
  * build with the reference's exact splits, then upload to the GPU.
  * leaf_size 0 is clamped to 1 (bvh.rs:73-75). Any later add invalidates it. */
 int gt_scene_build(gt_scene* scene, uint32_t leaf_size);
+/* No reference counterpart: the scene compile on the device, for large and
+ * changing scenes. Replaces the host SAH build of SurfaceScene::to_ray_scene
+ * and FlatBvh::build (scene.rs:1517-1547, bvh.rs:233-288) with a linear BVH
+ * (Karras 2012) over 63-bit Morton codes, leaves of at most leaf_size
+ * primitives. Primitive records are bit-identical to the host compile's and
+ * closest hits are the same (they do not depend on the tree); traversal
+ * counts differ because the tree does. A later mutation invalidates it like
+ * any build; the next render then builds on the host unless this is called
+ * again. */
+int gt_scene_build_device(gt_scene* scene, uint32_t leaf_size);
 
 typedef struct gt_scene_info {
   uint64_t primitive_count;
@@ -169,6 +179,9 @@ typedef struct gt_scene_info {
   uint64_t bvh_depth;           /* deepest root-to-leaf path, root = 1 */
   uint64_t device_bytes;        /* nodes + primitives + materials on the GPU */
   double bounds_min[3], bounds_max[3];
+  double build_ms;              /* device time of gt_scene_build_device; 0 after a host build */
+  uint32_t built_on_device;     /* 1 after gt_scene_build_device */
+  uint32_t reserved;
 } gt_scene_info;
 int gt_scene_get_info(const gt_scene* scene, gt_scene_info* out);
 
