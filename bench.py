@@ -229,6 +229,8 @@ def main() -> int:
     ap.add_argument("--bvh-width", type=int, default=0)
     ap.add_argument("--max-paths", type=int, default=0)
     ap.add_argument("--graph", action="store_true", help="replay the render as a captured hipGraph")
+    ap.add_argument("--device-build", action="store_true",
+                    help="compile the scenes on the GPU (gt_scene_build_device) instead of the host SAH build")
     ap.add_argument("--autotune", action="store_true", help="pick the tuning with gt_autotune (batch64: always)")
     ap.add_argument("--no-denoise-eval", action="store_true",
                     help="skip the time-to-relMSE comparison of plain and denoised rendering (ggx_env_1080p)")
@@ -269,9 +271,13 @@ def main() -> int:
     outs = (C.c_void_p * n)()
     npaths = 0
     prims = nodes = 0
-    # scene compiles (the host SAH build) run in parallel: ctypes drops the GIL
-    with ThreadPoolExecutor(max_workers=min(16, os.cpu_count() or 1)) as ex:
-        built = list(ex.map(lambda fr: api.RayScene(fr["data"], ctx).with_bvh(), frames))
+    if args.device_build:  # one context stream: one scene after another
+        built = [api.RayScene(fr["data"], ctx).with_device_bvh() for fr in frames]
+    else:  # the host SAH builds run in parallel: ctypes drops the GIL
+        with ThreadPoolExecutor(max_workers=min(16, os.cpu_count() or 1)) as ex:
+            built = list(ex.map(lambda fr: api.RayScene(fr["data"], ctx).with_bvh(), frames))
+    build_wall = time.perf_counter() - t0
+    build_device_ms = sum(rs.info().build_ms for rs in built)
     for i, fr in enumerate(frames):
         rs = built[i]
         fr["scene"] = rs
@@ -289,7 +295,8 @@ def main() -> int:
         d = C.c_void_p()
         abi.check(lib.gt_device_malloc(ctx.handle, W * H * 3 * 4, C.byref(d)))
         outs[i] = d
-    log(f"{n} scene(s) compiled in {time.perf_counter() - t0:.2f} s: {prims:,} prims, {nodes:,} BVH nodes")
+    log(f"{n} scene(s) compiled in {build_wall:.2f} s ({'device LBVH, ' + format(build_device_ms, '.1f') + ' ms on the GPU' if args.device_build else 'host SAH'}): "
+        f"{prims:,} prims, {nodes:,} BVH nodes")
     tuned = None
     if args.autotune or args.workload == "batch64":
         best = abi.GtTuning()
@@ -356,7 +363,10 @@ def main() -> int:
     cam0 = cams[0]
     config = {"workload": desc, "frames": n, "primitives": int(prims), "paths_per_step": int(npaths),
               "rays_per_step": int(rays_step), "samples_per_s": round(npaths * args.steps / wall, 1),
-              "hipgraph": bool(args.graph)}
+              "hipgraph": bool(args.graph),
+              "scene_compile": {"builder": "device LBVH" if args.device_build else "host SAH",
+                                "wall_s": round(build_wall, 3),
+                                "device_ms": round(build_device_ms, 2) if args.device_build else None}}
     if n == 1:
         config.update(width=W0, height=H0, spp=int(lib.gt_camera_effective_spp(C.byref(cam0))),
                       max_depth=int(cam0.max_depth))

@@ -5,8 +5,9 @@
 //   1. pack every primitive into its 64-byte record and its FP64 box (the
 //      host compile's formulas, so records and boxes are bit-identical),
 //      rounded outward to FP32; reduce the centroid bounds;
-//   2. 63-bit Morton codes of the centroids (21 bits per axis), radix-sorted
-//      with the primitive index as payload;
+//   2. 63-bit Morton codes of the centroids (21 bits per axis of one cubic
+//      grid over the centroid bounds), radix-sorted with the primitive index
+//      as payload;
 //   3. one internal node per adjacent pair of sorted keys: its range and
 //      split from the longest common prefixes (equal keys break ties by
 //      index), children and parent links;
@@ -206,15 +207,17 @@ __global__ __launch_bounds__(256) void k_morton(const double* __restrict__ cen, 
                                                 const unsigned long long* __restrict__ cbounds,
                                                 unsigned long long* __restrict__ keys, uint32_t* __restrict__ vals) {
 #pragma clang fp contract(off)
-  double lo[3], ext[3];
+  // one cubic grid over the centroid bounds: a flat axis gets only the bits its extent needs,
+  // so a height field is ordered by its two long axes first
+  double lo[3], ext = 0.0;
   for (int a = 0; a < 3; ++a) {
     lo[a] = dval(cbounds[a]);
-    ext[a] = dval(cbounds[3 + a]) - lo[a];
+    ext = fmax(ext, dval(cbounds[3 + a]) - lo[a]);
   }
   for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
     unsigned long long code = 0;
     for (int a = 0; a < 3; ++a) {
-      double t = ext[a] > 0.0 ? (cen[3 * size_t(i) + a] - lo[a]) / ext[a] : 0.0;
+      double t = ext > 0.0 ? (cen[3 * size_t(i) + a] - lo[a]) / ext : 0.0;
       t = fmin(fmax(t * 2097152.0, 0.0), 2097151.0);
       code |= spread21((unsigned long long)t) << (2 - a);
     }
