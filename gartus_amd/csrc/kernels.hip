@@ -36,10 +36,10 @@ __device__ __forceinline__ unsigned long long lanes_below() {
   return lane_id() == 0 ? 0ull : ((~0ull) >> (64 - lane_id()));
 }
 
-// Claims kChunk * 64 consecutive work items for the calling wave.
-__device__ __forceinline__ uint32_t wave_fetch(uint32_t* cursor) {
+// Claims per_lane * 64 consecutive work items for the calling wave.
+__device__ __forceinline__ uint32_t wave_fetch(uint32_t* cursor, uint32_t per_lane) {
   uint32_t base = 0;
-  if (lane_id() == 0) base = atomicAdd(cursor, kChunk * 64u);
+  if (lane_id() == 0) base = atomicAdd(cursor, per_lane * 64u);
   return __shfl(base, 0);
 }
 
@@ -165,6 +165,16 @@ __global__ void k_reset(uint32_t* ctr, uint32_t ncounters, unsigned long long* t
 // so every slice has at least one block whatever the grid.
 __device__ __forceinline__ uint32_t slice_step() { return gridDim.x < kSlices ? gridDim.x : kSlices; }
 
+// Items per lane per claim on a slice queue of `count` items: kChunk (one
+// atomic per 256 items) while the queue holds two such claims for every wave
+// serving the slice, else one, so that a short queue (late bounces, few
+// samples) spreads over all the waves instead of a few that each trace four
+// items one after another.
+__device__ __forceinline__ uint32_t items_per_lane(uint32_t count) {
+  const uint32_t waves = (gridDim.x / slice_step()) * (blockDim.x / 64u);
+  return count >= waves * (2u * kChunk * 64u) ? kChunk : 1u;
+}
+
 // Wave-reduced 64-bit add of per-lane counts (one atomic per wave).
 __device__ __forceinline__ void wave_count(unsigned long long* dst, uint32_t v) {
   for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off);
@@ -233,8 +243,9 @@ __global__ __attribute__((amdgpu_waves_per_eu(8))) void k_extend(Frame f0, Batch
   if (count == 0) continue;
   const uint32_t first = bounce == 0 ? q.ctr[ctr_at(0, S_BASE, s)] : 0;
   const uint32_t* __restrict__ active = q.active[bounce & 1] + size_t(s) * q.region;
+  const uint32_t per_lane = items_per_lane(count);
   for (;;) {
-    uint32_t base = wave_fetch(q.ctr + ctr_at(bounce, S_EXT_WORK, s));
+    uint32_t base = wave_fetch(q.ctr + ctr_at(bounce, S_EXT_WORK, s), per_lane);
     if (base >= count) break;
     uint32_t nodes = 0, prims = 0;
     uint32_t ids[kChunk], cats[kChunk];
@@ -243,7 +254,7 @@ __global__ __attribute__((amdgpu_waves_per_eu(8))) void k_extend(Frame f0, Batch
       uint32_t j = base + k * 64u + lane_id();
       cats[k] = kNumCategories;
       ids[k] = 0;
-      const bool valid = j < count;
+      const bool valid = k < per_lane && j < count;
       const uint32_t p = !valid ? 0u : bounce == 0 ? first + j : active[j];
       ids[k] = p;
       with_frame<MULTI>(f0, bt, valid, p, [&](const Frame& f, const Seg&) {
@@ -424,14 +435,15 @@ __global__ void k_shade(Frame f0, Batch bt, Queues q, uint32_t bounce) {
   if (count == 0) continue;
   const size_t region = size_t(s) * q.region;
   const uint32_t* __restrict__ list = q.cat[CAT] + region;
+  const uint32_t per_lane = items_per_lane(count);
   for (;;) {
-    uint32_t base = wave_fetch(q.ctr + ctr_at(bounce, S_CAT_WORK + CAT, s));
+    uint32_t base = wave_fetch(q.ctr + ctr_at(bounce, S_CAT_WORK + CAT, s), per_lane);
     if (base >= count) break;
     uint32_t ids[kChunk], outs[kChunk];
 #pragma unroll
     for (uint32_t k = 0; k < kChunk; ++k) {
       const uint32_t j = base + k * 64u + lane_id();
-      const bool valid = j < count;
+      const bool valid = k < per_lane && j < count;
       const uint32_t p = valid ? list[j] : 0u;
       ids[k] = p;
       outs[k] = 0;
@@ -499,11 +511,12 @@ __global__ __attribute__((amdgpu_waves_per_eu(8))) void k_connect(Frame f0, Batc
   if (count == 0) continue;
   uint32_t* cursor = q.ctr + ctr_at(bounce, kind == 0 ? S_SL_WORK : S_SE_WORK, s);
   const uint32_t* __restrict__ list = q.shadow[kind] + size_t(s) * q.region;
+  const uint32_t per_lane = items_per_lane(count);
   for (;;) {
-    uint32_t base = wave_fetch(cursor);
+    uint32_t base = wave_fetch(cursor, per_lane);
     if (base >= count) break;
     uint32_t nodes = 0, prims = 0;
-    for (uint32_t k = 0; k < kChunk; ++k) {
+    for (uint32_t k = 0; k < per_lane; ++k) {
       const uint32_t j = base + k * 64u + lane_id();
       const bool valid = j < count;
       const uint32_t p = valid ? list[j] : 0u;
