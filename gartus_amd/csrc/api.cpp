@@ -808,7 +808,11 @@ int run_jobs(gt_context* ctx, std::vector<FrameJob>& jobs, bool multi, int out_i
       const uint32_t* c = ctx->host_ctr + host_words * b;
       std::vector<uint64_t> tally(tally_words);
       std::memcpy(tally.data(), c + ctr_words, tally_words * sizeof(uint64_t));
-      stats->accepted_samples += tally[0];
+      if (!adaptive) {  // every sample of the batch but the dropped ones (adaptive: counted at the end)
+        uint64_t paths = 0;
+        for (auto& sg : batches[b]) paths += uint64_t(sg.nsamples) * jobs[multi ? sg.frame : 0].f.nact;
+        stats->accepted_samples += paths - tally[0];
+      }
       for (uint32_t bounce = 0; bounce < batch_depth[b]; ++bounce) {
         for (uint32_t sl = 0; sl < gtk::kSlices; ++sl) {
           stats->extension_rays += c[gtk::ctr_at(bounce, gtk::S_QCOUNT, sl)];
@@ -934,6 +938,14 @@ int render_adaptive(gt_context* ctx, FrameJob job, const gt_camera* cam, int out
   }
   if (job.d_out)
     gtk::adapt_finalize(L, ctx->ad_mean.p, first_map, first_map ? first_n : npix, job.d_out, out_is_double);
+  if (stats) {  // the samples the pixels used
+    GT_HIP(hipMemsetAsync(ctx->tally.p, 0, sizeof(unsigned long long), st));
+    gtk::adapt_count(L, ctx->ad_mean.p, first_map, first_map ? first_n : npix, ctx->tally.p);
+    unsigned long long used = 0;
+    GT_HIP(hipMemcpyAsync(&used, ctx->tally.p, sizeof used, hipMemcpyDeviceToHost, st));
+    GT_HIP(hipStreamSynchronize(st));
+    stats->accepted_samples = used;
+  }
   GT_HIP(hipGetLastError());
   return GT_OK;
 }

@@ -36,7 +36,8 @@ constexpr uint32_t kBounceWords = kSlots * kSlices * kLineWords;
 __host__ __device__ constexpr uint32_t ctr_at(uint32_t bounce, uint32_t slot, uint32_t slice) {
   return ((bounce * kSlots + slot) * kSlices + slice) * kLineWords;
 }
-// 64-bit tallies: [0] finite samples accumulated in the batch; with counting,
+// 64-bit tallies: [0] the batch's non-finite (dropped) samples (not counted
+// in adaptive rounds); with counting,
 // [1 + 4 b + k]: k = 0 extension node fetches, 1 extension prim tests,
 // 2 shadow node fetches, 3 shadow prim tests at bounce b.
 inline size_t tally_words(uint32_t depth) { return 1 + 4 * size_t(depth); }
@@ -140,6 +141,8 @@ void adapt(const Launch& l, const Work& w, const Queues& q, const Adapt& a);
 // appended to out in any order; *count must be 0 on entry.
 void adapt_compact(const Launch& l, const uint32_t* map, uint32_t n, const uint32_t* done, uint32_t* out,
                    uint32_t* count);
+// *out += the accepted samples of the n pixels of map (0..n-1 without one).
+void adapt_count(const Launch& l, const double4* mean, const uint32_t* map, uint32_t n, unsigned long long* out);
 // out[pixel] = the running mean (float or double rgb) for the n pixels of
 // map (pixels 0..n-1 without one).
 void adapt_finalize(const Launch& l, const double4* mean, const uint32_t* map, uint32_t n, void* out,

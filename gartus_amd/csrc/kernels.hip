@@ -20,6 +20,8 @@
 // NEE, next bounce), so a path's result does not depend on queue order.
 #include "kernels.h"
 
+#include <algorithm>
+
 #include "../../include/gartus_hip.h"
 
 using namespace gtd;
@@ -568,7 +570,7 @@ template <bool MULTI>
 __global__ void k_accumulate(Frame f0, Batch bt, Queues q) {
   const uint32_t stride = gridDim.x * blockDim.x;
   const uint32_t rounds = (bt.npix_total + stride - 1) / stride;  // wave-uniform trip count
-  uint32_t accepted = 0;
+  uint32_t dropped = 0;  // non-finite samples, counted (rare) rather than the accepted ones (every sample)
   for (uint32_t r = 0; r < rounds; ++r) {
     const uint32_t w = r * stride + blockIdx.x * blockDim.x + threadIdx.x;
     with_frame<MULTI, &Seg::pix0>(f0, bt, w < bt.npix_total, w, [&](const Frame& f, const Seg& sg) {
@@ -579,14 +581,16 @@ __global__ void k_accumulate(Frame f0, Batch bt, Queues q) {
         float4 L = q.radiance[sg.path0 + size_t(s) * f.nact + px];
         if (isfinite(L.x) && isfinite(L.y) && isfinite(L.z)) {
           a.x += double(L.x); a.y += double(L.y); a.z += double(L.z); a.w += 1.0;
-          ++accepted;
+        } else {
+          ++dropped;
         }
       }
       f.acc[pixel] = a;
     });
   }
-  for (int off = 32; off > 0; off >>= 1) accepted += __shfl_down(accepted, off);
-  if (lane_id() == 0 && accepted) atomicAdd(q.tally, (unsigned long long)accepted);
+  if (__ballot(dropped != 0u) == 0ull) return;  // the usual case: no atomic at all
+  for (int off = 32; off > 0; off >>= 1) dropped += __shfl_down(dropped, off);
+  if (lane_id() == 0) atomicAdd(q.tally, (unsigned long long)dropped);
 }
 
 // adaptive_error (camera.rs:2972-2982): the largest channel's standard error
@@ -607,7 +611,6 @@ __device__ __forceinline__ double adaptive_error(double4 m2, uint32_t n) {
 __global__ void k_adapt(Frame f, Batch bt, Queues q, Adapt ad) {
 #pragma clang fp contract(off)
   const Seg& sg = bt.seg0;
-  uint32_t accepted = 0;
   const uint32_t stride = gridDim.x * blockDim.x;
   const uint32_t rounds = (f.nact + stride - 1) / stride;  // wave-uniform trip count
   for (uint32_t r = 0; r < rounds; ++r) {
@@ -622,7 +625,6 @@ __global__ void k_adapt(Frame f, Batch bt, Queues q, Adapt ad) {
       const float4 L = q.radiance[sg.path0 + size_t(s) * f.nact + w];
       if (!(isfinite(L.x) && isfinite(L.y) && isfinite(L.z))) continue;
       ++n;
-      ++accepted;
       const double inv = 1.0 / double(n);  // delta / count as LinearRgb / f64 (colors.rs:185-190)
       const double dx = double(L.x) - mean.x, dy = double(L.y) - mean.y, dz = double(L.z) - mean.z;
       mean.x += inv * dx;
@@ -638,8 +640,16 @@ __global__ void k_adapt(Frame f, Batch bt, Queues q, Adapt ad) {
     ad.m2[pixel] = m2;
     if (stop) ad.done[pixel] = 1u;
   }
-  for (int off = 32; off > 0; off >>= 1) accepted += __shfl_down(accepted, off);
-  if (lane_id() == 0 && accepted) atomicAdd(q.tally, (unsigned long long)accepted);
+}
+
+// Samples the adaptive update used, summed over the pixels of map (or
+// 0..n-1): the accepted count each pixel's running mean carries.
+__global__ void k_adapt_count(const double4* mean, const uint32_t* map, uint32_t n, unsigned long long* out) {
+  uint32_t c = 0;
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
+    c += uint32_t(mean[map ? map[i] : i].w);
+  for (int off = 32; off > 0; off >>= 1) c += __shfl_down(c, off);
+  if (lane_id() == 0 && c) atomicAdd(out, (unsigned long long)c);
 }
 
 __global__ void k_adapt_compact(const uint32_t* map, uint32_t n, const uint32_t* done, uint32_t* out,
@@ -843,17 +853,28 @@ void connect(const Launch& l, const Work& w, const Queues& q, uint32_t bounce, u
   }
 #undef GT_CONNECT
 }
+// One thread per pixel (not the persistent grid): each pixel's loop over its
+// samples is a chain of dependent loads, so more threads hide more of it.
+static inline unsigned pixel_grid(uint32_t items, uint32_t block) {
+  return std::max(1u, (items + block - 1) / block);
+}
+
 void accumulate(const Launch& l, const Work& w, const Queues& q) {
-  if (w.multi) hipLaunchKernelGGL(k_accumulate<true>, dim3(l.grid), dim3(l.block), 0, l.stream, w.f, w.bt, q);
-  else hipLaunchKernelGGL(k_accumulate<false>, dim3(l.grid), dim3(l.block), 0, l.stream, w.f, w.bt, q);
+  const unsigned g = pixel_grid(w.bt.npix_total, l.block);
+  if (w.multi) hipLaunchKernelGGL(k_accumulate<true>, dim3(g), dim3(l.block), 0, l.stream, w.f, w.bt, q);
+  else hipLaunchKernelGGL(k_accumulate<false>, dim3(g), dim3(l.block), 0, l.stream, w.f, w.bt, q);
 }
 void adapt(const Launch& l, const Work& w, const Queues& q, const Adapt& a) {
-  hipLaunchKernelGGL(k_adapt, dim3(l.grid), dim3(l.block), 0, l.stream, w.f, w.bt, q, a);
+  hipLaunchKernelGGL(k_adapt, dim3(pixel_grid(w.f.nact, l.block)), dim3(l.block), 0, l.stream, w.f, w.bt, q, a);
 }
 
 void adapt_compact(const Launch& l, const uint32_t* map, uint32_t n, const uint32_t* done, uint32_t* out,
                    uint32_t* count) {
   hipLaunchKernelGGL(k_adapt_compact, dim3(l.grid), dim3(l.block), 0, l.stream, map, n, done, out, count);
+}
+
+void adapt_count(const Launch& l, const double4* mean, const uint32_t* map, uint32_t n, unsigned long long* out) {
+  hipLaunchKernelGGL(k_adapt_count, dim3(l.grid), dim3(l.block), 0, l.stream, mean, map, n, out);
 }
 
 void adapt_finalize(const Launch& l, const double4* mean, const uint32_t* map, uint32_t n, void* out,
