@@ -473,7 +473,7 @@ int ensure_queues(gt_context* ctx, size_t paths, uint32_t max_depth) {
     }
     GT_HIP(ctx->radiance.ensure(paths));
     GT_HIP(ctx->hits.ensure(paths));
-    for (int c = 0; c < gtk::kNumCategories; ++c) GT_HIP(ctx->cat[c].ensure(lists));
+    for (int c = gtk::CAT_EMIT; c < gtk::kNumCategories; ++c) GT_HIP(ctx->cat[c].ensure(lists));  // no miss list
     ctx->cap_paths = paths;
     ctx->region = uint32_t(region);
   }

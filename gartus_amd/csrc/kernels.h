@@ -9,6 +9,8 @@ namespace gtk {
 
 // Shading categories: extend bins every hit by the BSDF it needs, so each
 // shade kernel runs one BSDF without divergence across material types.
+// Misses are shaded by extend itself; CAT_MISS keeps its counter slot but has
+// no list and no kernel.
 enum { CAT_MISS = 0, CAT_EMIT = 1, CAT_LAMBERT = 2, CAT_GGX = 3, CAT_SPECULAR = 4, kNumCategories = 5 };
 
 // Work is split into kSlices slices, one per XCD: the dispatcher hands

@@ -30,7 +30,6 @@ namespace gtk {
 
 namespace {
 
-constexpr uint32_t kMiss = 0xffffffffu;
 constexpr uint32_t kChunk = 4;  // items per lane per cursor fetch (256 per wave)
 
 __device__ __forceinline__ uint32_t lane_id() { return __lane_id(); }
@@ -221,7 +220,7 @@ __global__ void k_raygen(Frame f0, Batch bt, Queues q) {
 }
 
 // Closest hit of every active path; each hit is binned by the shading
-// category of its material (misses in CAT_MISS).
+// category of its material; a miss is shaded on the spot.
 __device__ __forceinline__ uint32_t category_of(uint32_t meta) {
   switch ((meta >> GT_MATKIND_SHIFT) & 7u) {
     case GM_DIFFUSE_LIGHT: return CAT_EMIT;
@@ -266,13 +265,23 @@ __global__ __attribute__((amdgpu_waves_per_eu(8))) void k_extend(Frame f0, Batch
         Stats s{0, 0, 0, 0, 0, 0};
         bool hit = trace<false, COUNT>(f.scene, mk(o4.x, o4.y, o4.z), mk(d4.x, d4.y, d4.z), o4.w, SHADOW_ACNE, INF,
                                        st, &t, &prim, nullptr, &s, &meta);
-        q.hits[p] = hit ? make_float2(t, __uint_as_float(prim)) : make_float2(0.f, __uint_as_float(kMiss));
-        cats[k] = hit ? category_of(meta) : uint32_t(CAT_MISS);
+        if (hit) {
+          q.hits[p] = make_float2(t, __uint_as_float(prim));
+          cats[k] = category_of(meta);
+        } else {
+          // a miss ends the path here (camera.rs:1747-1750): background or environment, times the
+          // MIS weight; shading it in place saves a queue and a launch per bounce
+          const float4 t4 = q.thr[p];
+          const f3 d = mk(d4.x, d4.y, d4.z);
+          const f3 miss = (f.env.kind != 0 ? env_radiance(f.env, d) : background(f.cam, d)) * t4.w;
+          const float4 L = q.radiance[p];
+          q.radiance[p] = make_float4(L.x + t4.x * miss.x, L.y + t4.y * miss.y, L.z + t4.z * miss.z, 0.f);
+        }
         if (COUNT) { nodes += node_fetches(s, f.scene.width); prims += s.candidates; }
       });
     }
 #pragma unroll
-    for (uint32_t c = 0; c < kNumCategories; ++c) {
+    for (uint32_t c = CAT_EMIT; c < kNumCategories; ++c) {  // misses were shaded above
       uint32_t bits = 0;
 #pragma unroll
       for (uint32_t k = 0; k < kChunk; ++k) bits |= uint32_t(cats[k] == c) << k;
@@ -450,14 +459,6 @@ __global__ void k_shade(Frame f0, Batch bt, Queues q, uint32_t bounce) {
       ids[k] = p;
       outs[k] = 0;
       with_frame<MULTI>(f0, bt, valid, p, [&](const Frame& f, const Seg& sg) {
-        if (CAT == CAT_MISS) {  // camera.rs:1747-1750: background or environment, times the MIS weight
-          const float4 d4 = q.ray_d[p], t4 = q.thr[p];
-          const f3 d = mk(d4.x, d4.y, d4.z);
-          const f3 miss = (f.env.kind != 0 ? env_radiance(f.env, d) : background(f.cam, d)) * t4.w;
-          const float4 L = q.radiance[p];
-          q.radiance[p] = make_float4(L.x + t4.x * miss.x, L.y + t4.y * miss.y, L.z + t4.z * miss.z, 0.f);
-          return;
-        }
         const Item it = load_item(q, p);
         f3 pt = it.o + it.t * it.d;
         uint32_t mid, kind;
@@ -832,7 +833,6 @@ void extend(const Launch& l, const Work& w, const Queues& q, uint32_t bounce, bo
 }
 template <bool M>
 static void shade_all(const Launch& l, const Work& w, const Queues& q, uint32_t bounce) {
-  hipLaunchKernelGGL((k_shade<CAT_MISS, M>), dim3(l.grid), dim3(l.block), 0, l.stream, w.f, w.bt, q, bounce);
   hipLaunchKernelGGL((k_shade<CAT_EMIT, M>), dim3(l.grid), dim3(l.block), 0, l.stream, w.f, w.bt, q, bounce);
   hipLaunchKernelGGL((k_shade<CAT_LAMBERT, M>), dim3(l.grid), dim3(l.block), 0, l.stream, w.f, w.bt, q, bounce);
   hipLaunchKernelGGL((k_shade<CAT_GGX, M>), dim3(l.grid), dim3(l.block), 0, l.stream, w.f, w.bt, q, bounce);
