@@ -197,24 +197,51 @@ __global__ __launch_bounds__(256) void k_conv(const uint16_t* __restrict__ x, co
       conv_chunk<NT, DGRAD, 1>(halo, wts, lane, wave, acc);
     }
   }
-  const int row = h0 + wave;
-  if (row >= H) return;
+  // Epilogue through LDS: each wave writes its 64 pixels x NT*16 channels
+  // (bias and ReLU applied) as fp32 rows, then stores whole 16-byte pieces of
+  // consecutive channels, instead of 2-byte values scattered over 4 pixels.
+  constexpr int OS = NT * 16 + 4;  // fp32 row stride: the 4 lane groups' rows land in distinct banks
+  static_assert(4 * 64 * OS * sizeof(float) <= conv_lds_bytes<NT>(), "output staging must fit the LDS tile");
+  __syncthreads();  // every wave is done with the staged halo and weights
+  float* stage = reinterpret_cast<float*>(lds4) + wave * 64 * OS;
 #pragma unroll
   for (int j = 0; j < NT; ++j) {
-    const int co = co0 + j * 16 + (lane & 15);
-    const float b = (!DGRAD && bias) ? bias[co] : 0.f;
+    const int cl = j * 16 + (lane & 15);
+    const float b = (!DGRAD && bias) ? bias[co0 + cl] : 0.f;
 #pragma unroll
     for (int f = 0; f < 4; ++f)
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
-        const int colw = w0 + f * 16 + (lane >> 4) * 4 + r;
-        if (colw >= W) continue;
         float v = acc[f][j][r] + b;
         if (relu) v = fmaxf(v, 0.f);
-        const size_t o = ((size_t(n) * H + row) * W + colw) * cout + co;
-        if (OUT_F32) static_cast<float*>(y)[o] = v;
-        else static_cast<uint16_t*>(y)[o] = f2bf(v);
+        stage[(f * 16 + (lane >> 4) * 4 + r) * OS + cl] = v;
       }
+  }
+  __syncthreads();
+  const int row = h0 + wave;
+  if (row >= H) return;
+  const size_t pix0 = (size_t(n) * H + row) * W;
+  if (OUT_F32) {
+    constexpr int G = NT * 4;  // 4-channel pieces per pixel
+    for (int i = lane; i < 64 * G; i += 64) {
+      const int p = i / G, g = i % G;
+      if (w0 + p >= W) continue;
+      const float4 v = *reinterpret_cast<const float4*>(stage + p * OS + 4 * g);
+      *reinterpret_cast<float4*>(static_cast<float*>(y) + (pix0 + w0 + p) * cout + co0 + 4 * g) = v;
+    }
+  } else {
+    constexpr int G = NT * 2;  // 8-channel pieces per pixel
+    for (int i = lane; i < 64 * G; i += 64) {
+      const int p = i / G, g = i % G;
+      if (w0 + p >= W) continue;
+      const float4 a = *reinterpret_cast<const float4*>(stage + p * OS + 8 * g);
+      const float4 b = *reinterpret_cast<const float4*>(stage + p * OS + 8 * g + 4);
+      const uint4 o = make_uint4(uint32_t(f2bf(a.x)) | (uint32_t(f2bf(a.y)) << 16),
+                                 uint32_t(f2bf(a.z)) | (uint32_t(f2bf(a.w)) << 16),
+                                 uint32_t(f2bf(b.x)) | (uint32_t(f2bf(b.y)) << 16),
+                                 uint32_t(f2bf(b.z)) | (uint32_t(f2bf(b.w)) << 16));
+      *reinterpret_cast<uint4*>(static_cast<uint16_t*>(y) + (pix0 + w0 + p) * cout + co0 + 8 * g) = o;
+    }
   }
 }
 
