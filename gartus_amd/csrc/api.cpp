@@ -129,8 +129,7 @@ struct gt_env {
   double color[3] = {0, 0, 0};
   uint32_t w = 0, h = 0;
   bool uniform = true;
-  DevBuf<float4> tex;
-  DevBuf<float> pdf;
+  DevBuf<float4> tex;  // (r, g, b, solid-angle pdf) per texel
   DevBuf<uint2> alias;  // (keep threshold, alias texel) per texel
 };
 
@@ -524,7 +523,7 @@ int make_frame(gt_context* ctx, const gt_scene* scene, const gt_camera* camera, 
     for (int i = 0; i < 3; ++i) f->env.color[i] = float(env->color[i]);
     f->env.w = env->w; f->env.h = env->h;
     f->env.uniform = env->uniform ? 1u : 0u;
-    f->env.tex = env->tex.p; f->env.pdf = env->pdf.p; f->env.alias = env->alias.p;
+    f->env.tex = env->tex.p; f->env.alias = env->alias.p;
   }
   f->nee = camera->direct_lighting_mode == GT_LIGHTING_NEE ? 1u : 0u;
   f->has_lights = lights ? 1u : 0u;
@@ -1389,13 +1388,12 @@ int gt_env_create_rgb32f(gt_context* ctx, const float* rgb, uint32_t w, uint32_t
     }
   }
   e->uniform = total <= kEps;
-  std::vector<float> pdf(n);
   for (uint32_t y = 0; y < h; ++y) {
     double t0 = pi * (double(y) / double(h)), t1 = pi * (double(y + 1) / double(h));
     double sa = (2.0 * pi / double(w)) * std::fmax(std::cos(t0) - std::cos(t1), 0.0);  // :186-190
     for (uint32_t x = 0; x < w; ++x) {
       size_t i = size_t(y) * w + x;
-      pdf[i] = (e->uniform || sa <= kEps) ? 0.0f : float((weight[i] / total) / sa);
+      tex[i].w = (e->uniform || sa <= kEps) ? 0.0f : float((weight[i] / total) / sa);
     }
   }
   std::vector<uint2> alias(n);
@@ -1405,9 +1403,8 @@ int gt_env_create_rgb32f(gt_context* ctx, const float* rgb, uint32_t w, uint32_t
     for (size_t i = 0; i < n; ++i) alias[i] = make_uint2(thr[i], idx[i]);
   }
   GT_HIP(hipSetDevice(ctx->device));
-  GT_HIP(e->tex.ensure(n)); GT_HIP(e->pdf.ensure(n)); GT_HIP(e->alias.ensure(n));
+  GT_HIP(e->tex.ensure(n)); GT_HIP(e->alias.ensure(n));
   GT_HIP(hipMemcpy(e->tex.p, tex.data(), n * sizeof(float4), hipMemcpyHostToDevice));
-  GT_HIP(hipMemcpy(e->pdf.p, pdf.data(), n * sizeof(float), hipMemcpyHostToDevice));
   GT_HIP(hipMemcpy(e->alias.p, alias.data(), n * sizeof(uint2), hipMemcpyHostToDevice));
   *out = e.release();
   return GT_OK;

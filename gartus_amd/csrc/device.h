@@ -656,8 +656,10 @@ struct DEnv {
   uint32_t w, h;
   uint32_t uniform;     // total weight <= EPS: uniform sphere sampling
   float color[3];
-  const float4* __restrict__ tex;      // linear RGB texels, row-major, row 0 at +y
-  const float* __restrict__ pdf;       // solid-angle pdf per texel (:136-156)
+  // Linear RGB texels, row-major, row 0 at +y; .w holds the texel's
+  // solid-angle pdf (:136-156), so a pdf lookup shares the cache line of the
+  // radiance lookup around the same direction.
+  const float4* __restrict__ tex;
   // Alias table over the texels (Vose's method): slot i keeps texel i when
   // the low 32 bits of its draw are below .x and otherwise yields texel .y.
   // It samples the reference's texel distribution (rebuild_distribution,
@@ -686,7 +688,7 @@ __device__ __forceinline__ float env_pdf(const DEnv& e, f3 dir) {  // :136-156
   float fx = floorf((phi / TAU) * float(e.w)), fy = floorf((theta / PI) * float(e.h));
   uint32_t x = fx > 0.f ? min(uint32_t(fx), e.w - 1) : 0u;
   uint32_t y = fy > 0.f ? min(uint32_t(fy), e.h - 1) : 0u;
-  return e.pdf[size_t(y) * e.w + x];
+  return e.tex[size_t(y) * e.w + x].w;
 }
 // The sampled texel's radiance and pdf. The reference evaluates
 // pdf_value(direction) and radiance(direction) from the returned direction
@@ -739,7 +741,7 @@ __device__ __forceinline__ EnvSample env_sample(const DEnv& e, Rng& rng) {  // :
   float s, c;
   sincosf(phi, &s, &c);
   r.dir = mk(st * c, ct, st * s);
-  r.pdf = e.pdf[size_t(y) * e.w + x];
+  r.pdf = e.tex[size_t(y) * e.w + x].w;
   r.u = u;
   r.cos_theta = ct;
   r.texel = true;

@@ -273,7 +273,8 @@ __global__ __attribute__((amdgpu_waves_per_eu(8))) void k_extend(Frame f0, Batch
           // MIS weight; shading it in place saves a queue and a launch per bounce
           const float4 t4 = q.thr[p];
           const f3 d = mk(d4.x, d4.y, d4.z);
-          const f3 miss = (f.env.kind != 0 ? env_radiance(f.env, d) : background(f.cam, d)) * t4.w;
+          const float mw = t4.w < 0.f ? power_heuristic(-t4.w, env_pdf(f.env, d)) : t4.w;
+          const f3 miss = (f.env.kind != 0 ? env_radiance(f.env, d) : background(f.cam, d)) * mw;
           const float4 L = q.radiance[p];
           q.radiance[p] = make_float4(L.x + t4.x * miss.x, L.y + t4.y * miss.y, L.z + t4.z * miss.z, 0.f);
         }
@@ -405,7 +406,10 @@ __device__ __forceinline__ uint32_t shade_scatter(const Frame& f, const Queues& 
   if (!(isfinite(cpdf) && cpdf > EPS)) return out;
   float spdf = scattering_pdf(m, it.d, n, dir);
   if (!(isfinite(spdf) && spdf > 0.f)) return out;
-  float next_w = weight_env ? power_heuristic(cpdf, env_pdf(f.env, dir)) : 1.0f;
+  // The environment's MIS weight power_heuristic(cpdf, env_pdf(dir)) applies
+  // only if the continuation misses: it is stored as -cpdf (cpdf > EPS here)
+  // and evaluated by the miss shading in extend.
+  float next_w = weight_env ? -cpdf : 1.0f;
   T = mul(T, color * (spdf / cpdf));
   if (rr_survives(bounce, f.cam.rr_min_depth, &T, rng))
     out |= continue_path(q, it.path, p, it.time, dir, suppress, T, next_w);
