@@ -727,8 +727,8 @@ int run_jobs(gt_context* ctx, std::vector<FrameJob>& jobs, bool multi, int out_i
       }
       seg_off += batches[b].size();
       const uint32_t depth = batch_depth[b];
-      gtk::reset_counters(L, q, depth_all);
-      gtk::raygen(L, w, q);
+      gtk::reset_counters(L, q, depth_all, w.bt.total);
+      if (depth == 0) gtk::extend(L, w, q, 0, false);  // no bounce: every path is black, nothing is traced
       for (uint32_t bounce = 0; bounce < depth; ++bounce) {
         if (want_stats) GT_HIP(hipEventRecord(ctx->events[ev++], st));
         gtk::extend(L, w, q, bounce, count);

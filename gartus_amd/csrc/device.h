@@ -666,6 +666,12 @@ struct DEnv {
   // :158-184) with one draw and two dependent loads instead of a search.
   const uint2* __restrict__ alias;
 };
+__device__ __forceinline__ float power_heuristic(float a, float b) {  // camera.rs:2486-2496
+  if (!isfinite(a) || a <= EPS) return 0.f;
+  if (!isfinite(b) || b <= EPS) return 1.0f;
+  float s = a * a, o = b * b;
+  return s / (s + o);
+}
 __device__ __forceinline__ float rem_euclid(float x, float m) {
   float r = fmodf(x, m);
   return r < 0.f ? r + fabsf(m) : r;
@@ -689,6 +695,32 @@ __device__ __forceinline__ float env_pdf(const DEnv& e, f3 dir) {  // :136-156
   uint32_t x = fx > 0.f ? min(uint32_t(fx), e.w - 1) : 0u;
   uint32_t y = fy > 0.f ? min(uint32_t(fy), e.h - 1) : 0u;
   return e.tex[size_t(y) * e.w + x].w;
+}
+// env_radiance(dir) for a path that missed. On entry *mw is the path's MIS
+// weight, or -cpdf when the weight is the environment's power heuristic
+// power_heuristic(cpdf, env_pdf(dir)); on return it is the weight. The
+// direction maps to lat-long coordinates once for both lookups.
+__device__ __forceinline__ f3 env_radiance_weighted(const DEnv& e, f3 dir, float* mw) {
+  if (e.kind != 2) {
+    if (*mw < 0.f) *mw = power_heuristic(-*mw, 1.0f / (4.0f * PI));
+    return ld3(e.color);
+  }
+  f3 d = normalized(dir);
+  float phi = rem_euclid(atan2f(d.z, d.x), TAU);
+  float theta = acosf(clampf(d.y, -1.f, 1.f));
+  if (*mw < 0.f) {  // env_pdf
+    float pdf = 1.0f / (4.0f * PI);
+    if (!e.uniform) {
+      float fx = floorf((phi / TAU) * float(e.w)), fy = floorf((theta / PI) * float(e.h));
+      uint32_t x = fx > 0.f ? min(uint32_t(fx), e.w - 1) : 0u;
+      uint32_t y = fy > 0.f ? min(uint32_t(fy), e.h - 1) : 0u;
+      pdf = e.tex[size_t(y) * e.w + x].w;
+    }
+    *mw = power_heuristic(-*mw, pdf);
+  }
+  float u = phi / TAU, v = 1.0f - theta / PI;
+  if (!isfinite(u) || !isfinite(v)) return mk(0.f, 0.f, 0.f);
+  return env_radiance_uv(e, u, v);
 }
 // The sampled texel's radiance and pdf. The reference evaluates
 // pdf_value(direction) and radiance(direction) from the returned direction
@@ -754,12 +786,6 @@ __device__ __forceinline__ f3 env_sample_radiance(const DEnv& e, const EnvSample
   return env_radiance_uv(e, s.u, 1.0f - theta / PI);
 }
 
-__device__ __forceinline__ float power_heuristic(float a, float b) {  // camera.rs:2486-2496
-  if (!isfinite(a) || a <= EPS) return 0.f;
-  if (!isfinite(b) || b <= EPS) return 1.0f;
-  float s = a * a, o = b * b;
-  return s / (s + o);
-}
 
 // ---- camera rays (camera.rs:1658-1718) ------------------------------------------
 __device__ __forceinline__ void camera_ray(const GCamera& cam, uint32_t x, uint32_t y, uint32_t s, Rng& rng, f3* o,

@@ -128,9 +128,8 @@ struct Work {
   bool multi;
 };
 
-void reset_counters(const Launch& l, const Queues& q, uint32_t depth);
+void reset_counters(const Launch& l, const Queues& q, uint32_t depth, uint32_t total);
 void zero_acc(const Launch& l, double4* acc, size_t n);
-void raygen(const Launch& l, const Work& w, const Queues& q);
 void extend(const Launch& l, const Work& w, const Queues& q, uint32_t bounce, bool count);
 // One bounce of shading: one kernel per category (empty categories exit at once).
 void shade(const Launch& l, const Work& w, const Queues& q, uint32_t bounce);

@@ -1,8 +1,9 @@
 // kernels.hip — the wavefront path tracer for gfx950 (CDNA4).
 //
 // One batch of paths (a set of samples for every pixel) runs as
-//   raygen -> { extend(b) -> shade(b, per category) -> connect(b, light)
+//   reset -> { extend(b) -> shade(b, per category) -> connect(b, light)
 //   -> connect(b, env) } for b < max_depth -> accumulate
+// where extend(0) generates the camera rays itself
 // over path state in HBM. Every stage is a persistent kernel: a fixed grid
 // whose waves pull chunks of work from a device-side cursor, so the host never
 // reads a queue length and the loop needs no synchronisation.
@@ -156,9 +157,20 @@ __device__ __forceinline__ float scattering_pdf(const GMat& m, f3 d_in, f3 n, f3
 
 // Clears the counters of bounces 0..depth (one word per 128-byte line) and
 // the tallies.
-__global__ void k_reset(uint32_t* ctr, uint32_t ncounters, unsigned long long* tally, uint32_t ntally) {
-  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < ncounters; i += gridDim.x * blockDim.x)
-    ctr[size_t(i) * kLineWords] = 0;
+// Clears the counters, except that slice s of bounce 0 queues the camera
+// paths [total * s / kSlices, total * (s + 1) / kSlices).
+__global__ void k_reset(uint32_t* ctr, uint32_t ncounters, unsigned long long* tally, uint32_t ntally,
+                        uint32_t total) {
+  static_assert(S_QCOUNT == 0 && S_BASE == 1, "bounce 0 starts with the queue count and base of each slice");
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < ncounters; i += gridDim.x * blockDim.x) {
+    uint32_t v = 0;
+    if (i < 2 * kSlices) {  // ctr_at(0, S_QCOUNT or S_BASE, s)
+      const uint32_t s = i % kSlices;
+      const uint32_t lo = uint32_t(uint64_t(total) * s / kSlices), hi = uint32_t(uint64_t(total) * (s + 1) / kSlices);
+      v = i / kSlices == S_BASE ? lo : hi - lo;
+    }
+    ctr[size_t(i) * kLineWords] = v;
+  }
   for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < ntally; i += gridDim.x * blockDim.x) tally[i] = 0;
 }
 
@@ -191,36 +203,11 @@ __device__ __forceinline__ uint32_t node_fetches(const Stats& s, uint32_t width)
 
 // Camera rays for samples [sample0, sample0 + nsamples) of every pixel
 // (render_world_linear_pixel_with_context camera.rs:2660-2723; rays :1658-1718).
-template <bool MULTI>
-__global__ void k_raygen(Frame f0, Batch bt, Queues q) {
-  const uint32_t total = bt.total;
-  if (blockIdx.x == 0 && threadIdx.x < kSlices) {  // slice s starts extend(0) at path total * s / kSlices
-    const uint32_t s = threadIdx.x;
-    const uint32_t lo = uint32_t(uint64_t(total) * s / kSlices), hi = uint32_t(uint64_t(total) * (s + 1) / kSlices);
-    q.ctr[ctr_at(0, S_BASE, s)] = lo;
-    q.ctr[ctr_at(0, S_QCOUNT, s)] = hi - lo;
-  }
-  const uint32_t stride = gridDim.x * blockDim.x;
-  const uint32_t rounds = (total + stride - 1) / stride;  // wave-uniform trip count
-  for (uint32_t r = 0; r < rounds; ++r) {
-    const uint32_t p = r * stride + blockIdx.x * blockDim.x + threadIdx.x;
-    with_frame<MULTI>(f0, bt, p < total, p, [&](const Frame& f, const Seg& sg) {
-      uint32_t x, y, sample;
-      path_coords(f, sg, p, &x, &y, &sample);
-      Rng rng{stream_seed(pixel_seed(f.cam.seed, x, y), sample, 0)};
-      f3 o, d;
-      float time;
-      camera_ray(f.cam, x, y, sample, rng, &o, &d, &time);
-      q.ray_o[p] = make_float4(o.x, o.y, o.z, time);
-      q.ray_d[p] = make_float4(d.x, d.y, d.z, 0.f);
-      q.thr[p] = make_float4(1.f, 1.f, 1.f, 1.f);
-      q.radiance[p] = make_float4(0.f, 0.f, 0.f, 0.f);
-    });
-  }
-}
-
 // Closest hit of every active path; each hit is binned by the shading
-// category of its material; a miss is shaded on the spot.
+// category of its material; a miss is shaded on the spot. At bounce 0
+// (CAMERA) the kernel generates the camera rays itself (camera.rs:1658-1718)
+// and stores a path's state only if it hit something: a path that misses
+// ends here with the background as its radiance.
 __device__ __forceinline__ uint32_t category_of(uint32_t meta) {
   switch ((meta >> GT_MATKIND_SHIFT) & 7u) {
     case GM_DIFFUSE_LIGHT: return CAT_EMIT;
@@ -230,7 +217,7 @@ __device__ __forceinline__ uint32_t category_of(uint32_t meta) {
   }
 }
 
-template <bool COUNT, bool MULTI>
+template <bool COUNT, bool MULTI, bool CAMERA>
 // Traversal kernels run at 8 waves per SIMD (at most 64 VGPRs, a few spilled
 // values): more waves hide more of the dependent node-fetch latency. On one
 // MI355X this took the 720p terrain's extend from 14.6 to 13.8 ms per frame
@@ -258,8 +245,27 @@ __global__ __attribute__((amdgpu_waves_per_eu(8))) void k_extend(Frame f0, Batch
       const bool valid = k < per_lane && j < count;
       const uint32_t p = !valid ? 0u : bounce == 0 ? first + j : active[j];
       ids[k] = p;
-      with_frame<MULTI>(f0, bt, valid, p, [&](const Frame& f, const Seg&) {
-        float4 o4 = q.ray_o[p], d4 = q.ray_d[p];
+      with_frame<MULTI>(f0, bt, valid, p, [&](const Frame& f, const Seg& sg) {
+        float4 o4, d4;
+        if (CAMERA) {
+          if (f.cam.max_depth == 0) {  // ray_color with no bounce is black
+            q.radiance[p] = make_float4(0.f, 0.f, 0.f, 0.f);
+            return;
+          }
+          uint32_t x, y, sample;
+          path_coords(f, sg, p, &x, &y, &sample);
+          Rng rng{stream_seed(pixel_seed(f.cam.seed, x, y), sample, 0)};
+          f3 o, d;
+          float time;
+          camera_ray(f.cam, x, y, sample, rng, &o, &d, &time);
+          o4 = make_float4(o.x, o.y, o.z, time);
+          d4 = make_float4(d.x, d.y, d.z, 0.f);
+          q.ray_o[p] = o4;  // stored before the trace: holding the ray across it costs registers
+          q.ray_d[p] = d4;
+        } else {
+          o4 = q.ray_o[p];
+          d4 = q.ray_d[p];
+        }
         float t;
         uint32_t prim, meta = 0;
         Stats s{0, 0, 0, 0, 0, 0};
@@ -267,15 +273,20 @@ __global__ __attribute__((amdgpu_waves_per_eu(8))) void k_extend(Frame f0, Batch
                                        st, &t, &prim, nullptr, &s, &meta);
         if (hit) {
           q.hits[p] = make_float2(t, __uint_as_float(prim));
+          if (CAMERA) {  // the path's state: throughput 1, MIS weight 1, no radiance yet
+            q.thr[p] = make_float4(1.f, 1.f, 1.f, 1.f);
+            q.radiance[p] = make_float4(0.f, 0.f, 0.f, 0.f);
+          }
           cats[k] = category_of(meta);
         } else {
           // a miss ends the path here (camera.rs:1747-1750): background or environment, times the
           // MIS weight; shading it in place saves a queue and a launch per bounce
-          const float4 t4 = q.thr[p];
+          const float4 t4 = CAMERA ? make_float4(1.f, 1.f, 1.f, 1.f) : q.thr[p];
           const f3 d = mk(d4.x, d4.y, d4.z);
-          const float mw = t4.w < 0.f ? power_heuristic(-t4.w, env_pdf(f.env, d)) : t4.w;
-          const f3 miss = (f.env.kind != 0 ? env_radiance(f.env, d) : background(f.cam, d)) * mw;
-          const float4 L = q.radiance[p];
+          float mw = t4.w;
+          const f3 rad = f.env.kind != 0 ? env_radiance_weighted(f.env, d, &mw) : background(f.cam, d);
+          const f3 miss = rad * mw;
+          const float4 L = CAMERA ? make_float4(0.f, 0.f, 0.f, 0.f) : q.radiance[p];
           q.radiance[p] = make_float4(L.x + t4.x * miss.x, L.y + t4.y * miss.y, L.z + t4.z * miss.z, 0.f);
         }
         if (COUNT) { nodes += node_fetches(s, f.scene.width); prims += s.candidates; }
@@ -805,34 +816,37 @@ __global__ void k_intersect(Frame f, Queues q, const float4* rays, size_t n, flo
 uint32_t occupancy_blocks(int kernel, uint32_t block, size_t lds) {
   int nb = 0;
   hipError_t e = hipSuccess;
-  if (kernel == 0) e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_extend<false, false>, int(block), lds);
+  if (kernel == 0) e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_extend<false, false, false>, int(block), lds);
   else if (kernel == 1) e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_shade<CAT_GGX, false>, int(block), 0);
   else e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_connect<false, false>, int(block), lds);
   if (e != hipSuccess || nb <= 0) return 1;
   return uint32_t(nb);
 }
 
-void reset_counters(const Launch& l, const Queues& q, uint32_t depth) {
+void reset_counters(const Launch& l, const Queues& q, uint32_t depth, uint32_t total) {
   const uint32_t n = (depth + 1) * kSlots * kSlices;
   hipLaunchKernelGGL(k_reset, dim3((n + 255) / 256), dim3(256), 0, l.stream, q.ctr, n, q.tally,
-                     uint32_t(tally_words(depth)));
+                     uint32_t(tally_words(depth)), total);
 }
 void zero_acc(const Launch& l, double4* acc, size_t n) {
   hipLaunchKernelGGL(k_zero, dim3(l.grid), dim3(l.block), 0, l.stream, acc, n);
 }
-void raygen(const Launch& l, const Work& w, const Queues& q) {
-  if (w.multi) hipLaunchKernelGGL(k_raygen<true>, dim3(l.grid), dim3(l.block), 0, l.stream, w.f, w.bt, q);
-  else hipLaunchKernelGGL(k_raygen<false>, dim3(l.grid), dim3(l.block), 0, l.stream, w.f, w.bt, q);
-}
 void extend(const Launch& l, const Work& w, const Queues& q, uint32_t bounce, bool count) {
   const size_t lds = stack_lds_bytes(l.block, q.stack_cap);
-#define GT_EXTEND(C, M) \
-  hipLaunchKernelGGL((k_extend<C, M>), dim3(l.grid), dim3(l.block), lds, l.stream, w.f, w.bt, q, bounce)
-  if (w.multi) {
-    if (count) GT_EXTEND(true, true); else GT_EXTEND(false, true);
-  } else {
-    if (count) GT_EXTEND(true, false); else GT_EXTEND(false, false);
+#define GT_EXTEND(C, M, K) \
+  hipLaunchKernelGGL((k_extend<C, M, K>), dim3(l.grid), dim3(l.block), lds, l.stream, w.f, w.bt, q, bounce)
+#define GT_EXTEND_CM(K)                                                  \
+  if (w.multi) {                                                         \
+    if (count) GT_EXTEND(true, true, K); else GT_EXTEND(false, true, K); \
+  } else {                                                               \
+    if (count) GT_EXTEND(true, false, K); else GT_EXTEND(false, false, K); \
   }
+  if (bounce == 0) {
+    GT_EXTEND_CM(true)
+  } else {
+    GT_EXTEND_CM(false)
+  }
+#undef GT_EXTEND_CM
 #undef GT_EXTEND
 }
 template <bool M>

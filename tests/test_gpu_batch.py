@@ -91,6 +91,24 @@ def test_batch_split_over_small_queues(gpu_context):
         np.testing.assert_array_equal(a.pixels, b, err_msg=f"frame {i}")
 
 
+def test_frames_without_bounces_are_black(gpu_context):
+    """ray_color runs max_depth bounces (camera.rs:1742): at depth 0 every
+    sample is black and nothing is traced, alone and inside a batch whose
+    other frames bounce."""
+    say("batch: a depth-0 frame alone and beside a cornell frame")
+    data, cam = scenes.bench_scene()
+    cam.update(width=40, spp=3, depth=0)
+    flat = (api.RayScene(data).with_bvh(), _camera(cam), None, None)
+    alone = api.PathTracer(flat[1])._render(*flat).pixels
+    assert alone.shape[1:] == (40, 3)
+    assert not alone.any()
+    frames = [_frames()[0], flat]
+    singles = _singles(frames[:1])
+    images = api.render_batch(frames)
+    np.testing.assert_array_equal(images[0].pixels, singles[0])
+    assert not images[1].pixels.any()
+
+
 def test_batch_rejects_mixed_contexts(gpu_context):
     other = api.Context(0)
     data = scenes.randomized_ray_scene(1)
