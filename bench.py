@@ -135,6 +135,8 @@ def cpu_baseline(frames, seconds: float) -> dict:
             rays += c["extension_rays"] + c["shadow_rays"]
             paths += c["paths"]
         parts.append(f"{W}x{rows + 1}")
+        if i == 0:  # one core on a 64-pixel-wide strip of the same frame
+            one = one_core(scene, cam, lights, env, W, H, min(3.0, 0.25 * seconds))
         del scene
         if spent >= seconds:
             break
@@ -147,7 +149,39 @@ def cpu_baseline(frames, seconds: float) -> dict:
         "kind": "port",
         "sample": f"{what}, full spp, f64 oracle, reference RNG stream; "
                   f"{paths / spent:.4g} samples/s over {spent:.1f} s ({rays} rays)",
+        "cpu_model": cpu_model(),
+        "one_core": one,
     }
+
+
+def one_core(scene, cam, lights, env, W, H, seconds: float) -> dict:
+    """The oracle on one thread: a 64-pixel-wide strip of rows around the
+    middle of the frame, sized from a one-row pilot to about `seconds`."""
+    w = min(64, W)
+    x0, y0 = (W - w) // 2, H // 2
+    t = time.perf_counter()
+    _, c = scene.render(cam, lights, env, threads=1, crop=(x0, y0, w, 1))
+    spent = max(time.perf_counter() - t, 1e-4)
+    rays, paths = c["extension_rays"] + c["shadow_rays"], c["paths"]
+    rows = int(max(0, min(y0, (seconds - spent) / spent)))
+    if rows > 0:
+        t = time.perf_counter()
+        _, c = scene.render(cam, lights, env, threads=1, crop=(x0, y0 - rows, w, rows))
+        spent += time.perf_counter() - t
+        rays += c["extension_rays"] + c["shadow_rays"]
+        paths += c["paths"]
+    return {"value": rays / spent / 1e6, "unit": "Mrays/s", "samples_per_s": paths / spent,
+            "sample": f"frame 0, {w}x{rows + 1} pixels around the middle, {spent:.1f} s"}
+
+
+def cpu_model() -> str:
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
 
 
 def calibrate_copy(mib: int, reps: int) -> int:
@@ -208,13 +242,14 @@ def denoise_summary(ctx) -> dict:
     }
 
 
-def traffic_for(workload: str):
+def traffic_for(workload: str) -> dict:
+    """The extend kernel's measured traffic record for the workload (bytes
+    per launch below L2, L2 hit rate), or an empty dict."""
     try:
         rec = json.loads(TRAFFIC_FILE.read_text())
     except (OSError, ValueError):
-        return None
-    w = rec.get(workload)
-    return None if w is None else w.get("bytes_per_launch")
+        return {}
+    return rec.get(workload) or {}
 
 
 def main() -> int:
@@ -352,7 +387,8 @@ def main() -> int:
     timed = stats if not args.graph else [ref]
     avg_launch_ms = sum(s.extend_ms for s in timed) / sum(s.extend_launches for s in timed)
     achieved = (ext_bytes / launches) / (avg_launch_ms * 1e-3) / 1e9
-    traffic = traffic_for(args.workload)
+    measured = traffic_for(args.workload)
+    traffic = measured.get("bytes_per_launch")
     W0, H0 = frames[0]["wh"]
     out = np.zeros(W0 * H0 * 3, np.float32)
     abi.check(lib.gt_memcpy_to_host(ctx.handle, out.ctypes.data_as(C.c_void_p), outs[0], out.nbytes))
@@ -400,6 +436,8 @@ def main() -> int:
             # over the live launch time: BVH and primitives that fit in L2 are served from there, so
             # this can be far below the algorithmic figure
             "hbm_measured_gbs": None if traffic is None else round(traffic / (avg_launch_ms * 1e-3) / 1e9, 1),
+            # share of the extend kernel's L2 requests that hit (rocprofv3 TCC_HIT_sum, TCC_MISS_sum)
+            "l2_hit_rate": measured.get("l2_hit_rate"),
         },
     }
     if args.workload == "ggx_env_1080p" and not args.no_denoise_eval and DENOISER_WEIGHTS.exists():
