@@ -453,8 +453,9 @@ __device__ __forceinline__ uint32_t shade_specular(const Frame& f, const Queues&
   return 0;
 }
 
+// Shades the hits of one category, all slices.
 template <int CAT, bool MULTI>
-__global__ void k_shade(Frame f0, Batch bt, Queues q, uint32_t bounce) {
+__device__ __forceinline__ void shade_category(const Frame& f0, const Batch& bt, const Queues& q, uint32_t bounce) {
   constexpr bool kScatters = CAT == CAT_LAMBERT || CAT == CAT_GGX || CAT == CAT_SPECULAR;
   for (uint32_t s = blockIdx.x % slice_step(); s < kSlices; s += slice_step()) {
   const uint32_t count = q.ctr[ctr_at(bounce, S_CAT_COUNT + CAT, s)];
@@ -511,6 +512,16 @@ __global__ void k_shade(Frame f0, Batch bt, Queues q, uint32_t bounce) {
     }
   }
   }
+}
+
+// One launch per bounce drains the category queues in turn, so a wave that
+// finds one queue empty moves on to the next without a launch boundary.
+template <bool MULTI>
+__global__ void k_shade(Frame f0, Batch bt, Queues q, uint32_t bounce) {
+  shade_category<CAT_EMIT, MULTI>(f0, bt, q, bounce);
+  shade_category<CAT_LAMBERT, MULTI>(f0, bt, q, bounce);
+  shade_category<CAT_GGX, MULTI>(f0, bt, q, bounce);
+  shade_category<CAT_SPECULAR, MULTI>(f0, bt, q, bounce);
 }
 
 // Shadow rays of one kind. Light targets: closest hit, then the emission of
@@ -817,7 +828,7 @@ uint32_t occupancy_blocks(int kernel, uint32_t block, size_t lds) {
   int nb = 0;
   hipError_t e = hipSuccess;
   if (kernel == 0) e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_extend<false, false, false>, int(block), lds);
-  else if (kernel == 1) e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_shade<CAT_GGX, false>, int(block), 0);
+  else if (kernel == 1) e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_shade<false>, int(block), 0);
   else e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_connect<false, false>, int(block), lds);
   if (e != hipSuccess || nb <= 0) return 1;
   return uint32_t(nb);
@@ -849,16 +860,9 @@ void extend(const Launch& l, const Work& w, const Queues& q, uint32_t bounce, bo
 #undef GT_EXTEND_CM
 #undef GT_EXTEND
 }
-template <bool M>
-static void shade_all(const Launch& l, const Work& w, const Queues& q, uint32_t bounce) {
-  hipLaunchKernelGGL((k_shade<CAT_EMIT, M>), dim3(l.grid), dim3(l.block), 0, l.stream, w.f, w.bt, q, bounce);
-  hipLaunchKernelGGL((k_shade<CAT_LAMBERT, M>), dim3(l.grid), dim3(l.block), 0, l.stream, w.f, w.bt, q, bounce);
-  hipLaunchKernelGGL((k_shade<CAT_GGX, M>), dim3(l.grid), dim3(l.block), 0, l.stream, w.f, w.bt, q, bounce);
-  hipLaunchKernelGGL((k_shade<CAT_SPECULAR, M>), dim3(l.grid), dim3(l.block), 0, l.stream, w.f, w.bt, q, bounce);
-}
 void shade(const Launch& l, const Work& w, const Queues& q, uint32_t bounce) {
-  if (w.multi) shade_all<true>(l, w, q, bounce);
-  else shade_all<false>(l, w, q, bounce);
+  if (w.multi) hipLaunchKernelGGL((k_shade<true>), dim3(l.grid), dim3(l.block), 0, l.stream, w.f, w.bt, q, bounce);
+  else hipLaunchKernelGGL((k_shade<false>), dim3(l.grid), dim3(l.block), 0, l.stream, w.f, w.bt, q, bounce);
 }
 void connect(const Launch& l, const Work& w, const Queues& q, uint32_t bounce, uint32_t kind, bool count) {
   const size_t lds = stack_lds_bytes(l.block, q.stack_cap);
