@@ -21,10 +21,14 @@ LIB = PKG / "libgartus_hip.so"
 ARCH = os.environ.get("GARTUS_OFFLOAD_ARCH", "gfx950")
 
 # -packed-fp32-ops off: no v_pk_* FP32 forms are generated.
+# -ffp-contract=on: a multiply and an add fuse only within one source
+# expression, so a function's results do not depend on the code it is
+# inlined into (a batched and a single render of a frame stay bit-identical).
 HIP_FLAGS = [
     f"--offload-arch={ARCH}",
     "-O3",
     "-std=c++17",
+    "-ffp-contract=on",
     "-fPIC",
     "-Wall",
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
