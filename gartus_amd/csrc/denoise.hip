@@ -19,7 +19,6 @@ namespace gdn {
 
 namespace {
 
-typedef short v4s __attribute__((ext_vector_type(4)));
 typedef float v4f __attribute__((ext_vector_type(4)));
 
 __device__ __forceinline__ float bf2f(uint16_t b) { return __uint_as_float(uint32_t(b) << 16); }
@@ -30,9 +29,6 @@ __device__ __forceinline__ uint16_t f2bf(float f) {  // round to nearest even; N
   return uint16_t(u >> 16);
 }
 
-__device__ __forceinline__ v4f mfma(v4s a, v4s b, v4f c) {
-  return __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, c, 0, 0, 0);
-}
 
 // ---- convolution, forward and data gradient --------------------------------
 // A block owns a tile of kTR image rows x kTC columns of one image and NT*16
@@ -245,99 +241,185 @@ __global__ __launch_bounds__(256) void k_conv(const uint16_t* __restrict__ x, co
   }
 }
 
-// ---- convolution, weight gradient ------------------------------------------
-// Block = one wave. blockIdx.x = (tap, co group, ci group), blockIdx.y =
-// pixel split. The wave sums its split's pixels 16 at a time into
-// NCO x NCI fp32 tiles and writes them to partial[split]. The waves of the
-// centre tap and the first ci group also multiply their dPre fragments by an
-// all-ones B operand: column 0 of that product is the bias gradient
-// sum_p dPre[p][co], written to bpartial[split].
+// ---- convolution, weight gradient: LDS tiles and 16x16x32 MFMA -------------
+// dW[co][tap][ci] = sum over pixels p of dPre[p][co] * x[p + off(tap)][ci].
+// A block owns a band of kWR image rows of one image (its split) and NCO x
+// NCI 16-channel fragments of (co, ci), and walks the band in tiles of kWC
+// columns. Per tile it stages, transposed so that consecutive pixels of one
+// channel are adjacent, dPre = dy masked by the ReLU of the forward output
+// (kWR rows x kWC columns) and x ((kWR + 2) rows x kWX columns: 8 beyond the
+// tile on each side, so every 16-byte read stays aligned). Wave w takes row w:
+// per K step of 32 pixels, lane l reads 8 consecutive pixels of channel l % 16
+// (A from dPre, B from x at the tap's row), and the column shifts of the
+// three taps of that row come from its 16-byte neighbours by byte alignment.
+// The four waves' sums are added in wave order through LDS, and the centre
+// tap's waves of the first ci group also sum dPre against ones (the bias).
+constexpr int kWR = 4, kWC = 64, kWX = kWC + 16;
+constexpr int kSD = kWR * kWC + 8;        // dPre channel stride (elements): an odd number of 16-byte chunks
+constexpr int kSX = (kWR + 2) * kWX + 8;  // x channel stride, likewise
+
+constexpr size_t kWRed = 3 * 36 * 64 * sizeof(float);  // waves 1..3 hand wave 0 one (a, b) pair of sums
+
 template <int NCO, int NCI>
-__global__ __launch_bounds__(64) void k_wgrad(const uint16_t* __restrict__ x, int cin,
-                                              const uint16_t* __restrict__ dy, const uint16_t* __restrict__ mask,
-                                              int cout, int H, int W, int P, int split_px, int ncog,
-                                              float* __restrict__ partial, float* __restrict__ bpartial) {
-  const int lane = threadIdx.x;
-  const int tap = blockIdx.x % 9;
-  const int g = blockIdx.x / 9;
-  const int co0 = (g % ncog) * NCO * 16, ci0 = (g / ncog) * NCI * 16;
-  const int dyo = tap / 3 - 1, dxo = tap % 3 - 1;
-  const int pbeg = blockIdx.y * split_px, pend = min(P, pbeg + split_px);
-  v4f acc[NCO][NCI], bacc[NCO];
+constexpr size_t wgrad_lds_bytes() {
+  return std::max(size_t(NCO * 16 * kSD + NCI * 16 * kSX) * sizeof(uint16_t), kWRed);
+}
+
+// Elements [1, 9) of the 16 bf16 (lo, hi), or [-1, 7) of (hi) preceded by the
+// last element of lo: the B operands of the taps at column offsets +1 and -1.
+__device__ __forceinline__ uint4 shift_up(uint4 lo, uint4 hi) {  // elements 1..8 of lo:hi
+  return make_uint4(__builtin_amdgcn_alignbyte(lo.y, lo.x, 2), __builtin_amdgcn_alignbyte(lo.z, lo.y, 2),
+                    __builtin_amdgcn_alignbyte(lo.w, lo.z, 2), __builtin_amdgcn_alignbyte(hi.x, lo.w, 2));
+}
+
+template <int NCO, int NCI>
+__global__ __launch_bounds__(256) void k_wgrad(const uint16_t* __restrict__ x, int cin,
+                                               const uint16_t* __restrict__ dy, const uint16_t* __restrict__ mask,
+                                               int cout, int H, int W, int nbands, int ncog,
+                                               float* __restrict__ partial, float* __restrict__ bpartial) {
+  extern __shared__ uint4 wlds4[];
+  uint16_t* dT = reinterpret_cast<uint16_t*>(wlds4);
+  uint16_t* xT = dT + NCO * 16 * kSD;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int n = blockIdx.x / nbands, r0 = (blockIdx.x % nbands) * kWR;
+  const int co0 = (blockIdx.y % ncog) * NCO * 16, ci0 = (blockIdx.y / ncog) * NCI * 16;
+  const bool bias = ci0 == 0;  // uniform over the block
+  const bool row_in = r0 + wave < H;
+  v4f acc[9][NCO][NCI], bacc[NCO];
 #pragma unroll
   for (int a = 0; a < NCO; ++a) {
     bacc[a] = v4f{0.f, 0.f, 0.f, 0.f};
 #pragma unroll
-    for (int b = 0; b < NCI; ++b) acc[a][b] = v4f{0.f, 0.f, 0.f, 0.f};
+    for (int t = 0; t < 9; ++t)
+#pragma unroll
+      for (int b = 0; b < NCI; ++b) acc[t][a][b] = v4f{0.f, 0.f, 0.f, 0.f};
   }
-  const bool bias = tap == 4 && ci0 == 0;  // uniform over the wave
-  const v4s ones = v4s{short(0x3f80), short(0x3f80), short(0x3f80), short(0x3f80)};  // bf16 1.0
-  for (int pk0 = pbeg; pk0 < pend; pk0 += 16) {
-    // the 4 pixels of this lane's K slots
-    size_t own[4], nb[4];
-    bool vo[4], vn[4];
-#pragma unroll
-    for (int j = 0; j < 4; ++j) {
-      const int p = pk0 + (lane >> 4) * 4 + j;
-      vo[j] = p < pend;
-      own[j] = size_t(vo[j] ? p : 0);
-      int w = 0, h = 0, n = 0;
-      if (vo[j]) {
-        w = p % W;
-        const int t = p / W;
-        h = t % H;
-        n = t / H;
+  const uint4 ones = make_uint4(0x3f803f80u, 0x3f803f80u, 0x3f803f80u, 0x3f803f80u);  // bf16 1.0 x 8
+  for (int c0 = 0; c0 < W; c0 += kWC) {
+    __syncthreads();  // the previous tile's reads are done
+    // dPre, transposed: 16-byte loads of 8 channels, 2-byte stores
+    constexpr int DQ = NCO * 2;  // 8-channel pieces per pixel
+    for (int i = tid; i < kWR * kWC * DQ; i += 256) {
+      const int q = i % DQ, px = i / DQ, col = px % kWC, row = px / kWC;
+      const int h = r0 + row, w = c0 + col;
+      uint4 v = make_uint4(0, 0, 0, 0);
+      if (h < H && w < W) {
+        const size_t o = ((size_t(n) * H + h) * W + w) * cout + co0 + 8 * q;
+        v = *reinterpret_cast<const uint4*>(dy + o);
+        if (mask) v = mask8(v, *reinterpret_cast<const uint4*>(mask + o));
       }
-      const int hh = h + dyo, ww = w + dxo;
-      vn[j] = vo[j] && hh >= 0 && hh < H && ww >= 0 && ww < W;
-      nb[j] = vn[j] ? (size_t(n) * H + hh) * W + ww : 0;
+      const uint32_t e[4] = {v.x, v.y, v.z, v.w};
+      uint16_t* dst = dT + (8 * q) * kSD + row * kWC + col;
+#pragma unroll
+      for (int k = 0; k < 4; ++k) {
+        dst[(2 * k) * kSD] = uint16_t(e[k]);
+        dst[(2 * k + 1) * kSD] = uint16_t(e[k] >> 16);
+      }
     }
-    v4s A[NCO], B[NCI];
+    // x with a one-row halo and 8 columns each side, zero outside the image
+    constexpr int XQ = NCI * 2;
+    for (int i = tid; i < (kWR + 2) * kWX * XQ; i += 256) {
+      const int q = i % XQ, px = i / XQ, col = px % kWX, row = px / kWX;
+      const int h = r0 - 1 + row, w = c0 - 8 + col;
+      uint4 v = make_uint4(0, 0, 0, 0);
+      if (h >= 0 && h < H && w >= 0 && w < W)
+        v = *reinterpret_cast<const uint4*>(x + ((size_t(n) * H + h) * W + w) * cin + ci0 + 8 * q);
+      const uint32_t e[4] = {v.x, v.y, v.z, v.w};
+      uint16_t* dst = xT + (8 * q) * kSX + row * kWX + col;
 #pragma unroll
-    for (int a = 0; a < NCO; ++a) {
-      const int co = co0 + a * 16 + (lane & 15);
+      for (int k = 0; k < 4; ++k) {
+        dst[(2 * k) * kSX] = uint16_t(e[k]);
+        dst[(2 * k + 1) * kSX] = uint16_t(e[k] >> 16);
+      }
+    }
+    __syncthreads();
+    if (!row_in) continue;  // this wave's row is past the image: its dPre is zero
 #pragma unroll
-      for (int j = 0; j < 4; ++j) {
-        short v = 0;
-        if (vo[j]) {
-          v = short(dy[own[j] * cout + co]);
-          if (mask && !(bf2f(mask[own[j] * cout + co]) > 0.f)) v = 0;
+    for (int kc = 0; kc < kWC / 32; ++kc) {
+      const int cb = 32 * kc + 8 * (lane >> 4);
+      uint4 A[NCO];
+#pragma unroll
+      for (int a = 0; a < NCO; ++a)
+        A[a] = *reinterpret_cast<const uint4*>(dT + (a * 16 + (lane & 15)) * kSD + wave * kWC + cb);
+#pragma unroll
+      for (int b = 0; b < NCI; ++b) {
+#pragma unroll
+        for (int dr = 0; dr < 3; ++dr) {
+          const uint16_t* xr = xT + (b * 16 + (lane & 15)) * kSX + (wave + dr) * kWX + 8 + cb;
+          const uint4 prev = *reinterpret_cast<const uint4*>(xr - 8);
+          const uint4 cur = *reinterpret_cast<const uint4*>(xr);
+          const uint4 next = *reinterpret_cast<const uint4*>(xr + 8);
+          const uint4 left = make_uint4(__builtin_amdgcn_alignbyte(cur.x, prev.w, 2),
+                                        __builtin_amdgcn_alignbyte(cur.y, cur.x, 2),
+                                        __builtin_amdgcn_alignbyte(cur.z, cur.y, 2),
+                                        __builtin_amdgcn_alignbyte(cur.w, cur.z, 2));  // columns -1..6
+          const uint4 right = shift_up(cur, next);                                     // columns 1..8
+#pragma unroll
+          for (int a = 0; a < NCO; ++a) {
+            acc[dr * 3 + 0][a][b] = mfma32(A[a], left, acc[dr * 3 + 0][a][b]);
+            acc[dr * 3 + 1][a][b] = mfma32(A[a], cur, acc[dr * 3 + 1][a][b]);
+            acc[dr * 3 + 2][a][b] = mfma32(A[a], right, acc[dr * 3 + 2][a][b]);
+          }
         }
-        A[a][j] = v;
+      }
+      if (bias) {
+#pragma unroll
+        for (int a = 0; a < NCO; ++a) bacc[a] = mfma32(A[a], ones, bacc[a]);
       }
     }
-#pragma unroll
-    for (int b = 0; b < NCI; ++b) {
-      const int ci = ci0 + b * 16 + (lane & 15);
-#pragma unroll
-      for (int j = 0; j < 4; ++j) B[b][j] = vn[j] ? short(x[nb[j] * cin + ci]) : short(0);
-    }
-#pragma unroll
-    for (int a = 0; a < NCO; ++a)
-#pragma unroll
-      for (int b = 0; b < NCI; ++b) acc[a][b] = mfma(A[a], B[b], acc[a][b]);
-    if (bias) {
-#pragma unroll
-      for (int a = 0; a < NCO; ++a) bacc[a] = mfma(A[a], ones, bacc[a]);
-    }
   }
-  float* out = partial + size_t(blockIdx.y) * cout * 9 * cin;
+  // waves 1..3 hand their sums to wave 0 through LDS, one (a, b) pair at a time
+  float* red = reinterpret_cast<float*>(wlds4);
+  const size_t nw = size_t(cout) * 9 * cin;
+  float* out = partial + size_t(blockIdx.x) * nw;
 #pragma unroll
   for (int a = 0; a < NCO; ++a)
 #pragma unroll
-    for (int b = 0; b < NCI; ++b)
+    for (int b = 0; b < NCI; ++b) {
+      __syncthreads();
+      if (wave > 0) {
 #pragma unroll
-      for (int r = 0; r < 4; ++r) {
-        const int co = co0 + a * 16 + (lane >> 4) * 4 + r;
-        const int ci = ci0 + b * 16 + (lane & 15);
-        out[(size_t(co) * 9 + tap) * cin + ci] = acc[a][b][r];
+        for (int t = 0; t < 9; ++t)
+#pragma unroll
+          for (int r = 0; r < 4; ++r) red[((wave - 1) * 36 + t * 4 + r) * 64 + lane] = acc[t][a][b][r];
       }
-  if (bias && (lane & 15) == 0) {  // column 0 of the all-ones product
+      __syncthreads();
+      if (wave == 0) {
 #pragma unroll
-    for (int a = 0; a < NCO; ++a)
+        for (int t = 0; t < 9; ++t)
 #pragma unroll
-      for (int r = 0; r < 4; ++r)
-        bpartial[size_t(blockIdx.y) * cout + co0 + a * 16 + (lane >> 4) * 4 + r] = bacc[a][r];
+          for (int r = 0; r < 4; ++r) {
+            float v = acc[t][a][b][r];
+            v += red[(0 * 36 + t * 4 + r) * 64 + lane];
+            v += red[(1 * 36 + t * 4 + r) * 64 + lane];
+            v += red[(2 * 36 + t * 4 + r) * 64 + lane];
+            const int co = co0 + a * 16 + (lane >> 4) * 4 + r;
+            const int ci = ci0 + b * 16 + (lane & 15);
+            out[(size_t(co) * 9 + t) * cin + ci] = v;
+          }
+      }
+    }
+  if (bias) {
+    __syncthreads();
+    if (wave > 0) {
+#pragma unroll
+      for (int a = 0; a < NCO; ++a)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) red[((wave - 1) * 4 * NCO + a * 4 + r) * 64 + lane] = bacc[a][r];
+    }
+    __syncthreads();
+    if (wave == 0 && (lane & 15) == 0) {  // column 0 of the all-ones product
+#pragma unroll
+      for (int a = 0; a < NCO; ++a)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          float v = bacc[a][r];
+          v += red[((0 * NCO + a) * 4 + r) * 64 + lane];
+          v += red[((1 * NCO + a) * 4 + r) * 64 + lane];
+          v += red[((2 * NCO + a) * 4 + r) * 64 + lane];
+          bpartial[size_t(blockIdx.x) * cout + co0 + a * 16 + (lane >> 4) * 4 + r] = v;
+        }
+    }
   }
 }
 
@@ -590,7 +672,22 @@ void launch_conv(hipStream_t s, const uint16_t* x, const uint16_t* mask, int cin
   }
 }
 
-constexpr int kSplitPx = 1024;  // pixels per weight-gradient split
+// Weight-gradient splits: one per band of kWR rows of one image.
+int wgrad_splits(Dims d) { return d.n * ((d.h + kWR - 1) / kWR); }
+
+template <int NCO, int NCI>
+void launch_wgrad(hipStream_t s, const uint16_t* x, int cin, const uint16_t* dy, const uint16_t* mask, int cout,
+                  Dims d, float* pw, float* pb) {
+  constexpr size_t lds = wgrad_lds_bytes<NCO, NCI>();
+  static const hipError_t attr =  // once per instantiation: allow more than 64 KiB of LDS
+      hipFuncSetAttribute(reinterpret_cast<const void*>(&k_wgrad<NCO, NCI>),
+                          hipFuncAttributeMaxDynamicSharedMemorySize, int(lds));
+  (void)attr;
+  const int ncog = cout / (NCO * 16), ncig = cin / (NCI * 16);
+  const int nbands = (d.h + kWR - 1) / kWR;
+  hipLaunchKernelGGL((k_wgrad<NCO, NCI>), dim3(d.n * nbands, ncog * ncig), dim3(256), lds, s, x, cin, dy, mask, cout,
+                     d.h, d.w, nbands, ncog, pw, pb);
+}
 
 }  // namespace
 
@@ -612,32 +709,23 @@ void conv_dgrad(hipStream_t s, const uint16_t* dy, const uint16_t* y, int cout, 
 }
 
 size_t wgrad_scratch_floats(int cin, int cout, Dims d) {
-  const size_t splits = (size_t(d.pixels()) + kSplitPx - 1) / kSplitPx;
+  const size_t splits = size_t(wgrad_splits(d));
   return splits * (size_t(cout) * 9 * cin + cout);
 }
 
 void conv_wgrad(hipStream_t s, const uint16_t* x, int cin, const uint16_t* dy, const uint16_t* y, int cout, Dims d,
                 float* dw, float* db, float* scratch, size_t scratch_floats) {
-  const int P = d.pixels();
-  const int splits = (P + kSplitPx - 1) / kSplitPx;
+  const int splits = wgrad_splits(d);
   const size_t nw = size_t(cout) * 9 * cin;
   if (scratch_floats < size_t(splits) * (nw + cout)) return;  // sized by wgrad_scratch_floats
   float* pw = scratch;
   float* pb = scratch + size_t(splits) * nw;
-  const int tco = cout / 16, tci = cin / 16;
-  // tile groups of up to 4 x 4 (co x ci) 16-wide tiles per wave
-  const int nco = tco % 4 == 0 ? 4 : (tco % 3 == 0 ? 3 : (tco % 2 == 0 ? 2 : 1));
-  const int nci = tci % 4 == 0 ? 4 : (tci % 3 == 0 ? 3 : (tci % 2 == 0 ? 2 : 1));
-  const int ncog = tco / nco, ncig = tci / nci;
-  const dim3 grid(9 * ncog * ncig, splits), block(64);
-#define GDN_WG(A, B)                                                                                           \
-  if (nco == A && nci == B)                                                                                     \
-  hipLaunchKernelGGL((k_wgrad<A, B>), grid, block, 0, s, x, cin, dy, y, cout, d.h, d.w, P, kSplitPx, ncog, pw, pb)
-  GDN_WG(1, 1); GDN_WG(1, 2); GDN_WG(1, 3); GDN_WG(1, 4);
-  GDN_WG(2, 1); GDN_WG(2, 2); GDN_WG(2, 3); GDN_WG(2, 4);
-  GDN_WG(3, 1); GDN_WG(3, 2); GDN_WG(3, 3); GDN_WG(3, 4);
-  GDN_WG(4, 1); GDN_WG(4, 2); GDN_WG(4, 3); GDN_WG(4, 4);
-#undef GDN_WG
+  // channel fragments per block: two when the layer has an even number of 16-channel groups
+  const bool co2 = (cout / 16) % 2 == 0, ci2 = (cin / 16) % 2 == 0;
+  if (co2 && ci2) launch_wgrad<2, 2>(s, x, cin, dy, y, cout, d, pw, pb);
+  else if (co2) launch_wgrad<2, 1>(s, x, cin, dy, y, cout, d, pw, pb);
+  else if (ci2) launch_wgrad<1, 2>(s, x, cin, dy, y, cout, d, pw, pb);
+  else launch_wgrad<1, 1>(s, x, cin, dy, y, cout, d, pw, pb);
   hipLaunchKernelGGL(k_reduce, dim3(unsigned((nw + 31) / 32)), dim3(256), 0, s, pw, splits, nw, dw);
   hipLaunchKernelGGL(k_reduce, dim3(unsigned((cout + 31) / 32)), dim3(256), 0, s, pb, splits, size_t(cout), db);
 }

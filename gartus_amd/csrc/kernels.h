@@ -131,7 +131,7 @@ struct Work {
 void reset_counters(const Launch& l, const Queues& q, uint32_t depth, uint32_t total);
 void zero_acc(const Launch& l, double4* acc, size_t n);
 void extend(const Launch& l, const Work& w, const Queues& q, uint32_t bounce, bool count);
-// One bounce of shading: one kernel per category (empty categories exit at once).
+// One bounce of shading: one launch drains the category queues in turn.
 void shade(const Launch& l, const Work& w, const Queues& q, uint32_t bounce);
 void connect(const Launch& l, const Work& w, const Queues& q, uint32_t bounce, uint32_t kind, bool count);
 void accumulate(const Launch& l, const Work& w, const Queues& q);

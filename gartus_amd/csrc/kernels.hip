@@ -1,7 +1,7 @@
 // kernels.hip — the wavefront path tracer for gfx950 (CDNA4).
 //
 // One batch of paths (a set of samples for every pixel) runs as
-//   reset -> { extend(b) -> shade(b, per category) -> connect(b, light)
+//   reset -> { extend(b) -> shade(b, every category) -> connect(b, light)
 //   -> connect(b, env) } for b < max_depth -> accumulate
 // where extend(0) generates the camera rays itself
 // over path state in HBM. Every stage is a persistent kernel: a fixed grid
@@ -307,7 +307,7 @@ __global__ __attribute__((amdgpu_waves_per_eu(8))) void k_extend(Frame f0, Batch
   }
 }
 
-// ---- shading: one bounce of ray_color (camera.rs:1742-1858) per category --
+// ---- shading: one bounce of ray_color (camera.rs:1742-1858), by category --
 struct Item {
   uint32_t path;
   bool after_scatter;
