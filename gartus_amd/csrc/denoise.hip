@@ -720,11 +720,9 @@ void conv_wgrad(hipStream_t s, const uint16_t* x, int cin, const uint16_t* dy, c
   if (scratch_floats < size_t(splits) * (nw + cout)) return;  // sized by wgrad_scratch_floats
   float* pw = scratch;
   float* pb = scratch + size_t(splits) * nw;
-  // channel fragments per block: two when the layer has an even number of 16-channel groups
-  const bool co2 = (cout / 16) % 2 == 0, ci2 = (cin / 16) % 2 == 0;
-  if (co2 && ci2) launch_wgrad<2, 2>(s, x, cin, dy, y, cout, d, pw, pb);
-  else if (co2) launch_wgrad<2, 1>(s, x, cin, dy, y, cout, d, pw, pb);
-  else if (ci2) launch_wgrad<1, 2>(s, x, cin, dy, y, cout, d, pw, pb);
+  // two output-channel fragments per block when the layer has an even number of them, one input
+  // fragment: 2 x 2 needs 400 registers (one wave per SIMD) and was 5 % slower on one MI355X
+  if ((cout / 16) % 2 == 0) launch_wgrad<2, 1>(s, x, cin, dy, y, cout, d, pw, pb);
   else launch_wgrad<1, 1>(s, x, cin, dy, y, cout, d, pw, pb);
   hipLaunchKernelGGL(k_reduce, dim3(unsigned((nw + 31) / 32)), dim3(256), 0, s, pw, splits, nw, dw);
   hipLaunchKernelGGL(k_reduce, dim3(unsigned((cout + 31) / 32)), dim3(256), 0, s, pb, splits, size_t(cout), db);
