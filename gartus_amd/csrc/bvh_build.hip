@@ -12,13 +12,16 @@
 //      split from the longest common prefixes (equal keys break ties by
 //      index), children and parent links;
 //   4. boxes bottom up: every leaf walks towards the root and the second of
-//      two arrivals at a node unites both children and continues (an
-//      acquire-release counter per node, no waiting);
+//      two arrivals at a node unites both children, counts the emitted nodes
+//      of its subtree, and continues (an acquire-release counter per node, no
+//      waiting);
 //   5. subtrees of at most leaf_size primitives become leaves (a contiguous
 //      range of sorted records, the last flagged); every larger internal node
 //      writes its GNode2 record and its GNode4 record (its grandchildren,
-//      left-left first), indexed by the node itself, so no compaction pass is
-//      needed; the records of collapsed nodes are never referenced;
+//      left-left first) at its depth-first preorder position among the
+//      emitted nodes, so a node's nearer-left child usually follows it in
+//      memory (the emitted-node counts of step 4 give each node its position
+//      from its root path alone);
 //   6. records reordered by the sort, the tree depth from leaf-to-root walks.
 // The tree differs from the reference's SAH tree (bvh.rs:233-288); closest
 // hits do not depend on the tree, traversal counts do.
@@ -282,7 +285,8 @@ __global__ __launch_bounds__(256) void k_bounds(uint32_t n, const uint32_t* __re
                                                 const float4* __restrict__ plo, const float4* __restrict__ phi,
                                                 const uint32_t* __restrict__ parent_leaf,
                                                 const uint32_t* __restrict__ parent_int, const uint2* __restrict__ child,
-                                                float4* nlo, float4* nhi, uint32_t* arrivals) {
+                                                const uint2* __restrict__ range, uint32_t leaf_size,
+                                                float4* nlo, float4* nhi, uint32_t* icount, uint32_t* arrivals) {
   for (uint32_t k = blockIdx.x * blockDim.x + threadIdx.x; k < n; k += gridDim.x * blockDim.x) {
     uint32_t node = parent_leaf[k];
     while (node != kNone) {
@@ -295,8 +299,32 @@ __global__ __launch_bounds__(256) void k_bounds(uint32_t n, const uint32_t* __re
       child_box(c.y, order, plo, phi, nlo, nhi, &bl, &bh);
       nlo[node] = make_float4(fminf(al.x, bl.x), fminf(al.y, bl.y), fminf(al.z, bl.z), 0.0f);
       nhi[node] = make_float4(fmaxf(ah.x, bh.x), fmaxf(ah.y, bh.y), fmaxf(ah.z, bh.z), 0.0f);
+      // emitted internal nodes in the subtree: this one unless it falls inside a leaf
+      const uint2 rg = range[node];
+      icount[node] = uint32_t(rg.y - rg.x + 1u > leaf_size) + ((c.x & GT_LEAF_BIT) ? 0u : icount[c.x]) +
+                     ((c.y & GT_LEAF_BIT) ? 0u : icount[c.y]);
       node = parent_int[node];
     }
+  }
+}
+
+// Preorder position of every emitted internal node among the emitted nodes:
+// from the root, each step to a left child adds 1, each step to a right child
+// adds 1 plus the emitted nodes of the left sibling's subtree.
+__global__ __launch_bounds__(256) void k_preorder(uint32_t n, uint32_t leaf_size, const uint2* __restrict__ range,
+                                                  const uint2* __restrict__ child,
+                                                  const uint32_t* __restrict__ parent_int,
+                                                  const uint32_t* __restrict__ icount, uint32_t* __restrict__ pre) {
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i + 1 < n; i += gridDim.x * blockDim.x) {
+    const uint2 rg = range[i];
+    if (rg.y - rg.x + 1u <= leaf_size) continue;  // inside a leaf: never referenced
+    uint32_t pos = 0;
+    for (uint32_t c = i, p = parent_int[i]; p != kNone; c = p, p = parent_int[p]) {
+      const uint2 pc = child[p];
+      pos += 1u;
+      if (pc.y == c) pos += (pc.x & GT_LEAF_BIT) ? 0u : icount[pc.x];
+    }
+    pre[i] = pos;
   }
 }
 
@@ -316,23 +344,26 @@ __global__ __launch_bounds__(256) void k_depth(uint32_t n, const uint32_t* __res
 // leaf_size records, the first named) or an internal node index; also its
 // box and the last record of its range.
 struct Ref {
-  uint32_t ref, last;
+  uint32_t ref, last, node;  // node: the internal node's build index (for its children)
   float4 lo, hi;
   bool leaf;
 };
 __device__ inline Ref make_ref(uint32_t c, uint32_t leaf_size, const uint2* range, const uint32_t* order,
-                               const float4* plo, const float4* phi, const float4* nlo, const float4* nhi) {
+                               const float4* plo, const float4* phi, const float4* nlo, const float4* nhi,
+                               const uint32_t* pre) {
   Ref r;
   child_box(c, order, plo, phi, nlo, nhi, &r.lo, &r.hi);
   if (c & GT_LEAF_BIT) {
     r.leaf = true;
     r.ref = c;
+    r.node = 0;
     r.last = c & ~GT_LEAF_BIT;
     return r;
   }
   const uint2 rg = range[c];
   r.leaf = rg.y - rg.x + 1u <= leaf_size;
-  r.ref = r.leaf ? (GT_LEAF_BIT | rg.x) : c;
+  r.node = c;
+  r.ref = r.leaf ? (GT_LEAF_BIT | rg.x) : pre[c];
   r.last = rg.y;
   return r;
 }
@@ -341,8 +372,9 @@ __global__ __launch_bounds__(256) void k_emit(uint32_t n, uint32_t leaf_size, co
                                               const uint2* __restrict__ range, const uint32_t* __restrict__ order,
                                               const float4* __restrict__ plo, const float4* __restrict__ phi,
                                               const float4* __restrict__ nlo, const float4* __restrict__ nhi,
-                                              GNode2* __restrict__ nodes2, GNode4* __restrict__ nodes4,
-                                              uint8_t* __restrict__ last, unsigned long long* counts) {
+                                              const uint32_t* __restrict__ pre, GNode2* __restrict__ nodes2,
+                                              GNode4* __restrict__ nodes4, uint8_t* __restrict__ last,
+                                              unsigned long long* counts) {
   if (blockIdx.x == 0 && threadIdx.x == 0 && (n == 1 || n <= leaf_size)) last[n - 1] = 1;  // the root is a leaf
   uint32_t emitted = 0, leaves = 0;
   for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i + 1 < n; i += gridDim.x * blockDim.x) {
@@ -350,8 +382,8 @@ __global__ __launch_bounds__(256) void k_emit(uint32_t n, uint32_t leaf_size, co
     if (rg.y - rg.x + 1u <= leaf_size) continue;  // inside a leaf
     ++emitted;
     const uint2 c = child[i];
-    const Ref L = make_ref(c.x, leaf_size, range, order, plo, phi, nlo, nhi);
-    const Ref R = make_ref(c.y, leaf_size, range, order, plo, phi, nlo, nhi);
+    const Ref L = make_ref(c.x, leaf_size, range, order, plo, phi, nlo, nhi, pre);
+    const Ref R = make_ref(c.y, leaf_size, range, order, plo, phi, nlo, nhi, pre);
     if (L.leaf) { last[L.last] = 1; ++leaves; }
     if (R.leaf) { last[R.last] = 1; ++leaves; }
     GNode2 g;
@@ -360,7 +392,8 @@ __global__ __launch_bounds__(256) void k_emit(uint32_t n, uint32_t leaf_size, co
     g.left = L.ref;
     g.right = R.ref;
     g.pad0 = g.pad1 = 0;
-    nodes2[i] = g;
+    const uint32_t at = pre[i];
+    nodes2[at] = g;
     // the four-wide record: grandchildren through internal children, left subtree first
     Ref kids[4];
     int nk = 0;
@@ -370,9 +403,9 @@ __global__ __launch_bounds__(256) void k_emit(uint32_t n, uint32_t leaf_size, co
       if (r->leaf) {
         kids[nk++] = *r;
       } else {
-        const uint2 cc = child[r->ref];
-        kids[nk++] = make_ref(cc.x, leaf_size, range, order, plo, phi, nlo, nhi);
-        kids[nk++] = make_ref(cc.y, leaf_size, range, order, plo, phi, nlo, nhi);
+        const uint2 cc = child[r->node];
+        kids[nk++] = make_ref(cc.x, leaf_size, range, order, plo, phi, nlo, nhi, pre);
+        kids[nk++] = make_ref(cc.y, leaf_size, range, order, plo, phi, nlo, nhi, pre);
       }
     }
     GNode4 w;
@@ -388,7 +421,7 @@ __global__ __launch_bounds__(256) void k_emit(uint32_t n, uint32_t leaf_size, co
       w.hix[k] = kids[k].hi.x; w.hiy[k] = kids[k].hi.y; w.hiz[k] = kids[k].hi.z;
       w.child[k] = kids[k].ref;
     }
-    nodes4[i] = w;
+    nodes4[at] = w;
   }
   for (int off = 32; off > 0; off >>= 1) {
     emitted += __shfl_xor(emitted, off);
@@ -423,7 +456,7 @@ __global__ void k_init(unsigned long long* cbounds, uint32_t* depth, unsigned lo
 // Scratch layout, each part 256-byte aligned.
 struct Layout {
   size_t staged, packed, plo, phi, cen, keys, keys2, vals, vals2, child, range, pleaf, pint, nlo, nhi, arrivals,
-      last, cbounds, depth, counts, sort_tmp, sort_bytes, total;
+      icount, pre, last, cbounds, depth, counts, sort_tmp, sort_bytes, total;
 };
 inline size_t up(size_t x) { return (x + 255) & ~size_t(255); }
 
@@ -448,6 +481,8 @@ Layout layout(uint32_t n) {
   L.nlo = take(sizeof(float4) * m);
   L.nhi = take(sizeof(float4) * m);
   L.arrivals = take(4 * m);
+  L.icount = take(4 * m);
+  L.pre = take(4 * m);
   L.last = take(size_t(n));
   L.cbounds = take(6 * 8);
   L.depth = take(4);
@@ -489,6 +524,8 @@ hipError_t build(hipStream_t s, const gt::HostPrim* host_prims, uint32_t n, uint
   float4* nlo = static_cast<float4*>(at(L.nlo));
   float4* nhi = static_cast<float4*>(at(L.nhi));
   uint32_t* arrivals = static_cast<uint32_t*>(at(L.arrivals));
+  uint32_t* icount = static_cast<uint32_t*>(at(L.icount));
+  uint32_t* pre = static_cast<uint32_t*>(at(L.pre));
   uint8_t* last = static_cast<uint8_t*>(at(L.last));
   auto* cbounds = static_cast<unsigned long long*>(at(L.cbounds));
   uint32_t* depth = static_cast<uint32_t*>(at(L.depth));
@@ -522,12 +559,14 @@ hipError_t build(hipStream_t s, const gt::HostPrim* host_prims, uint32_t n, uint
     return done(e);
   if (n > 1) {
     hipLaunchKernelGGL(k_hierarchy, dim3(grid_for(n - 1)), dim3(256), 0, s, keys2, int(n), child, range, pleaf, pint);
-    hipLaunchKernelGGL(k_bounds, dim3(grid_for(n)), dim3(256), 0, s, n, order, plo, phi, pleaf, pint, child, nlo, nhi,
-                       arrivals);
+    hipLaunchKernelGGL(k_bounds, dim3(grid_for(n)), dim3(256), 0, s, n, order, plo, phi, pleaf, pint, child, range,
+                       leaf_size, nlo, nhi, icount, arrivals);
+    hipLaunchKernelGGL(k_preorder, dim3(grid_for(n - 1)), dim3(256), 0, s, n, leaf_size, range, child, pint, icount,
+                       pre);
     hipLaunchKernelGGL(k_depth, dim3(grid_for(n)), dim3(256), 0, s, n, pleaf, pint, depth);
   }
   hipLaunchKernelGGL(k_emit, dim3(std::max(1u, grid_for(n))), dim3(256), 0, s, n, leaf_size, child, range, order, plo,
-                     phi, nlo, nhi, nodes2, nodes4, last, counts);
+                     phi, nlo, nhi, pre, nodes2, nodes4, last, counts);
   hipLaunchKernelGGL(k_reorder, dim3(grid_for(n)), dim3(256), 0, s, n, order, packed, last, prims, ids);
   if ((e = hipGetLastError()) != hipSuccess) return done(e);
   if ((e = hipEventRecord(e1, s)) != hipSuccess) return done(e);
