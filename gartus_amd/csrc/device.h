@@ -686,11 +686,10 @@ __device__ __forceinline__ f3 env_radiance(const DEnv& e, f3 dir) {  // :98-107
   if (!isfinite(u) || !isfinite(v)) return mk(0.f, 0.f, 0.f);
   return env_radiance_uv(e, u, v);
 }
-__device__ __forceinline__ float env_pdf(const DEnv& e, f3 dir) {  // :136-156
+// pdf_value (:136-156) of a direction at lat-long angles (phi, theta): the
+// solid-angle pdf of the texel that holds it.
+__device__ __forceinline__ float env_pdf_at(const DEnv& e, float phi, float theta) {
   if (e.kind != 2 || e.uniform) return 1.0f / (4.0f * PI);
-  f3 d = normalized(dir);
-  float theta = acosf(clampf(d.y, -1.f, 1.f));
-  float phi = rem_euclid(atan2f(d.z, d.x), TAU);
   float fx = floorf((phi / TAU) * float(e.w)), fy = floorf((theta / PI) * float(e.h));
   uint32_t x = fx > 0.f ? min(uint32_t(fx), e.w - 1) : 0u;
   uint32_t y = fy > 0.f ? min(uint32_t(fy), e.h - 1) : 0u;
@@ -698,26 +697,17 @@ __device__ __forceinline__ float env_pdf(const DEnv& e, f3 dir) {  // :136-156
 }
 // env_radiance(dir) for a path that missed. On entry *mw is the path's MIS
 // weight, or -cpdf when the weight is the environment's power heuristic
-// power_heuristic(cpdf, env_pdf(dir)); on return it is the weight. The
+// power_heuristic(cpdf, pdf_value(dir)); on return it is the weight. The
 // direction maps to lat-long coordinates once for both lookups.
 __device__ __forceinline__ f3 env_radiance_weighted(const DEnv& e, f3 dir, float* mw) {
   if (e.kind != 2) {
-    if (*mw < 0.f) *mw = power_heuristic(-*mw, 1.0f / (4.0f * PI));
+    if (*mw < 0.f) *mw = power_heuristic(-*mw, env_pdf_at(e, 0.f, 0.f));
     return ld3(e.color);
   }
   f3 d = normalized(dir);
   float phi = rem_euclid(atan2f(d.z, d.x), TAU);
   float theta = acosf(clampf(d.y, -1.f, 1.f));
-  if (*mw < 0.f) {  // env_pdf
-    float pdf = 1.0f / (4.0f * PI);
-    if (!e.uniform) {
-      float fx = floorf((phi / TAU) * float(e.w)), fy = floorf((theta / PI) * float(e.h));
-      uint32_t x = fx > 0.f ? min(uint32_t(fx), e.w - 1) : 0u;
-      uint32_t y = fy > 0.f ? min(uint32_t(fy), e.h - 1) : 0u;
-      pdf = e.tex[size_t(y) * e.w + x].w;
-    }
-    *mw = power_heuristic(-*mw, pdf);
-  }
+  if (*mw < 0.f) *mw = power_heuristic(-*mw, env_pdf_at(e, phi, theta));
   float u = phi / TAU, v = 1.0f - theta / PI;
   if (!isfinite(u) || !isfinite(v)) return mk(0.f, 0.f, 0.f);
   return env_radiance_uv(e, u, v);

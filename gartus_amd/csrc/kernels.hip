@@ -315,7 +315,7 @@ struct Item {
   uint32_t path;
   bool after_scatter;
   f3 o, d, T;
-  float time, mw, t;
+  float time, t;
   uint32_t prim;
 };
 __device__ __forceinline__ Item load_item(const Queues& q, uint32_t p) {
@@ -328,7 +328,6 @@ __device__ __forceinline__ Item load_item(const Queues& q, uint32_t p) {
   it.time = o4.w;
   it.d = mk(d4.x, d4.y, d4.z);
   it.T = mk(t4.x, t4.y, t4.z);
-  it.mw = t4.w;
   it.t = h.x;
   it.prim = __float_as_uint(h.y);
   return it;
@@ -420,7 +419,7 @@ __device__ __forceinline__ uint32_t shade_scatter(const Frame& f, const Queues& 
   if (!(isfinite(cpdf) && cpdf > EPS)) return out;
   float spdf = scattering_pdf(m, it.d, n, dir);
   if (!(isfinite(spdf) && spdf > 0.f)) return out;
-  // The environment's MIS weight power_heuristic(cpdf, env_pdf(dir)) applies
+  // The environment's MIS weight power_heuristic(cpdf, pdf_value(dir)) applies
   // only if the continuation misses: it is stored as -cpdf (cpdf > EPS here)
   // and evaluated by the miss shading in extend.
   float next_w = weight_env ? -cpdf : 1.0f;
