@@ -15,9 +15,11 @@ struct Dims {
   int pixels() const { return n * h * w; }
 };
 
-// y = act(conv(x) + bias); y is bf16, or fp32 when out_f32.
+// y = act(conv(x) + bias); y is bf16, or fp32 when out_f32. A bf16 output can
+// also be written 2x2 max-pooled to `pooled` (rows and columns in pairs; an
+// odd last row or column is dropped).
 void conv_fwd(hipStream_t s, const uint16_t* x, int cin, const uint16_t* w, const float* bias, void* y, int cout,
-              Dims d, bool relu, bool out_f32);
+              Dims d, bool relu, bool out_f32, uint16_t* pooled = nullptr);
 // y = act(conv(cat(up2(a), b)) + bias) without materialising the concatenation: a has ca channels
 // at half resolution, b has cb channels at d; the same result as upcat_fwd followed by conv_fwd.
 void conv_fwd_upcat(hipStream_t s, const uint16_t* a, int ca, const uint16_t* b, int cb, const uint16_t* w,
@@ -31,8 +33,6 @@ void conv_wgrad(hipStream_t s, const uint16_t* x, int cin, const uint16_t* dy, c
                 float* dw, float* db, float* scratch, size_t scratch_floats);
 size_t wgrad_scratch_floats(int cin, int cout, Dims d);
 
-// 2x2 max pool (h, w even).
-void pool_fwd(hipStream_t s, const uint16_t* x, uint16_t* y, int c, Dims d);
 // dx = route dy to each window's first maximum, plus add[:, add_off : add_off + c] when add != null
 // (add has add_c channels at the full resolution).
 void pool_bwd(hipStream_t s, const uint16_t* x, const uint16_t* dy, const uint16_t* add, int add_c, int add_off,

@@ -167,8 +167,8 @@ template <int NT, bool DGRAD, bool OUT_F32>
 __global__ __launch_bounds__(256) void k_conv(const uint16_t* __restrict__ x, const uint16_t* __restrict__ mask,
                                               int cin, const uint16_t* __restrict__ xa, int ca,
                                               const uint16_t* __restrict__ wgt, const float* __restrict__ bias,
-                                              void* __restrict__ y, int cout, int co_first, int H, int W, int nrt,
-                                              int nct, int relu) {
+                                              void* __restrict__ y, uint16_t* __restrict__ pooled, int cout,
+                                              int co_first, int H, int W, int nrt, int nct, int relu) {
   extern __shared__ uint4 lds4[];
   uint16_t* halo = reinterpret_cast<uint16_t*>(lds4);
   uint16_t* wts = halo + kHR * kHC * kLS;
@@ -214,6 +214,30 @@ __global__ __launch_bounds__(256) void k_conv(const uint16_t* __restrict__ x, co
       }
   }
   __syncthreads();
+  if (!OUT_F32 && pooled) {  // also the 2x2 max pool of the tile: 2 rows of 32 pixels
+    // rounding to bf16 is monotonic, so the max of the rounded values is the rounded max
+    const int H2 = H / 2, W2 = W / 2;
+    constexpr int G = NT * 2;  // 8-channel pieces per pixel
+    for (int i = tid; i < 2 * 32 * G; i += 256) {
+      const int g = i % G, pc = (i / G) % 32, pr = i / (G * 32);
+      const int h2 = h0 / 2 + pr, w2 = w0 / 2 + pc;
+      if (h2 >= H2 || w2 >= W2) continue;
+      const float* a = reinterpret_cast<const float*>(lds4) + (2 * pr) * 64 * OS + (2 * pc) * OS + 8 * g;
+      uint32_t o[4];
+#pragma unroll
+      for (int k = 0; k < 4; ++k) {
+        float m[2];
+#pragma unroll
+        for (int e = 0; e < 2; ++e) {
+          const int c = 2 * k + e;
+          m[e] = fmaxf(fmaxf(a[c], a[OS + c]), fmaxf(a[64 * OS + c], a[64 * OS + OS + c]));
+        }
+        o[k] = uint32_t(f2bf(m[0])) | (uint32_t(f2bf(m[1])) << 16);
+      }
+      *reinterpret_cast<uint4*>(pooled + ((size_t(n) * H2 + h2) * W2 + w2) * cout + co0 + 8 * g) =
+          make_uint4(o[0], o[1], o[2], o[3]);
+    }
+  }
   const int row = h0 + wave;
   if (row >= H) return;
   const size_t pix0 = (size_t(n) * H + row) * W;
@@ -456,23 +480,6 @@ __global__ __launch_bounds__(256) void k_reduce(const float* __restrict__ partia
 }
 
 // ---- pooling, upsampling, concatenation -----------------------------------
-__global__ void k_pool_fwd(const uint16_t* __restrict__ x, uint16_t* __restrict__ y, int c, int H, int W, size_t n) {
-  const int H2 = H / 2, W2 = W / 2;
-  for (size_t i = blockIdx.x * size_t(blockDim.x) + threadIdx.x; i < n; i += size_t(gridDim.x) * blockDim.x) {
-    const int ch = int(i % c);
-    size_t q = i / c;
-    const int w2 = int(q % W2);
-    q /= W2;
-    const int h2 = int(q % H2);
-    const size_t b = q / H2;
-    const size_t base = ((b * H + 2 * h2) * W + 2 * w2) * c + ch;
-    float m = bf2f(x[base]);
-    m = fmaxf(m, bf2f(x[base + c]));
-    m = fmaxf(m, bf2f(x[base + size_t(W) * c]));
-    m = fmaxf(m, bf2f(x[base + size_t(W) * c + c]));
-    y[i] = f2bf(m);
-  }
-}
 
 __global__ void k_pool_bwd(const uint16_t* __restrict__ x, const uint16_t* __restrict__ dy,
                            const uint16_t* __restrict__ add, int add_c, int add_off, uint16_t* __restrict__ dx, int c,
@@ -646,7 +653,8 @@ unsigned grid_for(size_t n, unsigned block = 256) {
 
 template <bool DGRAD, bool F32, int NT>
 void launch_nt(hipStream_t s, const uint16_t* x, const uint16_t* mask, int cin, const uint16_t* xa, int ca,
-               const uint16_t* w, const float* bias, void* y, int cout, int co_first, int groups, Dims d, int relu) {
+               const uint16_t* w, const float* bias, void* y, uint16_t* pooled, int cout, int co_first, int groups,
+               Dims d, int relu) {
   const int nrt = (d.h + kTR - 1) / kTR, nct = (d.w + kTC - 1) / kTC;
   constexpr size_t lds = conv_lds_bytes<NT>();
   static const hipError_t attr =  // once per instantiation: allow more than 64 KiB of LDS
@@ -654,20 +662,21 @@ void launch_nt(hipStream_t s, const uint16_t* x, const uint16_t* mask, int cin, 
                           hipFuncAttributeMaxDynamicSharedMemorySize, int(lds));
   (void)attr;
   hipLaunchKernelGGL((k_conv<NT, DGRAD, F32>), dim3(d.n * nrt * nct, groups), dim3(256), lds, s, x, mask, cin, xa,
-                     ca, w, bias, y, cout, co_first, d.h, d.w, nrt, nct, relu);
+                     ca, w, bias, y, pooled, cout, co_first, d.h, d.w, nrt, nct, relu);
 }
 
 // Output channels in groups of 64 (NT = 4), then one launch for the rest.
 template <bool DGRAD, bool F32>
 void launch_conv(hipStream_t s, const uint16_t* x, const uint16_t* mask, int cin, const uint16_t* xa, int ca,
-                 const uint16_t* w, const float* bias, void* y, int cout, Dims d, int relu) {
+                 const uint16_t* w, const float* bias, void* y, int cout, Dims d, int relu,
+                 uint16_t* pooled = nullptr) {
   const int frags = cout / 16, full = frags / 4, rest = frags % 4;
-  if (full) launch_nt<DGRAD, F32, 4>(s, x, mask, cin, xa, ca, w, bias, y, cout, 0, full, d, relu);
+  if (full) launch_nt<DGRAD, F32, 4>(s, x, mask, cin, xa, ca, w, bias, y, pooled, cout, 0, full, d, relu);
   const int c0 = full * 64;
   switch (rest) {
-    case 1: launch_nt<DGRAD, F32, 1>(s, x, mask, cin, xa, ca, w, bias, y, cout, c0, 1, d, relu); break;
-    case 2: launch_nt<DGRAD, F32, 2>(s, x, mask, cin, xa, ca, w, bias, y, cout, c0, 1, d, relu); break;
-    case 3: launch_nt<DGRAD, F32, 3>(s, x, mask, cin, xa, ca, w, bias, y, cout, c0, 1, d, relu); break;
+    case 1: launch_nt<DGRAD, F32, 1>(s, x, mask, cin, xa, ca, w, bias, y, pooled, cout, c0, 1, d, relu); break;
+    case 2: launch_nt<DGRAD, F32, 2>(s, x, mask, cin, xa, ca, w, bias, y, pooled, cout, c0, 1, d, relu); break;
+    case 3: launch_nt<DGRAD, F32, 3>(s, x, mask, cin, xa, ca, w, bias, y, pooled, cout, c0, 1, d, relu); break;
     default: break;
   }
 }
@@ -692,9 +701,9 @@ void launch_wgrad(hipStream_t s, const uint16_t* x, int cin, const uint16_t* dy,
 }  // namespace
 
 void conv_fwd(hipStream_t s, const uint16_t* x, int cin, const uint16_t* w, const float* bias, void* y, int cout,
-              Dims d, bool relu, bool out_f32) {
+              Dims d, bool relu, bool out_f32, uint16_t* pooled) {
   if (out_f32) launch_conv<false, true>(s, x, nullptr, cin, nullptr, 0, w, bias, y, cout, d, relu ? 1 : 0);
-  else launch_conv<false, false>(s, x, nullptr, cin, nullptr, 0, w, bias, y, cout, d, relu ? 1 : 0);
+  else launch_conv<false, false>(s, x, nullptr, cin, nullptr, 0, w, bias, y, cout, d, relu ? 1 : 0, pooled);
 }
 
 void conv_fwd_upcat(hipStream_t s, const uint16_t* a, int ca, const uint16_t* b, int cb, const uint16_t* w,
@@ -726,11 +735,6 @@ void conv_wgrad(hipStream_t s, const uint16_t* x, int cin, const uint16_t* dy, c
   else launch_wgrad<1, 1>(s, x, cin, dy, y, cout, d, pw, pb);
   hipLaunchKernelGGL(k_reduce, dim3(unsigned((nw + 31) / 32)), dim3(256), 0, s, pw, splits, nw, dw);
   hipLaunchKernelGGL(k_reduce, dim3(unsigned((cout + 31) / 32)), dim3(256), 0, s, pb, splits, size_t(cout), db);
-}
-
-void pool_fwd(hipStream_t s, const uint16_t* x, uint16_t* y, int c, Dims d) {
-  const size_t n = size_t(d.n) * (d.h / 2) * (d.w / 2) * c;
-  hipLaunchKernelGGL(k_pool_fwd, dim3(grid_for(n)), dim3(256), 0, s, x, y, c, d.h, d.w, n);
 }
 
 void pool_bwd(hipStream_t s, const uint16_t* x, const uint16_t* dy, const uint16_t* add, int add_c, int add_off,

@@ -198,8 +198,9 @@ void forward(gt_denoiser* dn, bool keep_cat) {
     Conv& c0 = dn->enc_conv(l, 0);
     Conv& c1 = dn->enc_conv(l, 1);
     gdn::conv_fwd(s, in, c0.cin, c0.w16.p, dn->params.p + c0.b_off, v.enc_a.p, c0.cout, d, true, false);
-    gdn::conv_fwd(s, v.enc_a.p, c1.cin, c1.w16.p, dn->params.p + c1.b_off, v.enc.p, c1.cout, d, true, false);
-    if (l + 1 < L) gdn::pool_fwd(s, v.enc.p, dn->lv[l + 1]->in.p, dn->ch[l], d);
+    // the second convolution also writes the next level's input, pooled
+    gdn::conv_fwd(s, v.enc_a.p, c1.cin, c1.w16.p, dn->params.p + c1.b_off, v.enc.p, c1.cout, d, true, false,
+                  l + 1 < L ? dn->lv[l + 1]->in.p : nullptr);
   }
   for (int l = L - 2; l >= 0; --l) {
     Level& v = *dn->lv[l];
