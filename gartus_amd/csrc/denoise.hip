@@ -554,13 +554,21 @@ __global__ void k_pack(const float* __restrict__ beauty, const float* __restrict
     const int yy = int(t % Hp);
     const size_t b = t / Hp;
     const size_t s = ((b * h + min(yy, h - 1)) * w + min(xx, w - 1)) * 3;
-    uint16_t* o = x + p * 16;
+    uint16_t v[16];
+#pragma unroll
     for (int c = 0; c < 3; ++c) {
-      o[c] = f2bf(log1pf(fmaxf(beauty[s + c], 0.f)));
-      o[3 + c] = f2bf(albedo[s + c]);
-      o[6 + c] = f2bf(normal[s + c]);
+      v[c] = f2bf(log1pf(fmaxf(beauty[s + c], 0.f)));
+      v[3 + c] = f2bf(albedo[s + c]);
+      v[6 + c] = f2bf(normal[s + c]);
     }
-    for (int c = 9; c < 16; ++c) o[c] = 0;
+#pragma unroll
+    for (int c = 9; c < 16; ++c) v[c] = 0;
+    uint32_t u[8];
+#pragma unroll
+    for (int k = 0; k < 8; ++k) u[k] = uint32_t(v[2 * k]) | (uint32_t(v[2 * k + 1]) << 16);
+    uint4* o = reinterpret_cast<uint4*>(x + p * 16);  // the pixel's 32 bytes in two 16-byte stores
+    o[0] = make_uint4(u[0], u[1], u[2], u[3]);
+    o[1] = make_uint4(u[4], u[5], u[6], u[7]);
   }
 }
 
