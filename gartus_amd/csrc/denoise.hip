@@ -481,49 +481,79 @@ __global__ __launch_bounds__(256) void k_reduce(const float* __restrict__ partia
 
 // ---- pooling, upsampling, concatenation -----------------------------------
 
+// Pieces of 8 channels (16 bytes): every channel count and offset here is a
+// multiple of 16.
+__device__ __forceinline__ void unpack8(uint4 v, float (&f)[8]) {
+  const uint32_t u[4] = {v.x, v.y, v.z, v.w};
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    f[2 * k] = bf2f(uint16_t(u[k]));
+    f[2 * k + 1] = bf2f(uint16_t(u[k] >> 16));
+  }
+}
+__device__ __forceinline__ uint4 pack8(const float (&f)[8]) {
+  uint32_t u[4];
+#pragma unroll
+  for (int k = 0; k < 4; ++k) u[k] = uint32_t(f2bf(f[2 * k])) | (uint32_t(f2bf(f[2 * k + 1])) << 16);
+  return make_uint4(u[0], u[1], u[2], u[3]);
+}
+
 __global__ void k_pool_bwd(const uint16_t* __restrict__ x, const uint16_t* __restrict__ dy,
                            const uint16_t* __restrict__ add, int add_c, int add_off, uint16_t* __restrict__ dx, int c,
-                           int H, int W, size_t n) {
-  const int H2 = H / 2, W2 = W / 2;
-  for (size_t i = blockIdx.x * size_t(blockDim.x) + threadIdx.x; i < n; i += size_t(gridDim.x) * blockDim.x) {
-    const int ch = int(i % c);
-    const size_t p = i / c;
+                           int H, int W, size_t n8) {
+  const int H2 = H / 2, W2 = W / 2, c8 = c / 8;
+  for (size_t i = blockIdx.x * size_t(blockDim.x) + threadIdx.x; i < n8; i += size_t(gridDim.x) * blockDim.x) {
+    const int ch = int(i % c8) * 8;
+    const size_t p = i / c8;
     const int w = int(p % W);
     const size_t t = p / W;
     const int h = int(t % H);
     const size_t b = t / H;
-    // the window's first maximum in scan order takes the gradient
+    // per channel, the window's first maximum in scan order takes the gradient
     const size_t base = ((b * H + (h & ~1)) * W + (w & ~1)) * c + ch;
     const size_t cand[4] = {base, base + c, base + size_t(W) * c, base + size_t(W) * c + c};
-    int arg = 0;
-    float m = bf2f(x[cand[0]]);
+    float m[8], v[8];
+    int arg[8];
+    unpack8(*reinterpret_cast<const uint4*>(x + cand[0]), m);
+#pragma unroll
+    for (int e = 0; e < 8; ++e) arg[e] = 0;
+#pragma unroll
     for (int k = 1; k < 4; ++k) {
-      const float v = bf2f(x[cand[k]]);
-      if (v > m) { m = v; arg = k; }
+      unpack8(*reinterpret_cast<const uint4*>(x + cand[k]), v);
+#pragma unroll
+      for (int e = 0; e < 8; ++e)
+        if (v[e] > m[e]) { m[e] = v[e]; arg[e] = k; }
     }
     const int mine = (h & 1) * 2 + (w & 1);
-    float g = 0.f;
-    if (mine == arg) g = bf2f(dy[((b * H2 + h / 2) * W2 + w / 2) * c + ch]);
-    if (add) g += bf2f(add[p * add_c + add_off + ch]);
-    dx[i] = f2bf(g);
+    float g[8], up[8], ad[8];
+    unpack8(*reinterpret_cast<const uint4*>(dy + ((b * H2 + h / 2) * W2 + w / 2) * c + ch), up);
+    if (add) unpack8(*reinterpret_cast<const uint4*>(add + p * add_c + add_off + ch), ad);
+#pragma unroll
+    for (int e = 0; e < 8; ++e) {
+      g[e] = mine == arg[e] ? up[e] : 0.f;
+      if (add) g[e] += ad[e];
+    }
+    *reinterpret_cast<uint4*>(dx + p * c + ch) = pack8(g);
   }
 }
 
 __global__ void k_upcat_fwd(const uint16_t* __restrict__ a, int ca, const uint16_t* __restrict__ bsrc, int cb,
-                            uint16_t* __restrict__ out, int H, int W, size_t n) {
-  const int c = ca + cb;
-  for (size_t i = blockIdx.x * size_t(blockDim.x) + threadIdx.x; i < n; i += size_t(gridDim.x) * blockDim.x) {
-    const int ch = int(i % c);
-    const size_t p = i / c;
-    if (ch < ca) {
+                            uint16_t* __restrict__ out, int H, int W, size_t n8) {
+  const int c8 = (ca + cb) / 8;
+  for (size_t i = blockIdx.x * size_t(blockDim.x) + threadIdx.x; i < n8; i += size_t(gridDim.x) * blockDim.x) {
+    const int ch = int(i % c8) * 8;
+    const size_t p = i / c8;
+    uint4 v;
+    if (ch < ca) {  // the half-resolution input, nearest upsampling
       const int w = int(p % W);
       const size_t t = p / W;
       const int h = int(t % H);
       const size_t b = t / H;
-      out[i] = a[((b * (H / 2) + h / 2) * (W / 2) + w / 2) * ca + ch];
+      v = *reinterpret_cast<const uint4*>(a + ((b * (H / 2) + h / 2) * (W / 2) + w / 2) * ca + ch);
     } else {
-      out[i] = bsrc[p * cb + (ch - ca)];
+      v = *reinterpret_cast<const uint4*>(bsrc + p * cb + (ch - ca));
     }
+    *reinterpret_cast<uint4*>(out + i * 8) = v;
   }
 }
 
@@ -747,13 +777,14 @@ void conv_wgrad(hipStream_t s, const uint16_t* x, int cin, const uint16_t* dy, c
 
 void pool_bwd(hipStream_t s, const uint16_t* x, const uint16_t* dy, const uint16_t* add, int add_c, int add_off,
               uint16_t* dx, int c, Dims d) {
-  const size_t n = size_t(d.pixels()) * c;
-  hipLaunchKernelGGL(k_pool_bwd, dim3(grid_for(n)), dim3(256), 0, s, x, dy, add, add_c, add_off, dx, c, d.h, d.w, n);
+  const size_t n8 = size_t(d.pixels()) * (c / 8);
+  hipLaunchKernelGGL(k_pool_bwd, dim3(grid_for(n8)), dim3(256), 0, s, x, dy, add, add_c, add_off, dx, c, d.h, d.w,
+                     n8);
 }
 
 void upcat_fwd(hipStream_t s, const uint16_t* a, int ca, const uint16_t* b, int cb, uint16_t* out, Dims d) {
-  const size_t n = size_t(d.pixels()) * (ca + cb);
-  hipLaunchKernelGGL(k_upcat_fwd, dim3(grid_for(n)), dim3(256), 0, s, a, ca, b, cb, out, d.h, d.w, n);
+  const size_t n8 = size_t(d.pixels()) * ((ca + cb) / 8);
+  hipLaunchKernelGGL(k_upcat_fwd, dim3(grid_for(n8)), dim3(256), 0, s, a, ca, b, cb, out, d.h, d.w, n8);
 }
 
 void upcat_bwd(hipStream_t s, const uint16_t* dout, int ca, int cb, uint16_t* da, Dims d) {
