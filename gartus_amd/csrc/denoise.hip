@@ -300,7 +300,7 @@ template <int NCO, int NCI>
 __global__ __launch_bounds__(256) void k_wgrad(const uint16_t* __restrict__ x, int cin,
                                                const uint16_t* __restrict__ dy, const uint16_t* __restrict__ mask,
                                                int cout, int H, int W, int nbands, int ncog,
-                                               float* __restrict__ partial, float* __restrict__ bpartial) {
+                                               float* __restrict__ partial) {
   extern __shared__ uint4 wlds4[];
   uint16_t* dT = reinterpret_cast<uint16_t*>(wlds4);
   uint16_t* xT = dT + NCO * 16 * kSD;
@@ -395,7 +395,7 @@ __global__ __launch_bounds__(256) void k_wgrad(const uint16_t* __restrict__ x, i
   // waves 1..3 hand their sums to wave 0 through LDS, one (a, b) pair at a time
   float* red = reinterpret_cast<float*>(wlds4);
   const size_t nw = size_t(cout) * 9 * cin;
-  float* out = partial + size_t(blockIdx.x) * nw;
+  float* out = partial + size_t(blockIdx.x) * (nw + cout);  // this split's row: weights, then bias
 #pragma unroll
   for (int a = 0; a < NCO; ++a)
 #pragma unroll
@@ -441,18 +441,18 @@ __global__ __launch_bounds__(256) void k_wgrad(const uint16_t* __restrict__ x, i
           v += red[((0 * NCO + a) * 4 + r) * 64 + lane];
           v += red[((1 * NCO + a) * 4 + r) * 64 + lane];
           v += red[((2 * NCO + a) * 4 + r) * 64 + lane];
-          bpartial[size_t(blockIdx.x) * cout + co0 + a * 16 + (lane >> 4) * 4 + r] = v;
+          out[nw + co0 + a * 16 + (lane >> 4) * 4 + r] = v;
         }
     }
   }
 }
 
-// out[i] = sum over s < splits of partial[s * n + i]. A block of 256 threads
+// out[i] = sum over s < splits of partial[s * stride + i], i < n. A block of 256 threads
 // owns 32 consecutive outputs; its 8 thread groups sum the splits congruent
 // to their group index mod 8, in increasing order, and group sums are added
 // in group order: a fixed order, so results are deterministic.
 __global__ __launch_bounds__(256) void k_reduce(const float* __restrict__ partial, int splits, size_t n,
-                                                float* __restrict__ out) {
+                                                size_t stride, float* __restrict__ out) {
   __shared__ float part[8][33];
   const int o = threadIdx.x & 31, g = threadIdx.x >> 5;
   const size_t i = blockIdx.x * size_t(32) + o;
@@ -460,14 +460,14 @@ __global__ __launch_bounds__(256) void k_reduce(const float* __restrict__ partia
   if (i < n) {
     int k = g;
     for (; k + 24 < splits; k += 32) {  // four independent loads in flight
-      const float a = partial[size_t(k) * n + i], b = partial[size_t(k + 8) * n + i];
-      const float c = partial[size_t(k + 16) * n + i], d = partial[size_t(k + 24) * n + i];
+      const float a = partial[size_t(k) * stride + i], b = partial[size_t(k + 8) * stride + i];
+      const float c = partial[size_t(k + 16) * stride + i], d = partial[size_t(k + 24) * stride + i];
       acc += a;
       acc += b;
       acc += c;
       acc += d;
     }
-    for (; k < splits; k += 8) acc += partial[size_t(k) * n + i];
+    for (; k < splits; k += 8) acc += partial[size_t(k) * stride + i];
   }
   part[g][o] = acc;
   __syncthreads();
@@ -724,7 +724,7 @@ int wgrad_splits(Dims d) { return d.n * ((d.h + kWR - 1) / kWR); }
 
 template <int NCO, int NCI>
 void launch_wgrad(hipStream_t s, const uint16_t* x, int cin, const uint16_t* dy, const uint16_t* mask, int cout,
-                  Dims d, float* pw, float* pb) {
+                  Dims d, float* partial) {
   constexpr size_t lds = wgrad_lds_bytes<NCO, NCI>();
   static const hipError_t attr =  // once per instantiation: allow more than 64 KiB of LDS
       hipFuncSetAttribute(reinterpret_cast<const void*>(&k_wgrad<NCO, NCI>),
@@ -733,7 +733,7 @@ void launch_wgrad(hipStream_t s, const uint16_t* x, int cin, const uint16_t* dy,
   const int ncog = cout / (NCO * 16), ncig = cin / (NCI * 16);
   const int nbands = (d.h + kWR - 1) / kWR;
   hipLaunchKernelGGL((k_wgrad<NCO, NCI>), dim3(d.n * nbands, ncog * ncig), dim3(256), lds, s, x, cin, dy, mask, cout,
-                     d.h, d.w, nbands, ncog, pw, pb);
+                     d.h, d.w, nbands, ncog, partial);
 }
 
 }  // namespace
@@ -765,14 +765,18 @@ void conv_wgrad(hipStream_t s, const uint16_t* x, int cin, const uint16_t* dy, c
   const int splits = wgrad_splits(d);
   const size_t nw = size_t(cout) * 9 * cin;
   if (scratch_floats < size_t(splits) * (nw + cout)) return;  // sized by wgrad_scratch_floats
-  float* pw = scratch;
-  float* pb = scratch + size_t(splits) * nw;
   // two output-channel fragments per block when the layer has an even number of them, one input
   // fragment: 2 x 2 needs 400 registers (one wave per SIMD) and was 5 % slower on one MI355X
-  if ((cout / 16) % 2 == 0) launch_wgrad<2, 1>(s, x, cin, dy, y, cout, d, pw, pb);
-  else launch_wgrad<1, 1>(s, x, cin, dy, y, cout, d, pw, pb);
-  hipLaunchKernelGGL(k_reduce, dim3(unsigned((nw + 31) / 32)), dim3(256), 0, s, pw, splits, nw, dw);
-  hipLaunchKernelGGL(k_reduce, dim3(unsigned((cout + 31) / 32)), dim3(256), 0, s, pb, splits, size_t(cout), db);
+  if ((cout / 16) % 2 == 0) launch_wgrad<2, 1>(s, x, cin, dy, y, cout, d, scratch);
+  else launch_wgrad<1, 1>(s, x, cin, dy, y, cout, d, scratch);
+  const size_t row = nw + cout;  // a split's partial sums: weights, then bias
+  if (db == dw + nw) {  // the usual parameter layout: one reduction for both
+    hipLaunchKernelGGL(k_reduce, dim3(unsigned((row + 31) / 32)), dim3(256), 0, s, scratch, splits, row, row, dw);
+  } else {
+    hipLaunchKernelGGL(k_reduce, dim3(unsigned((nw + 31) / 32)), dim3(256), 0, s, scratch, splits, nw, row, dw);
+    hipLaunchKernelGGL(k_reduce, dim3(unsigned((size_t(cout) + 31) / 32)), dim3(256), 0, s, scratch + nw, splits,
+                       size_t(cout), row, db);
+  }
 }
 
 void pool_bwd(hipStream_t s, const uint16_t* x, const uint16_t* dy, const uint16_t* add, int add_c, int add_off,
