@@ -174,6 +174,31 @@ def one_core(scene, cam, lights, env, W, H, seconds: float) -> dict:
             "sample": f"frame 0, {w}x{rows + 1} pixels around the middle, {spent:.1f} s"}
 
 
+def time_progressive(ctx, scene, cam, lights, env, wh, tile: int, reps: int) -> dict:
+    """Wall time of gt_render_progressive (render_world*_tiled_progressive,
+    camera.rs:3172-3294) into a host canvas, one tile callback at a time, after
+    one untimed call."""
+    from gartus_amd import abi
+    W, H = wh
+    canvas = np.zeros((H, W, 3), np.uint8)
+    count = [0]
+
+    def on_tile(user, update):
+        count[0] += 1
+        return 0
+
+    fn = abi.PROGRESS_FN(on_tile)
+    ptr = canvas.ctypes.data_as(C.POINTER(C.c_uint8))
+    args = (ctx.handle, scene, C.byref(cam), lights, env, tile, fn, None, ptr)
+    abi.check(ctx.lib.gt_render_progressive(*args))
+    count[0] = 0
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        abi.check(ctx.lib.gt_render_progressive(*args))
+    ms = (time.perf_counter() - t0) / reps * 1e3
+    return {"tile": tile, "ms_per_frame": round(ms, 3), "tiles": count[0] // reps}
+
+
 def cpu_model() -> str:
     try:
         for line in open("/proc/cpuinfo"):
@@ -264,6 +289,10 @@ def main() -> int:
     ap.add_argument("--bvh-width", type=int, default=0)
     ap.add_argument("--max-paths", type=int, default=0)
     ap.add_argument("--graph", action="store_true", help="replay the render as a captured hipGraph")
+    ap.add_argument("--adaptive", nargs=3, type=float, metavar=("MIN", "MAX", "THRESHOLD"),
+                    help="single-frame workloads: render with adaptive sampling (camera.rs:2937-2982)")
+    ap.add_argument("--progressive", type=int, default=0, metavar="TILE",
+                    help="single-frame workloads: also time gt_render_progressive with tiles of TILE pixels")
     ap.add_argument("--device-build", action="store_true",
                     help="compile the scenes on the GPU (gt_scene_build_device) instead of the host SAH build")
     ap.add_argument("--autotune", action="store_true", help="pick the tuning with gt_autotune (batch64: always)")
@@ -323,6 +352,13 @@ def main() -> int:
         lights_h[i] = fr["lights"]._compile(ctx) if fr["lights"] is not None else None
         envs_h[i] = fr["env"]._compile(ctx) if fr["env"] is not None else None
         cams[i] = scenes.camera_record(fr["cam"])
+        if args.adaptive:  # with_adaptive_sampling (camera.rs:1244-1264): spp becomes max
+            mn = max(int(args.adaptive[0]), 1)
+            mx = max(int(args.adaptive[1]), mn)
+            cams[i].adaptive_min_samples, cams[i].adaptive_max_samples = mn, mx
+            cams[i].adaptive_error_threshold = args.adaptive[2]
+            cams[i].samples_per_pixel = mx
+            cams[i].pixel_sample_mode = abi.GT_PIXEL_RANDOM
         W, H = cams[i].image_width, lib.gt_camera_image_height(C.byref(cams[i]))
         spp = lib.gt_camera_effective_spp(C.byref(cams[i]))
         npaths += W * H * spp
@@ -389,6 +425,15 @@ def main() -> int:
     achieved = (ext_bytes / launches) / (avg_launch_ms * 1e-3) / 1e9
     measured = traffic_for(args.workload)
     traffic = measured.get("bytes_per_launch")
+    if args.adaptive:  # paths traced, not W * H * max
+        npaths = ref.camera_paths
+    progressive = None
+    if args.progressive and n == 1:
+        progressive = time_progressive(ctx, scenes_h[0], cams[0], lights_h[0], envs_h[0], frames[0]["wh"],
+                                       args.progressive, max(args.steps, 1))
+        progressive["plain_ms_per_frame"] = round(wall / args.steps * 1e3, 3)
+        log(f"progressive, tiles of {args.progressive}: {progressive['ms_per_frame']:.2f} ms per frame "
+            f"({progressive['tiles']} tile callbacks), plain {progressive['plain_ms_per_frame']:.2f} ms")
     W0, H0 = frames[0]["wh"]
     out = np.zeros(W0 * H0 * 3, np.float32)
     abi.check(lib.gt_memcpy_to_host(ctx.handle, out.ctypes.data_as(C.c_void_p), outs[0], out.nbytes))
@@ -408,6 +453,13 @@ def main() -> int:
                       max_depth=int(cam0.max_depth))
     if tuned:
         config["autotuned"] = tuned
+    if args.adaptive:
+        config["adaptive"] = {"min_samples": int(cam0.adaptive_min_samples),
+                              "max_samples": int(cam0.adaptive_max_samples),
+                              "threshold": float(cam0.adaptive_error_threshold),
+                              "camera_paths": int(ref.camera_paths), "accepted_samples": int(ref.accepted_samples)}
+    if progressive:
+        config["progressive"] = progressive
     result = {
         "metric": "Mrays/s",
         "value": round(value, 2),

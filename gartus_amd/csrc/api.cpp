@@ -888,6 +888,8 @@ void add_stats(gt_render_stats* t, const gt_render_stats& r) {
 // the sequential loop would; samples of a round after a pixel's stop are
 // traced but unused. Paths depend only on (pixel, sample), so the image does
 // not depend on the rounds.
+constexpr uint64_t kMinRoundPaths = uint64_t(1) << 22;  // adaptive rounds: enough paths to fill one MI355X
+
 int render_adaptive(gt_context* ctx, FrameJob job, const gt_camera* cam, int out_is_double,
                     gt_render_stats* stats, const uint32_t* first_map = nullptr, uint32_t first_n = 0) {
   const uint32_t npix = job.f.npix;
@@ -934,6 +936,12 @@ int render_adaptive(gt_context* ctx, FrameJob job, const gt_camera* cam, int out
     map = ctx->ad_map[cur].p;
     cur ^= 1;
     k = std::min(mx - s, std::max(mn, s / 2));
+    // A round of few paths leaves most of the chip idle for its fixed cost of ~30 launches: it
+    // takes at least kMinRoundPaths paths (or as many as the first round), even if that traces
+    // samples a pixel will not use. Each pixel still stops at the same sample.
+    const uint64_t floor_paths = std::min<uint64_t>(kMinRoundPaths, uint64_t(npix) * mn);
+    const uint64_t want = (floor_paths + nact - 1) / nact;
+    k = uint32_t(std::min<uint64_t>(mx - s, std::max<uint64_t>(k, want)));
   }
   if (job.d_out)
     gtk::adapt_finalize(L, ctx->ad_mean.p, first_map, first_map ? first_n : npix, job.d_out, out_is_double);
