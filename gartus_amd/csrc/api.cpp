@@ -63,7 +63,7 @@ struct gt_context {
   uint32_t cus = 0;
   gt_tuning tuning{};
   // wavefront state: per-path records and path-id queues
-  DevBuf<float4> ray_o, ray_d, thr, s_o[2], s_d[2], s_w[2], radiance;
+  DevBuf<float4> ray_o, ray_d, thr, s_d[2], s_w[2], radiance;
   DevBuf<float2> hits;
   DevBuf<uint32_t> active[2], cat[gtk::kNumCategories], shadow[2];
   DevBuf<uint32_t> ctr;
@@ -467,7 +467,7 @@ int ensure_queues(gt_context* ctx, size_t paths, uint32_t max_depth) {
     const size_t lists = region * gtk::kSlices;
     GT_HIP(ctx->ray_o.ensure(paths)); GT_HIP(ctx->ray_d.ensure(paths)); GT_HIP(ctx->thr.ensure(paths));
     for (int k = 0; k < 2; ++k) {
-      GT_HIP(ctx->s_o[k].ensure(paths)); GT_HIP(ctx->s_d[k].ensure(paths)); GT_HIP(ctx->s_w[k].ensure(paths));
+      GT_HIP(ctx->s_d[k].ensure(paths)); GT_HIP(ctx->s_w[k].ensure(paths));
       GT_HIP(ctx->active[k].ensure(lists)); GT_HIP(ctx->shadow[k].ensure(lists));
     }
     GT_HIP(ctx->radiance.ensure(paths));
@@ -494,7 +494,7 @@ gtk::Queues queues_of(gt_context* ctx) {
   gtk::Queues q;
   q.ray_o = ctx->ray_o.p; q.ray_d = ctx->ray_d.p; q.thr = ctx->thr.p;
   for (int k = 0; k < 2; ++k) {
-    q.s_o[k] = ctx->s_o[k].p; q.s_d[k] = ctx->s_d[k].p; q.s_w[k] = ctx->s_w[k].p;
+    q.s_d[k] = ctx->s_d[k].p; q.s_w[k] = ctx->s_w[k].p;
     q.active[k] = ctx->active[k].p; q.shadow[k] = ctx->shadow[k].p;
   }
   q.hits = ctx->hits.p;

@@ -338,19 +338,25 @@ __device__ __forceinline__ Item load_item(const Queues& q, uint32_t p) {
 // shadow ray, 4 environment shadow ray).
 enum { OUT_EXT = 1, OUT_SL = 2, OUT_SE = 4 };
 
-__device__ __forceinline__ uint32_t continue_path(const Queues& q, uint32_t p, f3 pt, float time, f3 dir,
-                                                  bool scattered, f3 T, float mw) {
-  q.ray_o[p] = make_float4(pt.x, pt.y, pt.z, time);
+// The continuation's direction and throughput; its origin is the vertex,
+// written by the caller.
+__device__ __forceinline__ uint32_t continue_dir(const Queues& q, uint32_t p, f3 dir, bool scattered, f3 T,
+                                                 float mw) {
   q.ray_d[p] = make_float4(dir.x, dir.y, dir.z, __uint_as_float(scattered ? 0x80000000u : 0u));
   q.thr[p] = make_float4(T.x, T.y, T.z, mw);
   return OUT_EXT;
+}
+__device__ __forceinline__ uint32_t continue_path(const Queues& q, uint32_t p, f3 pt, float time, f3 dir,
+                                                  bool scattered, f3 T, float mw) {
+  q.ray_o[p] = make_float4(pt.x, pt.y, pt.z, time);
+  return continue_dir(q, p, dir, scattered, T, mw);
 }
 
 // Lambertian (material.rs:508-532) or GGX (:1583-1616) vertex: light NEE,
 // environment NEE, then the continuation sample (camera.rs:1779-1857).
 template <bool GGX>
-__device__ __forceinline__ uint32_t shade_scatter(const Frame& f, const Queues& q, const Item& it, const GMat& m, f3 p,
-                                                  f3 n, Rng& rng, bool last, uint32_t bounce) {
+__device__ __forceinline__ uint32_t scatter_vertex(const Frame& f, const Queues& q, const Item& it, const GMat& m,
+                                                   f3 p, f3 n, Rng& rng, bool last, uint32_t bounce) {
   const bool has_env = f.env.kind != 0;
   uint32_t out = 0;
   MatPdf pdf;
@@ -374,7 +380,6 @@ __device__ __forceinline__ uint32_t shade_scatter(const Frame& f, const Queues& 
         float spdf = scattering_pdf(m, it.d, n, ldir);
         if (isfinite(spdf) && spdf > 0.f) {
           f3 w = mul(T, color) * (spdf / lpdf);
-          q.s_o[0][it.path] = make_float4(p.x, p.y, p.z, it.time);
           q.s_d[0][it.path] = make_float4(ldir.x, ldir.y, ldir.z, 0.f);
           q.s_w[0][it.path] = make_float4(w.x, w.y, w.z, 0.f);
           out |= OUT_SL;
@@ -388,7 +393,6 @@ __device__ __forceinline__ uint32_t shade_scatter(const Frame& f, const Queues& 
         if (isfinite(spdf) && spdf > 0.f) {
           float mis = power_heuristic(es.pdf, spdf);
           f3 w = mul(T, mul(color, env_sample_radiance(f.env, es)) * (mis * spdf / es.pdf));
-          q.s_o[1][it.path] = make_float4(p.x, p.y, p.z, it.time);
           q.s_d[1][it.path] = make_float4(es.dir.x, es.dir.y, es.dir.z, 0.f);
           q.s_w[1][it.path] = make_float4(w.x, w.y, w.z, 0.f);
           out |= OUT_SE;
@@ -424,8 +428,16 @@ __device__ __forceinline__ uint32_t shade_scatter(const Frame& f, const Queues& 
   // and evaluated by the miss shading in extend.
   float next_w = weight_env ? -cpdf : 1.0f;
   T = mul(T, color * (spdf / cpdf));
-  if (rr_survives(bounce, f.cam.rr_min_depth, &T, rng))
-    out |= continue_path(q, it.path, p, it.time, dir, suppress, T, next_w);
+  if (rr_survives(bounce, f.cam.rr_min_depth, &T, rng)) out |= continue_dir(q, it.path, dir, suppress, T, next_w);
+  return out;
+}
+// The vertex also becomes the origin of its shadow rays and continuation: one
+// write serves all three (connect reads it before the next extend does).
+template <bool GGX>
+__device__ __forceinline__ uint32_t shade_scatter(const Frame& f, const Queues& q, const Item& it, const GMat& m, f3 p,
+                                                  f3 n, Rng& rng, bool last, uint32_t bounce) {
+  const uint32_t out = scatter_vertex<GGX>(f, q, it, m, p, n, rng, last, bounce);
+  if (out) q.ray_o[it.path] = make_float4(p.x, p.y, p.z, it.time);
   return out;
 }
 
@@ -534,7 +546,6 @@ __global__ __attribute__((amdgpu_waves_per_eu(8))) void k_connect(Frame f0, Batc
                                                                    uint32_t kind) {
   extern __shared__ uint2 smem[];
   Stack st = make_stack(q, smem);
-  const float4* __restrict__ so = q.s_o[kind];
   const float4* __restrict__ sd = q.s_d[kind];
   const float4* __restrict__ sw = q.s_w[kind];
   for (uint32_t s = blockIdx.x % slice_step(); s < kSlices; s += slice_step()) {
@@ -553,7 +564,7 @@ __global__ __attribute__((amdgpu_waves_per_eu(8))) void k_connect(Frame f0, Batc
       const uint32_t p = valid ? list[j] : 0u;
       with_frame<MULTI>(f0, bt, valid, p, [&](const Frame& f, const Seg&) {
       Stats s{0, 0, 0, 0, 0, 0};
-      float4 o4 = so[p], d4 = sd[p], w4 = sw[p];
+      float4 o4 = q.ray_o[p], d4 = sd[p], w4 = sw[p];  // the origin is the shading vertex
       f3 o = mk(o4.x, o4.y, o4.z), d = mk(d4.x, d4.y, d4.z);
       float t;
       uint32_t prim;
