@@ -63,7 +63,8 @@ struct gt_context {
   uint32_t cus = 0;
   gt_tuning tuning{};
   // wavefront state: per-path records and path-id queues
-  DevBuf<float4> ray_o, ray_d, thr, s_d[2], s_w[2], radiance;
+  DevBuf<float4> ray_o, ray_d, thr, s_d[2], radiance;
+  DevBuf<float2> s_w[2];
   DevBuf<float2> hits;
   DevBuf<uint32_t> active[2], cat[gtk::kNumCategories], shadow[2];
   DevBuf<uint32_t> ctr;

@@ -52,8 +52,8 @@ struct Queues {
   float4* ray_d;    // direction, flags (bit 31: last event was a scattering event)
   float4* thr;      // throughput, miss-radiance weight
   float2* hits;     // t, reordered primitive index (0xffffffff = miss)
-  float4* s_d[2];   // shadow rays: [0] light targets, [1] environment; their origin is ray_o
-  float4* s_w[2];   // contribution weight
+  float4* s_d[2];   // shadow rays: [0] light targets, [1] environment: direction, weight x; origin ray_o
+  float2* s_w[2];   // weight y, z
   float4* radiance; // per-path radiance L
   uint32_t* active[2];            // paths to extend at bounce b, parity b & 1 (bounce 0: all paths)
   uint32_t* cat[kNumCategories];  // paths hit this bounce, per shading category

@@ -380,8 +380,8 @@ __device__ __forceinline__ uint32_t scatter_vertex(const Frame& f, const Queues&
         float spdf = scattering_pdf(m, it.d, n, ldir);
         if (isfinite(spdf) && spdf > 0.f) {
           f3 w = mul(T, color) * (spdf / lpdf);
-          q.s_d[0][it.path] = make_float4(ldir.x, ldir.y, ldir.z, 0.f);
-          q.s_w[0][it.path] = make_float4(w.x, w.y, w.z, 0.f);
+          q.s_d[0][it.path] = make_float4(ldir.x, ldir.y, ldir.z, w.x);
+          q.s_w[0][it.path] = make_float2(w.y, w.z);
           out |= OUT_SL;
         }
       }
@@ -393,8 +393,8 @@ __device__ __forceinline__ uint32_t scatter_vertex(const Frame& f, const Queues&
         if (isfinite(spdf) && spdf > 0.f) {
           float mis = power_heuristic(es.pdf, spdf);
           f3 w = mul(T, mul(color, env_sample_radiance(f.env, es)) * (mis * spdf / es.pdf));
-          q.s_d[1][it.path] = make_float4(es.dir.x, es.dir.y, es.dir.z, 0.f);
-          q.s_w[1][it.path] = make_float4(w.x, w.y, w.z, 0.f);
+          q.s_d[1][it.path] = make_float4(es.dir.x, es.dir.y, es.dir.z, w.x);
+          q.s_w[1][it.path] = make_float2(w.y, w.z);
           out |= OUT_SE;
         }
       }
@@ -547,7 +547,7 @@ __global__ __attribute__((amdgpu_waves_per_eu(8))) void k_connect(Frame f0, Batc
   extern __shared__ uint2 smem[];
   Stack st = make_stack(q, smem);
   const float4* __restrict__ sd = q.s_d[kind];
-  const float4* __restrict__ sw = q.s_w[kind];
+  const float2* __restrict__ sw = q.s_w[kind];
   for (uint32_t s = blockIdx.x % slice_step(); s < kSlices; s += slice_step()) {
   const uint32_t count = q.ctr[ctr_at(bounce, kind == 0 ? S_SL_COUNT : S_SE_COUNT, s)];
   if (count == 0) continue;
@@ -564,7 +564,9 @@ __global__ __attribute__((amdgpu_waves_per_eu(8))) void k_connect(Frame f0, Batc
       const uint32_t p = valid ? list[j] : 0u;
       with_frame<MULTI>(f0, bt, valid, p, [&](const Frame& f, const Seg&) {
       Stats s{0, 0, 0, 0, 0, 0};
-      float4 o4 = q.ray_o[p], d4 = sd[p], w4 = sw[p];  // the origin is the shading vertex
+      const float4 o4 = q.ray_o[p], d4 = sd[p];  // the origin is the shading vertex
+      const float2 w2 = sw[p];
+      const f3 wgt = mk(d4.w, w2.x, w2.y);
       f3 o = mk(o4.x, o4.y, o4.z), d = mk(d4.x, d4.y, d4.z);
       float t;
       uint32_t prim;
@@ -578,12 +580,12 @@ __global__ __attribute__((amdgpu_waves_per_eu(8))) void k_connect(Frame f0, Batc
           const GMat& m = f.scene.mats[mid];
           if (m.kind == GM_DIFFUSE_LIGHT && dot(d, outward) < 0.f) {
             f3 em = ld3(m.color);
-            if (finite3(em)) { add = mul(mk(w4.x, w4.y, w4.z), em); contributes = true; }
+            if (finite3(em)) { add = mul(wgt, em); contributes = true; }
           }
         }
       } else {
         if (!trace<true, COUNT>(f.scene, o, d, o4.w, SHADOW_ACNE, INF, st, &t, &prim, nullptr, &s)) {
-          add = mk(w4.x, w4.y, w4.z);
+          add = wgt;
           contributes = true;
         }
       }
