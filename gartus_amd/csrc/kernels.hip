@@ -276,10 +276,9 @@ __global__ __attribute__((amdgpu_waves_per_eu(8))) void k_extend(Frame f0, Batch
                                        st, &t, &prim, nullptr, &s, &meta);
         if (hit) {
           q.hits[p] = make_float2(t, __uint_as_float(prim));
-          if (CAMERA) {  // the path's state: throughput 1, MIS weight 1, no radiance yet
-            q.thr[p] = make_float4(1.f, 1.f, 1.f, 1.f);
-            q.radiance[p] = make_float4(0.f, 0.f, 0.f, 0.f);
-          }
+          // a camera path has no radiance yet; its throughput and MIS weight are 1, which shade
+          // assumes at bounce 0 without reading them
+          if (CAMERA) q.radiance[p] = make_float4(0.f, 0.f, 0.f, 0.f);
           cats[k] = category_of(meta);
         } else {
           // a miss ends the path here (camera.rs:1747-1750): background or environment, times the
@@ -318,9 +317,10 @@ struct Item {
   float time, t;
   uint32_t prim;
 };
-__device__ __forceinline__ Item load_item(const Queues& q, uint32_t p) {
+__device__ __forceinline__ Item load_item(const Queues& q, uint32_t p, uint32_t bounce) {
   Item it;
-  float4 o4 = q.ray_o[p], d4 = q.ray_d[p], t4 = q.thr[p];
+  // a camera path (bounce 0) starts with throughput 1 and MIS weight 1, not stored
+  float4 o4 = q.ray_o[p], d4 = q.ray_d[p], t4 = bounce == 0 ? make_float4(1.f, 1.f, 1.f, 1.f) : q.thr[p];
   float2 h = q.hits[p];
   it.path = p;
   it.after_scatter = (__float_as_uint(d4.w) >> 31) != 0;
@@ -489,7 +489,7 @@ __device__ __forceinline__ void shade_category(const Frame& f0, const Batch& bt,
       ids[k] = p;
       outs[k] = 0;
       with_frame<MULTI>(f0, bt, valid, p, [&](const Frame& f, const Seg& sg) {
-        const Item it = load_item(q, p);
+        const Item it = load_item(q, p, bounce);
         f3 pt = it.o + it.t * it.d;
         uint32_t mid, kind;
         f3 outward = prim_outward_normal(f.scene.prims, it.prim, pt, it.time, &mid, &kind);
