@@ -196,6 +196,7 @@ PROTOTYPES = {
     "gt_scene_get_info": (I, [P, C.POINTER(GtSceneInfo)]),
     "gt_intersect": (I, [P, P, C.POINTER(GtRay), C.c_size_t, C.c_float, C.POINTER(GtHit),
                          C.POINTER(GtTraversalStats)]),
+    "gt_intersect_device_noalloc": (I, [P, P, P, C.c_size_t, C.c_float, P, P]),
     "gt_lights_create": (I, [P, PP]),
     "gt_lights_destroy": (I, [P]),
     "gt_lights_add_sphere": (I, [P, D, C.c_double, C.c_double]),

@@ -1039,6 +1039,10 @@ int gt_context_create(int device, gt_context** out) {
   ctx->device = device;
   GT_HIP(hipSetDevice(device));
   GT_HIP(hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking));
+  // Set-up of the no-allocation entry point: the address of this device's
+  // copy of its scratch array, looked up once per device. A failure is kept
+  // and reported by that entry point; the others do not need it.
+  (void)gtk::noalloc_scratch_init(device);
   hipDeviceProp_t props;
   GT_HIP(hipGetDeviceProperties(&props, device));
   ctx->cus = uint32_t(props.multiProcessorCount);
@@ -1225,6 +1229,15 @@ int gt_scene_get_info(const gt_scene* s, gt_scene_info* out) {
 }
 
 // ---- intersect ----
+static_assert(sizeof(gt_ray) == 2 * sizeof(float4) && sizeof(gt_hit) == 2 * sizeof(float4),
+              "rays and hits are two float4 records on the device");
+
+static void copy_stats(const unsigned long long v[7], gt_traversal_stats* stats) {
+  stats->rays = v[0]; stats->node_bounds_tests = v[1]; stats->node_bounds_hits = v[2];
+  stats->leaf_visits = v[3]; stats->primitive_candidates = v[4]; stats->leaf_hit_results = v[5];
+  stats->max_stack_depth = v[6];
+}
+
 int gt_intersect(gt_context* ctx, const gt_scene* cs, const gt_ray* rays, size_t n, float t_min, gt_hit* out,
                  gt_traversal_stats* stats) {
   int rc = check_objects(ctx, cs, nullptr, nullptr);
@@ -1241,54 +1254,70 @@ int gt_intersect(gt_context* ctx, const gt_scene* cs, const gt_ray* rays, size_t
   gtk::Queues q = queues_of(ctx);
   gtk::Launch L{grid_for(ctx), ctx->tuning.block_size, ctx->stream};
   const size_t chunk = size_t(1) << 22;
-  size_t m = std::min(n, chunk);
-  DevBuf<float4> dr;
-  DevBuf<float> dt, du, dv, dn;
-  DevBuf<int32_t> dp;
-  DevBuf<uint32_t> dfront;
+  const size_t m = std::min(n, chunk);
+  DevBuf<float4> dr, dh;
   DevBuf<unsigned long long> dstats;
-  GT_HIP(dr.ensure(2 * m)); GT_HIP(dt.ensure(m)); GT_HIP(du.ensure(m)); GT_HIP(dv.ensure(m));
-  GT_HIP(dn.ensure(3 * m)); GT_HIP(dp.ensure(m)); GT_HIP(dfront.ensure(m));
+  GT_HIP(dr.ensure(2 * m));
+  GT_HIP(dh.ensure(2 * m));
   if (stats) {
     GT_HIP(dstats.ensure(8));
-    GT_HIP(hipMemsetAsync(dstats.p, 0, 8 * sizeof(unsigned long long), ctx->stream));
+    gtk::zero_words(ctx->stream, dstats.p, 8);
   }
-  std::vector<float4> hr(2 * m);
-  std::vector<float> ht(m), hu(m), hv(m), hn(3 * m);
-  std::vector<int32_t> hp(m);
-  std::vector<uint32_t> hf(m);
   for (size_t off = 0; off < n; off += chunk) {
-    size_t c = std::min(chunk, n - off);
-    for (size_t i = 0; i < c; ++i) {
-      const gt_ray& r = rays[off + i];
-      hr[2 * i] = make_float4(r.origin[0], r.origin[1], r.origin[2], r.time);
-      hr[2 * i + 1] = make_float4(r.direction[0], r.direction[1], r.direction[2], r.t_max);
-    }
-    GT_HIP(hipMemcpyAsync(dr.p, hr.data(), 2 * c * sizeof(float4), hipMemcpyHostToDevice, ctx->stream));
-    gtk::IntersectOut o{dt.p, du.p, dv.p, dp.p, dn.p, dfront.p, stats ? dstats.p : nullptr};
-    gtk::intersect(L, f, q, dr.p, c, t_min, o);
+    const size_t c = std::min(chunk, n - off);
+    GT_HIP(hipMemcpyAsync(dr.p, rays + off, c * sizeof(gt_ray), hipMemcpyHostToDevice, ctx->stream));
+    gtk::intersect(L, f, q, dr.p, c, t_min, dh.p, stats ? dstats.p : nullptr);
     GT_HIP(hipGetLastError());
-    GT_HIP(hipMemcpyAsync(ht.data(), dt.p, c * 4, hipMemcpyDeviceToHost, ctx->stream));
-    GT_HIP(hipMemcpyAsync(hu.data(), du.p, c * 4, hipMemcpyDeviceToHost, ctx->stream));
-    GT_HIP(hipMemcpyAsync(hv.data(), dv.p, c * 4, hipMemcpyDeviceToHost, ctx->stream));
-    GT_HIP(hipMemcpyAsync(hn.data(), dn.p, 3 * c * 4, hipMemcpyDeviceToHost, ctx->stream));
-    GT_HIP(hipMemcpyAsync(hp.data(), dp.p, c * 4, hipMemcpyDeviceToHost, ctx->stream));
-    GT_HIP(hipMemcpyAsync(hf.data(), dfront.p, c * 4, hipMemcpyDeviceToHost, ctx->stream));
+    GT_HIP(hipMemcpyAsync(out + off, dh.p, c * sizeof(gt_hit), hipMemcpyDeviceToHost, ctx->stream));
     GT_HIP(hipStreamSynchronize(ctx->stream));
-    for (size_t i = 0; i < c; ++i) {
-      gt_hit& h = out[off + i];
-      h.t = ht[i]; h.u = hu[i]; h.v = hv[i]; h.prim_id = hp[i];
-      h.normal[0] = hn[3 * i]; h.normal[1] = hn[3 * i + 1]; h.normal[2] = hn[3 * i + 2];
-      h.front_face = hf[i];
-    }
   }
   if (stats) {
     unsigned long long v[8];
     GT_HIP(hipMemcpy(v, dstats.p, sizeof v, hipMemcpyDeviceToHost));
-    stats->rays = v[0]; stats->node_bounds_tests = v[1]; stats->node_bounds_hits = v[2];
-    stats->leaf_visits = v[3]; stats->primitive_candidates = v[4]; stats->leaf_hit_results = v[5];
-    stats->max_stack_depth = v[6];
+    copy_stats(v, stats);
   }
+  return GT_OK;
+}
+
+int gt_intersect_device_noalloc(gt_context* ctx, const gt_scene* s, const gt_ray* d_rays, size_t n, float t_min,
+                                gt_hit* d_hits, gt_traversal_stats* d_stats) {
+  int rc = check_objects(ctx, s, nullptr, nullptr);
+  if (rc != GT_OK) return rc;
+  if (!s->built) return fail(GT_ESTATE, "scene is not built: gt_intersect_device_noalloc does not build or "
+                                        "upload (call gt_scene_build or gt_scene_build_device first)");
+  if (n != 0 && (!d_rays || !d_hits)) return fail(GT_EINVAL, "null rays or hits");
+  GT_HIP(hipSetDevice(ctx->device));
+  uint2* scratch = nullptr;
+  const hipError_t se = gtk::noalloc_scratch(ctx->device, &scratch);
+  if (se != hipSuccess || !scratch)
+    return fail(GT_EDEVICE, std::string("no-allocation scratch unavailable on this device: ") + hipGetErrorString(se));
+  gtk::Frame f;
+  std::memset(&f, 0, sizeof f);
+  f.scene = device_scene(s, ctx->tuning.bvh_width);
+  // Overflow levels of the per-lane stack beyond the LDS entries; the grid
+  // shrinks until lanes * levels fits the scratch, down to one block per CU.
+  const uint32_t cap = stack_cap(ctx), block = ctx->tuning.block_size;
+  const uint64_t need = stack_depth(s, f.scene.width);
+  const uint64_t levels = need > cap ? need - cap : 1;
+  const uint64_t fit_blocks = gtk::kNoAllocStackEntries / (levels * block);
+  if (fit_blocks < ctx->cus) {
+    const uint64_t max_levels = gtk::kNoAllocStackEntries / (uint64_t(ctx->cus) * block);
+    return fail(GT_EINVAL, "BVH too deep for the no-allocation scratch: traversal needs " + std::to_string(need) +
+                               " stack entries per ray, at most " + std::to_string(max_levels + cap) +
+                               " fit at block_size " + std::to_string(block) + " (use gt_intersect)");
+  }
+  gtk::Queues q{};
+  q.gstack = scratch;
+  q.stack_cap = cap;
+  const uint32_t grid = uint32_t(std::min<uint64_t>(grid_for(ctx), fit_blocks));
+  q.gstride = grid * block;
+  gtk::Launch L{grid, block, ctx->stream};
+  unsigned long long* st = reinterpret_cast<unsigned long long*>(d_stats);
+  if (st) gtk::zero_words(ctx->stream, st, 7);
+  if (n != 0) {
+    gtk::intersect(L, f, q, reinterpret_cast<const float4*>(d_rays), n, t_min, reinterpret_cast<float4*>(d_hits), st);
+  }
+  GT_HIP(hipGetLastError());
   return GT_OK;
 }
 

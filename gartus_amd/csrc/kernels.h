@@ -153,17 +153,23 @@ void finalize_map(const Launch& l, const double4* acc, const uint32_t* map, uint
                   int out_is_double);
 void aov(const Launch& l, const Frame& f, const Queues& q, float* albedo, float* normal);
 
-struct IntersectOut {
-  float* t;
-  float* u;
-  float* v;
-  int32_t* prim;
-  float* normal;
-  uint32_t* front;
-  unsigned long long* stats;  // 6 sums + max stack (7 words), or null
-};
+// Closest-hit queries: rays are gt_ray records (origin, time; direction,
+// t_max), hits gt_hit records (t, u, v, prim_id; normal, front_face), both
+// two float4 per entry. stats, when set, receives the BvhTraversalStats sums
+// (6 words) and the largest stack depth (1 word), added to what it holds.
 void intersect(const Launch& l, const Frame& f, const Queues& q, const float4* rays, size_t n, float tmin,
-               IntersectOut out);
+               float4* hits, unsigned long long* stats);
+// Sets n 64-bit words to zero with a kernel (no memset node under capture).
+void zero_words(hipStream_t stream, unsigned long long* p, uint32_t n);
+
+// Scratch of the no-allocation closest-hit entry point: the traversal-stack
+// overflow lives in a file-scope device array of kNoAllocStackEntries
+// entries, one copy per GPU. noalloc_scratch_init looks its address up on
+// the current device (once per device; call it with `device` current);
+// noalloc_scratch returns that address afterwards.
+constexpr size_t kNoAllocStackEntries = size_t(1) << 21;  // 16 MiB of 8-byte entries
+hipError_t noalloc_scratch_init(int device);
+hipError_t noalloc_scratch(int device, uint2** out);
 
 // Resident blocks per CU of a kernel: 0 extend, 1 shade (GGX), 2 connect.
 uint32_t occupancy_blocks(int kernel, uint32_t block, size_t lds_bytes);

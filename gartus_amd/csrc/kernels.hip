@@ -22,10 +22,17 @@
 #include "kernels.h"
 
 #include <algorithm>
+#include <mutex>
 
 #include "../../include/gartus_hip.h"
 
 using namespace gtd;
+
+// Traversal-stack overflow of gt_intersect_device_noalloc, which allocates
+// nothing: kNoAllocStackEntries 8-byte entries (16 MiB) per GPU, held until
+// the process ends. Every call on a GPU shares that GPU's copy, so calls that
+// overlap on one GPU race.
+static __device__ __align__(256) uint2 scratch[gtk::kNoAllocStackEntries];
 
 namespace gtk {
 
@@ -790,11 +797,16 @@ __global__ void k_aov(Frame f, Queues q, float* albedo, float* normal) {
 }
 
 // Closest-hit queries with the hit record of HitRecord::from_surface
-// (object.rs:435-450) and optional BvhTraversalStats (bvh.rs:9-52).
+// (object.rs:435-450) and optional BvhTraversalStats (bvh.rs:9-52). A ray
+// and a hit are two float4 each (gt_ray, gt_hit); the statistics are summed
+// per lane, then once per wave.
 template <bool STATS>
-__global__ void k_intersect(Frame f, Queues q, const float4* rays, size_t n, float tmin, IntersectOut out) {
+__global__ void k_intersect(Frame f, Queues q, const float4* __restrict__ rays, size_t n, float tmin,
+                            float4* __restrict__ hits, unsigned long long* stats) {
   extern __shared__ uint2 smem[];
   Stack st = make_stack(q, smem);
+  unsigned long long sums[6] = {0, 0, 0, 0, 0, 0};
+  uint32_t deepest = 0;
   for (size_t i = blockIdx.x * size_t(blockDim.x) + threadIdx.x; i < n; i += size_t(gridDim.x) * blockDim.x) {
     float4 o4 = rays[2 * i], d4 = rays[2 * i + 1];
     f3 o = mk(o4.x, o4.y, o4.z), d = mk(d4.x, d4.y, d4.z);
@@ -816,25 +828,36 @@ __global__ void k_intersect(Frame f, Queues q, const float4* rays, size_t n, flo
         u = phi / (2.0f * PI);
         v = theta / PI;
       }
-      out.t[i] = t; out.u[i] = u; out.v[i] = v;
-      out.prim[i] = int32_t(f.scene.prim_ids[prim]);
-      out.normal[3 * i] = nn.x; out.normal[3 * i + 1] = nn.y; out.normal[3 * i + 2] = nn.z;
-      out.front[i] = front;
+      hits[2 * i] = make_float4(t, u, v, __int_as_float(int32_t(f.scene.prim_ids[prim])));
+      hits[2 * i + 1] = make_float4(nn.x, nn.y, nn.z, __uint_as_float(front ? 1u : 0u));
     } else {
-      out.t[i] = 0.f; out.u[i] = 0.f; out.v[i] = 0.f; out.prim[i] = -1;
-      out.normal[3 * i] = 0.f; out.normal[3 * i + 1] = 0.f; out.normal[3 * i + 2] = 0.f;
-      out.front[i] = 0;
+      hits[2 * i] = make_float4(0.f, 0.f, 0.f, __int_as_float(-1));
+      hits[2 * i + 1] = make_float4(0.f, 0.f, 0.f, 0.f);
     }
     if (STATS) {
-      atomicAdd(&out.stats[0], 1ull);
-      atomicAdd(&out.stats[1], (unsigned long long)s.node_tests);
-      atomicAdd(&out.stats[2], (unsigned long long)s.node_hits);
-      atomicAdd(&out.stats[3], (unsigned long long)s.leaf_visits);
-      atomicAdd(&out.stats[4], (unsigned long long)s.candidates);
-      atomicAdd(&out.stats[5], (unsigned long long)s.leaf_hits);
-      atomicMax(&out.stats[6], (unsigned long long)s.max_stack);
+      sums[0] += 1;
+      sums[1] += s.node_tests;
+      sums[2] += s.node_hits;
+      sums[3] += s.leaf_visits;
+      sums[4] += s.candidates;
+      sums[5] += s.leaf_hits;
+      deepest = max(deepest, s.max_stack);
     }
   }
+  if (STATS) {  // every lane of the wave is back here: reduce, then one atomic per word
+#pragma unroll
+    for (int k = 0; k < 6; ++k) {
+      unsigned long long v = sums[k];
+      for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off);
+      if (lane_id() == 0 && v) atomicAdd(&stats[k], v);
+    }
+    for (int off = 32; off > 0; off >>= 1) deepest = max(deepest, uint32_t(__shfl_down(deepest, off)));
+    if (lane_id() == 0) atomicMax(&stats[6], (unsigned long long)deepest);
+  }
+}
+
+__global__ void k_zero_words(unsigned long long* p, uint32_t n) {
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) p[i] = 0ull;
 }
 
 }  // namespace
@@ -932,13 +955,47 @@ void aov(const Launch& l, const Frame& f, const Queues& q, float* albedo, float*
                      albedo, normal);
 }
 void intersect(const Launch& l, const Frame& f, const Queues& q, const float4* rays, size_t n, float tmin,
-               IntersectOut out) {
-  if (out.stats)
+               float4* hits, unsigned long long* stats) {
+  if (stats)
     hipLaunchKernelGGL(k_intersect<true>, dim3(l.grid), dim3(l.block), stack_lds_bytes(l.block, q.stack_cap),
-                       l.stream, f, q, rays, n, tmin, out);
+                       l.stream, f, q, rays, n, tmin, hits, stats);
   else
     hipLaunchKernelGGL(k_intersect<false>, dim3(l.grid), dim3(l.block), stack_lds_bytes(l.block, q.stack_cap),
-                       l.stream, f, q, rays, n, tmin, out);
+                       l.stream, f, q, rays, n, tmin, hits, stats);
+}
+void zero_words(hipStream_t stream, unsigned long long* p, uint32_t n) {
+  hipLaunchKernelGGL(k_zero_words, dim3((n + 255) / 256), dim3(256), 0, stream, p, n);
+}
+
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+bool g_scratch_done[kMaxDevices];
+hipError_t g_scratch_err[kMaxDevices];
+uint2* g_scratch_ptr[kMaxDevices];
+}  // namespace
+
+// The array's name in host code is a stand-in for the symbol, not a device
+// address; each device's copy is reached only through the address looked up
+// here with that device current.
+hipError_t noalloc_scratch_init(int device) {
+  if (device < 0 || device >= kMaxDevices) return hipErrorInvalidDevice;
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_done[device]) {
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    g_scratch_err[device] = e;
+    g_scratch_ptr[device] = e == hipSuccess ? static_cast<uint2*>(p) : nullptr;
+    g_scratch_done[device] = true;
+  }
+  return g_scratch_err[device];
+}
+hipError_t noalloc_scratch(int device, uint2** out) {
+  if (device < 0 || device >= kMaxDevices) return hipErrorInvalidDevice;
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_done[device]) return hipErrorNotInitialized;
+  *out = g_scratch_ptr[device];
+  return g_scratch_err[device];
 }
 
 }  // namespace gtk

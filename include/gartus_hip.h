@@ -213,6 +213,26 @@ typedef struct gt_traversal_stats {
 } gt_traversal_stats;
 int gt_intersect(gt_context* ctx, const gt_scene* scene, const gt_ray* rays, size_t n,
                  float t_min, gt_hit* out_hits, gt_traversal_stats* out_stats /* nullable */);
+/* No counterpart in the reference. gt_intersect for callers that cannot
+ * allocate device memory: code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc. It allocates nothing (no hipMalloc,
+ * hipFree or allocating library call) and does not synchronise: rays, hits
+ * and stats are device pointers, and the results land on the context's
+ * stream (gt_context_synchronize before reading them). Same kernels and
+ * results as gt_intersect, bit for bit.
+ *   - The scene must already be built (GT_ESTATE otherwise).
+ *   - d_stats (nullable) is a device gt_traversal_stats; it is cleared by a
+ *     kernel, then summed into.
+ *   - Its only scratch, the traversal-stack overflow, is a fixed device array
+ *     of 2^21 8-byte entries (16 MiB) per GPU, held until the process ends.
+ *     Deeper trees run on fewer blocks; a tree needing more than
+ *     2^21 / (CUs * block_size) levels beyond the LDS stack is rejected with
+ *     GT_EINVAL (at 256 CUs and 128 threads: 64 levels).
+ *   - Every call on a GPU shares that GPU's array: calls that overlap on one
+ *     GPU (two contexts or streams at once) race. */
+int gt_intersect_device_noalloc(gt_context* ctx, const gt_scene* scene, const gt_ray* d_rays,
+                                size_t n, float t_min, gt_hit* d_hits,
+                                gt_traversal_stats* d_stats /* nullable, device */);
 
 /* ---- light targets (SamplingTargetList / WeightedSamplingTargetList,
  *      scene.rs:363-609) ------------------------------------------------- */
