@@ -72,6 +72,24 @@ def test_random_double_and_pixel_seed_match_numpy_restatement():
         assert int(L.cr_pixel_seed(1, x, y)) == int(_np_mix(z))
 
 
+def test_pixel_seed_lands_on_published_splitmix64_outputs():
+    """pixel_seed (camera.rs:1720-1727) is the SplitMix64 output function
+    applied to seed ^ x K1 ^ y K2, with K1, K2 read off that source. SplitMix64
+    seeded with 1234567 first outputs mix(1234567 + gamma) (its published
+    vector), so a seed chosen to make seed ^ x K1 ^ y K2 equal to
+    1234567 + gamma must give that published output, and likewise for the
+    second state 1234567 + 2 gamma. This pins the finalizer's shifts and
+    multipliers and the way x and y enter, without restating them."""
+    M = (1 << 64) - 1
+    gamma, k1, k2 = 0x9E3779B97F4A7C15, 0x9E3779B97F4A7C15, 0xD1B54A32D192ED03
+    published = [6457827717110365317, 3203168211198807973]
+    for j, want in enumerate(published):
+        state = (1234567 + (j + 1) * gamma) & M
+        for x, y in [(0, 0), (1, 0), (0, 1), (1919, 1079), (77, 4)]:
+            seed = state ^ ((x * k1) & M) ^ ((y * k2) & M)
+            assert int(L.cr_pixel_seed(seed, x, y)) == want, (j, x, y)
+
+
 def test_counter_streams_are_distinct_and_deterministic():
     key = int(L.cr_pixel_seed(1, 3, 4))
     seeds = {int(L.cr_stream_seed(key, s, d)) for s in range(64) for d in range(11)}
