@@ -187,10 +187,13 @@ PROTOTYPES = {
     "gt_scene_add_ggx": (I, [P, C.c_double, C.c_double, C.c_double, C.c_double, U32]),
     "gt_scene_add_metal": (I, [P, C.c_double, C.c_double, C.c_double, C.c_double, U32]),
     "gt_scene_add_dielectric": (I, [P, C.c_double, U32]),
+    "gt_scene_add_isotropic": (I, [P, C.c_double, C.c_double, C.c_double, U32]),
+    "gt_scene_add_henyey_greenstein": (I, [P, C.c_double, C.c_double, C.c_double, C.c_double, U32]),
     "gt_scene_add_spheres": (I, [P, D, C.c_size_t, U32]),
     "gt_scene_add_moving_spheres": (I, [P, D, C.c_size_t, U32]),
     "gt_scene_add_triangles": (I, [P, D, C.c_size_t, U32]),
     "gt_scene_add_quads": (I, [P, D, C.c_size_t, U32]),
+    "gt_scene_add_mesh_triangles": (I, [P, D, D, C.c_size_t, U32]),
     "gt_scene_build": (I, [P, C.c_uint32]),
     "gt_scene_build_device": (I, [P, C.c_uint32]),
     "gt_scene_get_info": (I, [P, C.POINTER(GtSceneInfo)]),
@@ -211,6 +214,8 @@ PROTOTYPES = {
     "gt_render_hdr": (I, [P, P, C.POINTER(GtCamera), P, P, F, C.POINTER(GtRenderStats)]),
     "gt_render_hdr_device": (I, [P, P, C.POINTER(GtCamera), P, P, P, I, C.POINTER(GtRenderStats)]),
     "gt_render_aovs": (I, [P, P, C.POINTER(GtCamera), P, P, F, F, F, C.POINTER(GtRenderStats)]),
+    "gt_render_pixels_device": (I, [P, P, C.POINTER(GtCamera), P, P, U32, C.c_size_t, P, I,
+                                    C.POINTER(GtRenderStats)]),
     "gt_render_aovs_device": (I, [P, P, C.POINTER(GtCamera), P, P, F, F, F, C.POINTER(GtRenderStats)]),
     "gt_render_batch": (I, [P, C.c_size_t, PP, C.POINTER(GtCamera), PP, PP, C.POINTER(F),
                             C.POINTER(GtRenderStats)]),

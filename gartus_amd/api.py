@@ -15,14 +15,15 @@ from __future__ import annotations
 
 import ctypes as C
 import math
+import weakref
 from dataclasses import dataclass, replace
 from typing import Optional
 
 import numpy as np
 
 from . import abi
-from .scene_data import (DIELECTRIC, DIFFUSE_LIGHT, GGX, LAMBERTIAN, METAL, MOVING_SPHERE, QUAD, SPHERE,
-                         TRIANGLE, Material, SceneData)
+from .scene_data import (DIELECTRIC, DIFFUSE_LIGHT, GGX, HENYEY_GREENSTEIN, ISOTROPIC, LAMBERTIAN, MESH_TRIANGLE, METAL,
+                         MOVING_SPHERE, QUAD, SPHERE, TRIANGLE, Material, SceneData)
 
 # ---------------------------------------------------------------- context ---
 
@@ -30,13 +31,19 @@ _CONTEXT = None
 
 
 class Context:
-    """One HIP device and stream (gt_context)."""
+    """One HIP device and stream (gt_context). Scenes, light lists and
+    environments compiled on it belong to it: closing the context releases
+    them first (a native object must not outlive its context)."""
 
     def __init__(self, device: int = 0):
         self.lib = abi.load()
         h = C.c_void_p()
         abi.check(self.lib.gt_context_create(device, C.byref(h)))
         self.handle = h
+        self._owned = weakref.WeakSet()
+
+    def _adopt(self, obj) -> None:
+        self._owned.add(obj)
 
     @property
     def tuning(self) -> abi.GtTuning:
@@ -58,6 +65,8 @@ class Context:
 
     def close(self) -> None:
         if self.handle:
+            for obj in list(self._owned):
+                obj._release()
             self.lib.gt_context_destroy(self.handle)
             self.handle = None
 
@@ -115,6 +124,24 @@ class RayMaterial:
         return Material(METAL, c, min(max(float(fuzz), 0.0), 1.0))
 
     @staticmethod
+    def isotropic(albedo) -> Material:
+        """Isotropic::new (material.rs:1061-1066)."""
+        c = _v3(albedo)
+        if not all(math.isfinite(x) for x in c):
+            raise abi.InvalidArgument(abi.GT_EINVAL, "isotropic albedo must be finite")
+        return Material(ISOTROPIC, c)
+
+    @staticmethod
+    def henyey_greenstein(albedo, g: float) -> Material:
+        """HenyeyGreenstein::new (material.rs:1194-1204)."""
+        c = _v3(albedo)
+        if not all(math.isfinite(x) for x in c):
+            raise abi.InvalidArgument(abi.GT_EINVAL, "Henyey-Greenstein albedo must be finite")
+        if not (math.isfinite(g) and abs(g) < 1.0):
+            raise abi.InvalidArgument(abi.GT_EINVAL, "Henyey-Greenstein anisotropy must be finite and in (-1, 1)")
+        return Material(HENYEY_GREENSTEIN, c, float(g))
+
+    @staticmethod
     def dielectric(refraction_index: float) -> Material:
         return Material(DIELECTRIC, (1.0, 1.0, 1.0), float(refraction_index))
 
@@ -159,6 +186,19 @@ class RayScene:
 
     def add_triangles(self, material, triangles) -> None:
         self._add(TRIANGLE, triangles, material)
+
+    def add_mesh_triangle(self, p0, p1, p2, n0, n1, n2, material: int) -> None:
+        """A TriangleMesh triangle with vertex normals (MeshTriangle::with_vertex_normals,
+        mesh.rs:40-50): shaded with the interpolated normal (mesh.rs:105-119)."""
+        self._add(MESH_TRIANGLE, [*_v3(p0), *_v3(p1), *_v3(p2), *_v3(n0), *_v3(n1), *_v3(n2)], material)
+
+    def add_mesh_triangles(self, material, positions, normals) -> None:
+        """Many mesh triangles: positions (n, 9) and vertex normals (n, 9)."""
+        pos = np.asarray(positions, np.float64).reshape(-1, 9)
+        nrm = np.asarray(normals, np.float64).reshape(-1, 9)
+        if len(pos) != len(nrm):
+            raise abi.InvalidArgument(abi.GT_EINVAL, "one set of vertex normals per triangle")
+        self._add(MESH_TRIANGLE, np.concatenate([pos, nrm], axis=1), material)
 
     def add_quad(self, corner, u, v, material: int) -> None:
         self._add(QUAD, [*_v3(corner), *_v3(u), *_v3(v)], material)
@@ -228,6 +268,7 @@ class RayScene:
             h = C.c_void_p()
             abi.check(lib.gt_scene_create(ctx.handle, C.byref(h)))
             self._handle = h
+            ctx._adopt(self)
             mid = C.c_uint32()
             for m in self.data.materials:
                 r, g, b = m.color
@@ -239,12 +280,22 @@ class RayScene:
                     rc = lib.gt_scene_add_ggx(h, r, g, b, m.param, C.byref(mid))
                 elif m.kind == METAL:
                     rc = lib.gt_scene_add_metal(h, r, g, b, m.param, C.byref(mid))
+                elif m.kind == ISOTROPIC:
+                    rc = lib.gt_scene_add_isotropic(h, r, g, b, C.byref(mid))
+                elif m.kind == HENYEY_GREENSTEIN:
+                    rc = lib.gt_scene_add_henyey_greenstein(h, r, g, b, m.param, C.byref(mid))
                 else:
                     rc = lib.gt_scene_add_dielectric(h, m.param, C.byref(mid))
                 abi.check(rc)
             adders = {SPHERE: lib.gt_scene_add_spheres, MOVING_SPHERE: lib.gt_scene_add_moving_spheres,
                       TRIANGLE: lib.gt_scene_add_triangles, QUAD: lib.gt_scene_add_quads}
             for kind, data, mids in self.data.chunks:
+                if kind == MESH_TRIANGLE:
+                    pos = np.ascontiguousarray(data[:, :9])
+                    nrm = np.ascontiguousarray(data[:, 9:])
+                    abi.check(lib.gt_scene_add_mesh_triangles(h, _dp(pos), _dp(nrm), len(data),
+                                                              mids.ctypes.data_as(abi.U32)))
+                    continue
                 abi.check(adders[kind](h, _dp(data), len(data), mids.ctypes.data_as(abi.U32)))
             self._dirty = False
             if self._leaf_size is not None:
@@ -294,7 +345,12 @@ class RayScene:
         return out
 
     def close(self) -> None:
-        if self._handle is not None and self._ctx is not None:
+        self._release()
+
+    def _release(self) -> None:
+        """Destroys the native scene while its context is alive (a closed
+        context has released it already)."""
+        if self._handle is not None and self._ctx is not None and self._ctx.handle:
             self._ctx.lib.gt_scene_destroy(self._handle)
         self._handle = None
 
@@ -348,9 +404,11 @@ class WeightedSamplingTargetList:
         self._drop()
 
     def _drop(self) -> None:
-        if self._handle is not None:
+        if self._handle is not None and self._ctx is not None and self._ctx.handle:
             self._ctx.lib.gt_lights_destroy(self._handle)
         self._handle = None
+
+    _release = _drop
 
     def _compile(self, ctx: Context):
         if self._handle is not None and self._ctx is ctx:
@@ -373,6 +431,7 @@ class WeightedSamplingTargetList:
                 rc = ctx.lib.gt_lights_add_triangle(h, a, b, c, w)
             abi.check(rc)
         self._handle, self._ctx = h, ctx
+        ctx._adopt(self)
         return h
 
     def __del__(self):
@@ -427,13 +486,19 @@ class EnvironmentLight:
         else:
             hh, ww, _ = self.rgb.shape
             abi.check(ctx.lib.gt_env_create_rgb32f(ctx.handle, self.rgb.ctypes.data_as(abi.F), ww, hh, C.byref(h)))
+        self._release()
         self._handle, self._ctx = h, ctx
+        ctx._adopt(self)
         return h
+
+    def _release(self) -> None:
+        if self._handle is not None and self._ctx is not None and self._ctx.handle:
+            self._ctx.lib.gt_env_destroy(self._handle)
+        self._handle = None
 
     def __del__(self):
         try:
-            if self._handle is not None:
-                self._ctx.lib.gt_env_destroy(self._handle)
+            self._release()
         except Exception:
             pass
 
@@ -1081,3 +1146,36 @@ def render_batch(items, stats: bool = False):
     if stats:
         return result, {k: getattr(st, k) for k, _ in abi.GtRenderStats._fields_}
     return result
+
+
+def render_pixels(world: RayScene, camera: RayCamera, pixels, lights: Optional["SamplingTargetList"] = None,
+                  environment: Optional["EnvironmentLight"] = None, double: bool = True,
+                  stats: bool = False):
+    """gt_render_pixels_device: only the listed pixels (indices y * W + x) of
+    the camera's image, each equal to the same pixel of the full render.
+    Returns an (n, 3) array in list order (float64 by default), and the
+    render statistics when `stats`. The camera is used as given: a caller
+    that wants the PathTracer's forced next-event estimation for lights or
+    an environment sets it on the camera."""
+    ctx = world.context
+    sh = world._compile()
+    lh = lights._compile(ctx) if lights is not None else None
+    eh = environment._compile(ctx) if environment is not None else None
+    px = np.ascontiguousarray(np.asarray(pixels, dtype=np.uint32).reshape(-1))
+    w, h = camera.image_width(), camera.image_height()
+    item = 8 if double else 4
+    buf = C.c_void_p()
+    abi.check(ctx.lib.gt_device_malloc(ctx.handle, w * h * 3 * item, C.byref(buf)))
+    try:
+        st = abi.GtRenderStats()
+        abi.check(ctx.lib.gt_render_pixels_device(ctx.handle, sh, C.byref(camera.raw), lh, eh,
+                                                  px.ctypes.data_as(abi.U32), len(px), buf, 1 if double else 0,
+                                                  C.byref(st) if stats else None))
+        img = np.zeros((h * w, 3), np.float64 if double else np.float32)
+        abi.check(ctx.lib.gt_memcpy_to_host(ctx.handle, img.ctypes.data_as(C.c_void_p), buf, img.nbytes))
+    finally:
+        ctx.lib.gt_device_free(ctx.handle, buf)
+    out = img[px.astype(np.int64)]
+    if stats:
+        return out, {k: getattr(st, k) for k, _ in abi.GtRenderStats._fields_}
+    return out

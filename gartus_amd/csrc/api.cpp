@@ -63,8 +63,7 @@ struct gt_context {
   uint32_t cus = 0;
   gt_tuning tuning{};
   // wavefront state: per-path records and path-id queues
-  DevBuf<float4> ray_o, ray_d, thr, s_d[2], radiance;
-  DevBuf<float2> s_w[2];
+  DevBuf<float4> ray_o, ray_d, thr, s_d[2], s_w[2], radiance;
   DevBuf<float2> hits;
   DevBuf<uint32_t> active[2], cat[gtk::kNumCategories], shadow[2];
   DevBuf<uint32_t> ctr;
@@ -85,9 +84,10 @@ struct gt_context {
   uint32_t* host_ctr = nullptr;  // pinned copy of the counters, one block per batch
   size_t host_ctr_words = 0;
   std::vector<hipEvent_t> events;
-  // the captured launch sequence of the last render, and what it was captured for
+  // the captured launch sequence of the last render, and what it was captured for;
+  // seen_key: the last render's key, so a sequence is captured only when it repeats
   hipGraphExec_t graph_exec = nullptr;
-  std::vector<unsigned char> graph_key;
+  std::vector<unsigned char> graph_key, seen_key;
   ~gt_context() {
     if (graph_exec) (void)hipGraphExecDestroy(graph_exec);
     for (hipEvent_t e : events) (void)hipEventDestroy(e);
@@ -109,6 +109,8 @@ struct gt_scene {
   DevBuf<GPrim> dprims;
   DevBuf<uint32_t> prim_ids;
   DevBuf<GMat> dmats;
+  std::vector<float4> vnormals;  // mesh triangles' vertex normals, 3 per triangle (HostPrim::smooth - 1)
+  DevBuf<float4> dvnormals;
   uint64_t device_bytes = 0;
   uint64_t n_nodes2 = 0, n_nodes4 = 0;  // node records the traversal may reference
   bool on_device = false;               // built by gt_scene_build_device
@@ -155,6 +157,13 @@ int upload_materials(gt_scene* s, size_t* bytes) {
     GT_HIP(hipMemcpyAsync(s->dmats.p, gm.data(), gm.size() * sizeof(GMat), hipMemcpyHostToDevice, s->ctx->stream));
   GT_HIP(hipStreamSynchronize(s->ctx->stream));
   *bytes = gm.size() * sizeof(GMat);
+  if (!s->vnormals.empty()) {
+    GT_HIP(s->dvnormals.ensure(s->vnormals.size()));
+    GT_HIP(hipMemcpyAsync(s->dvnormals.p, s->vnormals.data(), s->vnormals.size() * sizeof(float4),
+                          hipMemcpyHostToDevice, s->ctx->stream));
+    GT_HIP(hipStreamSynchronize(s->ctx->stream));
+    *bytes += s->vnormals.size() * sizeof(float4);
+  }
   return GT_OK;
 }
 
@@ -428,6 +437,7 @@ gtd::DScene device_scene(const gt_scene* s, uint32_t width) {
   d.nodes2 = s->nodes2.p;
   d.nodes4 = s->nodes4.p;
   d.prims = s->dprims.p;
+  d.vnormals = s->vnormals.empty() ? nullptr : s->dvnormals.p;
   d.prim_ids = s->prim_ids.p;
   d.mats = s->dmats.p;
   for (int a = 0; a < 3; ++a) { d.root_lo[a] = s->packed.root_lo[a]; d.root_hi[a] = s->packed.root_hi[a]; }
@@ -774,9 +784,16 @@ int run_jobs(gt_context* ctx, std::vector<FrameJob>& jobs, bool multi, int out_i
     put(&L, sizeof L);
     const void* tables[2] = {ctx->frame_table.p, ctx->seg_table.p};
     put(tables, sizeof tables);
-    const uint32_t flags[3] = {multi ? 1u : 0u, uint32_t(out_is_double), depth_all};
+    const uint32_t flags[4] = {multi ? 1u : 0u, uint32_t(out_is_double), depth_all, count ? 1u : 0u};
     put(flags, sizeof flags);
-    if (!ctx->graph_exec || key != ctx->graph_key) {
+    const bool cached = ctx->graph_exec && key == ctx->graph_key;
+    if (!cached && key != ctx->seen_key) {
+      // first sight of this sequence: run it eagerly (a one-off render would pay for a capture
+      // and an instantiation it never replays) and capture it if the next render repeats it
+      ctx->seen_key = std::move(key);
+      int erc = enqueue();
+      if (erc != GT_OK) return erc;
+    } else if (!cached) {
       if (ctx->graph_exec) GT_HIP(hipGraphExecDestroy(ctx->graph_exec));
       ctx->graph_exec = nullptr;
       ctx->graph_key.clear();
@@ -793,8 +810,10 @@ int run_jobs(gt_context* ctx, std::vector<FrameJob>& jobs, bool multi, int out_i
       (void)hipGraphDestroy(graph);
       GT_HIP(ie);
       ctx->graph_key = std::move(key);
+      GT_HIP(hipGraphLaunch(ctx->graph_exec, st));
+    } else {
+      GT_HIP(hipGraphLaunch(ctx->graph_exec, st));
     }
-    GT_HIP(hipGraphLaunch(ctx->graph_exec, st));
   } else {
     int erc = enqueue();
     if (erc != GT_OK) return erc;
@@ -1050,7 +1069,7 @@ int gt_context_create(int device, gt_context** out) {
   ctx->tuning.blocks_per_cu = 0;
   ctx->tuning.max_paths = 0;
   ctx->tuning.bvh_width = 2;
-  ctx->tuning.use_graph = 0;
+  ctx->tuning.use_graph = 1;
   *out = ctx.release();
   return GT_OK;
 }
@@ -1164,6 +1183,16 @@ int gt_scene_add_metal(gt_scene* s, double r, double g, double b, double fuzz, u
   if (!finite3(r, g, b) || !std::isfinite(fuzz)) return fail(GT_EINVAL, "metal material values must be finite");  // material.rs:1968
   return add_material(s, GM_METAL, r, g, b, std::min(std::max(fuzz, 0.0), 1.0), id);
 }
+int gt_scene_add_isotropic(gt_scene* s, double r, double g, double b, uint32_t* id) {
+  if (!finite3(r, g, b)) return fail(GT_EINVAL, "isotropic albedo must be finite");  // material.rs:1062
+  return add_material(s, GM_ISOTROPIC, r, g, b, 0.0, id);
+}
+int gt_scene_add_henyey_greenstein(gt_scene* s, double r, double g, double b, double anisotropy, uint32_t* id) {
+  if (!finite3(r, g, b)) return fail(GT_EINVAL, "Henyey-Greenstein albedo must be finite");  // material.rs:1195
+  if (!(std::isfinite(anisotropy) && std::fabs(anisotropy) < 1.0))  // validate_henyey_greenstein_g :1328-1333
+    return fail(GT_EINVAL, "Henyey-Greenstein anisotropy must be finite and in (-1, 1)");
+  return add_material(s, GM_HENYEY_GREENSTEIN, r, g, b, anisotropy, id);
+}
 int gt_scene_add_dielectric(gt_scene* s, double ior, uint32_t* id) {
   if (!std::isfinite(ior)) return fail(GT_EINVAL, "refraction index must be finite");
   return add_material(s, GM_DIELECTRIC, 1.0, 1.0, 1.0, ior, id);
@@ -1195,6 +1224,24 @@ int gt_scene_add_spheres(gt_scene* s, const double* d, size_t n, const uint32_t*
 int gt_scene_add_moving_spheres(gt_scene* s, const double* d, size_t n, const uint32_t* m) { return add_prims(s, GK_MOVING_SPHERE, 7, d, n, m); }
 int gt_scene_add_triangles(gt_scene* s, const double* d, size_t n, const uint32_t* m) { return add_prims(s, GK_TRIANGLE, 9, d, n, m); }
 int gt_scene_add_quads(gt_scene* s, const double* d, size_t n, const uint32_t* m) { return add_prims(s, GK_QUAD, 9, d, n, m); }
+int gt_scene_add_mesh_triangles(gt_scene* s, const double* positions, const double* normals, size_t n,
+                                const uint32_t* m) {
+  if (!s) return fail(GT_EINVAL, "scene is null");
+  if (n == 0) return GT_OK;
+  if (!normals) return fail(GT_EINVAL, "null vertex normals");
+  if (s->vnormals.size() / 3 + n >= (size_t(1) << 31)) return fail(GT_EINVAL, "too many mesh triangles");
+  const size_t first = s->prims.size();
+  int rc = add_prims(s, GK_TRIANGLE, 9, positions, n, m);
+  if (rc != GT_OK) return rc;
+  for (size_t i = 0; i < n; ++i) {
+    s->prims[first + i].smooth = uint32_t(s->vnormals.size() / 3 + 1);
+    for (int k = 0; k < 3; ++k) {
+      const double* v = normals + 9 * i + 3 * k;
+      s->vnormals.push_back(make_float4(float(v[0]), float(v[1]), float(v[2]), 0.0f));
+    }
+  }
+  return GT_OK;
+}
 
 int gt_scene_build(gt_scene* s, uint32_t leaf_size) {
   if (!s) return fail(GT_EINVAL, "scene is null");
@@ -1619,6 +1666,44 @@ int gt_render_progressive(gt_context* ctx, const gt_scene* scene, const gt_camer
         }
       }
   }
+  return GT_OK;
+}
+
+int gt_render_pixels_device(gt_context* ctx, const gt_scene* scene, const gt_camera* camera,
+                            const gt_lights* lights, const gt_env* env, const uint32_t* pixels, size_t n,
+                            void* d_out, int out_is_double, gt_render_stats* stats) {
+  if (!ctx) return fail(GT_EINVAL, "context is null");
+  GT_HIP(hipSetDevice(ctx->device));
+  FrameJob job;
+  int rc = prepare_job(ctx, scene, camera, lights, env, &job);
+  if (rc != GT_OK) return rc;
+  if (stats) std::memset(stats, 0, sizeof *stats);
+  if (n == 0) return GT_OK;
+  if (!pixels || !d_out) return fail(GT_EINVAL, "null pixel list or output");
+  const uint32_t npix = job.f.npix;
+  if (n > npix) return fail(GT_EINVAL, "more pixels listed than the image has");
+  // every listed pixel owns one accumulator: indices in range, each at most once
+  std::vector<uint8_t> seen(npix, 0);
+  for (size_t i = 0; i < n; ++i) {
+    const uint32_t px = pixels[i];
+    if (px >= npix) return fail(GT_EINVAL, "pixel index " + std::to_string(px) + " outside the image");
+    if (seen[px]) return fail(GT_EINVAL, "pixel index " + std::to_string(px) + " listed twice");
+    seen[px] = 1;
+  }
+  GT_HIP(ctx->band_map.ensure(n));
+  GT_HIP(hipMemcpyAsync(ctx->band_map.p, pixels, n * sizeof(uint32_t), hipMemcpyHostToDevice, ctx->stream));
+  GT_HIP(hipStreamSynchronize(ctx->stream));
+  job.d_out = d_out;
+  if (adaptive_camera(camera))
+    return render_adaptive(ctx, job, camera, out_is_double, stats, ctx->band_map.p, uint32_t(n));
+  job.f.pixmap = ctx->band_map.p;
+  job.f.nact = uint32_t(n);
+  job.d_out = nullptr;
+  std::vector<FrameJob> jobs(1, job);
+  if ((rc = run_jobs(ctx, jobs, false, out_is_double, stats)) != GT_OK) return rc;
+  gtk::Launch L{grid_for(ctx), ctx->tuning.block_size, ctx->stream};
+  gtk::finalize_map(L, jobs[0].f.acc, ctx->band_map.p, uint32_t(n), d_out, out_is_double);
+  GT_HIP(hipGetLastError());
   return GT_OK;
 }
 

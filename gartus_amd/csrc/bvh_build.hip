@@ -118,6 +118,7 @@ __device__ GPrim pack(const gt::HostPrim& p) {
         g.p0[i] = float(q[i]); g.p1[i] = float(e1[i]); g.p2[i] = float(e2[i]);
         g.p3[i] = len < eps ? 0.0f : float(n[i] / len);
       }
+      g.p1[3] = __uint_as_float(p.smooth);
       break;
     }
     default: {

@@ -120,11 +120,16 @@ __device__ __forceinline__ float ggx_alpha(float r) {  // sampling.rs:310-312
   float c = clampf(r, 0.f, 1.f);
   return fmaxf(c * c, 1.0e-4f);
 }
-__device__ __forceinline__ float ggx_D(float ndh, float roughness) {  // sampling.rs:229-241
+// sampling.rs:229-241. The reference's denominator cos^2 (a^2 - 1) + 1 is
+// evaluated as sin^2 + a^2 cos^2 with sin^2 = |n x h|^2 from the vectors:
+// the same value for unit n and h, but free of the cancellation that makes
+// the FP32 form useless near the peak of a smooth lobe (at roughness 0.02,
+// a^2 = 1.6e-7 is a couple of ulps of 1).
+__device__ __forceinline__ float ggx_D(float ndh, float sin2, float roughness) {
   if (!isfinite(ndh) || ndh <= 0.f) return 0.f;
   float alpha = ggx_alpha(roughness);
   float a2 = alpha * alpha;
-  float den = ndh * ndh * (a2 - 1.0f) + 1.0f;
+  float den = sin2 + a2 * (ndh * ndh);
   if (den <= EPS) return 0.f;
   return a2 / (PI * den * den);
 }
@@ -138,14 +143,30 @@ __device__ __forceinline__ float schlick(float c, float f0) {  // sampling.rs:25
   f0 = clampf(f0, 0.f, 1.f);
   return f0 + (1.0f - f0) * pow5(1.0f - clampf(c, 0.f, 1.f));
 }
-// GgxReflectionPdf (sampling.rs:174-308) and CosinePdf (:48-84)
+// HenyeyGreensteinPdf::phase_value (sampling.rs:126-136)
+__device__ __forceinline__ float hg_phase(float cosine, float g) {
+  if (!isfinite(cosine) || !isfinite(g) || fabsf(g) >= 1.0f) return 0.f;
+  const float den = 1.0f + g * g - 2.0f * g * clampf(cosine, -1.f, 1.f);
+  if (den <= EPS) return 0.f;
+  return (1.0f - g * g) / (4.0f * PI * (den * sqrtf(den)));
+}
+// The material pdfs (MaterialPdf, pdf.rs:46-104): CosinePdf (sampling.rs:48-84),
+// GgxReflectionPdf (:174-308), SpherePdf (:31-41), HenyeyGreensteinPdf (:91-166).
+enum { PDF_COSINE = 0, PDF_GGX = 1, PDF_SPHERE = 2, PDF_HG = 3 };
 struct MatPdf {
   Onb basis;
   f3 outgoing;
   float roughness, alpha;
-  bool ggx;
+  float g;
+  uint32_t kind;
   __device__ float value(f3 d) const {
-    if (!ggx) {
+    if (kind == PDF_SPHERE) return 1.0f / (4.0f * PI);
+    if (kind == PDF_HG) {
+      const float l = sqrtf(len2(d));
+      if (l <= EPS) return 0.f;
+      return hg_phase(dot(d, basis.w) / l, g);
+    }
+    if (kind == PDF_COSINE) {
       float l = sqrtf(len2(d));
       if (l <= EPS) return 0.f;
       float c = dot(d, basis.w) / l;
@@ -157,19 +178,36 @@ struct MatPdf {
     float odh = dot(outgoing, half);
     if (len2(half) <= EPS || odh <= EPS) return 0.f;
     float ndh = dot(half, basis.w);
-    return ggx_D(ndh, roughness) * fmaxf(ndh, 0.f) / (4.0f * fabsf(odh));
+    return ggx_D(ndh, len2(cross(basis.w, half)), roughness) * fmaxf(ndh, 0.f) / (4.0f * fabsf(odh));
   }
   // *below (when given) reports a GGX reflection below the surface, where
   // the reference falls back to the outgoing direction.
   __device__ f3 generate(Rng& rng, bool* below = nullptr) const {
-    if (!ggx) return basis.local(rng.cosine_direction());
+    if (kind == PDF_SPHERE) return rng.unit_vector_spherical();
+    if (kind == PDF_HG) {  // sampling.rs:149-165
+      const float r1 = rng.next(), r2 = rng.next();
+      float ct;
+      if (fabsf(g) < 1e-3f) {
+        ct = 1.0f - 2.0f * r1;
+      } else {
+        const float term = (1.0f - g * g) / (1.0f - g + 2.0f * g * r1);
+        ct = clampf((1.0f + g * g - term * term) / (2.0f * g), -1.f, 1.f);
+      }
+      const float st = sqrtf(fmaxf(1.0f - ct * ct, 0.f));
+      float sn, cs;
+      sincosf(2.0f * PI * r2, &sn, &cs);
+      return basis.local(mk(cs * st, sn * st, ct));
+    }
+    if (kind == PDF_COSINE) return basis.local(rng.cosine_direction());
     float r1 = rng.next(), r2 = rng.next();
     float phi = 2.0f * PI * r1;
     float a2 = alpha * alpha;
     float cr2 = fminf(r2, 0.99999994f);
     float tan2 = a2 * cr2 / (1.0f - cr2);
     float ct = 1.0f / sqrtf(1.0f + tan2);
-    float st = sqrtf(fmaxf(1.0f - ct * ct, 0.f));
+    // sin from tan: the reference's sqrt(1 - cos^2) cancels in FP32 when
+    // tan^2 is a few ulps of 1, which would quantise a smooth lobe's angles
+    float st = sqrtf(tan2) * ct;
     float s, c;
     sincosf(phi, &s, &c);
     f3 half = basis.local(mk(c * st, s * st, ct));
@@ -188,7 +226,7 @@ __device__ __forceinline__ float ggx_weight(f3 n, f3 o, f3 i, float roughness, f
   if (len2(half) <= EPS) return 0.f;
   float ndh = dot(n, half), odh = dot(o, half);
   if (ndh <= 0.f || odh <= 0.f) return 0.f;
-  float D = ggx_D(ndh, roughness);
+  float D = ggx_D(ndh, len2(cross(n, half)), roughness);
   float G = smith_g1(ndi, roughness) * smith_g1(ndo, roughness);
   f0 = clampf(f0, 0.f, 1.f);
   if (f0 <= EPS) return 0.f;
@@ -205,17 +243,27 @@ __device__ __forceinline__ uint32_t meta_of(const float4& p0) { return __float_a
 // hit_sphere_in_interval (object.rs:986-1013). The discriminant uses the
 // perpendicular-distance form a*(r^2 - |oc - (h/a) d|^2), algebraically equal
 // to h^2 - a*c and free of its cancellation in FP32.
-__device__ __forceinline__ bool sphere_hit(f3 c, float r, f3 o, f3 d, float tmin, float tmax, float* root) {
+//
+// from_surface: the ray starts on this sphere (its origin is a hit on it).
+// In exact arithmetic one root is then 0 and the other 2h/a, so the sphere
+// is met again only by a ray heading inward, at the far root. FP32 cannot
+// place the origin on a large sphere within t_min (on the 1000-radius
+// ground sphere a coordinate's ulp is 6e-5 and the near root's error grows
+// as 1/cos at grazing angles), so the near root is skipped by this rule, not
+// compared with t_min: the decision the reference's f64 arithmetic makes.
+__device__ __forceinline__ bool sphere_hit(f3 c, float r, f3 o, f3 d, float tmin, float tmax, float* root,
+                                           bool from_surface = false) {
   f3 oc = c - o;
   float a = len2(d);
   if (a <= EPS) return false;
   float h = dot(d, oc);
+  if (from_surface && h <= 0.f) return false;
   f3 perp = oc - (h / a) * d;
   float disc = a * (r * r - len2(perp));
   if (disc < 0.f) return false;
   float sq = sqrtf(disc);
   float first = (h - sq) / a;
-  if (tmin < first && first < tmax) { *root = first; return true; }
+  if (!from_surface && tmin < first && first < tmax) { *root = first; return true; }
   float second = (h + sq) / a;
   if (tmin < second && second < tmax) { *root = second; return true; }
   return false;
@@ -254,13 +302,18 @@ __device__ __forceinline__ bool quad_hit(f3 corner, f3 u, f3 v, f3 n, float inv_
   return true;
 }
 // One primitive record against a ray interval; returns the root only.
+// from_surface: the ray starts on this primitive. A ray leaving a plane
+// cannot meet it again (the reference's t is 0 up to rounding, below
+// t_min); a sphere is met again only at its far root (sphere_hit).
 __device__ __forceinline__ bool prim_test(const GPrim* __restrict__ prims, uint32_t k, f3 o, f3 d, float time,
-                                          float tmin, float tmax, PrimHit* h, uint32_t* meta_out) {
+                                          float tmin, float tmax, PrimHit* h, uint32_t* meta_out,
+                                          bool from_surface) {
   const float4* rec = reinterpret_cast<const float4*>(prims + k);
   float4 p0 = rec[0];
   uint32_t meta = meta_of(p0);
   *meta_out = meta;
   uint32_t kind = (meta >> GT_KIND_SHIFT) & GT_KIND_MASK;
+  if (from_surface && (kind == GK_TRIANGLE || kind == GK_QUAD)) return false;
   float4 p1 = rec[1];
   if (kind == GK_TRIANGLE) {
     float4 p2 = rec[2];
@@ -279,10 +332,107 @@ __device__ __forceinline__ bool prim_test(const GPrim* __restrict__ prims, uint3
     r = p1.w;
   }
   float root;
-  if (!sphere_hit(c, r, o, d, tmin, tmax, &root)) return false;
+  if (!sphere_hit(c, r, o, d, tmin, tmax, &root, from_surface)) return false;
   h->t = root; h->u = 0.f; h->v = 0.f;
   return true;
 }
+// ---- FP64 refinement of the chosen hit ---------------------------------------
+// The FP32 traversal decides which primitive a ray hits. Its distance, and
+// the surface coordinates, are then recomputed once per ray in FP64 with the
+// reference's own expressions (contraction off, as the reference evaluates
+// them): near the ray origin FP32 cannot resolve the distance to a surface
+// (a ray 5 cm above the 1000-radius ground sphere loses 1e-3 of its t), FP64
+// from the same FP32 inputs reproduces the reference's t to FP32 rounding.
+// If FP64 rejects the primitive (an edge or silhouette case where the two
+// precisions decide differently) the FP32 values stand.
+struct d3 {
+  double x, y, z;
+};
+__device__ __forceinline__ d3 dd(float4 v) { return {double(v.x), double(v.y), double(v.z)}; }
+__device__ __forceinline__ d3 dd(f3 v) { return {double(v.x), double(v.y), double(v.z)}; }
+__device__ __forceinline__ double ddot(d3 a, d3 b) {
+#pragma clang fp contract(off)
+  return a.x * b.x + a.y * b.y + a.z * b.z;
+}
+__device__ __forceinline__ d3 dsub(d3 a, d3 b) { return {a.x - b.x, a.y - b.y, a.z - b.z}; }
+__device__ __forceinline__ d3 dcross(d3 a, d3 b) {
+#pragma clang fp contract(off)
+  return {a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x};
+}
+__device__ __forceinline__ bool refine_hit(const GPrim* __restrict__ prims, uint32_t k, f3 of, f3 df, float time,
+                                        float tmin, PrimHit* h) {
+#pragma clang fp contract(off)
+  const float4* rec = reinterpret_cast<const float4*>(prims + k);
+  const float4 p0 = rec[0], p1 = rec[1];
+  const uint32_t kind = (meta_of(p0) >> GT_KIND_SHIFT) & GT_KIND_MASK;
+  const d3 o = dd(of), d = dd(df);
+  const double lo = double(tmin);
+  if (kind == GK_TRIANGLE) {  // geometry.rs:239-265
+    const float4 p2 = rec[2];
+    const d3 v0 = dd(p0), e1 = dd(p1), e2 = dd(p2);
+    const d3 pv = dcross(d, e2);
+    const double det = ddot(e1, pv);
+    if (fabs(det) < 1e-10) return false;
+    const double inv = 1.0 / det;
+    const d3 tv = dsub(o, v0);
+    const double u = ddot(tv, pv) * inv;
+    if (!(u >= 0.0 && u <= 1.0)) return false;
+    const d3 qv = dcross(tv, e1);
+    const double v = ddot(d, qv) * inv;
+    if (v < 0.0 || u + v > 1.0) return false;
+    const double t = ddot(e2, qv) * inv;
+    if (!(lo < t)) return false;
+    h->t = float(t); h->u = float(u); h->v = float(v);
+    return true;
+  }
+  if (kind == GK_QUAD) {  // QuadGeometry::new and hit_ray (geometry.rs:293-312,394-415)
+    const float4 p2 = rec[2];
+    const d3 c = dd(p0), u = dd(p1), v = dd(p2);
+    const d3 n = dcross(u, v);
+    const double nls = ddot(n, n);
+    if (nls <= 2.220446049250313e-16) return false;
+    const double len = sqrt(nls);
+    const d3 nn = {n.x / len, n.y / len, n.z / len};
+    const double pd = ddot(nn, c);
+    const double denom = ddot(nn, d);
+    if (fabs(denom) < 1e-8) return false;
+    const double t = (pd - ddot(nn, o)) / denom;
+    if (!(lo < t)) return false;
+    const d3 w = {n.x / nls, n.y / nls, n.z / nls};
+    const d3 pl = dsub({o.x + t * d.x, o.y + t * d.y, o.z + t * d.z}, c);
+    const double a = ddot(w, dcross(pl, v)), b = ddot(w, dcross(u, pl));
+    if (!(a >= 0.0 && a <= 1.0) || !(b >= 0.0 && b <= 1.0)) return false;
+    h->t = float(t); h->u = float(a); h->v = float(b);
+    return true;
+  }
+  // hit_sphere_in_interval (object.rs:986-1013): of the roots above t_min,
+  // the one the FP32 test chose (the nearer to its distance)
+  d3 c = dd(p0);
+  double r = double(p1.x);
+  if (kind == GK_MOVING_SPHERE) {  // geometry.rs:507-509
+    const d3 c1 = dd(p1);
+    const double tm = double(time);
+    c = {c.x + tm * (c1.x - c.x), c.y + tm * (c1.y - c.y), c.z + tm * (c1.z - c.z)};
+    r = double(p1.w);
+  }
+  const d3 oc = dsub(c, o);
+  const double a = ddot(d, d);
+  if (a <= 2.220446049250313e-16) return false;
+  const double hh = ddot(d, oc);
+  const double cc = ddot(oc, oc) - r * r;
+  const double disc = hh * hh - a * cc;
+  if (disc < 0.0) return false;
+  const double sq = sqrt(disc);
+  const double first = (hh - sq) / a, second = (hh + sq) / a;
+  const double t32 = double(h->t);
+  double t;
+  if (lo < first && (!(lo < second) || fabs(first - t32) <= fabs(second - t32))) t = first;
+  else if (lo < second) t = second;
+  else return false;
+  h->t = float(t);
+  return true;
+}
+
 // Outward normal at a hit (object.rs:163-234); for spheres (p - c)/r, or the
 // normalised offset when |r| <= EPS (geometry.rs:476-483).
 __device__ __forceinline__ f3 prim_outward_normal(const GPrim* __restrict__ prims, uint32_t k, f3 p, float time,
@@ -309,11 +459,45 @@ __device__ __forceinline__ f3 prim_outward_normal(const GPrim* __restrict__ prim
   return out / r;
 }
 
+// Shading normal of a mesh triangle (MeshTriangle::shading_normal,
+// mesh.rs:105-119, oriented by set_shading_normal, object.rs:470-479): the
+// vertex normals interpolated at barycentrics (u, v) of v1 and v2, normalized
+// and flipped with the face; n itself (the oriented geometric normal) for a
+// flat triangle, any other primitive, or a degenerate interpolation.
+__device__ __forceinline__ f3 shading_normal(const GPrim* __restrict__ prims, const float4* __restrict__ vn,
+                                             uint32_t k, float u, float v, bool front, f3 n) {
+  const float4 p1 = reinterpret_cast<const float4*>(prims + k)[1];
+  const uint32_t kind = (meta_of(reinterpret_cast<const float4*>(prims + k)[0]) >> GT_KIND_SHIFT) & GT_KIND_MASK;
+  const uint32_t smooth = __float_as_uint(p1.w);
+  if (kind != GK_TRIANGLE || smooth == 0u) return n;
+  const float4* q = vn + 3 * size_t(smooth - 1u);
+  const float4 a = q[0], b = q[1], c = q[2];
+  const float w = 1.0f - u - v;
+  const f3 s = mk(w * a.x + u * b.x + v * c.x, w * a.y + u * b.y + v * c.y, w * a.z + u * b.z + v * c.z);
+  if (!(len2(s) > EPS)) return n;
+  const f3 sn = normalized(s);
+  return front ? sn : -sn;
+}
+// Barycentric coordinates (of v1, v2) of a point p in a triangle record, for
+// a hit whose FP64 refinement was rejected (an edge case).
+__device__ __forceinline__ void tri_barycentric(const GPrim* __restrict__ prims, uint32_t k, f3 p, float* u,
+                                                float* v) {
+  const float4* rec = reinterpret_cast<const float4*>(prims + k);
+  const float4 p0 = rec[0], p1 = rec[1], p2 = rec[2];
+  const f3 e1 = mk(p1.x, p1.y, p1.z), e2 = mk(p2.x, p2.y, p2.z), q = p - mk(p0.x, p0.y, p0.z);
+  const float d00 = dot(e1, e1), d01 = dot(e1, e2), d11 = dot(e2, e2), d20 = dot(q, e1), d21 = dot(q, e2);
+  const float den = d00 * d11 - d01 * d01;
+  if (!(fabsf(den) > 0.f)) { *u = 0.f; *v = 0.f; return; }
+  *u = (d11 * d20 - d01 * d21) / den;
+  *v = (d00 * d21 - d01 * d20) / den;
+}
+
 // ---- scene and traversal ---------------------------------------------------
 struct DScene {
   const GNode2* __restrict__ nodes2;
   const GNode4* __restrict__ nodes4;
   const GPrim* __restrict__ prims;
+  const float4* __restrict__ vnormals;  // mesh triangles' vertex normals, 3 per triangle
   const uint32_t* __restrict__ prim_ids;
   const GMat* __restrict__ mats;
   float root_lo[3], root_hi[3];
@@ -388,7 +572,7 @@ __device__ __forceinline__ bool slab(const RayPre& r, float lox, float hix, floa
 // interval shrinking on each accepted hit (scene.rs:849-874).
 template <bool ANY, bool STATS>
 __device__ __forceinline__ bool trace2(const DScene& sc, f3 o, f3 d, float time, float tmin, float tmax,
-                                       Stack& st, float* t_out, uint32_t* prim_out, PrimHit* hit_out,
+                                       uint32_t self, Stack& st, float* t_out, uint32_t* prim_out, PrimHit* hit_out,
                                        Stats* stats, uint32_t* meta_out = nullptr) {
   RayPre rp = ray_pre(o, d);
   float closest = tmax;
@@ -412,7 +596,7 @@ __device__ __forceinline__ bool trace2(const DScene& sc, f3 o, f3 d, float time,
         PrimHit h;
         uint32_t meta;
         if (STATS) stats->candidates += 1;
-        if (prim_test(sc.prims, k, o, d, time, tmin, closest, &h, &meta)) {
+        if (prim_test(sc.prims, k, o, d, time, tmin, closest, &h, &meta, k == self)) {
           closest = h.t;
           best = k;
           best_meta = meta;
@@ -472,7 +656,7 @@ __device__ __forceinline__ bool trace2(const DScene& sc, f3 o, f3 d, float time,
 // Four-wide variant: children of a wide node are visited nearest-first.
 template <bool ANY, bool STATS>
 __device__ __forceinline__ bool trace4(const DScene& sc, f3 o, f3 d, float time, float tmin, float tmax,
-                                       Stack& st, float* t_out, uint32_t* prim_out, PrimHit* hit_out,
+                                       uint32_t self, Stack& st, float* t_out, uint32_t* prim_out, PrimHit* hit_out,
                                        Stats* stats, uint32_t* meta_out = nullptr) {
   RayPre rp = ray_pre(o, d);
   float closest = tmax;
@@ -496,7 +680,7 @@ __device__ __forceinline__ bool trace4(const DScene& sc, f3 o, f3 d, float time,
         PrimHit h;
         uint32_t meta;
         if (STATS) stats->candidates += 1;
-        if (prim_test(sc.prims, k, o, d, time, tmin, closest, &h, &meta)) {
+        if (prim_test(sc.prims, k, o, d, time, tmin, closest, &h, &meta, k == self)) {
           closest = h.t; best = k; best_meta = meta; bh = h; any = true; leaf_hit = true;
           if (ANY) return true;
         }
@@ -556,13 +740,16 @@ __device__ __forceinline__ bool trace4(const DScene& sc, f3 o, f3 d, float time,
   return any;
 }
 
+// self: the primitive (leaf order) the ray starts on, kNoSelf for a ray from
+// a camera or a caller; prim_test decides it by rule, not by t_min.
+constexpr uint32_t kNoSelf = 0xffffffffu;
 template <bool ANY, bool STATS>
-__device__ __forceinline__ bool trace(const DScene& sc, f3 o, f3 d, float time, float tmin, float tmax, Stack& st,
-                                      float* t_out, uint32_t* prim_out, PrimHit* hit_out, Stats* stats,
+__device__ __forceinline__ bool trace(const DScene& sc, f3 o, f3 d, float time, float tmin, float tmax, uint32_t self,
+                                      Stack& st, float* t_out, uint32_t* prim_out, PrimHit* hit_out, Stats* stats,
                                       uint32_t* meta_out = nullptr) {
   if (sc.width == 4)
-    return trace4<ANY, STATS>(sc, o, d, time, tmin, tmax, st, t_out, prim_out, hit_out, stats, meta_out);
-  return trace2<ANY, STATS>(sc, o, d, time, tmin, tmax, st, t_out, prim_out, hit_out, stats, meta_out);
+    return trace4<ANY, STATS>(sc, o, d, time, tmin, tmax, self, st, t_out, prim_out, hit_out, stats, meta_out);
+  return trace2<ANY, STATS>(sc, o, d, time, tmin, tmax, self, st, t_out, prim_out, hit_out, stats, meta_out);
 }
 
 // ---- light targets (scene.rs:363-609, object.rs:1048-1161) --------------------

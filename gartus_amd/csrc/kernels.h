@@ -52,8 +52,9 @@ struct Queues {
   float4* ray_d;    // direction, flags (bit 31: last event was a scattering event)
   float4* thr;      // throughput, miss-radiance weight
   float2* hits;     // t, reordered primitive index (0xffffffff = miss)
-  float4* s_d[2];   // shadow rays: [0] light targets, [1] environment: direction, weight x; origin ray_o
-  float2* s_w[2];   // weight y, z
+  float4* s_d[2];   // shadow rays: [0] light targets, [1] environment: direction, and the bits of the
+                    // primitive the ray leaves (leaf order); origin ray_o
+  float4* s_w[2];   // weight (x, y, z, unused)
   float4* radiance; // per-path radiance L
   uint32_t* active[2];            // paths to extend at bounce b, parity b & 1 (bounce 0: all paths)
   uint32_t* cat[kNumCategories];  // paths hit this bounce, per shading category

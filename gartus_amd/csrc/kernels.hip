@@ -137,6 +137,14 @@ __device__ __forceinline__ void with_frame(const Frame& f0, const Batch& bt, boo
   }
 }
 
+// The direction's fourth word: bit 31 = the ray was scattered (emission it
+// meets is not counted: NEE took it), bits 0-30 = 1 + the primitive the ray
+// leaves (leaf order), 0 for a camera ray.
+__device__ __forceinline__ uint32_t ray_word(bool scattered, uint32_t self) {
+  return (scattered ? 0x80000000u : 0u) | ((self + 1u) & 0x7fffffffu);
+}
+__device__ __forceinline__ uint32_t ray_self(float w) { return (__float_as_uint(w) & 0x7fffffffu) - 1u; }
+
 // Russian roulette (camera.rs:2498-2523)
 __device__ __forceinline__ bool rr_survives(uint32_t bounce, int32_t min_depth, f3* T, Rng& rng) {
   if (min_depth < 0 || bounce < uint32_t(min_depth)) return true;
@@ -149,18 +157,26 @@ __device__ __forceinline__ bool rr_survives(uint32_t bounce, int32_t min_depth, 
   return true;
 }
 
-// Material::scattering_pdf for the RayScene materials (material.rs:522-529,
-// :1596-1612; the others return 0).
-__device__ __forceinline__ float scattering_pdf(const GMat& m, f3 d_in, f3 n, f3 dir) {
+// Material::scattering_pdf (material.rs:522-529, :1139-1141, :1286-1291,
+// :1596-1612; the others return 0). n is the oriented geometric normal, ns
+// the shading normal (equal to n except on mesh triangles).
+__device__ __forceinline__ float scattering_pdf(const GMat& m, f3 d_in, f3 n, f3 ns, f3 dir) {
   if (m.kind == GM_LAMBERTIAN) {
-    float c = dot(n, normalized(dir));
+    float c = dot(ns, normalized(dir));
     return c <= 0.f ? 0.f : c / PI;
   }
   if (m.kind == GM_GGX) {
     f3 outgoing = -normalized(d_in);
     f3 incoming = normalized(dir);
     if (dot(incoming, n) <= 0.f || dot(outgoing, n) <= 0.f) return 0.f;
-    return ggx_weight(n, outgoing, incoming, m.param, m.f0);
+    return ggx_weight(ns, outgoing, incoming, m.param, m.f0);
+  }
+  if (m.kind == GM_ISOTROPIC) return 1.0f / (4.0f * PI);
+  if (m.kind == GM_HENYEY_GREENSTEIN) {  // HenyeyGreensteinPdf::new(ray_in.direction, g).value
+    const f3 w = normalized(d_in);
+    const float l = sqrtf(len2(dir));
+    if (len2(w) <= EPS || l <= EPS) return 0.f;
+    return hg_phase(dot(dir, w) / l, m.param);
   }
   return 0.f;
 }
@@ -221,7 +237,9 @@ __device__ __forceinline__ uint32_t node_fetches(const Stats& s, uint32_t width)
 __device__ __forceinline__ uint32_t category_of(uint32_t meta) {
   switch ((meta >> GT_MATKIND_SHIFT) & 7u) {
     case GM_DIFFUSE_LIGHT: return CAT_EMIT;
-    case GM_LAMBERTIAN: return CAT_LAMBERT;
+    case GM_LAMBERTIAN:
+    case GM_ISOTROPIC:
+    case GM_HENYEY_GREENSTEIN: return CAT_LAMBERT;  // the scattering materials without a GGX lobe
     case GM_GGX: return CAT_GGX;
     default: return CAT_SPECULAR;
   }
@@ -239,7 +257,7 @@ __global__ __attribute__((amdgpu_waves_per_eu(8))) void k_extend(Frame f0, Batch
   for (uint32_t s = blockIdx.x % slice_step(); s < kSlices; s += slice_step()) {
   const uint32_t count = q.ctr[ctr_at(bounce, S_QCOUNT, s)];
   if (count == 0) continue;
-  const uint32_t first = bounce == 0 ? q.ctr[ctr_at(0, S_BASE, s)] : 0;
+  const uint32_t first = CAMERA ? q.ctr[ctr_at(0, S_BASE, s)] : 0;
   const uint32_t* __restrict__ active = q.active[bounce & 1] + size_t(s) * q.region;
   const uint32_t per_lane = items_per_lane(count);
   for (;;) {
@@ -253,7 +271,7 @@ __global__ __attribute__((amdgpu_waves_per_eu(8))) void k_extend(Frame f0, Batch
       cats[k] = kNumCategories;
       ids[k] = 0;
       const bool valid = k < per_lane && j < count;
-      const uint32_t p = !valid ? 0u : bounce == 0 ? first + j : active[j];
+      const uint32_t p = !valid ? 0u : CAMERA ? first + j : active[j];
       ids[k] = p;
       with_frame<MULTI>(f0, bt, valid, p, [&](const Frame& f, const Seg& sg) {
         float4 o4, d4;
@@ -280,7 +298,7 @@ __global__ __attribute__((amdgpu_waves_per_eu(8))) void k_extend(Frame f0, Batch
         uint32_t prim, meta = 0;
         Stats s{0, 0, 0, 0, 0, 0};
         bool hit = trace<false, COUNT>(f.scene, mk(o4.x, o4.y, o4.z), mk(d4.x, d4.y, d4.z), o4.w, SHADOW_ACNE, INF,
-                                       st, &t, &prim, nullptr, &s, &meta);
+                                       CAMERA ? kNoSelf : ray_self(d4.w), st, &t, &prim, nullptr, &s, &meta);
         if (hit) {
           q.hits[p] = make_float2(t, __uint_as_float(prim));
           // a camera path has no radiance yet; its throughput and MIS weight are 1, which shade
@@ -324,10 +342,12 @@ struct Item {
   float time, t;
   uint32_t prim;
 };
-__device__ __forceinline__ Item load_item(const Queues& q, uint32_t p, uint32_t bounce) {
+// CAMERA: the hits of camera rays made by k_extend<.., CAMERA = true>, whose
+// paths start with throughput 1 and MIS weight 1 and never store them.
+template <bool CAMERA>
+__device__ __forceinline__ Item load_item(const Queues& q, uint32_t p) {
   Item it;
-  // a camera path (bounce 0) starts with throughput 1 and MIS weight 1, not stored
-  float4 o4 = q.ray_o[p], d4 = q.ray_d[p], t4 = bounce == 0 ? make_float4(1.f, 1.f, 1.f, 1.f) : q.thr[p];
+  float4 o4 = q.ray_o[p], d4 = q.ray_d[p], t4 = CAMERA ? make_float4(1.f, 1.f, 1.f, 1.f) : q.thr[p];
   float2 h = q.hits[p];
   it.path = p;
   it.after_scatter = (__float_as_uint(d4.w) >> 31) != 0;
@@ -347,28 +367,36 @@ enum { OUT_EXT = 1, OUT_SL = 2, OUT_SE = 4 };
 
 // The continuation's direction and throughput; its origin is the vertex,
 // written by the caller.
-__device__ __forceinline__ uint32_t continue_dir(const Queues& q, uint32_t p, f3 dir, bool scattered, f3 T,
-                                                 float mw) {
-  q.ray_d[p] = make_float4(dir.x, dir.y, dir.z, __uint_as_float(scattered ? 0x80000000u : 0u));
+__device__ __forceinline__ uint32_t continue_dir(const Queues& q, uint32_t p, f3 dir, bool scattered, uint32_t self,
+                                                 f3 T, float mw) {
+  q.ray_d[p] = make_float4(dir.x, dir.y, dir.z, __uint_as_float(ray_word(scattered, self)));
   q.thr[p] = make_float4(T.x, T.y, T.z, mw);
   return OUT_EXT;
 }
 __device__ __forceinline__ uint32_t continue_path(const Queues& q, uint32_t p, f3 pt, float time, f3 dir,
-                                                  bool scattered, f3 T, float mw) {
+                                                  bool scattered, uint32_t self, f3 T, float mw) {
   q.ray_o[p] = make_float4(pt.x, pt.y, pt.z, time);
-  return continue_dir(q, p, dir, scattered, T, mw);
+  return continue_dir(q, p, dir, scattered, self, T, mw);
 }
 
-// Lambertian (material.rs:508-532) or GGX (:1583-1616) vertex: light NEE,
-// environment NEE, then the continuation sample (camera.rs:1779-1857).
+// Lambertian (material.rs:508-532), Isotropic (:1123-1141), Henyey-Greenstein
+// (:1270-1291) or GGX (:1583-1616) vertex: light NEE, environment NEE, then
+// the continuation sample (camera.rs:1779-1857). n is the oriented geometric
+// normal, ns the shading normal.
 template <bool GGX>
 __device__ __forceinline__ uint32_t scatter_vertex(const Frame& f, const Queues& q, const Item& it, const GMat& m,
-                                                   f3 p, f3 n, Rng& rng, bool last, uint32_t bounce) {
+                                                   f3 p, f3 n, f3 ns, Rng& rng, bool last, uint32_t bounce) {
   const bool has_env = f.env.kind != 0;
   uint32_t out = 0;
   MatPdf pdf;
-  pdf.ggx = GGX;
-  bool ok = onb_from_w(n, &pdf.basis);
+  pdf.kind = GGX ? PDF_GGX : m.kind == GM_ISOTROPIC ? PDF_SPHERE : m.kind == GM_HENYEY_GREENSTEIN ? PDF_HG : PDF_COSINE;
+  bool ok = true;
+  if (pdf.kind == PDF_HG) {  // HenyeyGreensteinPdf::new (sampling.rs:96-112): basis around the incident direction
+    pdf.g = m.param;
+    ok = onb_from_w(it.d, &pdf.basis);
+  } else if (pdf.kind != PDF_SPHERE) {
+    ok = onb_from_w(ns, &pdf.basis);
+  }
   if (GGX) {  // GgxReflectionPdf::new (sampling.rs:183-202)
     f3 outgoing = normalized(-normalized(it.d));
     ok = ok && len2(outgoing) > EPS && dot(outgoing, pdf.basis.w) > 0.f;
@@ -384,11 +412,11 @@ __device__ __forceinline__ uint32_t scatter_vertex(const Frame& f, const Queues&
       f3 ldir = lights_random(f.lights, p, it.time, rng);
       float lpdf = lights_pdf(f.lights, p, it.time, ldir);
       if (isfinite(lpdf) && lpdf > EPS) {
-        float spdf = scattering_pdf(m, it.d, n, ldir);
+        float spdf = scattering_pdf(m, it.d, n, ns, ldir);
         if (isfinite(spdf) && spdf > 0.f) {
           f3 w = mul(T, color) * (spdf / lpdf);
-          q.s_d[0][it.path] = make_float4(ldir.x, ldir.y, ldir.z, w.x);
-          q.s_w[0][it.path] = make_float2(w.y, w.z);
+          q.s_d[0][it.path] = make_float4(ldir.x, ldir.y, ldir.z, __uint_as_float(it.prim));
+          q.s_w[0][it.path] = make_float4(w.x, w.y, w.z, 0.f);
           out |= OUT_SL;
         }
       }
@@ -396,12 +424,12 @@ __device__ __forceinline__ uint32_t scatter_vertex(const Frame& f, const Queues&
     if (has_env) {  // estimate_environment_lighting camera.rs:2264-2297
       EnvSample es = env_sample(f.env, rng);
       if (isfinite(es.pdf) && es.pdf > EPS) {
-        float spdf = scattering_pdf(m, it.d, n, es.dir);
+        float spdf = scattering_pdf(m, it.d, n, ns, es.dir);
         if (isfinite(spdf) && spdf > 0.f) {
           float mis = power_heuristic(es.pdf, spdf);
           f3 w = mul(T, mul(color, env_sample_radiance(f.env, es)) * (mis * spdf / es.pdf));
-          q.s_d[1][it.path] = make_float4(es.dir.x, es.dir.y, es.dir.z, w.x);
-          q.s_w[1][it.path] = make_float2(w.y, w.z);
+          q.s_d[1][it.path] = make_float4(es.dir.x, es.dir.y, es.dir.z, __uint_as_float(it.prim));
+          q.s_w[1][it.path] = make_float4(w.x, w.y, w.z, 0.f);
           out |= OUT_SE;
         }
       }
@@ -428,33 +456,34 @@ __device__ __forceinline__ uint32_t scatter_vertex(const Frame& f, const Queues&
   }
   if (below && (f.cam.flags & GT_FLAG_GGX_REJECT_BELOW_HORIZON)) return out;
   if (!(isfinite(cpdf) && cpdf > EPS)) return out;
-  float spdf = scattering_pdf(m, it.d, n, dir);
+  float spdf = scattering_pdf(m, it.d, n, ns, dir);
   if (!(isfinite(spdf) && spdf > 0.f)) return out;
   // The environment's MIS weight power_heuristic(cpdf, pdf_value(dir)) applies
   // only if the continuation misses: it is stored as -cpdf (cpdf > EPS here)
   // and evaluated by the miss shading in extend.
   float next_w = weight_env ? -cpdf : 1.0f;
   T = mul(T, color * (spdf / cpdf));
-  if (rr_survives(bounce, f.cam.rr_min_depth, &T, rng)) out |= continue_dir(q, it.path, dir, suppress, T, next_w);
+  if (rr_survives(bounce, f.cam.rr_min_depth, &T, rng)) out |= continue_dir(q, it.path, dir, suppress, it.prim, T, next_w);
   return out;
 }
 // The vertex also becomes the origin of its shadow rays and continuation: one
 // write serves all three (connect reads it before the next extend does).
 template <bool GGX>
 __device__ __forceinline__ uint32_t shade_scatter(const Frame& f, const Queues& q, const Item& it, const GMat& m, f3 p,
-                                                  f3 n, Rng& rng, bool last, uint32_t bounce) {
-  const uint32_t out = scatter_vertex<GGX>(f, q, it, m, p, n, rng, last, bounce);
+                                                  f3 n, f3 ns, Rng& rng, bool last, uint32_t bounce) {
+  const uint32_t out = scatter_vertex<GGX>(f, q, it, m, p, n, ns, rng, last, bounce);
   if (out) q.ray_o[it.path] = make_float4(p.x, p.y, p.z, it.time);
   return out;
 }
 
-// Metal (material.rs:2087-2107) or Dielectric (:2321-2350) vertex.
+// Metal (material.rs:2087-2107: reflection about the shading normal, absorbed
+// below the geometric one) or Dielectric (:2321-2350, geometric normal) vertex.
 __device__ __forceinline__ uint32_t shade_specular(const Frame& f, const Queues& q, const Item& it, const GMat& m, f3 p,
-                                                   f3 n, bool front, Rng& rng, bool last, uint32_t bounce) {
+                                                   f3 n, f3 ns, bool front, Rng& rng, bool last, uint32_t bounce) {
   f3 T = it.T;
   f3 dir;
   if (m.kind == GM_METAL) {
-    f3 refl = reflected(normalized(it.d), n);
+    f3 refl = reflected(normalized(it.d), ns);
     dir = refl + m.param * rng.unit_vector_spherical();
     if (dot(dir, n) <= 0.f) return 0;  // absorbed
     T = mul(T, ld3(m.color));
@@ -470,12 +499,12 @@ __device__ __forceinline__ uint32_t shade_specular(const Frame& f, const Queues&
     dir = refl ? reflected(unit, n) : refracted(unit, n, ratio);
   }
   if (!last && rr_survives(bounce, f.cam.rr_min_depth, &T, rng))
-    return continue_path(q, it.path, p, it.time, dir, false, T, 1.0f);
+    return continue_path(q, it.path, p, it.time, dir, false, it.prim, T, 1.0f);
   return 0;
 }
 
 // Shades the hits of one category, all slices.
-template <int CAT, bool MULTI>
+template <int CAT, bool MULTI, bool CAMERA>
 __device__ __forceinline__ void shade_category(const Frame& f0, const Batch& bt, const Queues& q, uint32_t bounce) {
   constexpr bool kScatters = CAT == CAT_LAMBERT || CAT == CAT_GGX || CAT == CAT_SPECULAR;
   for (uint32_t s = blockIdx.x % slice_step(); s < kSlices; s += slice_step()) {
@@ -496,12 +525,20 @@ __device__ __forceinline__ void shade_category(const Frame& f0, const Batch& bt,
       ids[k] = p;
       outs[k] = 0;
       with_frame<MULTI>(f0, bt, valid, p, [&](const Frame& f, const Seg& sg) {
-        const Item it = load_item(q, p, bounce);
+        Item it = load_item<CAMERA>(q, p);
+        PrimHit ph{it.t, 0.f, 0.f};  // the chosen primitive's distance in FP64 (traversal decides in FP32)
+        const bool refined = refine_hit(f.scene.prims, it.prim, it.o, it.d, it.time, SHADOW_ACNE, &ph);
+        it.t = ph.t;
         f3 pt = it.o + it.t * it.d;
         uint32_t mid, kind;
         f3 outward = prim_outward_normal(f.scene.prims, it.prim, pt, it.time, &mid, &kind);
         bool front = dot(it.d, outward) < 0.f;  // object.rs:127-149
         f3 n = front ? outward : -outward;
+        f3 ns = n;
+        if (kind == GK_TRIANGLE) {  // a mesh triangle interpolates its vertex normals
+          if (!refined) tri_barycentric(f.scene.prims, it.prim, pt, &ph.u, &ph.v);
+          ns = shading_normal(f.scene.prims, f.scene.vnormals, it.prim, ph.u, ph.v, front, n);
+        }
         const GMat m = f.scene.mats[mid];
         if (CAT == CAT_EMIT) {  // emitted_at camera.rs:1753-1755, DiffuseLight material.rs:1009-1022
           if (!it.after_scatter && front) {
@@ -515,8 +552,8 @@ __device__ __forceinline__ void shade_category(const Frame& f0, const Batch& bt,
         path_coords(f, sg, p, &x, &y, &sample);
         Rng rng{stream_seed(pixel_seed(f.cam.seed, x, y), sample, 1 + bounce)};
         const bool last = bounce + 1 >= f.cam.max_depth;  // the camera's final bounce
-        if (CAT == CAT_SPECULAR) outs[k] = shade_specular(f, q, it, m, pt, n, front, rng, last, bounce);
-        else outs[k] = shade_scatter<CAT == CAT_GGX>(f, q, it, m, pt, n, rng, last, bounce);
+        if (CAT == CAT_SPECULAR) outs[k] = shade_specular(f, q, it, m, pt, n, ns, front, rng, last, bounce);
+        else outs[k] = shade_scatter<CAT == CAT_GGX>(f, q, it, m, pt, n, ns, rng, last, bounce);
       });
     }
     if (kScatters) {
@@ -537,12 +574,13 @@ __device__ __forceinline__ void shade_category(const Frame& f0, const Batch& bt,
 
 // One launch per bounce drains the category queues in turn, so a wave that
 // finds one queue empty moves on to the next without a launch boundary.
-template <bool MULTI>
+// CAMERA matches the extend launch of the same bounce (camera_bounce).
+template <bool MULTI, bool CAMERA>
 __global__ void k_shade(Frame f0, Batch bt, Queues q, uint32_t bounce) {
-  shade_category<CAT_EMIT, MULTI>(f0, bt, q, bounce);
-  shade_category<CAT_LAMBERT, MULTI>(f0, bt, q, bounce);
-  shade_category<CAT_GGX, MULTI>(f0, bt, q, bounce);
-  shade_category<CAT_SPECULAR, MULTI>(f0, bt, q, bounce);
+  shade_category<CAT_EMIT, MULTI, CAMERA>(f0, bt, q, bounce);
+  shade_category<CAT_LAMBERT, MULTI, CAMERA>(f0, bt, q, bounce);
+  shade_category<CAT_GGX, MULTI, CAMERA>(f0, bt, q, bounce);
+  shade_category<CAT_SPECULAR, MULTI, CAMERA>(f0, bt, q, bounce);
 }
 
 // Shadow rays of one kind. Light targets: closest hit, then the emission of
@@ -554,7 +592,7 @@ __global__ __attribute__((amdgpu_waves_per_eu(8))) void k_connect(Frame f0, Batc
   extern __shared__ uint2 smem[];
   Stack st = make_stack(q, smem);
   const float4* __restrict__ sd = q.s_d[kind];
-  const float2* __restrict__ sw = q.s_w[kind];
+  const float4* __restrict__ sw = q.s_w[kind];
   for (uint32_t s = blockIdx.x % slice_step(); s < kSlices; s += slice_step()) {
   const uint32_t count = q.ctr[ctr_at(bounce, kind == 0 ? S_SL_COUNT : S_SE_COUNT, s)];
   if (count == 0) continue;
@@ -572,31 +610,32 @@ __global__ __attribute__((amdgpu_waves_per_eu(8))) void k_connect(Frame f0, Batc
       with_frame<MULTI>(f0, bt, valid, p, [&](const Frame& f, const Seg&) {
       Stats s{0, 0, 0, 0, 0, 0};
       const float4 o4 = q.ray_o[p], d4 = sd[p];  // the origin is the shading vertex
-      const float2 w2 = sw[p];
-      const f3 wgt = mk(d4.w, w2.x, w2.y);
       f3 o = mk(o4.x, o4.y, o4.z), d = mk(d4.x, d4.y, d4.z);
+      const uint32_t self = __float_as_uint(d4.w);  // the shading vertex's primitive
       float t;
       uint32_t prim;
       f3 add = mk(0.f, 0.f, 0.f);
       bool contributes = false;
       if (kind == 0) {
-        if (trace<false, COUNT>(f.scene, o, d, o4.w, SHADOW_ACNE, INF, st, &t, &prim, nullptr, &s)) {
+        if (trace<false, COUNT>(f.scene, o, d, o4.w, SHADOW_ACNE, INF, self, st, &t, &prim, nullptr, &s)) {
           f3 pt = o + t * d;
           uint32_t mid, pk;
           f3 outward = prim_outward_normal(f.scene.prims, prim, pt, o4.w, &mid, &pk);
           const GMat& m = f.scene.mats[mid];
           if (m.kind == GM_DIFFUSE_LIGHT && dot(d, outward) < 0.f) {
             f3 em = ld3(m.color);
-            if (finite3(em)) { add = mul(wgt, em); contributes = true; }
+            if (finite3(em)) { add = em; contributes = true; }
           }
         }
       } else {
-        if (!trace<true, COUNT>(f.scene, o, d, o4.w, SHADOW_ACNE, INF, st, &t, &prim, nullptr, &s)) {
-          add = wgt;
+        if (!trace<true, COUNT>(f.scene, o, d, o4.w, SHADOW_ACNE, INF, self, st, &t, &prim, nullptr, &s)) {
+          add = mk(1.f, 1.f, 1.f);
           contributes = true;
         }
       }
-      if (contributes) {
+      if (contributes) {  // the weight is read only for a ray that lands
+        const float4 w4 = sw[p];
+        add = mul(mk(w4.x, w4.y, w4.z), add);
         float4 L = q.radiance[p];
         q.radiance[p] = make_float4(L.x + add.x, L.y + add.y, L.z + add.z, 0.f);
       }
@@ -780,11 +819,16 @@ __global__ void k_aov(Frame f, Queues q, float* albedo, float* normal) {
       float time, t;
       uint32_t prim;
       camera_ray(f.cam, x, y, s, rng, &o, &d, &time);
-      if (trace<false, false>(f.scene, o, d, time, SHADOW_ACNE, INF, st, &t, &prim, nullptr, nullptr)) {
+      PrimHit ph{0.f, 0.f, 0.f};
+      if (trace<false, false>(f.scene, o, d, time, SHADOW_ACNE, INF, kNoSelf, st, &t, &prim, &ph, nullptr)) {
+        refine_hit(f.scene.prims, prim, o, d, time, SHADOW_ACNE, &ph);
+        t = ph.t;
         f3 p = o + t * d;
         uint32_t mid, kind;
         f3 outward = prim_outward_normal(f.scene.prims, prim, p, time, &mid, &kind);
-        f3 n = dot(d, outward) < 0.f ? outward : -outward;
+        const bool front = dot(d, outward) < 0.f;
+        f3 n = front ? outward : -outward;
+        n = shading_normal(f.scene.prims, f.scene.vnormals, prim, ph.u, ph.v, front, n);  // 0.5 (shading_normal + 1)
         const GMat& m = f.scene.mats[mid];
         sa = sa + (m.kind == GM_DIELECTRIC ? mk(1.f, 1.f, 1.f) : ld3(m.color));
         sn = sn + 0.5f * (n + mk(1.f, 1.f, 1.f));
@@ -814,8 +858,10 @@ __global__ void k_intersect(Frame f, Queues q, const float4* __restrict__ rays, 
     uint32_t prim = 0;
     PrimHit ph{0.f, 0.f, 0.f};
     Stats s{0, 0, 0, 0, 0, 0};
-    bool hit = trace<false, STATS>(f.scene, o, d, o4.w, tmin, d4.w, st, &t, &prim, &ph, &s);
+    bool hit = trace<false, STATS>(f.scene, o, d, o4.w, tmin, d4.w, kNoSelf, st, &t, &prim, &ph, &s);
     if (hit) {
+      refine_hit(f.scene.prims, prim, o, d, o4.w, tmin, &ph);  // the chosen primitive's t, u, v in FP64
+      t = ph.t;
       f3 p = o + t * d;
       uint32_t mid, kind;
       f3 outward = prim_outward_normal(f.scene.prims, prim, p, o4.w, &mid, &kind);
@@ -866,11 +912,16 @@ uint32_t occupancy_blocks(int kernel, uint32_t block, size_t lds) {
   int nb = 0;
   hipError_t e = hipSuccess;
   if (kernel == 0) e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_extend<false, false, false>, int(block), lds);
-  else if (kernel == 1) e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_shade<false>, int(block), 0);
+  else if (kernel == 1) e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_shade<false, false>, int(block), 0);
   else e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_connect<false, false>, int(block), lds);
   if (e != hipSuccess || nb <= 0) return 1;
   return uint32_t(nb);
 }
+
+// The one place that decides which bounce's extend generates camera rays;
+// the shade launch of that bounce takes the same template choice, so the
+// camera paths' unstored throughput and the generated path ids stay paired.
+static inline bool camera_bounce(uint32_t bounce) { return bounce == 0; }
 
 void reset_counters(const Launch& l, const Queues& q, uint32_t depth, uint32_t total) {
   const uint32_t n = (depth + 1) * kSlots * kSlices;
@@ -890,7 +941,7 @@ void extend(const Launch& l, const Work& w, const Queues& q, uint32_t bounce, bo
   } else {                                                               \
     if (count) GT_EXTEND(true, false, K); else GT_EXTEND(false, false, K); \
   }
-  if (bounce == 0) {
+  if (camera_bounce(bounce)) {
     GT_EXTEND_CM(true)
   } else {
     GT_EXTEND_CM(false)
@@ -899,8 +950,14 @@ void extend(const Launch& l, const Work& w, const Queues& q, uint32_t bounce, bo
 #undef GT_EXTEND
 }
 void shade(const Launch& l, const Work& w, const Queues& q, uint32_t bounce) {
-  if (w.multi) hipLaunchKernelGGL((k_shade<true>), dim3(l.grid), dim3(l.block), 0, l.stream, w.f, w.bt, q, bounce);
-  else hipLaunchKernelGGL((k_shade<false>), dim3(l.grid), dim3(l.block), 0, l.stream, w.f, w.bt, q, bounce);
+#define GT_SHADE(M, K) \
+  hipLaunchKernelGGL((k_shade<M, K>), dim3(l.grid), dim3(l.block), 0, l.stream, w.f, w.bt, q, bounce)
+  if (camera_bounce(bounce)) {
+    if (w.multi) GT_SHADE(true, true); else GT_SHADE(false, true);
+  } else {
+    if (w.multi) GT_SHADE(true, false); else GT_SHADE(false, false);
+  }
+#undef GT_SHADE
 }
 void connect(const Launch& l, const Work& w, const Queues& q, uint32_t bounce, uint32_t kind, bool count) {
   const size_t lds = stack_lds_bytes(l.block, q.stack_cap);

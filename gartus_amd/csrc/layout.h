@@ -12,7 +12,9 @@
 // primitive, type-tagged in the first word:
 //   sphere         p0 = (center, meta)  p1 = (r, 0, 0, 0)
 //   moving sphere  p0 = (c0, meta)      p1 = (c1, r)
-//   triangle       p0 = (v0, meta)      p1 = (v1-v0, 0) p2 = (v2-v0, 0) p3 = (unit normal, 0)
+//   triangle       p0 = (v0, meta)      p1 = (v1-v0, s) p2 = (v2-v0, 0) p3 = (unit normal, 0)
+//                  s: the bits of 0 for a flat triangle, of 1 + i for a mesh triangle
+//                  whose vertex normals are entries 3i..3i+2 of the scene's normal array
 //   quad           p0 = (corner, meta)  p1 = (u, 1/|u x v|) p2 = (v, 0) p3 = (unit normal, plane d)
 // meta = material id (25 bits) | material kind << 25 (3 bits) | primitive
 // kind << 28 (3 bits) | last-in-leaf << 31. The material kind lets the
@@ -28,7 +30,8 @@
 #define GT_MATERIAL_MASK 0x01ffffffu
 
 enum { GK_SPHERE = 0, GK_MOVING_SPHERE = 1, GK_TRIANGLE = 2, GK_QUAD = 3 };
-enum { GM_LAMBERTIAN = 0, GM_DIFFUSE_LIGHT = 1, GM_GGX = 2, GM_METAL = 3, GM_DIELECTRIC = 4 };
+enum { GM_LAMBERTIAN = 0, GM_DIFFUSE_LIGHT = 1, GM_GGX = 2, GM_METAL = 3, GM_DIELECTRIC = 4, GM_ISOTROPIC = 5,
+       GM_HENYEY_GREENSTEIN = 6 };
 
 struct alignas(16) GNode2 {
   float lx[2], ly[2];  // left child x/y slabs (lo, hi)
@@ -58,7 +61,7 @@ static_assert(sizeof(GPrim) == 64, "primitive record must be 64 bytes");
 struct alignas(16) GMat {
   float color[3];
   uint32_t kind;
-  float param;     // ggx roughness (clamped), metal fuzz (clamped), dielectric ior
+  float param;     // ggx roughness (clamped), metal fuzz (clamped), dielectric ior, henyey-greenstein g
   float f0;        // ggx: specular_luminance(color)
   float pad[2];
 };

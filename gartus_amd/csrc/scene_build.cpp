@@ -237,6 +237,7 @@ GPrim pack_prim(const HostPrim& p, bool last) {
         g.p0[i] = float(q[i]); g.p1[i] = float(e1[i]); g.p2[i] = float(e2[i]);
         g.p3[i] = len < std::numeric_limits<double>::epsilon() ? 0.0f : float(n[i] / len);
       }
+      g.p1[3] = from_bits(p.smooth);
       break;
     }
     default: {  // QuadGeometry::new (geometry.rs:293-312)

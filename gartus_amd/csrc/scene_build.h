@@ -16,6 +16,7 @@ struct HostPrim {
   double q[9];       // sphere: c, r | moving: c0, c1, r | triangle: p0, p1, p2 | quad: corner, u, v
   uint32_t material;
   uint32_t mat_kind;   // GM_* of the material, packed into the record's meta word
+  uint32_t smooth;     // triangles: 0 flat, 1 + i for a mesh triangle with vertex normals i (3 entries each)
 };
 
 struct HostMat {

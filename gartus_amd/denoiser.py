@@ -73,6 +73,7 @@ class Denoiser:
         self.levels, self.channels = levels, tuple(channels[:levels])
         h = C.c_void_p()
         abi.check(self.ctx.lib.gt_denoiser_create(self.ctx.handle, C.byref(cfg), C.byref(h)))
+        self.ctx._adopt(self)
         self.handle = h
         n = C.c_uint64()
         abi.check(self.ctx.lib.gt_denoiser_param_count(h, C.byref(n)))
@@ -164,10 +165,15 @@ class Denoiser:
         return float(loss.value)
 
     def close(self) -> None:
-        if getattr(self, "handle", None):
+        self._release()
+
+    def _release(self) -> None:
+        """Frees the network and its buffers while the context is alive (a
+        closed context has released them already)."""
+        if getattr(self, "handle", None) and self.ctx.handle:
             self._dev.close()
             self.ctx.lib.gt_denoiser_destroy(self.handle)
-            self.handle = None
+        self.handle = None
 
     def __del__(self):
         try:

@@ -10,17 +10,18 @@ from dataclasses import dataclass, field
 
 import numpy as np
 
-# material kinds (layout.h GM_*), primitive kinds (GK_*)
-LAMBERTIAN, DIFFUSE_LIGHT, GGX, METAL, DIELECTRIC = range(5)
-SPHERE, MOVING_SPHERE, TRIANGLE, QUAD = range(4)
-PRIM_STRIDE = {SPHERE: 4, MOVING_SPHERE: 7, TRIANGLE: 9, QUAD: 9}
+# material kinds (layout.h GM_*), primitive kinds (GK_*); a mesh triangle
+# carries its three vertex normals after its positions (18 values)
+LAMBERTIAN, DIFFUSE_LIGHT, GGX, METAL, DIELECTRIC, ISOTROPIC, HENYEY_GREENSTEIN = range(7)
+SPHERE, MOVING_SPHERE, TRIANGLE, QUAD, MESH_TRIANGLE = range(5)
+PRIM_STRIDE = {SPHERE: 4, MOVING_SPHERE: 7, TRIANGLE: 9, QUAD: 9, MESH_TRIANGLE: 18}
 
 
 @dataclass
 class Material:
     kind: int
     color: tuple = (0.0, 0.0, 0.0)
-    param: float = 0.0   # ggx roughness, metal fuzz, dielectric ior
+    param: float = 0.0   # ggx roughness, metal fuzz, dielectric ior, henyey-greenstein anisotropy
 
     @property
     def is_emissive(self) -> bool:  # RayMaterial::is_emissive
@@ -62,8 +63,9 @@ class SceneData:
         out = []
         for kind, data, mids in self.chunks:
             for row, m in zip(data, mids):
-                if self.materials[int(m)].is_emissive:
-                    out.append((kind, np.array(row), 1.0))
+                if self.materials[int(m)].is_emissive:  # a mesh triangle is a triangle target
+                    out.append((TRIANGLE, np.array(row[:9]), 1.0) if kind == MESH_TRIANGLE
+                               else (kind, np.array(row), 1.0))
         return out
 
     def as_float32_exact(self) -> "SceneData":

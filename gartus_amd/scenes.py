@@ -546,3 +546,30 @@ def batch_scene(seed: int):
 def batch_scenes(n: int = 64, seed: int = 1000):
     """The batched workload: n scenes from consecutive seeds."""
     return [batch_scene(seed + i) for i in range(n)]
+
+
+def smooth_sphere_mesh(center, radius: float, n_theta: int = 16, n_phi: int = 32):
+    """A latitude-longitude tessellation of a sphere as mesh triangles with
+    per-vertex normals (the sphere's radial directions), the kind of
+    TriangleMesh with vertex normals mesh.rs:24-151 shades smoothly. Returns
+    positions (n, 9) and normals (n, 9), outward-wound."""
+    c = np.asarray(center, np.float64)
+    th = np.linspace(0.0, math.pi, n_theta + 1)
+    ph = np.linspace(0.0, 2.0 * math.pi, n_phi + 1)
+
+    def vert(i, j):
+        n = np.array([math.sin(th[i]) * math.cos(ph[j]), math.cos(th[i]), math.sin(th[i]) * math.sin(ph[j])])
+        return c + radius * n, n
+
+    pos, nrm = [], []
+    for i in range(n_theta):
+        for j in range(n_phi):
+            (a, na), (b, nb) = vert(i, j), vert(i + 1, j)
+            (cc, nc), (d, nd) = vert(i + 1, j + 1), vert(i, j + 1)
+            for tri in ((a, na, d, nd, b, nb), (b, nb, d, nd, cc, nc)):
+                p0, n0, p1, n1, p2, n2 = tri
+                if np.linalg.norm(np.cross(p1 - p0, p2 - p0)) < 1e-12:  # the pole's degenerate triangles
+                    continue
+                pos.append(np.concatenate([p0, p1, p2]))
+                nrm.append(np.concatenate([n0, n1, n2]))
+    return np.array(pos), np.array(nrm)

@@ -129,7 +129,10 @@ typedef struct gt_tuning {
   uint32_t blocks_per_cu;   /* persistent blocks per CU; 0 = occupancy maximum */
   uint64_t max_paths;       /* wavefront queue capacity in paths; 0 = automatic */
   uint32_t bvh_width;       /* 2 or 4 */
-  uint32_t use_graph;       /* capture the per-batch bounce loop in a hipGraph */
+  uint32_t use_graph;       /* replay the render's launch sequence as a hipGraph (default 1): a
+                               sequence is captured the second time a render repeats it (same
+                               scene, camera, outputs and tuning) and replayed while it stays
+                               the same; renders that ask for statistics run eagerly */
   uint32_t count_traversal; /* count node fetches and primitive tests (gt_render_stats) */
   uint32_t stack_lds;       /* traversal-stack entries per lane kept in LDS (4..32; deeper levels
                                spill to a per-lane global slice); 0 = 8 */
@@ -148,6 +151,14 @@ int gt_scene_add_diffuse_light(gt_scene* scene, double r, double g, double b, ui
 int gt_scene_add_ggx(gt_scene* scene, double r, double g, double b, double roughness, uint32_t* id);
 int gt_scene_add_metal(gt_scene* scene, double r, double g, double b, double fuzz, uint32_t* id);
 int gt_scene_add_dielectric(gt_scene* scene, double refraction_index, uint32_t* id);
+/* RayMaterial::Isotropic (material.rs:1048-1141): scatters uniformly over the
+ * sphere (SpherePdf), scattering_pdf 1/(4 pi); albedo must be finite. */
+int gt_scene_add_isotropic(gt_scene* scene, double r, double g, double b, uint32_t* id);
+/* RayMaterial::HenyeyGreenstein (material.rs:1182-1333): the Henyey-Greenstein
+ * phase function around the incident direction (HenyeyGreensteinPdf,
+ * sampling.rs:91-166); albedo finite, anisotropy finite and in (-1, 1). */
+int gt_scene_add_henyey_greenstein(gt_scene* scene, double r, double g, double b, double anisotropy,
+                                   uint32_t* id);
 /* RayScene::add_{sphere,moving_sphere,triangle,quad}[s] (scene.rs:1237-1286).
  * Layouts per primitive: sphere (cx,cy,cz,r); moving sphere
  * (c0x,c0y,c0z,c1x,c1y,c1z,r); triangle (p0,p1,p2); quad (corner,u,v).
@@ -157,6 +168,17 @@ int gt_scene_add_spheres(gt_scene* scene, const double* data, size_t n, const ui
 int gt_scene_add_moving_spheres(gt_scene* scene, const double* data, size_t n, const uint32_t* material_ids);
 int gt_scene_add_triangles(gt_scene* scene, const double* data, size_t n, const uint32_t* material_ids);
 int gt_scene_add_quads(gt_scene* scene, const double* data, size_t n, const uint32_t* material_ids);
+/* Triangles with per-vertex shading normals: the triangles of a TriangleMesh
+ * (MeshTriangle::with_vertex_normals, mesh.rs:24-151), added to the RayScene
+ * (the reference traverses a mesh with its own BVH; closest hits do not
+ * depend on the tree). positions: (p0, p1, p2) per triangle; normals: the
+ * three vertex normals (n0, n1, n2), 9 values per triangle. The geometric
+ * normal orients the face; the shading normal is n0 w + n1 u + n2 v at the
+ * hit's barycentrics, normalized and flipped with the face, or the geometric
+ * normal when that sum is (near) zero (mesh.rs:105-119, object.rs:470-479).
+ * Materials use it where the reference reads HitRecord::shading_normal. */
+int gt_scene_add_mesh_triangles(gt_scene* scene, const double* positions, const double* normals, size_t n,
+                                const uint32_t* material_ids);
 /* RayScene::build_bvh_with_options (scene.rs:1562, bvh.rs:233-288): SAH
  * build with the reference's exact splits, then upload to the GPU.
  * leaf_size 0 is clamped to 1 (bvh.rs:73-75). Any later add invalidates it. */
@@ -298,6 +320,22 @@ int gt_render_aovs_device(gt_context* ctx, const gt_scene* scene, const gt_camer
                           const gt_lights* lights, const gt_env* env,
                           float* d_beauty, float* d_albedo, float* d_normal,
                           gt_render_stats* out_stats);
+
+/* ---- rendering a subset of the pixels ----------------------------------
+ * No direct reference counterpart: the reference renders whole images or
+ * tiles (render_tiles, camera.rs:3272-3294). Renders only the n listed
+ * pixels (indices y * W + x, each < W*H, none listed twice: GT_EINVAL
+ * otherwise) with the same per-(pixel, sample, bounce) streams, so a listed
+ * pixel's value equals the same pixel of gt_render_hdr_device bit for bit.
+ * Used for converged references on crops. pixels is a host array; d_out is
+ * the device image (W*H*3 float, or double when out_is_double) of which only
+ * the listed pixels are written; the result is on the context's stream.
+ * stats->camera_paths counts n * spp. An adaptive camera samples the listed
+ * pixels adaptively. */
+int gt_render_pixels_device(gt_context* ctx, const gt_scene* scene, const gt_camera* camera,
+                            const gt_lights* lights, const gt_env* env, const uint32_t* pixels,
+                            size_t n, void* d_out, int out_is_double,
+                            gt_render_stats* out_stats /* nullable */);
 
 /* ---- progressive rendering ----------------------------------------------
  * RayCamera::render_world*_tiled_progressive (camera.rs:3172-3294,

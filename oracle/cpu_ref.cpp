@@ -221,6 +221,9 @@ struct Aabb {
 
 // ---- primitives (src/gmath/geometry.rs, src/graphics/raytracing/object.rs) -
 enum { K_SPHERE = 0, K_MOVING_SPHERE = 1, K_TRIANGLE = 2, K_QUAD = 3 };
+// cr_scene_add_prims kind of a mesh triangle with vertex normals (MeshTriangle,
+// mesh.rs:24-151): stored as a K_TRIANGLE with `smooth` set.
+enum { K_MESH_TRIANGLE = 4 };
 
 struct Prim {
   int kind = K_SPHERE;
@@ -228,6 +231,8 @@ struct Prim {
   double radius = 0;
   V3 qn, qw;         // quad normal and w (geometry.rs:293-312)
   double qd = 0;     // quad plane d
+  bool smooth = false;  // mesh triangle: vertex normals na, nb, nc (mesh.rs:24-151)
+  V3 na, nb, nc;
   uint32_t material = 0;
 };
 
@@ -253,6 +258,7 @@ static inline V3 moving_center(const Prim& p, double time) { return p.a + time *
 
 struct Surf {
   V3 point, normal;
+  V3 shading;        // HitRecord::shading_normal: the normal unless a mesh sets it (object.rs:470-479)
   double t = 0, u = 0, v = 0;
   bool front = false;
 };
@@ -262,6 +268,7 @@ static inline Surf with_uv(const Ray& r, V3 p, V3 outward, double t, double u, d
   s.point = p; s.t = t; s.u = u; s.v = v;
   s.front = dot(r.d, outward) < 0.0;
   s.normal = s.front ? outward : -outward;
+  s.shading = s.normal;
   return s;
 }
 
@@ -344,6 +351,14 @@ static bool prim_intersect(const Prim& p, const Ray& r, double tmin, double tmax
       V3 normal = normalized(cross(p.b - p.a, p.c - p.a));  // geometry.rs:176-185
       if (length_squared(normal) <= EPS) return false;
       *s = with_uv(r, r.at(h.t), normal, h.t, h.u, h.v);
+      if (p.smooth) {  // MeshTriangle::hit / shading_normal (mesh.rs:53-119), set_shading_normal (object.rs:470-479)
+        const double w = 1.0 - h.u - h.v;
+        V3 sn = (w * p.na) + (h.u * p.nb) + (h.v * p.nc);
+        if (length_squared(sn) > EPS) {
+          sn = normalized(sn);
+          s->shading = s->front ? sn : -sn;
+        }
+      }
       return true;
     }
     default: {
@@ -412,12 +427,26 @@ static double schlick(double cosine, double f0) {  // sampling.rs:255-261
   return f0 + (1.0 - f0) * pow5(1.0 - clampd(cosine, 0.0, 1.0));
 }
 
-struct MatPdf {  // MaterialPdf::{Cosine, GgxReflection} (pdf.rs:46-104)
-  int kind = 0;  // 0 cosine, 1 ggx
+// HenyeyGreensteinPdf::phase_value (sampling.rs:126-136)
+static double hg_phase(double cosine, double g) {
+  if (!std::isfinite(cosine) || !std::isfinite(g) || std::fabs(g) >= 1.0) return 0.0;
+  double den = 1.0 + g * g - 2.0 * g * clampd(cosine, -1.0, 1.0);
+  if (den <= EPS) return 0.0;
+  return (1.0 - g * g) / (4.0 * PI * std::pow(den, 1.5));
+}
+struct MatPdf {  // MaterialPdf::{Cosine, GgxReflection, Sphere, HenyeyGreenstein} (pdf.rs:46-104)
+  int kind = 0;  // 0 cosine, 1 ggx, 2 sphere, 3 henyey-greenstein
   Onb basis;
   V3 outgoing;
   double roughness = 0, alpha = 0;
+  double g = 0;
   double value(V3 d) const {
+    if (kind == 2) return 1.0 / (4.0 * PI);  // SpherePdf (sampling.rs:33-36)
+    if (kind == 3) {  // sampling.rs:139-147
+      double len = length(d);
+      if (len <= EPS) return 0.0;
+      return hg_phase(dot(d, basis.w) / len, g);
+    }
     if (kind == 0) {  // sampling.rs:65-77
       double len = length(d);
       if (len <= EPS) return 0.0;
@@ -435,6 +464,20 @@ struct MatPdf {  // MaterialPdf::{Cosine, GgxReflection} (pdf.rs:46-104)
   // *below (when given) reports a GGX reflection below the surface, where
   // the reference returns the outgoing direction (sampling.rs:313-318).
   V3 generate(Rng& rng, bool* below = nullptr) const {
+    if (kind == 2) return rng.unit_vector_spherical();  // SpherePdf (sampling.rs:38-40)
+    if (kind == 3) {  // sampling.rs:149-165
+      double r1 = rng.random_double(), r2 = rng.random_double();
+      double ct;
+      if (std::fabs(g) < 1e-3) {
+        ct = 1.0 - 2.0 * r1;
+      } else {
+        double term = (1.0 - g * g) / (1.0 - g + 2.0 * g * r1);
+        ct = clampd((1.0 + g * g - term * term) / (2.0 * g), -1.0, 1.0);
+      }
+      double st = std::sqrt(fmaxr(1.0 - ct * ct, 0.0));
+      double phi = 2.0 * PI * r2;
+      return basis.local(V3(std::cos(phi) * st, std::sin(phi) * st, ct));
+    }
     if (kind == 0) return basis.local(rng.cosine_direction());  // sampling.rs:79-81
     double r1 = rng.random_double(), r2 = rng.random_double();  // sampling.rs:283-307
     double phi = 2.0 * PI * r1;
@@ -483,11 +526,12 @@ static double ggx_weight(V3 normal, V3 outgoing, V3 incoming, double roughness, 
 static double specular_luminance(V3 c) { return clampd(0.2126 * c.x + 0.7152 * c.y + 0.0722 * c.z, 0.0, 1.0); }  // :1724-1726
 
 // ---- materials (src/graphics/raytracing/material.rs) -------------------------
-enum { M_LAMBERTIAN = 0, M_DIFFUSE_LIGHT = 1, M_GGX = 2, M_METAL = 3, M_DIELECTRIC = 4 };
+enum { M_LAMBERTIAN = 0, M_DIFFUSE_LIGHT = 1, M_GGX = 2, M_METAL = 3, M_DIELECTRIC = 4, M_ISOTROPIC = 5,
+       M_HENYEY_GREENSTEIN = 6 };
 struct Mat {
   int kind = M_LAMBERTIAN;
   V3 color;
-  double param = 0;  // ggx roughness, metal fuzz, dielectric ior
+  double param = 0;  // ggx roughness, metal fuzz, dielectric ior, henyey-greenstein g
 };
 
 struct Hit {
@@ -505,7 +549,7 @@ struct Scatter {
 
 static Scatter mat_scatter(const Mat& m, const Ray& in, const Hit& h, Rng& rng) {  // dispatch :2646-2661
   Scatter sc;
-  const V3 shading = h.s.normal;  // RayScene: shading normal = geometric (object.rs:435-450)
+  const V3 shading = h.s.shading;  // the geometric normal unless a mesh triangle interpolates one
   switch (m.kind) {
     case M_LAMBERTIAN: {  // :508-520
       MatPdf p;
@@ -544,20 +588,40 @@ static Scatter mat_scatter(const Mat& m, const Ray& in, const Hit& h, Rng& rng) 
       sc.kind = 1; sc.ray = {h.s.point, dir, in.time}; sc.attenuation = V3(1.0, 1.0, 1.0);
       return sc;
     }
+    case M_ISOTROPIC: {  // :1123-1137
+      sc.kind = 2; sc.attenuation = m.color; sc.pdf.kind = 2;
+      return sc;
+    }
+    case M_HENYEY_GREENSTEIN: {  // :1270-1284, HenyeyGreensteinPdf::new (sampling.rs:96-112)
+      MatPdf p;
+      p.kind = 3;
+      p.g = m.param;
+      if (!onb_from_w(in.d, &p.basis)) return sc;
+      sc.kind = 2; sc.attenuation = m.color; sc.pdf = p;
+      return sc;
+    }
     default:
       return sc;  // DiffuseLight does not scatter
   }
 }
 static double mat_scattering_pdf(const Mat& m, const Ray& in, const Hit& h, V3 scattered) {
   if (m.kind == M_LAMBERTIAN) {  // :522-529
-    double c = dot(h.s.normal, normalized(scattered));
+    double c = dot(h.s.shading, normalized(scattered));
     return c <= 0.0 ? 0.0 : c / PI;
   }
   if (m.kind == M_GGX) {  // :1596-1612
     V3 outgoing = -normalized(in.d);
     V3 incoming = normalized(scattered);
     if (dot(incoming, h.s.normal) <= 0.0 || dot(outgoing, h.s.normal) <= 0.0) return 0.0;
-    return ggx_weight(h.s.normal, outgoing, incoming, m.param, specular_luminance(m.color));
+    return ggx_weight(h.s.shading, outgoing, incoming, m.param, specular_luminance(m.color));
+  }
+  if (m.kind == M_ISOTROPIC) return 1.0 / (4.0 * PI);  // :1139-1141
+  if (m.kind == M_HENYEY_GREENSTEIN) {  // :1286-1291
+    MatPdf p;
+    p.kind = 3;
+    p.g = m.param;
+    if (!onb_from_w(in.d, &p.basis)) return 0.0;
+    return p.value(scattered);
   }
   return 0.0;
 }
@@ -1358,7 +1422,7 @@ static void render_aov_pixel(const Ctx& c, uint32_t x, uint32_t y, V3* albedo, V
     Ray rn = cam.sample_ray(x, y, s, pn.rng);
     Hit h;
     if (c.scene->hit(ra, SHADOW_ACNE_EPSILON, INF, &h)) sa = sa + mat_denoise_albedo(*h.mat);
-    if (c.scene->hit(rn, SHADOW_ACNE_EPSILON, INF, &h)) sn = sn + 0.5 * (h.s.normal + V3(1.0, 1.0, 1.0));
+    if (c.scene->hit(rn, SHADOW_ACNE_EPSILON, INF, &h)) sn = sn + 0.5 * (h.s.shading + V3(1.0, 1.0, 1.0));
   }
   *albedo = color_div(sa, double(cam.spp));  // camera.rs:3095
   *normal = color_div(sn, double(cam.spp));
@@ -1395,6 +1459,15 @@ int cr_scene_add_material(void* sp, int kind, double r, double g, double b, doub
     m.param = clampd(param, 0.0, 1.0);  // material.rs:1962-1975
   } else if (kind == M_DIELECTRIC) {
     m.color = V3(1.0, 1.0, 1.0);
+  } else if (kind == M_HENYEY_GREENSTEIN) {  // validate_henyey_greenstein_g (material.rs:1328-1333)
+    if (!(std::isfinite(param) && std::fabs(param) < 1.0)) {
+      g_err = "Henyey-Greenstein anisotropy must be finite and in (-1, 1)";
+      return GT_EINVAL;
+    }
+  } else if (kind != M_LAMBERTIAN && kind != M_DIFFUSE_LIGHT && kind != M_GGX && kind != M_METAL &&
+             kind != M_ISOTROPIC) {
+    g_err = "unknown material kind";
+    return GT_EINVAL;
   }
   *id = uint32_t(s->mats.size());
   s->mats.push_back(m);
@@ -1411,6 +1484,13 @@ int cr_scene_add_prims(void* sp, int kind, const double* d, size_t n, const uint
     else if (kind == K_MOVING_SPHERE) {
       const double* q = d + 7 * i;
       p = make_moving_sphere(V3(q[0], q[1], q[2]), V3(q[3], q[4], q[5]), q[6]);
+    } else if (kind == K_MESH_TRIANGLE) {  // positions (9) then vertex normals (9)
+      const double* q = d + 18 * i;
+      p = make_triangle(V3(q[0], q[1], q[2]), V3(q[3], q[4], q[5]), V3(q[6], q[7], q[8]));
+      p.smooth = true;
+      p.na = V3(q[9], q[10], q[11]);
+      p.nb = V3(q[12], q[13], q[14]);
+      p.nc = V3(q[15], q[16], q[17]);
     } else {
       const double* q = d + 9 * i;
       if (kind == K_TRIANGLE) p = make_triangle(V3(q[0], q[1], q[2]), V3(q[3], q[4], q[5]), V3(q[6], q[7], q[8]));

@@ -121,27 +121,34 @@ def test_batch_rejects_mixed_contexts(gpu_context):
 
 
 def test_graph_replays_equal_eager_renders(gpu_context):
-    """With gt_tuning.use_graph the launch sequence is captured once and
-    replayed; every replay, with eager renders of other frames in between,
-    must reproduce the eager image exactly (single and batched renders)."""
-    say("graph: capture + 3 replays, single and batched")
+    """With gt_tuning.use_graph (the default) a render's launch sequence runs
+    eagerly the first time, is captured the second time it repeats and is
+    replayed after that; renders with use_graph off in between do not disturb
+    it. Every render (first sight, capture, two replays, single and batched)
+    must reproduce the eager image exactly."""
+    say("graph: first sight, capture, 2 replays; single and batched")
     frames = _frames()
-    eager = _singles(frames)
-    eager_batch = [im.pixels.copy() for im in api.render_batch(frames[:3])]
     world, cam, L, env = frames[2]
     try:
-        gpu_context.set_tuning(use_graph=1)
-        for rep in range(3):
+        gpu_context.set_tuning(use_graph=0)
+        eager = _singles(frames)
+        eager_batch = [im.pixels.copy() for im in api.render_batch(frames[:3])]
+        for rep in range(4):
+            gpu_context.set_tuning(use_graph=1)
             img = api.PathTracer(cam)._render(world, cam, L, env).pixels
-            np.testing.assert_array_equal(img, eager[2], err_msg=f"single replay {rep}")
+            np.testing.assert_array_equal(img, eager[2], err_msg=f"single render {rep}")
             gpu_context.set_tuning(use_graph=0)
             other = api.PathTracer(frames[0][1])._render(*frames[0]).pixels  # eager work in between
             np.testing.assert_array_equal(other, eager[0])
+        for rep in range(4):
             gpu_context.set_tuning(use_graph=1)
             for i, im in enumerate(api.render_batch(frames[:3])):
-                np.testing.assert_array_equal(im.pixels, eager_batch[i], err_msg=f"batch replay {rep} frame {i}")
+                np.testing.assert_array_equal(im.pixels, eager_batch[i], err_msg=f"batch render {rep} frame {i}")
+            gpu_context.set_tuning(use_graph=0)
+            other = api.PathTracer(frames[1][1])._render(*frames[1]).pixels
+            np.testing.assert_array_equal(other, eager[1])
     finally:
-        gpu_context.set_tuning(use_graph=0)
+        gpu_context.set_tuning(use_graph=1)
 
 
 def test_autotune_picks_a_valid_tuning_and_keeps_images(gpu_context):
