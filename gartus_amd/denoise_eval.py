@@ -1,29 +1,38 @@
-"""Time to a target relMSE: the plain path tracer against path tracer + denoiser.
+"""Time to a target relMSE on the 1920x1080 GGX/environment frame: plain
+path tracing against path tracing + the learned denoiser.
 
-On the 1920x1080 GGX/environment frame:
-  - reference: the average of two independent 16,384-spp renders,
-    accumulated in FP64 on the GPU. Their disagreement gives the reference's
-    own error: relMSE(a, b) / 4 for the average;
-  - plain: renders at increasing spp, each timed (device work plus launch,
-    synchronised) and scored by relMSE against the reference;
-  - denoised: a low-spp render with its albedo/normal AOVs plus one network
-    inference, timed together and scored the same way.
+Error. relMSE(x, r) = mean over pixels and channels of (x - r)^2 / (r^2 + eps),
+eps = 1e-2, over 16 fixed crops of 64x64 pixels on a 4x4 grid of the frame
+(65,536 pixels, 3.2% of the frame), reported in full and trimmed (the 0.1% of
+crop pixels with the largest error left out). Every method renders the whole
+frame; only the crops are scored.
 
-Error is relMSE (eps 1e-2) in two forms: over every pixel, and without the
-0.1% of pixels with the largest error. The time to target uses the second.
-The sun is 5e4 over 0.5 degrees, and specular or glossy paths that reach it
-carry single samples of 5e4 or more. Those caustics make the plain mean
-converge slowly and leave the reference with an error of its own, reported
-beside the results (with heavy tails relMSE(a, b) / 4 is only indicative:
-relMSE is asymmetric in its arguments). Time to a target is
-interpolated on log(time) against log(relMSE) between the two measured
-points that bracket it.
+Reference. The crops rendered at 2 x 2^20 spp, accumulated in FP64 on the
+GPU (gt_render_pixels_device renders only the listed pixels, each equal to
+the same pixel of a full render), from two seeds that no evaluated render
+uses. It is made once by `python -m gartus_amd.denoise_eval
+--make-reference` and committed (gartus_amd/data/c3_crops_reference.npz):
+2^21 spp on the crops is about a minute of GPU time. Its own error is
+estimated from its two halves as relMSE(a, b) / 4, and a CPU oracle
+reference at 16k spp on the crops' centres cross-checks it
+(tests/golden/c3_crops_cpu16k.npz). A target is reported only when it is at
+least 5 times that error.
 
-With the reference's estimator, below-horizon GGX samples fall back to a
-retro-reflection and compound (gt_camera flags); that evaluation runs by
-default. --reject-ggx-fallback evaluates the well-posed estimator.
+Estimator. Both methods use GT_FLAG_GGX_REJECT_BELOW_HORIZON: with the
+reference's below-horizon GGX fallback the estimator's variance is unbounded
+on this scene (a single sample reaches 3e8 against a median pixel of 0.3), so
+no method converges and no reference exists.
 
-    python -m gartus_amd.denoise_eval --weights W.npy [--ref-spp 16384] [--json out.json]
+Timing. plain at S spp: the full frame's render time, measured directly up
+to 4096 spp and as the measured time per sample times S above (a render is
+a sequence of equal batches, linear in S); the error at S is measured on the
+crops rendered at S spp. denoised: the full frame at s spp with its AOVs plus
+one inference, timed together. The time to a target is interpolated between
+the measured points that bracket it on log-log axes; a target no measured
+point reaches is reported as not reached, never extrapolated.
+
+    python -m gartus_amd.denoise_eval --weights W.npy [--json out.json]
+    python -m gartus_amd.denoise_eval --make-reference
 """
 from __future__ import annotations
 
@@ -32,6 +41,7 @@ import json
 import math
 import sys
 import time
+from pathlib import Path
 
 import numpy as np
 
@@ -39,35 +49,88 @@ from . import abi, api, scenes
 from .denoise_train import DeviceArray, SceneHandle, relmse, relmse_trimmed
 from .denoiser import Denoiser
 
-TARGET = 0.01
+DATA = Path(__file__).resolve().parent / "data"
+REFERENCE_FILE = DATA / "c3_crops_reference.npz"
+FLAGS = abi.GT_FLAG_GGX_REJECT_BELOW_HORIZON
+WIDTH, HEIGHT = 1920, 1080
+CROP = 64
+REF_SEEDS = (987_654_321, 123_456_789)
+REF_HALF_SPP = 1 << 20
+TARGETS = (0.01, 0.02, 0.03, 0.05, 0.1)
 
 
-def _timed(fn, reps: int = 1) -> float:
+def crop_origins():
+    """Top-left corners of the 16 scored crops: a 4x4 grid over the frame."""
+    xs = [80 + i * 464 for i in range(4)]
+    ys = [40 + j * 264 for j in range(4)]
+    return [(x, y) for y in ys for x in xs]
+
+
+def crop_pixels(size: int = CROP, centre: int = 0) -> np.ndarray:
+    """Pixel indices (y * W + x) of the crops, crop by crop, row-major; with
+    centre > 0, only the centre x centre pixels of each crop."""
+    out = []
+    for x0, y0 in crop_origins():
+        off = (size - centre) // 2 if centre else 0
+        n = centre or size
+        ys, xs = np.mgrid[y0 + off:y0 + off + n, x0 + off:x0 + off + n]
+        out.append((ys * WIDTH + xs).ravel())
+    return np.concatenate(out).astype(np.uint32)
+
+
+def frame(ctx: api.Context):
+    data, env, cam = scenes.ggx_environment()
+    cam["width"] = WIDTH
+    return SceneHandle(ctx, data, env, cam, FLAGS)
+
+
+def _camera(sh: SceneHandle, spp: int, seed: int) -> api.RayCamera:
+    c = api.RayCamera.__new__(api.RayCamera)
+    c.raw = sh.camera(spp, seed)
+    return c
+
+
+def render_crops(sh: SceneHandle, spp: int, seed: int, pixels: np.ndarray) -> np.ndarray:
+    """The listed pixels at spp, FP64 accumulation: (n, 3)."""
+    return api.render_pixels(sh.scene, _camera(sh, spp, seed), pixels, environment=sh.env, double=True)
+
+
+def make_reference(ctx: api.Context, half_spp: int = REF_HALF_SPP, log=print, out: Path = REFERENCE_FILE) -> dict:
+    sh = frame(ctx)
+    pix = crop_pixels()
+    halves = []
+    for seed in REF_SEEDS:
+        t0 = time.perf_counter()
+        halves.append(render_crops(sh, half_spp, seed, pix))
+        log(f"reference half (seed {seed}): {len(pix)} pixels x {half_spp} spp in {time.perf_counter() - t0:.1f} s")
+    a, b = halves
+    out = Path(out)
+    out.parent.mkdir(parents=True, exist_ok=True)
+    np.savez_compressed(out, pixels=pix, half_a=a.astype(np.float32), half_b=b.astype(np.float32),
+                        half_spp=np.int64(half_spp), seeds=np.array(REF_SEEDS, np.int64), flags=np.int64(FLAGS))
+    err = relmse(a, b) / 4.0
+    log(f"saved {out}: relMSE between halves {relmse(a, b):.4g}, own error of the mean ~{err:.4g}")
+    return {"pixels": pix, "ref": 0.5 * (a + b), "own_error": err}
+
+
+def load_reference() -> dict:
+    z = np.load(REFERENCE_FILE)
+    a, b = z["half_a"].astype(np.float64), z["half_b"].astype(np.float64)
+    return {"pixels": z["pixels"], "ref": 0.5 * (a + b), "half_a": a, "half_b": b, "half_spp": int(z["half_spp"]),
+            "own_error": relmse(a, b) / 4.0, "own_error_trimmed": relmse_trimmed(a, b) / 4.0}
+
+
+def _timed(fn) -> float:
     """Wall time of fn; every call used here returns after its device work."""
     t0 = time.perf_counter()
-    for _ in range(reps):
-        fn()
-    return (time.perf_counter() - t0) / reps
+    fn()
+    return time.perf_counter() - t0
 
 
-def time_to_target(points, target: float = TARGET):
-    """points: [(time_s, relmse)] ordered by cost. Returns (time, how); the
-    time is None when no measured point reaches the target (no
-    extrapolation: trimmed relMSE is not monotone in spp on this scene)."""
-    pts = sorted(points)
-    for (t0, e0), (t1, e1) in zip(pts, pts[1:]):
-        if e0 > target >= e1:
-            f = (math.log(e0) - math.log(target)) / (math.log(e0) - math.log(e1))
-            return math.exp(math.log(t0) + f * (math.log(t1) - math.log(t0))), "interpolated"
-    hit = [t for t, e in pts if e <= target]
-    if hit:
-        return min(hit), "measured"
-    return None, "not reached by any measured point"
-
-
-def _reached(points, target):
-    """Interpolated time at which the measured series first reaches target
-    (None when no measured point does)."""
+def reached(points, target):
+    """Time at which a series of (time, error) points first reaches target:
+    log-log interpolation between the bracketing measured points; None when
+    no measured point reaches it."""
     pts = sorted(points)
     if pts and pts[0][1] <= target:
         return pts[0][0]
@@ -78,102 +141,99 @@ def _reached(points, target):
     return None
 
 
-def evaluate(dn: Denoiser, ref_spp: int = 16384, plain_spp=(4, 16, 64, 256, 1024, 4096),
-             denoised_spp=(1, 2, 4, 8, 16), width: int = 1920, flags: int = 0, log=print):
+def evaluate(dn: Denoiser, plain_spp=(4, 16, 64, 256, 1024, 4096, 16384, 65536, 262144),
+             denoised_spp=(1, 2, 4, 16, 64, 256), direct_max_spp: int = 4096, log=print) -> dict:
     ctx = dn.ctx
-    data, env, cam = scenes.ggx_environment()
-    cam["width"] = width
-    sh = SceneHandle(ctx, data, env, cam, flags)
+    ref = load_reference()
+    pix, r = ref["pixels"], ref["ref"]
+    sh = frame(ctx)
     h, w = sh.h, sh.w
-    # two independent references; their average is the reference, and their
-    # disagreement measures its own error: for independent halves a and b,
-    # relMSE(a, b) ~ 2 e(N) and the average has e(2N) ~ relMSE(a, b) / 4
-    ref_d = DeviceArray(ctx, (h, w, 3), np.float64)
-    t0 = time.perf_counter()
-    sh.render(ref_d, ref_spp, seed=987_654_321, double=True)
-    ref_a = ref_d.numpy()
-    sh.render(ref_d, ref_spp, seed=123_456_789, double=True)
-    ref_b = ref_d.numpy()
-    ref = 0.5 * (ref_a + ref_b)
-    floor = relmse(ref_a, ref_b) / 4.0
-    floor_t = relmse_trimmed(ref_a, ref_b) / 4.0
-    del ref_a, ref_b
-    log(f"reference: 2 x {ref_spp} spp in {time.perf_counter() - t0:.1f} s; its own error: relMSE {floor:.4g}, "
-        f"trimmed {floor_t:.4g}")
+    log(f"reference: {len(pix)} crop pixels at 2 x {ref['half_spp']} spp; its own error: relMSE "
+        f"{ref['own_error']:.3g}, trimmed {ref['own_error_trimmed']:.3g}")
     img = DeviceArray(ctx, (h, w, 3))
-    plain = []
     sh.render(img, 1, seed=5)  # warm
+    # full-frame time per sample, from the largest directly timed render
+    plain = []
+    per_sample = None
     for spp in plain_spp:
-        t = _timed(lambda: sh.render(img, spp, seed=1000 + spp))
-        x = img.numpy()
-        e, et = relmse(x, ref), relmse_trimmed(x, ref)
-        plain.append({"spp": spp, "time_s": t, "relmse": e, "relmse_trimmed": et})
-        log(f"plain {spp:5d} spp: {t * 1e3:8.2f} ms relMSE {e:.5g} trimmed {et:.5f}")
+        seed = 1000 + spp
+        if spp <= direct_max_spp:
+            t = _timed(lambda: sh.render(img, spp, seed=seed))
+            per_sample = t / spp
+            x = img.numpy().reshape(-1, 3)[pix.astype(np.int64)].astype(np.float64)
+            how = "measured"
+        else:
+            x = render_crops(sh, spp, seed, pix)
+            t = per_sample * spp
+            how = "crop pixels rendered at this spp; time = measured full-frame time per sample x spp"
+        e, et = relmse(x, r), relmse_trimmed(x, r)
+        plain.append({"spp": spp, "time_s": t, "time_how": how, "relmse": e, "relmse_trimmed": et})
+        log(f"plain {spp:7d} spp: {t:9.3f} s relMSE {e:.5g} trimmed {et:.5g}")
     b, a, nr, out = (DeviceArray(ctx, (1, h, w, 3)) for _ in range(4))
-    den = []
     sh.render_aovs(b.sub(0), a.sub(0), nr.sub(0), 1, seed=7)  # warm
     dn.infer(b, a, nr, out=out)
+    den = []
     for spp in denoised_spp:
         def run():
             sh.render_aovs(b.sub(0), a.sub(0), nr.sub(0), spp, seed=2000 + spp)
             dn.infer(b, a, nr, out=out)
         t = _timed(run)
         t_inf = _timed(lambda: dn.infer(b, a, nr, out=out))
-        y, x = out.numpy()[0], b.numpy()[0]
-        e, et = relmse(y, ref), relmse_trimmed(y, ref)
-        e_in = relmse_trimmed(x, ref)
+        y = out.numpy()[0].reshape(-1, 3)[pix.astype(np.int64)].astype(np.float64)
+        x = b.numpy()[0].reshape(-1, 3)[pix.astype(np.int64)].astype(np.float64)
+        e, et = relmse(y, r), relmse_trimmed(y, r)
         den.append({"spp": spp, "time_s": t, "infer_s": t_inf, "relmse": e, "relmse_trimmed": et,
-                    "relmse_trimmed_input": e_in})
-        log(f"denoised {spp:3d} spp: {t * 1e3:8.2f} ms (inference {t_inf * 1e3:.2f} ms) relMSE {e:.5g} "
-            f"trimmed {et:.5f} (input trimmed {e_in:.5f})")
-    tp, how_p = time_to_target([(p["time_s"], p["relmse_trimmed"]) for p in plain])
-    td, how_d = time_to_target([(p["time_s"], p["relmse_trimmed"]) for p in den])
-    # every target either series reaches; the reference's own error (above)
-    # bounds what differences are meaningful
+                    "relmse_input": relmse(x, r)})
+        log(f"denoised {spp:4d} spp: {t * 1e3:9.2f} ms (inference {t_inf * 1e3:.2f} ms) relMSE {e:.5g} "
+            f"trimmed {et:.5g} (input {relmse(x, r):.5g})")
+    floor = max(ref["own_error"], 1e-12)
     targets = {}
-    for target in (0.01, 0.02, 0.03, 0.04, 0.05, 0.06, 0.08, 0.1):
-        a = _reached([(p["time_s"], p["relmse_trimmed"]) for p in plain], target)
-        b = _reached([(p["time_s"], p["relmse_trimmed"]) for p in den], target)
-        targets[str(target)] = {"plain_s": a, "denoised_s": b,
-                                "speedup": (a / b) if (a is not None and b is not None) else None}
-    result = {
-        "frame": f"{w}x{h} GGX/HDR-environment, depth 10",
-        "camera_flags": flags,
-        "reference_spp": 2 * ref_spp,
-        "reference_error_relmse": floor,
-        "reference_error_relmse_trimmed": floor_t,
-        "time_to_targets": targets,
+    for target in TARGETS:
+        if target < 5.0 * floor:
+            targets[str(target)] = {"measurable": False,
+                                    "why": f"below 5x the reference's own error ({floor:.3g})"}
+            continue
+        row = {"measurable": True}
+        for metric in ("relmse", "relmse_trimmed"):
+            tp = reached([(p["time_s"], p[metric]) for p in plain], target)
+            td = reached([(p["time_s"], p[metric]) for p in den], target)
+            row[metric] = {"plain_s": tp, "denoised_s": td,
+                           "denoised_over_plain": (td / tp) if (tp and td) else None}
+        targets[str(target)] = row
+        log(f"time to relMSE {target}: plain {row['relmse']['plain_s']}, denoised {row['relmse']['denoised_s']} "
+            f"(trimmed: plain {row['relmse_trimmed']['plain_s']}, denoised {row['relmse_trimmed']['denoised_s']})")
+    best_den = min(den, key=lambda d: d["relmse"])
+    return {
+        "frame": f"{w}x{h} GGX/HDR-environment, depth 10, GT_FLAG_GGX_REJECT_BELOW_HORIZON",
+        "scored_pixels": f"{len(pix)} (16 crops of {CROP}x{CROP} on a 4x4 grid)",
         "relmse_eps": 1e-2,
-        "relmse_trim_fraction": 1e-3,
-        "target_relmse": TARGET,
-        "target_on": "relmse_trimmed",
+        "reference_spp": 2 * ref["half_spp"],
+        "reference_error_relmse": ref["own_error"],
+        "reference_error_relmse_trimmed": ref["own_error_trimmed"],
+        "target": 0.01,
+        "north_star_ratio": 1.0 / 8.0,
+        "time_to_targets": targets,
         "plain": plain,
         "denoised": den,
-        "plain_time_to_target_s": tp,
-        "plain_how": how_p,
-        "denoised_time_to_target_s": td,
-        "denoised_how": how_d,
-        "speedup": (tp / td) if (tp is not None and td is not None) else None,
+        "denoised_best_relmse": best_den["relmse"],
     }
-    log(f"time to trimmed relMSE {TARGET}: plain {tp} s ({how_p}), denoised {td} s ({how_d}); "
-        f"the reference's own error, estimated from its two halves: trimmed relMSE {floor_t:.4f}")
-    for k, v in targets.items():
-        log(f"time to trimmed relMSE {k}: plain {v['plain_s']}, denoised {v['denoised_s']}, speedup {v['speedup']}")
-    return result
 
 
 def main(argv=None) -> int:
-    ap = argparse.ArgumentParser()
-    ap.add_argument("--weights", required=True)
-    ap.add_argument("--ref-spp", type=int, default=16384)
+    ap = argparse.ArgumentParser(description=__doc__.split("\n\n")[0])
+    ap.add_argument("--weights", default=str(Path(__file__).resolve().parent / "weights" / "denoiser_c3.npy"))
     ap.add_argument("--json", default="")
-    ap.add_argument("--reject-ggx-fallback", action="store_true",
-                    help="render with GT_FLAG_GGX_REJECT_BELOW_HORIZON (the well-posed estimator)")
+    ap.add_argument("--make-reference", action="store_true", help="render and save the crop reference")
+    ap.add_argument("--half-spp", type=int, default=REF_HALF_SPP)
+    ap.add_argument("--out", default=str(REFERENCE_FILE), help="where --make-reference writes")
     args = ap.parse_args(argv)
-    dn = Denoiser(api.default_context())
+    ctx = api.default_context()
+    if args.make_reference:
+        make_reference(ctx, args.half_spp, log=lambda m: print(m, flush=True), out=args.out)
+        return 0
+    dn = Denoiser(ctx)
     dn.load(args.weights)
-    flags = abi.GT_FLAG_GGX_REJECT_BELOW_HORIZON if args.reject_ggx_fallback else 0
-    res = evaluate(dn, ref_spp=args.ref_spp, flags=flags, log=lambda m: print(m, flush=True))
+    res = evaluate(dn, log=lambda m: print(m, flush=True))
     if args.json:
         with open(args.json, "w") as f:
             json.dump(res, f, indent=1)
