@@ -39,9 +39,23 @@ sys.path.insert(0, str(ROOT))
 
 HBM_PEAK_GBS = 8000.0  # MI355X HBM3E
 # HBM bytes per extend launch measured with rocprofv3 --pmc FETCH_SIZE and
-# WRITE_SIZE, scaled by the same counters on a plain copy of known size
+# WRITE_SIZE, scaled by the same counters on a plain copy of known size, with
+# the L2 hit rate and shader-sequencer counters; each record carries the hash
+# of the kernel sources it was measured with
 TRAFFIC_FILE = ROOT / "profiles" / "extend_hbm_traffic.json"
+KERNEL_SOURCES = ["gartus_amd/csrc/kernels.hip", "gartus_amd/csrc/kernels.h", "gartus_amd/csrc/device.h",
+                  "gartus_amd/csrc/layout.h"]
 WORKLOADS = ["ggx_env_1080p", "cornell_256", "terrain_720p", "batch64"]
+
+
+def kernel_sources_sha() -> str:
+    """First 16 hex digits of the SHA-256 of the tracer kernels' sources: a
+    traffic record applies only to the kernels it was measured on."""
+    import hashlib
+    h = hashlib.sha256()
+    for rel in KERNEL_SOURCES:
+        h.update((ROOT / rel).read_bytes())
+    return h.hexdigest()[:16]
 
 
 def log(msg: str) -> None:
@@ -239,42 +253,275 @@ def summarize_counters(paths: list[str]) -> int:
     return 0
 
 
+def _short_kernel(name: str) -> str:
+    """'void gtk::(anonymous namespace)::k_extend<false, true, false>(gtk::Frame, ...)' -> 'k_extend<false, true, false>'."""
+    name = name.replace("void ", "").replace("gtk::(anonymous namespace)::", "")
+    depth = 0
+    for i, ch in enumerate(name):
+        depth += ch == "<"
+        depth -= ch == ">"
+        if ch == "(" and depth == 0:
+            return name[:i]
+    return name
+
+
+def _counter_sums(path: Path) -> dict:
+    """{kernel: {counter: [values per dispatch]}} of a rocprofv3 counter_collection.csv."""
+    import csv
+    out: dict = {}
+    with open(path, newline="") as f:
+        for row in csv.DictReader(f):
+            out.setdefault(_short_kernel(row["Kernel_Name"]), {}).setdefault(row["Counter_Name"], []).append(
+                float(row["Counter_Value"]))
+    return out
+
+
+# the counter passes behind a profile record, each a run of its own (OUT = the output folder)
+PROFILE_PASSES = {
+    "sq": "SQ_WAVE_CYCLES SQ_WAIT_ANY SQ_ACTIVE_INST_VALU SQ_THREAD_CYCLES_VALU SQ_INSTS_VALU SQ_BUSY_CYCLES SQ_WAVES "
+          "GRBM_GUI_ACTIVE",
+    "fetch": "FETCH_SIZE",
+    "write": "WRITE_SIZE TCC_HIT_sum TCC_MISS_sum",
+}
+TUNING_ARGS = {"batch64": "--block 64 --bvh-width 4 --max-paths 67108864"}
+
+
+def _profile_commands(workload: str) -> list[str]:
+    base = "python -u bench.py --steps 1 --warmup 0 --no-cpu-baseline --no-denoise-eval --graph off"
+    extra = (" " + TUNING_ARGS[workload]) if workload in TUNING_ARGS else ""
+    cmds = [f"rocprofv3 --pmc {ctrs} --output-format csv -d OUT/{p}_{workload} -o run -- {base} --workload {workload}"
+            f"{extra}" for p, ctrs in PROFILE_PASSES.items()]
+    cmds.append(f"rocprofv3 --kernel-trace --stats --output-format csv -d OUT/trace_{workload} -o run -- python -u "
+                f"bench.py --steps 5 --warmup 1 --no-cpu-baseline --no-denoise-eval --graph off --workload {workload}"
+                f"{extra}")
+    for c in ("FETCH_SIZE", "WRITE_SIZE"):
+        cmds.append(f"rocprofv3 --pmc {c} --output-format csv -d OUT/cal_{c.split('_')[0].lower()} -o run -- "
+                    f"python -u bench.py --calibrate-copy 1024 --steps 3")
+    cmds.append(f"python bench.py --record-profile OUT --workload {workload}")
+    return cmds
+
+
+def record_profile(out: Path, workload: str, calibrate_mib: int = 1024, dest: Path | None = None) -> int:
+    """Turns one profiling set of a workload (the passes of _profile_commands,
+    under `out`) into its traffic record in profiles/extend_hbm_traffic.json,
+    stamped with the kernel sources' hash, and a summary
+    profiles/<workload>_counters.md. The extend kernel's figures average
+    every timed (non-counting) extend dispatch, bounce 0 and later bounces
+    alike, as bench.py's roofline.avg_launch_ms does."""
+    import csv
+    sq = _counter_sums(out / f"sq_{workload}" / "run_counter_collection.csv")
+    fe = _counter_sums(out / f"fetch_{workload}" / "run_counter_collection.csv")
+    wr = _counter_sums(out / f"write_{workload}" / "run_counter_collection.csv")
+    cal_f = _counter_sums(out / "cal_fetch" / "run_counter_collection.csv")
+    cal_w = _counter_sums(out / "cal_write" / "run_counter_collection.csv")
+
+    def copy_mean(c, ctr):
+        vals = [v for k, d in c.items() if "neg_kernel" in k for v in d.get(ctr, [])]
+        return sum(vals) / len(vals)
+    kib = calibrate_mib * 1024.0
+    f_fetch = kib / copy_mean(cal_f, "FETCH_SIZE")
+    f_write = kib / copy_mean(cal_w, "WRITE_SIZE")
+
+    def family(prefix):
+        return [k for k in fe if k.startswith(prefix)]
+
+    def tot(c, kernels, ctr):
+        return sum(sum(c.get(k, {}).get(ctr, [])) for k in kernels)
+
+    def ndisp(c, kernels, ctr):
+        return sum(len(c.get(k, {}).get(ctr, [])) for k in kernels)
+
+    def row(kernels):
+        n = ndisp(fe, kernels, "FETCH_SIZE")
+        rd = tot(fe, kernels, "FETCH_SIZE") * f_fetch * 1024 / n
+        wt = tot(wr, kernels, "WRITE_SIZE") * f_write * 1024 / n
+        hit, miss = tot(wr, kernels, "TCC_HIT_sum"), tot(wr, kernels, "TCC_MISS_sum")
+        wave = tot(sq, kernels, "SQ_WAVE_CYCLES")
+        valu = tot(sq, kernels, "SQ_ACTIVE_INST_VALU")
+        return {"dispatches": n, "read_bytes": rd, "write_bytes": wt,
+                "l2_hit_rate": hit / (hit + miss) if hit + miss else None,
+                "wave_wait_frac": tot(sq, kernels, "SQ_WAIT_ANY") / wave if wave else None,
+                "valu_busy_frac": valu / wave if wave else None,
+                "valu_lane_util": tot(sq, kernels, "SQ_THREAD_CYCLES_VALU") / (64.0 * valu) if valu else None}
+
+    # the timed kernels: counting instantiations (first template argument true) left out
+    timed = {k for k in fe if k.startswith(("k_extend<false", "k_shade", "k_connect<false", "k_accumulate"))}
+    ext = sorted(k for k in timed if k.startswith("k_extend<false"))
+    e = row(ext)
+    # the same kernels' durations over the timed steps of the trace run
+    trace = out / f"trace_{workload}"
+    durs: dict = {}
+    with open(trace / "run_kernel_trace.csv", newline="") as f:
+        rows = sorted(csv.DictReader(f), key=lambda r: int(r["Dispatch_Id"]))
+    for r in rows:
+        durs.setdefault(_short_kernel(r["Kernel_Name"]), []).append(
+            (int(r["End_Timestamp"]) - int(r["Start_Timestamp"])) * 1e-3)
+    bench_line = None
+    log_file = out / f"trace_{workload}.log"
+    if log_file.exists():
+        for line in log_file.read_text().splitlines():
+            if line.startswith("{") and '"metric"' in line:
+                bench_line = json.loads(line)
+    if bench_line is None:
+        raise SystemExit(f"{log_file}: no bench.py line (the trace run's output); it names the kernel sources")
+    measured_sha = bench_line["roofline"].get("kernel_sources_sha16")
+    if measured_sha != kernel_sources_sha():
+        raise SystemExit(f"the profiles were measured on kernel sources {measured_sha}, these are "
+                         f"{kernel_sources_sha()}: profile them again")
+    steps, warmup = bench_line["steps"], bench_line["warmup"]
+    renders = 1 + warmup + steps  # the statistics render, the warm-up, the timed steps (the counting render aside)
+    ext_seq = [(int(r["Dispatch_Id"]), (int(r["End_Timestamp"]) - int(r["Start_Timestamp"])) * 1e-3)
+               for r in rows if _short_kernel(r["Kernel_Name"]) in ext]
+    per_render = len(ext_seq) // renders
+    timed_ext = [d for _, d in ext_seq[-steps * per_render:]]
+    avg_us = sum(timed_ext) / len(timed_ext)
+    bytes_per_launch = e["read_bytes"] + e["write_bytes"]
+    dest = dest or ROOT / "profiles"
+    traffic_file = dest / TRAFFIC_FILE.name
+    rec = json.loads(traffic_file.read_text()) if traffic_file.exists() else {}
+    rec[workload] = {
+        "kernel": "k_extend<false, *, *> (bounce 0 and later bounces)",
+        "kernel_sources_sha16": kernel_sources_sha(),
+        "bytes_per_launch": round(bytes_per_launch),
+        "fetch_factor": round(f_fetch, 4),
+        "write_factor": round(f_write, 4),
+        "l2_hit_rate": round(e["l2_hit_rate"], 4),
+        "wave_wait_frac": round(e["wave_wait_frac"], 4),
+        "valu_busy_frac": round(e["valu_busy_frac"], 4),
+        "valu_lane_util": round(e["valu_lane_util"], 4),
+        "rocprof_avg_launch_us": round(avg_us, 1),
+        "source": f"profiles/{workload}_counters.md",
+        "commands": _profile_commands(workload),
+    }
+    traffic_file.write_text(json.dumps(rec, indent=1) + "\n")
+
+    stats = {}
+    with open(trace / "run_kernel_stats.csv", newline="") as f:
+        for r in csv.DictReader(f):
+            stats[_short_kernel(r["Name"])] = r
+    lines = [f"# Counters per kernel: {workload}", "",
+             "Each pass is a run of its own (no trace domains with `--pmc`), from the repository root on one "
+             "MI355X with `TMPDIR=/tmp`; OUT is the output folder:", ""]
+    lines += ["    " + c for c in _profile_commands(workload)]
+    lines += ["", f"Calibration on a plain copy of {calibrate_mib} MiB read + {calibrate_mib} MiB written: "
+              f"FETCH_SIZE factor {f_fetch:.4f}, WRITE_SIZE factor {f_write:.4f} (HBM bytes = counter KiB x "
+              "1024 x factor). FETCH_SIZE counts what L2 requests from the memory side, so lines the "
+              "Infinity Cache serves are counted too: an upper bound on HBM reads.", "",
+              "Per dispatch means over the counter runs' timed renders (the counting render left out). Wait = "
+              "SQ_WAIT_ANY / SQ_WAVE_CYCLES (share of wave time waiting on anything, mostly memory); VALU busy = "
+              "SQ_ACTIVE_INST_VALU / SQ_WAVE_CYCLES; lane use = SQ_THREAD_CYCLES_VALU / (64 x "
+              "SQ_ACTIVE_INST_VALU), the active-lane share of VALU issue (divergence). Durations: mean of the "
+              "trace run's dispatches (all renders, from run_kernel_stats.csv).", "",
+              "| kernel | dispatches | mean us | HBM read MB | HBM write MB | GB/s | L2 hit | wait | VALU busy | "
+              "lane use |", "|---|---:|---:|---:|---:|---:|---:|---:|---:|---:|"]
+    order = sorted(timed, key=lambda k: -float(stats.get(k, {}).get("TotalDurationNs", 0) or 0))
+    for k in order:
+        r = row([k])
+        us = float(stats[k]["AverageNs"]) * 1e-3 if k in stats else float("nan")
+        gbs = (r["read_bytes"] + r["write_bytes"]) / (us * 1e-6) / 1e9 if us == us else float("nan")
+        f3 = (lambda v: "-" if v is None else f"{v:.3f}")
+        lines.append(f"| {k} | {r['dispatches']} | {us:.1f} | {r['read_bytes'] / 1e6:.1f} | "
+                     f"{r['write_bytes'] / 1e6:.1f} | {gbs:.0f} | {f3(r['l2_hit_rate'])} | "
+                     f"{f3(r['wave_wait_frac'])} | {f3(r['valu_busy_frac'])} | {f3(r['valu_lane_util'])} |")
+    lines += ["", "## The roofline kernel (extend)", "",
+              f"- HBM traffic per launch: {bytes_per_launch:,.0f} bytes (reads {e['read_bytes']:,.0f}, writes "
+              f"{e['write_bytes']:,.0f}), over {e['dispatches']} dispatches; L2 hit rate {e['l2_hit_rate']:.3f}.",
+              f"- Shader sequencer: waves wait {100 * e['wave_wait_frac']:.1f} % of their cycles, issue VALU "
+              f"{100 * e['valu_busy_frac']:.1f} %, with {100 * e['valu_lane_util']:.1f} % of lanes active.",
+              f"- Launch time over the trace run's {steps} timed steps ({len(timed_ext)} dispatches): rocprofv3 "
+              f"{avg_us:.1f} us; bench.py's own HIP events in the same run: roofline.avg_launch_ms = "
+              f"{1e3 * bench_line['roofline']['avg_launch_ms']:.1f} us; measured HBM rate {bytes_per_launch / (avg_us * 1e-6) / 1e9:,.0f} GB/s, "
+              f"{bytes_per_launch / (avg_us * 1e-6) / 1e9 / HBM_PEAK_GBS:.3f} of the 8 TB/s peak.",
+              f"- Kernel sources: sha256[:16] {kernel_sources_sha()} of {', '.join(KERNEL_SOURCES)}."]
+    lines += ["", "bench.py's line in the trace run (eager, instrumented):", "",
+              f"    {bench_line['value']} Mrays/s, {bench_line['ms_per_step']} ms per step, "
+              f"{bench_line['config']['workload']}"]
+    (dest / f"{workload}_counters.md").write_text("\n".join(lines) + "\n")
+    log(f"wrote {traffic_file} [{workload}] and {dest / (workload + '_counters.md')}")
+    return 0
+
+
 DENOISER_WEIGHTS = ROOT / "gartus_amd" / "weights" / "denoiser_c3.npy"
 
 
 def denoise_summary(ctx) -> dict:
-    """Time to trimmed relMSE targets on this frame, plain path tracing
-    against path tracing + the trained denoiser (gartus_amd.denoise_eval),
-    with GT_FLAG_GGX_REJECT_BELOW_HORIZON: the reference's own estimator
-    diverges on this scene (its GGX below-horizon fallback compounds)."""
-    from gartus_amd import abi
+    """Time to relMSE targets on this frame, plain path tracing against path
+    tracing + the trained denoiser (gartus_amd.denoise_eval): scored on 16
+    fixed 64x64 crops against their committed 2 x 2^20-spp FP64 reference,
+    with GT_FLAG_GGX_REJECT_BELOW_HORIZON (the reference's own estimator has
+    unbounded variance on this scene)."""
     from gartus_amd.denoise_eval import evaluate
     from gartus_amd.denoiser import Denoiser
     dn = Denoiser(ctx)
     dn.load(str(DENOISER_WEIGHTS))
-    r = evaluate(dn, ref_spp=16384, flags=abi.GT_FLAG_GGX_REJECT_BELOW_HORIZON, log=log)
+    r = evaluate(dn, log=log)
     dn.close()
-    best_plain = min(r["plain"], key=lambda p: p["relmse_trimmed"])
-    d4 = next(d for d in r["denoised"] if d["spp"] == 4)
+    t01 = r["time_to_targets"].get("0.01", {})
+    full = t01.get("relmse", {}) if t01.get("measurable") else {}
     return {
-        "metric": "time to trimmed relMSE (eps 1e-2, worst 0.1% of pixels left out)",
+        "metric": "time to relMSE (eps 1e-2) on 16 fixed 64x64 crops of the frame, full and trimmed (0.1%)",
         "estimator": "GT_FLAG_GGX_REJECT_BELOW_HORIZON",
         "reference_spp": r["reference_spp"],
-        "reference_error_relmse_trimmed": round(r["reference_error_relmse_trimmed"], 5),
+        "reference_error_relmse": round(r["reference_error_relmse"], 6),
+        "reference_error_relmse_trimmed": round(r["reference_error_relmse_trimmed"], 6),
         "time_to_targets_s": r["time_to_targets"],
-        "plain_best": {k: best_plain[k] for k in ("spp", "time_s", "relmse_trimmed")},
-        "denoised_4spp": {k: d4[k] for k in ("time_s", "infer_s", "relmse_trimmed", "relmse_trimmed_input")},
+        "relmse_0.01": {"plain_s": full.get("plain_s"), "denoised_s": full.get("denoised_s"),
+                        "denoised_over_plain": full.get("denoised_over_plain"), "north_star_max_ratio": 0.125},
+        "plain": [{k: p[k] for k in ("spp", "time_s", "relmse", "relmse_trimmed")} for p in r["plain"]],
+        "denoised": [{k: d[k] for k in ("spp", "time_s", "infer_s", "relmse", "relmse_trimmed", "relmse_input")}
+                     for d in r["denoised"]],
     }
 
 
 def traffic_for(workload: str) -> dict:
     """The extend kernel's measured traffic record for the workload (bytes
-    per launch below L2, L2 hit rate), or an empty dict."""
+    per launch below L2, L2 hit rate, shader counters), or an empty dict when
+    there is none or it was measured on other kernel sources."""
     try:
         rec = json.loads(TRAFFIC_FILE.read_text())
     except (OSError, ValueError):
         return {}
-    return rec.get(workload) or {}
+    r = rec.get(workload) or {}
+    if r.get("kernel_sources_sha16") != kernel_sources_sha():
+        return {"stale": r.get("kernel_sources_sha16")} if r else {}
+    return r
+
+
+def roofline_record(measured: dict, traffic, hbm_gbs, algorithmic_gbs: float, alg_bytes: float,
+                    avg_launch_ms: float) -> dict:
+    """The extend kernel against the HBM roofline. achieved = HBM bytes the
+    kernel moves per launch (rocprofv3 FETCH_SIZE + WRITE_SIZE, copy-
+    calibrated, measured on these kernel sources) over its live launch time.
+    The algorithmic bytes of the traversal (ray in, hit out, every node and
+    primitive fetch counted as if it came from memory) are reported beside
+    it: where the scene fits in the caches they are served by L2 and exceed
+    what HBM sees. Without a current traffic record the algorithmic figure
+    stands in, and `basis` says so."""
+    rec = {"bound": "hbm"}
+    if hbm_gbs is not None:
+        rec.update(achieved=round(hbm_gbs, 1), peak=HBM_PEAK_GBS, unit="GB/s", frac=round(hbm_gbs / HBM_PEAK_GBS, 4),
+                   traffic=traffic, basis="measured HBM traffic per launch / live launch time")
+    else:
+        rec.update(achieved=round(algorithmic_gbs, 1), peak=HBM_PEAK_GBS, unit="GB/s",
+                   frac=round(algorithmic_gbs / HBM_PEAK_GBS, 4), traffic=None,
+                   basis="algorithmic bytes (no cache reuse) / live launch time: no traffic record measured on "
+                         "these kernel sources" + (f" (record is for {measured['stale']})" if "stale" in measured
+                                                   else ""))
+    rec.update({
+        "kernel": "k_extend (closest-hit traversal)",
+        "kernel_sources_sha16": kernel_sources_sha(),
+        "avg_launch_ms": round(avg_launch_ms, 4),
+        "algorithmic_bytes_per_launch": round(alg_bytes),
+        "algorithmic_gbs": round(algorithmic_gbs, 1),
+        "l2_hit_rate": measured.get("l2_hit_rate"),
+        # shader-sequencer view of the same kernel (same rocprofv3 runs): share of wave cycles spent
+        # waiting on memory, VALU issue share, active-lane share of VALU instructions
+        "wave_wait_frac": measured.get("wave_wait_frac"),
+        "valu_busy_frac": measured.get("valu_busy_frac"),
+        "valu_lane_util": measured.get("valu_lane_util"),
+        "counters_source": measured.get("source"),
+    })
+    return rec
 
 
 def main() -> int:
@@ -288,7 +535,9 @@ def main() -> int:
     ap.add_argument("--block", type=int, default=0)
     ap.add_argument("--bvh-width", type=int, default=0)
     ap.add_argument("--max-paths", type=int, default=0)
-    ap.add_argument("--graph", action="store_true", help="replay the render as a captured hipGraph")
+    ap.add_argument("--graph", choices=["auto", "on", "off"], default="auto",
+                    help="time the render as a captured hipGraph replay (on), eagerly (off), or both with the "
+                         "library's default as the value (auto)")
     ap.add_argument("--adaptive", nargs=3, type=float, metavar=("MIN", "MAX", "THRESHOLD"),
                     help="single-frame workloads: render with adaptive sampling (camera.rs:2937-2982)")
     ap.add_argument("--progressive", type=int, default=0, metavar="TILE",
@@ -302,9 +551,14 @@ def main() -> int:
                     help="run only a plain device copy of MIB MiB (counter calibration)")
     ap.add_argument("--summarize-counters", nargs="+", metavar="CSV",
                     help="print per-kernel counter means from rocprofv3 CSV files and exit")
+    ap.add_argument("--record-profile", metavar="OUT",
+                    help="turn the workload's rocprofv3 passes under OUT into its traffic record and counter "
+                         "summary under profiles/, and exit")
     args = ap.parse_args()
     if args.summarize_counters:
         return summarize_counters(args.summarize_counters)
+    if args.record_profile:
+        return record_profile(Path(args.record_profile), args.workload)
     if args.calibrate_copy:
         return calibrate_copy(args.calibrate_copy, max(args.steps, 1))
     if args.gpus != 1:
@@ -369,7 +623,8 @@ def main() -> int:
     log(f"{n} scene(s) compiled in {build_wall:.2f} s ({'device LBVH, ' + format(build_device_ms, '.1f') + ' ms on the GPU' if args.device_build else 'host SAH'}): "
         f"{prims:,} prims, {nodes:,} BVH nodes")
     tuned = None
-    if args.autotune or args.workload == "batch64":
+    explicit = bool(args.block or args.bvh_width or args.max_paths)
+    if args.autotune or (args.workload == "batch64" and not explicit):
         best = abi.GtTuning()
         ms = C.c_double()
         t0 = time.perf_counter()
@@ -377,12 +632,10 @@ def main() -> int:
         tuned = {"block_size": best.block_size, "bvh_width": best.bvh_width, "max_paths": best.max_paths,
                  "stack_lds": best.stack_lds}
         log(f"autotune ({time.perf_counter() - t0:.1f} s): {tuned}, {ms.value:.2f} ms per render")
-    if args.graph:
-        ctx.set_tuning(use_graph=1)
+    default_graph = bool(ctx.tuning.use_graph)
 
-    def render(stats: bool = True, count: bool = False):
-        if count:
-            ctx.set_tuning(count_traversal=1, use_graph=0)
+    def render(stats: bool = True, count: bool = False, graph: bool = False):
+        ctx.set_tuning(count_traversal=1 if count else 0, use_graph=1 if graph and not count else 0)
         st = abi.GtRenderStats()
         sp = C.byref(st) if stats else None
         if n == 1:
@@ -390,27 +643,43 @@ def main() -> int:
                                                outs[0], 0, sp))
         else:
             abi.check(lib.gt_render_batch_device(ctx.handle, n, scenes_h, cams, lights_h, envs_h, outs, 0, sp))
-        if count:
-            ctx.set_tuning(count_traversal=0, use_graph=1 if args.graph else 0)
         return st
 
+    def timed(graph: bool):
+        """warmup, then the K timed steps between two synchronisations; eager
+        steps are instrumented (per-kernel HIP events), graph replays are not
+        (a render that asks for statistics is never captured). The library
+        captures a launch sequence the second time it sees it, so graph mode
+        warms up with one more render: capture and instantiation stay out of
+        the timed steps."""
+        for i in range(args.warmup + (1 if graph else 0)):
+            st = render(stats=not graph, graph=graph)
+            if not graph:
+                log(f"warmup {i}: {st.device_ms:.1f} ms device")
+        ctx.synchronize()
+        t0 = time.perf_counter()
+        sts = [render(stats=not graph, graph=graph) for _ in range(args.steps)]
+        ctx.synchronize()
+        return time.perf_counter() - t0, sts
+
     # one instrumented render gives the ray counts and per-kernel times (the
-    # counts are deterministic); with --graph the timed steps replay the
-    # captured graph uninstrumented
+    # counts are deterministic)
     ref = render(stats=True)
-    for i in range(args.warmup):
-        st = render(stats=not args.graph)
-        if not args.graph:
-            log(f"warmup {i}: {st.device_ms:.1f} ms device")
-    ctx.synchronize()
-    t0 = time.perf_counter()
-    stats = [render(stats=not args.graph) for _ in range(args.steps)]
-    ctx.synchronize()
-    wall = time.perf_counter() - t0
-    if not args.graph:
-        for i, st in enumerate(stats):
-            log(f"step {i}: {st.device_ms:.1f} ms device, extend {st.extend_ms:.1f} ms, shade {st.shade_ms:.1f} ms, "
-                f"connect {st.connect_ms:.1f} ms")
+    modes = {"off": [False], "on": [True], "auto": [False, True]}[args.graph]
+    walls = {}
+    stats = [ref]
+    for graph in modes:
+        w, sts = timed(graph)
+        walls[graph] = w
+        if not graph:
+            stats = sts
+            for i, st in enumerate(sts):
+                log(f"step {i}: {st.device_ms:.1f} ms device, extend {st.extend_ms:.1f} ms, "
+                    f"shade {st.shade_ms:.1f} ms, connect {st.connect_ms:.1f} ms")
+        log(f"{'graph replay' if graph else 'eager'}: {w / args.steps * 1e3:.3f} ms per step")
+    use_graph = {"off": False, "on": True, "auto": default_graph}[args.graph]
+    ctx.set_tuning(use_graph=1 if default_graph else 0)
+    wall = walls[use_graph]
     rays_step = ref.extension_rays + ref.shadow_rays
     rays = rays_step * args.steps
     ext_rays = ref.extension_rays
@@ -420,11 +689,11 @@ def main() -> int:
     launches = ref.extend_launches
     pb = prim_bytes(frames)
     ext_bytes = ext_rays * (32 + 16) + 64 * cst.ext_node_fetches + pb * cst.ext_prim_tests
-    timed = stats if not args.graph else [ref]
-    avg_launch_ms = sum(s.extend_ms for s in timed) / sum(s.extend_launches for s in timed)
-    achieved = (ext_bytes / launches) / (avg_launch_ms * 1e-3) / 1e9
+    avg_launch_ms = sum(s.extend_ms for s in stats) / sum(s.extend_launches for s in stats)
+    algorithmic_gbs = (ext_bytes / launches) / (avg_launch_ms * 1e-3) / 1e9
     measured = traffic_for(args.workload)
     traffic = measured.get("bytes_per_launch")
+    hbm_gbs = None if traffic is None else traffic / (avg_launch_ms * 1e-3) / 1e9
     if args.adaptive:  # paths traced, not W * H * max
         npaths = ref.camera_paths
     progressive = None
@@ -444,7 +713,9 @@ def main() -> int:
     cam0 = cams[0]
     config = {"workload": desc, "frames": n, "primitives": int(prims), "paths_per_step": int(npaths),
               "rays_per_step": int(rays_step), "samples_per_s": round(npaths * args.steps / wall, 1),
-              "hipgraph": bool(args.graph),
+              "hipgraph": use_graph,
+              "ms_per_step_eager": round(walls[False] / args.steps * 1e3, 3) if False in walls else None,
+              "ms_per_step_graph": round(walls[True] / args.steps * 1e3, 3) if True in walls else None,
               "scene_compile": {"builder": "device LBVH" if args.device_build else "host SAH",
                                 "wall_s": round(build_wall, 3),
                                 "device_ms": round(build_device_ms, 2) if args.device_build else None}}
@@ -474,26 +745,13 @@ def main() -> int:
         "dtype": "f32",
         "data": "synthetic (seeded procedural scenes and sky)",
         "config": config,
-        "roofline": {
-            "bound": "hbm",
-            "achieved": round(achieved, 1),
-            "peak": HBM_PEAK_GBS,
-            "unit": "GB/s",
-            "frac": round(achieved / HBM_PEAK_GBS, 4),
-            "traffic": traffic,
-            "kernel": "k_extend (closest-hit traversal)",
-            "bytes_per_launch": round(ext_bytes / launches),
-            "avg_launch_ms": round(avg_launch_ms, 4),
-            # HBM bytes actually moved per launch (rocprofv3 FETCH_SIZE + WRITE_SIZE, copy-calibrated)
-            # over the live launch time: BVH and primitives that fit in L2 are served from there, so
-            # this can be far below the algorithmic figure
-            "hbm_measured_gbs": None if traffic is None else round(traffic / (avg_launch_ms * 1e-3) / 1e9, 1),
-            # share of the extend kernel's L2 requests that hit (rocprofv3 TCC_HIT_sum, TCC_MISS_sum)
-            "l2_hit_rate": measured.get("l2_hit_rate"),
-        },
+        "roofline": roofline_record(measured, traffic, hbm_gbs, algorithmic_gbs, ext_bytes / launches,
+                                    avg_launch_ms),
     }
-    if args.workload == "ggx_env_1080p" and not args.no_denoise_eval and DENOISER_WEIGHTS.exists():
-        log("denoiser: time to relMSE, plain against denoised (well-posed estimator, 2 x 16384-spp reference) ...")
+    from gartus_amd.denoise_eval import REFERENCE_FILE
+    if (args.workload == "ggx_env_1080p" and not args.no_denoise_eval and DENOISER_WEIGHTS.exists()
+            and REFERENCE_FILE.exists()):
+        log("denoiser: time to relMSE, plain against denoised (crop reference, 2 x 2^20 spp) ...")
         result["denoise"] = denoise_summary(ctx)
     if not args.no_cpu_baseline:
         log("CPU baseline ...")
