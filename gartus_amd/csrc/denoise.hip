@@ -4,11 +4,11 @@
 // Every 3x3 convolution is an implicit GEMM on the matrix cores:
 //   forward      M = pixels, N = Cout, K = 9 * Cin   (v_mfma_f32_16x16x32_bf16)
 //   data grad    M = pixels, N = Cin,  K = 9 * Cout  (taps mirrored; same kernel)
-//   weight grad  M = Cout,   N = Cin,  K = pixels    (v_mfma_f32_16x16x16_bf16, per tap)
-// Operand lanes, with k-group size q = 4 (16x16x16) or 8 (16x16x32): lane l
-// holds A[l % 16][q (l / 16) + j] and B[q (l / 16) + j][l % 16] for j < q,
-// and D[4 (l / 16) + r][l % 16] for r < 4. Channel counts are multiples of
-// 16, so every fragment is one aligned 8- or 16-byte load.
+//   weight grad  M = Cout,   N = Cin,  K = pixels    (v_mfma_f32_16x16x32_bf16, all 9 taps per pass)
+// Operand lanes: lane l holds A[l % 16][8 (l / 16) + j] and
+// B[8 (l / 16) + j][l % 16] for j < 8, and D[4 (l / 16) + r][l % 16] for
+// r < 4. Channel counts are multiples of 16, so every fragment is one
+// aligned 16-byte load.
 // Accumulation is fp32; weight gradients are summed per split and reduced
 // in a fixed order, so training steps are deterministic.
 #include "denoise.h"
