@@ -156,11 +156,16 @@ class GtToneMap(C.Structure):
     _fields_ = [("exposure", C.c_double), ("gamma", C.c_double), ("op", C.c_uint32), ("reserved", C.c_uint32)]
 
 
+GT_DENOISE_LOSS_REL_L2, GT_DENOISE_LOSS_L1_TONEMAP = 0, 1
+
+
 class GtDenoiserConfig(C.Structure):
     _fields_ = [
         ("levels", C.c_uint32),
         ("channels", C.c_uint32 * 6),
         ("seed", C.c_uint64),
+        ("loss", C.c_uint32),
+        ("reserved", C.c_uint32),
     ]
 
 

@@ -8,6 +8,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include "../../include/gartus_denoise.h"
+
 namespace gdn {
 
 struct Dims {
@@ -46,12 +48,14 @@ void upcat_bwd(hipStream_t s, const uint16_t* dout, int ca, int cb, uint16_t* da
 // normal, zeros; pixels beyond (h, w) replicate the edge.
 void pack_input(hipStream_t s, const float* beauty, const float* albedo, const float* normal, int h, int w,
                 uint16_t* x, Dims dp);
-// Loss L1 on Reinhard-tone-mapped radiance: per element |T(z) - t/(1+t)|, T(z) = 1 - exp(-z) for
-// z >= 0 and z below; mean over n*h*w*3. dz (bf16, 16 channels) gets d loss / d z, zero in padding.
-// Block partial sums go to partial[0..blocks).
+// Training loss, mean over n*h*w*3 elements; dz (bf16, 16 channels) gets d loss / d z, zero in
+// padding; block partial sums go to partial[0..blocks). kind (GT_DENOISE_LOSS_*):
+//   relative L2: (y - t)^2 / (max(y, 0)^2 + 0.01), y = expm1(z) for z >= 0 and z below, the
+//     denominator held constant in the gradient;
+//   L1 on Reinhard-tone-mapped radiance: |T(z) - t/(1+t)|, T(z) = 1 - exp(-z) for z >= 0 and z below.
 int loss_blocks(Dims dp);
-void loss_l1(hipStream_t s, const float* z, const float* target, int h, int w, uint16_t* dz, float* partial,
-             Dims dp);
+void loss(hipStream_t s, int kind, const float* z, const float* target, int h, int w, uint16_t* dz, float* partial,
+          Dims dp);
 // radiance = expm1(max(z, 0)) for the valid region, fp32 [n][h][w][3].
 void unpack_output(hipStream_t s, const float* z, float* out, int h, int w, Dims dp);
 

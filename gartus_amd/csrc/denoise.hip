@@ -604,7 +604,26 @@ __global__ void k_pack(const float* __restrict__ beauty, const float* __restrict
 
 __device__ __forceinline__ float tone(float z) { return z >= 0.f ? 1.f - expf(-z) : z; }
 __device__ __forceinline__ float dtone(float z) { return z >= 0.f ? expf(-z) : 1.f; }
+constexpr float kRelL2Eps = 0.01f;
 
+// One element's loss and d loss / d z.
+template <int KIND>
+__device__ __forceinline__ float loss_elem(float zc, float tc, float* dl) {
+  if (KIND == GT_DENOISE_LOSS_REL_L2) {
+    const float y = zc >= 0.f ? expm1f(zc) : zc;
+    const float dy = zc >= 0.f ? expf(zc) : 1.f;
+    const float yc = fmaxf(y, 0.f);
+    const float inv = 1.f / (yc * yc + kRelL2Eps);  // held constant: the gradient's root is the mean target
+    const float d = y - tc;
+    *dl = 2.f * d * inv * dy;
+    return d * d * inv;
+  }
+  const float d = tone(zc) - tc / (1.f + tc);
+  *dl = (d > 0.f ? 1.f : (d < 0.f ? -1.f : 0.f)) * dtone(zc);
+  return fabsf(d);
+}
+
+template <int KIND>
 __global__ __launch_bounds__(256) void k_loss(const float* __restrict__ z, const float* __restrict__ target, int h,
                                               int w, uint16_t* __restrict__ dz, float* __restrict__ partial, int Hp,
                                               int Wp, size_t npx, float inv_count) {
@@ -623,12 +642,9 @@ __global__ __launch_bounds__(256) void k_loss(const float* __restrict__ z, const
     }
     const float* tg = target + ((b * h + yy) * w + xx) * 3;
     for (int c = 0; c < 3; ++c) {
-      const float zc = z[p * 16 + c];
-      const float tc = fmaxf(tg[c], 0.f);
-      const float d = tone(zc) - tc / (1.f + tc);
-      acc += fabsf(d);
-      const float s = d > 0.f ? 1.f : (d < 0.f ? -1.f : 0.f);
-      g[c] = f2bf(s * dtone(zc) * inv_count);
+      float dl;
+      acc += loss_elem<KIND>(z[p * 16 + c], fmaxf(tg[c], 0.f), &dl);
+      g[c] = f2bf(dl * inv_count);
     }
   }
   red[threadIdx.x] = acc;
@@ -807,12 +823,16 @@ int loss_blocks(Dims dp) {
   return int(std::min<size_t>((n + 255) / 256, 1024));
 }
 
-void loss_l1(hipStream_t s, const float* z, const float* target, int h, int w, uint16_t* dz, float* partial,
-             Dims dp) {
+void loss(hipStream_t s, int kind, const float* z, const float* target, int h, int w, uint16_t* dz, float* partial,
+          Dims dp) {
   const size_t n = size_t(dp.pixels());
   const float inv = 1.0f / float(size_t(dp.n) * h * w * 3);
-  hipLaunchKernelGGL(k_loss, dim3(loss_blocks(dp)), dim3(256), 0, s, z, target, h, w, dz, partial, dp.h, dp.w, n,
-                     inv);
+  if (kind == GT_DENOISE_LOSS_REL_L2)
+    hipLaunchKernelGGL(k_loss<GT_DENOISE_LOSS_REL_L2>, dim3(loss_blocks(dp)), dim3(256), 0, s, z, target, h, w, dz,
+                       partial, dp.h, dp.w, n, inv);
+  else
+    hipLaunchKernelGGL(k_loss<GT_DENOISE_LOSS_L1_TONEMAP>, dim3(loss_blocks(dp)), dim3(256), 0, s, z, target, h, w,
+                       dz, partial, dp.h, dp.w, n, inv);
 }
 
 void unpack_output(hipStream_t s, const float* z, float* out, int h, int w, Dims dp) {

@@ -68,6 +68,7 @@ struct gt_denoiser {
   hipStream_t stream = nullptr;
   int levels = 0;
   int ch[6] = {0};
+  int loss_kind = GT_DENOISE_LOSS_REL_L2;
   std::vector<std::unique_ptr<Conv>> convs;  // enc l: 2l, 2l+1; dec l: 2L + 2(L-2-l), +1; output: last
   size_t nparams = 0;
   Buf<float> params, grads, adam_m, adam_v;
@@ -267,7 +268,7 @@ void backward(gt_denoiser* dn) {
 }
 
 int loss_into(gt_denoiser* dn, const float* target, uint32_t h, uint32_t w, float* out_loss) {
-  gdn::loss_l1(dn->stream, dn->z.p, target, int(h), int(w), dn->dz.p, dn->loss_partial.p, dn->shape);
+  gdn::loss(dn->stream, dn->loss_kind, dn->z.p, target, int(h), int(w), dn->dz.p, dn->loss_partial.p, dn->shape);
   const int nb = gdn::loss_blocks(dn->shape);
   dn->host_loss.resize(size_t(nb));
   GT_CHECK_HIP(hipMemcpyAsync(dn->host_loss.data(), dn->loss_partial.p, size_t(nb) * sizeof(float),
@@ -294,6 +295,7 @@ void gt_denoiser_default_config(gt_denoiser_config* cfg) {
   const uint32_t ch[5] = {32, 48, 64, 96, 128};
   for (int i = 0; i < 5; ++i) cfg->channels[i] = ch[i];
   cfg->seed = 1;
+  cfg->loss = GT_DENOISE_LOSS_REL_L2;
 }
 
 int gt_denoiser_create(gt_context* ctx, const gt_denoiser_config* cfg, gt_denoiser** out) {
@@ -303,7 +305,10 @@ int gt_denoiser_create(gt_context* ctx, const gt_denoiser_config* cfg, gt_denois
   for (uint32_t l = 0; l < cfg->levels; ++l)
     if (cfg->channels[l] == 0 || cfg->channels[l] % 16 || cfg->channels[l] > 128)
       return bad(GT_EINVAL, "channels must be multiples of 16 in 16..128");
+  if (cfg->loss != GT_DENOISE_LOSS_REL_L2 && cfg->loss != GT_DENOISE_LOSS_L1_TONEMAP)
+    return bad(GT_EINVAL, "unknown loss");
   std::unique_ptr<gt_denoiser> dn(new gt_denoiser());
+  dn->loss_kind = int(cfg->loss);
   dn->ctx = ctx;
   dn->device = gt::context_device(ctx);
   dn->stream = gt::context_stream(ctx);

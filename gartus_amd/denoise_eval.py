@@ -141,8 +141,8 @@ def reached(points, target):
     return None
 
 
-def evaluate(dn: Denoiser, plain_spp=(4, 16, 64, 256, 1024, 4096, 16384, 65536, 262144),
-             denoised_spp=(1, 2, 4, 16, 64, 256), direct_max_spp: int = 4096, log=print) -> dict:
+def evaluate(dn: Denoiser, plain_spp=(4, 16, 64, 256, 1024, 4096, 16384, 65536, 262144, 524288),
+             denoised_spp=(1, 4, 16, 64, 256, 1024, 4096, 16384), direct_max_spp: int = 4096, log=print) -> dict:
     ctx = dn.ctx
     ref = load_reference()
     pix, r = ref["pixels"], ref["ref"]

@@ -1,12 +1,14 @@
 """Training data, training loop and error measurement for the learned denoiser.
 
 Data. Seeded scenes in the style of the 1080p GGX/environment workload
-(scenes.c3_style_scene) are rendered by this library's own path tracer as
-256x256 crops:
-  - inputs: a 4-spp render with its first-hit albedo and normal AOVs
-    (gt_render_aovs_device);
-  - target: a 1024-spp render of the same crop (gt_render_hdr_device) with
-    another RNG seed, so input and target noise are independent.
+(scenes.c3_style_scene), rendered by this library's own path tracer:
+  - make_crop_dataset (--crops): one random 256x256 crop of a 1920x1080 frame
+    per scene, inputs at 1-4096 spp with their first-hit albedo and normal
+    AOVs, target at 16384 spp with another seed;
+  - make_dataset: whole 256x256 renders, a 4-spp input and a 1024-spp target.
+Input and target noise are independent. The default loss, relative L2 with
+the prediction's magnitude held in its denominator, is minimised by the
+targets' mean, so noisy but unbiased targets teach the expected radiance.
 All images stay in device memory as torch tensors; torch supplies the memory
 and the batch gathers only, and the network runs on the HIP library. A process
 that uses torch must import it before the library loads (abi.load), which
@@ -16,7 +18,8 @@ Error. relMSE(x, r) = mean over pixels and channels of (x - r)^2 / (r^2 + eps),
 with eps = 1e-2, against a high-spp reference accumulated in FP64.
 
 Usage, on a GPU (writes the trained parameters as .npy):
-    python -m gartus_amd.denoise_train --scenes 400 --steps 3000 --out weights.npy
+    python -m gartus_amd.denoise_train --crops --reject-ggx-fallback --scenes 320 --spp-ref 16384 \
+        --steps 6000 --out weights.npy
 """
 from __future__ import annotations
 
@@ -131,6 +134,16 @@ class SceneHandle:
                                                     C.c_void_p(_ptr(out)), 1 if double else 0, None))
         self.ctx.synchronize()
 
+    def render_pixels(self, out, spp: int, seed: int, pixels: np.ndarray) -> None:
+        """The listed pixels (y * W + x) into a full-frame FP64 device tensor
+        out [h][w][3]; the others are left as they were."""
+        c = self.camera(spp, seed)
+        px = np.ascontiguousarray(pixels, dtype=np.uint32)
+        abi.check(self.ctx.lib.gt_render_pixels_device(self.ctx.handle, self.sh, C.byref(c), None, self.eh,
+                                                       px.ctypes.data_as(abi.U32), len(px), C.c_void_p(_ptr(out)),
+                                                       1, None))
+        self.ctx.synchronize()
+
     def render_aovs(self, beauty, albedo, normal, spp: int, seed: int) -> None:
         c = self.camera(spp, seed)
         abi.check(self.ctx.lib.gt_render_aovs_device(
@@ -158,6 +171,65 @@ def make_dataset(ctx: api.Context, n_scenes: int, first_seed: int = 10_000, size
     return data
 
 
+CROP_SPP_LEVELS = (1, 4, 16, 64, 256, 1024, 4096)
+FULL_FRAME_MAX_SPP = 64  # input levels up to this are cut from full-frame AOV renders
+
+
+def make_crop_dataset(ctx: api.Context, n_scenes: int, first_seed: int = 20_000, size: int = 256,
+                      frame=(1920, 1080), spp_levels=CROP_SPP_LEVELS, spp_ref: int = 16384, flags: int = 0,
+                      log=print):
+    """Random size x size crops of full-resolution frames of seeded scenes in
+    the style of the GGX/environment workload (scenes.c3_style_scene at
+    `frame`), one crop per scene at a seeded position, as device tensors:
+      - beauty, albedo, normal [n][levels][size][size][3]: the crop's inputs at
+        each spp of spp_levels. Levels up to 64 spp are cut from full-frame
+        gt_render_aovs_device renders; higher levels render only the crop's
+        pixels (gt_render_pixels_device, each pixel equal to the full
+        render's) and take their albedo and normal from the 64-spp level,
+        whose first-hit features are already converged;
+      - target [n][size][size][3]: the crop at spp_ref from another seed,
+        FP64 accumulation, so input and target noise are independent."""
+    import torch
+    spp_levels = tuple(sorted(spp_levels))
+    if not spp_levels or spp_levels[0] > FULL_FRAME_MAX_SPP:
+        raise ValueError(f"the lowest input spp must be at most {FULL_FRAME_MAX_SPP} (it supplies the AOVs)")
+    W, H = frame
+    L = len(spp_levels)
+    shape = (n_scenes, L, size, size, 3)
+    data = {k: torch.empty(shape, dtype=torch.float32, device="cuda") for k in ("beauty", "albedo", "normal")}
+    data["target"] = torch.empty((n_scenes, size, size, 3), dtype=torch.float32, device="cuda")
+    full = {k: torch.empty((H, W, 3), dtype=torch.float32, device="cuda") for k in ("beauty", "albedo", "normal")}
+    wide = torch.empty((H, W, 3), dtype=torch.float64, device="cuda")
+    t0 = time.perf_counter()
+    for i in range(n_scenes):
+        seed = first_seed + i
+        sd, env, cam = scenes.c3_style_scene(seed, W, H)
+        sh = SceneHandle(ctx, sd, env, cam, flags)
+        pick = np.random.default_rng(seed)
+        x0, y0 = int(pick.integers(0, W - size + 1)), int(pick.integers(0, H - size + 1))
+        ys, xs = np.mgrid[y0:y0 + size, x0:x0 + size]
+        pix = np.ascontiguousarray((ys * W + xs).ravel().astype(np.uint32))
+        crop = (slice(y0, y0 + size), slice(x0, x0 + size))
+        torch.cuda.synchronize()
+        aov_level = None
+        for j, spp in enumerate(spp_levels):
+            if spp <= FULL_FRAME_MAX_SPP:
+                sh.render_aovs(full["beauty"], full["albedo"], full["normal"], spp, 16 * seed + j)
+                for k in ("beauty", "albedo", "normal"):
+                    data[k][i, j] = full[k][crop]
+                aov_level = j
+            else:
+                sh.render_pixels(wide, spp, 16 * seed + j, pix)
+                data["beauty"][i, j] = wide[crop].float()
+                data["albedo"][i, j] = data["albedo"][i, aov_level]
+                data["normal"][i, j] = data["normal"][i, aov_level]
+        sh.render_pixels(wide, spp_ref, 16 * seed + 15, pix)
+        data["target"][i] = wide[crop].float()
+        if (i + 1) % 25 == 0 or i + 1 == n_scenes:
+            log(f"dataset: {i + 1}/{n_scenes} crops rendered ({time.perf_counter() - t0:.1f} s)")
+    return data
+
+
 def augment(batch, k: int):
     """The k-th of the 8 flips/transposes of a [n][h][w][3] square batch."""
     if k & 1:
@@ -177,10 +249,17 @@ def train(dn: Denoiser, data, steps: int, batch: int = 16, lr: float = 1e-4, see
     g.manual_seed(seed)
     losses = []
     t0 = time.perf_counter()
+    levels = data["beauty"].shape[1] if data["beauty"].dim() == 5 else 0
     for step in range(steps):
         idx = torch.randint(0, n, (batch,), generator=g).cuda()
         k = int(torch.randint(0, 8, (1,), generator=g))
-        b, a, nr, t = (augment(data[key].index_select(0, idx), k) for key in ("beauty", "albedo", "normal", "target"))
+        if levels:  # each element at a random input spp of its crop
+            lv = torch.randint(0, levels, (batch,), generator=g).cuda()
+            b, a, nr = (augment(data[key][idx, lv], k) for key in ("beauty", "albedo", "normal"))
+            t = augment(data["target"].index_select(0, idx), k)
+        else:
+            b, a, nr, t = (augment(data[key].index_select(0, idx), k)
+                           for key in ("beauty", "albedo", "normal", "target"))
         losses.append(dn.train_step(b, a, nr, t, lr=lr))
         if (step + 1) % log_every == 0 or step + 1 == steps:
             recent = float(np.mean(losses[-log_every:]))
@@ -199,6 +278,13 @@ def main(argv=None) -> int:
     ap.add_argument("--init", default="", help="start from these parameters (.npy)")
     ap.add_argument("--reject-ggx-fallback", action="store_true",
                     help="render with GT_FLAG_GGX_REJECT_BELOW_HORIZON (the well-posed estimator)")
+    ap.add_argument("--crops", action="store_true",
+                    help="train on random 256x256 crops of 1920x1080 frames at input spp "
+                         f"{CROP_SPP_LEVELS} (make_crop_dataset) instead of whole 256x256 renders at 4 spp")
+    ap.add_argument("--loss", choices=["rel-l2", "l1"], default="rel-l2")
+    ap.add_argument("--spp-levels", default=",".join(map(str, CROP_SPP_LEVELS)),
+                    help="input spp of the crop dataset, comma separated")
+    ap.add_argument("--first-seed", type=int, default=20_000)
     ap.add_argument("--out", required=True)
     args = ap.parse_args(argv)
 
@@ -206,12 +292,18 @@ def main(argv=None) -> int:
         print(m, flush=True)
 
     ctx = api.default_context()
-    dn = Denoiser(ctx)
+    loss = abi.GT_DENOISE_LOSS_REL_L2 if args.loss == "rel-l2" else abi.GT_DENOISE_LOSS_L1_TONEMAP
+    dn = Denoiser(ctx, loss=loss)
     if args.init:
         dn.load(args.init)
-    log(f"denoiser: {dn.param_count} parameters")
+    log(f"denoiser: {dn.param_count} parameters, loss {args.loss}")
     flags = abi.GT_FLAG_GGX_REJECT_BELOW_HORIZON if args.reject_ggx_fallback else 0
-    data = make_dataset(ctx, args.scenes, spp_ref=args.spp_ref, flags=flags, log=log)
+    if args.crops:
+        levels = tuple(int(v) for v in args.spp_levels.split(","))
+        data = make_crop_dataset(ctx, args.scenes, first_seed=args.first_seed, spp_levels=levels,
+                                 spp_ref=args.spp_ref, flags=flags, log=log)
+    else:
+        data = make_dataset(ctx, args.scenes, spp_ref=args.spp_ref, flags=flags, log=log)
     train(dn, data, args.steps, args.batch, args.lr, log=log)
     dn.save(args.out)
     log(f"saved {args.out}")

@@ -63,13 +63,16 @@ class Denoiser:
     """U-Net denoiser: `levels` resolutions with `channels[l]` features."""
 
     def __init__(self, context: Optional[Context] = None, levels: int = 5,
-                 channels: Sequence[int] = (32, 48, 64, 96, 128), seed: int = 1):
+                 channels: Sequence[int] = (32, 48, 64, 96, 128), seed: int = 1,
+                 loss: int = abi.GT_DENOISE_LOSS_REL_L2):
         self.ctx = context or default_context()
         cfg = abi.GtDenoiserConfig()
         cfg.levels = levels
         for i, c in enumerate(channels[:levels]):
             cfg.channels[i] = c
         cfg.seed = seed
+        cfg.loss = loss
+        self.loss = loss
         self.levels, self.channels = levels, tuple(channels[:levels])
         h = C.c_void_p()
         abi.check(self.ctx.lib.gt_denoiser_create(self.ctx.handle, C.byref(cfg), C.byref(h)))

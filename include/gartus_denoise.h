@@ -22,9 +22,17 @@
  *   - Weights and activations are bf16 with fp32 accumulation, on the matrix
  *     cores. Master weights, Adam state, loss and gradient reductions are fp32.
  *
- * Training loss: L1 between Reinhard-tone-mapped prediction and target,
- * T(x) = x / (1 + x). The prediction's tone map is 1 - exp(-z) for z >= 0,
- * and z below 0.
+ * Training loss, per pixel and channel, averaged (gt_denoiser_config.loss):
+ *   - GT_DENOISE_LOSS_REL_L2 (the default): relative L2,
+ *     (y - t)^2 / (sg(y)^2 + 0.01) with y = expm1(z) for z >= 0 (z below 0)
+ *     and sg(y) = max(y, 0) held constant in the gradient. Over noisy
+ *     targets its gradient vanishes at the targets' mean, so the network
+ *     learns the expected radiance (Monte Carlo targets are unbiased, not
+ *     symmetric: their median lies below their mean);
+ *   - GT_DENOISE_LOSS_L1_TONEMAP: L1 between Reinhard-tone-mapped prediction
+ *     and target, T(x) = x / (1 + x); the prediction's tone map is
+ *     1 - exp(-z) for z >= 0, and z below 0. It learns the targets' median in
+ *     tone-mapped space, darker than their mean where the noise is skewed.
  *
  * Images are fp32, [n][h][w][3], row-major, in device memory on the
  * context's device. h and w may be any size: the network pads to a multiple
@@ -45,13 +53,18 @@ extern "C" {
 
 typedef struct gt_denoiser gt_denoiser;
 
+enum { GT_DENOISE_LOSS_REL_L2 = 0, GT_DENOISE_LOSS_L1_TONEMAP = 1 };
+
 typedef struct gt_denoiser_config {
   uint32_t levels;        /* 2..6 */
   uint32_t channels[6];   /* features per level, multiples of 16, at most 128 */
   uint64_t seed;          /* He-normal initialisation (SplitMix64 + Box-Muller) */
+  uint32_t loss;          /* GT_DENOISE_LOSS_*: the training loss */
+  uint32_t reserved;      /* 0 */
 } gt_denoiser_config;
 
-/* The standard network: 5 levels, 32-48-64-96-128 channels (1.02 M parameters). */
+/* The standard network: 5 levels, 32-48-64-96-128 channels (1.02 M parameters),
+ * relative-L2 loss. */
 void gt_denoiser_default_config(gt_denoiser_config* cfg);
 
 int gt_denoiser_create(gt_context* ctx, const gt_denoiser_config* cfg, gt_denoiser** out);

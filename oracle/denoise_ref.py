@@ -192,15 +192,36 @@ def forward(net: Net, beauty, albedo, normal, params=None, bf=False):
     return z, cache
 
 
-def loss_and_dz(z, target, bf=False):
+LOSS_REL_L2, LOSS_L1_TONEMAP = 0, 1
+REL_L2_EPS = 0.01
+
+
+def rel_l2_elem(zc, t):
+    """Relative L2 per element, (y - t)^2 / (max(y, 0)^2 + eps) with
+    y = expm1(z) (z below 0), and its derivative in z with the denominator
+    held constant."""
+    pos = zc >= 0.0
+    y = np.where(pos, np.expm1(np.maximum(zc, 0.0)), zc)
+    dy = np.where(pos, np.exp(np.maximum(zc, 0.0)), 1.0)
+    inv = 1.0 / (np.maximum(y, 0.0) ** 2 + REL_L2_EPS)
+    d = y - t
+    return d * d * inv, 2.0 * d * inv * dy
+
+
+def loss_and_dz(z, target, bf=False, kind=LOSS_REL_L2):
     n, h, w, _ = target.shape
     zc = z[:, :h, :w, :3]
     t = np.maximum(target, 0.0)
-    d = tone(zc) - t / (1.0 + t)
     count = n * h * w * 3
-    loss = np.abs(d).sum() / count
     dz = np.zeros_like(z)
-    dz[:, :h, :w, :3] = np.sign(d) * dtone(zc) / count
+    if kind == LOSS_REL_L2:
+        e, g = rel_l2_elem(zc, t)
+        loss = e.sum() / count
+        dz[:, :h, :w, :3] = g / count
+    else:
+        d = tone(zc) - t / (1.0 + t)
+        loss = np.abs(d).sum() / count
+        dz[:, :h, :w, :3] = np.sign(d) * dtone(zc) / count
     return loss, (bf16(dz) if bf else dz)
 
 
@@ -247,9 +268,9 @@ def backward(net: Net, cache, dz, bf=False):
     return grads
 
 
-def loss_and_gradients(net: Net, beauty, albedo, normal, target, params=None, bf=False):
+def loss_and_gradients(net: Net, beauty, albedo, normal, target, params=None, bf=False, kind=LOSS_REL_L2):
     z, cache = forward(net, beauty, albedo, normal, params, bf)
-    loss, dz = loss_and_dz(z, target, bf)
+    loss, dz = loss_and_dz(z, target, bf, kind)
     return loss, backward(net, cache, dz, bf), z
 
 

@@ -66,18 +66,71 @@ def test_pool_routes_to_first_maximum():
 
 
 def test_loss_closed_form():
+    L1 = dr.LOSS_L1_TONEMAP
     t = np.full((1, 2, 2, 3), 3.0)                 # T(t) = 3/4
     z = np.zeros((1, 2, 2, 16))
     z[..., :3] = np.log(4.0)                       # 1 - exp(-z) = 3/4
-    loss, dz = dr.loss_and_dz(z, t)
+    loss, dz = dr.loss_and_dz(z, t, kind=L1)
     assert loss == pytest.approx(0.0, abs=1e-15)
     z[..., :3] = 0.0                                # T = 0: |0 - 3/4|
-    loss, dz = dr.loss_and_dz(z, t)
+    loss, dz = dr.loss_and_dz(z, t, kind=L1)
     assert loss == pytest.approx(0.75)
     assert np.allclose(dz[..., :3], -1.0 / 12.0) and not dz[..., 3:].any()
     z[..., :3] = -2.0                               # below zero the tone map is the identity
-    loss, _ = dr.loss_and_dz(z, t)
+    loss, _ = dr.loss_and_dz(z, t, kind=L1)
     assert loss == pytest.approx(2.75)
+
+
+def test_relative_l2_closed_form():
+    t = np.full((1, 2, 2, 3), 3.0)
+    z = np.zeros((1, 2, 2, 16))
+    z[..., :3] = np.log(4.0)                       # y = expm1(z) = 3: no error
+    loss, dz = dr.loss_and_dz(z, t)
+    assert loss == pytest.approx(0.0, abs=1e-15) and not dz.any()
+    z[..., :3] = 0.0                                # y = 0: (0 - 3)^2 / (0 + 0.01)
+    loss, dz = dr.loss_and_dz(z, t)
+    assert loss == pytest.approx(900.0)
+    assert np.allclose(dz[..., :3], 2.0 * -3.0 / 0.01 / 12.0) and not dz[..., 3:].any()
+    z[..., :3] = -2.0                               # y = z below zero; the denominator uses max(y, 0)
+    loss, dz = dr.loss_and_dz(z, t)
+    assert loss == pytest.approx(2500.0)
+    assert np.allclose(dz[..., :3], 2.0 * -5.0 / 0.01 / 12.0)
+
+
+def test_relative_l2_gradient_holds_the_denominator():
+    """d/dz of (y(z) - t)^2 / c with c = max(y(z0), 0)^2 + eps frozen at z0,
+    by finite differences, on both branches of y."""
+    rng = np.random.default_rng(4)
+    z0 = np.concatenate([rng.uniform(-1.5, -0.1, 20), rng.uniform(0.1, 3.0, 20)])
+    t = rng.uniform(0.0, 5.0, 40)
+    _, g = dr.rel_l2_elem(z0, t)
+    c = np.maximum(np.where(z0 >= 0, np.expm1(z0), z0), 0.0) ** 2 + dr.REL_L2_EPS
+    y = lambda z: np.where(z >= 0, np.expm1(z), z)
+    h = 1e-6
+    fd = ((y(z0 + h) - t) ** 2 - (y(z0 - h) - t) ** 2) / (2 * h) / c
+    np.testing.assert_allclose(g, fd, rtol=1e-6)
+
+
+def test_relative_l2_learns_the_mean_and_l1_the_median():
+    """One prediction y fit to many noisy targets of a skewed (Monte Carlo
+    like) distribution: the relative-L2 gradient, its denominator shared,
+    vanishes at the targets' mean; the L1 tone-mapped loss is smallest at
+    their median, below the mean. This is why relative L2 is the default."""
+    rng = np.random.default_rng(8)
+    t = rng.lognormal(-1.0, 1.2, 4001)
+    mean, median = t.mean(), np.median(t)
+    assert median < 0.6 * mean
+
+    def grad_rel(y):
+        return dr.rel_l2_elem(np.full_like(t, np.log1p(y)), t)[1].sum()
+    lo, hi = 1e-3, 50.0                              # bisection on the gradient's sign
+    for _ in range(100):
+        mid = 0.5 * (lo + hi)
+        lo, hi = (mid, hi) if grad_rel(mid) < 0 else (lo, mid)
+    assert lo == pytest.approx(mean, rel=1e-9)
+    ys = np.linspace(0.01, 3.0 * mean, 3001)
+    l1 = [np.abs((1 - 1 / (1 + y)) - t / (1 + t)).sum() for y in ys]
+    assert ys[int(np.argmin(l1))] == pytest.approx(median, abs=2 * (ys[1] - ys[0]))
 
 
 def test_bf16_rounding():
@@ -98,8 +151,12 @@ def _problem(levels, channels, n=2, h=8, w=8, seed=3):
 
 @pytest.mark.parametrize("levels,channels,h,w", [(2, (16, 16), 8, 8), (3, (16, 32, 16), 7, 9)])
 def test_gradient_matches_finite_differences(levels, channels, h, w):
+    """Through the network with the L1 loss, whose gradient is the true
+    derivative (the relative-L2 gradient holds its denominator, checked
+    element by element above; the chain through the network is the same)."""
     net, b, a, nr, t = _problem(levels, channels, h=h, w=w)
-    loss, g, _ = dr.loss_and_gradients(net, b, a, nr, t)
+    kind = dr.LOSS_L1_TONEMAP
+    loss, g, _ = dr.loss_and_gradients(net, b, a, nr, t, kind=kind)
     rng = np.random.default_rng(11)
     idx = set(rng.choice(net.nparams, 40, replace=False).tolist())
     for i in range(len(net.convs)):  # at least one weight and the first bias of every convolution
@@ -111,9 +168,9 @@ def test_gradient_matches_finite_differences(levels, channels, h, w):
     for i in sorted(idx):
         p = net.params.copy()
         p[i] += eps
-        lp = dr.loss_and_gradients(net, b, a, nr, t, params=p)[0]
+        lp = dr.loss_and_gradients(net, b, a, nr, t, params=p, kind=kind)[0]
         p[i] -= 2 * eps
-        lm = dr.loss_and_gradients(net, b, a, nr, t, params=p)[0]
+        lm = dr.loss_and_gradients(net, b, a, nr, t, params=p, kind=kind)[0]
         fd = (lp - lm) / (2 * eps)
         assert fd == pytest.approx(g[i], rel=1e-4, abs=1e-8), f"param {i}"
         checked += 1

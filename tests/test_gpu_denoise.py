@@ -21,6 +21,7 @@ import numpy as np
 import pytest
 
 import denoise_ref as dr
+from gartus_amd import abi
 from gartus_amd.denoiser import Denoiser
 
 pytestmark = pytest.mark.gpu
@@ -57,17 +58,19 @@ def _problem(levels, channels, n, h, w, seed):
 CASES = [(2, (16, 16), 2, 16, 16), (3, (32, 48, 16), 1, 20, 28), (5, (32, 48, 64, 96, 128), 2, 32, 48)]
 
 
+@pytest.mark.parametrize("loss", [abi.GT_DENOISE_LOSS_REL_L2, abi.GT_DENOISE_LOSS_L1_TONEMAP], ids=["rel-l2", "l1"])
 @pytest.mark.parametrize("levels,channels,n,h,w", CASES)
-def test_forward_and_gradients_match_oracle(gpu_context, levels, channels, n, h, w):
-    say(f"denoiser levels {levels} channels {channels} batch {n}x{h}x{w}")
+def test_forward_and_gradients_match_oracle(gpu_context, levels, channels, n, h, w, loss):
+    say(f"denoiser levels {levels} channels {channels} batch {n}x{h}x{w} loss {loss}")
     net, b, a, nr, t = _problem(levels, channels, n, h, w, seed=levels)
-    dn = Denoiser(gpu_context, levels, channels)
+    dn = Denoiser(gpu_context, levels, channels, loss=loss)
     assert dn.param_count == net.nparams
     dn.set_params(net.params.astype(np.float32))
     np.testing.assert_array_equal(dn.get_params(), net.params.astype(np.float32))
 
     loss_ref, g_ref, z_ref = dr.loss_and_gradients(net, b.astype(np.float64), a.astype(np.float64),
-                                                   nr.astype(np.float64), t.astype(np.float64), bf=True)
+                                                   nr.astype(np.float64), t.astype(np.float64), bf=True,
+                                                   kind=loss)
     rad_gpu = dn.infer(b, a, nr)
     rad_ref = dr.radiance(z_ref, h, w)
     # compare in the network's output space, z = log1p(radiance) where z >= 0
