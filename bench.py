@@ -452,8 +452,7 @@ def denoise_summary(ctx) -> dict:
     unbounded variance on this scene)."""
     from gartus_amd.denoise_eval import evaluate
     from gartus_amd.denoiser import Denoiser
-    dn = Denoiser(ctx)
-    dn.load(str(DENOISER_WEIGHTS))
+    dn = Denoiser.from_file(str(DENOISER_WEIGHTS), ctx)
     r = evaluate(dn, log=log)
     dn.close()
     t01 = r["time_to_targets"].get("0.01", {})
@@ -521,7 +520,24 @@ def roofline_record(measured: dict, traffic, hbm_gbs, algorithmic_gbs: float, al
         "valu_lane_util": measured.get("valu_lane_util"),
         "counters_source": measured.get("source"),
     })
+    rec["limiter"] = limiter(rec)
     return rec
+
+
+def limiter(rec: dict) -> str | None:
+    """What the counters say bounds the kernel: HBM bandwidth when the
+    measured traffic runs near peak; otherwise, when waves spend most of
+    their cycles waiting with little VALU issue, the latency of dependent
+    memory fetches; None without counters."""
+    wait, busy = rec.get("wave_wait_frac"), rec.get("valu_busy_frac")
+    if rec.get("traffic") is None or wait is None or busy is None:
+        return None
+    if rec["frac"] >= 0.7:
+        return "HBM bandwidth"
+    if wait >= 0.5 and busy < 0.25:
+        return (f"latency of dependent memory fetches: HBM at {rec['frac']:.2f} of peak, waves waiting "
+                f"{100 * wait:.0f}% of their cycles, VALU issuing {100 * busy:.0f}%")
+    return f"mixed: HBM at {rec['frac']:.2f} of peak, waves waiting {100 * wait:.0f}%, VALU {100 * busy:.0f}%"
 
 
 def main() -> int:

@@ -119,3 +119,11 @@ def test_record_profile_turns_counter_passes_into_a_stamped_record(tmp_path):
     (out / "trace_w.log").write_text(json.dumps(line) + "\n")
     with pytest.raises(SystemExit):
         bench.record_profile(out, "w", dest=dest)
+
+
+def test_limiter_reads_the_counters():
+    base = {"traffic": 1, "frac": 0.37, "wave_wait_frac": 0.63, "valu_busy_frac": 0.08}
+    assert bench.limiter(base).startswith("latency of dependent memory fetches")
+    assert bench.limiter(dict(base, frac=0.8)) == "HBM bandwidth"
+    assert bench.limiter(dict(base, traffic=None)) is None
+    assert bench.limiter(dict(base, wave_wait_frac=0.2)).startswith("mixed")
