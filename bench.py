@@ -437,6 +437,23 @@ def record_profile(out: Path, workload: str, calibrate_mib: int = 1024, dest: Pa
               f"    {bench_line['value']} Mrays/s, {bench_line['ms_per_step']} ms per step, "
               f"{bench_line['config']['workload']}"]
     (dest / f"{workload}_counters.md").write_text("\n".join(lines) + "\n")
+    # the trace run's per-kernel statistics
+    t = [f"# Kernel trace: {workload}", "", "Command (from the repository root, on one MI355X, `TMPDIR=/tmp`):", "",
+         "    " + _profile_commands(workload)[3], "",
+         f"The run renders the workload {renders + 1} times: a statistics render, {warmup} warm-up, {steps} timed "
+         "steps and a traversal-counting render (the instantiations whose first template argument is true). "
+         "k_extend<COUNT, MULTI, CAMERA>, k_connect<COUNT, MULTI>, k_shade<MULTI, CAMERA>; MULTI is the batched "
+         "(frame-table) form, CAMERA the bounce of the camera rays. Durations in microseconds, from "
+         "run_kernel_stats.csv.", "", "| kernel | calls | total us | average us | % |", "|---|---:|---:|---:|---:|"]
+    for k, r in sorted(stats.items(), key=lambda kv: -float(kv[1]["TotalDurationNs"])):
+        t.append(f"| {k} | {r['Calls']} | {float(r['TotalDurationNs']) * 1e-3:.1f} | {float(r['AverageNs']) * 1e-3:.2f} "
+                 f"| {float(r['Percentage']):.2f} |")
+    t += ["", f"Extend over the {steps} timed steps ({len(timed_ext)} dispatches, the last {steps} renders before "
+          f"the counting one): rocprofv3 {avg_us:.1f} us per launch; bench.py's HIP events in the same run "
+          f"(roofline.avg_launch_ms) {1e3 * bench_line['roofline']['avg_launch_ms']:.1f} us.", "",
+          "bench.py's per-step lines in the same run:", ""]
+    t += ["    " + ln for ln in log_file.read_text().splitlines() if ln.startswith(("step ", "eager:", "graph"))]
+    (dest / f"{workload}_kernel_trace.md").write_text("\n".join(t) + "\n")
     log(f"wrote {traffic_file} [{workload}] and {dest / (workload + '_counters.md')}")
     return 0
 
