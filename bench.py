@@ -568,6 +568,7 @@ def main() -> int:
     ap.add_argument("--block", type=int, default=0)
     ap.add_argument("--bvh-width", type=int, default=0)
     ap.add_argument("--max-paths", type=int, default=0)
+    ap.add_argument("--stack-lds", type=int, default=0, help="traversal-stack entries per lane in LDS (0: default)")
     ap.add_argument("--graph", choices=["auto", "on", "off"], default="auto",
                     help="time the render as a captured hipGraph replay (on), eagerly (off), or both with the "
                          "library's default as the value (auto)")
@@ -610,6 +611,8 @@ def main() -> int:
         kw["bvh_width"] = args.bvh_width
     if args.max_paths:
         kw["max_paths"] = args.max_paths
+    if args.stack_lds:
+        kw["stack_lds"] = args.stack_lds
     if kw:
         ctx.set_tuning(**kw)
     t0 = time.perf_counter()
@@ -656,7 +659,7 @@ def main() -> int:
     log(f"{n} scene(s) compiled in {build_wall:.2f} s ({'device LBVH, ' + format(build_device_ms, '.1f') + ' ms on the GPU' if args.device_build else 'host SAH'}): "
         f"{prims:,} prims, {nodes:,} BVH nodes")
     tuned = None
-    explicit = bool(args.block or args.bvh_width or args.max_paths)
+    explicit = bool(args.block or args.bvh_width or args.max_paths or args.stack_lds)
     if args.autotune or (args.workload == "batch64" and not explicit):
         best = abi.GtTuning()
         ms = C.c_double()

@@ -165,7 +165,7 @@ class GtDenoiserConfig(C.Structure):
         ("channels", C.c_uint32 * 6),
         ("seed", C.c_uint64),
         ("loss", C.c_uint32),
-        ("reserved", C.c_uint32),
+        ("beauty_halves", C.c_uint32),
     ]
 
 

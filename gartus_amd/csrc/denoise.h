@@ -46,8 +46,10 @@ void upcat_bwd(hipStream_t s, const uint16_t* dout, int ca, int cb, uint16_t* da
 
 // x (16 channels, padded dims dp) from fp32 images [n][h][w][3]: log1p(max(beauty, 0)), albedo,
 // normal, zeros; pixels beyond (h, w) replicate the edge.
-void pack_input(hipStream_t s, const float* beauty, const float* albedo, const float* normal, int h, int w,
-                uint16_t* x, Dims dp);
+// With beauty_b (two half-sample images a = beauty, b = beauty_b): log1p of their mean, and
+// log1p(a) - log1p(b) in channels 9..11.
+void pack_input(hipStream_t s, const float* beauty, const float* beauty_b, const float* albedo, const float* normal,
+                int h, int w, uint16_t* x, Dims dp);
 // Training loss, mean over n*h*w*3 elements; dz (bf16, 16 channels) gets d loss / d z, zero in
 // padding; block partial sums go to partial[0..blocks). kind (GT_DENOISE_LOSS_*):
 //   relative L2: (y - t)^2 / (max(y, 0)^2 + 0.01), y = expm1(z) for z >= 0 and z below, the

@@ -575,9 +575,9 @@ __global__ void k_upcat_bwd(const uint16_t* __restrict__ dout, int ca, int cb, u
 }
 
 // ---- input, loss, output ---------------------------------------------------
-__global__ void k_pack(const float* __restrict__ beauty, const float* __restrict__ albedo,
-                       const float* __restrict__ normal, int h, int w, uint16_t* __restrict__ x, int Hp, int Wp,
-                       size_t npx) {
+__global__ void k_pack(const float* __restrict__ beauty, const float* __restrict__ beauty_b,
+                       const float* __restrict__ albedo, const float* __restrict__ normal, int h, int w,
+                       uint16_t* __restrict__ x, int Hp, int Wp, size_t npx) {
   for (size_t p = blockIdx.x * size_t(blockDim.x) + threadIdx.x; p < npx; p += size_t(gridDim.x) * blockDim.x) {
     const int xx = int(p % Wp);
     const size_t t = p / Wp;
@@ -586,13 +586,20 @@ __global__ void k_pack(const float* __restrict__ beauty, const float* __restrict
     const size_t s = ((b * h + min(yy, h - 1)) * w + min(xx, w - 1)) * 3;
     uint16_t v[16];
 #pragma unroll
+    for (int c = 9; c < 16; ++c) v[c] = 0;
+#pragma unroll
     for (int c = 0; c < 3; ++c) {
-      v[c] = f2bf(log1pf(fmaxf(beauty[s + c], 0.f)));
+      const float a = beauty[s + c];
+      if (beauty_b) {
+        const float b2 = beauty_b[s + c];
+        v[c] = f2bf(log1pf(fmaxf(0.5f * (a + b2), 0.f)));
+        v[9 + c] = f2bf(log1pf(fmaxf(a, 0.f)) - log1pf(fmaxf(b2, 0.f)));
+      } else {
+        v[c] = f2bf(log1pf(fmaxf(a, 0.f)));
+      }
       v[3 + c] = f2bf(albedo[s + c]);
       v[6 + c] = f2bf(normal[s + c]);
     }
-#pragma unroll
-    for (int c = 9; c < 16; ++c) v[c] = 0;
     uint32_t u[8];
 #pragma unroll
     for (int k = 0; k < 8; ++k) u[k] = uint32_t(v[2 * k]) | (uint32_t(v[2 * k + 1]) << 16);
@@ -812,10 +819,11 @@ void upcat_bwd(hipStream_t s, const uint16_t* dout, int ca, int cb, uint16_t* da
   hipLaunchKernelGGL(k_upcat_bwd, dim3(grid_for(n)), dim3(256), 0, s, dout, ca, cb, da, d.h, d.w, n);
 }
 
-void pack_input(hipStream_t s, const float* beauty, const float* albedo, const float* normal, int h, int w,
-                uint16_t* x, Dims dp) {
+void pack_input(hipStream_t s, const float* beauty, const float* beauty_b, const float* albedo, const float* normal,
+                int h, int w, uint16_t* x, Dims dp) {
   const size_t n = size_t(dp.pixels());
-  hipLaunchKernelGGL(k_pack, dim3(grid_for(n)), dim3(256), 0, s, beauty, albedo, normal, h, w, x, dp.h, dp.w, n);
+  hipLaunchKernelGGL(k_pack, dim3(grid_for(n)), dim3(256), 0, s, beauty, beauty_b, albedo, normal, h, w, x, dp.h,
+                     dp.w, n);
 }
 
 int loss_blocks(Dims dp) {

@@ -69,6 +69,7 @@ struct gt_denoiser {
   int levels = 0;
   int ch[6] = {0};
   int loss_kind = GT_DENOISE_LOSS_REL_L2;
+  bool beauty_halves = false;
   std::vector<std::unique_ptr<Conv>> convs;  // enc l: 2l, 2l+1; dec l: 2L + 2(L-2-l), +1; output: last
   size_t nparams = 0;
   Buf<float> params, grads, adam_m, adam_v;
@@ -182,7 +183,8 @@ int prepare(gt_denoiser* dn, uint32_t n, uint32_t h, uint32_t w, const float* b,
   GT_CHECK_HIP(hipSetDevice(dn->device));
   int rc = ensure_shape(dn, gdn::Dims{int(n), int(hp), int(wp)});
   if (rc != GT_OK) return rc;
-  gdn::pack_input(dn->stream, b, a, nr, int(h), int(w), dn->x0.p, dn->shape);
+  const float* b2 = dn->beauty_halves ? b + size_t(n) * h * w * 3 : nullptr;
+  gdn::pack_input(dn->stream, b, b2, a, nr, int(h), int(w), dn->x0.p, dn->shape);
   return GT_OK;
 }
 
@@ -307,8 +309,10 @@ int gt_denoiser_create(gt_context* ctx, const gt_denoiser_config* cfg, gt_denois
       return bad(GT_EINVAL, "channels must be multiples of 16 in 16..128");
   if (cfg->loss != GT_DENOISE_LOSS_REL_L2 && cfg->loss != GT_DENOISE_LOSS_L1_TONEMAP)
     return bad(GT_EINVAL, "unknown loss");
+  if (cfg->beauty_halves > 1) return bad(GT_EINVAL, "beauty_halves must be 0 or 1");
   std::unique_ptr<gt_denoiser> dn(new gt_denoiser());
   dn->loss_kind = int(cfg->loss);
+  dn->beauty_halves = cfg->beauty_halves != 0;
   dn->ctx = ctx;
   dn->device = gt::context_device(ctx);
   dn->stream = gt::context_stream(ctx);

@@ -142,7 +142,7 @@ def reached(points, target):
 
 
 def evaluate(dn: Denoiser, plain_spp=(4, 16, 64, 256, 1024, 4096, 16384, 65536, 262144, 524288),
-             denoised_spp=(1, 4, 16, 64, 256, 1024, 4096, 16384), direct_max_spp: int = 4096, log=print) -> dict:
+             denoised_spp=(2, 4, 16, 64, 256, 1024, 4096, 16384), direct_max_spp: int = 4096, log=print) -> dict:
     ctx = dn.ctx
     ref = load_reference()
     pix, r = ref["pixels"], ref["ref"]
@@ -169,18 +169,33 @@ def evaluate(dn: Denoiser, plain_spp=(4, 16, 64, 256, 1024, 4096, 16384, 65536, 
         e, et = relmse(x, r), relmse_trimmed(x, r)
         plain.append({"spp": spp, "time_s": t, "time_how": how, "relmse": e, "relmse_trimmed": et})
         log(f"plain {spp:7d} spp: {t:9.3f} s relMSE {e:.5g} trimmed {et:.5g}")
-    b, a, nr, out = (DeviceArray(ctx, (1, h, w, 3)) for _ in range(4))
-    sh.render_aovs(b.sub(0), a.sub(0), nr.sub(0), 1, seed=7)  # warm
+    # a halves network takes each render as two of half the samples (the first with the AOVs)
+    halves = getattr(dn, "halves", False)
+    a, nr, out = (DeviceArray(ctx, (1, h, w, 3)) for _ in range(3))
+    b = DeviceArray(ctx, (2, 1, h, w, 3) if halves else (1, h, w, 3))
+    first = b.sub(0).sub(0) if halves else b.sub(0)
+
+    def render_input(spp, seed):
+        if halves:
+            sh.render_aovs(first, a.sub(0), nr.sub(0), spp // 2, seed=seed)
+            sh.render(b.sub(1).sub(0), spp // 2, seed=seed + 500_000)
+        else:
+            sh.render_aovs(first, a.sub(0), nr.sub(0), spp, seed=seed)
+    render_input(2, 7)  # warm
     dn.infer(b, a, nr, out=out)
     den = []
     for spp in denoised_spp:
+        if halves and spp % 2:
+            continue
+
         def run():
-            sh.render_aovs(b.sub(0), a.sub(0), nr.sub(0), spp, seed=2000 + spp)
+            render_input(spp, 2000 + spp)
             dn.infer(b, a, nr, out=out)
         t = _timed(run)
         t_inf = _timed(lambda: dn.infer(b, a, nr, out=out))
         y = out.numpy()[0].reshape(-1, 3)[pix.astype(np.int64)].astype(np.float64)
-        x = b.numpy()[0].reshape(-1, 3)[pix.astype(np.int64)].astype(np.float64)
+        xb = b.numpy().reshape(2, -1, 3).mean(axis=0) if halves else b.numpy()[0].reshape(-1, 3)
+        x = xb[pix.astype(np.int64)].astype(np.float64)
         e, et = relmse(y, r), relmse_trimmed(y, r)
         den.append({"spp": spp, "time_s": t, "infer_s": t_inf, "relmse": e, "relmse_trimmed": et,
                     "relmse_input": relmse(x, r)})
@@ -231,8 +246,7 @@ def main(argv=None) -> int:
     if args.make_reference:
         make_reference(ctx, args.half_spp, log=lambda m: print(m, flush=True), out=args.out)
         return 0
-    dn = Denoiser(ctx)
-    dn.load(args.weights)
+    dn = Denoiser.from_file(args.weights, ctx)
     res = evaluate(dn, log=lambda m: print(m, flush=True))
     if args.json:
         with open(args.json, "w") as f:

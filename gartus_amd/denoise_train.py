@@ -177,7 +177,7 @@ FULL_FRAME_MAX_SPP = 64  # input levels up to this are cut from full-frame AOV r
 
 def make_crop_dataset(ctx: api.Context, n_scenes: int, first_seed: int = 20_000, size: int = 256,
                       frame=(1920, 1080), spp_levels=CROP_SPP_LEVELS, spp_ref: int = 16384, flags: int = 0,
-                      log=print):
+                      halves: bool = False, log=print):
     """Random size x size crops of full-resolution frames of seeded scenes in
     the style of the GGX/environment workload (scenes.c3_style_scene at
     `frame`), one crop per scene at a seeded position, as device tensors:
@@ -188,15 +188,22 @@ def make_crop_dataset(ctx: api.Context, n_scenes: int, first_seed: int = 20_000,
         render's) and take their albedo and normal from the 64-spp level,
         whose first-hit features are already converged;
       - target [n][size][size][3]: the crop at spp_ref from another seed,
-        FP64 accumulation, so input and target noise are independent."""
+        FP64 accumulation, so input and target noise are independent.
+    With halves, beauty is [n][levels][2][size][size][3]: each level's
+    samples as two renders of half as many (levels must be even), the
+    first with the AOVs."""
     import torch
     spp_levels = tuple(sorted(spp_levels))
     if not spp_levels or spp_levels[0] > FULL_FRAME_MAX_SPP:
         raise ValueError(f"the lowest input spp must be at most {FULL_FRAME_MAX_SPP} (it supplies the AOVs)")
+    if halves and any(s % 2 for s in spp_levels):
+        raise ValueError("with halves every input spp must be even")
     W, H = frame
     L = len(spp_levels)
     shape = (n_scenes, L, size, size, 3)
-    data = {k: torch.empty(shape, dtype=torch.float32, device="cuda") for k in ("beauty", "albedo", "normal")}
+    data = {k: torch.empty(shape, dtype=torch.float32, device="cuda") for k in ("albedo", "normal")}
+    data["beauty"] = torch.empty((n_scenes, L, 2, size, size, 3) if halves else shape, dtype=torch.float32,
+                                 device="cuda")
     data["target"] = torch.empty((n_scenes, size, size, 3), dtype=torch.float32, device="cuda")
     full = {k: torch.empty((H, W, 3), dtype=torch.float32, device="cuda") for k in ("beauty", "albedo", "normal")}
     wide = torch.empty((H, W, 3), dtype=torch.float64, device="cuda")
@@ -213,16 +220,24 @@ def make_crop_dataset(ctx: api.Context, n_scenes: int, first_seed: int = 20_000,
         torch.cuda.synchronize()
         aov_level = None
         for j, spp in enumerate(spp_levels):
-            if spp <= FULL_FRAME_MAX_SPP:
-                sh.render_aovs(full["beauty"], full["albedo"], full["normal"], spp, 16 * seed + j)
-                for k in ("beauty", "albedo", "normal"):
-                    data[k][i, j] = full[k][crop]
-                aov_level = j
-            else:
-                sh.render_pixels(wide, spp, 16 * seed + j, pix)
-                data["beauty"][i, j] = wide[crop].float()
-                data["albedo"][i, j] = data["albedo"][i, aov_level]
-                data["normal"][i, j] = data["normal"][i, aov_level]
+            parts = (spp // 2, spp // 2) if halves else (spp,)
+            for half, s in enumerate(parts):
+                dst = data["beauty"][i, j, half] if halves else data["beauty"][i, j]
+                rseed = 16 * seed + 2 * j + half
+                if spp <= FULL_FRAME_MAX_SPP:
+                    if half == 0:
+                        sh.render_aovs(full["beauty"], full["albedo"], full["normal"], s, rseed)
+                        data["albedo"][i, j] = full["albedo"][crop]
+                        data["normal"][i, j] = full["normal"][crop]
+                        aov_level = j
+                    else:
+                        sh.render(full["beauty"], s, rseed)
+                    dst.copy_(full["beauty"][crop])
+                else:
+                    sh.render_pixels(wide, s, rseed, pix)
+                    dst.copy_(wide[crop].float())
+                    data["albedo"][i, j] = data["albedo"][i, aov_level]
+                    data["normal"][i, j] = data["normal"][i, aov_level]
         sh.render_pixels(wide, spp_ref, 16 * seed + 15, pix)
         data["target"][i] = wide[crop].float()
         if (i + 1) % 25 == 0 or i + 1 == n_scenes:
@@ -231,13 +246,13 @@ def make_crop_dataset(ctx: api.Context, n_scenes: int, first_seed: int = 20_000,
 
 
 def augment(batch, k: int):
-    """The k-th of the 8 flips/transposes of a [n][h][w][3] square batch."""
+    """The k-th of the 8 flips/transposes of a [..][h][w][3] square batch."""
     if k & 1:
-        batch = batch.flip(2)
+        batch = batch.flip(-2)
     if k & 2:
-        batch = batch.flip(1)
+        batch = batch.flip(-3)
     if k & 4:
-        batch = batch.transpose(1, 2)
+        batch = batch.transpose(-3, -2)
     return batch.contiguous()
 
 
@@ -249,13 +264,18 @@ def train(dn: Denoiser, data, steps: int, batch: int = 16, lr: float = 1e-4, see
     g.manual_seed(seed)
     losses = []
     t0 = time.perf_counter()
-    levels = data["beauty"].shape[1] if data["beauty"].dim() == 5 else 0
+    # crop datasets hold [n][levels] inputs (albedo [n][levels][h][w][3]; beauty with a halves axis
+    # after the level for a halves network); whole-render datasets [n][h][w][3]
+    levels = data["albedo"].shape[1] if data["albedo"].dim() == 5 else 0
+    dev = data["albedo"].device
     for step in range(steps):
-        idx = torch.randint(0, n, (batch,), generator=g).cuda()
+        idx = torch.randint(0, n, (batch,), generator=g).to(dev)
         k = int(torch.randint(0, 8, (1,), generator=g))
         if levels:  # each element at a random input spp of its crop
-            lv = torch.randint(0, levels, (batch,), generator=g).cuda()
+            lv = torch.randint(0, levels, (batch,), generator=g).to(dev)
             b, a, nr = (augment(data[key][idx, lv], k) for key in ("beauty", "albedo", "normal"))
+            if b.dim() == 5:  # halves: [batch][2][h][w][3] -> [2][batch][h][w][3]
+                b = b.transpose(0, 1).contiguous()
             t = augment(data["target"].index_select(0, idx), k)
         else:
             b, a, nr, t = (augment(data[key].index_select(0, idx), k)
@@ -285,6 +305,8 @@ def main(argv=None) -> int:
     ap.add_argument("--spp-levels", default=",".join(map(str, CROP_SPP_LEVELS)),
                     help="input spp of the crop dataset, comma separated")
     ap.add_argument("--first-seed", type=int, default=20_000)
+    ap.add_argument("--halves", action="store_true",
+                    help="a network that takes each input as two half-sample renders (beauty_halves)")
     ap.add_argument("--out", required=True)
     args = ap.parse_args(argv)
 
@@ -293,7 +315,7 @@ def main(argv=None) -> int:
 
     ctx = api.default_context()
     loss = abi.GT_DENOISE_LOSS_REL_L2 if args.loss == "rel-l2" else abi.GT_DENOISE_LOSS_L1_TONEMAP
-    dn = Denoiser(ctx, loss=loss)
+    dn = Denoiser(ctx, loss=loss, halves=args.halves)
     if args.init:
         dn.load(args.init)
     log(f"denoiser: {dn.param_count} parameters, loss {args.loss}")
@@ -301,11 +323,11 @@ def main(argv=None) -> int:
     if args.crops:
         levels = tuple(int(v) for v in args.spp_levels.split(","))
         data = make_crop_dataset(ctx, args.scenes, first_seed=args.first_seed, spp_levels=levels,
-                                 spp_ref=args.spp_ref, flags=flags, log=log)
+                                 spp_ref=args.spp_ref, flags=flags, halves=args.halves, log=log)
     else:
         data = make_dataset(ctx, args.scenes, spp_ref=args.spp_ref, flags=flags, log=log)
     train(dn, data, args.steps, args.batch, args.lr, log=log)
-    dn.save(args.out)
+    dn.save(args.out, training=" ".join(["python -m gartus_amd.denoise_train"] + list(argv or sys.argv[1:])))
     log(f"saved {args.out}")
     return 0
 

@@ -7,6 +7,8 @@ The network runs on the HIP library; nothing here computes.
 from __future__ import annotations
 
 import ctypes as C
+import json
+from pathlib import Path
 from typing import Optional, Sequence
 
 import numpy as np
@@ -64,7 +66,7 @@ class Denoiser:
 
     def __init__(self, context: Optional[Context] = None, levels: int = 5,
                  channels: Sequence[int] = (32, 48, 64, 96, 128), seed: int = 1,
-                 loss: int = abi.GT_DENOISE_LOSS_REL_L2):
+                 loss: int = abi.GT_DENOISE_LOSS_REL_L2, halves: bool = False):
         self.ctx = context or default_context()
         cfg = abi.GtDenoiserConfig()
         cfg.levels = levels
@@ -72,7 +74,8 @@ class Denoiser:
             cfg.channels[i] = c
         cfg.seed = seed
         cfg.loss = loss
-        self.loss = loss
+        cfg.beauty_halves = 1 if halves else 0
+        self.loss, self.halves = loss, bool(halves)
         self.levels, self.channels = levels, tuple(channels[:levels])
         h = C.c_void_p()
         abi.check(self.ctx.lib.gt_denoiser_create(self.ctx.handle, C.byref(cfg), C.byref(h)))
@@ -95,21 +98,46 @@ class Denoiser:
             raise abi.InvalidArgument(abi.GT_EINVAL, "parameter count mismatch")
         abi.check(self.ctx.lib.gt_denoiser_set_params(self.handle, p.ctypes.data_as(abi.F)))
 
-    def save(self, path: str) -> None:
+    def config(self) -> dict:
+        return {"levels": self.levels, "channels": list(self.channels), "beauty_halves": self.halves,
+                "loss": "rel-l2" if self.loss == abi.GT_DENOISE_LOSS_REL_L2 else "l1-tonemap"}
+
+    def save(self, path: str, **info) -> None:
+        """The parameters as .npy and, beside them (same stem, .json), the
+        network's configuration with `info` (how it was trained)."""
         np.save(path, self.get_params(), allow_pickle=False)
+        Path(path).with_suffix(".json").write_text(json.dumps(dict(self.config(), **info), indent=1) + "\n")
+
+    @classmethod
+    def from_file(cls, path: str, context: Optional[Context] = None) -> "Denoiser":
+        """A network built from the configuration saved beside `path` (the
+        standard one when there is none) with the parameters in `path`."""
+        meta = Path(path).with_suffix(".json")
+        cfg = json.loads(meta.read_text()) if meta.exists() else {}
+        loss = abi.GT_DENOISE_LOSS_L1_TONEMAP if cfg.get("loss") == "l1-tonemap" else abi.GT_DENOISE_LOSS_REL_L2
+        dn = cls(context, levels=int(cfg.get("levels", 5)), channels=tuple(cfg.get("channels", (32, 48, 64, 96, 128))),
+                 loss=loss, halves=bool(cfg.get("beauty_halves", False)))
+        dn.load(path)
+        return dn
 
     def load(self, path: str) -> None:
         self.set_params(np.load(path, allow_pickle=False))
 
     # inputs --------------------------------------------------------------
     def _inputs(self, beauty, albedo, normal, target=None):
+        """A halves network takes beauty as [2][n][h][w][3] (or [2][h][w][3]):
+        two renders of half the samples each."""
         arrays = [beauty, albedo, normal] + ([target] if target is not None else [])
         shape = tuple(beauty.shape)
+        if self.halves:
+            if len(shape) < 4 or shape[0] != 2:
+                raise abi.InvalidArgument(abi.GT_EINVAL, "a halves denoiser takes beauty as [2][n][h][w][3]")
+            shape = shape[1:]
         if len(shape) == 3:
             shape = (1,) + shape
         if len(shape) != 4 or shape[3] != 3:
             raise abi.InvalidArgument(abi.GT_EINVAL, "images must be [n][h][w][3]")
-        for a in arrays:
+        for a in arrays[1:]:
             if tuple(a.shape)[-3:] != shape[1:] or (len(a.shape) == 4 and a.shape[0] != shape[0]):
                 raise abi.InvalidArgument(abi.GT_EINVAL, "image shapes differ")
         if _is_device(beauty):
@@ -147,7 +175,7 @@ class Denoiser:
         abi.check(self.ctx.lib.gt_denoiser_infer(self.handle, n, h, w, *ptrs, dst))
         if not _is_torch(beauty):
             abi.check(self.ctx.lib.gt_memcpy_to_host(self.ctx.handle, res.ctypes.data_as(C.c_void_p), dst, res.nbytes))
-            if len(beauty.shape) == 3:
+            if len(albedo.shape) == 3:
                 res = res[0]
         return res
 

@@ -17,7 +17,12 @@
  *   - Between levels: 2x2 max pooling down, nearest 2x upsampling up, and a
  *     skip concatenation.
  *   - Output: a final 3x3 convolution to three channels.
- *   - Input: 9 channels, i.e. log1p(beauty), albedo, normal, padded to 16.
+ *   - Input: log1p(beauty), albedo, normal, padded to 16 channels. With
+ *     beauty_halves, the beauty is two independent renders of half the
+ *     samples each, a and b: the network sees log1p of their mean where it
+ *     would see log1p(beauty), and log1p(a) - log1p(b) in channels 9-11. A
+ *     caustic shows in both halves, a single rare sample in one, which is
+ *     how the network tells a converging feature from a firefly.
  *   - Output z: radiance = expm1(max(z, 0)).
  *   - Weights and activations are bf16 with fp32 accumulation, on the matrix
  *     cores. Master weights, Adam state, loss and gradient reductions are fp32.
@@ -60,7 +65,8 @@ typedef struct gt_denoiser_config {
   uint32_t channels[6];   /* features per level, multiples of 16, at most 128 */
   uint64_t seed;          /* He-normal initialisation (SplitMix64 + Box-Muller) */
   uint32_t loss;          /* GT_DENOISE_LOSS_*: the training loss */
-  uint32_t reserved;      /* 0 */
+  uint32_t beauty_halves; /* 0: d_beauty is one image [n][h][w][3]; 1: two half-sample images
+                             [2][n][h][w][3] (a, then b) */
 } gt_denoiser_config;
 
 /* The standard network: 5 levels, 32-48-64-96-128 channels (1.02 M parameters),

@@ -139,14 +139,20 @@ def up2_backward(g):
     return g.reshape(n, H // 2, 2, W // 2, 2, c).sum(axis=(2, 4))
 
 
-def pack_input(beauty, albedo, normal, multiple):
+def pack_input(beauty, albedo, normal, multiple, beauty_b=None):
+    """With beauty_b (two half-sample renders a = beauty, b): log1p of their
+    mean in channels 0-2 and log1p(a) - log1p(b) in channels 9-11."""
     n, h, w, _ = beauty.shape
     hp, wp = -(-h // multiple) * multiple, -(-w // multiple) * multiple
     x = np.zeros((n, hp, wp, IN_CH))
     yy = np.minimum(np.arange(hp), h - 1)
     xx = np.minimum(np.arange(wp), w - 1)
     sel = (slice(None), yy[:, None], xx[None, :])
-    x[..., 0:3] = np.log1p(np.maximum(beauty, 0.0))[sel]
+    if beauty_b is None:
+        x[..., 0:3] = np.log1p(np.maximum(beauty, 0.0))[sel]
+    else:
+        x[..., 0:3] = np.log1p(np.maximum(0.5 * (beauty + beauty_b), 0.0))[sel]
+        x[..., 9:12] = (np.log1p(np.maximum(beauty, 0.0)) - np.log1p(np.maximum(beauty_b, 0.0)))[sel]
     x[..., 3:6] = albedo[sel]
     x[..., 6:9] = normal[sel]
     return x
@@ -162,10 +168,10 @@ def dtone(z):
 
 # ---- network ------------------------------------------------------------------
 
-def forward(net: Net, beauty, albedo, normal, params=None, bf=False):
+def forward(net: Net, beauty, albedo, normal, params=None, bf=False, beauty_b=None):
     """Returns (z, cache)."""
     r = bf16 if bf else (lambda a: a)
-    x0 = r(pack_input(beauty, albedo, normal, 1 << (net.L - 1)))
+    x0 = r(pack_input(beauty, albedo, normal, 1 << (net.L - 1), beauty_b))
     W = {}
     for i in range(len(net.convs)):
         w, b = net.wb(i, params)
@@ -268,8 +274,9 @@ def backward(net: Net, cache, dz, bf=False):
     return grads
 
 
-def loss_and_gradients(net: Net, beauty, albedo, normal, target, params=None, bf=False, kind=LOSS_REL_L2):
-    z, cache = forward(net, beauty, albedo, normal, params, bf)
+def loss_and_gradients(net: Net, beauty, albedo, normal, target, params=None, bf=False, kind=LOSS_REL_L2,
+                       beauty_b=None):
+    z, cache = forward(net, beauty, albedo, normal, params, bf, beauty_b)
     loss, dz = loss_and_dz(z, target, bf, kind)
     return loss, backward(net, cache, dz, bf), z
 

@@ -127,3 +127,4 @@ def test_limiter_reads_the_counters():
     assert bench.limiter(dict(base, frac=0.8)) == "HBM bandwidth"
     assert bench.limiter(dict(base, traffic=None)) is None
     assert bench.limiter(dict(base, wave_wait_frac=0.2)).startswith("mixed")
+

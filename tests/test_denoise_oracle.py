@@ -179,3 +179,26 @@ def test_gradient_matches_finite_differences(levels, channels, h, w):
     w0, _ = net.offsets[0]
     wg = g[w0:w0 + net.convs[0][1] * 9 * 16].reshape(-1, 9, 16)
     assert not wg[:, :, 9:].any()
+
+
+def test_halves_pack_mean_and_log_difference():
+    a = np.array([[[[1.0, 3.0, 0.0]]]])
+    b = np.array([[[[3.0, 1.0, -1.0]]]])
+    x = dr.pack_input(a, np.zeros_like(a), np.zeros_like(a), 1, beauty_b=b)
+    np.testing.assert_allclose(x[0, 0, 0, 0:3], np.log1p([2.0, 2.0, 0.0]))
+    np.testing.assert_allclose(x[0, 0, 0, 9:12], [np.log(2.0) - np.log(4.0), np.log(4.0) - np.log(2.0), 0.0])
+    assert not x[0, 0, 0, 12:].any()
+
+
+def test_halves_network_with_zero_difference_weights_is_the_single_network():
+    """Weights trained for one beauty image carry over: with zero weights on
+    channels 9-11, the halves network on (a, b) equals the single-image
+    network on their mean."""
+    net, b, a, nr, _ = _problem(2, (16, 16))
+    w0, b0 = net.offsets[0]
+    net.params[w0:b0].reshape(net.convs[0][1], 9, 16)[:, :, 9:] = 0.0
+    rng = np.random.default_rng(2)
+    b2 = rng.uniform(0.0, 3.0, b.shape)
+    z_halves, _ = dr.forward(net, b, a, nr, beauty_b=b2)
+    z_mean, _ = dr.forward(net, 0.5 * (b + b2), a, nr)
+    np.testing.assert_allclose(z_halves, z_mean, rtol=1e-12, atol=1e-12)

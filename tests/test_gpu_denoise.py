@@ -123,3 +123,38 @@ def test_training_reduces_loss(gpu_context):
         last = dn.train_step(*args, lr=2e-3)
     assert np.isfinite(last) and last < 0.5 * first, (first, last)
     dn.close()
+
+
+@pytest.mark.parametrize("levels,channels,n,h,w", CASES[1:])
+def test_halves_network_matches_oracle(gpu_context, levels, channels, n, h, w):
+    """beauty_halves: two half-sample renders packed as log1p of their mean
+    and the log difference (channels 9-11), which here carry weight."""
+    say(f"denoiser with beauty halves, levels {levels} batch {n}x{h}x{w}")
+    net, b, a, nr, t = _problem(levels, channels, n, h, w, seed=20 + levels)
+    rng = np.random.default_rng(30 + levels)
+    w0, b0 = net.offsets[0]
+    wts = net.params[w0:b0].reshape(net.convs[0][1], 9, 16).copy()
+    wts[:, :, 9:12] = dr.bf16(rng.normal(scale=0.2, size=wts[:, :, 9:12].shape))
+    net.params[w0:b0] = wts.ravel()
+    b2 = rng.gamma(1.0, 1.0, (n, h, w, 3)).astype(np.float32)
+    dn = Denoiser(gpu_context, levels, channels, halves=True)
+    dn.set_params(net.params.astype(np.float32))
+    loss_ref, g_ref, z_ref = dr.loss_and_gradients(net, b.astype(np.float64), a.astype(np.float64),
+                                                   nr.astype(np.float64), t.astype(np.float64), bf=True,
+                                                   beauty_b=b2.astype(np.float64))
+    halves = np.stack([b, b2])
+    zg, zr = np.log1p(dn.infer(halves, a, nr)), np.log1p(dr.radiance(z_ref, h, w))
+    err = np.abs(zg - zr)
+    assert np.mean(err <= 0.02 + 0.02 * np.abs(zr)) > 0.995, f"max {err.max()}"
+    loss_gpu, g_gpu = dn.loss_and_gradients(halves, a, nr, t)
+    # relative L2 divides by the squared prediction (+0.01), so the bf16-level output differences
+    # already bounded above move it more than they move L1
+    assert loss_gpu == pytest.approx(loss_ref, rel=1e-2)
+    w0, b0 = net.offsets[0]
+    gr, gg = g_ref[w0:b0], g_gpu[w0:b0].astype(np.float64)
+    rel = np.linalg.norm(gg - gr) / np.linalg.norm(gr)
+    cos = gg @ gr / (np.linalg.norm(gg) * np.linalg.norm(gr))
+    assert rel < 0.01 * np.sqrt(len(net.convs)) and cos > 0.998, (rel, cos)  # the bar of the test above
+    # the first convolution's gradient reaches the difference channels
+    assert np.abs(gg.reshape(-1, 9, 16)[:, :, 9:12]).max() > 0
+    dn.close()
