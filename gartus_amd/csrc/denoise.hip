@@ -575,6 +575,8 @@ __global__ void k_upcat_bwd(const uint16_t* __restrict__ dout, int ca, int cb, u
 }
 
 // ---- input, loss, output ---------------------------------------------------
+constexpr float kHalvesClip = 2.f;  // bound of the halves' log difference (channels 9-11)
+
 __global__ void k_pack(const float* __restrict__ beauty, const float* __restrict__ beauty_b,
                        const float* __restrict__ albedo, const float* __restrict__ normal, int h, int w,
                        uint16_t* __restrict__ x, int Hp, int Wp, size_t npx) {
@@ -593,7 +595,9 @@ __global__ void k_pack(const float* __restrict__ beauty, const float* __restrict
       if (beauty_b) {
         const float b2 = beauty_b[s + c];
         v[c] = f2bf(log1pf(fmaxf(0.5f * (a + b2), 0.f)));
-        v[9 + c] = f2bf(log1pf(fmaxf(a, 0.f)) - log1pf(fmaxf(b2, 0.f)));
+        // clipped: a firefly's log ratio (up to ~10) would take the network far outside
+        // what it was trained on, while a ratio of e^2 already marks one half as an outlier
+        v[9 + c] = f2bf(fminf(fmaxf(log1pf(fmaxf(a, 0.f)) - log1pf(fmaxf(b2, 0.f)), -kHalvesClip), kHalvesClip));
       } else {
         v[c] = f2bf(log1pf(fmaxf(a, 0.f)));
       }

@@ -20,7 +20,8 @@
  *   - Input: log1p(beauty), albedo, normal, padded to 16 channels. With
  *     beauty_halves, the beauty is two independent renders of half the
  *     samples each, a and b: the network sees log1p of their mean where it
- *     would see log1p(beauty), and log1p(a) - log1p(b) in channels 9-11. A
+ *     would see log1p(beauty), and log1p(a) - log1p(b) clipped to [-2, 2]
+ *     in channels 9-11. A
  *     caustic shows in both halves, a single rare sample in one, which is
  *     how the network tells a converging feature from a firefly.
  *   - Output z: radiance = expm1(max(z, 0)).

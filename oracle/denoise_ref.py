@@ -139,9 +139,13 @@ def up2_backward(g):
     return g.reshape(n, H // 2, 2, W // 2, 2, c).sum(axis=(2, 4))
 
 
+HALVES_CLIP = 2.0
+
+
 def pack_input(beauty, albedo, normal, multiple, beauty_b=None):
     """With beauty_b (two half-sample renders a = beauty, b): log1p of their
-    mean in channels 0-2 and log1p(a) - log1p(b) in channels 9-11."""
+    mean in channels 0-2 and log1p(a) - log1p(b), clipped to +-2, in
+    channels 9-11."""
     n, h, w, _ = beauty.shape
     hp, wp = -(-h // multiple) * multiple, -(-w // multiple) * multiple
     x = np.zeros((n, hp, wp, IN_CH))
@@ -152,7 +156,8 @@ def pack_input(beauty, albedo, normal, multiple, beauty_b=None):
         x[..., 0:3] = np.log1p(np.maximum(beauty, 0.0))[sel]
     else:
         x[..., 0:3] = np.log1p(np.maximum(0.5 * (beauty + beauty_b), 0.0))[sel]
-        x[..., 9:12] = (np.log1p(np.maximum(beauty, 0.0)) - np.log1p(np.maximum(beauty_b, 0.0)))[sel]
+        d = np.log1p(np.maximum(beauty, 0.0)) - np.log1p(np.maximum(beauty_b, 0.0))
+        x[..., 9:12] = np.clip(d, -HALVES_CLIP, HALVES_CLIP)[sel]
     x[..., 3:6] = albedo[sel]
     x[..., 6:9] = normal[sel]
     return x

@@ -182,11 +182,13 @@ def test_gradient_matches_finite_differences(levels, channels, h, w):
 
 
 def test_halves_pack_mean_and_log_difference():
-    a = np.array([[[[1.0, 3.0, 0.0]]]])
-    b = np.array([[[[3.0, 1.0, -1.0]]]])
+    a = np.array([[[[1.0, 3.0, 0.0]], [[100.0, 0.0, 0.5]]]])
+    b = np.array([[[[3.0, 1.0, -1.0]], [[0.0, 100.0, 0.5]]]])
     x = dr.pack_input(a, np.zeros_like(a), np.zeros_like(a), 1, beauty_b=b)
     np.testing.assert_allclose(x[0, 0, 0, 0:3], np.log1p([2.0, 2.0, 0.0]))
     np.testing.assert_allclose(x[0, 0, 0, 9:12], [np.log(2.0) - np.log(4.0), np.log(4.0) - np.log(2.0), 0.0])
+    # a firefly's log ratio is clipped to +-2
+    np.testing.assert_allclose(x[0, 1, 0, 9:12], [2.0, -2.0, 0.0])
     assert not x[0, 0, 0, 12:].any()
 
 

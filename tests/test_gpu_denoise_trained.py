@@ -1,9 +1,10 @@
 """What the trained denoiser gains over plain path tracing.
 
-The committed parameters (gartus_amd/weights/denoiser_c3.npy) were trained
-by gartus_amd.denoise_train on scenes.c3_style_scene seeds 10000-10399
-(whole 256x256 renders) and then 20000-20319 (random 256x256 crops of
-1920x1080 frames, inputs at 1-16384 spp). These tests use scenes outside the
+The committed parameters (gartus_amd/weights/denoiser_c3.npy, configuration
+and training commands beside them in denoiser_c3.json) were trained by
+gartus_amd.denoise_train on scenes.c3_style_scene seeds 10000-10399, 20000-
+20319 and 21000-21199 (random 256x256 crops of 1920x1080 frames, inputs at
+1-16384 spp). These tests use scenes outside the
 training set, in the regime the time-to-relMSE comparison runs in: a
 256x256 crop of a 1920x1080 frame, rendered by this tracer at 16 and at
 1024 spp with its AOVs, against a 65,536-spp FP64 reference of the crop.
