@@ -14,12 +14,14 @@
 #include "denoise.h"
 
 #include <algorithm>
+#include <atomic>
 
 namespace gdn {
 
 namespace {
 
 typedef float v4f __attribute__((ext_vector_type(4)));
+typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
 
 __device__ __forceinline__ float bf2f(uint16_t b) { return __uint_as_float(uint32_t(b) << 16); }
 __device__ __forceinline__ uint16_t f2bf(float f) {  // round to nearest even; NaN stays NaN
@@ -58,6 +60,7 @@ __device__ __forceinline__ v4f mfma32(uint4 a, uint4 b, v4f c) {
   return __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(v8bf, a), __builtin_bit_cast(v8bf, b), c, 0, 0,
                                                  0);
 }
+
 
 // Zeroes the bf16 elements of a whose mask element is not positive.
 __device__ __forceinline__ uint4 mask8(uint4 a, uint4 m) {
@@ -263,6 +266,365 @@ __global__ __launch_bounds__(256) void k_conv(const uint16_t* __restrict__ x, co
       *reinterpret_cast<uint4*>(static_cast<uint16_t*>(y) + (pix0 + w0 + p) * cout + co0 + 8 * g) = o;
     }
   }
+}
+
+// ---- convolution, forward: persistent blocks, halos copied straight to LDS --
+// The same tiles and MFMA steps as k_conv, for layers of up to 48 output
+// channels (one group). A block keeps the layer's weights in LDS for its
+// whole life and walks a run of tiles. A tile's input channels run in chunks
+// of up to 32 that never straddle the two parts of a decoder concatenation;
+// a (tile, chunk) pair is one stage. The halo of stage s + 1 is copied from
+// global memory into the second of two LDS buffers by buffer loads that
+// write LDS directly (no registers), while the waves run the MFMA steps of
+// stage s; border pixels are read at an offset past the buffer's range and
+// land as zeros. The halo has no padding: piece q (16 bytes) of pixel p sits
+// at slot p * 2G + (q ^ swz<G>(p)), so 16 consecutive pixels' reads of one
+// piece fall on distinct banks of the 64-bank LDS. The two buffers are
+// separate LDS objects and each step names its pair statically, so the
+// compiler sees that the copy in flight and the reads do not overlap and
+// puts no wait between them. Blocks are dealt round-robin to the 8 XCDs;
+// each XCD's blocks walk one contiguous eighth of the tiles, so tiles that
+// share halo rows are read through the same L2.
+typedef __attribute__((address_space(3))) void lds_void;
+
+constexpr uint32_t kOutOfRange = 0x80000000u;  // a buffer offset past every tensor this kernel reads
+constexpr int kWaitVm0 = 0x0F70;    // s_waitcnt vmcnt(0)
+constexpr int kWaitLgkm0 = 0xC07F;  // s_waitcnt lgkmcnt(0)
+
+// A workgroup barrier for LDS data only: this wave's LDS reads and writes are
+// complete, then the waves meet. Unlike __syncthreads it does not also wait
+// for the LDS copies in flight; those are waited for per buffer.
+__device__ __forceinline__ void lds_barrier() {
+  __atomic_signal_fence(__ATOMIC_SEQ_CST);
+  __builtin_amdgcn_s_waitcnt(kWaitLgkm0);
+  __builtin_amdgcn_s_barrier();
+  __atomic_signal_fence(__ATOMIC_SEQ_CST);
+}
+
+// The halo buffers hold a chunk of up to 16*GMAX channels in whole 64-slot
+// copies, and the output tile, which is staged in the buffer it was computed
+// from: 256 pixels x NT*16 channels, bf16 (fp32 for the network output), rows
+// padded by 16 bytes when the padded tile still fits.
+template <int NT, bool OUT_F32>
+constexpr int pipe_raw_row_bytes() {
+  return OUT_F32 ? NT * 16 * 4 : NT * 16 * 2;
+}
+template <int NT, bool OUT_F32, int GMAX>
+constexpr int pipe_buf_slots() {
+  return std::max((kHR * kHC * 2 * GMAX + 63) / 64 * 64, 256 * pipe_raw_row_bytes<NT, OUT_F32>() / 16);
+}
+template <int NT, bool OUT_F32, int GMAX>
+constexpr int pipe_row_bytes() {
+  constexpr int raw = pipe_raw_row_bytes<NT, OUT_F32>();
+  return 256 * (raw + 16) <= pipe_buf_slots<NT, OUT_F32, GMAX>() * 16 ? raw + 16 : raw;
+}
+template <int NT, bool OUT_F32>
+constexpr int pipe_row_pieces() {  // 16-byte pieces of one output pixel
+  return OUT_F32 ? NT * 4 : NT * 2;
+}
+constexpr int pipe_pool_per_thread(int nt) { return (2 * 32 * nt * 2 + 255) / 256; }
+
+template <int G>
+__device__ __forceinline__ int swz(int p) {
+  return G == 2 ? (p >> 2) & 3 : (p >> 3) & 1;
+}
+
+// Chunks of a layer's input: channels [0, ca) of the half-resolution part
+// first, then [ca, cin) of the full-resolution part, each in runs of 16*GMAX
+// channels and a last run of 16 when GMAX is 2 and the part has an odd
+// number of 16-channel groups.
+struct PipeChunk {
+  int c0, g;
+  bool half_res;
+};
+template <int GMAX>
+__device__ __forceinline__ int pipe_nchunks(int cin, int ca) {
+  constexpr int CS = 16 * GMAX;
+  return (ca + CS - 1) / CS + (cin - ca + CS - 1) / CS;
+}
+template <int GMAX>
+__device__ __forceinline__ PipeChunk pipe_chunk(int k, int cin, int ca) {
+  constexpr int CS = 16 * GMAX;
+  const int na = (ca + CS - 1) / CS;
+  if (k < na) return {CS * k, std::min(GMAX, (ca - CS * k) / 16), true};
+  const int c0 = ca + CS * (k - na);
+  return {c0, std::min(GMAX, (cin - c0) / 16), false};
+}
+
+typedef int32_t i32x4 __attribute__((ext_vector_type(4)));
+
+// A raw buffer resource over [base, base + bytes): reads past the end return
+// zeros.
+__device__ __forceinline__ i32x4 buffer_rsrc(const void* base, uint32_t bytes) {
+  const uint64_t a = reinterpret_cast<uint64_t>(base);
+  i32x4 r;
+  r.x = __builtin_amdgcn_readfirstlane(int32_t(uint32_t(a)));
+  r.y = __builtin_amdgcn_readfirstlane(int32_t(uint32_t(a >> 32) & 0xffffu));  // stride 0
+  r.z = __builtin_amdgcn_readfirstlane(int32_t(bytes));
+  r.w = 0x00020000;
+  return r;
+}
+
+// Copies 16 bytes per lane from byte offset off of the buffer src to LDS:
+// lane l writes dst + 16 l, dst being the same LDS address in every lane (it
+// goes to M0). Written in assembly so that the compiler does not take the
+// copy in flight for a write to every LDS array and wait for it before the
+// reads of the other buffer; the kernel waits for it itself (vmcnt) before it
+// reads the buffer. The copy writes no register. The compiler keeps nothing in
+// M0 across these blocks in this kernel (no LDS-copy builtins, message sends
+// or relative register moves), so clobbering it is safe here.
+__device__ __forceinline__ void lds_copy16(const uint4* dst, i32x4 src, uint32_t off) {
+  const uint32_t m0 = __builtin_amdgcn_readfirstlane(uint32_t(reinterpret_cast<uintptr_t>((const lds_void*)dst)));
+#pragma clang diagnostic push
+#pragma clang diagnostic ignored "-Winline-asm"
+  asm volatile("s_mov_b32 m0, %0\n\ts_nop 0\n\tbuffer_load_dwordx4 %1, %2, 0 offen lds" ::"s"(m0), "v"(off), "s"(src)
+               : "memory", "m0");
+#pragma clang diagnostic pop
+}
+
+// Issues the copies of one chunk's halo into buf: wave w copies slots
+// [64 i, 64 i + 64) for i = w, w + 4, ... Channel c of the source (src_c
+// channels per pixel) is the chunk's first.
+template <int G>
+__device__ __forceinline__ void pipe_copy_halo(uint4* buf, i32x4 src, int src_c, int c,
+                                               bool half_res, int n, int h0, int w0, int H, int W, int wave,
+                                               int lane) {
+  constexpr int QP = 2 * G, SL = kHR * kHC * QP, NI = (SL + 63) / 64;
+  const int Hs = half_res ? H / 2 : H, Ws = half_res ? W / 2 : W;
+#pragma unroll
+  for (int i0 = 0; i0 < NI; i0 += 4) {
+    const int i = i0 + wave;
+    if (i < NI) {
+      const int s = i * 64 + lane, px = s / QP;
+      const int q = (s % QP) ^ swz<G>(px);
+      const int hh = h0 - 1 + px / kHC, ww = w0 - 1 + px % kHC;
+      uint32_t off = kOutOfRange;
+      if (s < SL && hh >= 0 && hh < H && ww >= 0 && ww < W) {
+        const int sh = half_res ? hh >> 1 : hh, sw = half_res ? ww >> 1 : ww;
+        off = uint32_t((((size_t(n) * Hs + sh) * Ws + sw) * src_c + c + 8 * q) * 2);
+      }
+      lds_copy16(buf + i * 64, src, off);
+    }
+  }
+}
+
+// The MFMA steps of one chunk (G 16-channel groups) from the swizzled halo
+// and the chunk's weights from slot wb of wts (rows tap * NT*16 + co, 2G
+// pieces each, swizzled the same way by row). Both arrays are indexed from
+// their bases in one step.
+template <int NT, int G>
+__device__ __forceinline__ void pipe_chunk_mfma(const uint4* halo, const uint4* wts, int wb, int lane, int wave,
+                                                v4f (&acc)[4][NT]) {
+  constexpr int QP = 2 * G;
+  const int half = (lane >> 4) & 1, col = lane & 15;
+#pragma unroll
+  for (int i = 0; i < (9 * G + 1) / 2; ++i) {
+    const int sub = 2 * i + (lane >> 5);
+    const bool valid = sub < 9 * G;
+    const int tap = valid ? sub / G : 0;
+    const int q = (sub % G) * 2 + half;
+    const int dy = tap / 3 - 1, dx = tap % 3 - 1;
+    uint4 b[NT];
+#pragma unroll
+    for (int j = 0; j < NT; ++j) {
+      const int r = tap * NT * 16 + j * 16 + col;
+      b[j] = valid ? wts[wb + r * QP + (q ^ swz<G>(r))] : make_uint4(0, 0, 0, 0);
+    }
+    const int p0 = (wave + 1 + dy) * kHC + col + 1 + dx;
+#pragma unroll
+    for (int f = 0; f < 4; ++f) {
+      const int p = p0 + 16 * f;
+      const uint4 a = valid ? halo[p * QP + (q ^ swz<G>(p))] : make_uint4(0, 0, 0, 0);
+#pragma unroll
+      for (int j = 0; j < NT; ++j) acc[f][j] = mfma32(a, b[j], acc[f][j]);
+    }
+  }
+}
+
+// A finished tile's output, read back from its LDS staging into registers:
+// this wave's row (pieces lane, lane + 64, ...) and this thread's pieces of
+// the 2x2 max pool. The stores are issued one step later, right after the
+// next halo copy, so that the wait for that copy does not also wait for them.
+template <int NT, bool OUT_F32>
+struct PipeOut {
+  u32x4 row[pipe_row_pieces<NT, OUT_F32>()];  // vector types, not uint4: they stay in registers
+  u32x4 pool[pipe_pool_per_thread(NT)];
+  int tile = -1;
+};
+
+// Bias and ReLU of the accumulators, staged through buf (bf16, or fp32 for the
+// network output) and read back into out; clears acc for the next tile.
+template <int NT, bool OUT_F32, int GMAX>
+__device__ __forceinline__ void pipe_finish(uint4* buf, v4f (&acc)[4][NT], const float (&bj)[NT], int relu,
+                                            bool pooled, PipeOut<NT, OUT_F32>& out, int t, int tid, int lane,
+                                            int wave) {
+  constexpr int RB = pipe_row_bytes<NT, OUT_F32, GMAX>(), GP = pipe_row_pieces<NT, OUT_F32>();
+  char* st = reinterpret_cast<char*>(buf);
+  lds_barrier();  // every wave is done with the halo in buf
+#pragma unroll
+  for (int j = 0; j < NT; ++j)
+#pragma unroll
+    for (int f = 0; f < 4; ++f)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        float v = acc[f][j][r] + bj[j];
+        if (relu) v = fmaxf(v, 0.f);
+        const int px = wave * 64 + f * 16 + (lane >> 4) * 4 + r, c = j * 16 + (lane & 15);
+        if (OUT_F32) *reinterpret_cast<float*>(st + px * RB + 4 * c) = v;
+        else *reinterpret_cast<uint16_t*>(st + px * RB + 2 * c) = f2bf(v);
+        acc[f][j][r] = 0.f;
+      }
+  lds_barrier();
+#pragma unroll
+  for (int k = 0; k < GP; ++k) {
+    const int i = lane + 64 * k, p = i / GP, g = i % GP;
+    out.row[k] = *reinterpret_cast<const u32x4*>(st + (wave * 64 + p) * RB + 16 * g);
+  }
+  if (!OUT_F32 && pooled) {  // the staged values are already bf16: the max of them is the pooled value
+    constexpr int G8 = NT * 2;
+#pragma unroll
+    for (int k = 0; k < pipe_pool_per_thread(NT); ++k) {
+      const int i = tid + 256 * k;
+      if (i >= 2 * 32 * G8) break;
+      const int g = i % G8, pc = (i / G8) % 32, pr = i / (G8 * 32);
+      const char* a = st + ((2 * pr) * 64 + 2 * pc) * RB + 16 * g;
+      const u32x4 v[4] = {*reinterpret_cast<const u32x4*>(a), *reinterpret_cast<const u32x4*>(a + RB),
+                          *reinterpret_cast<const u32x4*>(a + 64 * RB),
+                          *reinterpret_cast<const u32x4*>(a + 65 * RB)};
+      u32x4 o;
+#pragma unroll
+      for (int e = 0; e < 4; ++e) {
+        const uint32_t w[4] = {v[0][e], v[1][e], v[2][e], v[3][e]};
+        float lo = bf2f(uint16_t(w[0])), hi = bf2f(uint16_t(w[0] >> 16));
+#pragma unroll
+        for (int m = 1; m < 4; ++m) {
+          lo = fmaxf(lo, bf2f(uint16_t(w[m])));
+          hi = fmaxf(hi, bf2f(uint16_t(w[m] >> 16)));
+        }
+        o[e] = uint32_t(f2bf(lo)) | (uint32_t(f2bf(hi)) << 16);
+      }
+      out.pool[k] = o;
+    }
+  }
+  out.tile = t;
+}
+
+// The global stores of a finished tile held in out.
+template <int NT, bool OUT_F32>
+__device__ __forceinline__ void pipe_store(PipeOut<NT, OUT_F32>& out, void* y, uint16_t* pooled, int cout, int co0,
+                                           int H, int W, int nrt, int nct, int tid, int lane, int wave) {
+  constexpr int GP = pipe_row_pieces<NT, OUT_F32>();
+  const int t = out.tile;
+  out.tile = -1;
+  const int ct = t % nct, rt = (t / nct) % nrt, n = t / (nct * nrt);
+  const int h0 = rt * kTR, w0 = ct * kTC;
+  const int row = h0 + wave;
+  if (row < H) {
+    const size_t pix0 = (size_t(n) * H + row) * W;
+#pragma unroll
+    for (int k = 0; k < GP; ++k) {
+      const int i = lane + 64 * k, p = i / GP, g = i % GP;
+      if (w0 + p >= W) continue;
+      if (OUT_F32)
+        *reinterpret_cast<u32x4*>(static_cast<float*>(y) + (pix0 + w0 + p) * cout + co0 + 4 * g) = out.row[k];
+      else
+        *reinterpret_cast<u32x4*>(static_cast<uint16_t*>(y) + (pix0 + w0 + p) * cout + co0 + 8 * g) = out.row[k];
+    }
+  }
+  if (!OUT_F32 && pooled) {
+    constexpr int G8 = NT * 2;
+    const int H2 = H / 2, W2 = W / 2;
+#pragma unroll
+    for (int k = 0; k < pipe_pool_per_thread(NT); ++k) {
+      const int i = tid + 256 * k;
+      if (i >= 2 * 32 * G8) break;
+      const int g = i % G8, pc = (i / G8) % 32, pr = i / (G8 * 32);
+      const int h2 = h0 / 2 + pr, w2 = w0 / 2 + pc;
+      if (h2 < H2 && w2 < W2)
+        *reinterpret_cast<u32x4*>(pooled + ((size_t(n) * H2 + h2) * W2 + w2) * cout + co0 + 8 * g) = out.pool[k];
+    }
+  }
+}
+
+template <int NT, bool OUT_F32, int CIN, int GMAX>
+__global__ __launch_bounds__(256) void k_conv_pipe(const uint16_t* __restrict__ x,
+                                                   const uint16_t* __restrict__ xa, int ca,
+                                                   const uint16_t* __restrict__ wgt, const float* __restrict__ bias,
+                                                   void* __restrict__ y, uint16_t* __restrict__ pooled, int cout,
+                                                   int co_first, int N, int H, int W, int nrt, int nct, int relu) {
+  __shared__ uint4 bufA[pipe_buf_slots<NT, OUT_F32, GMAX>()];
+  __shared__ uint4 bufB[pipe_buf_slots<NT, OUT_F32, GMAX>()];
+  __shared__ uint4 wts4[9 * NT * 16 * CIN / 8];
+  constexpr int cin = CIN;
+  const int tid = threadIdx.x, lane = tid & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+  const int co0 = co_first + blockIdx.y * NT * 16;
+  const int cx = cin - ca;
+  const i32x4 rx = buffer_rsrc(x, uint32_t(size_t(N) * H * W * cx * 2));
+  const i32x4 ra = buffer_rsrc(xa ? xa : x, xa ? uint32_t(size_t(N) * (H / 2) * (W / 2) * ca * 2) : 0u);
+
+  // this block's tiles: the XCD's eighth of them, every per-th from slot0
+  const int ntiles = N * nrt * nct;
+  const int xcd = blockIdx.x & 7, per = gridDim.x >> 3, slot0 = blockIdx.x >> 3;
+  const int t_begin = int(int64_t(ntiles) * xcd / 8), t_end = int(int64_t(ntiles) * (xcd + 1) / 8);
+  const int mine = t_end - t_begin > slot0 ? (t_end - t_begin - slot0 + per - 1) / per : 0;
+  const int nch = pipe_nchunks<GMAX>(cin, ca);
+  const int S = mine * nch;
+  if (S == 0) return;  // uniform over the block: no barrier is skipped by part of it
+
+  // the weights, once: piece c/8 of row (tap, co) of the chunk holding channel c
+  const int npc = cin / 8;
+  for (int i = tid; i < 9 * NT * 16 * npc; i += 256) {
+    const int cp = i % npc, rt = i / npc, co = rt / 9, tap = rt % 9, c = 8 * cp;
+    constexpr int CS = 16 * GMAX;
+    const int c0 = c < ca ? c / CS * CS : ca + (c - ca) / CS * CS;
+    const int g = std::min(GMAX, ((c < ca ? ca : cin) - c0) / 16);
+    const int r = tap * NT * 16 + co, q = (c - c0) >> 3;
+    const int sw = g == 2 ? swz<2>(r) : swz<1>(r);
+    wts4[9 * NT * 16 * (c0 / 8) + r * 2 * g + (q ^ sw)] =
+        *reinterpret_cast<const uint4*>(wgt + (size_t(co0 + co) * 9 + tap) * cin + c);
+  }
+  float bj[NT];
+#pragma unroll
+  for (int j = 0; j < NT; ++j) bj[j] = bias ? bias[co0 + j * 16 + (lane & 15)] : 0.f;
+
+  auto copy = [&](uint4* buf, int s) {
+    const int k = s / nch, ch = s - k * nch;
+    const int t = t_begin + slot0 + k * per;
+    const int ct = t % nct, rt = (t / nct) % nrt, n = t / (nct * nrt);
+    const PipeChunk c = pipe_chunk<GMAX>(ch, cin, ca);
+    const i32x4 src = c.half_res ? ra : rx;
+    const int src_c = c.half_res ? ca : cx, c_src = c.half_res ? c.c0 : c.c0 - ca;
+    if (GMAX == 2 && c.g == 2)
+      pipe_copy_halo<2>(buf, src, src_c, c_src, c.half_res, n, rt * kTR, ct * kTC, H, W, wave, lane);
+    else pipe_copy_halo<1>(buf, src, src_c, c_src, c.half_res, n, rt * kTR, ct * kTC, H, W, wave, lane);
+  };
+  v4f acc[4][NT];
+#pragma unroll
+  for (int f = 0; f < 4; ++f)
+#pragma unroll
+    for (int j = 0; j < NT; ++j) acc[f][j] = v4f{0.f, 0.f, 0.f, 0.f};
+  PipeOut<NT, OUT_F32> out;
+  auto step = [&](uint4* cur, uint4* nxt, int s) {
+    __builtin_amdgcn_s_waitcnt(kWaitVm0);  // this wave's copies into cur (and its last stores) are done
+    lds_barrier();  // ... and every wave's; nxt was last read in the previous step
+    if (s + 1 < S) copy(nxt, s + 1);
+    if (out.tile >= 0) pipe_store<NT, OUT_F32>(out, y, pooled, cout, co0, H, W, nrt, nct, tid, lane, wave);
+    const int k = s / nch, ch = s - k * nch;
+    const PipeChunk c = pipe_chunk<GMAX>(ch, cin, ca);
+    const int wb = 9 * NT * 16 * (c.c0 / 8);
+    if (GMAX == 2 && c.g == 2) pipe_chunk_mfma<NT, 2>(cur, wts4, wb, lane, wave, acc);
+    else pipe_chunk_mfma<NT, 1>(cur, wts4, wb, lane, wave, acc);
+    if (ch == nch - 1)
+      pipe_finish<NT, OUT_F32, GMAX>(cur, acc, bj, relu, pooled != nullptr, out, t_begin + slot0 + k * per, tid, lane,
+                               wave);
+  };
+  copy(bufA, 0);
+  for (int s = 0; s < S; s += 2) {
+    step(bufA, bufB, s);
+    if (s + 1 < S) step(bufB, bufA, s + 1);
+  }
+  if (out.tile >= 0) pipe_store<NT, OUT_F32>(out, y, pooled, cout, co0, H, W, nrt, nct, tid, lane, wave);
 }
 
 // ---- convolution, weight gradient: LDS tiles and 16x16x32 MFMA -------------
@@ -731,10 +1093,66 @@ void launch_nt(hipStream_t s, const uint16_t* x, const uint16_t* mask, int cin, 
 }
 
 // Output channels in groups of 64 (NT = 4), then one launch for the rest.
+int device_cus() {
+  static std::atomic<int> cus{0};
+  int n = cus.load(std::memory_order_relaxed);
+  if (!n) {
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess ||
+        hipDeviceGetAttribute(&n, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || n <= 0)
+      n = 256;
+    cus.store(n, std::memory_order_relaxed);
+  }
+  return n;
+}
+
+// The persistent forward convolution, for the layer shapes where it is faster
+// than k_conv on one MI355X: 16 or 32 input and output channels (the default
+// network's full-resolution convolutions other than the decoder's
+// concatenation), tensors addressable by 31-bit buffer offsets. False for the
+// others, which take k_conv: with 48 output channels or 80 input channels a
+// block needs more registers or LDS, one block fits a CU, and the layer ran
+// 10-30 % slower than k_conv.
+template <int NT, bool F32, int CIN, int GMAX>
+bool launch_pipe_nt(hipStream_t s, const uint16_t* x, const uint16_t* xa, int ca, const uint16_t* w,
+                    const float* bias, void* y, uint16_t* pooled, int cout, Dims d, int relu) {
+  if (size_t(d.n) * d.h * d.w * (CIN - ca) * 2 >= kOutOfRange) return false;
+  if (xa && size_t(d.n) * (d.h / 2) * (d.w / 2) * ca * 2 >= kOutOfRange) return false;
+  static const int per_cu = [] {  // resident blocks per CU
+    int n = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_conv_pipe<NT, F32, CIN, GMAX>, 256, 0) != hipSuccess || n <= 0)
+      n = 1;
+    return n;
+  }();
+  const int nrt = (d.h + kTR - 1) / kTR, nct = (d.w + kTC - 1) / kTC;
+  const int ntiles = d.n * nrt * nct;
+  const int nb = (std::min(ntiles, device_cus() * per_cu) + 7) / 8 * 8;  // a multiple of the 8 XCDs
+  hipLaunchKernelGGL((k_conv_pipe<NT, F32, CIN, GMAX>), dim3(nb, 1), dim3(256), 0, s, x, xa, ca, w, bias, y, pooled, cout,
+                     0, d.n, d.h, d.w, nrt, nct, relu);
+  return true;
+}
+
+template <bool F32>
+bool launch_pipe(hipStream_t s, const uint16_t* x, int cin, const uint16_t* xa, int ca, const uint16_t* w,
+                 const float* bias, void* y, uint16_t* pooled, int cout, Dims d, int relu) {
+  const int key = (cout / 16) * 1000 + cin;
+  if (F32) {
+    if (key == 1032) return launch_pipe_nt<1, true, 32, 1>(s, x, xa, ca, w, bias, y, pooled, cout, d, relu);
+    return false;
+  }
+  switch (key) {
+    case 2016: return launch_pipe_nt<2, false, 16, 1>(s, x, xa, ca, w, bias, y, pooled, cout, d, relu);
+    case 2032: return launch_pipe_nt<2, false, 32, 1>(s, x, xa, ca, w, bias, y, pooled, cout, d, relu);
+    case 2080: return launch_pipe_nt<2, false, 80, 1>(s, x, xa, ca, w, bias, y, pooled, cout, d, relu);
+    default: return false;
+  }
+}
+
 template <bool DGRAD, bool F32>
 void launch_conv(hipStream_t s, const uint16_t* x, const uint16_t* mask, int cin, const uint16_t* xa, int ca,
                  const uint16_t* w, const float* bias, void* y, int cout, Dims d, int relu,
                  uint16_t* pooled = nullptr) {
+  if (!DGRAD && launch_pipe<F32>(s, x, cin, xa, ca, w, bias, y, pooled, cout, d, relu)) return;
   const int frags = cout / 16, full = frags / 4, rest = frags % 4;
   if (full) launch_nt<DGRAD, F32, 4>(s, x, mask, cin, xa, ca, w, bias, y, pooled, cout, 0, full, d, relu);
   const int c0 = full * 64;
