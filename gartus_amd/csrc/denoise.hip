@@ -1149,19 +1149,32 @@ bool launch_pipe(hipStream_t s, const uint16_t* x, int cin, const uint16_t* xa, 
 }
 
 template <bool DGRAD, bool F32>
+void launch_groups(hipStream_t s, int nt, int first_frag, int groups, const uint16_t* x, const uint16_t* mask, int cin,
+                   const uint16_t* xa, int ca, const uint16_t* w, const float* bias, void* y, int cout, Dims d,
+                   int relu, uint16_t* pooled) {
+  const int c0 = first_frag * 16;
+  switch (nt) {
+    case 1: launch_nt<DGRAD, F32, 1>(s, x, mask, cin, xa, ca, w, bias, y, pooled, cout, c0, groups, d, relu); break;
+    case 2: launch_nt<DGRAD, F32, 2>(s, x, mask, cin, xa, ca, w, bias, y, pooled, cout, c0, groups, d, relu); break;
+    case 3: launch_nt<DGRAD, F32, 3>(s, x, mask, cin, xa, ca, w, bias, y, pooled, cout, c0, groups, d, relu); break;
+    default: launch_nt<DGRAD, F32, 4>(s, x, mask, cin, xa, ca, w, bias, y, pooled, cout, c0, groups, d, relu); break;
+  }
+}
+
+// Output channels in groups of NT 16-channel fragments, then one launch for
+// the rest. Levels with few tiles take narrower groups (more blocks, each
+// staging the same halo for fewer channels), so that the launch still has
+// blocks for most CUs.
+template <bool DGRAD, bool F32>
 void launch_conv(hipStream_t s, const uint16_t* x, const uint16_t* mask, int cin, const uint16_t* xa, int ca,
                  const uint16_t* w, const float* bias, void* y, int cout, Dims d, int relu,
                  uint16_t* pooled = nullptr) {
   if (!DGRAD && launch_pipe<F32>(s, x, cin, xa, ca, w, bias, y, pooled, cout, d, relu)) return;
-  const int frags = cout / 16, full = frags / 4, rest = frags % 4;
-  if (full) launch_nt<DGRAD, F32, 4>(s, x, mask, cin, xa, ca, w, bias, y, pooled, cout, 0, full, d, relu);
-  const int c0 = full * 64;
-  switch (rest) {
-    case 1: launch_nt<DGRAD, F32, 1>(s, x, mask, cin, xa, ca, w, bias, y, pooled, cout, c0, 1, d, relu); break;
-    case 2: launch_nt<DGRAD, F32, 2>(s, x, mask, cin, xa, ca, w, bias, y, pooled, cout, c0, 1, d, relu); break;
-    case 3: launch_nt<DGRAD, F32, 3>(s, x, mask, cin, xa, ca, w, bias, y, pooled, cout, c0, 1, d, relu); break;
-    default: break;
-  }
+  const int tiles = d.n * ((d.h + kTR - 1) / kTR) * ((d.w + kTC - 1) / kTC);
+  const int nt = tiles >= 1024 ? 4 : tiles >= 256 ? 2 : 1;
+  const int frags = cout / 16, full = frags / nt, rest = frags % nt;
+  if (full) launch_groups<DGRAD, F32>(s, nt, 0, full, x, mask, cin, xa, ca, w, bias, y, cout, d, relu, pooled);
+  if (rest) launch_groups<DGRAD, F32>(s, rest, full * nt, 1, x, mask, cin, xa, ca, w, bias, y, cout, d, relu, pooled);
 }
 
 // Weight-gradient splits: one per band of kWR rows of one image.
