@@ -1107,12 +1107,12 @@ int device_cus() {
 }
 
 // The persistent forward convolution, for the layer shapes where it is faster
-// than k_conv on one MI355X: 16 or 32 input and output channels (the default
-// network's full-resolution convolutions other than the decoder's
-// concatenation), tensors addressable by 31-bit buffer offsets. False for the
-// others, which take k_conv: with 48 output channels or 80 input channels a
-// block needs more registers or LDS, one block fits a CU, and the layer ran
-// 10-30 % slower than k_conv.
+// than k_conv on one MI355X: the default network's full-resolution layers and
+// the first convolution of the next level (32 -> 48), tensors addressable by
+// 31-bit buffer offsets. These fit two blocks per CU. False for the others,
+// which take k_conv: with 48 output channels and 48 or more input channels
+// the resident weights leave room for one block per CU, and those layers ran
+// 10-40 % slower than k_conv.
 template <int NT, bool F32, int CIN, int GMAX>
 bool launch_pipe_nt(hipStream_t s, const uint16_t* x, const uint16_t* xa, int ca, const uint16_t* w,
                     const float* bias, void* y, uint16_t* pooled, int cout, Dims d, int relu) {
@@ -1144,6 +1144,7 @@ bool launch_pipe(hipStream_t s, const uint16_t* x, int cin, const uint16_t* xa, 
     case 2016: return launch_pipe_nt<2, false, 16, 1>(s, x, xa, ca, w, bias, y, pooled, cout, d, relu);
     case 2032: return launch_pipe_nt<2, false, 32, 1>(s, x, xa, ca, w, bias, y, pooled, cout, d, relu);
     case 2080: return launch_pipe_nt<2, false, 80, 1>(s, x, xa, ca, w, bias, y, pooled, cout, d, relu);
+    case 3032: return launch_pipe_nt<3, false, 32, 1>(s, x, xa, ca, w, bias, y, pooled, cout, d, relu);
     default: return false;
   }
 }
