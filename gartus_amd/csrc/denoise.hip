@@ -272,19 +272,16 @@ __global__ __launch_bounds__(256) void k_conv(const uint16_t* __restrict__ x, co
 // The same tiles and MFMA steps as k_conv, for layers of up to 48 output
 // channels (one group). A block keeps the layer's weights in LDS for its
 // whole life and walks a run of tiles. A tile's input channels run in chunks
-// of up to 32 that never straddle the two parts of a decoder concatenation;
+// of 16*GMAX that never straddle the two parts of a decoder concatenation;
 // a (tile, chunk) pair is one stage. The halo of stage s + 1 is copied from
 // global memory into the second of two LDS buffers by buffer loads that
 // write LDS directly (no registers), while the waves run the MFMA steps of
 // stage s; border pixels are read at an offset past the buffer's range and
 // land as zeros. The halo has no padding: piece q (16 bytes) of pixel p sits
-// at slot p * 2G + (q ^ swz<G>(p)), so 16 consecutive pixels' reads of one
-// piece fall on distinct banks of the 64-bank LDS. The two buffers are
-// separate LDS objects and each step names its pair statically, so the
-// compiler sees that the copy in flight and the reads do not overlap and
-// puts no wait between them. Blocks are dealt round-robin to the 8 XCDs;
-// each XCD's blocks walk one contiguous eighth of the tiles, so tiles that
-// share halo rows are read through the same L2.
+// at slot p * 2G + (q ^ swz<G>(p)), which spreads 16 consecutive pixels'
+// reads of one piece over the banks of the 64-bank LDS. Blocks are dealt
+// round-robin to the 8 XCDs; each XCD's blocks walk one contiguous eighth of
+// the tiles, so tiles that share halo rows are read through the same L2.
 typedef __attribute__((address_space(3))) void lds_void;
 
 constexpr uint32_t kOutOfRange = 0x80000000u;  // a buffer offset past every tensor this kernel reads
