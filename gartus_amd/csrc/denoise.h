@@ -22,6 +22,12 @@ struct Dims {
 // odd last row or column is dropped).
 void conv_fwd(hipStream_t s, const uint16_t* x, int cin, const uint16_t* w, const float* bias, void* y, int cout,
               Dims d, bool relu, bool out_f32, uint16_t* pooled = nullptr);
+// The network output convolution (cin channels -> 16, no ReLU) written straight as radiance
+// expm1(max(z, 0)) of channels 0..2, fp32 [n][h][w][3] for the h x w image inside d; the same
+// values as conv_fwd (out_f32) followed by unpack_output. False, and nothing launched, for shapes
+// the fused kernel does not take.
+bool conv_fwd_radiance(hipStream_t s, const uint16_t* x, int cin, const uint16_t* w, const float* bias, float* out,
+                       int h, int w_img, Dims d);
 // y = act(conv(cat(up2(a), b)) + bias) without materialising the concatenation: a has ca channels
 // at half resolution, b has cb channels at d; the same result as upcat_fwd followed by conv_fwd.
 void conv_fwd_upcat(hipStream_t s, const uint16_t* a, int ca, const uint16_t* b, int cb, const uint16_t* w,

@@ -451,10 +451,12 @@ struct PipeOut {
 
 // Bias and ReLU of the accumulators, staged through buf (bf16, or fp32 for the
 // network output) and read back into out; clears acc for the next tile.
+// With rad (the network output only) lane l keeps instead the radiance
+// expm1(max(z, 0)) of channels 0..2 of its pixel, the wave's row's pixel l.
 template <int NT, bool OUT_F32, int GMAX>
 __device__ __forceinline__ void pipe_finish(uint4* buf, v4f (&acc)[4][NT], const float (&bj)[NT], int relu,
-                                            bool pooled, PipeOut<NT, OUT_F32>& out, int t, int tid, int lane,
-                                            int wave) {
+                                            bool pooled, bool rad, PipeOut<NT, OUT_F32>& out, int t, int tid,
+                                            int lane, int wave) {
   constexpr int RB = pipe_row_bytes<NT, OUT_F32, GMAX>(), GP = pipe_row_pieces<NT, OUT_F32>();
   char* st = reinterpret_cast<char*>(buf);
   lds_barrier();  // every wave is done with the halo in buf
@@ -472,10 +474,16 @@ __device__ __forceinline__ void pipe_finish(uint4* buf, v4f (&acc)[4][NT], const
         acc[f][j][r] = 0.f;
       }
   lds_barrier();
+  if (OUT_F32 && rad) {
+    const float* z = reinterpret_cast<const float*>(st + (wave * 64 + lane) * RB);
+    out.row[0] = u32x4{__float_as_uint(expm1f(fmaxf(z[0], 0.f))), __float_as_uint(expm1f(fmaxf(z[1], 0.f))),
+                       __float_as_uint(expm1f(fmaxf(z[2], 0.f))), 0u};
+  } else {
 #pragma unroll
-  for (int k = 0; k < GP; ++k) {
-    const int i = lane + 64 * k, p = i / GP, g = i % GP;
-    out.row[k] = *reinterpret_cast<const u32x4*>(st + (wave * 64 + p) * RB + 16 * g);
+    for (int k = 0; k < GP; ++k) {
+      const int i = lane + 64 * k, p = i / GP, g = i % GP;
+      out.row[k] = *reinterpret_cast<const u32x4*>(st + (wave * 64 + p) * RB + 16 * g);
+    }
   }
   if (!OUT_F32 && pooled) {  // the staged values are already bf16: the max of them is the pooled value
     constexpr int G8 = NT * 2;
@@ -506,17 +514,27 @@ __device__ __forceinline__ void pipe_finish(uint4* buf, v4f (&acc)[4][NT], const
   out.tile = t;
 }
 
-// The global stores of a finished tile held in out.
+// The global stores of a finished tile held in out. With rad: radiance
+// [n][rh][rw][3] of the pixels inside rh x rw (the caller's image in the
+// padded tensor), instead of y.
 template <int NT, bool OUT_F32>
-__device__ __forceinline__ void pipe_store(PipeOut<NT, OUT_F32>& out, void* y, uint16_t* pooled, int cout, int co0,
-                                           int H, int W, int nrt, int nct, int tid, int lane, int wave) {
+__device__ __forceinline__ void pipe_store(PipeOut<NT, OUT_F32>& out, void* y, uint16_t* pooled, float* rad, int rh,
+                                           int rw, int cout, int co0, int H, int W, int nrt, int nct, int tid,
+                                           int lane, int wave) {
   constexpr int GP = pipe_row_pieces<NT, OUT_F32>();
   const int t = out.tile;
   out.tile = -1;
   const int ct = t % nct, rt = (t / nct) % nrt, n = t / (nct * nrt);
   const int h0 = rt * kTR, w0 = ct * kTC;
   const int row = h0 + wave;
-  if (row < H) {
+  if (OUT_F32 && rad) {
+    if (row < rh && w0 + lane < rw) {
+      float* o = rad + ((size_t(n) * rh + row) * rw + w0 + lane) * 3;
+      o[0] = __uint_as_float(out.row[0][0]);
+      o[1] = __uint_as_float(out.row[0][1]);
+      o[2] = __uint_as_float(out.row[0][2]);
+    }
+  } else if (row < H) {
     const size_t pix0 = (size_t(n) * H + row) * W;
 #pragma unroll
     for (int k = 0; k < GP; ++k) {
@@ -548,7 +566,8 @@ __global__ __launch_bounds__(256) void k_conv_pipe(const uint16_t* __restrict__ 
                                                    const uint16_t* __restrict__ xa, int ca,
                                                    const uint16_t* __restrict__ wgt, const float* __restrict__ bias,
                                                    void* __restrict__ y, uint16_t* __restrict__ pooled, int cout,
-                                                   int co_first, int N, int H, int W, int nrt, int nct, int relu) {
+                                                   int co_first, int N, int H, int W, int nrt, int nct, int relu,
+                                                   float* __restrict__ rad, int rh, int rw) {
   __shared__ uint4 bufA[pipe_buf_slots<NT, OUT_F32, GMAX>()];
   __shared__ uint4 bufB[pipe_buf_slots<NT, OUT_F32, GMAX>()];
   __shared__ uint4 wts4[9 * NT * 16 * CIN / 8];
@@ -606,22 +625,24 @@ __global__ __launch_bounds__(256) void k_conv_pipe(const uint16_t* __restrict__ 
     __builtin_amdgcn_s_waitcnt(kWaitVm0);  // this wave's copies into cur (and its last stores) are done
     lds_barrier();  // ... and every wave's; nxt was last read in the previous step
     if (s + 1 < S) copy(nxt, s + 1);
-    if (out.tile >= 0) pipe_store<NT, OUT_F32>(out, y, pooled, cout, co0, H, W, nrt, nct, tid, lane, wave);
+    if (out.tile >= 0)
+      pipe_store<NT, OUT_F32>(out, y, pooled, rad, rh, rw, cout, co0, H, W, nrt, nct, tid, lane, wave);
     const int k = s / nch, ch = s - k * nch;
     const PipeChunk c = pipe_chunk<GMAX>(ch, cin, ca);
     const int wb = 9 * NT * 16 * (c.c0 / 8);
     if (GMAX == 2 && c.g == 2) pipe_chunk_mfma<NT, 2>(cur, wts4, wb, lane, wave, acc);
     else pipe_chunk_mfma<NT, 1>(cur, wts4, wb, lane, wave, acc);
     if (ch == nch - 1)
-      pipe_finish<NT, OUT_F32, GMAX>(cur, acc, bj, relu, pooled != nullptr, out, t_begin + slot0 + k * per, tid, lane,
-                               wave);
+      pipe_finish<NT, OUT_F32, GMAX>(cur, acc, bj, relu, pooled != nullptr, rad != nullptr, out,
+                                     t_begin + slot0 + k * per, tid, lane, wave);
   };
   copy(bufA, 0);
   for (int s = 0; s < S; s += 2) {
     step(bufA, bufB, s);
     if (s + 1 < S) step(bufB, bufA, s + 1);
   }
-  if (out.tile >= 0) pipe_store<NT, OUT_F32>(out, y, pooled, cout, co0, H, W, nrt, nct, tid, lane, wave);
+  if (out.tile >= 0)
+    pipe_store<NT, OUT_F32>(out, y, pooled, rad, rh, rw, cout, co0, H, W, nrt, nct, tid, lane, wave);
 }
 
 // ---- convolution, weight gradient: LDS tiles and 16x16x32 MFMA -------------
@@ -1112,7 +1133,8 @@ int device_cus() {
 // 10-40 % slower than k_conv.
 template <int NT, bool F32, int CIN, int GMAX>
 bool launch_pipe_nt(hipStream_t s, const uint16_t* x, const uint16_t* xa, int ca, const uint16_t* w,
-                    const float* bias, void* y, uint16_t* pooled, int cout, Dims d, int relu) {
+                    const float* bias, void* y, uint16_t* pooled, int cout, Dims d, int relu, float* rad = nullptr,
+                    int rh = 0, int rw = 0) {
   if (size_t(d.n) * d.h * d.w * (CIN - ca) * 2 >= kOutOfRange) return false;
   if (xa && size_t(d.n) * (d.h / 2) * (d.w / 2) * ca * 2 >= kOutOfRange) return false;
   static const int per_cu = [] {  // resident blocks per CU
@@ -1125,7 +1147,7 @@ bool launch_pipe_nt(hipStream_t s, const uint16_t* x, const uint16_t* xa, int ca
   const int ntiles = d.n * nrt * nct;
   const int nb = (std::min(ntiles, device_cus() * per_cu) + 7) / 8 * 8;  // a multiple of the 8 XCDs
   hipLaunchKernelGGL((k_conv_pipe<NT, F32, CIN, GMAX>), dim3(nb, 1), dim3(256), 0, s, x, xa, ca, w, bias, y, pooled, cout,
-                     0, d.n, d.h, d.w, nrt, nct, relu);
+                     0, d.n, d.h, d.w, nrt, nct, relu, rad, rh, rw);
   return true;
 }
 
@@ -1198,6 +1220,12 @@ void conv_fwd(hipStream_t s, const uint16_t* x, int cin, const uint16_t* w, cons
               Dims d, bool relu, bool out_f32, uint16_t* pooled) {
   if (out_f32) launch_conv<false, true>(s, x, nullptr, cin, nullptr, 0, w, bias, y, cout, d, relu ? 1 : 0);
   else launch_conv<false, false>(s, x, nullptr, cin, nullptr, 0, w, bias, y, cout, d, relu ? 1 : 0, pooled);
+}
+
+bool conv_fwd_radiance(hipStream_t s, const uint16_t* x, int cin, const uint16_t* w, const float* bias, float* out,
+                       int h, int w_img, Dims d) {
+  if (cin != 32) return false;
+  return launch_pipe_nt<1, true, 32, 1>(s, x, nullptr, 0, w, bias, nullptr, nullptr, 16, d, 0, out, h, w_img);
 }
 
 void conv_fwd_upcat(hipStream_t s, const uint16_t* a, int ca, const uint16_t* b, int cb, const uint16_t* w,

@@ -190,8 +190,9 @@ int prepare(gt_denoiser* dn, uint32_t n, uint32_t h, uint32_t w, const float* b,
 
 // keep_cat: also materialise each decoder level's concatenation (the weight
 // gradient of its first convolution reads it); inference reads the two
-// halves in place.
-void forward(gt_denoiser* dn, bool keep_cat) {
+// halves in place. With radiance, the output convolution writes the h x w
+// radiance there directly when it can (true) instead of z (false).
+bool forward(gt_denoiser* dn, bool keep_cat, float* radiance = nullptr, int h = 0, int w = 0) {
   hipStream_t s = dn->stream;
   const int L = dn->levels;
   for (int l = 0; l < L; ++l) {
@@ -221,8 +222,12 @@ void forward(gt_denoiser* dn, bool keep_cat) {
     gdn::conv_fwd(s, v.dec_a.p, c1.cin, c1.w16.p, dn->params.p + c1.b_off, v.dec.p, c1.cout, d, true, false);
   }
   Conv& co = dn->out_conv();
+  if (radiance && gdn::conv_fwd_radiance(s, dn->lv[0]->dec.p, co.cin, co.w16.p, dn->params.p + co.b_off, radiance, h,
+                                         w, dn->shape))
+    return true;
   gdn::conv_fwd(s, dn->lv[0]->dec.p, co.cin, co.w16.p, dn->params.p + co.b_off, dn->z.p, co.cout, dn->shape, false,
                 true);
+  return false;
 }
 
 // Gradients of every parameter into dn->grads, from dn->dz.
@@ -396,8 +401,8 @@ int gt_denoiser_infer(gt_denoiser* dn, uint32_t n, uint32_t h, uint32_t w, const
   if (!out) return bad(GT_EINVAL, "null output");
   int rc = prepare(dn, n, h, w, b, a, nr);
   if (rc != GT_OK) return rc;
-  forward(dn, false);
-  gdn::unpack_output(dn->stream, dn->z.p, out, int(h), int(w), dn->shape);
+  if (!forward(dn, false, out, int(h), int(w)))
+    gdn::unpack_output(dn->stream, dn->z.p, out, int(h), int(w), dn->shape);
   GT_CHECK_HIP(hipGetLastError());
   GT_CHECK_HIP(hipStreamSynchronize(dn->stream));
   return GT_OK;
