@@ -587,6 +587,80 @@ __device__ __forceinline__ bool trace2(const DScene& sc, f3 o, f3 d, float time,
   if (STATS) { stats->node_hits += 1; stats->max_stack = max(stats->max_stack, 1u); }
   uint32_t cur = sc.root_ref;
   st.n = 0;
+  if (ANY && !STATS) {
+    // Any-hit rays walk while-while with speculation (Aila and Laine,
+    // "Understanding the efficiency of ray traversal on GPUs", HPG 2009): a
+    // ray that reaches a leaf parks it and walks on through inner nodes until
+    // every ray of the wave has a leaf parked, then the wave tests leaves
+    // together. An any-hit query never narrows its interval before it
+    // returns, so it visits the same nodes in either loop and its answer is
+    // the same. (For closest hits the walk would cull with a stale closest hit
+    // and could reach leaves that the reference's pruning skips.)
+    constexpr uint32_t kNone = 0xffffffffu;  // no node: has the leaf bit, so it ends the inner loop
+    uint32_t leaf = kNone;
+    if (cur & GT_LEAF_BIT) { leaf = cur; cur = kNone; }
+    auto next = [&]() -> uint32_t {  // pop, skipping entries that cannot beat the closest hit
+      while (st.n > 0) {
+        const uint2 e = st.pop();
+        if (__uint_as_float(e.y) < closest) return e.x;
+      }
+      return kNone;
+    };
+    while (cur != kNone || leaf != kNone) {
+      while (!(cur & GT_LEAF_BIT)) {
+        const float4* nd = reinterpret_cast<const float4*>(sc.nodes2 + cur);
+        const float4 a = nd[0], b = nd[1], c = nd[2];
+        const uint4 refs = *reinterpret_cast<const uint4*>(nd + 3);
+        float le, re;
+        const bool lh = slab(rp, a.x, a.y, a.z, a.w, c.x, c.y, tmin, closest, &le);
+        const bool rh = slab(rp, b.x, b.y, b.z, b.w, c.z, c.w, tmin, closest, &re);
+        if (lh && rh) {
+          const bool right_first = re < le;
+          st.push(right_first ? refs.x : refs.y, right_first ? le : re);
+          cur = right_first ? refs.y : refs.x;
+        } else if (lh) {
+          cur = refs.x;
+        } else if (rh) {
+          cur = refs.y;
+        } else {
+          cur = next();
+        }
+        if (cur != kNone && (cur & GT_LEAF_BIT) && leaf == kNone) {
+          leaf = cur;
+          cur = next();
+        }
+        if (!__any(leaf == kNone)) break;  // every ray still walking has a leaf parked
+      }
+      while (leaf != kNone) {
+        for (uint32_t k = leaf & ~GT_LEAF_BIT;; ++k) {
+          PrimHit h;
+          uint32_t meta;
+          if (prim_test(sc.prims, k, o, d, time, tmin, closest, &h, &meta, k == self)) {
+            closest = h.t;
+            best = k;
+            best_meta = meta;
+            bh = h;
+            any = true;
+            if (ANY) return true;
+          }
+          if (meta & GT_LAST_BIT) break;
+        }
+        if (cur != kNone && (cur & GT_LEAF_BIT)) {
+          leaf = cur;
+          cur = next();
+        } else {
+          leaf = kNone;
+        }
+      }
+    }
+    if (any) {
+      *t_out = closest;
+      *prim_out = best;
+      if (hit_out) *hit_out = bh;
+      if (meta_out) *meta_out = best_meta;
+    }
+    return any;
+  }
   for (;;) {
     if (cur & GT_LEAF_BIT) {
       uint32_t k = cur & ~GT_LEAF_BIT;
