@@ -299,12 +299,21 @@ __device__ __forceinline__ void lds_barrier() {
 }
 
 // The halo buffers hold a chunk of up to 16*GMAX channels in whole 64-slot
-// copies, and the output tile, which is staged in the buffer it was computed
-// from: 256 pixels x NT*16 channels, bf16 (fp32 for the network output), rows
-// padded by 16 bytes when the padded tile still fits.
+// copies. The output tile is staged in the buffer it was computed from, in
+// passes of at most 32 channels (bf16; 16 channels fp32 for the network
+// output): 256 pixels x the pass's channels, rows padded by 16 bytes when the
+// padded pass still fits.
 template <int NT, bool OUT_F32>
-constexpr int pipe_raw_row_bytes() {
-  return OUT_F32 ? NT * 16 * 4 : NT * 16 * 2;
+constexpr int pipe_pass_channels() {
+  return OUT_F32 ? 16 : std::min(NT * 16, 32);
+}
+template <int NT, bool OUT_F32>
+constexpr int pipe_passes() {
+  return (NT * 16 + pipe_pass_channels<NT, OUT_F32>() - 1) / pipe_pass_channels<NT, OUT_F32>();
+}
+template <int NT, bool OUT_F32>
+constexpr int pipe_raw_row_bytes() {  // one pass
+  return pipe_pass_channels<NT, OUT_F32>() * (OUT_F32 ? 4 : 2);
 }
 template <int NT, bool OUT_F32, int GMAX>
 constexpr int pipe_buf_slots() {
@@ -450,65 +459,75 @@ struct PipeOut {
 };
 
 // Bias and ReLU of the accumulators, staged through buf (bf16, or fp32 for the
-// network output) and read back into out; clears acc for the next tile.
-// With rad (the network output only) lane l keeps instead the radiance
-// expm1(max(z, 0)) of channels 0..2 of its pixel, the wave's row's pixel l.
+// network output) pass by pass and read back into out; clears acc for the
+// next tile. With rad (the network output only) lane l keeps instead the
+// radiance expm1(max(z, 0)) of channels 0..2 of its pixel, the wave's row's
+// pixel l.
 template <int NT, bool OUT_F32, int GMAX>
 __device__ __forceinline__ void pipe_finish(uint4* buf, v4f (&acc)[4][NT], const float (&bj)[NT], int relu,
                                             bool pooled, bool rad, PipeOut<NT, OUT_F32>& out, int t, int tid,
                                             int lane, int wave) {
   constexpr int RB = pipe_row_bytes<NT, OUT_F32, GMAX>(), GP = pipe_row_pieces<NT, OUT_F32>();
+  constexpr int PC = pipe_pass_channels<NT, OUT_F32>(), PJ = PC / 16;  // channels, fragments per pass
+  constexpr int CP = OUT_F32 ? 4 : 8;                                  // channels per 16-byte piece
   char* st = reinterpret_cast<char*>(buf);
-  lds_barrier();  // every wave is done with the halo in buf
 #pragma unroll
-  for (int j = 0; j < NT; ++j)
+  for (int pass = 0; pass < pipe_passes<NT, OUT_F32>(); ++pass) {
+    lds_barrier();  // every wave is done with buf: the halo, or the previous pass
 #pragma unroll
-    for (int f = 0; f < 4; ++f)
+    for (int jj = 0; jj < PJ; ++jj) {
+      const int j = pass * PJ + jj;
+      if (j >= NT) break;
 #pragma unroll
-      for (int r = 0; r < 4; ++r) {
-        float v = acc[f][j][r] + bj[j];
-        if (relu) v = fmaxf(v, 0.f);
-        const int px = wave * 64 + f * 16 + (lane >> 4) * 4 + r, c = j * 16 + (lane & 15);
-        if (OUT_F32) *reinterpret_cast<float*>(st + px * RB + 4 * c) = v;
-        else *reinterpret_cast<uint16_t*>(st + px * RB + 2 * c) = f2bf(v);
-        acc[f][j][r] = 0.f;
-      }
-  lds_barrier();
-  if (OUT_F32 && rad) {
-    const float* z = reinterpret_cast<const float*>(st + (wave * 64 + lane) * RB);
-    out.row[0] = u32x4{__float_as_uint(expm1f(fmaxf(z[0], 0.f))), __float_as_uint(expm1f(fmaxf(z[1], 0.f))),
-                       __float_as_uint(expm1f(fmaxf(z[2], 0.f))), 0u};
-  } else {
+      for (int f = 0; f < 4; ++f)
 #pragma unroll
-    for (int k = 0; k < GP; ++k) {
-      const int i = lane + 64 * k, p = i / GP, g = i % GP;
-      out.row[k] = *reinterpret_cast<const u32x4*>(st + (wave * 64 + p) * RB + 16 * g);
-    }
-  }
-  if (!OUT_F32 && pooled) {  // the staged values are already bf16: the max of them is the pooled value
-    constexpr int G8 = NT * 2;
-#pragma unroll
-    for (int k = 0; k < pipe_pool_per_thread(NT); ++k) {
-      const int i = tid + 256 * k;
-      if (i >= 2 * 32 * G8) break;
-      const int g = i % G8, pc = (i / G8) % 32, pr = i / (G8 * 32);
-      const char* a = st + ((2 * pr) * 64 + 2 * pc) * RB + 16 * g;
-      const u32x4 v[4] = {*reinterpret_cast<const u32x4*>(a), *reinterpret_cast<const u32x4*>(a + RB),
-                          *reinterpret_cast<const u32x4*>(a + 64 * RB),
-                          *reinterpret_cast<const u32x4*>(a + 65 * RB)};
-      u32x4 o;
-#pragma unroll
-      for (int e = 0; e < 4; ++e) {
-        const uint32_t w[4] = {v[0][e], v[1][e], v[2][e], v[3][e]};
-        float lo = bf2f(uint16_t(w[0])), hi = bf2f(uint16_t(w[0] >> 16));
-#pragma unroll
-        for (int m = 1; m < 4; ++m) {
-          lo = fmaxf(lo, bf2f(uint16_t(w[m])));
-          hi = fmaxf(hi, bf2f(uint16_t(w[m] >> 16)));
+        for (int r = 0; r < 4; ++r) {
+          float v = acc[f][j][r] + bj[j];
+          if (relu) v = fmaxf(v, 0.f);
+          const int px = wave * 64 + f * 16 + (lane >> 4) * 4 + r, c = jj * 16 + (lane & 15);
+          if (OUT_F32) *reinterpret_cast<float*>(st + px * RB + 4 * c) = v;
+          else *reinterpret_cast<uint16_t*>(st + px * RB + 2 * c) = f2bf(v);
+          acc[f][j][r] = 0.f;
         }
-        o[e] = uint32_t(f2bf(lo)) | (uint32_t(f2bf(hi)) << 16);
+    }
+    lds_barrier();
+    const int g0 = pass * PC / CP, g1 = std::min(GP, (pass + 1) * PC / CP);  // this pass's pieces
+    if (OUT_F32 && rad) {
+      const float* z = reinterpret_cast<const float*>(st + (wave * 64 + lane) * RB);
+      out.row[0] = u32x4{__float_as_uint(expm1f(fmaxf(z[0], 0.f))), __float_as_uint(expm1f(fmaxf(z[1], 0.f))),
+                         __float_as_uint(expm1f(fmaxf(z[2], 0.f))), 0u};
+    } else {
+#pragma unroll
+      for (int k = 0; k < GP; ++k) {
+        const int i = lane + 64 * k, p = i / GP, g = i % GP;
+        if (g >= g0 && g < g1) out.row[k] = *reinterpret_cast<const u32x4*>(st + (wave * 64 + p) * RB + 16 * (g - g0));
       }
-      out.pool[k] = o;
+    }
+    if (!OUT_F32 && pooled) {  // the staged values are already bf16: the max of them is the pooled value
+      constexpr int G8 = NT * 2;
+#pragma unroll
+      for (int k = 0; k < pipe_pool_per_thread(NT); ++k) {
+        const int i = tid + 256 * k;
+        const int g = i % G8, pc = (i / G8) % 32, pr = i / (G8 * 32);
+        if (i >= 2 * 32 * G8 || g < g0 || g >= g1) continue;
+        const char* a = st + ((2 * pr) * 64 + 2 * pc) * RB + 16 * (g - g0);
+        const u32x4 v[4] = {*reinterpret_cast<const u32x4*>(a), *reinterpret_cast<const u32x4*>(a + RB),
+                            *reinterpret_cast<const u32x4*>(a + 64 * RB),
+                            *reinterpret_cast<const u32x4*>(a + 65 * RB)};
+        u32x4 o;
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+          const uint32_t w[4] = {v[0][e], v[1][e], v[2][e], v[3][e]};
+          float lo = bf2f(uint16_t(w[0])), hi = bf2f(uint16_t(w[0] >> 16));
+#pragma unroll
+          for (int m = 1; m < 4; ++m) {
+            lo = fmaxf(lo, bf2f(uint16_t(w[m])));
+            hi = fmaxf(hi, bf2f(uint16_t(w[m] >> 16)));
+          }
+          o[e] = uint32_t(f2bf(lo)) | (uint32_t(f2bf(hi)) << 16);
+        }
+        out.pool[k] = o;
+      }
     }
   }
   out.tile = t;
@@ -562,7 +581,7 @@ __device__ __forceinline__ void pipe_store(PipeOut<NT, OUT_F32>& out, void* y, u
 }
 
 template <int NT, bool OUT_F32, int CIN, int GMAX>
-__global__ __launch_bounds__(256) void k_conv_pipe(const uint16_t* __restrict__ x,
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k_conv_pipe(const uint16_t* __restrict__ x,
                                                    const uint16_t* __restrict__ xa, int ca,
                                                    const uint16_t* __restrict__ wgt, const float* __restrict__ bias,
                                                    void* __restrict__ y, uint16_t* __restrict__ pooled, int cout,
@@ -1126,11 +1145,10 @@ int device_cus() {
 
 // The persistent forward convolution, for the layer shapes where it is faster
 // than k_conv on one MI355X: the default network's full-resolution layers and
-// the first convolution of the next level (32 -> 48), tensors addressable by
-// 31-bit buffer offsets. These fit two blocks per CU. False for the others,
-// which take k_conv: with 48 output channels and 48 or more input channels
-// the resident weights leave room for one block per CU, and those layers ran
-// 10-40 % slower than k_conv.
+// level 1's 32 -> 48 and 48 -> 48, tensors addressable by 31-bit buffer
+// offsets. These fit two blocks per CU. False for the others, which take
+// k_conv: the decoder's 64 + 48 -> 48 keeps 97 KB of weights resident, one
+// block fits a CU, and it ran 40 % slower than k_conv.
 template <int NT, bool F32, int CIN, int GMAX>
 bool launch_pipe_nt(hipStream_t s, const uint16_t* x, const uint16_t* xa, int ca, const uint16_t* w,
                     const float* bias, void* y, uint16_t* pooled, int cout, Dims d, int relu, float* rad = nullptr,
@@ -1164,6 +1182,7 @@ bool launch_pipe(hipStream_t s, const uint16_t* x, int cin, const uint16_t* xa, 
     case 2032: return launch_pipe_nt<2, false, 32, 1>(s, x, xa, ca, w, bias, y, pooled, cout, d, relu);
     case 2080: return launch_pipe_nt<2, false, 80, 1>(s, x, xa, ca, w, bias, y, pooled, cout, d, relu);
     case 3032: return launch_pipe_nt<3, false, 32, 1>(s, x, xa, ca, w, bias, y, pooled, cout, d, relu);
+    case 3048: return launch_pipe_nt<3, false, 48, 1>(s, x, xa, ca, w, bias, y, pooled, cout, d, relu);
     default: return false;
   }
 }
