@@ -67,10 +67,19 @@ void loss(hipStream_t s, int kind, const float* z, const float* target, int h, i
 // radiance = expm1(max(z, 0)) for the valid region, fp32 [n][h][w][3].
 void unpack_output(hipStream_t s, const float* z, float* out, int h, int w, Dims dp);
 
-// Adam on fp32 master weights; also writes the bf16 copies (w16 [cout][9][cin] and wt16
-// [cin][9][cout]) when w16 != null.
-void adam(hipStream_t s, float* p, const float* g, float* m, float* v, size_t n, float lr, float b1, float b2,
-          float eps, float c1, float c2, uint16_t* w16, uint16_t* wt16, int cin, int cout);
+// A convolution's weights [w_off, w_off + cout*9*cin) in the parameter vector, with their
+// bf16 copies.
+struct AdamSeg {
+  size_t w_off;
+  int cin, cout;
+  uint16_t* w16;
+  uint16_t* wt16;
+};
+// Adam on fp32 master weights, over the whole parameter vector p[0, n) in one launch; the
+// weights of each of the nseg segments (segs in device memory) also get their bf16 copies
+// (w16 [cout][9][cin] and wt16 [cin][9][cout]).
+void adam_all(hipStream_t s, float* p, const float* g, float* m, float* v, size_t n, float lr, float b1, float b2,
+              float eps, float c1, float c2, const AdamSeg* segs, int nseg);
 // bf16 copies of fp32 weights (after loading parameters).
 void to_bf16(hipStream_t s, const float* p, uint16_t* w16, uint16_t* wt16, int cin, int cout);
 

@@ -1080,9 +1080,9 @@ __global__ void k_unpack(const float* __restrict__ z, float* __restrict__ out, i
 }
 
 // ---- optimiser ---------------------------------------------------------------
-__global__ void k_adam(float* __restrict__ p, const float* __restrict__ g, float* __restrict__ m,
-                       float* __restrict__ v, size_t n, float lr, float b1, float b2, float eps, float c1, float c2,
-                       uint16_t* __restrict__ w16, uint16_t* __restrict__ wt16, int cin, int cout) {
+__global__ void k_adam_all(float* __restrict__ p, const float* __restrict__ g, float* __restrict__ m,
+                           float* __restrict__ v, size_t n, float lr, float b1, float b2, float eps, float c1, float c2,
+                           const AdamSeg* __restrict__ segs, int nseg) {
   for (size_t i = blockIdx.x * size_t(blockDim.x) + threadIdx.x; i < n; i += size_t(gridDim.x) * blockDim.x) {
     const float gi = g[i];
     const float mi = b1 * m[i] + (1.f - b1) * gi;
@@ -1091,11 +1091,16 @@ __global__ void k_adam(float* __restrict__ p, const float* __restrict__ g, float
     v[i] = vi;
     const float pi = p[i] - lr * (mi / c1) / (sqrtf(vi / c2) + eps);
     p[i] = pi;
-    if (w16) {
+    for (int k = 0; k < nseg; ++k) {  // the segment holding i, if any (biases have no copies)
+      const AdamSeg sg = segs[k];
+      const size_t cin = size_t(sg.cin), cout = size_t(sg.cout);
+      if (i < sg.w_off || i - sg.w_off >= cout * 9 * cin) continue;
+      const size_t j = i - sg.w_off;
       const uint16_t b = f2bf(pi);
-      w16[i] = b;
-      const size_t ci = i % cin, t = (i / cin) % 9, co = i / (size_t(cin) * 9);
-      wt16[(ci * 9 + t) * cout + co] = b;
+      sg.w16[j] = b;
+      const size_t ci = j % cin, t = (j / cin) % 9, co = j / (cin * 9);
+      sg.wt16[(ci * 9 + t) * cout + co] = b;
+      break;
     }
   }
 }
@@ -1328,10 +1333,10 @@ void unpack_output(hipStream_t s, const float* z, float* out, int h, int w, Dims
   hipLaunchKernelGGL(k_unpack, dim3(grid_for(n)), dim3(256), 0, s, z, out, h, w, dp.h, dp.w, n);
 }
 
-void adam(hipStream_t s, float* p, const float* g, float* m, float* v, size_t n, float lr, float b1, float b2,
-          float eps, float c1, float c2, uint16_t* w16, uint16_t* wt16, int cin, int cout) {
-  hipLaunchKernelGGL(k_adam, dim3(grid_for(n)), dim3(256), 0, s, p, g, m, v, n, lr, b1, b2, eps, c1, c2, w16, wt16,
-                     cin, cout);
+void adam_all(hipStream_t s, float* p, const float* g, float* m, float* v, size_t n, float lr, float b1, float b2,
+              float eps, float c1, float c2, const AdamSeg* segs, int nseg) {
+  hipLaunchKernelGGL(k_adam_all, dim3(grid_for(n)), dim3(256), 0, s, p, g, m, v, n, lr, b1, b2, eps, c1, c2, segs,
+                     nseg);
 }
 
 void to_bf16(hipStream_t s, const float* p, uint16_t* w16, uint16_t* wt16, int cin, int cout) {

@@ -73,6 +73,7 @@ struct gt_denoiser {
   std::vector<std::unique_ptr<Conv>> convs;  // enc l: 2l, 2l+1; dec l: 2L + 2(L-2-l), +1; output: last
   size_t nparams = 0;
   Buf<float> params, grads, adam_m, adam_v;
+  Buf<gdn::AdamSeg> adam_segs;  // every convolution's weights and bf16 copies, for the one Adam launch
   uint64_t step = 0;
   // shape-dependent buffers
   gdn::Dims shape{0, 0, 0};  // padded level-0 dims the buffers were sized for
@@ -356,10 +357,14 @@ int gt_denoiser_create(gt_context* ctx, const gt_denoiser_config* cfg, gt_denois
   GT_CHECK_HIP(dn->adam_v.ensure(off));
   GT_CHECK_HIP(hipMemsetAsync(dn->adam_m.p, 0, off * sizeof(float), dn->stream));
   GT_CHECK_HIP(hipMemsetAsync(dn->adam_v.p, 0, off * sizeof(float), dn->stream));
+  std::vector<gdn::AdamSeg> segs;
   for (auto& c : dn->convs) {
     GT_CHECK_HIP(c->w16.ensure(c->nw()));
     GT_CHECK_HIP(c->wt16.ensure(c->nw()));
+    segs.push_back(gdn::AdamSeg{c->w_off, c->cin, c->cout, c->w16.p, c->wt16.p});
   }
+  GT_CHECK_HIP(dn->adam_segs.ensure(segs.size()));
+  GT_CHECK_HIP(hipMemcpy(dn->adam_segs.p, segs.data(), segs.size() * sizeof(gdn::AdamSeg), hipMemcpyHostToDevice));
   std::vector<float> host;
   init_params(dn.get(), host, cfg->seed);
   int rc = upload_params(dn.get(), host.data());
@@ -437,14 +442,8 @@ int gt_denoiser_train_step(gt_denoiser* dn, uint32_t n, uint32_t h, uint32_t w, 
   dn->step += 1;
   const float c1 = float(1.0 - std::pow(double(kBeta1), double(dn->step)));
   const float c2 = float(1.0 - std::pow(double(kBeta2), double(dn->step)));
-  for (auto& c : dn->convs) {
-    gdn::adam(dn->stream, dn->params.p + c->w_off, dn->grads.p + c->w_off, dn->adam_m.p + c->w_off,
-              dn->adam_v.p + c->w_off, c->nw(), lr, kBeta1, kBeta2, kAdamEps, c1, c2, c->w16.p, c->wt16.p, c->cin,
-              c->cout);
-    gdn::adam(dn->stream, dn->params.p + c->b_off, dn->grads.p + c->b_off, dn->adam_m.p + c->b_off,
-              dn->adam_v.p + c->b_off, size_t(c->cout), lr, kBeta1, kBeta2, kAdamEps, c1, c2, nullptr, nullptr, 0,
-              0);
-  }
+  gdn::adam_all(dn->stream, dn->params.p, dn->grads.p, dn->adam_m.p, dn->adam_v.p, dn->nparams, lr, kBeta1, kBeta2,
+                kAdamEps, c1, c2, dn->adam_segs.p, int(dn->convs.size()));
   GT_CHECK_HIP(hipGetLastError());
   GT_CHECK_HIP(hipStreamSynchronize(dn->stream));
   return GT_OK;
