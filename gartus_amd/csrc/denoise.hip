@@ -696,16 +696,16 @@ __device__ __forceinline__ uint4 shift_up(uint4 lo, uint4 hi) {  // elements 1..
 }
 
 template <int NCO, int NCI>
-__global__ __launch_bounds__(256) void k_wgrad(const uint16_t* __restrict__ x, int cin,
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k_wgrad(const uint16_t* __restrict__ x, int cin,
                                                const uint16_t* __restrict__ dy, const uint16_t* __restrict__ mask,
-                                               int cout, int H, int W, int nbands, int ncog,
+                                               int cout, int H, int W, int nbands, int ncog, int ci_first,
                                                float* __restrict__ partial) {
   extern __shared__ uint4 wlds4[];
   uint16_t* dT = reinterpret_cast<uint16_t*>(wlds4);
   uint16_t* xT = dT + NCO * 16 * kSD;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int n = blockIdx.x / nbands, r0 = (blockIdx.x % nbands) * kWR;
-  const int co0 = (blockIdx.y % ncog) * NCO * 16, ci0 = (blockIdx.y / ncog) * NCI * 16;
+  const int co0 = (blockIdx.y % ncog) * NCO * 16, ci0 = ci_first + (blockIdx.y / ncog) * NCI * 16;
   const bool bias = ci0 == 0;  // uniform over the block
   const bool row_in = r0 + wave < H;
   v4f acc[9][NCO][NCI], bacc[NCO];
@@ -1225,17 +1225,18 @@ void launch_conv(hipStream_t s, const uint16_t* x, const uint16_t* mask, int cin
 int wgrad_splits(Dims d) { return d.n * ((d.h + kWR - 1) / kWR); }
 
 template <int NCO, int NCI>
+// Input channels [ci_first, ci_first + ncig * NCI * 16) of the weight gradient.
 void launch_wgrad(hipStream_t s, const uint16_t* x, int cin, const uint16_t* dy, const uint16_t* mask, int cout,
-                  Dims d, float* partial) {
+                  Dims d, float* partial, int ci_first, int ncig) {
   constexpr size_t lds = wgrad_lds_bytes<NCO, NCI>();
   static const hipError_t attr =  // once per instantiation: allow more than 64 KiB of LDS
       hipFuncSetAttribute(reinterpret_cast<const void*>(&k_wgrad<NCO, NCI>),
                           hipFuncAttributeMaxDynamicSharedMemorySize, int(lds));
   (void)attr;
-  const int ncog = cout / (NCO * 16), ncig = cin / (NCI * 16);
+  const int ncog = cout / (NCO * 16);
   const int nbands = (d.h + kWR - 1) / kWR;
   hipLaunchKernelGGL((k_wgrad<NCO, NCI>), dim3(d.n * nbands, ncog * ncig), dim3(256), lds, s, x, cin, dy, mask, cout,
-                     d.h, d.w, nbands, ncog, partial);
+                     d.h, d.w, nbands, ncog, ci_first, partial);
 }
 
 }  // namespace
@@ -1273,10 +1274,17 @@ void conv_wgrad(hipStream_t s, const uint16_t* x, int cin, const uint16_t* dy, c
   const int splits = wgrad_splits(d);
   const size_t nw = size_t(cout) * 9 * cin;
   if (scratch_floats < size_t(splits) * (nw + cout)) return;  // sized by wgrad_scratch_floats
-  // two output-channel fragments per block when the layer has an even number of them, one input
-  // fragment: 2 x 2 needs 400 registers (one wave per SIMD) and was 5 % slower on one MI355X
-  if ((cout / 16) % 2 == 0) launch_wgrad<2, 1>(s, x, cin, dy, y, cout, d, scratch);
-  else launch_wgrad<1, 1>(s, x, cin, dy, y, cout, d, scratch);
+  // Two output- and two input-channel fragments per block where the layer has them, so that
+  // fewer blocks stage the same dPre and x tiles; an odd last input fragment takes a launch of its
+  // own (it writes its own columns of the same partial rows).
+  const int pairs = cin / 32, odd = (cin / 16) % 2;
+  if ((cout / 16) % 2 == 0) {
+    if (pairs) launch_wgrad<2, 2>(s, x, cin, dy, y, cout, d, scratch, 0, pairs);
+    if (odd) launch_wgrad<2, 1>(s, x, cin, dy, y, cout, d, scratch, 32 * pairs, 1);
+  } else {
+    if (pairs) launch_wgrad<1, 2>(s, x, cin, dy, y, cout, d, scratch, 0, pairs);
+    if (odd) launch_wgrad<1, 1>(s, x, cin, dy, y, cout, d, scratch, 32 * pairs, 1);
+  }
   const size_t row = nw + cout;  // a split's partial sums: weights, then bias
   if (db == dw + nw) {  // the usual parameter layout: one reduction for both
     hipLaunchKernelGGL(k_reduce, dim3(unsigned((row + 31) / 32)), dim3(256), 0, s, scratch, splits, row, row, dw);
