@@ -32,9 +32,10 @@ bool conv_fwd_radiance(hipStream_t s, const uint16_t* x, int cin, const uint16_t
 // at half resolution, b has cb channels at d; the same result as upcat_fwd followed by conv_fwd.
 void conv_fwd_upcat(hipStream_t s, const uint16_t* a, int ca, const uint16_t* b, int cb, const uint16_t* w,
                     const float* bias, uint16_t* y, int cout, Dims d, bool relu);
-// dx = conv^T(dy * [y > 0]) (mask only when y != null).
+// dx = conv^T(dy * [y > 0]) (mask only when y != null), then dx * [dx_mask > 0] when dx_mask != null
+// (the ReLU of the layer that produced the convolution's input, applied where its gradient is made).
 void conv_dgrad(hipStream_t s, const uint16_t* dy, const uint16_t* y, int cout, const uint16_t* wt, uint16_t* dx,
-                int cin, Dims d);
+                int cin, Dims d, const uint16_t* dx_mask = nullptr);
 // dW[co][t][ci] = sum_p dPre[p][co] x[p + off_t][ci], dB[co] = sum_p dPre[p][co], dPre = dy * [y > 0];
 // summed in a fixed order (per-split partials in `scratch`, then reduced).
 void conv_wgrad(hipStream_t s, const uint16_t* x, int cin, const uint16_t* dy, const uint16_t* y, int cout, Dims d,
@@ -42,13 +43,15 @@ void conv_wgrad(hipStream_t s, const uint16_t* x, int cin, const uint16_t* dy, c
 size_t wgrad_scratch_floats(int cin, int cout, Dims d);
 
 // dx = route dy to each window's first maximum, plus add[:, add_off : add_off + c] when add != null
-// (add has add_c channels at the full resolution).
+// (add has add_c channels at the full resolution); with relu_mask, zero where x is not positive.
 void pool_bwd(hipStream_t s, const uint16_t* x, const uint16_t* dy, const uint16_t* add, int add_c, int add_off,
-              uint16_t* dx, int c, Dims d);
+              uint16_t* dx, int c, Dims d, bool relu_mask = false);
 // out[:, 0:ca] = a at half resolution (nearest 2x up), out[:, ca:ca+cb] = b; d is the full resolution.
 void upcat_fwd(hipStream_t s, const uint16_t* a, int ca, const uint16_t* b, int cb, uint16_t* out, Dims d);
-// da = sum over the 4 children of dout[:, 0:ca] (dout has ca + cb channels).
-void upcat_bwd(hipStream_t s, const uint16_t* dout, int ca, int cb, uint16_t* da, Dims d);
+// da = sum over the 4 children of dout[:, 0:ca] (dout has ca + cb channels), zero where
+// da_mask (same layout as da) is not positive when da_mask != null.
+void upcat_bwd(hipStream_t s, const uint16_t* dout, int ca, int cb, uint16_t* da, Dims d,
+               const uint16_t* da_mask = nullptr);
 
 // x (16 channels, padded dims dp) from fp32 images [n][h][w][3]: log1p(max(beauty, 0)), albedo,
 // normal, zeros; pixels beyond (h, w) replicate the edge.

@@ -171,7 +171,8 @@ __global__ __launch_bounds__(256) void k_conv(const uint16_t* __restrict__ x, co
                                               int cin, const uint16_t* __restrict__ xa, int ca,
                                               const uint16_t* __restrict__ wgt, const float* __restrict__ bias,
                                               void* __restrict__ y, uint16_t* __restrict__ pooled, int cout,
-                                              int co_first, int H, int W, int nrt, int nct, int relu) {
+                                              int co_first, int H, int W, int nrt, int nct, int relu,
+                                              const uint16_t* __restrict__ omask) {
   extern __shared__ uint4 lds4[];
   uint16_t* halo = reinterpret_cast<uint16_t*>(lds4);
   uint16_t* wts = halo + kHR * kHC * kLS;
@@ -263,7 +264,9 @@ __global__ __launch_bounds__(256) void k_conv(const uint16_t* __restrict__ x, co
                                  uint32_t(f2bf(a.z)) | (uint32_t(f2bf(a.w)) << 16),
                                  uint32_t(f2bf(b.x)) | (uint32_t(f2bf(b.y)) << 16),
                                  uint32_t(f2bf(b.z)) | (uint32_t(f2bf(b.w)) << 16));
-      *reinterpret_cast<uint4*>(static_cast<uint16_t*>(y) + (pix0 + w0 + p) * cout + co0 + 8 * g) = o;
+      const size_t off = (pix0 + w0 + p) * cout + co0 + 8 * g;
+      *reinterpret_cast<uint4*>(static_cast<uint16_t*>(y) + off) =
+          omask ? mask8(o, *reinterpret_cast<const uint4*>(omask + off)) : o;
     }
   }
 }
@@ -899,7 +902,7 @@ __device__ __forceinline__ uint4 pack8(const float (&f)[8]) {
 
 __global__ void k_pool_bwd(const uint16_t* __restrict__ x, const uint16_t* __restrict__ dy,
                            const uint16_t* __restrict__ add, int add_c, int add_off, uint16_t* __restrict__ dx, int c,
-                           int H, int W, size_t n8) {
+                           int H, int W, size_t n8, int relu_mask) {
   const int H2 = H / 2, W2 = W / 2, c8 = c / 8;
   for (size_t i = blockIdx.x * size_t(blockDim.x) + threadIdx.x; i < n8; i += size_t(gridDim.x) * blockDim.x) {
     const int ch = int(i % c8) * 8;
@@ -932,6 +935,13 @@ __global__ void k_pool_bwd(const uint16_t* __restrict__ x, const uint16_t* __res
       g[e] = mine == arg[e] ? up[e] : 0.f;
       if (add) g[e] += ad[e];
     }
+    if (relu_mask) {  // x is the ReLU output: its gradient is zero where x is not positive
+      float xs[8];
+      unpack8(*reinterpret_cast<const uint4*>(x + cand[mine]), xs);
+#pragma unroll
+      for (int e = 0; e < 8; ++e)
+        if (!(xs[e] > 0.f)) g[e] = 0.f;
+    }
     *reinterpret_cast<uint4*>(dx + p * c + ch) = pack8(g);
   }
 }
@@ -956,8 +966,8 @@ __global__ void k_upcat_fwd(const uint16_t* __restrict__ a, int ca, const uint16
   }
 }
 
-__global__ void k_upcat_bwd(const uint16_t* __restrict__ dout, int ca, int cb, uint16_t* __restrict__ da, int H,
-                            int W, size_t n) {
+__global__ void k_upcat_bwd(const uint16_t* __restrict__ dout, int ca, int cb, uint16_t* __restrict__ da,
+                            const uint16_t* __restrict__ da_mask, int H, int W, size_t n) {
   const int c = ca + cb, H2 = H / 2, W2 = W / 2;
   for (size_t i = blockIdx.x * size_t(blockDim.x) + threadIdx.x; i < n; i += size_t(gridDim.x) * blockDim.x) {
     const int ch = int(i % ca);
@@ -969,7 +979,7 @@ __global__ void k_upcat_bwd(const uint16_t* __restrict__ dout, int ca, int cb, u
     const size_t base = ((b * H + 2 * h2) * W + 2 * w2) * c + ch;
     float s = bf2f(dout[base]) + bf2f(dout[base + c]);
     s += bf2f(dout[base + size_t(W) * c]) + bf2f(dout[base + size_t(W) * c + c]);
-    da[i] = f2bf(s);
+    da[i] = (da_mask && !(bf2f(da_mask[i]) > 0.f)) ? uint16_t(0) : f2bf(s);
   }
 }
 
@@ -1123,7 +1133,7 @@ unsigned grid_for(size_t n, unsigned block = 256) {
 template <bool DGRAD, bool F32, int NT>
 void launch_nt(hipStream_t s, const uint16_t* x, const uint16_t* mask, int cin, const uint16_t* xa, int ca,
                const uint16_t* w, const float* bias, void* y, uint16_t* pooled, int cout, int co_first, int groups,
-               Dims d, int relu) {
+               Dims d, int relu, const uint16_t* omask) {
   const int nrt = (d.h + kTR - 1) / kTR, nct = (d.w + kTC - 1) / kTC;
   constexpr size_t lds = conv_lds_bytes<NT>();
   static const hipError_t attr =  // once per instantiation: allow more than 64 KiB of LDS
@@ -1131,7 +1141,7 @@ void launch_nt(hipStream_t s, const uint16_t* x, const uint16_t* mask, int cin, 
                           hipFuncAttributeMaxDynamicSharedMemorySize, int(lds));
   (void)attr;
   hipLaunchKernelGGL((k_conv<NT, DGRAD, F32>), dim3(d.n * nrt * nct, groups), dim3(256), lds, s, x, mask, cin, xa,
-                     ca, w, bias, y, pooled, cout, co_first, d.h, d.w, nrt, nct, relu);
+                     ca, w, bias, y, pooled, cout, co_first, d.h, d.w, nrt, nct, relu, omask);
 }
 
 // Output channels in groups of 64 (NT = 4), then one launch for the rest.
@@ -1195,13 +1205,13 @@ bool launch_pipe(hipStream_t s, const uint16_t* x, int cin, const uint16_t* xa, 
 template <bool DGRAD, bool F32>
 void launch_groups(hipStream_t s, int nt, int first_frag, int groups, const uint16_t* x, const uint16_t* mask, int cin,
                    const uint16_t* xa, int ca, const uint16_t* w, const float* bias, void* y, int cout, Dims d,
-                   int relu, uint16_t* pooled) {
+                   int relu, uint16_t* pooled, const uint16_t* omask) {
   const int c0 = first_frag * 16;
   switch (nt) {
-    case 1: launch_nt<DGRAD, F32, 1>(s, x, mask, cin, xa, ca, w, bias, y, pooled, cout, c0, groups, d, relu); break;
-    case 2: launch_nt<DGRAD, F32, 2>(s, x, mask, cin, xa, ca, w, bias, y, pooled, cout, c0, groups, d, relu); break;
-    case 3: launch_nt<DGRAD, F32, 3>(s, x, mask, cin, xa, ca, w, bias, y, pooled, cout, c0, groups, d, relu); break;
-    default: launch_nt<DGRAD, F32, 4>(s, x, mask, cin, xa, ca, w, bias, y, pooled, cout, c0, groups, d, relu); break;
+    case 1: launch_nt<DGRAD, F32, 1>(s, x, mask, cin, xa, ca, w, bias, y, pooled, cout, c0, groups, d, relu, omask); break;
+    case 2: launch_nt<DGRAD, F32, 2>(s, x, mask, cin, xa, ca, w, bias, y, pooled, cout, c0, groups, d, relu, omask); break;
+    case 3: launch_nt<DGRAD, F32, 3>(s, x, mask, cin, xa, ca, w, bias, y, pooled, cout, c0, groups, d, relu, omask); break;
+    default: launch_nt<DGRAD, F32, 4>(s, x, mask, cin, xa, ca, w, bias, y, pooled, cout, c0, groups, d, relu, omask); break;
   }
 }
 
@@ -1212,13 +1222,14 @@ void launch_groups(hipStream_t s, int nt, int first_frag, int groups, const uint
 template <bool DGRAD, bool F32>
 void launch_conv(hipStream_t s, const uint16_t* x, const uint16_t* mask, int cin, const uint16_t* xa, int ca,
                  const uint16_t* w, const float* bias, void* y, int cout, Dims d, int relu,
-                 uint16_t* pooled = nullptr) {
+                 uint16_t* pooled = nullptr, const uint16_t* omask = nullptr) {
   if (!DGRAD && launch_pipe<F32>(s, x, cin, xa, ca, w, bias, y, pooled, cout, d, relu)) return;
   const int tiles = d.n * ((d.h + kTR - 1) / kTR) * ((d.w + kTC - 1) / kTC);
   const int nt = tiles >= 1024 ? 4 : tiles >= 256 ? 2 : 1;
   const int frags = cout / 16, full = frags / nt, rest = frags % nt;
-  if (full) launch_groups<DGRAD, F32>(s, nt, 0, full, x, mask, cin, xa, ca, w, bias, y, cout, d, relu, pooled);
-  if (rest) launch_groups<DGRAD, F32>(s, rest, full * nt, 1, x, mask, cin, xa, ca, w, bias, y, cout, d, relu, pooled);
+  if (full) launch_groups<DGRAD, F32>(s, nt, 0, full, x, mask, cin, xa, ca, w, bias, y, cout, d, relu, pooled, omask);
+  if (rest)
+    launch_groups<DGRAD, F32>(s, rest, full * nt, 1, x, mask, cin, xa, ca, w, bias, y, cout, d, relu, pooled, omask);
 }
 
 // Weight-gradient splits: one per band of kWR rows of one image.
@@ -1259,9 +1270,9 @@ void conv_fwd_upcat(hipStream_t s, const uint16_t* a, int ca, const uint16_t* b,
 }
 
 void conv_dgrad(hipStream_t s, const uint16_t* dy, const uint16_t* y, int cout, const uint16_t* wt, uint16_t* dx,
-                int cin, Dims d) {
+                int cin, Dims d, const uint16_t* dx_mask) {
   // the transposed problem: input dy (cout channels), output dx (cin channels)
-  launch_conv<true, false>(s, dy, y, cout, nullptr, 0, wt, nullptr, dx, cin, d, 0);
+  launch_conv<true, false>(s, dy, y, cout, nullptr, 0, wt, nullptr, dx, cin, d, 0, nullptr, dx_mask);
 }
 
 size_t wgrad_scratch_floats(int cin, int cout, Dims d) {
@@ -1296,10 +1307,10 @@ void conv_wgrad(hipStream_t s, const uint16_t* x, int cin, const uint16_t* dy, c
 }
 
 void pool_bwd(hipStream_t s, const uint16_t* x, const uint16_t* dy, const uint16_t* add, int add_c, int add_off,
-              uint16_t* dx, int c, Dims d) {
+              uint16_t* dx, int c, Dims d, bool relu_mask) {
   const size_t n8 = size_t(d.pixels()) * (c / 8);
   hipLaunchKernelGGL(k_pool_bwd, dim3(grid_for(n8)), dim3(256), 0, s, x, dy, add, add_c, add_off, dx, c, d.h, d.w,
-                     n8);
+                     n8, relu_mask ? 1 : 0);
 }
 
 void upcat_fwd(hipStream_t s, const uint16_t* a, int ca, const uint16_t* b, int cb, uint16_t* out, Dims d) {
@@ -1307,9 +1318,9 @@ void upcat_fwd(hipStream_t s, const uint16_t* a, int ca, const uint16_t* b, int 
   hipLaunchKernelGGL(k_upcat_fwd, dim3(grid_for(n8)), dim3(256), 0, s, a, ca, b, cb, out, d.h, d.w, n8);
 }
 
-void upcat_bwd(hipStream_t s, const uint16_t* dout, int ca, int cb, uint16_t* da, Dims d) {
+void upcat_bwd(hipStream_t s, const uint16_t* dout, int ca, int cb, uint16_t* da, Dims d, const uint16_t* da_mask) {
   const size_t n = size_t(d.n) * (d.h / 2) * (d.w / 2) * ca;
-  hipLaunchKernelGGL(k_upcat_bwd, dim3(grid_for(n)), dim3(256), 0, s, dout, ca, cb, da, d.h, d.w, n);
+  hipLaunchKernelGGL(k_upcat_bwd, dim3(grid_for(n)), dim3(256), 0, s, dout, ca, cb, da, da_mask, d.h, d.w, n);
 }
 
 void pack_input(hipStream_t s, const float* beauty, const float* beauty_b, const float* albedo, const float* normal,

@@ -241,22 +241,26 @@ void backward(gt_denoiser* dn) {
   auto wgrad = [&](Conv& c, const uint16_t* x, const uint16_t* dy, const uint16_t* mask, gdn::Dims d) {
     gdn::conv_wgrad(s, x, c.cin, dy, mask, c.cout, d, G + c.w_off, G + c.b_off, S, SN);
   };
+  // Each gradient is masked by the ReLU of the tensor it belongs to where it is made (the data
+  // gradient, upsample or pooling backward that writes it), so the weight and data gradients
+  // that read it need no mask.
   // output convolution (no ReLU)
   Conv& co = dn->out_conv();
   wgrad(co, dn->lv[0]->dec.p, dn->dz.p, nullptr, dn->shape);
-  gdn::conv_dgrad(s, dn->dz.p, nullptr, co.cout, co.wt16.p, dn->lv[0]->g_dec.p, co.cin, dn->shape);
+  gdn::conv_dgrad(s, dn->dz.p, nullptr, co.cout, co.wt16.p, dn->lv[0]->g_dec.p, co.cin, dn->shape, dn->lv[0]->dec.p);
   // decoder, nearest the output first
   for (int l = 0; l <= L - 2; ++l) {
     Level& v = *dn->lv[l];
     const gdn::Dims d = level_dims(dn, l);
     Conv& c1 = dn->dec_conv(l, 1);
     Conv& c0 = dn->dec_conv(l, 0);
-    wgrad(c1, v.dec_a.p, v.g_dec.p, v.dec.p, d);
-    gdn::conv_dgrad(s, v.g_dec.p, v.dec.p, c1.cout, c1.wt16.p, v.g_dec_a.p, c1.cin, d);
-    wgrad(c0, v.cat.p, v.g_dec_a.p, v.dec_a.p, d);
-    gdn::conv_dgrad(s, v.g_dec_a.p, v.dec_a.p, c0.cout, c0.wt16.p, v.g_cat.p, c0.cin, d);
+    wgrad(c1, v.dec_a.p, v.g_dec.p, nullptr, d);
+    gdn::conv_dgrad(s, v.g_dec.p, nullptr, c1.cout, c1.wt16.p, v.g_dec_a.p, c1.cin, d, v.dec_a.p);
+    wgrad(c0, v.cat.p, v.g_dec_a.p, nullptr, d);
+    gdn::conv_dgrad(s, v.g_dec_a.p, nullptr, c0.cout, c0.wt16.p, v.g_cat.p, c0.cin, d);
     uint16_t* g_up = l == L - 2 ? dn->lv[L - 1]->g_enc.p : dn->lv[l + 1]->g_dec.p;
-    gdn::upcat_bwd(s, v.g_cat.p, dn->ch[l + 1], dn->ch[l], g_up, d);
+    const uint16_t* up = l == L - 2 ? dn->lv[L - 1]->enc.p : dn->lv[l + 1]->dec.p;
+    gdn::upcat_bwd(s, v.g_cat.p, dn->ch[l + 1], dn->ch[l], g_up, d, up);
   }
   // encoder, deepest first
   for (int l = L - 1; l >= 0; --l) {
@@ -264,14 +268,14 @@ void backward(gt_denoiser* dn) {
     const gdn::Dims d = level_dims(dn, l);
     if (l < L - 1)  // pooled path plus the skip half of the concatenation
       gdn::pool_bwd(s, v.enc.p, dn->lv[l + 1]->g_in.p, v.g_cat.p, dn->ch[l + 1] + dn->ch[l], dn->ch[l + 1],
-                    v.g_enc.p, dn->ch[l], d);
+                    v.g_enc.p, dn->ch[l], d, true);
     Conv& c1 = dn->enc_conv(l, 1);
     Conv& c0 = dn->enc_conv(l, 0);
-    wgrad(c1, v.enc_a.p, v.g_enc.p, v.enc.p, d);
-    gdn::conv_dgrad(s, v.g_enc.p, v.enc.p, c1.cout, c1.wt16.p, v.g_enc_a.p, c1.cin, d);
+    wgrad(c1, v.enc_a.p, v.g_enc.p, nullptr, d);
+    gdn::conv_dgrad(s, v.g_enc.p, nullptr, c1.cout, c1.wt16.p, v.g_enc_a.p, c1.cin, d, v.enc_a.p);
     const uint16_t* in = l == 0 ? dn->x0.p : v.in.p;
-    wgrad(c0, in, v.g_enc_a.p, v.enc_a.p, d);
-    if (l > 0) gdn::conv_dgrad(s, v.g_enc_a.p, v.enc_a.p, c0.cout, c0.wt16.p, v.g_in.p, c0.cin, d);
+    wgrad(c0, in, v.g_enc_a.p, nullptr, d);
+    if (l > 0) gdn::conv_dgrad(s, v.g_enc_a.p, nullptr, c0.cout, c0.wt16.p, v.g_in.p, c0.cin, d);
   }
 }
 
