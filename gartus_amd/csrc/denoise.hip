@@ -421,7 +421,7 @@ __device__ __forceinline__ void pipe_copy_halo(uint4* buf, i32x4 src, int src_c,
 // and the chunk's weights from slot wb of wts (rows tap * NT*16 + co, 2G
 // pieces each, swizzled the same way by row). Both arrays are indexed from
 // their bases in one step.
-template <int NT, int G>
+template <int NT, int G, bool MIRROR>
 __device__ __forceinline__ void pipe_chunk_mfma(const uint4* halo, const uint4* wts, int wb, int lane, int wave,
                                                 v4f (&acc)[4][NT]) {
   constexpr int QP = 2 * G;
@@ -432,7 +432,7 @@ __device__ __forceinline__ void pipe_chunk_mfma(const uint4* halo, const uint4* 
     const bool valid = sub < 9 * G;
     const int tap = valid ? sub / G : 0;
     const int q = (sub % G) * 2 + half;
-    const int dy = tap / 3 - 1, dx = tap % 3 - 1;
+    const int dy = MIRROR ? 1 - tap / 3 : tap / 3 - 1, dx = MIRROR ? 1 - tap % 3 : tap % 3 - 1;
     uint4 b[NT];
 #pragma unroll
     for (int j = 0; j < NT; ++j) {
@@ -458,6 +458,7 @@ template <int NT, bool OUT_F32>
 struct PipeOut {
   u32x4 row[pipe_row_pieces<NT, OUT_F32>()];  // vector types, not uint4: they stay in registers
   u32x4 pool[pipe_pool_per_thread(NT)];
+  u32x4 mrow[OUT_F32 ? 1 : pipe_row_pieces<NT, OUT_F32>()];  // the output mask of the row pieces
   int tile = -1;
 };
 
@@ -469,7 +470,22 @@ struct PipeOut {
 template <int NT, bool OUT_F32, int GMAX>
 __device__ __forceinline__ void pipe_finish(uint4* buf, v4f (&acc)[4][NT], const float (&bj)[NT], int relu,
                                             bool pooled, bool rad, PipeOut<NT, OUT_F32>& out, int t, int tid,
-                                            int lane, int wave) {
+                                            int lane, int wave, const uint16_t* omask, int cout, int co0, int H,
+                                            int W, int nrt, int nct) {
+  if (!OUT_F32 && omask) {
+    // the output mask of this lane's row pieces, loaded now: the next step's wait for its halo
+    // copy (issued before these loads) covers them, so the stores wait for nothing more
+    constexpr int GP = pipe_row_pieces<NT, OUT_F32>();
+    const int ct = t % nct, rt = (t / nct) % nrt, n = t / (nct * nrt);
+    const int row = rt * kTR + wave, w0 = ct * kTC;
+#pragma unroll
+    for (int k = 0; k < GP; ++k) {
+      const int i = lane + 64 * k, p = i / GP, g = i % GP;
+      out.mrow[k] = u32x4{0u, 0u, 0u, 0u};
+      if (row < H && w0 + p < W)
+        out.mrow[k] = *reinterpret_cast<const u32x4*>(omask + ((size_t(n) * H + row) * W + w0 + p) * cout + co0 + 8 * g);
+    }
+  }
   constexpr int RB = pipe_row_bytes<NT, OUT_F32, GMAX>(), GP = pipe_row_pieces<NT, OUT_F32>();
   constexpr int PC = pipe_pass_channels<NT, OUT_F32>(), PJ = PC / 16;  // channels, fragments per pass
   constexpr int CP = OUT_F32 ? 4 : 8;                                  // channels per 16-byte piece
@@ -541,8 +557,8 @@ __device__ __forceinline__ void pipe_finish(uint4* buf, v4f (&acc)[4][NT], const
 // padded tensor), instead of y.
 template <int NT, bool OUT_F32>
 __device__ __forceinline__ void pipe_store(PipeOut<NT, OUT_F32>& out, void* y, uint16_t* pooled, float* rad, int rh,
-                                           int rw, int cout, int co0, int H, int W, int nrt, int nct, int tid,
-                                           int lane, int wave) {
+                                           int rw, const uint16_t* omask, int cout, int co0, int H, int W, int nrt,
+                                           int nct, int tid, int lane, int wave) {
   constexpr int GP = pipe_row_pieces<NT, OUT_F32>();
   const int t = out.tile;
   out.tile = -1;
@@ -562,10 +578,19 @@ __device__ __forceinline__ void pipe_store(PipeOut<NT, OUT_F32>& out, void* y, u
     for (int k = 0; k < GP; ++k) {
       const int i = lane + 64 * k, p = i / GP, g = i % GP;
       if (w0 + p >= W) continue;
-      if (OUT_F32)
+      if (OUT_F32) {
         *reinterpret_cast<u32x4*>(static_cast<float*>(y) + (pix0 + w0 + p) * cout + co0 + 4 * g) = out.row[k];
-      else
-        *reinterpret_cast<u32x4*>(static_cast<uint16_t*>(y) + (pix0 + w0 + p) * cout + co0 + 8 * g) = out.row[k];
+      } else {
+        const size_t off = (pix0 + w0 + p) * cout + co0 + 8 * g;
+        u32x4 v = out.row[k];
+        if (omask) {  // zero where the mask is not positive (the ReLU of the tensor this gradient belongs to)
+          const u32x4 m = out.mrow[k];
+#pragma unroll
+          for (int e = 0; e < 4; ++e)
+            v[e] &= (bf2f(uint16_t(m[e])) > 0.f ? 0x0000ffffu : 0u) | (bf2f(uint16_t(m[e] >> 16)) > 0.f ? 0xffff0000u : 0u);
+        }
+        *reinterpret_cast<u32x4*>(static_cast<uint16_t*>(y) + off) = v;
+      }
     }
   }
   if (!OUT_F32 && pooled) {
@@ -583,13 +608,14 @@ __device__ __forceinline__ void pipe_store(PipeOut<NT, OUT_F32>& out, void* y, u
   }
 }
 
-template <int NT, bool OUT_F32, int CIN, int GMAX>
+template <int NT, bool OUT_F32, int CIN, int GMAX, bool MIRROR>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k_conv_pipe(const uint16_t* __restrict__ x,
                                                    const uint16_t* __restrict__ xa, int ca,
                                                    const uint16_t* __restrict__ wgt, const float* __restrict__ bias,
                                                    void* __restrict__ y, uint16_t* __restrict__ pooled, int cout,
                                                    int co_first, int N, int H, int W, int nrt, int nct, int relu,
-                                                   float* __restrict__ rad, int rh, int rw) {
+                                                   float* __restrict__ rad, int rh, int rw,
+                                                   const uint16_t* __restrict__ omask) {
   __shared__ uint4 bufA[pipe_buf_slots<NT, OUT_F32, GMAX>()];
   __shared__ uint4 bufB[pipe_buf_slots<NT, OUT_F32, GMAX>()];
   __shared__ uint4 wts4[9 * NT * 16 * CIN / 8];
@@ -648,15 +674,15 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
     lds_barrier();  // ... and every wave's; nxt was last read in the previous step
     if (s + 1 < S) copy(nxt, s + 1);
     if (out.tile >= 0)
-      pipe_store<NT, OUT_F32>(out, y, pooled, rad, rh, rw, cout, co0, H, W, nrt, nct, tid, lane, wave);
+      pipe_store<NT, OUT_F32>(out, y, pooled, rad, rh, rw, omask, cout, co0, H, W, nrt, nct, tid, lane, wave);
     const int k = s / nch, ch = s - k * nch;
     const PipeChunk c = pipe_chunk<GMAX>(ch, cin, ca);
     const int wb = 9 * NT * 16 * (c.c0 / 8);
-    if (GMAX == 2 && c.g == 2) pipe_chunk_mfma<NT, 2>(cur, wts4, wb, lane, wave, acc);
-    else pipe_chunk_mfma<NT, 1>(cur, wts4, wb, lane, wave, acc);
+    if (GMAX == 2 && c.g == 2) pipe_chunk_mfma<NT, 2, MIRROR>(cur, wts4, wb, lane, wave, acc);
+    else pipe_chunk_mfma<NT, 1, MIRROR>(cur, wts4, wb, lane, wave, acc);
     if (ch == nch - 1)
       pipe_finish<NT, OUT_F32, GMAX>(cur, acc, bj, relu, pooled != nullptr, rad != nullptr, out,
-                                     t_begin + slot0 + k * per, tid, lane, wave);
+                                     t_begin + slot0 + k * per, tid, lane, wave, omask, cout, co0, H, W, nrt, nct);
   };
   copy(bufA, 0);
   for (int s = 0; s < S; s += 2) {
@@ -664,7 +690,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
     if (s + 1 < S) step(bufB, bufA, s + 1);
   }
   if (out.tile >= 0)
-    pipe_store<NT, OUT_F32>(out, y, pooled, rad, rh, rw, cout, co0, H, W, nrt, nct, tid, lane, wave);
+    pipe_store<NT, OUT_F32>(out, y, pooled, rad, rh, rw, omask, cout, co0, H, W, nrt, nct, tid, lane, wave);
 }
 
 // ---- convolution, weight gradient: LDS tiles and 16x16x32 MFMA -------------
@@ -1164,24 +1190,48 @@ int device_cus() {
 // offsets. These fit two blocks per CU. False for the others, which take
 // k_conv: the decoder's 64 + 48 -> 48 keeps 97 KB of weights resident, one
 // block fits a CU, and it ran 40 % slower than k_conv.
-template <int NT, bool F32, int CIN, int GMAX>
+template <int NT, bool F32, int CIN, int GMAX, bool MIRROR = false>
 bool launch_pipe_nt(hipStream_t s, const uint16_t* x, const uint16_t* xa, int ca, const uint16_t* w,
                     const float* bias, void* y, uint16_t* pooled, int cout, Dims d, int relu, float* rad = nullptr,
-                    int rh = 0, int rw = 0) {
+                    int rh = 0, int rw = 0, const uint16_t* omask = nullptr) {
   if (size_t(d.n) * d.h * d.w * (CIN - ca) * 2 >= kOutOfRange) return false;
   if (xa && size_t(d.n) * (d.h / 2) * (d.w / 2) * ca * 2 >= kOutOfRange) return false;
   static const int per_cu = [] {  // resident blocks per CU
     int n = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_conv_pipe<NT, F32, CIN, GMAX>, 256, 0) != hipSuccess || n <= 0)
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_conv_pipe<NT, F32, CIN, GMAX, MIRROR>, 256, 0) !=
+            hipSuccess ||
+        n <= 0)
       n = 1;
     return n;
   }();
   const int nrt = (d.h + kTR - 1) / kTR, nct = (d.w + kTC - 1) / kTC;
   const int ntiles = d.n * nrt * nct;
   const int nb = (std::min(ntiles, device_cus() * per_cu) + 7) / 8 * 8;  // a multiple of the 8 XCDs
-  hipLaunchKernelGGL((k_conv_pipe<NT, F32, CIN, GMAX>), dim3(nb, 1), dim3(256), 0, s, x, xa, ca, w, bias, y, pooled, cout,
-                     0, d.n, d.h, d.w, nrt, nct, relu, rad, rh, rw);
+  hipLaunchKernelGGL((k_conv_pipe<NT, F32, CIN, GMAX, MIRROR>), dim3(nb, 1), dim3(256), 0, s, x, xa, ca, w, bias, y,
+                     pooled, cout, 0, d.n, d.h, d.w, nrt, nct, relu, rad, rh, rw, omask);
   return true;
+}
+
+// The data gradient (taps mirrored, input already masked) through the same
+// kernel, for the full-resolution shapes that fit one group; omask zeroes the
+// output where the forward tensor it belongs to is not positive.
+bool launch_pipe_dgrad(hipStream_t s, const uint16_t* x, int cin, const uint16_t* w, void* y, int cout, Dims d,
+                       const uint16_t* omask) {
+  switch ((cout / 16) * 1000 + cin) {
+    case 2016:
+      return launch_pipe_nt<2, false, 16, 1, true>(s, x, nullptr, 0, w, nullptr, y, nullptr, cout, d, 0, nullptr, 0, 0,
+                                                   omask);
+    case 2032:
+      return launch_pipe_nt<2, false, 32, 1, true>(s, x, nullptr, 0, w, nullptr, y, nullptr, cout, d, 0, nullptr, 0, 0,
+                                                   omask);
+    case 2048:
+      return launch_pipe_nt<2, false, 48, 1, true>(s, x, nullptr, 0, w, nullptr, y, nullptr, cout, d, 0, nullptr, 0, 0,
+                                                   omask);
+    case 3048:
+      return launch_pipe_nt<3, false, 48, 1, true>(s, x, nullptr, 0, w, nullptr, y, nullptr, cout, d, 0, nullptr, 0, 0,
+                                                   omask);
+    default: return false;
+  }
 }
 
 template <bool F32>
@@ -1224,6 +1274,9 @@ void launch_conv(hipStream_t s, const uint16_t* x, const uint16_t* mask, int cin
                  const uint16_t* w, const float* bias, void* y, int cout, Dims d, int relu,
                  uint16_t* pooled = nullptr, const uint16_t* omask = nullptr) {
   if (!DGRAD && launch_pipe<F32>(s, x, cin, xa, ca, w, bias, y, pooled, cout, d, relu)) return;
+  if (DGRAD && !F32 && !mask && !xa && d.n * ((d.h + kTR - 1) / kTR) * ((d.w + kTC - 1) / kTC) >= 1024 &&
+      launch_pipe_dgrad(s, x, cin, w, y, cout, d, omask))
+    return;
   const int tiles = d.n * ((d.h + kTR - 1) / kTR) * ((d.w + kTC - 1) / kTC);
   const int nt = tiles >= 1024 ? 4 : tiles >= 256 ? 2 : 1;
   const int frags = cout / 16, full = frags / nt, rest = frags % nt;
