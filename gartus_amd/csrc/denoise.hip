@@ -726,6 +726,7 @@ __device__ __forceinline__ uint4 shift_up(uint4 lo, uint4 hi) {  // elements 1..
 
 template <int NCO, int NCI>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k_wgrad(const uint16_t* __restrict__ x, int cin,
+                                               const uint16_t* __restrict__ xa, int ca,
                                                const uint16_t* __restrict__ dy, const uint16_t* __restrict__ mask,
                                                int cout, int H, int W, int nbands, int ncog, int ci_first,
                                                float* __restrict__ partial) {
@@ -774,8 +775,11 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
       const int q = i % XQ, px = i / XQ, col = px % kWX, row = px / kWX;
       const int h = r0 - 1 + row, w = c0 - 8 + col;
       uint4 v = make_uint4(0, 0, 0, 0);
-      if (h >= 0 && h < H && w >= 0 && w < W)
-        v = *reinterpret_cast<const uint4*>(x + ((size_t(n) * H + h) * W + w) * cin + ci0 + 8 * q);
+      if (h >= 0 && h < H && w >= 0 && w < W) {
+        const int c = ci0 + 8 * q;  // channels below ca come from xa at half resolution (nearest)
+        v = c < ca ? *reinterpret_cast<const uint4*>(xa + ((size_t(n) * (H / 2) + h / 2) * (W / 2) + w / 2) * ca + c)
+                   : *reinterpret_cast<const uint4*>(x + ((size_t(n) * H + h) * W + w) * (cin - ca) + (c - ca));
+      }
       const uint32_t e[4] = {v.x, v.y, v.z, v.w};
       uint16_t* dst = xT + (8 * q) * kSX + row * kWX + col;
 #pragma unroll
@@ -1290,8 +1294,8 @@ int wgrad_splits(Dims d) { return d.n * ((d.h + kWR - 1) / kWR); }
 
 template <int NCO, int NCI>
 // Input channels [ci_first, ci_first + ncig * NCI * 16) of the weight gradient.
-void launch_wgrad(hipStream_t s, const uint16_t* x, int cin, const uint16_t* dy, const uint16_t* mask, int cout,
-                  Dims d, float* partial, int ci_first, int ncig) {
+void launch_wgrad(hipStream_t s, const uint16_t* x, int cin, const uint16_t* xa, int ca, const uint16_t* dy,
+                  const uint16_t* mask, int cout, Dims d, float* partial, int ci_first, int ncig) {
   constexpr size_t lds = wgrad_lds_bytes<NCO, NCI>();
   static const hipError_t attr =  // once per instantiation: allow more than 64 KiB of LDS
       hipFuncSetAttribute(reinterpret_cast<const void*>(&k_wgrad<NCO, NCI>),
@@ -1299,8 +1303,8 @@ void launch_wgrad(hipStream_t s, const uint16_t* x, int cin, const uint16_t* dy,
   (void)attr;
   const int ncog = cout / (NCO * 16);
   const int nbands = (d.h + kWR - 1) / kWR;
-  hipLaunchKernelGGL((k_wgrad<NCO, NCI>), dim3(d.n * nbands, ncog * ncig), dim3(256), lds, s, x, cin, dy, mask, cout,
-                     d.h, d.w, nbands, ncog, ci_first, partial);
+  hipLaunchKernelGGL((k_wgrad<NCO, NCI>), dim3(d.n * nbands, ncog * ncig), dim3(256), lds, s, x, cin, xa, ca, dy, mask,
+                     cout, d.h, d.w, nbands, ncog, ci_first, partial);
 }
 
 }  // namespace
@@ -1334,7 +1338,7 @@ size_t wgrad_scratch_floats(int cin, int cout, Dims d) {
 }
 
 void conv_wgrad(hipStream_t s, const uint16_t* x, int cin, const uint16_t* dy, const uint16_t* y, int cout, Dims d,
-                float* dw, float* db, float* scratch, size_t scratch_floats) {
+                float* dw, float* db, float* scratch, size_t scratch_floats, const uint16_t* xa, int ca) {
   const int splits = wgrad_splits(d);
   const size_t nw = size_t(cout) * 9 * cin;
   if (scratch_floats < size_t(splits) * (nw + cout)) return;  // sized by wgrad_scratch_floats
@@ -1343,11 +1347,11 @@ void conv_wgrad(hipStream_t s, const uint16_t* x, int cin, const uint16_t* dy, c
   // own (it writes its own columns of the same partial rows).
   const int pairs = cin / 32, odd = (cin / 16) % 2;
   if ((cout / 16) % 2 == 0) {
-    if (pairs) launch_wgrad<2, 2>(s, x, cin, dy, y, cout, d, scratch, 0, pairs);
-    if (odd) launch_wgrad<2, 1>(s, x, cin, dy, y, cout, d, scratch, 32 * pairs, 1);
+    if (pairs) launch_wgrad<2, 2>(s, x, cin, xa, ca, dy, y, cout, d, scratch, 0, pairs);
+    if (odd) launch_wgrad<2, 1>(s, x, cin, xa, ca, dy, y, cout, d, scratch, 32 * pairs, 1);
   } else {
-    if (pairs) launch_wgrad<1, 2>(s, x, cin, dy, y, cout, d, scratch, 0, pairs);
-    if (odd) launch_wgrad<1, 1>(s, x, cin, dy, y, cout, d, scratch, 32 * pairs, 1);
+    if (pairs) launch_wgrad<1, 2>(s, x, cin, xa, ca, dy, y, cout, d, scratch, 0, pairs);
+    if (odd) launch_wgrad<1, 1>(s, x, cin, xa, ca, dy, y, cout, d, scratch, 32 * pairs, 1);
   }
   const size_t row = nw + cout;  // a split's partial sums: weights, then bias
   if (db == dw + nw) {  // the usual parameter layout: one reduction for both

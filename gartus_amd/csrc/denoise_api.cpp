@@ -55,7 +55,6 @@ struct Conv {
 struct Level {
   Buf<uint16_t> in;         // pooled input (l > 0)
   Buf<uint16_t> enc_a, enc; // encoder convolutions
-  Buf<uint16_t> cat;        // [up(deeper), enc] (l < L-1)
   Buf<uint16_t> dec_a, dec; // decoder convolutions (l < L-1)
   Buf<uint16_t> g_in, g_enc_a, g_enc, g_cat, g_dec_a, g_dec;
 };
@@ -148,7 +147,6 @@ int ensure_shape(gt_denoiser* dn, gdn::Dims d0) {
     GT_CHECK_HIP(lv->g_enc.ensure(P * c));
     if (l < L - 1) {
       const int cc = dn->ch[l + 1] + c;
-      GT_CHECK_HIP(lv->cat.ensure(P * cc));
       GT_CHECK_HIP(lv->g_cat.ensure(P * cc));
       GT_CHECK_HIP(lv->dec_a.ensure(P * c));
       GT_CHECK_HIP(lv->dec.ensure(P * c));
@@ -189,11 +187,11 @@ int prepare(gt_denoiser* dn, uint32_t n, uint32_t h, uint32_t w, const float* b,
   return GT_OK;
 }
 
-// keep_cat: also materialise each decoder level's concatenation (the weight
-// gradient of its first convolution reads it); inference reads the two
-// halves in place. With radiance, the output convolution writes the h x w
-// radiance there directly when it can (true) instead of z (false).
-bool forward(gt_denoiser* dn, bool keep_cat, float* radiance = nullptr, int h = 0, int w = 0) {
+// The decoder's concatenations are read in place from their two halves (here
+// and by the weight gradient), never written. With radiance, the output
+// convolution writes the h x w radiance there directly when it can (true)
+// instead of z (false).
+bool forward(gt_denoiser* dn, float* radiance = nullptr, int h = 0, int w = 0) {
   hipStream_t s = dn->stream;
   const int L = dn->levels;
   for (int l = 0; l < L; ++l) {
@@ -213,13 +211,8 @@ bool forward(gt_denoiser* dn, bool keep_cat, float* radiance = nullptr, int h = 
     const uint16_t* up = l == L - 2 ? dn->lv[L - 1]->enc.p : dn->lv[l + 1]->dec.p;
     Conv& c0 = dn->dec_conv(l, 0);
     Conv& c1 = dn->dec_conv(l, 1);
-    if (keep_cat) {
-      gdn::upcat_fwd(s, up, dn->ch[l + 1], v.enc.p, dn->ch[l], v.cat.p, d);
-      gdn::conv_fwd(s, v.cat.p, c0.cin, c0.w16.p, dn->params.p + c0.b_off, v.dec_a.p, c0.cout, d, true, false);
-    } else {
-      gdn::conv_fwd_upcat(s, up, dn->ch[l + 1], v.enc.p, dn->ch[l], c0.w16.p, dn->params.p + c0.b_off, v.dec_a.p,
-                          c0.cout, d, true);
-    }
+    gdn::conv_fwd_upcat(s, up, dn->ch[l + 1], v.enc.p, dn->ch[l], c0.w16.p, dn->params.p + c0.b_off, v.dec_a.p,
+                        c0.cout, d, true);
     gdn::conv_fwd(s, v.dec_a.p, c1.cin, c1.w16.p, dn->params.p + c1.b_off, v.dec.p, c1.cout, d, true, false);
   }
   Conv& co = dn->out_conv();
@@ -238,8 +231,9 @@ void backward(gt_denoiser* dn) {
   float* G = dn->grads.p;
   float* S = dn->scratch.p;
   const size_t SN = dn->scratch.n;
-  auto wgrad = [&](Conv& c, const uint16_t* x, const uint16_t* dy, const uint16_t* mask, gdn::Dims d) {
-    gdn::conv_wgrad(s, x, c.cin, dy, mask, c.cout, d, G + c.w_off, G + c.b_off, S, SN);
+  auto wgrad = [&](Conv& c, const uint16_t* x, const uint16_t* dy, const uint16_t* mask, gdn::Dims d,
+                   const uint16_t* xa = nullptr, int ca = 0) {
+    gdn::conv_wgrad(s, x, c.cin, dy, mask, c.cout, d, G + c.w_off, G + c.b_off, S, SN, xa, ca);
   };
   // Each gradient is masked by the ReLU of the tensor it belongs to where it is made (the data
   // gradient, upsample or pooling backward that writes it), so the weight and data gradients
@@ -256,10 +250,10 @@ void backward(gt_denoiser* dn) {
     Conv& c0 = dn->dec_conv(l, 0);
     wgrad(c1, v.dec_a.p, v.g_dec.p, nullptr, d);
     gdn::conv_dgrad(s, v.g_dec.p, nullptr, c1.cout, c1.wt16.p, v.g_dec_a.p, c1.cin, d, v.dec_a.p);
-    wgrad(c0, v.cat.p, v.g_dec_a.p, nullptr, d);
+    const uint16_t* up = l == L - 2 ? dn->lv[L - 1]->enc.p : dn->lv[l + 1]->dec.p;
+    wgrad(c0, v.enc.p, v.g_dec_a.p, nullptr, d, up, dn->ch[l + 1]);
     gdn::conv_dgrad(s, v.g_dec_a.p, nullptr, c0.cout, c0.wt16.p, v.g_cat.p, c0.cin, d);
     uint16_t* g_up = l == L - 2 ? dn->lv[L - 1]->g_enc.p : dn->lv[l + 1]->g_dec.p;
-    const uint16_t* up = l == L - 2 ? dn->lv[L - 1]->enc.p : dn->lv[l + 1]->dec.p;
     gdn::upcat_bwd(s, v.g_cat.p, dn->ch[l + 1], dn->ch[l], g_up, d, up);
   }
   // encoder, deepest first
@@ -410,7 +404,7 @@ int gt_denoiser_infer(gt_denoiser* dn, uint32_t n, uint32_t h, uint32_t w, const
   if (!out) return bad(GT_EINVAL, "null output");
   int rc = prepare(dn, n, h, w, b, a, nr);
   if (rc != GT_OK) return rc;
-  if (!forward(dn, false, out, int(h), int(w)))
+  if (!forward(dn, out, int(h), int(w)))
     gdn::unpack_output(dn->stream, dn->z.p, out, int(h), int(w), dn->shape);
   GT_CHECK_HIP(hipGetLastError());
   GT_CHECK_HIP(hipStreamSynchronize(dn->stream));
@@ -423,7 +417,7 @@ int gt_denoiser_loss_and_gradients(gt_denoiser* dn, uint32_t n, uint32_t h, uint
   if (!target) return bad(GT_EINVAL, "null target");
   int rc = prepare(dn, n, h, w, b, a, nr);
   if (rc != GT_OK) return rc;
-  forward(dn, true);
+  forward(dn);
   if ((rc = loss_into(dn, target, h, w, out_loss)) != GT_OK) return rc;
   backward(dn);
   GT_CHECK_HIP(hipGetLastError());
@@ -440,7 +434,7 @@ int gt_denoiser_train_step(gt_denoiser* dn, uint32_t n, uint32_t h, uint32_t w, 
   if (!(std::isfinite(lr) && lr > 0.0f)) return bad(GT_EINVAL, "learning rate must be positive and finite");
   int rc = prepare(dn, n, h, w, b, a, nr);
   if (rc != GT_OK) return rc;
-  forward(dn, true);
+  forward(dn);
   if ((rc = loss_into(dn, target, h, w, out_loss)) != GT_OK) return rc;
   backward(dn);
   dn->step += 1;
