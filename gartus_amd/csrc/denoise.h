@@ -29,7 +29,7 @@ void conv_fwd(hipStream_t s, const uint16_t* x, int cin, const uint16_t* w, cons
 bool conv_fwd_radiance(hipStream_t s, const uint16_t* x, int cin, const uint16_t* w, const float* bias, float* out,
                        int h, int w_img, Dims d);
 // y = act(conv(cat(up2(a), b)) + bias) without materialising the concatenation: a has ca channels
-// at half resolution, b has cb channels at d; the same result as upcat_fwd followed by conv_fwd.
+// at half resolution (nearest 2x up), b has cb channels at d.
 void conv_fwd_upcat(hipStream_t s, const uint16_t* a, int ca, const uint16_t* b, int cb, const uint16_t* w,
                     const float* bias, uint16_t* y, int cout, Dims d, bool relu);
 // dx = conv^T(dy * [y > 0]) (mask only when y != null), then dx * [dx_mask > 0] when dx_mask != null
@@ -49,8 +49,6 @@ size_t wgrad_scratch_floats(int cin, int cout, Dims d);
 // (add has add_c channels at the full resolution); with relu_mask, zero where x is not positive.
 void pool_bwd(hipStream_t s, const uint16_t* x, const uint16_t* dy, const uint16_t* add, int add_c, int add_off,
               uint16_t* dx, int c, Dims d, bool relu_mask = false);
-// out[:, 0:ca] = a at half resolution (nearest 2x up), out[:, ca:ca+cb] = b; d is the full resolution.
-void upcat_fwd(hipStream_t s, const uint16_t* a, int ca, const uint16_t* b, int cb, uint16_t* out, Dims d);
 // da = sum over the 4 children of dout[:, 0:ca] (dout has ca + cb channels), zero where
 // da_mask (same layout as da) is not positive when da_mask != null.
 void upcat_bwd(hipStream_t s, const uint16_t* dout, int ca, int cb, uint16_t* da, Dims d,

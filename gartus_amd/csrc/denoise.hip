@@ -976,26 +976,6 @@ __global__ void k_pool_bwd(const uint16_t* __restrict__ x, const uint16_t* __res
   }
 }
 
-__global__ void k_upcat_fwd(const uint16_t* __restrict__ a, int ca, const uint16_t* __restrict__ bsrc, int cb,
-                            uint16_t* __restrict__ out, int H, int W, size_t n8) {
-  const int c8 = (ca + cb) / 8;
-  for (size_t i = blockIdx.x * size_t(blockDim.x) + threadIdx.x; i < n8; i += size_t(gridDim.x) * blockDim.x) {
-    const int ch = int(i % c8) * 8;
-    const size_t p = i / c8;
-    uint4 v;
-    if (ch < ca) {  // the half-resolution input, nearest upsampling
-      const int w = int(p % W);
-      const size_t t = p / W;
-      const int h = int(t % H);
-      const size_t b = t / H;
-      v = *reinterpret_cast<const uint4*>(a + ((b * (H / 2) + h / 2) * (W / 2) + w / 2) * ca + ch);
-    } else {
-      v = *reinterpret_cast<const uint4*>(bsrc + p * cb + (ch - ca));
-    }
-    *reinterpret_cast<uint4*>(out + i * 8) = v;
-  }
-}
-
 __global__ void k_upcat_bwd(const uint16_t* __restrict__ dout, int ca, int cb, uint16_t* __restrict__ da,
                             const uint16_t* __restrict__ da_mask, int H, int W, size_t n) {
   const int c = ca + cb, H2 = H / 2, W2 = W / 2;
@@ -1368,11 +1348,6 @@ void pool_bwd(hipStream_t s, const uint16_t* x, const uint16_t* dy, const uint16
   const size_t n8 = size_t(d.pixels()) * (c / 8);
   hipLaunchKernelGGL(k_pool_bwd, dim3(grid_for(n8)), dim3(256), 0, s, x, dy, add, add_c, add_off, dx, c, d.h, d.w,
                      n8, relu_mask ? 1 : 0);
-}
-
-void upcat_fwd(hipStream_t s, const uint16_t* a, int ca, const uint16_t* b, int cb, uint16_t* out, Dims d) {
-  const size_t n8 = size_t(d.pixels()) * ((ca + cb) / 8);
-  hipLaunchKernelGGL(k_upcat_fwd, dim3(grid_for(n8)), dim3(256), 0, s, a, ca, b, cb, out, d.h, d.w, n8);
 }
 
 void upcat_bwd(hipStream_t s, const uint16_t* dout, int ca, int cb, uint16_t* da, Dims d, const uint16_t* da_mask) {
