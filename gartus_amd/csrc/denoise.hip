@@ -47,11 +47,33 @@ constexpr int kTR = 4;               // tile rows, one per wave
 constexpr int kTC = 64;              // tile columns, 4 fragments of 16
 constexpr int kHR = kTR + 2, kHC = kTC + 2;
 constexpr int kCK = 32;              // input channels per staged chunk
-constexpr int kLS = kCK + 8;         // LDS row stride in bf16: 80 B keeps 16-B reads conflict free
+constexpr int kLS = kCK;             // LDS row (pixel or weight row) in bf16: 64 B, pieces swizzled by swz<2>
+
+// A 16-byte LDS read (ds_read_b128) is served 16 lanes at a time: lanes
+// 0-31 and 32-63 are separate halves, and within a half the quads (4 lanes)
+// whose index l >> 2 has an even number of set bits go in one pass, the others
+// in the next. Two lanes of a pass conflict when their addresses differ but
+// agree in bits 4-7 (the same four of the 64 banks). An MFMA operand read
+// takes pixels P .. P + 15 (lane & 15) of piece q = (lane >> 4) & 1 and, in
+// lanes 32-63, the next sub-slice. With 32-byte pixels (16 channels, G = 1)
+// every pass then covers the 16 bank groups once as they are; with 64-byte
+// pixels (G = 2) piece q of pixel p goes to slot q ^ swz<2>(p), and the
+// weight rows the same way by row. Both layouts were checked conflict free
+// against SQ_LDS_BANK_CONFLICT for every read of a chunk, mirrored taps and
+// 16-channel chunks in 64-byte rows included.
+template <int G>
+__device__ __forceinline__ int swz(int p) {
+  return G == 2 ? (p >> 1) & 3 : 0;
+}
 
 template <int NT>
-constexpr size_t conv_lds_bytes() {
-  return size_t(kHR * kHC + 9 * NT * 16) * kLS * sizeof(uint16_t);
+constexpr int conv_out_stride() {  // fp32 row stride of the output staging: the 4 lane groups' rows land in distinct banks
+  return NT * 16 + 4;
+}
+template <int NT>
+constexpr size_t conv_lds_bytes() {  // the halo and weights of a chunk, or the output staging
+  return std::max(size_t(kHR * kHC + 9 * NT * 16) * kLS * sizeof(uint16_t),
+                  size_t(4 * 64 * conv_out_stride<NT>()) * sizeof(float));
 }
 
 typedef __bf16 v8bf __attribute__((ext_vector_type(8)));
@@ -85,15 +107,17 @@ __device__ __forceinline__ void conv_chunk(const uint16_t* halo, const uint16_t*
     const int sub = 2 * i + (lane >> 5);
     const bool valid = sub < 9 * G;
     const int tap = valid ? sub / G : 0;
-    const int ch = (sub % G) * 16 + 8 * half;
+    const int q = (sub % G) * 2 + half;  // 16-byte piece of the pixel or weight row
     int dy = tap / 3 - 1, dx = tap % 3 - 1;
     if (DGRAD) dy = -dy, dx = -dx;
     uint4 b[NT];
 #pragma unroll
-    for (int j = 0; j < NT; ++j)
-      b[j] = valid ? *reinterpret_cast<const uint4*>(wts + (tap * NT * 16 + j * 16 + col) * kLS + ch)
-                   : make_uint4(0, 0, 0, 0);
-    const uint16_t* arow = halo + ((wave + 1 + dy) * kHC + col + 1 + dx) * kLS + ch;
+    for (int j = 0; j < NT; ++j) {
+      const int r = tap * NT * 16 + j * 16 + col;
+      b[j] = valid ? *reinterpret_cast<const uint4*>(wts + r * kLS + 8 * (q ^ swz<2>(r))) : make_uint4(0, 0, 0, 0);
+    }
+    const int p0 = (wave + 1 + dy) * kHC + col + 1 + dx;  // pixels p0 + 16 f share the swizzle of p0
+    const uint16_t* arow = halo + p0 * kLS + 8 * (q ^ swz<2>(p0));
 #pragma unroll
     for (int f = 0; f < 4; ++f) {
       const uint4 a = valid ? *reinterpret_cast<const uint4*>(arow + f * 16 * kLS) : make_uint4(0, 0, 0, 0);
@@ -143,7 +167,7 @@ __device__ __forceinline__ void conv_stage(uint16_t* halo, uint16_t* wts, const 
 #pragma unroll
     for (int k = 0; k < HI; ++k) {
       const int i = tid + 256 * k;
-      if (i < HP) *reinterpret_cast<uint4*>(halo + (i / QP) * kLS + 8 * (i % QP)) = hv[k];
+      if (i < HP) *reinterpret_cast<uint4*>(halo + (i / QP) * kLS + 8 * ((i % QP) ^ swz<2>(i / QP))) = hv[k];
     }
   }
   __builtin_amdgcn_sched_barrier(0);  // keep the two batches apart: fewer live registers
@@ -161,7 +185,7 @@ __device__ __forceinline__ void conv_stage(uint16_t* halo, uint16_t* wts, const 
 #pragma unroll
     for (int k = 0; k < WI; ++k) {
       const int i = tid + 256 * k;
-      if (i < WP) *reinterpret_cast<uint4*>(wts + (i / QP) * kLS + 8 * (i % QP)) = wv[k];
+      if (i < WP) *reinterpret_cast<uint4*>(wts + (i / QP) * kLS + 8 * ((i % QP) ^ swz<2>(i / QP))) = wv[k];
     }
   }
 }
@@ -200,8 +224,7 @@ __global__ __launch_bounds__(256) void k_conv(const uint16_t* __restrict__ x, co
   // Epilogue through LDS: each wave writes its 64 pixels x NT*16 channels
   // (bias and ReLU applied) as fp32 rows, then stores whole 16-byte pieces of
   // consecutive channels, instead of 2-byte values scattered over 4 pixels.
-  constexpr int OS = NT * 16 + 4;  // fp32 row stride: the 4 lane groups' rows land in distinct banks
-  static_assert(4 * 64 * OS * sizeof(float) <= conv_lds_bytes<NT>(), "output staging must fit the LDS tile");
+  constexpr int OS = conv_out_stride<NT>();
   __syncthreads();  // every wave is done with the staged halo and weights
   float* stage = reinterpret_cast<float*>(lds4) + wave * 64 * OS;
 #pragma unroll
@@ -281,8 +304,7 @@ __global__ __launch_bounds__(256) void k_conv(const uint16_t* __restrict__ x, co
 // write LDS directly (no registers), while the waves run the MFMA steps of
 // stage s; border pixels are read at an offset past the buffer's range and
 // land as zeros. The halo has no padding: piece q (16 bytes) of pixel p sits
-// at slot p * 2G + (q ^ swz<G>(p)), which spreads 16 consecutive pixels'
-// reads of one piece over the banks of the 64-bank LDS. Blocks are dealt
+// at slot p * 2G + (q ^ swz<G>(p)), and the weights likewise by row. Blocks are dealt
 // round-robin to the 8 XCDs; each XCD's blocks walk one contiguous eighth of
 // the tiles, so tiles that share halo rows are read through the same L2.
 typedef __attribute__((address_space(3))) void lds_void;
@@ -332,11 +354,6 @@ constexpr int pipe_row_pieces() {  // 16-byte pieces of one output pixel
   return OUT_F32 ? NT * 4 : NT * 2;
 }
 constexpr int pipe_pool_per_thread(int nt) { return (2 * 32 * nt * 2 + 255) / 256; }
-
-template <int G>
-__device__ __forceinline__ int swz(int p) {
-  return G == 2 ? (p >> 2) & 3 : (p >> 3) & 1;
-}
 
 // Chunks of a layer's input: channels [0, ca) of the half-resolution part
 // first, then [ca, cin) of the full-resolution part, each in runs of 16*GMAX
