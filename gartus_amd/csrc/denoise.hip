@@ -724,14 +724,32 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
 // The four waves' sums are added in wave order through LDS, and the centre
 // tap's waves of the first ci group also sum dPre against ones (the bias).
 constexpr int kWR = 4, kWC = 64, kWX = kWC + 16;
-constexpr int kSD = kWR * kWC + 8;        // dPre channel stride (elements): an odd number of 16-byte chunks
-constexpr int kSX = (kWR + 2) * kWX + 8;  // x channel stride, likewise
+// Two stagings. With 2 x 2 fragments a thread transposes 4 pixels x 8
+// channels in registers and writes 8-byte rows, and the channel strides are
+// 34 and 62 16-byte pieces (an odd multiple of 2), which puts the 16 lanes of
+// one LDS read pass (channels l & 15 at one or two 8-pixel offsets) in
+// distinct bank groups. The smaller variants, whose time goes more to
+// staging than to MFMA steps, keep coalesced 16-byte loads of one pixel's
+// channels and 2-byte stores, with strides of 33 and 61 pieces.
+template <int NCO, int NCI>
+constexpr bool wgrad_tr4() {
+  return NCO == 2 && NCI == 2;
+}
+template <int NCO, int NCI>
+constexpr int wgrad_sd() {  // dPre channel stride (elements)
+  return kWR * kWC + (wgrad_tr4<NCO, NCI>() ? 16 : 8);
+}
+template <int NCO, int NCI>
+constexpr int wgrad_sx() {  // x channel stride (elements)
+  return (kWR + 2) * kWX + (wgrad_tr4<NCO, NCI>() ? 16 : 8);
+}
 
 constexpr size_t kWRed = 3 * 36 * 64 * sizeof(float);  // waves 1..3 hand wave 0 one (a, b) pair of sums
 
 template <int NCO, int NCI>
 constexpr size_t wgrad_lds_bytes() {
-  return std::max(size_t(NCO * 16 * kSD + NCI * 16 * kSX) * sizeof(uint16_t), kWRed);
+  return std::max(size_t(NCO * 16 * wgrad_sd<NCO, NCI>() + NCI * 16 * wgrad_sx<NCO, NCI>()) * sizeof(uint16_t),
+                  kWRed);
 }
 
 // Elements [1, 9) of the 16 bf16 (lo, hi), or [-1, 7) of (hi) preceded by the
@@ -747,6 +765,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
                                                const uint16_t* __restrict__ dy, const uint16_t* __restrict__ mask,
                                                int cout, int H, int W, int nbands, int ncog, int ci_first,
                                                float* __restrict__ partial) {
+  constexpr bool kTR4 = wgrad_tr4<NCO, NCI>();
+  constexpr int kSD = wgrad_sd<NCO, NCI>(), kSX = wgrad_sx<NCO, NCI>();
   extern __shared__ uint4 wlds4[];
   uint16_t* dT = reinterpret_cast<uint16_t*>(wlds4);
   uint16_t* xT = dT + NCO * 16 * kSD;
@@ -767,42 +787,105 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
   const uint4 ones = make_uint4(0x3f803f80u, 0x3f803f80u, 0x3f803f80u, 0x3f803f80u);  // bf16 1.0 x 8
   for (int c0 = 0; c0 < W; c0 += kWC) {
     __syncthreads();  // the previous tile's reads are done
-    // dPre, transposed: 16-byte loads of 8 channels, 2-byte stores
-    constexpr int DQ = NCO * 2;  // 8-channel pieces per pixel
-    for (int i = tid; i < kWR * kWC * DQ; i += 256) {
-      const int q = i % DQ, px = i / DQ, col = px % kWC, row = px / kWC;
-      const int h = r0 + row, w = c0 + col;
-      uint4 v = make_uint4(0, 0, 0, 0);
-      if (h < H && w < W) {
-        const size_t o = ((size_t(n) * H + h) * W + w) * cout + co0 + 8 * q;
-        v = *reinterpret_cast<const uint4*>(dy + o);
-        if (mask) v = mask8(v, *reinterpret_cast<const uint4*>(mask + o));
-      }
-      const uint32_t e[4] = {v.x, v.y, v.z, v.w};
-      uint16_t* dst = dT + (8 * q) * kSD + row * kWC + col;
+    if constexpr (kTR4) {
+      // dPre and x, transposed so that consecutive pixels of one channel are
+      // adjacent: a thread loads 4 consecutive pixels of one 8-channel piece,
+      // transposes the 4 x 8 block in registers and writes 8 channel rows of 8
+      // bytes. Items: dPre first (64 groups of 4 pixels per piece), then x (128
+      // slots per piece, 120 used: kWR + 2 rows of kWX / 4 groups). The group
+      // runs fastest, so that one write instruction covers 256 consecutive
+      // bytes of one channel row.
+      constexpr int DQ = NCO * 2, XQ = NCI * 2;  // 8-channel pieces per pixel
+      constexpr int ND = 64 * DQ, NX = 128 * XQ, XG = kWX / 4;
+      for (int i = tid; i < ND + NX; i += 256) {  // ND is a multiple of 64: the branch is uniform per wave
+        uint4 v[4];
+        uint16_t* dst;
+        int stride;
+        if (i < ND) {
+          const int q = i >> 6, g = i & 63, row = g >> 4, col = (g & 15) * 4;
+          const int h = r0 + row;
+          const size_t o0 = ((size_t(n) * H + h) * W + c0 + col) * cout + co0 + 8 * q;
 #pragma unroll
-      for (int k = 0; k < 4; ++k) {
-        dst[(2 * k) * kSD] = uint16_t(e[k]);
-        dst[(2 * k + 1) * kSD] = uint16_t(e[k] >> 16);
-      }
-    }
-    // x with a one-row halo and 8 columns each side, zero outside the image
-    constexpr int XQ = NCI * 2;
-    for (int i = tid; i < (kWR + 2) * kWX * XQ; i += 256) {
-      const int q = i % XQ, px = i / XQ, col = px % kWX, row = px / kWX;
-      const int h = r0 - 1 + row, w = c0 - 8 + col;
-      uint4 v = make_uint4(0, 0, 0, 0);
-      if (h >= 0 && h < H && w >= 0 && w < W) {
-        const int c = ci0 + 8 * q;  // channels below ca come from xa at half resolution (nearest)
-        v = c < ca ? *reinterpret_cast<const uint4*>(xa + ((size_t(n) * (H / 2) + h / 2) * (W / 2) + w / 2) * ca + c)
-                   : *reinterpret_cast<const uint4*>(x + ((size_t(n) * H + h) * W + w) * (cin - ca) + (c - ca));
-      }
-      const uint32_t e[4] = {v.x, v.y, v.z, v.w};
-      uint16_t* dst = xT + (8 * q) * kSX + row * kWX + col;
+          for (int k = 0; k < 4; ++k) {
+            v[k] = make_uint4(0, 0, 0, 0);
+            if (h < H && c0 + col + k < W) {
+              const size_t o = o0 + size_t(k) * cout;
+              v[k] = *reinterpret_cast<const uint4*>(dy + o);
+              if (mask) v[k] = mask8(v[k], *reinterpret_cast<const uint4*>(mask + o));
+            }
+          }
+          dst = dT + (8 * q) * kSD + row * kWC + col;
+          stride = kSD;
+        } else {
+          const int j = i - ND, q = j >> 7, g = j & 127;
+          if (g >= (kWR + 2) * XG) continue;
+          const int row = g / XG, col = (g % XG) * 4;
+          const int h = r0 - 1 + row, c = ci0 + 8 * q;  // channels below ca come from xa at half resolution (nearest)
 #pragma unroll
-      for (int k = 0; k < 4; ++k) {
-        dst[(2 * k) * kSX] = uint16_t(e[k]);
-        dst[(2 * k + 1) * kSX] = uint16_t(e[k] >> 16);
+          for (int k = 0; k < 4; ++k) {
+            const int w = c0 - 8 + col + k;
+            v[k] = make_uint4(0, 0, 0, 0);
+            if (h >= 0 && h < H && w >= 0 && w < W)
+              v[k] = c < ca
+                         ? *reinterpret_cast<const uint4*>(xa + ((size_t(n) * (H / 2) + h / 2) * (W / 2) + w / 2) * ca + c)
+                         : *reinterpret_cast<const uint4*>(x + ((size_t(n) * H + h) * W + w) * (cin - ca) + (c - ca));
+          }
+          dst = xT + (8 * q) * kSX + row * kWX + col;
+          stride = kSX;
+        }
+        // channel e of pixel k is half e & 1 of word e >> 1 of v[k]
+#pragma unroll
+        for (int e = 0; e < 8; ++e) {
+          const uint32_t sel = (e & 1) ? 0x07060302u : 0x05040100u;
+          uint32_t t[2];
+#pragma unroll
+          for (int kp = 0; kp < 2; ++kp) {
+            const uint4 lo = v[2 * kp], hi = v[2 * kp + 1];
+            const uint32_t a = (e >> 1) == 0 ? lo.x : (e >> 1) == 1 ? lo.y : (e >> 1) == 2 ? lo.z : lo.w;
+            const uint32_t b = (e >> 1) == 0 ? hi.x : (e >> 1) == 1 ? hi.y : (e >> 1) == 2 ? hi.z : hi.w;
+            t[kp] = __builtin_amdgcn_perm(b, a, sel);  // pixel 2kp in the low half, 2kp + 1 in the high half
+          }
+          *reinterpret_cast<uint2*>(dst + e * stride) = make_uint2(t[0], t[1]);
+        }
+      }
+    } else {
+      // dPre, transposed: 16-byte loads of 8 channels, 2-byte stores
+      constexpr int DQ = NCO * 2;  // 8-channel pieces per pixel
+      for (int i = tid; i < kWR * kWC * DQ; i += 256) {
+        const int q = i % DQ, px = i / DQ, col = px % kWC, row = px / kWC;
+        const int h = r0 + row, w = c0 + col;
+        uint4 v = make_uint4(0, 0, 0, 0);
+        if (h < H && w < W) {
+          const size_t o = ((size_t(n) * H + h) * W + w) * cout + co0 + 8 * q;
+          v = *reinterpret_cast<const uint4*>(dy + o);
+          if (mask) v = mask8(v, *reinterpret_cast<const uint4*>(mask + o));
+        }
+        const uint32_t e[4] = {v.x, v.y, v.z, v.w};
+        uint16_t* dst = dT + (8 * q) * kSD + row * kWC + col;
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+          dst[(2 * k) * kSD] = uint16_t(e[k]);
+          dst[(2 * k + 1) * kSD] = uint16_t(e[k] >> 16);
+        }
+      }
+      // x with a one-row halo and 8 columns each side, zero outside the image
+      constexpr int XQ = NCI * 2;
+      for (int i = tid; i < (kWR + 2) * kWX * XQ; i += 256) {
+        const int q = i % XQ, px = i / XQ, col = px % kWX, row = px / kWX;
+        const int h = r0 - 1 + row, w = c0 - 8 + col;
+        uint4 v = make_uint4(0, 0, 0, 0);
+        if (h >= 0 && h < H && w >= 0 && w < W) {
+          const int c = ci0 + 8 * q;  // channels below ca come from xa at half resolution (nearest)
+          v = c < ca ? *reinterpret_cast<const uint4*>(xa + ((size_t(n) * (H / 2) + h / 2) * (W / 2) + w / 2) * ca + c)
+                     : *reinterpret_cast<const uint4*>(x + ((size_t(n) * H + h) * W + w) * (cin - ca) + (c - ca));
+        }
+        const uint32_t e[4] = {v.x, v.y, v.z, v.w};
+        uint16_t* dst = xT + (8 * q) * kSX + row * kWX + col;
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+          dst[(2 * k) * kSX] = uint16_t(e[k]);
+          dst[(2 * k + 1) * kSX] = uint16_t(e[k] >> 16);
+        }
       }
     }
     __syncthreads();
