@@ -759,6 +759,65 @@ __device__ __forceinline__ uint4 shift_up(uint4 lo, uint4 hi) {  // elements 1..
                     __builtin_amdgcn_alignbyte(lo.w, lo.z, 2), __builtin_amdgcn_alignbyte(hi.x, lo.w, 2));
 }
 
+// The end of a weight-gradient block: waves 1..3 hand their sums to wave 0
+// through LDS (red, kWRed bytes), one (a, b) pair at a time, in wave order,
+// and wave 0 writes the block's row of partial sums (out: weights, then the
+// bias at nw). The caller has synchronised the block after its last LDS read.
+template <int NCO, int NCI>
+__device__ __forceinline__ void wgrad_finish(float* red, v4f (&acc)[9][NCO][NCI], v4f (&bacc)[NCO], bool bias,
+                                             float* __restrict__ out, size_t nw, int cin, int co0, int ci0, int wave,
+                                             int lane) {
+#pragma unroll
+  for (int a = 0; a < NCO; ++a)
+#pragma unroll
+    for (int b = 0; b < NCI; ++b) {
+      __syncthreads();
+      if (wave > 0) {
+#pragma unroll
+        for (int t = 0; t < 9; ++t)
+#pragma unroll
+          for (int r = 0; r < 4; ++r) red[((wave - 1) * 36 + t * 4 + r) * 64 + lane] = acc[t][a][b][r];
+      }
+      __syncthreads();
+      if (wave == 0) {
+#pragma unroll
+        for (int t = 0; t < 9; ++t)
+#pragma unroll
+          for (int r = 0; r < 4; ++r) {
+            float v = acc[t][a][b][r];
+            v += red[(0 * 36 + t * 4 + r) * 64 + lane];
+            v += red[(1 * 36 + t * 4 + r) * 64 + lane];
+            v += red[(2 * 36 + t * 4 + r) * 64 + lane];
+            const int co = co0 + a * 16 + (lane >> 4) * 4 + r;
+            const int ci = ci0 + b * 16 + (lane & 15);
+            out[(size_t(co) * 9 + t) * cin + ci] = v;
+          }
+      }
+    }
+  if (bias) {
+    __syncthreads();
+    if (wave > 0) {
+#pragma unroll
+      for (int a = 0; a < NCO; ++a)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) red[((wave - 1) * 4 * NCO + a * 4 + r) * 64 + lane] = bacc[a][r];
+    }
+    __syncthreads();
+    if (wave == 0 && (lane & 15) == 0) {  // column 0 of the all-ones product
+#pragma unroll
+      for (int a = 0; a < NCO; ++a)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          float v = bacc[a][r];
+          v += red[((0 * NCO + a) * 4 + r) * 64 + lane];
+          v += red[((1 * NCO + a) * 4 + r) * 64 + lane];
+          v += red[((2 * NCO + a) * 4 + r) * 64 + lane];
+          out[nw + co0 + a * 16 + (lane >> 4) * 4 + r] = v;
+        }
+    }
+  }
+}
+
 template <int NCO, int NCI>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k_wgrad(const uint16_t* __restrict__ x, int cin,
                                                const uint16_t* __restrict__ xa, int ca,
@@ -925,58 +984,162 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
     }
   }
   // waves 1..3 hand their sums to wave 0 through LDS, one (a, b) pair at a time
-  float* red = reinterpret_cast<float*>(wlds4);
   const size_t nw = size_t(cout) * 9 * cin;
-  float* out = partial + size_t(blockIdx.x) * (nw + cout);  // this split's row: weights, then bias
+  wgrad_finish<NCO, NCI>(reinterpret_cast<float*>(wlds4), acc, bacc, bias,
+                         partial + size_t(blockIdx.x) * (nw + cout), nw, cin, co0, ci0, wave, lane);
+}
+
+// ---- convolution, weight gradient from untransposed tiles ------------------
+// The sums of k_wgrad, for a dy that needs no mask. dPre (kWR rows x kWC
+// columns) and x (kWR + 2 rows x kWXC columns: one beyond the tile on each
+// side) lie in LDS as they lie in HBM, one 32-byte row of 16 channels per
+// pixel, in one region per 16-channel fragment, copied straight from global
+// memory by buffer loads that write LDS (each region from its own tensor, so
+// a block's channels may straddle the decoder concatenation). With NBUF = 2
+// the next tile's copy runs while the waves work on this one. The MFMA
+// operands come from ds_read_b64_tr_b16: over 16 lanes, lane l names 4
+// channels of pixel row (l >> 2) & 3 of a 4 x 16 block and receives channel
+// l's 4 pixels, so two reads give a lane its 8 K values (pixels) and a column
+// tap is an offset of the pixel index. Slots are swizzled by bit 3 of the
+// pixel, which puts the two 16-lane groups of a read pass (pixels 8 apart) in
+// different banks.
+constexpr int kWXC = kWC + 2;
+constexpr int kWDS = (kWR * kWC * 2 + 63) / 64 * 64;         // 16-byte slots of one dPre fragment region
+constexpr int kWXS = ((kWR + 2) * kWXC * 2 + 63) / 64 * 64;  // ... of one x fragment region
+
+__device__ __forceinline__ int wgt_slot(int p, int k) {  // piece k (0, 1) of pixel p
+  return (2 * p + k) ^ (((p >> 3) & 1) << 3);
+}
+__device__ __forceinline__ int wgt_pixel(int slot) {  // the pixel of a slot (wgt_slot is its own inverse on bit 3)
+  const int q = slot >> 1;
+  return q ^ (((q >> 3) & 1) << 2);
+}
+
+template <int NCO, int NCI, int NBUF>
+constexpr size_t wgrad_tr_lds_bytes() {
+  return std::max(size_t(NBUF) * (NCO * kWDS + NCI * kWXS) * 16, kWRed);
+}
+
+typedef uint32_t u32x2 __attribute__((ext_vector_type(2)));
+typedef short v4s __attribute__((ext_vector_type(4)));
+
+// The 8 K values of this lane for a 16 x 32 operand whose K runs over the
+// pixels p0 + 0 .. 31 of a fragment region (byte address base).
+__device__ __forceinline__ uint4 wgt_operand(const char* base, int p0, int lane) {
+  const int g = lane >> 4, r = (lane >> 2) & 3, c = lane & 3;
+  u32x2 v[2];
 #pragma unroll
-  for (int a = 0; a < NCO; ++a)
+  for (int h = 0; h < 2; ++h) {
+    const int p = p0 + 8 * g + 4 * h + r;
+    const char* a = base + wgt_slot(p, c >> 1) * 16 + (c & 1) * 8;
+    v[h] = __builtin_bit_cast(u32x2, __builtin_amdgcn_ds_read_tr16_b64_v4i16(
+                                         (__attribute__((address_space(3))) v4s*)((const lds_void*)a)));
+  }
+  return make_uint4(v[0].x, v[0].y, v[1].x, v[1].y);
+}
+
+template <int NCO, int NCI, int NBUF>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k_wgrad_tr(
+    const uint16_t* __restrict__ x, int cin, const uint16_t* __restrict__ xa, int ca, const uint16_t* __restrict__ dy,
+    int cout, int N, int H, int W, int nbands, int ncog, int ci_first, float* __restrict__ partial) {
+  constexpr int BUF = NCO * kWDS + NCI * kWXS;  // slots of one buffer: dPre regions, then x regions
+  extern __shared__ uint4 wlds4[];
+  const int tid = threadIdx.x, lane = tid & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+  const int n = blockIdx.x / nbands, r0 = (blockIdx.x % nbands) * kWR;
+  const int co0 = (blockIdx.y % ncog) * NCO * 16, ci0 = ci_first + (blockIdx.y / ncog) * NCI * 16;
+  const bool bias = ci0 == 0;  // uniform over the block
+  const bool row_in = r0 + wave < H;
+  const int cx = cin - ca;
+  const i32x4 rd = buffer_rsrc(dy, uint32_t(size_t(N) * H * W * cout * 2));
+  const i32x4 rx = buffer_rsrc(x, uint32_t(size_t(N) * H * W * cx * 2));
+  const i32x4 ra = buffer_rsrc(xa ? xa : x, xa ? uint32_t(size_t(N) * (H / 2) * (W / 2) * ca * 2) : 0u);
+  v4f acc[9][NCO][NCI], bacc[NCO];
+#pragma unroll
+  for (int a = 0; a < NCO; ++a) {
+    bacc[a] = v4f{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+    for (int t = 0; t < 9; ++t)
+#pragma unroll
+      for (int b = 0; b < NCI; ++b) acc[t][a][b] = v4f{0.f, 0.f, 0.f, 0.f};
+  }
+  // the copies of the tile at column c0 into buffer buf: wave w issues slots [64 i, 64 i + 64) of
+  // each region for i = w, w + 4, ...
+  auto copy = [&](uint4* buf, int c0) {
+#pragma unroll
+    for (int a = 0; a < NCO; ++a) {
+#pragma unroll
+      for (int i0 = 0; i0 < kWDS / 64; i0 += 4) {
+        const int i = i0 + wave, sl = i * 64 + lane, p = wgt_pixel(sl);
+        const int h = r0 + p / kWC, w = c0 + p % kWC;
+        uint32_t off = kOutOfRange;
+        if (p < kWR * kWC && h < H && w < W)
+          off = uint32_t((((size_t(n) * H + h) * W + w) * cout + co0 + 16 * a + 8 * (sl & 1)) * 2);
+        lds_copy16(buf + a * kWDS + i * 64, rd, off);
+      }
+    }
 #pragma unroll
     for (int b = 0; b < NCI; ++b) {
-      __syncthreads();
-      if (wave > 0) {
+      const int c = ci0 + 16 * b;  // the fragment's first channel: below ca it comes from xa at half resolution
+      const bool half_res = c < ca;
 #pragma unroll
-        for (int t = 0; t < 9; ++t)
-#pragma unroll
-          for (int r = 0; r < 4; ++r) red[((wave - 1) * 36 + t * 4 + r) * 64 + lane] = acc[t][a][b][r];
-      }
-      __syncthreads();
-      if (wave == 0) {
-#pragma unroll
-        for (int t = 0; t < 9; ++t)
-#pragma unroll
-          for (int r = 0; r < 4; ++r) {
-            float v = acc[t][a][b][r];
-            v += red[(0 * 36 + t * 4 + r) * 64 + lane];
-            v += red[(1 * 36 + t * 4 + r) * 64 + lane];
-            v += red[(2 * 36 + t * 4 + r) * 64 + lane];
-            const int co = co0 + a * 16 + (lane >> 4) * 4 + r;
-            const int ci = ci0 + b * 16 + (lane & 15);
-            out[(size_t(co) * 9 + t) * cin + ci] = v;
-          }
-      }
-    }
-  if (bias) {
-    __syncthreads();
-    if (wave > 0) {
-#pragma unroll
-      for (int a = 0; a < NCO; ++a)
-#pragma unroll
-        for (int r = 0; r < 4; ++r) red[((wave - 1) * 4 * NCO + a * 4 + r) * 64 + lane] = bacc[a][r];
-    }
-    __syncthreads();
-    if (wave == 0 && (lane & 15) == 0) {  // column 0 of the all-ones product
-#pragma unroll
-      for (int a = 0; a < NCO; ++a)
-#pragma unroll
-        for (int r = 0; r < 4; ++r) {
-          float v = bacc[a][r];
-          v += red[((0 * NCO + a) * 4 + r) * 64 + lane];
-          v += red[((1 * NCO + a) * 4 + r) * 64 + lane];
-          v += red[((2 * NCO + a) * 4 + r) * 64 + lane];
-          out[nw + co0 + a * 16 + (lane >> 4) * 4 + r] = v;
+      for (int i0 = 0; i0 < kWXS / 64; i0 += 4) {
+        const int i = i0 + wave;
+        if (i < kWXS / 64) {
+          const int sl = i * 64 + lane, p = wgt_pixel(sl);
+          const int h = r0 - 1 + p / kWXC, w = c0 - 1 + p % kWXC;
+          uint32_t off = kOutOfRange;
+          if (p < (kWR + 2) * kWXC && h >= 0 && h < H && w >= 0 && w < W)
+            off = half_res ? uint32_t((((size_t(n) * (H / 2) + h / 2) * (W / 2) + w / 2) * ca + c + 8 * (sl & 1)) * 2)
+                           : uint32_t((((size_t(n) * H + h) * W + w) * cx + c - ca + 8 * (sl & 1)) * 2);
+          lds_copy16(buf + NCO * kWDS + b * kWXS + i * 64, half_res ? ra : rx, off);
         }
+      }
+    }
+  };
+  const uint4 ones = make_uint4(0x3f803f80u, 0x3f803f80u, 0x3f803f80u, 0x3f803f80u);  // bf16 1.0 x 8
+  const int ntiles = (W + kWC - 1) / kWC;
+  if (NBUF == 2) copy(wlds4, 0);
+  for (int t = 0; t < ntiles; ++t) {
+    uint4* cur = wlds4 + (NBUF == 2 ? (t & 1) * BUF : 0);
+    if (NBUF == 2) {
+      __builtin_amdgcn_s_waitcnt(kWaitVm0);  // this wave's copies of tile t are done
+      lds_barrier();                        // ... and every wave's; the other buffer's reads are done
+      if (t + 1 < ntiles) copy(wlds4 + ((t + 1) & 1) * BUF, (t + 1) * kWC);
+    } else {
+      lds_barrier();  // the previous tile's reads are done
+      copy(cur, t * kWC);
+      __builtin_amdgcn_s_waitcnt(kWaitVm0);
+      lds_barrier();
+    }
+    if (!row_in) continue;  // this wave's row is past the image: its dPre is zero
+    const char* dT = reinterpret_cast<const char*>(cur);
+    const char* xT = reinterpret_cast<const char*>(cur + NCO * kWDS);
+#pragma unroll
+    for (int kc = 0; kc < kWC / 32; ++kc) {
+      uint4 A[NCO];
+#pragma unroll
+      for (int a = 0; a < NCO; ++a) A[a] = wgt_operand(dT + a * kWDS * 16, wave * kWC + 32 * kc, lane);
+#pragma unroll
+      for (int b = 0; b < NCI; ++b) {
+#pragma unroll
+        for (int tap = 0; tap < 9; ++tap) {
+          const int dr = tap / 3, dx = tap % 3 - 1;
+          const uint4 B = wgt_operand(xT + b * kWXS * 16, (wave + dr) * kWXC + 32 * kc + 1 + dx, lane);
+#pragma unroll
+          for (int a = 0; a < NCO; ++a) acc[tap][a][b] = mfma32(A[a], B, acc[tap][a][b]);
+        }
+      }
+      if (bias) {
+#pragma unroll
+        for (int a = 0; a < NCO; ++a) bacc[a] = mfma32(A[a], ones, bacc[a]);
+      }
     }
   }
+  __syncthreads();  // every wave is done with the tiles
+  const size_t nw = size_t(cout) * 9 * cin;
+  wgrad_finish<NCO, NCI>(reinterpret_cast<float*>(wlds4), acc, bacc, bias,
+                         partial + size_t(blockIdx.x) * (nw + cout), nw, cin, co0, ci0, wave, lane);
 }
 
 // out[i] = sum over s < splits of partial[s * stride + i], i < n. A block of 256 threads
@@ -1369,6 +1532,20 @@ void launch_conv(hipStream_t s, const uint16_t* x, const uint16_t* mask, int cin
     launch_groups<DGRAD, F32>(s, rest, full * nt, 1, x, mask, cin, xa, ca, w, bias, y, cout, d, relu, pooled, omask);
 }
 
+template <int NCO, int NCI, int NBUF>
+void launch_wgrad_tr(hipStream_t s, const uint16_t* x, int cin, const uint16_t* xa, int ca, const uint16_t* dy,
+                     int cout, Dims d, float* partial, int ci_first, int ncig) {
+  constexpr size_t lds = wgrad_tr_lds_bytes<NCO, NCI, NBUF>();
+  static const hipError_t attr =  // once per instantiation: allow more than 64 KiB of LDS
+      hipFuncSetAttribute(reinterpret_cast<const void*>(&k_wgrad_tr<NCO, NCI, NBUF>),
+                          hipFuncAttributeMaxDynamicSharedMemorySize, int(lds));
+  (void)attr;
+  const int ncog = cout / (NCO * 16);
+  const int nbands = (d.h + kWR - 1) / kWR;
+  hipLaunchKernelGGL((k_wgrad_tr<NCO, NCI, NBUF>), dim3(d.n * nbands, ncog * ncig), dim3(256), lds, s, x, cin, xa, ca,
+                     dy, cout, d.n, d.h, d.w, nbands, ncog, ci_first, partial);
+}
+
 // Weight-gradient splits: one per band of kWR rows of one image.
 int wgrad_splits(Dims d) { return d.n * ((d.h + kWR - 1) / kWR); }
 
@@ -1426,11 +1603,24 @@ void conv_wgrad(hipStream_t s, const uint16_t* x, int cin, const uint16_t* dy, c
   // fewer blocks stage the same dPre and x tiles; an odd last input fragment takes a launch of its
   // own (it writes its own columns of the same partial rows).
   const int pairs = cin / 32, odd = (cin / 16) % 2;
+  // Unmasked tensors within 31-bit buffer offsets take k_wgrad_tr (single-buffered for 2 x 2
+  // fragments: two buffers would take 86 KB of LDS and leave one block per CU), except 1 x 1,
+  // which measured faster on k_wgrad's coalesced 2-byte staging.
+  const bool tr = !y && size_t(d.n) * d.h * d.w * std::max(cout, cin) * 2 < kOutOfRange;
   if ((cout / 16) % 2 == 0) {
-    if (pairs) launch_wgrad<2, 2>(s, x, cin, xa, ca, dy, y, cout, d, scratch, 0, pairs);
-    if (odd) launch_wgrad<2, 1>(s, x, cin, xa, ca, dy, y, cout, d, scratch, 32 * pairs, 1);
+    if (pairs) {
+      if (tr) launch_wgrad_tr<2, 2, 1>(s, x, cin, xa, ca, dy, cout, d, scratch, 0, pairs);
+      else launch_wgrad<2, 2>(s, x, cin, xa, ca, dy, y, cout, d, scratch, 0, pairs);
+    }
+    if (odd) {
+      if (tr) launch_wgrad_tr<2, 1, 2>(s, x, cin, xa, ca, dy, cout, d, scratch, 32 * pairs, 1);
+      else launch_wgrad<2, 1>(s, x, cin, xa, ca, dy, y, cout, d, scratch, 32 * pairs, 1);
+    }
   } else {
-    if (pairs) launch_wgrad<1, 2>(s, x, cin, xa, ca, dy, y, cout, d, scratch, 0, pairs);
+    if (pairs) {
+      if (tr) launch_wgrad_tr<1, 2, 2>(s, x, cin, xa, ca, dy, cout, d, scratch, 0, pairs);
+      else launch_wgrad<1, 2>(s, x, cin, xa, ca, dy, y, cout, d, scratch, 0, pairs);
+    }
     if (odd) launch_wgrad<1, 1>(s, x, cin, xa, ca, dy, y, cout, d, scratch, 32 * pairs, 1);
   }
   const size_t row = nw + cout;  // a split's partial sums: weights, then bias
