@@ -990,22 +990,27 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
 }
 
 // ---- convolution, weight gradient from untransposed tiles ------------------
-// The sums of k_wgrad, for a dy that needs no mask. dPre (kWR rows x kWC
-// columns) and x (kWR + 2 rows x kWXC columns: one beyond the tile on each
+// The sums of k_wgrad, for a dy that needs no mask. dPre (kWR rows x TC
+// columns) and x (kWR + 2 rows x TC + 2 columns: one beyond the tile on each
 // side) lie in LDS as they lie in HBM, one 32-byte row of 16 channels per
 // pixel, in one region per 16-channel fragment, copied straight from global
 // memory by buffer loads that write LDS (each region from its own tensor, so
-// a block's channels may straddle the decoder concatenation). With NBUF = 2
-// the next tile's copy runs while the waves work on this one. The MFMA
+// a block's channels may straddle the decoder concatenation), in tiles of TC
+// columns; the next tile's copy runs while the waves work on this one. The MFMA
 // operands come from ds_read_b64_tr_b16: over 16 lanes, lane l names 4
 // channels of pixel row (l >> 2) & 3 of a 4 x 16 block and receives channel
 // l's 4 pixels, so two reads give a lane its 8 K values (pixels) and a column
 // tap is an offset of the pixel index. Slots are swizzled by bit 3 of the
 // pixel, which puts the two 16-lane groups of a read pass (pixels 8 apart) in
 // different banks.
-constexpr int kWXC = kWC + 2;
-constexpr int kWDS = (kWR * kWC * 2 + 63) / 64 * 64;         // 16-byte slots of one dPre fragment region
-constexpr int kWXS = ((kWR + 2) * kWXC * 2 + 63) / 64 * 64;  // ... of one x fragment region
+template <int TC>  // tile columns
+constexpr int wgt_dslots() {  // 16-byte slots of one dPre fragment region
+  return (kWR * TC * 2 + 63) / 64 * 64;
+}
+template <int TC>
+constexpr int wgt_xslots() {  // ... of one x fragment region (TC + 2 columns)
+  return ((kWR + 2) * (TC + 2) * 2 + 63) / 64 * 64;
+}
 
 __device__ __forceinline__ int wgt_slot(int p, int k) {  // piece k (0, 1) of pixel p
   return (2 * p + k) ^ (((p >> 3) & 1) << 3);
@@ -1015,9 +1020,9 @@ __device__ __forceinline__ int wgt_pixel(int slot) {  // the pixel of a slot (wg
   return q ^ (((q >> 3) & 1) << 2);
 }
 
-template <int NCO, int NCI, int NBUF>
-constexpr size_t wgrad_tr_lds_bytes() {
-  return std::max(size_t(NBUF) * (NCO * kWDS + NCI * kWXS) * 16, kWRed);
+template <int NCO, int NCI, int TC>
+constexpr size_t wgrad_tr_lds_bytes() {  // two buffers
+  return std::max(size_t(2) * (NCO * wgt_dslots<TC>() + NCI * wgt_xslots<TC>()) * 16, kWRed);
 }
 
 typedef uint32_t u32x2 __attribute__((ext_vector_type(2)));
@@ -1038,10 +1043,11 @@ __device__ __forceinline__ uint4 wgt_operand(const char* base, int p0, int lane)
   return make_uint4(v[0].x, v[0].y, v[1].x, v[1].y);
 }
 
-template <int NCO, int NCI, int NBUF>
+template <int NCO, int NCI, int TC>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k_wgrad_tr(
     const uint16_t* __restrict__ x, int cin, const uint16_t* __restrict__ xa, int ca, const uint16_t* __restrict__ dy,
     int cout, int N, int H, int W, int nbands, int ncog, int ci_first, float* __restrict__ partial) {
+  constexpr int kWDS = wgt_dslots<TC>(), kWXS = wgt_xslots<TC>(), kWXC = TC + 2;
   constexpr int BUF = NCO * kWDS + NCI * kWXS;  // slots of one buffer: dPre regions, then x regions
   extern __shared__ uint4 wlds4[];
   const int tid = threadIdx.x, lane = tid & 63;
@@ -1071,9 +1077,9 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
 #pragma unroll
       for (int i0 = 0; i0 < kWDS / 64; i0 += 4) {
         const int i = i0 + wave, sl = i * 64 + lane, p = wgt_pixel(sl);
-        const int h = r0 + p / kWC, w = c0 + p % kWC;
+        const int h = r0 + p / TC, w = c0 + p % TC;
         uint32_t off = kOutOfRange;
-        if (p < kWR * kWC && h < H && w < W)
+        if (p < kWR * TC && h < H && w < W)
           off = uint32_t((((size_t(n) * H + h) * W + w) * cout + co0 + 16 * a + 8 * (sl & 1)) * 2);
         lds_copy16(buf + a * kWDS + i * 64, rd, off);
       }
@@ -1098,28 +1104,21 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
     }
   };
   const uint4 ones = make_uint4(0x3f803f80u, 0x3f803f80u, 0x3f803f80u, 0x3f803f80u);  // bf16 1.0 x 8
-  const int ntiles = (W + kWC - 1) / kWC;
-  if (NBUF == 2) copy(wlds4, 0);
+  const int ntiles = (W + TC - 1) / TC;
+  copy(wlds4, 0);
   for (int t = 0; t < ntiles; ++t) {
-    uint4* cur = wlds4 + (NBUF == 2 ? (t & 1) * BUF : 0);
-    if (NBUF == 2) {
-      __builtin_amdgcn_s_waitcnt(kWaitVm0);  // this wave's copies of tile t are done
-      lds_barrier();                        // ... and every wave's; the other buffer's reads are done
-      if (t + 1 < ntiles) copy(wlds4 + ((t + 1) & 1) * BUF, (t + 1) * kWC);
-    } else {
-      lds_barrier();  // the previous tile's reads are done
-      copy(cur, t * kWC);
-      __builtin_amdgcn_s_waitcnt(kWaitVm0);
-      lds_barrier();
-    }
+    uint4* cur = wlds4 + (t & 1) * BUF;
+    __builtin_amdgcn_s_waitcnt(kWaitVm0);  // this wave's copies of tile t are done
+    lds_barrier();                        // ... and every wave's; the other buffer's reads are done
+    if (t + 1 < ntiles) copy(wlds4 + ((t + 1) & 1) * BUF, (t + 1) * TC);
     if (!row_in) continue;  // this wave's row is past the image: its dPre is zero
     const char* dT = reinterpret_cast<const char*>(cur);
     const char* xT = reinterpret_cast<const char*>(cur + NCO * kWDS);
 #pragma unroll
-    for (int kc = 0; kc < kWC / 32; ++kc) {
+    for (int kc = 0; kc < TC / 32; ++kc) {
       uint4 A[NCO];
 #pragma unroll
-      for (int a = 0; a < NCO; ++a) A[a] = wgt_operand(dT + a * kWDS * 16, wave * kWC + 32 * kc, lane);
+      for (int a = 0; a < NCO; ++a) A[a] = wgt_operand(dT + a * kWDS * 16, wave * TC + 32 * kc, lane);
 #pragma unroll
       for (int b = 0; b < NCI; ++b) {
 #pragma unroll
@@ -1532,17 +1531,17 @@ void launch_conv(hipStream_t s, const uint16_t* x, const uint16_t* mask, int cin
     launch_groups<DGRAD, F32>(s, rest, full * nt, 1, x, mask, cin, xa, ca, w, bias, y, cout, d, relu, pooled, omask);
 }
 
-template <int NCO, int NCI, int NBUF>
+template <int NCO, int NCI, int TC>
 void launch_wgrad_tr(hipStream_t s, const uint16_t* x, int cin, const uint16_t* xa, int ca, const uint16_t* dy,
                      int cout, Dims d, float* partial, int ci_first, int ncig) {
-  constexpr size_t lds = wgrad_tr_lds_bytes<NCO, NCI, NBUF>();
+  constexpr size_t lds = wgrad_tr_lds_bytes<NCO, NCI, TC>();
   static const hipError_t attr =  // once per instantiation: allow more than 64 KiB of LDS
-      hipFuncSetAttribute(reinterpret_cast<const void*>(&k_wgrad_tr<NCO, NCI, NBUF>),
+      hipFuncSetAttribute(reinterpret_cast<const void*>(&k_wgrad_tr<NCO, NCI, TC>),
                           hipFuncAttributeMaxDynamicSharedMemorySize, int(lds));
   (void)attr;
   const int ncog = cout / (NCO * 16);
   const int nbands = (d.h + kWR - 1) / kWR;
-  hipLaunchKernelGGL((k_wgrad_tr<NCO, NCI, NBUF>), dim3(d.n * nbands, ncog * ncig), dim3(256), lds, s, x, cin, xa, ca,
+  hipLaunchKernelGGL((k_wgrad_tr<NCO, NCI, TC>), dim3(d.n * nbands, ncog * ncig), dim3(256), lds, s, x, cin, xa, ca,
                      dy, cout, d.n, d.h, d.w, nbands, ncog, ci_first, partial);
 }
 
@@ -1603,25 +1602,28 @@ void conv_wgrad(hipStream_t s, const uint16_t* x, int cin, const uint16_t* dy, c
   // fewer blocks stage the same dPre and x tiles; an odd last input fragment takes a launch of its
   // own (it writes its own columns of the same partial rows).
   const int pairs = cin / 32, odd = (cin / 16) % 2;
-  // Unmasked tensors within 31-bit buffer offsets take k_wgrad_tr (single-buffered for 2 x 2
-  // fragments: two buffers would take 86 KB of LDS and leave one block per CU), except 1 x 1,
-  // which measured faster on k_wgrad's coalesced 2-byte staging.
+  // Unmasked tensors within 31-bit buffer offsets take k_wgrad_tr, double-buffered, in tiles of
+  // 32 columns (two blocks of 2 x 2 fragments fit a CU only at that size; 1 x 2 and 1 x 1 also
+  // measured faster) or 64 (2 x 1); the others k_wgrad.
   const bool tr = !y && size_t(d.n) * d.h * d.w * std::max(cout, cin) * 2 < kOutOfRange;
   if ((cout / 16) % 2 == 0) {
     if (pairs) {
-      if (tr) launch_wgrad_tr<2, 2, 1>(s, x, cin, xa, ca, dy, cout, d, scratch, 0, pairs);
+      if (tr) launch_wgrad_tr<2, 2, 32>(s, x, cin, xa, ca, dy, cout, d, scratch, 0, pairs);
       else launch_wgrad<2, 2>(s, x, cin, xa, ca, dy, y, cout, d, scratch, 0, pairs);
     }
     if (odd) {
-      if (tr) launch_wgrad_tr<2, 1, 2>(s, x, cin, xa, ca, dy, cout, d, scratch, 32 * pairs, 1);
+      if (tr) launch_wgrad_tr<2, 1, 64>(s, x, cin, xa, ca, dy, cout, d, scratch, 32 * pairs, 1);
       else launch_wgrad<2, 1>(s, x, cin, xa, ca, dy, y, cout, d, scratch, 32 * pairs, 1);
     }
   } else {
     if (pairs) {
-      if (tr) launch_wgrad_tr<1, 2, 2>(s, x, cin, xa, ca, dy, cout, d, scratch, 0, pairs);
+      if (tr) launch_wgrad_tr<1, 2, 32>(s, x, cin, xa, ca, dy, cout, d, scratch, 0, pairs);
       else launch_wgrad<1, 2>(s, x, cin, xa, ca, dy, y, cout, d, scratch, 0, pairs);
     }
-    if (odd) launch_wgrad<1, 1>(s, x, cin, xa, ca, dy, y, cout, d, scratch, 32 * pairs, 1);
+    if (odd) {
+      if (tr) launch_wgrad_tr<1, 1, 32>(s, x, cin, xa, ca, dy, cout, d, scratch, 32 * pairs, 1);
+      else launch_wgrad<1, 1>(s, x, cin, xa, ca, dy, y, cout, d, scratch, 32 * pairs, 1);
+    }
   }
   const size_t row = nw + cout;  // a split's partial sums: weights, then bias
   if (db == dw + nw) {  // the usual parameter layout: one reduction for both
