@@ -95,7 +95,7 @@ def kernel_layouts():
 
 
 def wgrad_layouts():
-    for kSD, kSX, when in ((272, 496, "now (2 x 2 fragments)"), (264, 488, "before; the smaller variants now")):
+    for kSD, kSX, when in ((272, 496, "2 x 2 fragments (masked or large tensors)"), (264, 488, "before")):
         dT0, xT0 = 0, 2 * 16 * kSD
         a_reads, x_reads = [], []
         for wave in range(4):
@@ -129,10 +129,32 @@ def wgrad_layouts():
     add("k_wgrad dPre staging, 4 x 8 register transposes, 8-byte stores, stride 272", 1, pats)
 
 
+def wgrad_tr_layouts():
+    """k_wgrad_tr's operand reads (ds_read_b64_tr_b16): lane l of 16-lane group g names pixel
+    p0 + 8 g + 4 h + (l >> 2) & 3 and 8-byte chunk l & 3 of a fragment region of 32-byte pixel rows,
+    slots swizzled by bit 3 of the pixel (and, for comparison, not swizzled)."""
+    swizzled = lambda p, k: (2 * p + k) ^ (((p >> 3) & 1) << 3)
+    plain = lambda p, k: 2 * p + k
+    for TC in (32, 64):
+        for name, slot in (("swizzled by pixel bit 3", swizzled), ("plain", plain)):
+            for what in ("dPre", "x"):
+                pats = []
+                taps = [(0, 0)] if what == "dPre" else [(dr, dx) for dr in range(3) for dx in (-1, 0, 1)]
+                for wave in range(4):
+                    for kc in range(TC // 32):
+                        for h in range(2):
+                            for dr, dx in taps:
+                                p0 = wave * TC + 32 * kc if what == "dPre" else (wave + dr) * (TC + 2) + 32 * kc + 1 + dx
+                                pats.append([slot(p0 + 8 * (l >> 4) + 4 * h + ((l >> 2) & 3), (l & 3) >> 1) * 16 +
+                                             (l & 1) * 8 for l in range(64)])
+                add(f"k_wgrad_tr {what} operand reads, {TC}-column tiles, {name}", 4, pats)
+
+
 def main():
     pair_tests()
     kernel_layouts()
     wgrad_layouts()
+    wgrad_tr_layouts()
     with open("patterns.bin", "wb") as fo:
         fo.write(struct.pack("<i", len(entries)))
         for _, kind, pats in entries:
