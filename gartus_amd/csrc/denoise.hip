@@ -712,8 +712,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
 
 // ---- convolution, weight gradient: LDS tiles and 16x16x32 MFMA -------------
 // dW[co][tap][ci] = sum over pixels p of dPre[p][co] * x[p + off(tap)][ci].
-// A block owns a band of kWR image rows of one image (its split) and NCO x
-// NCI 16-channel fragments of (co, ci), and walks the band in tiles of kWC
+// A block owns kWBands bands of kWR image rows of one image (its split) and NCO x
+// NCI 16-channel fragments of (co, ci), and walks its bands in tiles of kWC
 // columns. Per tile it stages, transposed so that consecutive pixels of one
 // channel are adjacent, dPre = dy masked by the ReLU of the forward output
 // (kWR rows x kWC columns) and x ((kWR + 2) rows x kWX columns: 8 beyond the
@@ -745,6 +745,9 @@ constexpr int wgrad_sx() {  // x channel stride (elements)
 }
 
 constexpr size_t kWRed = 3 * 36 * 64 * sizeof(float);  // waves 1..3 hand wave 0 one (a, b) pair of sums
+// Bands of kWR rows per block: a block's sums cover kWBands bands (its split),
+// which halves the partial sums written and reduced against one band per block.
+constexpr int kWBands = 2;
 
 template <int NCO, int NCI>
 constexpr size_t wgrad_lds_bytes() {
@@ -830,10 +833,9 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
   uint16_t* dT = reinterpret_cast<uint16_t*>(wlds4);
   uint16_t* xT = dT + NCO * 16 * kSD;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-  const int n = blockIdx.x / nbands, r0 = (blockIdx.x % nbands) * kWR;
+  const int n = blockIdx.x / nbands, band0 = (blockIdx.x % nbands) * kWBands;  // nbands: splits per image
   const int co0 = (blockIdx.y % ncog) * NCO * 16, ci0 = ci_first + (blockIdx.y / ncog) * NCI * 16;
   const bool bias = ci0 == 0;  // uniform over the block
-  const bool row_in = r0 + wave < H;
   v4f acc[9][NCO][NCI], bacc[NCO];
 #pragma unroll
   for (int a = 0; a < NCO; ++a) {
@@ -844,6 +846,9 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
       for (int b = 0; b < NCI; ++b) acc[t][a][b] = v4f{0.f, 0.f, 0.f, 0.f};
   }
   const uint4 ones = make_uint4(0x3f803f80u, 0x3f803f80u, 0x3f803f80u, 0x3f803f80u);  // bf16 1.0 x 8
+  for (int band = band0; band < band0 + kWBands && band * kWR < H; ++band) {
+  const int r0 = band * kWR;
+  const bool row_in = r0 + wave < H;
   for (int c0 = 0; c0 < W; c0 += kWC) {
     __syncthreads();  // the previous tile's reads are done
     if constexpr (kTR4) {
@@ -983,6 +988,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
       }
     }
   }
+  }
+  __syncthreads();  // every wave is done with the tiles
   // waves 1..3 hand their sums to wave 0 through LDS, one (a, b) pair at a time
   const size_t nw = size_t(cout) * 9 * cin;
   wgrad_finish<NCO, NCI>(reinterpret_cast<float*>(wlds4), acc, bacc, bias,
@@ -1046,16 +1053,15 @@ __device__ __forceinline__ uint4 wgt_operand(const char* base, int p0, int lane)
 template <int NCO, int NCI, int TC>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k_wgrad_tr(
     const uint16_t* __restrict__ x, int cin, const uint16_t* __restrict__ xa, int ca, const uint16_t* __restrict__ dy,
-    int cout, int N, int H, int W, int nbands, int ncog, int ci_first, float* __restrict__ partial) {
+    int cout, int N, int H, int W, int nsplit, int ncog, int ci_first, float* __restrict__ partial) {
   constexpr int kWDS = wgt_dslots<TC>(), kWXS = wgt_xslots<TC>(), kWXC = TC + 2;
   constexpr int BUF = NCO * kWDS + NCI * kWXS;  // slots of one buffer: dPre regions, then x regions
   extern __shared__ uint4 wlds4[];
   const int tid = threadIdx.x, lane = tid & 63;
   const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
-  const int n = blockIdx.x / nbands, r0 = (blockIdx.x % nbands) * kWR;
+  const int n = blockIdx.x / nsplit, band0 = (blockIdx.x % nsplit) * kWBands;
   const int co0 = (blockIdx.y % ncog) * NCO * 16, ci0 = ci_first + (blockIdx.y / ncog) * NCI * 16;
   const bool bias = ci0 == 0;  // uniform over the block
-  const bool row_in = r0 + wave < H;
   const int cx = cin - ca;
   const i32x4 rd = buffer_rsrc(dy, uint32_t(size_t(N) * H * W * cout * 2));
   const i32x4 rx = buffer_rsrc(x, uint32_t(size_t(N) * H * W * cx * 2));
@@ -1069,9 +1075,9 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
 #pragma unroll
       for (int b = 0; b < NCI; ++b) acc[t][a][b] = v4f{0.f, 0.f, 0.f, 0.f};
   }
-  // the copies of the tile at column c0 into buffer buf: wave w issues slots [64 i, 64 i + 64) of
-  // each region for i = w, w + 4, ...
-  auto copy = [&](uint4* buf, int c0) {
+  // the copies of the tile at rows r0.., column c0 into buffer buf: wave w issues slots
+  // [64 i, 64 i + 64) of each region for i = w, w + 4, ...
+  auto copy = [&](uint4* buf, int r0, int c0) {
 #pragma unroll
     for (int a = 0; a < NCO; ++a) {
 #pragma unroll
@@ -1104,14 +1110,16 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
     }
   };
   const uint4 ones = make_uint4(0x3f803f80u, 0x3f803f80u, 0x3f803f80u, 0x3f803f80u);  // bf16 1.0 x 8
+  // steps: the tiles of band band0, then of band0 + 1, ...
   const int ntiles = (W + TC - 1) / TC;
-  copy(wlds4, 0);
-  for (int t = 0; t < ntiles; ++t) {
+  const int nb = min(kWBands, (H + kWR - 1) / kWR - band0), S = nb * ntiles;
+  copy(wlds4, band0 * kWR, 0);
+  for (int t = 0; t < S; ++t) {
     uint4* cur = wlds4 + (t & 1) * BUF;
-    __builtin_amdgcn_s_waitcnt(kWaitVm0);  // this wave's copies of tile t are done
+    __builtin_amdgcn_s_waitcnt(kWaitVm0);  // this wave's copies of step t are done
     lds_barrier();                        // ... and every wave's; the other buffer's reads are done
-    if (t + 1 < ntiles) copy(wlds4 + ((t + 1) & 1) * BUF, (t + 1) * TC);
-    if (!row_in) continue;  // this wave's row is past the image: its dPre is zero
+    if (t + 1 < S) copy(wlds4 + ((t + 1) & 1) * BUF, (band0 + (t + 1) / ntiles) * kWR, (t + 1) % ntiles * TC);
+    if ((band0 + t / ntiles) * kWR + wave >= H) continue;  // this wave's row is past the image: its dPre is zero
     const char* dT = reinterpret_cast<const char*>(cur);
     const char* xT = reinterpret_cast<const char*>(cur + NCO * kWDS);
 #pragma unroll
@@ -1531,6 +1539,9 @@ void launch_conv(hipStream_t s, const uint16_t* x, const uint16_t* mask, int cin
     launch_groups<DGRAD, F32>(s, rest, full * nt, 1, x, mask, cin, xa, ca, w, bias, y, cout, d, relu, pooled, omask);
 }
 
+// Weight-gradient splits: one per kWBands bands of kWR rows of one image.
+int wgrad_splits(Dims d) { return d.n * (((d.h + kWR - 1) / kWR + kWBands - 1) / kWBands); }
+
 template <int NCO, int NCI, int TC>
 void launch_wgrad_tr(hipStream_t s, const uint16_t* x, int cin, const uint16_t* xa, int ca, const uint16_t* dy,
                      int cout, Dims d, float* partial, int ci_first, int ncig) {
@@ -1540,13 +1551,11 @@ void launch_wgrad_tr(hipStream_t s, const uint16_t* x, int cin, const uint16_t* 
                           hipFuncAttributeMaxDynamicSharedMemorySize, int(lds));
   (void)attr;
   const int ncog = cout / (NCO * 16);
-  const int nbands = (d.h + kWR - 1) / kWR;
-  hipLaunchKernelGGL((k_wgrad_tr<NCO, NCI, TC>), dim3(d.n * nbands, ncog * ncig), dim3(256), lds, s, x, cin, xa, ca,
-                     dy, cout, d.n, d.h, d.w, nbands, ncog, ci_first, partial);
+  const int nsplit = wgrad_splits(d) / d.n;
+  hipLaunchKernelGGL((k_wgrad_tr<NCO, NCI, TC>), dim3(d.n * nsplit, ncog * ncig), dim3(256), lds, s, x, cin, xa, ca,
+                     dy, cout, d.n, d.h, d.w, nsplit, ncog, ci_first, partial);
 }
 
-// Weight-gradient splits: one per band of kWR rows of one image.
-int wgrad_splits(Dims d) { return d.n * ((d.h + kWR - 1) / kWR); }
 
 template <int NCO, int NCI>
 // Input channels [ci_first, ci_first + ncig * NCI * 16) of the weight gradient.
@@ -1558,9 +1567,9 @@ void launch_wgrad(hipStream_t s, const uint16_t* x, int cin, const uint16_t* xa,
                           hipFuncAttributeMaxDynamicSharedMemorySize, int(lds));
   (void)attr;
   const int ncog = cout / (NCO * 16);
-  const int nbands = (d.h + kWR - 1) / kWR;
-  hipLaunchKernelGGL((k_wgrad<NCO, NCI>), dim3(d.n * nbands, ncog * ncig), dim3(256), lds, s, x, cin, xa, ca, dy, mask,
-                     cout, d.h, d.w, nbands, ncog, ci_first, partial);
+  const int nsplit = wgrad_splits(d) / d.n;
+  hipLaunchKernelGGL((k_wgrad<NCO, NCI>), dim3(d.n * nsplit, ncog * ncig), dim3(256), lds, s, x, cin, xa, ca, dy, mask,
+                     cout, d.h, d.w, nsplit, ncog, ci_first, partial);
 }
 
 }  // namespace
