@@ -1447,7 +1447,7 @@ int device_cus() {
 template <int NT, bool F32, int CIN, int GMAX, bool MIRROR = false>
 bool launch_pipe_nt(hipStream_t s, const uint16_t* x, const uint16_t* xa, int ca, const uint16_t* w,
                     const float* bias, void* y, uint16_t* pooled, int cout, Dims d, int relu, float* rad = nullptr,
-                    int rh = 0, int rw = 0, const uint16_t* omask = nullptr) {
+                    int rh = 0, int rw = 0, const uint16_t* omask = nullptr, int co_first = 0) {
   if (size_t(d.n) * d.h * d.w * (CIN - ca) * 2 >= kOutOfRange) return false;
   if (xa && size_t(d.n) * (d.h / 2) * (d.w / 2) * ca * 2 >= kOutOfRange) return false;
   static const int per_cu = [] {  // resident blocks per CU
@@ -1462,7 +1462,7 @@ bool launch_pipe_nt(hipStream_t s, const uint16_t* x, const uint16_t* xa, int ca
   const int ntiles = d.n * nrt * nct;
   const int nb = (std::min(ntiles, device_cus() * per_cu) + 7) / 8 * 8;  // a multiple of the 8 XCDs
   hipLaunchKernelGGL((k_conv_pipe<NT, F32, CIN, GMAX, MIRROR>), dim3(nb, 1), dim3(256), 0, s, x, xa, ca, w, bias, y,
-                     pooled, cout, 0, d.n, d.h, d.w, nrt, nct, relu, rad, rh, rw, omask);
+                     pooled, cout, co_first, d.n, d.h, d.w, nrt, nct, relu, rad, rh, rw, omask);
   return true;
 }
 
@@ -1471,6 +1471,31 @@ bool launch_pipe_nt(hipStream_t s, const uint16_t* x, const uint16_t* xa, int ca
 // output where the forward tensor it belongs to is not positive.
 bool launch_pipe_dgrad(hipStream_t s, const uint16_t* x, int cin, const uint16_t* w, void* y, int cout, Dims d,
                        const uint16_t* omask) {
+  // wider outputs (the gradient of the decoder's concatenation, 80 or 112 channels from 32 or 48):
+  // launches of 48 output channels, then one of the rest
+  if (cout > 48 && cout % 16 == 0 && (cin == 32 || cin == 48)) {
+    if (size_t(d.n) * d.h * d.w * std::max(cin, cout) * 2 >= kOutOfRange) return false;
+    for (int c = 0; c < cout;) {
+      const int nt = std::min(3, (cout - c) / 16);
+      if (cin == 32) {
+        if (nt == 3) launch_pipe_nt<3, false, 32, 1, true>(s, x, nullptr, 0, w, nullptr, y, nullptr, cout, d, 0,
+                                                            nullptr, 0, 0, omask, c);
+        else if (nt == 2) launch_pipe_nt<2, false, 32, 1, true>(s, x, nullptr, 0, w, nullptr, y, nullptr, cout, d, 0,
+                                                                 nullptr, 0, 0, omask, c);
+        else launch_pipe_nt<1, false, 32, 1, true>(s, x, nullptr, 0, w, nullptr, y, nullptr, cout, d, 0, nullptr, 0,
+                                                    0, omask, c);
+      } else {
+        if (nt == 3) launch_pipe_nt<3, false, 48, 1, true>(s, x, nullptr, 0, w, nullptr, y, nullptr, cout, d, 0,
+                                                            nullptr, 0, 0, omask, c);
+        else if (nt == 2) launch_pipe_nt<2, false, 48, 1, true>(s, x, nullptr, 0, w, nullptr, y, nullptr, cout, d, 0,
+                                                                 nullptr, 0, 0, omask, c);
+        else launch_pipe_nt<1, false, 48, 1, true>(s, x, nullptr, 0, w, nullptr, y, nullptr, cout, d, 0, nullptr, 0,
+                                                    0, omask, c);
+      }
+      c += 16 * nt;
+    }
+    return true;
+  }
   switch ((cout / 16) * 1000 + cin) {
     case 2016:
       return launch_pipe_nt<2, false, 16, 1, true>(s, x, nullptr, 0, w, nullptr, y, nullptr, cout, d, 0, nullptr, 0, 0,
