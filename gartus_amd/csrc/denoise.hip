@@ -1203,63 +1203,74 @@ __device__ __forceinline__ uint4 pack8(const float (&f)[8]) {
 __global__ void k_pool_bwd(const uint16_t* __restrict__ x, const uint16_t* __restrict__ dy,
                            const uint16_t* __restrict__ add, int add_c, int add_off, uint16_t* __restrict__ dx, int c,
                            int H, int W, size_t n8, int relu_mask) {
+  // one 2x2 window and 8 channels per thread: n8 = windows * c / 8
   const int H2 = H / 2, W2 = W / 2, c8 = c / 8;
   for (size_t i = blockIdx.x * size_t(blockDim.x) + threadIdx.x; i < n8; i += size_t(gridDim.x) * blockDim.x) {
     const int ch = int(i % c8) * 8;
-    const size_t p = i / c8;
-    const int w = int(p % W);
-    const size_t t = p / W;
-    const int h = int(t % H);
-    const size_t b = t / H;
+    const size_t q = i / c8;
+    const int w2 = int(q % W2);
+    const size_t t = q / W2;
+    const int h2 = int(t % H2);
+    const size_t b = t / H2;
+    const size_t p0 = (b * H + 2 * h2) * W + 2 * w2;
+    const size_t px[4] = {p0, p0 + 1, p0 + W, p0 + W + 1};  // scan order
+    float xv[4][8];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) unpack8(*reinterpret_cast<const uint4*>(x + px[k] * c + ch), xv[k]);
     // per channel, the window's first maximum in scan order takes the gradient
-    const size_t base = ((b * H + (h & ~1)) * W + (w & ~1)) * c + ch;
-    const size_t cand[4] = {base, base + c, base + size_t(W) * c, base + size_t(W) * c + c};
-    float m[8], v[8];
     int arg[8];
-    unpack8(*reinterpret_cast<const uint4*>(x + cand[0]), m);
-#pragma unroll
-    for (int e = 0; e < 8; ++e) arg[e] = 0;
-#pragma unroll
-    for (int k = 1; k < 4; ++k) {
-      unpack8(*reinterpret_cast<const uint4*>(x + cand[k]), v);
-#pragma unroll
-      for (int e = 0; e < 8; ++e)
-        if (v[e] > m[e]) { m[e] = v[e]; arg[e] = k; }
-    }
-    const int mine = (h & 1) * 2 + (w & 1);
-    float g[8], up[8], ad[8];
-    unpack8(*reinterpret_cast<const uint4*>(dy + ((b * H2 + h / 2) * W2 + w / 2) * c + ch), up);
-    if (add) unpack8(*reinterpret_cast<const uint4*>(add + p * add_c + add_off + ch), ad);
 #pragma unroll
     for (int e = 0; e < 8; ++e) {
-      g[e] = mine == arg[e] ? up[e] : 0.f;
-      if (add) g[e] += ad[e];
-    }
-    if (relu_mask) {  // x is the ReLU output: its gradient is zero where x is not positive
-      float xs[8];
-      unpack8(*reinterpret_cast<const uint4*>(x + cand[mine]), xs);
+      float m = xv[0][e];
+      arg[e] = 0;
 #pragma unroll
-      for (int e = 0; e < 8; ++e)
-        if (!(xs[e] > 0.f)) g[e] = 0.f;
+      for (int k = 1; k < 4; ++k)
+        if (xv[k][e] > m) { m = xv[k][e]; arg[e] = k; }
     }
-    *reinterpret_cast<uint4*>(dx + p * c + ch) = pack8(g);
+    float up[8];
+    unpack8(*reinterpret_cast<const uint4*>(dy + q * c + ch), up);
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      float g[8], ad[8];
+      if (add) unpack8(*reinterpret_cast<const uint4*>(add + px[k] * add_c + add_off + ch), ad);
+#pragma unroll
+      for (int e = 0; e < 8; ++e) {
+        g[e] = k == arg[e] ? up[e] : 0.f;
+        if (add) g[e] += ad[e];
+        if (relu_mask && !(xv[k][e] > 0.f)) g[e] = 0.f;  // x is the ReLU output
+      }
+      *reinterpret_cast<uint4*>(dx + px[k] * c + ch) = pack8(g);
+    }
   }
 }
 
 __global__ void k_upcat_bwd(const uint16_t* __restrict__ dout, int ca, int cb, uint16_t* __restrict__ da,
-                            const uint16_t* __restrict__ da_mask, int H, int W, size_t n) {
-  const int c = ca + cb, H2 = H / 2, W2 = W / 2;
-  for (size_t i = blockIdx.x * size_t(blockDim.x) + threadIdx.x; i < n; i += size_t(gridDim.x) * blockDim.x) {
-    const int ch = int(i % ca);
-    size_t q = i / ca;
+                            const uint16_t* __restrict__ da_mask, int H, int W, size_t n8) {
+  // 8 channels of one half-resolution pixel per thread: n8 = pixels * ca / 8
+  const int c = ca + cb, H2 = H / 2, W2 = W / 2, ca8 = ca / 8;
+  for (size_t i = blockIdx.x * size_t(blockDim.x) + threadIdx.x; i < n8; i += size_t(gridDim.x) * blockDim.x) {
+    const int ch = int(i % ca8) * 8;
+    size_t q = i / ca8;
     const int w2 = int(q % W2);
     q /= W2;
     const int h2 = int(q % H2);
     const size_t b = q / H2;
     const size_t base = ((b * H + 2 * h2) * W + 2 * w2) * c + ch;
-    float s = bf2f(dout[base]) + bf2f(dout[base + c]);
-    s += bf2f(dout[base + size_t(W) * c]) + bf2f(dout[base + size_t(W) * c + c]);
-    da[i] = (da_mask && !(bf2f(da_mask[i]) > 0.f)) ? uint16_t(0) : f2bf(s);
+    float v0[8], v1[8], v2[8], v3[8], o[8];
+    unpack8(*reinterpret_cast<const uint4*>(dout + base), v0);
+    unpack8(*reinterpret_cast<const uint4*>(dout + base + c), v1);
+    unpack8(*reinterpret_cast<const uint4*>(dout + base + size_t(W) * c), v2);
+    unpack8(*reinterpret_cast<const uint4*>(dout + base + size_t(W) * c + c), v3);
+#pragma unroll
+    for (int e = 0; e < 8; ++e) {
+      float s = v0[e] + v1[e];
+      s += v2[e] + v3[e];
+      o[e] = s;
+    }
+    const size_t off = (i / ca8) * ca + ch;
+    uint4 r = pack8(o);
+    if (da_mask) r = mask8(r, *reinterpret_cast<const uint4*>(da_mask + off));
+    *reinterpret_cast<uint4*>(da + off) = r;
   }
 }
 
@@ -1334,18 +1345,22 @@ __global__ __launch_bounds__(256) void k_loss(const float* __restrict__ z, const
     const size_t t = p / Wp;
     const int yy = int(t % Hp);
     const size_t b = t / Hp;
-    uint16_t* g = dz + p * 16;
-    for (int c = 3; c < 16; ++c) g[c] = 0;
+    uint4* g = reinterpret_cast<uint4*>(dz + p * 16);  // channels 0-7, 8-15; only 0-2 carry a gradient
+    g[1] = make_uint4(0, 0, 0, 0);
     if (yy >= h || xx >= w) {
-      g[0] = g[1] = g[2] = 0;
+      g[0] = make_uint4(0, 0, 0, 0);
       continue;
     }
     const float* tg = target + ((b * h + yy) * w + xx) * 3;
+    const float4 zp = *reinterpret_cast<const float4*>(z + p * 16);
+    const float zc[3] = {zp.x, zp.y, zp.z};
+    uint16_t gc[3];
     for (int c = 0; c < 3; ++c) {
       float dl;
-      acc += loss_elem<KIND>(z[p * 16 + c], fmaxf(tg[c], 0.f), &dl);
-      g[c] = f2bf(dl * inv_count);
+      acc += loss_elem<KIND>(zc[c], fmaxf(tg[c], 0.f), &dl);
+      gc[c] = f2bf(dl * inv_count);
     }
+    g[0] = make_uint4(uint32_t(gc[0]) | (uint32_t(gc[1]) << 16), uint32_t(gc[2]), 0, 0);
   }
   red[threadIdx.x] = acc;
   __syncthreads();
@@ -1671,14 +1686,14 @@ void conv_wgrad(hipStream_t s, const uint16_t* x, int cin, const uint16_t* dy, c
 
 void pool_bwd(hipStream_t s, const uint16_t* x, const uint16_t* dy, const uint16_t* add, int add_c, int add_off,
               uint16_t* dx, int c, Dims d, bool relu_mask) {
-  const size_t n8 = size_t(d.pixels()) * (c / 8);
+  const size_t n8 = size_t(d.n) * (d.h / 2) * (d.w / 2) * (c / 8);  // windows x 8-channel pieces
   hipLaunchKernelGGL(k_pool_bwd, dim3(grid_for(n8)), dim3(256), 0, s, x, dy, add, add_c, add_off, dx, c, d.h, d.w,
                      n8, relu_mask ? 1 : 0);
 }
 
 void upcat_bwd(hipStream_t s, const uint16_t* dout, int ca, int cb, uint16_t* da, Dims d, const uint16_t* da_mask) {
-  const size_t n = size_t(d.n) * (d.h / 2) * (d.w / 2) * ca;
-  hipLaunchKernelGGL(k_upcat_bwd, dim3(grid_for(n)), dim3(256), 0, s, dout, ca, cb, da, da_mask, d.h, d.w, n);
+  const size_t n8 = size_t(d.n) * (d.h / 2) * (d.w / 2) * (ca / 8);
+  hipLaunchKernelGGL(k_upcat_bwd, dim3(grid_for(n8)), dim3(256), 0, s, dout, ca, cb, da, da_mask, d.h, d.w, n8);
 }
 
 void pack_input(hipStream_t s, const float* beauty, const float* beauty_b, const float* albedo, const float* normal,
