@@ -36,13 +36,13 @@ void conv_fwd_upcat(hipStream_t s, const uint16_t* a, int ca, const uint16_t* b,
 // (the ReLU of the layer that produced the convolution's input, applied where its gradient is made).
 void conv_dgrad(hipStream_t s, const uint16_t* dy, const uint16_t* y, int cout, const uint16_t* wt, uint16_t* dx,
                 int cin, Dims d, const uint16_t* dx_mask = nullptr);
-// dW[co][t][ci] = sum_p dPre[p][co] x[p + off_t][ci], dB[co] = sum_p dPre[p][co], dPre = dy * [y > 0];
-// summed in a fixed order (per-split partials in `scratch`, then reduced).
+// dW[co][t][ci] = sum_p dy[p][co] x[p + off_t][ci], dB[co] = sum_p dy[p][co] (dy already masked by the
+// ReLU of the forward output where it was made); summed in a fixed order (per-split partials in
+// `scratch`, then reduced).
 // With xa, x is the decoder's concatenation [up2(xa) (ca channels), x (cin - ca channels)] read in
 // place, as conv_fwd_upcat reads it.
-void conv_wgrad(hipStream_t s, const uint16_t* x, int cin, const uint16_t* dy, const uint16_t* y, int cout, Dims d,
-                float* dw, float* db, float* scratch, size_t scratch_floats, const uint16_t* xa = nullptr,
-                int ca = 0);
+void conv_wgrad(hipStream_t s, const uint16_t* x, int cin, const uint16_t* dy, int cout, Dims d, float* dw, float* db,
+                float* scratch, size_t scratch_floats, const uint16_t* xa = nullptr, int ca = 0);
 size_t wgrad_scratch_floats(int cin, int cout, Dims d);
 
 // dx = route dy to each window's first maximum, plus add[:, add_off : add_off + c] when add != null

@@ -710,57 +710,19 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
     pipe_store<NT, OUT_F32>(out, y, pooled, rad, rh, rw, omask, cout, co0, H, W, nrt, nct, tid, lane, wave);
 }
 
-// ---- convolution, weight gradient: LDS tiles and 16x16x32 MFMA -------------
-// dW[co][tap][ci] = sum over pixels p of dPre[p][co] * x[p + off(tap)][ci].
-// A block owns kWBands bands of kWR image rows of one image (its split) and NCO x
-// NCI 16-channel fragments of (co, ci), and walks its bands in tiles of kWC
-// columns. Per tile it stages, transposed so that consecutive pixels of one
-// channel are adjacent, dPre = dy masked by the ReLU of the forward output
-// (kWR rows x kWC columns) and x ((kWR + 2) rows x kWX columns: 8 beyond the
-// tile on each side, so every 16-byte read stays aligned). Wave w takes row w:
-// per K step of 32 pixels, lane l reads 8 consecutive pixels of channel l % 16
-// (A from dPre, B from x at the tap's row), and the column shifts of the
-// three taps of that row come from its 16-byte neighbours by byte alignment.
-// The four waves' sums are added in wave order through LDS, and the centre
-// tap's waves of the first ci group also sum dPre against ones (the bias).
-constexpr int kWR = 4, kWC = 64, kWX = kWC + 16;
-// Two stagings. With 2 x 2 fragments a thread transposes 4 pixels x 8
-// channels in registers and writes 8-byte rows, and the channel strides are
-// 34 and 62 16-byte pieces (an odd multiple of 2), which puts the 16 lanes of
-// one LDS read pass (channels l & 15 at one or two 8-pixel offsets) in
-// distinct bank groups. The smaller variants, whose time goes more to
-// staging than to MFMA steps, keep coalesced 16-byte loads of one pixel's
-// channels and 2-byte stores, with strides of 33 and 61 pieces.
-template <int NCO, int NCI>
-constexpr bool wgrad_tr4() {
-  return NCO == 2 && NCI == 2;
-}
-template <int NCO, int NCI>
-constexpr int wgrad_sd() {  // dPre channel stride (elements)
-  return kWR * kWC + (wgrad_tr4<NCO, NCI>() ? 16 : 8);
-}
-template <int NCO, int NCI>
-constexpr int wgrad_sx() {  // x channel stride (elements)
-  return (kWR + 2) * kWX + (wgrad_tr4<NCO, NCI>() ? 16 : 8);
-}
-
+// ---- convolution, weight gradient -----------------------------------------
+// dW[co][tap][ci] = sum over pixels p of dPre[p][co] * x[p + off(tap)][ci],
+// with dPre the output gradient (masked by the ReLU of the forward output by
+// the kernel that made it). A block owns kWBands bands of kWR image rows of
+// one image (its split) and NCO x NCI 16-channel fragments of (co, ci); wave w
+// takes row w of each band. The four waves' sums are added in wave order
+// through LDS, and the blocks of the first ci group also sum dPre against
+// ones (the bias).
+constexpr int kWR = 4;
 constexpr size_t kWRed = 3 * 36 * 64 * sizeof(float);  // waves 1..3 hand wave 0 one (a, b) pair of sums
 // Bands of kWR rows per block: a block's sums cover kWBands bands (its split),
 // which halves the partial sums written and reduced against one band per block.
 constexpr int kWBands = 2;
-
-template <int NCO, int NCI>
-constexpr size_t wgrad_lds_bytes() {
-  return std::max(size_t(NCO * 16 * wgrad_sd<NCO, NCI>() + NCI * 16 * wgrad_sx<NCO, NCI>()) * sizeof(uint16_t),
-                  kWRed);
-}
-
-// Elements [1, 9) of the 16 bf16 (lo, hi), or [-1, 7) of (hi) preceded by the
-// last element of lo: the B operands of the taps at column offsets +1 and -1.
-__device__ __forceinline__ uint4 shift_up(uint4 lo, uint4 hi) {  // elements 1..8 of lo:hi
-  return make_uint4(__builtin_amdgcn_alignbyte(lo.y, lo.x, 2), __builtin_amdgcn_alignbyte(lo.z, lo.y, 2),
-                    __builtin_amdgcn_alignbyte(lo.w, lo.z, 2), __builtin_amdgcn_alignbyte(hi.x, lo.w, 2));
-}
 
 // The end of a weight-gradient block: waves 1..3 hand their sums to wave 0
 // through LDS (red, kWRed bytes), one (a, b) pair at a time, in wave order,
@@ -821,195 +783,21 @@ __device__ __forceinline__ void wgrad_finish(float* red, v4f (&acc)[9][NCO][NCI]
   }
 }
 
-template <int NCO, int NCI>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k_wgrad(const uint16_t* __restrict__ x, int cin,
-                                               const uint16_t* __restrict__ xa, int ca,
-                                               const uint16_t* __restrict__ dy, const uint16_t* __restrict__ mask,
-                                               int cout, int H, int W, int nbands, int ncog, int ci_first,
-                                               float* __restrict__ partial) {
-  constexpr bool kTR4 = wgrad_tr4<NCO, NCI>();
-  constexpr int kSD = wgrad_sd<NCO, NCI>(), kSX = wgrad_sx<NCO, NCI>();
-  extern __shared__ uint4 wlds4[];
-  uint16_t* dT = reinterpret_cast<uint16_t*>(wlds4);
-  uint16_t* xT = dT + NCO * 16 * kSD;
-  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-  const int n = blockIdx.x / nbands, band0 = (blockIdx.x % nbands) * kWBands;  // nbands: splits per image
-  const int co0 = (blockIdx.y % ncog) * NCO * 16, ci0 = ci_first + (blockIdx.y / ncog) * NCI * 16;
-  const bool bias = ci0 == 0;  // uniform over the block
-  v4f acc[9][NCO][NCI], bacc[NCO];
-#pragma unroll
-  for (int a = 0; a < NCO; ++a) {
-    bacc[a] = v4f{0.f, 0.f, 0.f, 0.f};
-#pragma unroll
-    for (int t = 0; t < 9; ++t)
-#pragma unroll
-      for (int b = 0; b < NCI; ++b) acc[t][a][b] = v4f{0.f, 0.f, 0.f, 0.f};
-  }
-  const uint4 ones = make_uint4(0x3f803f80u, 0x3f803f80u, 0x3f803f80u, 0x3f803f80u);  // bf16 1.0 x 8
-  for (int band = band0; band < band0 + kWBands && band * kWR < H; ++band) {
-  const int r0 = band * kWR;
-  const bool row_in = r0 + wave < H;
-  for (int c0 = 0; c0 < W; c0 += kWC) {
-    __syncthreads();  // the previous tile's reads are done
-    if constexpr (kTR4) {
-      // dPre and x, transposed so that consecutive pixels of one channel are
-      // adjacent: a thread loads 4 consecutive pixels of one 8-channel piece,
-      // transposes the 4 x 8 block in registers and writes 8 channel rows of 8
-      // bytes. Items: dPre first (64 groups of 4 pixels per piece), then x (128
-      // slots per piece, 120 used: kWR + 2 rows of kWX / 4 groups). The group
-      // runs fastest, so that one write instruction covers 256 consecutive
-      // bytes of one channel row.
-      constexpr int DQ = NCO * 2, XQ = NCI * 2;  // 8-channel pieces per pixel
-      constexpr int ND = 64 * DQ, NX = 128 * XQ, XG = kWX / 4;
-      for (int i = tid; i < ND + NX; i += 256) {  // ND is a multiple of 64: the branch is uniform per wave
-        uint4 v[4];
-        uint16_t* dst;
-        int stride;
-        if (i < ND) {
-          const int q = i >> 6, g = i & 63, row = g >> 4, col = (g & 15) * 4;
-          const int h = r0 + row;
-          const size_t o0 = ((size_t(n) * H + h) * W + c0 + col) * cout + co0 + 8 * q;
-#pragma unroll
-          for (int k = 0; k < 4; ++k) {
-            v[k] = make_uint4(0, 0, 0, 0);
-            if (h < H && c0 + col + k < W) {
-              const size_t o = o0 + size_t(k) * cout;
-              v[k] = *reinterpret_cast<const uint4*>(dy + o);
-              if (mask) v[k] = mask8(v[k], *reinterpret_cast<const uint4*>(mask + o));
-            }
-          }
-          dst = dT + (8 * q) * kSD + row * kWC + col;
-          stride = kSD;
-        } else {
-          const int j = i - ND, q = j >> 7, g = j & 127;
-          if (g >= (kWR + 2) * XG) continue;
-          const int row = g / XG, col = (g % XG) * 4;
-          const int h = r0 - 1 + row, c = ci0 + 8 * q;  // channels below ca come from xa at half resolution (nearest)
-#pragma unroll
-          for (int k = 0; k < 4; ++k) {
-            const int w = c0 - 8 + col + k;
-            v[k] = make_uint4(0, 0, 0, 0);
-            if (h >= 0 && h < H && w >= 0 && w < W)
-              v[k] = c < ca
-                         ? *reinterpret_cast<const uint4*>(xa + ((size_t(n) * (H / 2) + h / 2) * (W / 2) + w / 2) * ca + c)
-                         : *reinterpret_cast<const uint4*>(x + ((size_t(n) * H + h) * W + w) * (cin - ca) + (c - ca));
-          }
-          dst = xT + (8 * q) * kSX + row * kWX + col;
-          stride = kSX;
-        }
-        // channel e of pixel k is half e & 1 of word e >> 1 of v[k]
-#pragma unroll
-        for (int e = 0; e < 8; ++e) {
-          const uint32_t sel = (e & 1) ? 0x07060302u : 0x05040100u;
-          uint32_t t[2];
-#pragma unroll
-          for (int kp = 0; kp < 2; ++kp) {
-            const uint4 lo = v[2 * kp], hi = v[2 * kp + 1];
-            const uint32_t a = (e >> 1) == 0 ? lo.x : (e >> 1) == 1 ? lo.y : (e >> 1) == 2 ? lo.z : lo.w;
-            const uint32_t b = (e >> 1) == 0 ? hi.x : (e >> 1) == 1 ? hi.y : (e >> 1) == 2 ? hi.z : hi.w;
-            t[kp] = __builtin_amdgcn_perm(b, a, sel);  // pixel 2kp in the low half, 2kp + 1 in the high half
-          }
-          *reinterpret_cast<uint2*>(dst + e * stride) = make_uint2(t[0], t[1]);
-        }
-      }
-    } else {
-      // dPre, transposed: 16-byte loads of 8 channels, 2-byte stores
-      constexpr int DQ = NCO * 2;  // 8-channel pieces per pixel
-      for (int i = tid; i < kWR * kWC * DQ; i += 256) {
-        const int q = i % DQ, px = i / DQ, col = px % kWC, row = px / kWC;
-        const int h = r0 + row, w = c0 + col;
-        uint4 v = make_uint4(0, 0, 0, 0);
-        if (h < H && w < W) {
-          const size_t o = ((size_t(n) * H + h) * W + w) * cout + co0 + 8 * q;
-          v = *reinterpret_cast<const uint4*>(dy + o);
-          if (mask) v = mask8(v, *reinterpret_cast<const uint4*>(mask + o));
-        }
-        const uint32_t e[4] = {v.x, v.y, v.z, v.w};
-        uint16_t* dst = dT + (8 * q) * kSD + row * kWC + col;
-#pragma unroll
-        for (int k = 0; k < 4; ++k) {
-          dst[(2 * k) * kSD] = uint16_t(e[k]);
-          dst[(2 * k + 1) * kSD] = uint16_t(e[k] >> 16);
-        }
-      }
-      // x with a one-row halo and 8 columns each side, zero outside the image
-      constexpr int XQ = NCI * 2;
-      for (int i = tid; i < (kWR + 2) * kWX * XQ; i += 256) {
-        const int q = i % XQ, px = i / XQ, col = px % kWX, row = px / kWX;
-        const int h = r0 - 1 + row, w = c0 - 8 + col;
-        uint4 v = make_uint4(0, 0, 0, 0);
-        if (h >= 0 && h < H && w >= 0 && w < W) {
-          const int c = ci0 + 8 * q;  // channels below ca come from xa at half resolution (nearest)
-          v = c < ca ? *reinterpret_cast<const uint4*>(xa + ((size_t(n) * (H / 2) + h / 2) * (W / 2) + w / 2) * ca + c)
-                     : *reinterpret_cast<const uint4*>(x + ((size_t(n) * H + h) * W + w) * (cin - ca) + (c - ca));
-        }
-        const uint32_t e[4] = {v.x, v.y, v.z, v.w};
-        uint16_t* dst = xT + (8 * q) * kSX + row * kWX + col;
-#pragma unroll
-        for (int k = 0; k < 4; ++k) {
-          dst[(2 * k) * kSX] = uint16_t(e[k]);
-          dst[(2 * k + 1) * kSX] = uint16_t(e[k] >> 16);
-        }
-      }
-    }
-    __syncthreads();
-    if (!row_in) continue;  // this wave's row is past the image: its dPre is zero
-#pragma unroll
-    for (int kc = 0; kc < kWC / 32; ++kc) {
-      const int cb = 32 * kc + 8 * (lane >> 4);
-      uint4 A[NCO];
-#pragma unroll
-      for (int a = 0; a < NCO; ++a)
-        A[a] = *reinterpret_cast<const uint4*>(dT + (a * 16 + (lane & 15)) * kSD + wave * kWC + cb);
-#pragma unroll
-      for (int b = 0; b < NCI; ++b) {
-#pragma unroll
-        for (int dr = 0; dr < 3; ++dr) {
-          const uint16_t* xr = xT + (b * 16 + (lane & 15)) * kSX + (wave + dr) * kWX + 8 + cb;
-          const uint4 prev = *reinterpret_cast<const uint4*>(xr - 8);
-          const uint4 cur = *reinterpret_cast<const uint4*>(xr);
-          const uint4 next = *reinterpret_cast<const uint4*>(xr + 8);
-          const uint4 left = make_uint4(__builtin_amdgcn_alignbyte(cur.x, prev.w, 2),
-                                        __builtin_amdgcn_alignbyte(cur.y, cur.x, 2),
-                                        __builtin_amdgcn_alignbyte(cur.z, cur.y, 2),
-                                        __builtin_amdgcn_alignbyte(cur.w, cur.z, 2));  // columns -1..6
-          const uint4 right = shift_up(cur, next);                                     // columns 1..8
-#pragma unroll
-          for (int a = 0; a < NCO; ++a) {
-            acc[dr * 3 + 0][a][b] = mfma32(A[a], left, acc[dr * 3 + 0][a][b]);
-            acc[dr * 3 + 1][a][b] = mfma32(A[a], cur, acc[dr * 3 + 1][a][b]);
-            acc[dr * 3 + 2][a][b] = mfma32(A[a], right, acc[dr * 3 + 2][a][b]);
-          }
-        }
-      }
-      if (bias) {
-#pragma unroll
-        for (int a = 0; a < NCO; ++a) bacc[a] = mfma32(A[a], ones, bacc[a]);
-      }
-    }
-  }
-  }
-  __syncthreads();  // every wave is done with the tiles
-  // waves 1..3 hand their sums to wave 0 through LDS, one (a, b) pair at a time
-  const size_t nw = size_t(cout) * 9 * cin;
-  wgrad_finish<NCO, NCI>(reinterpret_cast<float*>(wlds4), acc, bacc, bias,
-                         partial + size_t(blockIdx.x) * (nw + cout), nw, cin, co0, ci0, wave, lane);
-}
-
-// ---- convolution, weight gradient from untransposed tiles ------------------
-// The sums of k_wgrad, for a dy that needs no mask. dPre (kWR rows x TC
+// The kernel walks its bands in tiles of TC columns. dPre (kWR rows x TC
 // columns) and x (kWR + 2 rows x TC + 2 columns: one beyond the tile on each
 // side) lie in LDS as they lie in HBM, one 32-byte row of 16 channels per
 // pixel, in one region per 16-channel fragment, copied straight from global
 // memory by buffer loads that write LDS (each region from its own tensor, so
-// a block's channels may straddle the decoder concatenation), in tiles of TC
-// columns; the next tile's copy runs while the waves work on this one. The MFMA
+// a block's channels may straddle the decoder concatenation); the next tile's
+// copy runs while the waves work on this one. The MFMA
 // operands come from ds_read_b64_tr_b16: over 16 lanes, lane l names 4
 // channels of pixel row (l >> 2) & 3 of a 4 x 16 block and receives channel
 // l's 4 pixels, so two reads give a lane its 8 K values (pixels) and a column
 // tap is an offset of the pixel index. Slots are swizzled by bit 3 of the
 // pixel, which puts the two 16-lane groups of a read pass (pixels 8 apart) in
-// different banks.
+// different banks. The buffer resources start at the first row the block's
+// bands read, so offsets stay small for tensors of any size (kWBands * kWR + 2
+// rows of the widest tensor must stay below 2 GB).
 template <int TC>  // tile columns
 constexpr int wgt_dslots() {  // 16-byte slots of one dPre fragment region
   return (kWR * TC * 2 + 63) / 64 * 64;
@@ -1063,9 +851,14 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
   const int co0 = (blockIdx.y % ncog) * NCO * 16, ci0 = ci_first + (blockIdx.y / ncog) * NCI * 16;
   const bool bias = ci0 == 0;  // uniform over the block
   const int cx = cin - ca;
-  const i32x4 rd = buffer_rsrc(dy, uint32_t(size_t(N) * H * W * cout * 2));
-  const i32x4 rx = buffer_rsrc(x, uint32_t(size_t(N) * H * W * cx * 2));
-  const i32x4 ra = buffer_rsrc(xa ? xa : x, xa ? uint32_t(size_t(N) * (H / 2) * (W / 2) * ca * 2) : 0u);
+  // rows [hb, he) hold everything the block reads (x's halo row above and below included)
+  const int hb = max(band0 * kWR - 1, 0), he = min((band0 + kWBands) * kWR + 1, H);
+  const i32x4 rd = buffer_rsrc(dy + (size_t(n) * H + hb) * W * cout, uint32_t(size_t(he - hb) * W * cout * 2));
+  const i32x4 rx = buffer_rsrc(x + (size_t(n) * H + hb) * W * cx, uint32_t(size_t(he - hb) * W * cx * 2));
+  const int hb2 = hb / 2, he2 = (he + 1) / 2;  // the same rows of the half-resolution part
+  const i32x4 ra = xa ? buffer_rsrc(xa + (size_t(n) * (H / 2) + hb2) * (W / 2) * ca,
+                                    uint32_t(size_t(he2 - hb2) * (W / 2) * ca * 2))
+                      : buffer_rsrc(x, 0u);
   v4f acc[9][NCO][NCI], bacc[NCO];
 #pragma unroll
   for (int a = 0; a < NCO; ++a) {
@@ -1086,7 +879,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
         const int h = r0 + p / TC, w = c0 + p % TC;
         uint32_t off = kOutOfRange;
         if (p < kWR * TC && h < H && w < W)
-          off = uint32_t((((size_t(n) * H + h) * W + w) * cout + co0 + 16 * a + 8 * (sl & 1)) * 2);
+          off = uint32_t((((size_t(h) - hb) * W + w) * cout + co0 + 16 * a + 8 * (sl & 1)) * 2);
         lds_copy16(buf + a * kWDS + i * 64, rd, off);
       }
     }
@@ -1102,8 +895,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
           const int h = r0 - 1 + p / kWXC, w = c0 - 1 + p % kWXC;
           uint32_t off = kOutOfRange;
           if (p < (kWR + 2) * kWXC && h >= 0 && h < H && w >= 0 && w < W)
-            off = half_res ? uint32_t((((size_t(n) * (H / 2) + h / 2) * (W / 2) + w / 2) * ca + c + 8 * (sl & 1)) * 2)
-                           : uint32_t((((size_t(n) * H + h) * W + w) * cx + c - ca + 8 * (sl & 1)) * 2);
+            off = half_res ? uint32_t((((size_t(h) / 2 - hb2) * (W / 2) + w / 2) * ca + c + 8 * (sl & 1)) * 2)
+                           : uint32_t((((size_t(h) - hb) * W + w) * cx + c - ca + 8 * (sl & 1)) * 2);
           lds_copy16(buf + NCO * kWDS + b * kWXS + i * 64, half_res ? ra : rx, off);
         }
       }
@@ -1597,21 +1390,6 @@ void launch_wgrad_tr(hipStream_t s, const uint16_t* x, int cin, const uint16_t* 
 }
 
 
-template <int NCO, int NCI>
-// Input channels [ci_first, ci_first + ncig * NCI * 16) of the weight gradient.
-void launch_wgrad(hipStream_t s, const uint16_t* x, int cin, const uint16_t* xa, int ca, const uint16_t* dy,
-                  const uint16_t* mask, int cout, Dims d, float* partial, int ci_first, int ncig) {
-  constexpr size_t lds = wgrad_lds_bytes<NCO, NCI>();
-  static const hipError_t attr =  // once per instantiation: allow more than 64 KiB of LDS
-      hipFuncSetAttribute(reinterpret_cast<const void*>(&k_wgrad<NCO, NCI>),
-                          hipFuncAttributeMaxDynamicSharedMemorySize, int(lds));
-  (void)attr;
-  const int ncog = cout / (NCO * 16);
-  const int nsplit = wgrad_splits(d) / d.n;
-  hipLaunchKernelGGL((k_wgrad<NCO, NCI>), dim3(d.n * nsplit, ncog * ncig), dim3(256), lds, s, x, cin, xa, ca, dy, mask,
-                     cout, d.h, d.w, nsplit, ncog, ci_first, partial);
-}
-
 }  // namespace
 
 void conv_fwd(hipStream_t s, const uint16_t* x, int cin, const uint16_t* w, const float* bias, void* y, int cout,
@@ -1642,37 +1420,26 @@ size_t wgrad_scratch_floats(int cin, int cout, Dims d) {
   return splits * (size_t(cout) * 9 * cin + cout);
 }
 
-void conv_wgrad(hipStream_t s, const uint16_t* x, int cin, const uint16_t* dy, const uint16_t* y, int cout, Dims d,
-                float* dw, float* db, float* scratch, size_t scratch_floats, const uint16_t* xa, int ca) {
+void conv_wgrad(hipStream_t s, const uint16_t* x, int cin, const uint16_t* dy, int cout, Dims d, float* dw, float* db,
+                float* scratch, size_t scratch_floats, const uint16_t* xa, int ca) {
   const int splits = wgrad_splits(d);
   const size_t nw = size_t(cout) * 9 * cin;
   if (scratch_floats < size_t(splits) * (nw + cout)) return;  // sized by wgrad_scratch_floats
+  // a block's buffer resources span kWBands * kWR + 2 rows of the widest tensor (the network's
+  // widths and 256 channels at most keep this far below 2 GB)
+  if (size_t(kWBands * kWR + 2) * d.w * std::max(cin, cout) * 2 >= kOutOfRange) return;
   // Two output- and two input-channel fragments per block where the layer has them, so that
   // fewer blocks stage the same dPre and x tiles; an odd last input fragment takes a launch of its
   // own (it writes its own columns of the same partial rows).
   const int pairs = cin / 32, odd = (cin / 16) % 2;
-  // Unmasked tensors within 31-bit buffer offsets take k_wgrad_tr, double-buffered, in tiles of
-  // 32 columns (two blocks of 2 x 2 fragments fit a CU only at that size; 1 x 2 and 1 x 1 also
-  // measured faster) or 64 (2 x 1); the others k_wgrad.
-  const bool tr = !y && size_t(d.n) * d.h * d.w * std::max(cout, cin) * 2 < kOutOfRange;
+  // Double-buffered tiles of 32 columns (two blocks of 2 x 2 fragments fit a CU with both buffers
+  // only at that size; 1 x 2 and 1 x 1 also measured faster) or 64 (2 x 1).
   if ((cout / 16) % 2 == 0) {
-    if (pairs) {
-      if (tr) launch_wgrad_tr<2, 2, 32>(s, x, cin, xa, ca, dy, cout, d, scratch, 0, pairs);
-      else launch_wgrad<2, 2>(s, x, cin, xa, ca, dy, y, cout, d, scratch, 0, pairs);
-    }
-    if (odd) {
-      if (tr) launch_wgrad_tr<2, 1, 64>(s, x, cin, xa, ca, dy, cout, d, scratch, 32 * pairs, 1);
-      else launch_wgrad<2, 1>(s, x, cin, xa, ca, dy, y, cout, d, scratch, 32 * pairs, 1);
-    }
+    if (pairs) launch_wgrad_tr<2, 2, 32>(s, x, cin, xa, ca, dy, cout, d, scratch, 0, pairs);
+    if (odd) launch_wgrad_tr<2, 1, 64>(s, x, cin, xa, ca, dy, cout, d, scratch, 32 * pairs, 1);
   } else {
-    if (pairs) {
-      if (tr) launch_wgrad_tr<1, 2, 32>(s, x, cin, xa, ca, dy, cout, d, scratch, 0, pairs);
-      else launch_wgrad<1, 2>(s, x, cin, xa, ca, dy, y, cout, d, scratch, 0, pairs);
-    }
-    if (odd) {
-      if (tr) launch_wgrad_tr<1, 1, 32>(s, x, cin, xa, ca, dy, cout, d, scratch, 32 * pairs, 1);
-      else launch_wgrad<1, 1>(s, x, cin, xa, ca, dy, y, cout, d, scratch, 32 * pairs, 1);
-    }
+    if (pairs) launch_wgrad_tr<1, 2, 32>(s, x, cin, xa, ca, dy, cout, d, scratch, 0, pairs);
+    if (odd) launch_wgrad_tr<1, 1, 32>(s, x, cin, xa, ca, dy, cout, d, scratch, 32 * pairs, 1);
   }
   const size_t row = nw + cout;  // a split's partial sums: weights, then bias
   if (db == dw + nw) {  // the usual parameter layout: one reduction for both

@@ -231,16 +231,16 @@ void backward(gt_denoiser* dn) {
   float* G = dn->grads.p;
   float* S = dn->scratch.p;
   const size_t SN = dn->scratch.n;
-  auto wgrad = [&](Conv& c, const uint16_t* x, const uint16_t* dy, const uint16_t* mask, gdn::Dims d,
-                   const uint16_t* xa = nullptr, int ca = 0) {
-    gdn::conv_wgrad(s, x, c.cin, dy, mask, c.cout, d, G + c.w_off, G + c.b_off, S, SN, xa, ca);
+  auto wgrad = [&](Conv& c, const uint16_t* x, const uint16_t* dy, gdn::Dims d, const uint16_t* xa = nullptr,
+                   int ca = 0) {
+    gdn::conv_wgrad(s, x, c.cin, dy, c.cout, d, G + c.w_off, G + c.b_off, S, SN, xa, ca);
   };
   // Each gradient is masked by the ReLU of the tensor it belongs to where it is made (the data
   // gradient, upsample or pooling backward that writes it), so the weight and data gradients
   // that read it need no mask.
   // output convolution (no ReLU)
   Conv& co = dn->out_conv();
-  wgrad(co, dn->lv[0]->dec.p, dn->dz.p, nullptr, dn->shape);
+  wgrad(co, dn->lv[0]->dec.p, dn->dz.p, dn->shape);
   gdn::conv_dgrad(s, dn->dz.p, nullptr, co.cout, co.wt16.p, dn->lv[0]->g_dec.p, co.cin, dn->shape, dn->lv[0]->dec.p);
   // decoder, nearest the output first
   for (int l = 0; l <= L - 2; ++l) {
@@ -248,10 +248,10 @@ void backward(gt_denoiser* dn) {
     const gdn::Dims d = level_dims(dn, l);
     Conv& c1 = dn->dec_conv(l, 1);
     Conv& c0 = dn->dec_conv(l, 0);
-    wgrad(c1, v.dec_a.p, v.g_dec.p, nullptr, d);
+    wgrad(c1, v.dec_a.p, v.g_dec.p, d);
     gdn::conv_dgrad(s, v.g_dec.p, nullptr, c1.cout, c1.wt16.p, v.g_dec_a.p, c1.cin, d, v.dec_a.p);
     const uint16_t* up = l == L - 2 ? dn->lv[L - 1]->enc.p : dn->lv[l + 1]->dec.p;
-    wgrad(c0, v.enc.p, v.g_dec_a.p, nullptr, d, up, dn->ch[l + 1]);
+    wgrad(c0, v.enc.p, v.g_dec_a.p, d, up, dn->ch[l + 1]);
     gdn::conv_dgrad(s, v.g_dec_a.p, nullptr, c0.cout, c0.wt16.p, v.g_cat.p, c0.cin, d);
     uint16_t* g_up = l == L - 2 ? dn->lv[L - 1]->g_enc.p : dn->lv[l + 1]->g_dec.p;
     gdn::upcat_bwd(s, v.g_cat.p, dn->ch[l + 1], dn->ch[l], g_up, d, up);
@@ -265,10 +265,10 @@ void backward(gt_denoiser* dn) {
                     v.g_enc.p, dn->ch[l], d, true);
     Conv& c1 = dn->enc_conv(l, 1);
     Conv& c0 = dn->enc_conv(l, 0);
-    wgrad(c1, v.enc_a.p, v.g_enc.p, nullptr, d);
+    wgrad(c1, v.enc_a.p, v.g_enc.p, d);
     gdn::conv_dgrad(s, v.g_enc.p, nullptr, c1.cout, c1.wt16.p, v.g_enc_a.p, c1.cin, d, v.enc_a.p);
     const uint16_t* in = l == 0 ? dn->x0.p : v.in.p;
-    wgrad(c0, in, v.g_enc_a.p, nullptr, d);
+    wgrad(c0, in, v.g_enc_a.p, d);
     if (l > 0) gdn::conv_dgrad(s, v.g_enc_a.p, nullptr, c0.cout, c0.wt16.p, v.g_in.p, c0.cin, d);
   }
 }
