@@ -95,7 +95,8 @@ def kernel_layouts():
 
 
 def wgrad_layouts():
-    for kSD, kSX, when in ((272, 496, "2 x 2 fragments (masked or large tensors)"), (264, 488, "before")):
+    """The removed kernel k_wgrad: transposed staging and 16-byte operand reads."""
+    for kSD, kSX, when in ((272, 496, "2 x 2 fragments"), (264, 488, "before")):
         dT0, xT0 = 0, 2 * 16 * kSD
         a_reads, x_reads = [], []
         for wave in range(4):
@@ -107,8 +108,8 @@ def wgrad_layouts():
                     for dr in range(3):
                         x_reads.append([2 * (xT0 + (b * 16 + (l & 15)) * kSX + (wave + dr) * WX + 8 + 32 * kc +
                                              8 * (l >> 4)) for l in range(64)])
-        add(f"k_wgrad dPre operand reads, strides {kSD}/{kSX}, {when}", 0, a_reads)
-        add(f"k_wgrad x operand reads (centre piece), strides {kSD}/{kSX}, {when}", 0, x_reads)
+        add(f"k_wgrad (removed) dPre operand reads, strides {kSD}/{kSX}, {when}", 0, a_reads)
+        add(f"k_wgrad (removed) x operand reads (centre piece), strides {kSD}/{kSX}, {when}", 0, x_reads)
     # staging writes
     for kS, DQ in ((264, 2), (264, 4)):
         pats = []
@@ -116,7 +117,7 @@ def wgrad_layouts():
             for e in range(8):
                 pats.append([2 * ((8 * ((64 * wave + l) % DQ) + e) * kS + ((64 * wave + l) // DQ) // 64 * 64 +
                                   ((64 * wave + l) // DQ) % 64) for l in range(64)])
-        add(f"k_wgrad dPre staging, 2-byte stores, {DQ} pieces per pixel, stride {kS}", 2, pats)
+        add(f"k_wgrad (removed) dPre staging, 2-byte stores, {DQ} pieces per pixel, stride {kS}", 2, pats)
     pats = []
     for wave in range(4):
         for e in range(8):
@@ -126,7 +127,7 @@ def wgrad_layouts():
                 q, g = i >> 6, i & 63
                 p.append(2 * ((8 * q + e) * 272 + (g >> 4) * 64 + (g & 15) * 4))
             pats.append(p)
-    add("k_wgrad dPre staging, 4 x 8 register transposes, 8-byte stores, stride 272", 1, pats)
+    add("k_wgrad (removed) dPre staging, 4 x 8 register transposes, 8-byte stores, stride 272", 1, pats)
 
 
 def wgrad_tr_layouts():
