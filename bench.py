@@ -482,7 +482,7 @@ def denoise_summary(ctx) -> dict:
         "reference_error_relmse_trimmed": round(r["reference_error_relmse_trimmed"], 6),
         "time_to_targets_s": r["time_to_targets"],
         "relmse_0.01": {"plain_s": full.get("plain_s"), "denoised_s": full.get("denoised_s"),
-                        "denoised_over_plain": full.get("denoised_over_plain"), "north_star_max_ratio": 0.125},
+                        "denoised_over_plain": full.get("denoised_over_plain"), "target_max_ratio": 0.125},
         "plain": [{k: p[k] for k in ("spp", "time_s", "relmse", "relmse_trimmed")} for p in r["plain"]],
         "denoised": [{k: d[k] for k in ("spp", "time_s", "infer_s", "relmse", "relmse_trimmed", "relmse_input")}
                      for d in r["denoised"]],

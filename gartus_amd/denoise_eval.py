@@ -226,7 +226,7 @@ def evaluate(dn: Denoiser, plain_spp=(4, 16, 64, 256, 1024, 4096, 16384, 65536, 
         "reference_error_relmse": ref["own_error"],
         "reference_error_relmse_trimmed": ref["own_error_trimmed"],
         "target": 0.01,
-        "north_star_ratio": 1.0 / 8.0,
+        "target_max_ratio": 1.0 / 8.0,
         "time_to_targets": targets,
         "plain": plain,
         "denoised": den,

@@ -9,7 +9,7 @@
  *   - first-hit albedo;
  *   - normal AOVs.
  * It returns an estimate of the converged image. The plain path tracer stays
- * as the fallback, and as the judge of the denoiser's error.
+ * as the fallback, and as the reference the denoiser's error is measured against.
  *
  * Network: a convolutional encoder-decoder (U-Net).
  *   - `levels` resolutions, with channels[l] features at level l.
